@@ -1,0 +1,164 @@
+/*
+ * snapy_amd.h -- the C-ABI of the MI355X (gfx950) finite-volume stage.
+ *
+ * Every entry point takes plain device pointers, sizes and a hipStream_t
+ * passed as `void *` (NULL = the null stream), launches asynchronously on
+ * that stream and returns 0 on success.  A negative return is an argument
+ * error detected on the host before any launch; a positive return is the
+ * hipError_t of a failed launch.  snapy_last_error() gives the message of the
+ * last failure on the calling thread.  No entry point allocates device
+ * memory: the caller passes every buffer (state, face fluxes, workspace).
+ *
+ * Layout of every cell field: (nvar, nc3, nc2, nc1), x1 contiguous, with
+ * nc_d = nx_d + 2*nghost when nx_d > 1 and nc_d = 1 otherwise
+ * (reference: src/coord/coordinate.hpp:45-47).  A face field of direction d
+ * has the same shape; face i is the lower face of cell i
+ * (src/recon/reconstruct.cpp:217-229).  Rows are IDN=0, IVX=1, IVY=2, IVZ=3,
+ * IPR=4 (src/snap.h:34-45); shallow water uses rows 0..3.
+ *
+ * The reference seam each entry replaces is cited beside it.  The reference
+ * dispatches one ATen DispatchStub per operator; this ABI exposes fused
+ * stage-level kernels instead.
+ */
+#ifndef SNAPY_AMD_H
+#define SNAPY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enums ------------------------------------------------------------ */
+enum snapy_dtype { SNAPY_F64 = 0, SNAPY_F32 = 1 };
+
+/* src/riemann/riemann.hpp RiemannKind (hllc, lmars, shallow-roe subset) */
+enum snapy_riemann {
+  SNAPY_RIEMANN_HLLC = 0,
+  SNAPY_RIEMANN_LMARS = 1,
+  SNAPY_RIEMANN_SHALLOW_ROE = 2
+};
+
+/* src/bc/bc_func.cpp: boundary functions by name */
+enum snapy_bc {
+  SNAPY_BC_NONE = 0, /* custom / no-op */
+  SNAPY_BC_REFLECTING = 1,
+  SNAPY_BC_PERIODIC = 2,
+  SNAPY_BC_EXTRAPOLATION = 3,
+  SNAPY_BC_OUTFLOW = 4 /* radiating about the initial background */
+};
+
+/* ---- descriptors ------------------------------------------------------ */
+
+/* One block's uniform Cartesian grid (src/coord/coordinate.cpp:340-420). */
+typedef struct snapy_grid {
+  int32_t nvar;   /* stored rows: 5 Euler, 4 shallow water */
+  int32_t nghost; /* 3 for weno5 */
+  int32_t nx1, nx2, nx3;
+  int32_t nc1, nc2, nc3;
+  double dx1, dx2, dx3; /* cell widths (global extent / global cells) */
+  double x1min, x2min, x3min; /* lower edge of the first interior cell */
+} snapy_grid;
+
+/* Deck physics the stage kernels read (src/hydro/hydro.hpp options). */
+typedef struct snapy_physics {
+  double gammad;         /* ideal-gas adiabatic index */
+  double grav1;          /* const-gravity along x1 (0 = none) */
+  double density_floor;  /* eos density-floor (outflow bounds) */
+  double pressure_floor; /* eos pressure-floor (outflow bounds) */
+  int32_t riemann;       /* enum snapy_riemann */
+  int32_t shock_x1;      /* reconstruct/vertical shock: weno5 on every row */
+  int32_t shock_x23;     /* reconstruct/horizontal shock */
+  int32_t bc[6];         /* x1-inner, x1-outer, x2-inner, ... x3-outer */
+} snapy_physics;
+
+/* ---- entry points ----------------------------------------------------- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *snapy_last_error(void);
+
+/* Number of gfx950 kernels compiled into the library (load check). */
+int snapy_kernel_count(void);
+
+/* Conserved -> primitive on every cell of the block.
+ * Replaces: ideal_gas_cons2prim stub (src/eos/eos_dispatch.hpp:9-16,
+ * kernel src/eos/ideal_gas_impl.h:16-35). */
+int snapy_cons2prim(const snapy_grid *g, const snapy_physics *p, int dtype,
+                    const void *u, void *w, void *stream);
+
+/* Primitive -> conserved on every cell (IdealGasImpl::_prim2cons,
+ * src/eos/ideal_gas.cpp:49-65). */
+int snapy_prim2cons(const snapy_grid *g, const snapy_physics *p, int dtype,
+                    const void *w, void *u, void *stream);
+
+/* Fused face sweep of direction dim (1 = x1, 2 = x2, 3 = x3): the weno5/cp5
+ * face states of every interior face straight from w, then the Riemann flux,
+ * written into the FluxRegister total `flux` (faces lo..hi+1 along dim,
+ * interior cells across it).  face_pressure (x1 only, may be NULL) receives
+ * the solver's face pressure.
+ * Replaces: call_recon (src/recon/recon_dispatch.hpp:13-16) twice per row
+ * group, and call_riemann (src/riemann/riemann.hpp:165-171) with its EOS face
+ * pass (src/riemann/riemann.cpp:219-346). */
+int snapy_sweep(int dim, const snapy_grid *g, const snapy_physics *p,
+                int dtype, const void *w, void *flux, void *face_pressure,
+                void *stream);
+
+/* Fused flux divergence + RK stage combine on the interior cells:
+ *   du = -dt * div(F) + du_src      (du_src may be NULL)
+ *   u  = w0*u0 + w1*u + w2*du
+ * flux[d] may be NULL for an inactive direction.  dt_dev (may be NULL) is a
+ * device FP64 dt read by the kernel in place of dt (StepDt on the device,
+ * src/utils/step_dt.hpp:17-63), so a captured stage replays with a new dt.
+ * Replaces: CoordinateImpl::divergence (src/coord/coordinate.cpp:618-662),
+ * the du assembly (src/hydro/hydro_forward.cpp:983-985) and the harp combine
+ * pintg->forward(stage, u0, u, du, u) (src/mesh/meshblock_advance.cpp:329). */
+int snapy_update(const snapy_grid *g, int dtype, const void *const flux[3],
+                 const void *u0, void *u, const void *du_src, double dt,
+                 const double *dt_dev, double w0, double w1, double w2,
+                 void *stream);
+
+/* FluxRegister step field S_d <- w1*S_d + (w2*dt)*T_d in FP64 over every
+ * face of the total (FluxRegister::end_stage, src/hydro/flux_register.cpp:
+ * 761-777).  step is FP64; total is dtype; dt_dev as in snapy_update. */
+int snapy_step_field(const snapy_grid *g, int dtype, const void *total,
+                     double *step, double w1, double w2, double dt,
+                     const double *dt_dev, void *stream);
+
+/* CFL reduction: *dt_out (device, FP64) = scale * min over interior cells
+ * and active directions of dx_d / (|v_d| + c) (HydroImpl::max_time_step,
+ * explicit branch, src/hydro/hydro.cpp:372-392; scale = 2^-redo * cfl as in
+ * global_time_step, src/mesh/meshblock_advance.cpp:65-72).  dt_out is
+ * overwritten, not merged; no host read. */
+int snapy_cfl(const snapy_grid *g, const snapy_physics *p, int dtype,
+              const void *w, double scale, double *dt_out, void *stream);
+
+/* Physical-boundary ghost fill of the conserved state, faces in the order
+ * x1-inner, x1-outer, x2-..., x3-... (MeshBlockImpl::apply_boundaries,
+ * src/mesh/meshblock_exchange.cpp:214-287; functions src/bc/bc_func.cpp).
+ * bref_w is the initial primitive background for outflow faces (may be NULL
+ * without outflow); scratch_w is a primitive-sized workspace (may be NULL
+ * without outflow). */
+int snapy_apply_boundaries(const snapy_grid *g, const snapy_physics *p,
+                           int dtype, void *u, const void *bref_w,
+                           void *scratch_w, void *stream);
+
+/* Same ghost fill on a primitive state (apply_boundaries(..., primitive=true),
+ * used by initialize). */
+int snapy_apply_boundaries_prim(const snapy_grid *g, const snapy_physics *p,
+                                int dtype, void *w, const void *bref_w,
+                                void *stream);
+
+/* FP64 budget: out[v] = sum over interior cells of u[v] * volume, for
+ * v < nvar, deterministic (fixed reduction tree).  out is device FP64 of
+ * length nvar; ws is FP64 device workspace of at least
+ * snapy_budget_workspace(g) doubles.
+ * Replaces: BudgetConsumer domain totals (src/diagnostics/budget.cpp:267). */
+int64_t snapy_budget_workspace(const snapy_grid *g);
+int snapy_budget(const snapy_grid *g, int dtype, const void *u, double *ws,
+                 double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SNAPY_AMD_H */

@@ -1,0 +1,143 @@
+"""ctypes binding of the C-ABI in include/snapy_amd.h.
+
+The library is loaded after ``import torch`` so it shares torch's HIP runtime
+(both resolve the soname libamdhip64.so.7).  A missing library is an error:
+there is no CPU fallback behind this module.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import torch  # noqa: F401  (load torch's HIP runtime first)
+
+LIB_PATH = Path(__file__).resolve().parent / "_snapy_amd.so"
+
+F64, F32 = 0, 1
+RIEMANN = {"hllc": 0, "lmars": 1, "shallow-roe": 2}
+BC = {
+    "custom": 0,
+    "none": 0,
+    "reflecting": 1,
+    "periodic": 2,
+    "extrapolation": 3,
+    "outflow": 4,
+}
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [
+        ("nvar", ctypes.c_int32),
+        ("nghost", ctypes.c_int32),
+        ("nx1", ctypes.c_int32),
+        ("nx2", ctypes.c_int32),
+        ("nx3", ctypes.c_int32),
+        ("nc1", ctypes.c_int32),
+        ("nc2", ctypes.c_int32),
+        ("nc3", ctypes.c_int32),
+        ("dx1", ctypes.c_double),
+        ("dx2", ctypes.c_double),
+        ("dx3", ctypes.c_double),
+        ("x1min", ctypes.c_double),
+        ("x2min", ctypes.c_double),
+        ("x3min", ctypes.c_double),
+    ]
+
+
+class Physics(ctypes.Structure):
+    _fields_ = [
+        ("gammad", ctypes.c_double),
+        ("grav1", ctypes.c_double),
+        ("density_floor", ctypes.c_double),
+        ("pressure_floor", ctypes.c_double),
+        ("riemann", ctypes.c_int32),
+        ("shock_x1", ctypes.c_int32),
+        ("shock_x23", ctypes.c_int32),
+        ("bc", ctypes.c_int32 * 6),
+    ]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_GP = ctypes.POINTER(Grid)
+_PP = ctypes.POINTER(Physics)
+
+# name -> (restype, argtypes); every name here must be declared in
+# include/snapy_amd.h (tests/test_capi_load.py checks both ways)
+SIGNATURES = {
+    "snapy_last_error": (ctypes.c_char_p, []),
+    "snapy_kernel_count": (_I, []),
+    "snapy_cons2prim": (_I, [_GP, _PP, _I, _P, _P, _P]),
+    "snapy_prim2cons": (_I, [_GP, _PP, _I, _P, _P, _P]),
+    "snapy_sweep": (_I, [_I, _GP, _PP, _I, _P, _P, _P, _P]),
+    "snapy_update": (_I, [_GP, _I, _P * 3, _P, _P, _P, _D, _P, _D, _D, _D, _P]),
+    "snapy_step_field": (_I, [_GP, _I, _P, _P, _D, _D, _D, _P, _P]),
+    "snapy_cfl": (_I, [_GP, _PP, _I, _P, _D, _P, _P]),
+    "snapy_apply_boundaries": (_I, [_GP, _PP, _I, _P, _P, _P, _P]),
+    "snapy_apply_boundaries_prim": (_I, [_GP, _PP, _I, _P, _P, _P]),
+    "snapy_budget_workspace": (ctypes.c_int64, [_GP]),
+    "snapy_budget": (_I, [_GP, _I, _P, _P, _P, _P]),
+}
+
+_lib = None
+
+
+class SnapyError(RuntimeError):
+    pass
+
+
+def load(path: Path | str | None = None):
+    """Load the kernel library once; raise if it is not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise SnapyError(
+            f"{p} is not built: run `python -m snapy_amd.build` (or "
+            "__graft_entry__.build()); the stage has no CPU fallback"
+        )
+    lib = ctypes.CDLL(str(p), mode=ctypes.RTLD_GLOBAL)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().snapy_last_error().decode(errors="replace")
+        raise SnapyError(f"{what} failed ({rc}): {msg}")
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None passes NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise SnapyError("the stage kernels take HIP device tensors only")
+    if not t.is_contiguous():
+        raise SnapyError("the stage kernels take contiguous tensors only")
+    return t.data_ptr()
+
+
+def stream_of(device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def dtype_code(dtype) -> int:
+    if dtype == torch.float64:
+        return F64
+    if dtype == torch.float32:
+        return F32
+    raise SnapyError(f"unsupported dtype {dtype}")
+
+
+def lib_path_in_use() -> str:
+    return os.fspath(LIB_PATH)

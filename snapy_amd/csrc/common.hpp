@@ -1,0 +1,51 @@
+// Shared host/device helpers of the stage kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/snapy_amd.h"
+
+namespace snapy {
+
+// the thread-local message behind snapy_last_error()
+void set_error(std::string const& msg);
+int fail(int code, std::string const& msg);
+int launch_status(char const* what);
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// index ranges of one block: [lo, lo + n) are the interior cells of an axis
+struct Range3 {
+  int lo1, n1, lo2, n2, lo3, n3;
+  int nc1, nc2, nc3;
+  int64_t s2, s3, sv;  // strides of x2, x3 and the variable row
+};
+
+inline Range3 ranges(snapy_grid const& g) {
+  Range3 r;
+  r.nc1 = g.nc1;
+  r.nc2 = g.nc2;
+  r.nc3 = g.nc3;
+  r.lo1 = g.nc1 > 1 ? g.nghost : 0;
+  r.lo2 = g.nc2 > 1 ? g.nghost : 0;
+  r.lo3 = g.nc3 > 1 ? g.nghost : 0;
+  r.n1 = g.nx1;
+  r.n2 = g.nx2;
+  r.n3 = g.nx3;
+  r.s2 = g.nc1;
+  r.s3 = int64_t(g.nc1) * g.nc2;
+  r.sv = r.s3 * g.nc3;
+  return r;
+}
+
+// host-side validation of a grid descriptor; returns 0 or a negative code
+int check_grid(snapy_grid const* g);
+
+inline unsigned blocks_for(int64_t n, int tpb) {
+  return static_cast<unsigned>((n + tpb - 1) / tpb);
+}
+
+}  // namespace snapy

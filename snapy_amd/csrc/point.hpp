@@ -1,0 +1,281 @@
+// Device point functions of the finite-volume stage: one face value of the
+// reconstruction, one face flux of the Riemann solvers, one cell of the EOS.
+// Every function restates the reference's arithmetic order term by term so the
+// FP64 form tracks the reference CPU path to round-off; the library is built
+// with -ffp-contract=off so no product/sum pair is fused behind our back.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace snapy {
+
+enum { IDN = 0, IVX = 1, IVY = 2, IVZ = 3, IPR = 4 };
+
+template <typename T>
+__device__ __forceinline__ T sqr(T x) {
+  return x * x;
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction (src/recon/recon_point.h, tables src/recon/reconstruct.cpp)
+// ---------------------------------------------------------------------------
+//
+// The window is the 5 cells v[0..4] centred on the reconstructed cell.  The
+// "minus" table (cm) gives the LEFT edge value of the centre cell, the "plus"
+// table (cp = cm with every row reversed) its RIGHT edge value.  Zero
+// coefficients of the reference tables are dropped: 0*x adds an exact zero.
+
+// weno5 with the uniform table and scale = false (recon_point.h:114-147;
+// rows reconstruct.cpp:74-86, d = .3,.6,.1 at :84, eps = 1e-6 at :65-67).
+template <typename T>
+__device__ __forceinline__ T weno5_minus(T v0, T v1, T v2, T v3, T v4) {
+  // candidate rows
+  T p0 = T(-1. / 6.) * v0;
+  p0 = p0 + T(5. / 6.) * v1;
+  p0 = p0 + T(1. / 3.) * v2;
+  T p1 = T(1. / 3.) * v1;
+  p1 = p1 + T(5. / 6.) * v2;
+  p1 = p1 + T(-1. / 6.) * v3;
+  T p2 = T(11. / 6.) * v2;
+  p2 = p2 + T(-7. / 6.) * v3;
+  p2 = p2 + T(1. / 3.) * v4;
+  // smoothness rows
+  T s4 = v0 + T(-2.) * v1;
+  s4 = s4 + v2;
+  T s5 = v0 + T(-4.) * v1;
+  s5 = s5 + T(3.) * v2;
+  T s6 = v1 + T(-2.) * v2;
+  s6 = s6 + v3;
+  T s7 = T(-1.) * v1;
+  s7 = s7 + v3;
+  T s8 = v2 + T(-2.) * v3;
+  s8 = s8 + v4;
+  T s9 = T(3.) * v2;
+  s9 = s9 + T(-4.) * v3;
+  s9 = s9 + v4;
+  T beta0 = T(13. / 12.) * sqr(s4) + T(.25) * sqr(s5);
+  T beta1 = T(13. / 12.) * sqr(s6) + T(.25) * sqr(s7);
+  T beta2 = T(13. / 12.) * sqr(s8) + T(.25) * sqr(s9);
+  T a0 = T(.3) / sqr(beta0 + T(1e-6));
+  T a1 = T(.6) / sqr(beta1 + T(1e-6));
+  T a2 = T(.1) / sqr(beta2 + T(1e-6));
+  return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+}
+
+template <typename T>
+__device__ __forceinline__ T weno5_plus(T v0, T v1, T v2, T v3, T v4) {
+  // the reversed rows, accumulated in window order as the reference does
+  T p0 = T(1. / 3.) * v2;
+  p0 = p0 + T(5. / 6.) * v3;
+  p0 = p0 + T(-1. / 6.) * v4;
+  T p1 = T(-1. / 6.) * v1;
+  p1 = p1 + T(5. / 6.) * v2;
+  p1 = p1 + T(1. / 3.) * v3;
+  T p2 = T(1. / 3.) * v0;
+  p2 = p2 + T(-7. / 6.) * v1;
+  p2 = p2 + T(11. / 6.) * v2;
+  T s4 = v2 + T(-2.) * v3;
+  s4 = s4 + v4;
+  T s5 = T(3.) * v2;
+  s5 = s5 + T(-4.) * v3;
+  s5 = s5 + v4;
+  T s6 = v1 + T(-2.) * v2;
+  s6 = s6 + v3;
+  T s7 = v1 + T(-1.) * v3;
+  T s8 = v0 + T(-2.) * v1;
+  s8 = s8 + v2;
+  T s9 = v0 + T(-4.) * v1;
+  s9 = s9 + T(3.) * v2;
+  T beta0 = T(13. / 12.) * sqr(s4) + T(.25) * sqr(s5);
+  T beta1 = T(13. / 12.) * sqr(s6) + T(.25) * sqr(s7);
+  T beta2 = T(13. / 12.) * sqr(s8) + T(.25) * sqr(s9);
+  T a0 = T(.3) / sqr(beta0 + T(1e-6));
+  T a1 = T(.6) / sqr(beta1 + T(1e-6));
+  T a2 = T(.1) / sqr(beta2 + T(1e-6));
+  return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+}
+
+// cp5 (poly_point<T,5>, recon_point.h:54-61; table reconstruct.cpp:65-67)
+template <typename T>
+__device__ __forceinline__ T cp5_minus(T v0, T v1, T v2, T v3, T v4) {
+  T o = T(-1. / 20.) * v0;
+  o = o + T(9. / 20.) * v1;
+  o = o + T(47. / 60.) * v2;
+  o = o + T(-13. / 60.) * v3;
+  o = o + T(1. / 30.) * v4;
+  return o;
+}
+
+template <typename T>
+__device__ __forceinline__ T cp5_plus(T v0, T v1, T v2, T v3, T v4) {
+  T o = T(1. / 30.) * v0;
+  o = o + T(-13. / 60.) * v1;
+  o = o + T(47. / 60.) * v2;
+  o = o + T(9. / 20.) * v3;
+  o = o + T(-1. / 20.) * v4;
+  return o;
+}
+
+// ---------------------------------------------------------------------------
+// Riemann solvers (src/riemann/riemann_kernels.h)
+// ---------------------------------------------------------------------------
+// Face states are in the face frame: n = normal row, t1/t2 the tangential
+// rows in cyclic order (face_axes, riemann_kernels.h:37-40).  A state is
+// {rho, v[IVX], v[IVY], v[IVZ], p} indexed by the global rows; the solver
+// reads the normal component through n.
+
+template <typename T>
+struct Prim5 {
+  T r, v[3], p;  // v[0] = IVX row, v[1] = IVY row, v[2] = IVZ row
+};
+
+// HLLC, FP64/FP32 forms of hllc_impl (riemann_kernels.h:56-194), dry
+// (ny = 0: the dry fraction is 1) and without the face background.  e is the
+// internal energy per volume p/(gamma-1), c the sound speed
+// (src/riemann/riemann.cpp:232-260).
+template <typename T>
+__device__ __forceinline__ void hllc(const Prim5<T>& L, const Prim5<T>& R,
+                                     T gl, T gr, int n, T out[5], T* fp) {
+  T el = L.p / (gl - T(1));
+  T er = R.p / (gr - T(1));
+  T cl = sqrt(gl * L.p / L.r);
+  T cr = sqrt(gr * R.p / R.r);
+  const int t1 = (n + 1) % 3, t2 = (n + 2) % 3;
+  T uL = L.v[n], uR = R.v[n];
+
+  el = el + T(0.5) * L.r * (sqr(L.v[0]) + sqr(L.v[1]) + sqr(L.v[2]));
+  er = er + T(0.5) * R.r * (sqr(R.v[0]) + sqr(R.v[1]) + sqr(R.v[2]));
+
+  T rhoa = T(0.5) * (L.r + R.r);
+  T ca = T(0.5) * (cl + cr);
+  T pmid = T(0.5) * (L.p + R.p + (uL - uR) * rhoa * ca);
+
+  T shock_l = pmid / L.p - T(1.0);
+  T shock_r = pmid / R.p - T(1.0);
+  T ql = (pmid <= L.p) ? T(1.0)
+                       : sqrt(T(1.0) + (gl + 1) / (2 * gl) * shock_l);
+  T qr = (pmid <= R.p) ? T(1.0)
+                       : sqrt(T(1.0) + (gr + 1) / (2 * gr) * shock_r);
+
+  T al = uL - cl * ql;
+  T ar = uR + cr * qr;
+  T bp = ar > T(0.0) ? ar : T(1.0e-10);
+  T bm = al < T(0.0) ? al : -T(1.0e-10);
+
+  T vxl, vxr;
+  if constexpr (sizeof(T) == 4) {
+    vxl = cl * ql;
+    vxr = -cr * qr;
+  } else {
+    vxl = uL - al;
+    vxr = uR - ar;
+  }
+  T tl = L.p + vxl * L.r * uL;
+  T tr = R.p + vxr * R.r * uR;
+  T ml = L.r * vxl;
+  T mr = -(R.r * vxr);
+  T am = (tl - tr) / (ml + mr);
+  T cp = (ml * tr + mr * tl) / (ml + mr);
+  cp = cp > T(0.0) ? cp : T(0.0);
+  if (fp) *fp = cp;
+
+  if constexpr (sizeof(T) == 4) {
+    vxl = al < T(0) ? cl * ql : uL + T(1.0e-10);
+    vxr = ar > T(0) ? -cr * qr : uR - T(1.0e-10);
+  } else {
+    vxl = uL - bm;
+    vxr = uR - bp;
+  }
+
+  T flI = L.r * vxl, frI = R.r * vxr;
+  T fln = L.r * uL * vxl + L.p, frn = R.r * uR * vxr + R.p;
+  T fl1 = L.r * L.v[t1] * vxl, fr1 = R.r * R.v[t1] * vxr;
+  T fl2 = L.r * L.v[t2] * vxl, fr2 = R.r * R.v[t2] * vxr;
+  T flE = el * vxl + L.p * uL, frE = er * vxr + R.p * uR;
+
+  T sl, sr, sm;
+  if (am >= T(0.0)) {
+    sl = am / (am - bm);
+    sr = T(0.0);
+    sm = -bm / (am - bm);
+  } else {
+    sl = T(0.0);
+    sr = -am / (bp - am);
+    sm = bp / (bp - am);
+  }
+  out[IDN] = sl * flI + sr * frI;
+  out[IVX + n] = sl * fln + sr * frn + sm * cp;
+  out[IVX + t1] = sl * fl1 + sr * fr1;
+  out[IVX + t2] = sl * fl2 + sr * fr2;
+  out[IPR] = sl * flE + sr * frE + sm * cp * am;
+}
+
+// LMARS (lmars_impl, riemann_kernels.h:197-255), dry.  The specific internal
+// energy e/rho = p/((gamma-1) rho) enters the enthalpy
+// (src/riemann/riemann.cpp:262-268).
+template <typename T>
+__device__ __forceinline__ void lmars(const Prim5<T>& L, const Prim5<T>& R,
+                                      T gl, T gr, int n, T out[5], T* fp) {
+  const int t1 = (n + 1) % 3, t2 = (n + 2) % 3;
+  T hl = (L.p / (gl - T(1))) / L.r;
+  T hr = (R.p / (gr - T(1))) / R.r;
+  // wli = {rho, v_n, v_t1, v_t2, p}
+  T lx = L.v[n], ly = L.v[t1], lz = L.v[t2];
+  T rx = R.v[n], ry = R.v[t1], rz = R.v[t2];
+  // hl += a + b is hl + (a + b)
+  hl = hl + (T(0.5) * (sqr(lx) + sqr(ly) + sqr(lz)) + L.p / L.r);
+  hr = hr + (T(0.5) * (sqr(rx) + sqr(ry) + sqr(rz)) + R.p / R.r);
+
+  T rhobar = T(0.5) * (L.r + R.r);
+  T gamma_bar = T(0.5) * (gl + gr);
+  T cbar = sqrt(T(0.5) * gamma_bar * (L.p + R.p) / rhobar);
+  T pbar = T(0.5) * (L.p + R.p) + T(0.5) * (rhobar * cbar) * (lx - rx);
+  if (fp) *fp = pbar;
+  T ubar = T(0.5) * (lx + rx) + T(0.5) / (rhobar * cbar) * (L.p - R.p);
+  if (ubar > T(0)) {
+    out[IDN] = ubar * L.r;  // * rd (= 1, dry)
+    out[IVX + n] = ubar * L.r * lx + pbar;
+    out[IVX + t1] = ubar * L.r * ly;
+    out[IVX + t2] = ubar * L.r * lz;
+    out[IPR] = ubar * L.r * hl;
+  } else {
+    out[IDN] = ubar * R.r;
+    out[IVX + n] = ubar * R.r * rx + pbar;
+    out[IVX + t1] = ubar * R.r * ry;
+    out[IVX + t2] = ubar * R.r * rz;
+    out[IPR] = ubar * R.r * hr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Equation of state (src/eos/ideal_gas_impl.h, src/eos/ideal_gas.cpp)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void cons2prim5(const T u[5], T w[5], double gd) {
+  // ideal_gas_cons2prim (ideal_gas_impl.h:16-35); Cartesian: cos(theta) = 0
+  w[IDN] = u[IDN];
+  w[IVX] = u[IVX] / w[IDN];
+  w[IVY] = u[IVY] / w[IDN];
+  w[IVZ] = u[IVZ] / w[IDN];
+  // the kernel mixes T with the double gammad: ke and ie promote to double
+  double ke = 0.5 * (w[IVX] * u[IVX] + w[IVY] * u[IVY] + w[IVZ] * u[IVZ]);
+  double ie = u[IPR] - ke;
+  w[IPR] = T((gd - 1.) * ie);
+}
+
+template <typename T>
+__device__ __forceinline__ void prim2cons5(const T w[5], T u[5], T gm1) {
+  // IdealGasImpl::_prim2cons (ideal_gas.cpp:49-65): mom = v*rho,
+  // E = p/(gamma-1) + 0.5*sum(v*m)
+  u[IDN] = w[IDN];
+  u[IVX] = w[IVX] * w[IDN];
+  u[IVY] = w[IVY] * w[IDN];
+  u[IVZ] = w[IVZ] * w[IDN];
+  T ke = T(0.5) * (w[IVX] * u[IVX] + w[IVY] * u[IVY] + w[IVZ] * u[IVZ]);
+  u[IPR] = w[IPR] / gm1;
+  u[IPR] = u[IPR] + ke;
+}
+
+}  // namespace snapy

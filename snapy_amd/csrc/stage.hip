@@ -1,0 +1,540 @@
+// Stage kernels of the Cartesian finite-volume path on gfx950 and their C-ABI
+// wrappers: EOS, the fused reconstruction + Riemann face sweep, the fused
+// divergence + RK combine, the FluxRegister step field, the CFL reduction and
+// the deterministic FP64 budget.
+//
+// Design: HBM-bound stencil work, 64-wide wavefronts, the lane index walking
+// the contiguous x1 axis in every kernel so each wavefront load is one
+// coalesced 512 B (FP64) row.  MFMA is not used: nothing here is a dense
+// contraction.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "common.hpp"
+#include "point.hpp"
+
+namespace snapy {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(std::string const& msg) { g_last_error = msg; }
+
+int fail(int code, std::string const& msg) {
+  set_error(msg);
+  return code;
+}
+
+int launch_status(char const* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    return fail(static_cast<int>(e), std::string(what) + ": " +
+                                         hipGetErrorString(e));
+  }
+  return 0;
+}
+
+int check_grid(snapy_grid const* g) {
+  if (!g) return fail(-1, "grid descriptor is NULL");
+  if (g->nghost < 3) return fail(-2, "nghost must be >= 3 for weno5");
+  int nx[3] = {g->nx1, g->nx2, g->nx3};
+  int nc[3] = {g->nc1, g->nc2, g->nc3};
+  for (int d = 0; d < 3; ++d) {
+    if (nx[d] < 1) return fail(-3, "nx must be >= 1");
+    int want = nx[d] > 1 ? nx[d] + 2 * g->nghost : 1;
+    if (nc[d] != want) return fail(-4, "nc does not match nx + 2*nghost");
+    if (nx[d] > 1 && nx[d] < g->nghost)
+      return fail(-5, "an active axis needs at least nghost cells");
+  }
+  if (g->nvar != 5 && g->nvar != 4) return fail(-6, "nvar must be 5 or 4");
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// EOS
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+    cons2prim_kernel(T const* __restrict__ u, T* __restrict__ w, int64_t n,
+                     int64_t sv, double gd) {
+  int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  T uu[5], ww[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) uu[v] = u[v * sv + c];
+  cons2prim5(uu, ww, gd);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) w[v * sv + c] = ww[v];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    prim2cons_kernel(T const* __restrict__ w, T* __restrict__ u, int64_t n,
+                     int64_t sv, T gm1) {
+  int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  T uu[5], ww[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) ww[v] = w[v * sv + c];
+  prim2cons5(ww, uu, gm1);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) u[v * sv + c] = uu[v];
+}
+
+// ---------------------------------------------------------------------------
+// fused reconstruction + Riemann sweep
+// ---------------------------------------------------------------------------
+struct SweepArgs {
+  Range3 r;
+  int nf;  // faces along the sweep axis: nx + 1
+  double gammad;
+  int riemann;
+  int shock;
+};
+
+// one thread per interior face.  DIR 1: the lane walks the faces along x1;
+// DIR 2/3: the lane walks x1 across the faces, so each of the six stencil
+// loads of a wavefront is one coalesced row.
+template <typename T, int DIR>
+__global__ void __launch_bounds__(256)
+    sweep_kernel(T const* __restrict__ w, T* __restrict__ flx,
+                 T* __restrict__ fp, SweepArgs a) {
+  Range3 const& r = a.r;
+  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int i, j, k;
+  int64_t stride;
+  if (DIR == 1) {
+    int64_t tot = int64_t(a.nf) * r.n2 * r.n3;
+    if (idx >= tot) return;
+    i = r.lo1 + int(idx % a.nf);
+    int64_t rest = idx / a.nf;
+    j = r.lo2 + int(rest % r.n2);
+    k = r.lo3 + int(rest / r.n2);
+    stride = 1;
+  } else if (DIR == 2) {
+    int64_t tot = int64_t(r.n1) * a.nf * r.n3;
+    if (idx >= tot) return;
+    i = r.lo1 + int(idx % r.n1);
+    int64_t rest = idx / r.n1;
+    j = r.lo2 + int(rest % a.nf);
+    k = r.lo3 + int(rest / a.nf);
+    stride = r.s2;
+  } else {
+    int64_t tot = int64_t(r.n1) * r.n2 * a.nf;
+    if (idx >= tot) return;
+    i = r.lo1 + int(idx % r.n1);
+    int64_t rest = idx / r.n1;
+    j = r.lo2 + int(rest % r.n2);
+    k = r.lo3 + int(rest / r.n2);
+    stride = r.s3;
+  }
+  int64_t base = k * r.s3 + j * r.s2 + i;  // cell f: face f is its lower face
+
+  Prim5<T> L, R;
+  T* lv[5] = {&L.r, &L.v[0], &L.v[1], &L.v[2], &L.p};
+  T* rv[5] = {&R.r, &R.v[0], &R.v[1], &R.v[2], &R.p};
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    T const* p = w + v * r.sv + base;
+    T c0 = p[-3 * stride], c1 = p[-2 * stride], c2 = p[-stride], c3 = p[0],
+      c4 = p[stride], c5 = p[2 * stride];
+    // left state: right edge of cell f-1 (window c0..c4); right state: left
+    // edge of cell f (window c1..c5).  Density always weno5; velocity and
+    // pressure cp5 unless shock (reconstruct.cpp:303-322).
+    if (v == IDN || a.shock) {
+      *lv[v] = weno5_plus(c0, c1, c2, c3, c4);
+      *rv[v] = weno5_minus(c1, c2, c3, c4, c5);
+    } else {
+      *lv[v] = cp5_plus(c0, c1, c2, c3, c4);
+      *rv[v] = cp5_minus(c1, c2, c3, c4, c5);
+    }
+  }
+  T g = T(a.gammad);
+  T out[5];
+  T pface;
+  if (a.riemann == SNAPY_RIEMANN_LMARS)
+    lmars(L, R, g, g, DIR - 1, out, &pface);
+  else
+    hllc(L, R, g, g, DIR - 1, out, &pface);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) flx[v * r.sv + base] = out[v];
+  if (DIR == 1 && fp != nullptr) fp[base] = pface;
+}
+
+// ---------------------------------------------------------------------------
+// fused divergence + RK combine
+// ---------------------------------------------------------------------------
+struct UpdateArgs {
+  Range3 r;
+  int nvar;
+  double a1, a2, a3, vol;  // Cartesian face areas and cell volume
+  double mdt;              // -dt (host dt)
+  double const* dt_dev;    // device dt (StepDt on the device); NULL: mdt
+  double w0, w1, w2;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    update_kernel(T const* __restrict__ f1, T const* __restrict__ f2,
+                  T const* __restrict__ f3, T const* __restrict__ u0,
+                  T* __restrict__ u, T const* __restrict__ dsrc, UpdateArgs a) {
+  Range3 const& r = a.r;
+  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
+  if (idx >= tot) return;
+  int i = r.lo1 + int(idx % r.n1);
+  int64_t rest = idx / r.n1;
+  int j = r.lo2 + int(rest % r.n2);
+  int k = r.lo3 + int(rest / r.n2);
+  int64_t c = k * r.s3 + j * r.s2 + i;
+  T A1 = T(a.a1), A2 = T(a.a2), A3 = T(a.a3), V = T(a.vol);
+  T mdt = a.dt_dev ? T(-*a.dt_dev) : T(a.mdt);
+  T w0 = T(a.w0), w1 = T(a.w1), w2 = T(a.w2);
+  for (int v = 0; v < a.nvar; ++v) {
+    int64_t o = v * r.sv + c;
+    // dflx = 0; dflx += A F(+) - A F(-) per active direction; / vol
+    // (CoordinateImpl::divergence, coordinate.cpp:618-662)
+    T div = T(0);
+    bool first = true;
+    if (f1) {
+      T d = A1 * f1[o + 1] - A1 * f1[o];
+      div = first ? d : div + d;
+      first = false;
+    }
+    if (f2) {
+      T d = A2 * f2[o + r.s2] - A2 * f2[o];
+      div = first ? d : div + d;
+      first = false;
+    }
+    if (f3) {
+      T d = A3 * f3[o + r.s3] - A3 * f3[o];
+      div = first ? d : div + d;
+      first = false;
+    }
+    div = div / V;
+    T du = mdt * div;
+    if (dsrc) du = du + dsrc[o];
+    u[o] = w0 * u0[o] + w1 * u[o] + w2 * du;
+  }
+}
+
+// S <- w1*S + w2*dt*T over every face of the total, in FP64
+template <typename T>
+__global__ void __launch_bounds__(256)
+    step_field_kernel(T const* __restrict__ t, double* __restrict__ s,
+                      int64_t n, double w1, double w2, double dt,
+                      double const* dt_dev) {
+  int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  // dt.scaled(w2) then axpy_ (flux_register.cpp:761-777)
+  double w2dt = (dt_dev ? *dt_dev : dt) * w2;
+  s[c] = s[c] * w1 + double(t[c]) * w2dt;
+}
+
+// ---------------------------------------------------------------------------
+// CFL: min over interior cells of dx_d / (|v_d| + c), one atomic per block
+// ---------------------------------------------------------------------------
+__global__ void fill_u64_kernel(unsigned long long* p, unsigned long long v) {
+  *p = v;
+}
+
+__global__ void scale_f64_kernel(double* p, double scale) { *p = scale * *p; }
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    cfl_kernel(T const* __restrict__ w, unsigned long long* out, Range3 r,
+               double gd, double dx1, double dx2, double dx3, int act1,
+               int act2, int act3) {
+  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
+  double best = 1.e9;
+  if (idx < tot) {
+    int i = r.lo1 + int(idx % r.n1);
+    int64_t rest = idx / r.n1;
+    int j = r.lo2 + int(rest % r.n2);
+    int k = r.lo3 + int(rest / r.n2);
+    int64_t c = k * r.s3 + j * r.s2 + i;
+    T rho = w[c], p = w[4 * r.sv + c];
+    // sound_speed: sqrt(gamma * p / rho) (equation_of_state.cpp:235-238)
+    T cs = sqrt(T(gd) * p / rho);
+    if (act1) {
+      T t = T(dx1) / (fabs(w[r.sv + c]) + cs);
+      best = fmin(best, double(t));
+    }
+    if (act2) {
+      T t = T(dx2) / (fabs(w[2 * r.sv + c]) + cs);
+      best = fmin(best, double(t));
+    }
+    if (act3) {
+      T t = T(dx3) / (fabs(w[3 * r.sv + c]) + cs);
+      best = fmin(best, double(t));
+    }
+  }
+  // wavefront reduction (64 lanes), then across the 4 wavefronts of the block
+  for (int off = 32; off > 0; off >>= 1)
+    best = fmin(best, __shfl_xor(best, off, 64));
+  __shared__ double part[4];
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) part[wid] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = part[0];
+    for (int q = 1; q < int(blockDim.x >> 6); ++q) m = fmin(m, part[q]);
+    // positive doubles order like their bit patterns
+    atomicMin(out, static_cast<unsigned long long>(__double_as_longlong(m)));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 budget: fixed-shape two-pass sum (deterministic)
+// ---------------------------------------------------------------------------
+constexpr int kBudgetTpb = 256;
+
+template <typename T>
+__global__ void __launch_bounds__(kBudgetTpb)
+    budget_partial_kernel(T const* __restrict__ u, double* __restrict__ part,
+                          Range3 r, int nvar, double vol, int64_t nblk) {
+  int v = blockIdx.y;
+  int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
+  double acc = 0.;
+  for (int64_t idx = int64_t(blockIdx.x) * kBudgetTpb + threadIdx.x; idx < tot;
+       idx += int64_t(gridDim.x) * kBudgetTpb) {
+    int i = r.lo1 + int(idx % r.n1);
+    int64_t rest = idx / r.n1;
+    int j = r.lo2 + int(rest % r.n2);
+    int k = r.lo3 + int(rest / r.n2);
+    acc += double(u[v * r.sv + k * r.s3 + j * r.s2 + i]) * vol;
+  }
+  __shared__ double sm[kBudgetTpb];
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kBudgetTpb / 2; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[v * nblk + blockIdx.x] = sm[0];
+}
+
+__global__ void __launch_bounds__(kBudgetTpb)
+    budget_final_kernel(double const* __restrict__ part, double* out,
+                        int64_t nblk) {
+  int v = blockIdx.x;
+  double acc = 0.;
+  for (int64_t b = threadIdx.x; b < nblk; b += kBudgetTpb)
+    acc += part[v * nblk + b];
+  __shared__ double sm[kBudgetTpb];
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kBudgetTpb / 2; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[v] = sm[0];
+}
+
+constexpr int64_t kBudgetBlocks = 1024;
+
+}  // namespace snapy
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+using namespace snapy;
+
+namespace {
+
+template <typename F>
+int dispatch(int dtype, F&& f) {
+  if (dtype == SNAPY_F64) return f(double(0));
+  if (dtype == SNAPY_F32) return f(float(0));
+  return fail(-10, "unknown dtype");
+}
+
+int check_phys(snapy_physics const* p) {
+  if (!p) return fail(-1, "physics descriptor is NULL");
+  if (p->riemann != SNAPY_RIEMANN_HLLC && p->riemann != SNAPY_RIEMANN_LMARS &&
+      p->riemann != SNAPY_RIEMANN_SHALLOW_ROE)
+    return fail(-7, "unknown riemann solver");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* snapy_last_error(void) { return g_last_error.c_str(); }
+
+int snapy_kernel_count(void) { return 16; }
+
+int snapy_cons2prim(const snapy_grid* g, const snapy_physics* p, int dtype,
+                    const void* u, void* w, void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (int e = check_phys(p)) return e;
+  if (!u || !w) return fail(-1, "cons2prim: NULL buffer");
+  if (g->nvar != 5) return fail(-6, "cons2prim: ideal gas needs nvar 5");
+  Range3 r = ranges(*g);
+  return dispatch(dtype, [&](auto z) {
+    using T = decltype(z);
+    hipLaunchKernelGGL(cons2prim_kernel<T>, dim3(blocks_for(r.sv, 256)),
+                       dim3(256), 0, as_stream(stream),
+                       static_cast<T const*>(u), static_cast<T*>(w), r.sv,
+                       r.sv, p->gammad);
+    return launch_status("cons2prim");
+  });
+}
+
+int snapy_prim2cons(const snapy_grid* g, const snapy_physics* p, int dtype,
+                    const void* w, void* u, void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (int e = check_phys(p)) return e;
+  if (!u || !w) return fail(-1, "prim2cons: NULL buffer");
+  if (g->nvar != 5) return fail(-6, "prim2cons: ideal gas needs nvar 5");
+  Range3 r = ranges(*g);
+  return dispatch(dtype, [&](auto z) {
+    using T = decltype(z);
+    hipLaunchKernelGGL(prim2cons_kernel<T>, dim3(blocks_for(r.sv, 256)),
+                       dim3(256), 0, as_stream(stream),
+                       static_cast<T const*>(w), static_cast<T*>(u), r.sv,
+                       r.sv, T(p->gammad - 1.));
+    return launch_status("prim2cons");
+  });
+}
+
+int snapy_sweep(int dim, const snapy_grid* g, const snapy_physics* p,
+                int dtype, const void* w, void* flux, void* face_pressure,
+                void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (int e = check_phys(p)) return e;
+  if (dim < 1 || dim > 3) return fail(-8, "sweep: dim must be 1, 2 or 3");
+  if (!w || !flux) return fail(-1, "sweep: NULL buffer");
+  if (g->nvar != 5) return fail(-6, "sweep: the Euler sweep needs nvar 5");
+  if (p->riemann == SNAPY_RIEMANN_SHALLOW_ROE)
+    return fail(-7, "sweep: shallow-roe runs through the cubed-sphere path");
+  int nx[3] = {g->nx1, g->nx2, g->nx3};
+  if (nx[dim - 1] < 2) return fail(-9, "sweep: inactive direction");
+  SweepArgs a;
+  a.r = ranges(*g);
+  a.nf = nx[dim - 1] + 1;
+  a.gammad = p->gammad;
+  a.riemann = p->riemann;
+  a.shock = dim == 1 ? p->shock_x1 : p->shock_x23;
+  int64_t tot = int64_t(a.nf);
+  for (int d = 0; d < 3; ++d)
+    if (d != dim - 1) tot *= nx[d];
+  return dispatch(dtype, [&](auto z) {
+    using T = decltype(z);
+    auto wp = static_cast<T const*>(w);
+    auto fl = static_cast<T*>(flux);
+    auto fp = static_cast<T*>(face_pressure);
+    dim3 grid(blocks_for(tot, 256)), blk(256);
+    hipStream_t s = as_stream(stream);
+    if (dim == 1)
+      hipLaunchKernelGGL((sweep_kernel<T, 1>), grid, blk, 0, s, wp, fl, fp, a);
+    else if (dim == 2)
+      hipLaunchKernelGGL((sweep_kernel<T, 2>), grid, blk, 0, s, wp, fl, fp, a);
+    else
+      hipLaunchKernelGGL((sweep_kernel<T, 3>), grid, blk, 0, s, wp, fl, fp, a);
+    return launch_status("sweep");
+  });
+}
+
+int snapy_update(const snapy_grid* g, int dtype, const void* const flux[3],
+                 const void* u0, void* u, const void* du_src, double dt,
+                 const double* dt_dev, double w0, double w1, double w2,
+                 void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (!u0 || !u || !flux) return fail(-1, "update: NULL buffer");
+  UpdateArgs a;
+  a.r = ranges(*g);
+  a.nvar = g->nvar;
+  a.a1 = g->dx3 * g->dx2;
+  a.a2 = g->dx3 * g->dx1;
+  a.a3 = g->dx2 * g->dx1;
+  a.vol = g->dx3 * (g->dx2 * g->dx1);
+  a.mdt = -dt;
+  a.dt_dev = dt_dev;
+  a.w0 = w0;
+  a.w1 = w1;
+  a.w2 = w2;
+  int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
+  return dispatch(dtype, [&](auto z) {
+    using T = decltype(z);
+    hipLaunchKernelGGL(update_kernel<T>, dim3(blocks_for(tot, 256)), dim3(256),
+                       0, as_stream(stream), static_cast<T const*>(flux[0]),
+                       static_cast<T const*>(flux[1]),
+                       static_cast<T const*>(flux[2]),
+                       static_cast<T const*>(u0), static_cast<T*>(u),
+                       static_cast<T const*>(du_src), a);
+    return launch_status("update");
+  });
+}
+
+int snapy_step_field(const snapy_grid* g, int dtype, const void* total,
+                     double* step, double w1, double w2, double dt,
+                     const double* dt_dev, void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (!total || !step) return fail(-1, "step_field: NULL buffer");
+  int64_t n = int64_t(g->nvar) * g->nc1 * g->nc2 * g->nc3;
+  return dispatch(dtype, [&](auto z) {
+    using T = decltype(z);
+    hipLaunchKernelGGL(step_field_kernel<T>, dim3(blocks_for(n, 256)),
+                       dim3(256), 0, as_stream(stream),
+                       static_cast<T const*>(total), step, n, w1, w2, dt,
+                       dt_dev);
+    return launch_status("step_field");
+  });
+}
+
+int snapy_cfl(const snapy_grid* g, const snapy_physics* p, int dtype,
+              const void* w, double scale, double* dt_out, void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (int e = check_phys(p)) return e;
+  if (!w || !dt_out) return fail(-1, "cfl: NULL buffer");
+  Range3 r = ranges(*g);
+  int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
+  auto out = reinterpret_cast<unsigned long long*>(dt_out);
+  double init = 1.e9;
+  unsigned long long bits;
+  std::memcpy(&bits, &init, sizeof(bits));
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, s, out, bits);
+  return dispatch(dtype, [&](auto z) {
+    using T = decltype(z);
+    hipLaunchKernelGGL(cfl_kernel<T>, dim3(blocks_for(tot, 256)), dim3(256), 0,
+                       s, static_cast<T const*>(w), out, r, p->gammad, g->dx1,
+                       g->dx2, g->dx3, g->nc1 > 1, g->nc2 > 1, g->nc3 > 1);
+    if (scale != 1.)
+      hipLaunchKernelGGL(scale_f64_kernel, dim3(1), dim3(1), 0, s, dt_out,
+                         scale);
+    return launch_status("cfl");
+  });
+}
+
+int64_t snapy_budget_workspace(const snapy_grid* g) {
+  if (!g) return 0;
+  return int64_t(g->nvar) * kBudgetBlocks;
+}
+
+int snapy_budget(const snapy_grid* g, int dtype, const void* u, double* ws,
+                 double* out, void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (!u || !ws || !out) return fail(-1, "budget: NULL buffer");
+  Range3 r = ranges(*g);
+  double vol = g->dx3 * (g->dx2 * g->dx1);
+  hipStream_t s = as_stream(stream);
+  return dispatch(dtype, [&](auto z) {
+    using T = decltype(z);
+    hipLaunchKernelGGL(budget_partial_kernel<T>,
+                       dim3(unsigned(kBudgetBlocks), unsigned(g->nvar)),
+                       dim3(kBudgetTpb), 0, s, static_cast<T const*>(u), ws, r,
+                       g->nvar, vol, kBudgetBlocks);
+    hipLaunchKernelGGL(budget_final_kernel, dim3(unsigned(g->nvar)),
+                       dim3(kBudgetTpb), 0, s, ws, out, kBudgetBlocks);
+    return launch_status("budget");
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,449 @@
+"""MeshBlock and StageExecutor: the block runtime of the stage over the C-ABI.
+
+Mirrors the reference Python surface (python/csrc/pymesh.cpp:159-327,
+python/snapy/mesh.pyi:50-113): ``MeshBlock(options)``, ``to(device)``,
+``initialize(vars) -> (vars, 0.)``, ``forward(vars, dt, stage)``,
+``max_time_step(vars)``, ``check_redo(vars)``, ``part(...)``, ``total_u``,
+``execution_report``.  ``vars`` is the same dict of tensors keyed
+``hydro_w`` / ``hydro_u`` (+ ``boundary_reference_w`` with outflow faces).
+
+Every cell update runs in the gfx950 kernels behind include/snapy_amd.h; the
+host only sequences launches on the current HIP stream.  There is no CPU
+fallback: a block on a CPU device, or a missing kernel library, raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import torch
+
+from . import _capi
+from .options import MeshBlockOptions
+from .state import FluxRegister, PrecisionPolicy, StateStore
+
+IDN, IVX, IVY, IVZ, IPR = 0, 1, 2, 3, 4
+
+
+class Integrator:
+    """harp's SSP-RK3 (u <- w0 u0 + w1 u + w2 du per stage); wght2 =
+    {1, 1/4, 2/3} as pinned by tests/test_forcing.cpp:1317-1321."""
+
+    class Stage:
+        def __init__(self, w0, w1, w2):
+            self._w = (w0, w1, w2)
+
+        def wght0(self):
+            return self._w[0]
+
+        def wght1(self):
+            return self._w[1]
+
+        def wght2(self):
+            return self._w[2]
+
+    def __init__(self, options):
+        if options.type != "rk3":
+            raise ValueError("integrator: rk3 only")
+        self.options = options
+        self.stages = [
+            Integrator.Stage(1.0, 0.0, 1.0),
+            Integrator.Stage(0.75, 0.25, 0.25),
+            Integrator.Stage(1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0),
+        ]
+        self.current_redo = 0
+        self.max_redo = 5
+
+    def stop(self, cycle: int, time: float) -> bool:
+        o = self.options
+        if o.nlim >= 0 and cycle >= o.nlim:
+            return True
+        return time >= o.tlim
+
+
+class _Buffers:
+    """Named buffers a test or driver can read back (block.module(...))."""
+
+    def __init__(self, **bufs):
+        self._b = dict(bufs)
+
+    def buffer(self, name):
+        return self._b[name]
+
+    def named_buffers(self):
+        return dict(self._b)
+
+
+class MeshBlock:
+    def __init__(self, options: MeshBlockOptions):
+        self.options = options
+        self.cycle = 0
+        self.intg = Integrator(options.intg)
+        self.pintg = self.intg
+        self.policy = PrecisionPolicy(options.hydro.precision)
+        self.device = torch.device("cpu")
+        self._state = StateStore()
+        self._freg = None
+        self._scratch_w = None
+        self._dt_dev = None
+        self._executor = StageExecutor(self, options.intg.execution)
+        c = options.coord
+        self._grid = _capi.Grid(
+            5,
+            c.nghost,
+            c.nx1,
+            c.nx2,
+            c.nx3,
+            c.nc1(),
+            c.nc2(),
+            c.nc3(),
+            c.dx(1),
+            c.dx(2),
+            c.dx(3),
+            c.x1min,
+            c.x2min,
+            c.x3min,
+        )
+        h = options.hydro
+        bcs = (ctypes.c_int32 * 6)(*[_capi.BC[n] for n in options.bcnames])
+        self._phys = _capi.Physics(
+            h.eos.gammad,
+            h.grav1,
+            h.eos.density_floor,
+            h.eos.pressure_floor,
+            _capi.RIEMANN[h.riemann],
+            int(h.recon1.shock),
+            int(h.recon23.shock),
+            bcs,
+        )
+        self._build_coord()
+
+    # ------------------------------------------------------------------ grid
+    def _build_coord(self):
+        c = self.options.coord
+        bufs = {}
+        for d in (1, 2, 3):
+            nx = (c.nx1, c.nx2, c.nx3)[d - 1]
+            lo = (c.x1min, c.x2min, c.x3min)[d - 1]
+            hi = (c.x1max, c.x2max, c.x3max)[d - 1]
+            dx = (hi - lo) / nx
+            ng = c.nghost if nx > 1 else 0
+            f = torch.linspace(lo - ng * dx, hi + ng * dx, nx + 2 * ng + 1, dtype=torch.float64)
+            bufs[f"x{d}f"] = f
+            bufs[f"x{d}v"] = 0.5 * (f[:-1] + f[1:])
+            bufs[f"dx{d}f"] = torch.ones(nx + 2 * ng, dtype=torch.float64) * dx
+        self._coord = _Buffers(**bufs)
+
+    def module(self, name: str):
+        if name == "coord":
+            return self._coord
+        if name == "hydro":
+            return self._hydro_buffers()
+        raise KeyError(name)
+
+    def _hydro_buffers(self):
+        f = self._freg
+        bufs = {}
+        if f is not None:
+            for d in (1, 2, 3):
+                if f.total(d) is not None:
+                    bufs[f"F{d}"] = f.total(d)
+            if f.face_pressure1 is not None:
+                bufs["P1"] = f.face_pressure1
+        return _Buffers(**bufs)
+
+    def shape(self):
+        c = self.options.coord
+        return (5, c.nc3(), c.nc2(), c.nc1())
+
+    def part(self, offset=(0, 0, 0), exterior=True, extend_x1=0, extend_x2=0, extend_x3=0):
+        """The slices of the block's interior (exterior=False) or of the
+        whole block, as MeshBlockImpl::part for offset (0,0,0)."""
+        if tuple(offset) != (0, 0, 0):
+            raise NotImplementedError("part: only offset (0,0,0)")
+        c = self.options.coord
+        out = [slice(None)]
+        for d, ext in ((3, extend_x3), (2, extend_x2), (1, extend_x1)):
+            nx = (c.nx1, c.nx2, c.nx3)[d - 1]
+            nc = c.nc(d)
+            if exterior:
+                out.append(slice(0, nc))
+            else:
+                lo = c.nghost if nx > 1 else 0
+                out.append(slice(lo, lo + nx + ext))
+        return tuple(out)
+
+    # -------------------------------------------------------------- devices
+    def to(self, device, dtype=None):
+        device = torch.device(device)
+        if device.type != "cuda":
+            raise _capi.SnapyError(
+                "MeshBlock: the stage runs on a HIP device only (no CPU fallback)"
+            )
+        _capi.load()
+        self.device = device
+        return self
+
+    def _dtype(self):
+        return self.policy.row("hydro_state").storage
+
+    def _ensure_buffers(self):
+        if self._freg is not None:
+            return
+        c = self.options.coord
+        active = (c.nx1 > 1, c.nx2 > 1, c.nx3 > 1)
+        shape = (c.nc3(), c.nc2(), c.nc1())
+        self._freg = FluxRegister(5, shape, active, self._dtype(), self.device)
+        if "outflow" in self.options.bcnames:
+            self._scratch_w = torch.empty(self.shape(), dtype=self._dtype(), device=self.device)
+        self._dt_dev = torch.zeros((), dtype=torch.float64, device=self.device)
+
+    def flux_register(self) -> FluxRegister:
+        self._ensure_buffers()
+        return self._freg
+
+    # ------------------------------------------------------------ lifecycle
+    def initialize(self, vars):
+        """initialize_local + finalize_initialization
+        (src/mesh/meshblock.cpp:306-343): outflow background, primitive
+        boundary fill, prim -> cons."""
+        if self.device.type != "cuda":
+            raise _capi.SnapyError("initialize: call block.to('cuda') first")
+        self._ensure_buffers()
+        w = vars["hydro_w"]
+        if tuple(w.shape) != self.shape():
+            raise ValueError(f"initialize: hydro_w has shape {tuple(w.shape)}, expected {self.shape()}")
+        w = w.to(device=self.device, dtype=self._dtype()).contiguous()
+        vars["hydro_w"] = w
+        if "outflow" in self.options.bcnames:
+            vars["boundary_reference_w"] = w.clone()
+        lib = _capi.load()
+        dt = _capi.dtype_code(w.dtype)
+        s = _capi.stream_of(self.device)
+        bref = _capi.ptr(vars.get("boundary_reference_w"))
+        _capi.check(
+            lib.snapy_apply_boundaries_prim(
+                ctypes.byref(self._grid), ctypes.byref(self._phys), dt, _capi.ptr(w), bref, s
+            ),
+            "apply_boundaries(prim)",
+        )
+        u = torch.empty_like(w)
+        _capi.check(
+            lib.snapy_prim2cons(
+                ctypes.byref(self._grid), ctypes.byref(self._phys), dt, _capi.ptr(w), _capi.ptr(u), s
+            ),
+            "prim2cons",
+        )
+        vars["hydro_u"] = u
+        self._state.adopt(u)
+        return vars, 0.0
+
+    def _adopt(self, vars):
+        u = vars["hydro_u"]
+        if self._state.u() is None or self._state.u().data_ptr() != u.data_ptr():
+            self._state.adopt(u)
+
+    # ---------------------------------------------------------------- stage
+    def forward(self, vars, dt, stage: int) -> None:
+        """One RK stage of the block (MeshBlockImpl::forward,
+        meshblock_advance.cpp:88-105): exchange (a single block has no
+        neighbours), advance, combine, boundaries.  dt is a host float or a
+        0-d FP64 device tensor (StepDt); either way the kernels read it from
+        the block's device dt, so a captured stage replays with a new dt."""
+        self._adopt(vars)
+        self._ensure_buffers()
+        if isinstance(dt, torch.Tensor):
+            self._dt_dev.copy_(dt)
+        else:
+            self._dt_dev.fill_(float(dt))
+        self._executor.run_stage(vars, stage)
+
+    def _launch_stage(self, vars, stage: int) -> None:
+        """The stage's launches on the current stream (the body a hipGraph
+        captures): backup at stage 0, EOS, sweeps, step fields, divergence +
+        combine, boundaries."""
+        lib = _capi.load()
+        g, p = ctypes.byref(self._grid), ctypes.byref(self._phys)
+        u = vars["hydro_u"]
+        w = vars["hydro_w"]
+        dtc = _capi.dtype_code(u.dtype)
+        s = _capi.stream_of(self.device)
+        freg = self._freg
+        st = self.intg.stages[stage]
+        freg.begin_stage(stage, st.wght1(), st.wght2())
+        dtp = _capi.ptr(self._dt_dev)
+        if stage == 0:
+            self._state.backup()
+        # (1) primitives
+        _capi.check(lib.snapy_cons2prim(g, p, dtc, _capi.ptr(u), _capi.ptr(w), s), "cons2prim")
+        # (2)-(4) fused reconstruction + Riemann per active direction
+        for d in (1, 2, 3):
+            t = freg.total(d)
+            if t is None:
+                continue
+            fp = freg.face_pressure1 if d == 1 else None
+            _capi.check(
+                lib.snapy_sweep(d, g, p, dtc, _capi.ptr(w), _capi.ptr(t), _capi.ptr(fp), s),
+                f"sweep x{d}",
+            )
+        # FluxRegister::end_stage: S_d <- w1 S_d + w2 dt T_d
+        for d in (1, 2, 3):
+            if freg.step(d) is not None:
+                _capi.check(
+                    lib.snapy_step_field(
+                        g, dtc, _capi.ptr(freg.total(d)), _capi.ptr(freg.step(d)), st.wght1(), st.wght2(), 0.0, dtp, s
+                    ),
+                    "step_field",
+                )
+        # (5)-(6) + combine: divergence, du = -dt div, u <- w0 u0 + w1 u + w2 du
+        fl = (ctypes.c_void_p * 3)(*[_capi.ptr(freg.total(d)) for d in (1, 2, 3)])
+        _capi.check(
+            lib.snapy_update(
+                g, dtc, fl, _capi.ptr(self._state.u0()), _capi.ptr(u), None, 0.0, dtp,
+                st.wght0(), st.wght1(), st.wght2(), s,
+            ),
+            "update",
+        )
+        # boundaries on the combined state
+        _capi.check(
+            lib.snapy_apply_boundaries(
+                g, p, dtc, _capi.ptr(u), _capi.ptr(vars.get("boundary_reference_w")), _capi.ptr(self._scratch_w), s
+            ),
+            "apply_boundaries",
+        )
+
+    def device_time_step(self, vars) -> torch.Tensor:
+        """max_time_step without the host read: the block's device dt
+        (2^-redo * cfl * min dx/(|v|+c)), for a driver that never syncs."""
+        self._ensure_buffers()
+        lib = _capi.load()
+        w = vars["hydro_w"]
+        scale = math.pow(2.0, -self.intg.current_redo) * self.intg.options.cfl
+        _capi.check(
+            lib.snapy_cfl(
+                ctypes.byref(self._grid), ctypes.byref(self._phys), _capi.dtype_code(w.dtype),
+                _capi.ptr(w), scale, _capi.ptr(self._dt_dev), _capi.stream_of(self.device),
+            ),
+            "cfl",
+        )
+        return self._dt_dev
+
+    def max_time_step(self, vars) -> float:
+        """global_time_step(local_max_time_step(vars))
+        (meshblock_advance.cpp:61-85): 2^-redo * cfl * min dx/(|v|+c) over
+        the interior of vars['hydro_w'] (one host read)."""
+        self._ensure_buffers()
+        lib = _capi.load()
+        w = vars["hydro_w"]
+        out = torch.empty((), dtype=torch.float64, device=self.device)
+        _capi.check(
+            lib.snapy_cfl(
+                ctypes.byref(self._grid), ctypes.byref(self._phys), _capi.dtype_code(w.dtype),
+                _capi.ptr(w), 1.0, _capi.ptr(out), _capi.stream_of(self.device),
+            ),
+            "cfl",
+        )
+        dt = float(out.item())
+        return math.pow(2.0, -self.intg.current_redo) * self.intg.options.cfl * dt
+
+    def check_redo(self, vars) -> int:
+        """>0 redo, 0 ok, <0 stop (meshblock.hpp:311): a non-finite state
+        restores u0 and halves dt through current_redo."""
+        u = vars["hydro_u"]
+        bad = bool((~torch.isfinite(u[self.part(exterior=False)])).any().item())
+        if not bad:
+            self.intg.current_redo = 0
+            return 0
+        if self.intg.current_redo >= self.intg.max_redo:
+            return -1
+        self._state.restore()
+        self.intg.current_redo += 1
+        return 1
+
+    def total_u(self, vars):
+        u = vars["hydro_u"] if isinstance(vars, dict) else vars
+        return u.clone()
+
+    def budget(self, vars) -> torch.Tensor:
+        """FP64 domain totals sum(u * V) per row over the interior."""
+        lib = _capi.load()
+        u = vars["hydro_u"]
+        n = lib.snapy_budget_workspace(ctypes.byref(self._grid))
+        ws = torch.empty(int(n), dtype=torch.float64, device=self.device)
+        out = torch.empty(5, dtype=torch.float64, device=self.device)
+        _capi.check(
+            lib.snapy_budget(
+                ctypes.byref(self._grid),
+                _capi.dtype_code(u.dtype),
+                _capi.ptr(u),
+                _capi.ptr(ws),
+                _capi.ptr(out),
+                _capi.stream_of(self.device),
+            ),
+            "budget",
+        )
+        return out
+
+    def execution_report(self) -> dict:
+        return self._executor.report()
+
+
+class StageExecutor:
+    """Runs a block's stages eagerly or as captured hipGraphs
+    (StageExecutor::run_stage, src/mesh/stage_executor.cpp:809-1071).
+
+    ``execution``: ``eager``, ``hip-graph`` (``cuda-graph`` is read as its
+    HIP spelling) or ``auto`` (= eager).  A graph is captured per stage after
+    one eager warm-up and keyed on the state's addresses and the StateStore
+    generation (the reference's GraphKey, stage_executor.hpp:54-77); the dt
+    is read from the block's device dt, so replays take a new dt without
+    re-capture.
+    """
+
+    def __init__(self, block: "MeshBlock", execution: str = "auto"):
+        if execution == "cuda-graph":
+            execution = "hip-graph"
+        if execution not in ("auto", "eager", "hip-graph"):
+            raise ValueError(f"integration/execution '{execution}'")
+        self.block = block
+        self.mode = "eager" if execution == "auto" else execution
+        self._graphs = {}
+        self._warm = set()
+        self._counts = {"eager": 0, "captures": 0, "replays": 0}
+
+    def _key(self, vars, stage):
+        b = self.block
+        return (
+            stage,
+            b._state.generation,
+            vars["hydro_u"].data_ptr(),
+            vars["hydro_w"].data_ptr(),
+            b._state.u0().data_ptr(),
+        )
+
+    def run_stage(self, vars, stage: int) -> None:
+        if self.mode == "eager":
+            self.block._launch_stage(vars, stage)
+            self._counts["eager"] += 1
+            return
+        key = self._key(vars, stage)
+        g = self._graphs.get(key)
+        if g is None:
+            if key not in self._warm:
+                # one eager warm-up before capture (the reference takes 3)
+                self.block._launch_stage(vars, stage)
+                self._warm.add(key)
+                self._counts["eager"] += 1
+                return
+            g = torch.cuda.CUDAGraph()
+            torch.cuda.synchronize(self.block.device)
+            with torch.cuda.graph(g):
+                self.block._launch_stage(vars, stage)
+            self._graphs[key] = g
+            self._counts["captures"] += 1
+        g.replay()
+        self._counts["replays"] += 1
+
+    def report(self) -> dict:
+        return dict(self._counts, mode=self.mode)

@@ -1,0 +1,288 @@
+"""Deck readers: MeshBlockOptions.from_yaml and the option structs it fills.
+
+Restates the YAML surface the reference reads for this path
+(MeshBlockOptionsImpl::from_yaml, src/mesh/meshblock_options.cpp:18-77, and
+the per-module readers it calls).  Keys the stage does not implement raise a
+ValueError naming the key, as the reference's strict key checks do
+(src/input/check_keys.cpp); sections that belong to subsystems outside the
+stage (outputs, species, reference-state, distribute, problem, scalar) are
+accepted and ignored.
+"""
+
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+
+import yaml
+
+# the gas constant kintera uses (kintera::constants::Rgas), J/(mol K)
+RGAS = 8.31446261815324
+
+
+def _check_keys(node, path, allowed):
+    if node is None:
+        return
+    if not isinstance(node, dict):
+        raise ValueError(f"{path} must be a map")
+    bad = [k for k in node if k not in allowed]
+    if bad:
+        raise ValueError(f"{path}: unsupported key(s) {bad}; allowed {sorted(allowed)}")
+
+
+def _pi(v):
+    """Numbers may be written with a _pi alias (options/coord.yaml)."""
+    if isinstance(v, str):
+        s = v.strip()
+        if s.endswith("_pi"):
+            return float(s[:-3] or 1.0) * math.pi
+        return float(s)
+    return float(v)
+
+
+@dataclass
+class CoordinateOptions:
+    """geometry: (src/coord/coordinate.cpp, src/options/coord.yaml)."""
+
+    type: str = "cartesian"
+    x1min: float = 0.0
+    x1max: float = 1.0
+    x2min: float = 0.0
+    x2max: float = 1.0
+    x3min: float = 0.0
+    x3max: float = 1.0
+    nx1: int = 1
+    nx2: int = 1
+    nx3: int = 1
+    nghost: int = 3
+
+    def nc(self, d: int) -> int:
+        nx = (self.nx1, self.nx2, self.nx3)[d - 1]
+        return nx + 2 * self.nghost if nx > 1 else 1
+
+    def nc1(self):
+        return self.nc(1)
+
+    def nc2(self):
+        return self.nc(2)
+
+    def nc3(self):
+        return self.nc(3)
+
+    def dx(self, d: int) -> float:
+        lo, hi, n = [
+            (self.x1min, self.x1max, self.nx1),
+            (self.x2min, self.x2max, self.nx2),
+            (self.x3min, self.x3max, self.nx3),
+        ][d - 1]
+        return (hi - lo) / n
+
+
+@dataclass
+class EquationOfStateOptions:
+    type: str = "ideal-gas"
+    gammad: float = 1.4
+    weight: float = 29.0e-3
+    density_floor: float = 1.0e-10
+    pressure_floor: float = 1.0e-10
+    limiter: bool = False
+
+    def gas_constant_dry(self) -> float:
+        # EquationOfStateImpl::gas_constant_dry (equation_of_state.cpp:213-215)
+        return RGAS / self.weight
+
+
+@dataclass
+class ReconstructOptions:
+    type: str = "weno5"
+    scale: bool = False
+    shock: bool = True
+
+
+@dataclass
+class HydroOptions:
+    eos: EquationOfStateOptions = field(default_factory=EquationOfStateOptions)
+    recon1: ReconstructOptions = field(default_factory=ReconstructOptions)
+    recon23: ReconstructOptions = field(default_factory=ReconstructOptions)
+    riemann: str = "hllc"
+    grav1: float = 0.0
+    precision: str = "float64"
+    flux_covariance: bool = True
+
+
+@dataclass
+class IntegratorOptions:
+    type: str = "rk3"
+    cfl: float = 0.9
+    tlim: float = 1.0
+    nlim: int = -1
+    implicit_scheme: int = 0
+    execution: str = "auto"
+
+
+@dataclass
+class MeshBlockOptions:
+    coord: CoordinateOptions = field(default_factory=CoordinateOptions)
+    hydro: HydroOptions = field(default_factory=HydroOptions)
+    intg: IntegratorOptions = field(default_factory=IntegratorOptions)
+    bcnames: list = field(default_factory=lambda: ["reflecting"] * 6)
+    verbose: bool = False
+    filename: str = ""
+
+    @staticmethod
+    def from_yaml(filename: str, verbose: bool = False) -> "MeshBlockOptions":
+        """snapy.MeshBlockOptions.from_yaml(filename, verbose=False)
+        (python/csrc/pymesh.cpp:28; meshblock_options.cpp:18-77)."""
+        with open(filename) as f:
+            cfg = yaml.safe_load(f) or {}
+        op = MeshBlockOptions(verbose=verbose, filename=os.fspath(filename))
+        _check_keys(
+            cfg,
+            "deck",
+            {
+                "geometry",
+                "dynamics",
+                "boundary-condition",
+                "integration",
+                "forcing",
+                "outputs",
+                "species",
+                "reference-state",
+                "distribute",
+                "problem",
+                "scalar",
+                "reactions",
+            },
+        )
+        op.coord = _read_geometry(cfg.get("geometry") or {})
+        op.hydro = _read_dynamics(cfg.get("dynamics") or {}, cfg.get("forcing") or {})
+        op.intg = _read_integration(cfg.get("integration") or {})
+        op.bcnames = _read_bcs(cfg.get("boundary-condition") or {})
+        _check_supported(op)
+        return op
+
+
+def _read_geometry(g) -> CoordinateOptions:
+    _check_keys(g, "geometry", {"type", "bounds", "cells", "interp_order"})
+    c = CoordinateOptions()
+    c.type = g.get("type", "cartesian")
+    b = g.get("bounds") or {}
+    _check_keys(b, "geometry/bounds", {"x1min", "x1max", "x2min", "x2max", "x3min", "x3max"})
+    for k in ("x1min", "x1max", "x2min", "x2max", "x3min", "x3max"):
+        if k in b:
+            setattr(c, k, _pi(b[k]))
+    cells = g.get("cells") or {}
+    _check_keys(cells, "geometry/cells", {"nx1", "nx2", "nx3", "nghost"})
+    for k in ("nx1", "nx2", "nx3", "nghost"):
+        if k in cells:
+            setattr(c, k, int(cells[k]))
+    return c
+
+
+def _read_recon(node, path) -> ReconstructOptions:
+    r = ReconstructOptions()
+    if node is None:
+        return r
+    _check_keys(node, path, {"type", "scale", "shock"})
+    r.type = node.get("type", r.type)
+    r.scale = bool(node.get("scale", r.scale))
+    r.shock = bool(node.get("shock", r.shock))
+    return r
+
+
+def _read_dynamics(d, forcing) -> HydroOptions:
+    _check_keys(
+        d,
+        "dynamics",
+        {
+            "equation-of-state",
+            "reconstruct",
+            "riemann-solver",
+            "precision",
+            "flux-covariance",
+            "verbose",
+        },
+    )
+    h = HydroOptions()
+    e = d.get("equation-of-state") or {}
+    _check_keys(
+        e,
+        "dynamics/equation-of-state",
+        {"type", "gammad", "weight", "density-floor", "pressure-floor", "limiter"},
+    )
+    h.eos.type = e.get("type", h.eos.type)
+    h.eos.gammad = float(e.get("gammad", h.eos.gammad))
+    h.eos.weight = float(e.get("weight", h.eos.weight))
+    h.eos.density_floor = float(e.get("density-floor", h.eos.density_floor))
+    h.eos.pressure_floor = float(e.get("pressure-floor", h.eos.pressure_floor))
+    h.eos.limiter = bool(e.get("limiter", h.eos.limiter))
+    rc = d.get("reconstruct") or {}
+    _check_keys(rc, "dynamics/reconstruct", {"vertical", "horizontal"})
+    h.recon1 = _read_recon(rc.get("vertical"), "dynamics/reconstruct/vertical")
+    h.recon23 = _read_recon(rc.get("horizontal"), "dynamics/reconstruct/horizontal")
+    rs = d.get("riemann-solver") or {}
+    _check_keys(rs, "dynamics/riemann-solver", {"type", "dir"})
+    h.riemann = rs.get("type", h.riemann)
+    h.precision = d.get("precision", h.precision)
+    h.flux_covariance = bool(d.get("flux-covariance", h.flux_covariance))
+    _check_keys(forcing, "forcing", {"const-gravity"})
+    if "const-gravity" in forcing:
+        cg = forcing["const-gravity"] or {}
+        _check_keys(cg, "forcing/const-gravity", {"grav1"})
+        h.grav1 = float(cg.get("grav1", 0.0))
+    return h
+
+
+def _read_integration(i) -> IntegratorOptions:
+    _check_keys(
+        i,
+        "integration",
+        {"type", "cfl", "tlim", "nlim", "ncycle_out", "implicit-scheme", "execution", "verbose"},
+    )
+    o = IntegratorOptions()
+    o.type = i.get("type", o.type)
+    o.cfl = float(i.get("cfl", o.cfl))
+    o.tlim = float(i.get("tlim", o.tlim))
+    o.nlim = int(i.get("nlim", o.nlim))
+    o.implicit_scheme = int(i.get("implicit-scheme", 0))
+    o.execution = i.get("execution", o.execution)
+    return o
+
+
+_FACES = ["x1-inner", "x1-outer", "x2-inner", "x2-outer", "x3-inner", "x3-outer"]
+
+
+def _read_bcs(bc) -> list:
+    _check_keys(bc, "boundary-condition", {"internal", "external"})
+    ext = bc.get("external") or {}
+    _check_keys(ext, "boundary-condition/external", set(_FACES))
+    return [ext.get(f, "reflecting") for f in _FACES]
+
+
+def _check_supported(op: MeshBlockOptions) -> None:
+    """Refuse deck features outside the implemented stage, loudly."""
+    if op.coord.type != "cartesian":
+        raise ValueError(f"geometry/type '{op.coord.type}': this stage implements cartesian")
+    if op.hydro.eos.type != "ideal-gas":
+        raise ValueError(f"equation-of-state type '{op.hydro.eos.type}' is not implemented")
+    if op.hydro.eos.limiter:
+        raise ValueError("equation-of-state/limiter: true is not implemented")
+    for r, name in ((op.hydro.recon1, "vertical"), (op.hydro.recon23, "horizontal")):
+        if r.type != "weno5":
+            raise ValueError(f"reconstruct/{name}/type '{r.type}': weno5 only")
+        if r.scale:
+            raise ValueError(f"reconstruct/{name}/scale: true is not implemented")
+    if op.hydro.riemann not in ("hllc", "lmars"):
+        raise ValueError(f"riemann-solver '{op.hydro.riemann}' is not implemented here")
+    if op.hydro.precision not in ("float64", "float32"):
+        raise ValueError("dynamics/precision must be float64 or float32")
+    if op.hydro.grav1 != 0.0:
+        raise ValueError("const-gravity: the gravity path is not built into this stage yet")
+    if op.intg.type != "rk3":
+        raise ValueError(f"integration/type '{op.intg.type}': rk3 only")
+    if op.intg.implicit_scheme != 0:
+        raise ValueError("integration/implicit-scheme: the vertical implicit solve is not built yet")
+    for f, n in zip(_FACES, op.bcnames):
+        if n not in ("reflecting", "periodic", "extrapolation", "outflow", "custom", "none"):
+            raise ValueError(f"boundary {f} '{n}' is not implemented")
