@@ -53,8 +53,8 @@ def sod_workload():
     block.to("cuda")
     x1v = block.module("coord").buffer("x1v").view(1, 1, -1)
     w = torch.zeros(block.shape(), dtype=torch.float64)
-    w[kIDN] = torch.where(x1v < 0.5, 1.0, 0.125)
-    w[kIPR] = torch.where(x1v < 0.5, 1.0, 0.1)
+    w[kIDN] = torch.where(x1v < 0.5, torch.tensor(1.0, dtype=torch.float64), torch.tensor(0.125, dtype=torch.float64))
+    w[kIPR] = torch.where(x1v < 0.5, torch.tensor(1.0, dtype=torch.float64), torch.tensor(0.1, dtype=torch.float64))
     vars, _ = block.initialize({"hydro_w": w.cuda()})
     cells = op.coord.nx1 * op.coord.nx2 * op.coord.nx3
     cfg = {
