@@ -95,7 +95,9 @@ int snapy_prim2cons(const snapy_grid *g, const snapy_physics *p, int dtype,
  * face states of every interior face straight from w, then the Riemann flux,
  * written into the FluxRegister total `flux` (faces lo..hi+1 along dim,
  * interior cells across it).  face_pressure (x1 only, may be NULL) receives
- * the solver's face pressure.
+ * the solver's face pressure.  At a reflecting (wall) boundary face the mass
+ * and energy rows are written as zero (FluxRegister boundary.prescribed,
+ * src/layout/seam_register.cpp:184-220).
  * Replaces: call_recon (src/recon/recon_dispatch.hpp:13-16) twice per row
  * group, and call_riemann (src/riemann/riemann.hpp:165-171) with its EOS face
  * pass (src/riemann/riemann.cpp:219-346). */

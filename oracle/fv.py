@@ -347,6 +347,23 @@ class Grid:
         return tuple(out)
 
 
+def prescribe_walls(flux, tdim, bcs, axis, grid):
+    """boundary.prescribed on the wall planes of one direction
+    (FluxRegister::apply_prescribed, src/hydro/flux_register.cpp:707-748;
+    supply_prescribed, src/layout/seam_register.cpp:184-220): at a reflecting
+    face (a FaceKind::wall, src/mesh/meshblock_options.cpp:188-209) the rows
+    other than the momentum rows {IVX, IVY, IVZ} are set to zero."""
+    nx = (grid.nx1, grid.nx2, grid.nx3)[axis - 1]
+    planes = []
+    if bcs[2 * (axis - 1)] == "reflecting":
+        planes.append(grid.nghost)
+    if bcs[2 * (axis - 1) + 1] == "reflecting":
+        planes.append(grid.nghost + nx)
+    for p in planes:
+        for row in (IDN, IPR):
+            flux[row][_sl(3, tdim - 1, slice(p, p + 1))] = 0.0
+
+
 def divergence(grid, flux):
     """CoordinateImpl::divergence, Cartesian uniform (coordinate.cpp:618-662,
     areas/volume :534-556).  flux = {1: F1, 2: F2, 3: F3} (missing = inactive)."""
@@ -587,6 +604,7 @@ class Block:
             wlr = reconstruct(self.w, tdim, shock)
             solver = hllc if d.riemann == "hllc" else lmars
             f, p = solver(wlr[ILT], wlr[IRT], d.gammad, tdim)
+            prescribe_walls(f, tdim, d.bcs, axis, g)
             flux[axis] = f
             if axis == 1:
                 self.face_pressure1 = p

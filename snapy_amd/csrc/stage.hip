@@ -93,6 +93,7 @@ struct SweepArgs {
   double gammad;
   int riemann;
   int shock;
+  int wall_lo, wall_hi;  // the sweep's lower/upper boundary face is a wall
 };
 
 // one thread per interior face.  DIR 1: the lane walks the faces along x1;
@@ -159,6 +160,15 @@ __global__ void __launch_bounds__(256)
     lmars(L, R, g, g, DIR - 1, out, &pface);
   else
     hllc(L, R, g, g, DIR - 1, out, &pface);
+  // boundary.prescribed at a wall plane (FluxRegister::apply_prescribed,
+  // flux_register.cpp:707-748, planes and free rows from supply_prescribed,
+  // seam_register.cpp:184-220): the momentum rows stay free, the mass and
+  // energy rows are zero
+  int f = DIR == 1 ? i - r.lo1 : (DIR == 2 ? j - r.lo2 : k - r.lo3);
+  if ((f == 0 && a.wall_lo) || (f == a.nf - 1 && a.wall_hi)) {
+    out[IDN] = T(0);
+    out[IPR] = T(0);
+  }
 #pragma unroll
   for (int v = 0; v < 5; ++v) flx[v * r.sv + base] = out[v];
   if (DIR == 1 && fp != nullptr) fp[base] = pface;
@@ -421,6 +431,10 @@ int snapy_sweep(int dim, const snapy_grid* g, const snapy_physics* p,
   a.gammad = p->gammad;
   a.riemann = p->riemann;
   a.shock = dim == 1 ? p->shock_x1 : p->shock_x23;
+  // a reflecting face is a wall (MeshBlockOptionsImpl::face_kind,
+  // src/mesh/meshblock_options.cpp:188-209)
+  a.wall_lo = p->bc[2 * (dim - 1)] == SNAPY_BC_REFLECTING;
+  a.wall_hi = p->bc[2 * (dim - 1) + 1] == SNAPY_BC_REFLECTING;
   int64_t tot = int64_t(a.nf);
   for (int d = 0; d < 3; ++d)
     if (d != dim - 1) tot *= nx[d];
