@@ -150,6 +150,39 @@ int snapy_apply_boundaries_prim(const snapy_grid *g, const snapy_physics *p,
                                 int dtype, void *w, const void *bref_w,
                                 void *stream);
 
+/* Hydrostatic x1 reference of every column (the well-balanced x1 path that
+ * runs whenever grav1 != 0, src/hydro/hydro_forward.cpp:479-546):
+ * ref = 5 FP64 planes (psf_lo, psf_hi, pref, dsf, dref), each
+ * (nc3, nc2, nc1), from the FP64 primitive w.  One block owns its whole x1
+ * column (nb1 = 1), uniform x1, wall clamp on.
+ * Replaces: call_hydro_ref_x1 (src/hydro/hydro_dispatch.hpp:26-42; kernel
+ * src/hydro/hydro_ref_x1_impl.h:24-272). */
+int snapy_hydro_ref_x1(const snapy_grid *g, const snapy_physics *p,
+                       const double *w, double *ref, void *stream);
+
+/* Fused well-balanced x1 sweep: perturbation reconstruction about ref with
+ * even wall ghosts, the reference face pressure/density restored with their
+ * positivity fallbacks, then the Riemann flux into the x1 total (FP64).
+ * Replaces: WellBalancedX1::forward (src/recon/well_balanced_x1.cpp:16-154),
+ * X1Reference::evaluate's face densities (src/recon/x1_reference.cpp:
+ * 177-243) and the x1 call_riemann. */
+int snapy_sweep_x1_wb(const snapy_grid *g, const snapy_physics *p, int dtype,
+                      const void *w, const double *ref, void *flux,
+                      void *face_pressure, void *stream);
+
+/* snapy_update with x1 gravity: du also receives the const-gravity source
+ * (src/forcing/const_gravity.cpp:30-55) and the face-based gravity work plus
+ * the corrected potential-energy work in place of the cell-centred work
+ * (src/hydro/hydro_forward.cpp:1049-1124, src/hydro/gravity_work.hpp).
+ * x1tab is an FP64 device table of 8 rows of nc1: x1v, the face-work
+ * factors grav1*(x1f[i+1]-x1v[i]) and grav1*(x1f[i]-x1v[i]), grav1*dx1^2/12,
+ * the three centroid-slope weights (interior cells), and a spare row. */
+int snapy_update_gravity(const snapy_grid *g, const snapy_physics *p,
+                         int dtype, const void *const flux[3], const void *u0,
+                         void *u, const void *w, const double *x1tab,
+                         const void *du_src, double dt, const double *dt_dev,
+                         double w0, double w1, double w2, void *stream);
+
 /* FP64 budget: out[v] = sum over interior cells of u[v] * volume, for
  * v < nvar, deterministic (fixed reduction tree).  out is device FP64 of
  * length nvar; ws is FP64 device workspace of at least

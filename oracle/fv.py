@@ -564,6 +564,7 @@ class Deck:
     cfl: float = 0.9
     dfloor: float = 1e-10
     pfloor: float = 1e-10
+    grav1: float = 0.0
 
 
 @dataclass
@@ -601,7 +602,16 @@ class Block:
             if g.nc(axis) == 1:
                 continue
             shock = d.shock_x1 if axis == 1 else d.shock_x23
-            wlr = reconstruct(self.w, tdim, shock)
+            if axis == 1 and d.grav1 != 0.0:
+                # the well-balanced x1 path whenever gravity is on
+                # (hydro_forward.cpp:479-546)
+                from . import gravity
+
+                mirror = tuple(b != "outflow" for b in d.bcs[0:2])
+                refl = tuple(b == "reflecting" for b in d.bcs[0:2])
+                wlr = gravity.wb_reconstruct_x1(self.w, g, d.grav1, shock, mirror, refl)
+            else:
+                wlr = reconstruct(self.w, tdim, shock)
             solver = hllc if d.riemann == "hllc" else lmars
             f, p = solver(wlr[ILT], wlr[IRT], d.gammad, tdim)
             prescribe_walls(f, tdim, d.bcs, axis, g)
@@ -613,6 +623,13 @@ class Block:
         du = np.zeros_like(div)
         sub = (slice(None),) + g.interior()
         du[sub] = -dt * div[sub]
+        if d.grav1 != 0.0:
+            from . import gravity
+
+            gravity.const_gravity(du, self.w, dt, d.grav1)
+            if 1 in flux:
+                gec = gravity.gravity_energy_correction(g, self.w, flux[1], dt, d.grav1)
+                du[IPR][..., g.nghost : g.nghost + g.nx1] += gec
         return du
 
     def forward(self, dt, stage):

@@ -1,0 +1,417 @@
+// The gravity path of the x1 sweep on gfx950: the hydrostatic x1 reference
+// scan, the fused well-balanced x1 reconstruction + Riemann sweep, and the
+// fused divergence + const gravity + face gravity work + RK combine.
+//
+// Uniform Cartesian block owning its whole x1 column (nb1 = 1), conversion
+// order 2, dry ideal gas, non-hydrostatic factor 1 (the const-gravity
+// default, src/forcing/forcing.hpp:41).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cfloat>
+
+#include "common.hpp"
+#include "point.hpp"
+
+namespace snapy {
+namespace {
+
+// ---------------------------------------------------------------------------
+// hydrostatic x1 reference, one thread per column
+// (hydro_ref_x1_impl, src/hydro/hydro_ref_x1_impl.h:24-272, uniform branch)
+// ---------------------------------------------------------------------------
+// ref layout: 5 planes of (nc3, nc2, nc1): psf_lo, psf_hi, pref, dsf, dref
+enum { R_PSF_LO = 0, R_PSF_HI = 1, R_PREF = 2, R_DSF = 3, R_DREF = 4 };
+
+__constant__ double kW6[6] = {11. / 1440., -31. / 480., 401. / 720.,
+                              401. / 720., -31. / 480., 11. / 1440.};
+__constant__ double kW6E[2][6] = {
+    {95. / 288., 1427. / 1440., -133. / 240., 241. / 720., -173. / 1440.,
+     3. / 160.},
+    {-3. / 160., 637. / 1440., 511. / 720., -43. / 240., 77. / 1440.,
+     -11. / 1440.}};
+
+__device__ __forceinline__ double wall_rop(double r0, double r1, int k) {
+  double q = r0 / r1, qk = q;
+  for (int n = 1; n < k; ++n) qk *= q;
+  double e = r1 > r0 ? r0 * qk : r0 + double(k) * (r0 - r1);
+  return e > 0. ? e : r0;
+}
+
+struct RefArgs {
+  Range3 r;
+  int iu;        // last interior x1 cell
+  double grav;   // downward magnitude (> 0)
+  double dx1;    // uniform x1 width
+  int wall_clamp;
+};
+
+__device__ __forceinline__ double rop_smooth(double const* rho,
+                                             double const* pr, int i, int jlo,
+                                             int jhi, bool ext_lo,
+                                             bool ext_hi) {
+  auto rop = [&](int j) { return rho[j] / pr[j]; };
+  if (!(ext_lo && i - 2 < jlo) && !(ext_hi && i + 2 > jhi))
+    return rop(i < jlo ? jlo : (i > jhi ? jhi : i));
+  double v[5];
+  for (int m = -2; m <= 2; ++m) {
+    int j = i + m;
+    if (j < jlo)
+      v[m + 2] = ext_lo ? wall_rop(rop(jlo), rop(jlo + 1), jlo - j) : rop(jlo);
+    else if (j > jhi)
+      v[m + 2] = ext_hi ? wall_rop(rop(jhi), rop(jhi - 1), j - jhi) : rop(jhi);
+    else
+      v[m + 2] = rop(j);
+  }
+  return (v[0] + 4. * v[1] + 6. * v[2] + 4. * v[3] + v[4]) / 16.;
+}
+
+__global__ void __launch_bounds__(64)
+    hydro_ref_x1_kernel(double const* __restrict__ w, double* __restrict__ ref,
+                        RefArgs a) {
+  Range3 const& r = a.r;
+  int64_t col = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t ncol = int64_t(r.nc2) * r.nc3;
+  if (col >= ncol) return;
+  int nc1 = r.nc1, iu = a.iu;
+  int64_t base = col * nc1;
+  double const* rho = w + base;
+  double const* pr = w + 4 * r.sv + base;
+  double* lo_ = ref + R_PSF_LO * r.sv + base;
+  double* hi_ = ref + R_PSF_HI * r.sv + base;
+  double const g = a.grav, dx = a.dx1, mp = DBL_MIN;
+  // scan from the top anchor (hydro_ref_x1_scan_impl)
+  double rdt_top = pr[iu] / rho[iu];
+  double top = pr[iu] * exp(-g * 0.5 * dx / rdt_top);
+  double face = top;
+  for (int i = iu; i >= 0; --i) {
+    double dp = g * rho[i] * dx;
+    double lo = face + dp, hi = face;
+    lo_[i] = lo > mp ? lo : mp;
+    hi_[i] = hi > mp ? hi : mp;
+    face = lo;
+  }
+  face = top;
+  for (int i = iu + 1; i < nc1; ++i) {
+    double dp = g * rho[i] * dx;
+    double lo = face, hi = lo - dp;
+    lo_[i] = lo > mp ? lo : mp;
+    hi_[i] = hi > mp ? hi : mp;
+    face = hi;
+  }
+  // cells (hydro_ref_x1_cell_impl), phys_in = phys_out = true
+  int il = nc1 - 1 - iu;
+  bool wc = a.wall_clamp;
+  bool thin = (iu - il + 1) < 5, fits = iu >= 4;
+  int jlo = wc ? il : 0, jhi = wc ? iu : nc1 - 1;
+  bool ext_lo = wc && il + 1 <= iu, ext_hi = wc && iu - 1 >= il;
+  auto facev = [&](int f) { return f < nc1 ? lo_[f] : hi_[nc1 - 1]; };
+  double rs_prev = 0.;
+  for (int i = 0; i < nc1; ++i) {
+    bool wall_in = wc && i >= il && i < il + 2;
+    bool wall_out = wc && i > iu - 2 && i <= iu;
+    bool clamp_in = wall_in && fits && !thin;
+    bool clamp_out = wall_out && fits && !thin;
+    double lo = lo_[i], hi = hi_[i];
+    double cell = 0.5 * (lo + hi);
+    double lower = lo < hi ? lo : hi, upper = lo > hi ? lo : hi;
+    auto six = [&](int start, double const* wt, bool rev) {
+      double value = 0.;
+      for (int m = 0; m < 6; ++m)
+        value += wt[rev ? 5 - m : m] * facev(start + m);
+      if (value >= lower && value <= upper) cell = value;
+    };
+    if (i >= 2 && i < nc1 - 2 && !wall_in && !wall_out) six(i - 2, kW6, false);
+    if (clamp_in) six(il, kW6E[i - il], false);
+    if (clamp_out) {
+      int s = iu + 1 - 5;
+      six(s, kW6E[4 - (i - s)], true);
+    }
+    double rs = rop_smooth(rho, pr, i, jlo, jhi, ext_lo, ext_hi);
+    double rf = i > 0 ? 0.5 * (rs_prev + rs) : rs;
+    rs_prev = rs;
+    ref[R_PREF * r.sv + base + i] = cell;
+    ref[R_DREF * r.sv + base + i] = cell * rs;
+    ref[R_DSF * r.sv + base + i] = lo * rf;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fused well-balanced x1 reconstruction + Riemann sweep
+// (WellBalancedX1::forward, src/recon/well_balanced_x1.cpp:16-154, with the
+// face densities of X1Reference::evaluate, src/recon/x1_reference.cpp:
+// 177-243), one thread per interior x1 face
+// ---------------------------------------------------------------------------
+struct WbArgs {
+  Range3 r;
+  int nf;
+  double gammad;
+  int riemann, shock;
+  int mirror_lo, mirror_hi;  // even perturbation ghosts at a physical x1 face
+  int refl_lo, refl_hi;      // reflecting wall: one face density both sides
+  int wall_lo, wall_hi;      // prescribed wall planes
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    sweep_x1_wb_kernel(T const* __restrict__ w, double const* __restrict__ ref,
+                       T* __restrict__ flx, T* __restrict__ fp, WbArgs a) {
+  Range3 const& r = a.r;
+  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t tot = int64_t(a.nf) * r.n2 * r.n3;
+  if (idx >= tot) return;
+  int fi = int(idx % a.nf);
+  int i = r.lo1 + fi;
+  int64_t rest = idx / a.nf;
+  int j = r.lo2 + int(rest % r.n2);
+  int k = r.lo3 + int(rest / r.n2);
+  int64_t row = k * r.s3 + j * r.s2;
+  int il = r.lo1, iu = r.lo1 + r.n1 - 1;
+  double const* pref = ref + R_PREF * r.sv + row;
+  double const* dref = ref + R_DREF * r.sv + row;
+  double psf = ref[R_PSF_LO * r.sv + row + i];
+  double dsf = ref[R_DSF * r.sv + row + i];
+
+  // perturbation of row c at cell m, with the even mirror at the walls
+  auto pert = [&](int c, int m) -> T {
+    int src = m;
+    if (m < il && a.mirror_lo) src = 2 * il - 1 - m;
+    if (m > iu && a.mirror_hi) src = 2 * iu + 1 - m;
+    double rf = c == IPR ? pref[src] : dref[src];
+    return T(double(w[c * r.sv + row + src]) - rf);
+  };
+
+  Prim5<T> L, R;
+  // velocities: the plain face states
+  T* lv[3] = {&L.v[0], &L.v[1], &L.v[2]};
+  T* rv[3] = {&R.v[0], &R.v[1], &R.v[2]};
+  for (int v = IVX; v <= IVZ; ++v) {
+    T const* p = w + v * r.sv + row + i;
+    T c0 = p[-3], c1 = p[-2], c2 = p[-1], c3 = p[0], c4 = p[1], c5 = p[2];
+    if (a.shock) {
+      *lv[v - 1] = weno5_plus(c0, c1, c2, c3, c4);
+      *rv[v - 1] = weno5_minus(c1, c2, c3, c4, c5);
+    } else {
+      *lv[v - 1] = cp5_plus(c0, c1, c2, c3, c4);
+      *rv[v - 1] = cp5_minus(c1, c2, c3, c4, c5);
+    }
+  }
+  // pressure perturbation (cp5 unless shock), density perturbation (weno5)
+  {
+    T c0 = pert(IPR, i - 3), c1 = pert(IPR, i - 2), c2 = pert(IPR, i - 1),
+      c3 = pert(IPR, i), c4 = pert(IPR, i + 1), c5 = pert(IPR, i + 2);
+    T pl, pr;
+    if (a.shock) {
+      pl = weno5_plus(c0, c1, c2, c3, c4);
+      pr = weno5_minus(c1, c2, c3, c4, c5);
+    } else {
+      pl = cp5_plus(c0, c1, c2, c3, c4);
+      pr = cp5_minus(c1, c2, c3, c4, c5);
+    }
+    double tl = double(pl) + psf, tr = double(pr) + psf;
+    L.p = T(tl > 0. ? tl : psf);
+    R.p = T(tr > 0. ? tr : psf);
+  }
+  {
+    T c0 = pert(IDN, i - 3), c1 = pert(IDN, i - 2), c2 = pert(IDN, i - 1),
+      c3 = pert(IDN, i), c4 = pert(IDN, i + 1), c5 = pert(IDN, i + 2);
+    T dlp = weno5_plus(c0, c1, c2, c3, c4);
+    T drp = weno5_minus(c1, c2, c3, c4, c5);
+    // face densities of the reference: cp5 rows on dref guarded by the two
+    // cells' range, else the scan's dsf; one mean value on a reflecting wall
+    double d0 = dref[i - 3], d1 = dref[i - 2], d2 = dref[i - 1], d3 = dref[i],
+           d4 = dref[i + 1], d5 = dref[i + 2];
+    double fl = cp5_plus(d0, d1, d2, d3, d4);
+    double fr = cp5_minus(d1, d2, d3, d4, d5);
+    double lo = d2 < d3 ? d2 : d3, hi = d2 > d3 ? d2 : d3;
+    double sl = (fl >= lo && fl <= hi) ? fl : dsf;
+    double sr = (fr >= lo && fr <= hi) ? fr : dsf;
+    if ((fi == 0 && a.refl_lo) || (fi == a.nf - 1 && a.refl_hi)) {
+      double mean = 0.5 * (sl + sr);
+      sl = mean;
+      sr = mean;
+    }
+    double tl = double(dlp) + sl, tr = double(drp) + sr;
+    L.r = T(tl > 0. ? tl : double(w[row + i - 1]));
+    R.r = T(tr > 0. ? tr : double(w[row + i]));
+  }
+  T g = T(a.gammad);
+  T out[5];
+  T pface;
+  if (a.riemann == SNAPY_RIEMANN_LMARS)
+    lmars(L, R, g, g, 0, out, &pface);
+  else
+    hllc(L, R, g, g, 0, out, &pface);
+  if ((fi == 0 && a.wall_lo) || (fi == a.nf - 1 && a.wall_hi)) {
+    out[IDN] = T(0);
+    out[IPR] = T(0);
+  }
+  for (int v = 0; v < 5; ++v) flx[v * r.sv + row + i] = out[v];
+  if (fp) fp[row + i] = pface;
+}
+
+// ---------------------------------------------------------------------------
+// fused divergence + const gravity + gravity work + RK combine
+// ---------------------------------------------------------------------------
+// x1 table: 8 rows of nc1 doubles
+enum { T_X1V = 0, T_GW_HI = 1, T_GW_LO = 2, T_GVAR = 3, T_WT0 = 4, T_WT1 = 5,
+       T_WT2 = 6 };
+
+struct GUpdArgs {
+  Range3 r;
+  double a1, a2, a3, vol;
+  double mdt;
+  double const* dt_dev;
+  double w0, w1, w2;
+  double grav1;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    update_gravity_kernel(T const* __restrict__ f1, T const* __restrict__ f2,
+                          T const* __restrict__ f3, T const* __restrict__ u0,
+                          T* __restrict__ u, T const* __restrict__ w,
+                          double const* __restrict__ tab,
+                          T const* __restrict__ dsrc, GUpdArgs a) {
+  Range3 const& r = a.r;
+  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
+  if (idx >= tot) return;
+  int ii = int(idx % r.n1);
+  int i = r.lo1 + ii;
+  int64_t rest = idx / r.n1;
+  int j = r.lo2 + int(rest % r.n2);
+  int k = r.lo3 + int(rest / r.n2);
+  int64_t row = k * r.s3 + j * r.s2;
+  int64_t c = row + i;
+  T A1 = T(a.a1), A2 = T(a.a2), A3 = T(a.a3), V = T(a.vol);
+  double dt = a.dt_dev ? *a.dt_dev : -a.mdt;
+  T mdt = T(-dt), tdt = T(dt);
+  T w0 = T(a.w0), w1 = T(a.w1), w2 = T(a.w2), g1 = T(a.grav1);
+  T rho = w[c], vx = w[r.sv + c];
+  T du[5];
+  for (int v = 0; v < 5; ++v) {
+    int64_t o = v * r.sv + c;
+    T div = A1 * f1[o + 1] - A1 * f1[o];
+    if (f2) div = div + (A2 * f2[o + r.s2] - A2 * f2[o]);
+    if (f3) div = div + (A3 * f3[o + r.s3] - A3 * f3[o]);
+    div = div / V;
+    du[v] = mdt * div;
+  }
+  // ConstGravityImpl::forward_split (const_gravity.cpp:35-43), nh = 1
+  du[IVX] = du[IVX] + tdt * rho * g1 * T(1.);
+  du[IPR] = du[IPR] + tdt * rho * vx * g1 * T(1.);
+  if (dsrc)
+    for (int v = 0; v < 5; ++v) du[v] = du[v] + dsrc[v * r.sv + c];
+  // gravity_energy_correction (hydro_forward.cpp:1049-1124), Cartesian
+  T const* m1 = f1 + row;  // the IDN row of F1 along this column
+  auto drho = [&](int m) {  // -dt * x1_face_difference / V at cell lo1 + m
+    int q = r.lo1 + m;
+    return mdt * ((A1 * m1[q + 1] - A1 * m1[q]) / V);
+  };
+  T gw = (T(tab[T_GW_HI * r.nc1 + i]) * (A1 * m1[i + 1]) -
+          T(tab[T_GW_LO * r.nc1 + i]) * (A1 * m1[i])) /
+         V;
+  T fgw = tdt * gw;
+  int jj = (ii < 1 ? 1 : (ii > r.n1 - 2 ? r.n1 - 2 : ii)) - 1;
+  T s = T(tab[T_WT0 * r.nc1 + i]) * drho(jj);
+  s = s + T(tab[T_WT1 * r.nc1 + i]) * drho(jj + 1);
+  s = s + T(tab[T_WT2 * r.nc1 + i]) * drho(jj + 2);
+  fgw = fgw + T(tab[T_GVAR * r.nc1 + i]) * s;
+  T ogw = tdt * rho * vx * g1 * T(1.);
+  du[IPR] = du[IPR] + (fgw - ogw);
+  for (int v = 0; v < 5; ++v) {
+    int64_t o = v * r.sv + c;
+    u[o] = w0 * u0[o] + w1 * u[o] + w2 * du[v];
+  }
+}
+
+}  // namespace
+}  // namespace snapy
+
+using namespace snapy;
+
+extern "C" {
+
+int snapy_hydro_ref_x1(const snapy_grid* g, const snapy_physics* p,
+                       const double* w, double* ref, void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (!p || !w || !ref) return fail(-1, "hydro_ref_x1: NULL argument");
+  if (g->nc1 < 6) return fail(-9, "hydro_ref_x1: x1 is not resolved");
+  if (p->grav1 == 0.) return fail(-12, "hydro_ref_x1: no x1 gravity");
+  RefArgs a;
+  a.r = ranges(*g);
+  a.iu = a.r.lo1 + g->nx1 - 1;
+  a.grav = -p->grav1;
+  a.dx1 = g->dx1;
+  a.wall_clamp = 1;
+  int64_t ncol = int64_t(g->nc2) * g->nc3;
+  hipLaunchKernelGGL(hydro_ref_x1_kernel, dim3(blocks_for(ncol, 64)), dim3(64),
+                     0, as_stream(stream), w, ref, a);
+  return launch_status("hydro_ref_x1");
+}
+
+int snapy_sweep_x1_wb(const snapy_grid* g, const snapy_physics* p, int dtype,
+                      const void* w, const double* ref, void* flux,
+                      void* face_pressure, void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (!p || !w || !ref || !flux) return fail(-1, "sweep_x1_wb: NULL argument");
+  if (g->nx1 < 2) return fail(-9, "sweep_x1_wb: x1 is not resolved");
+  if (dtype != SNAPY_F64)
+    return fail(-10, "sweep_x1_wb: the total-state reference path is FP64");
+  WbArgs a;
+  a.r = ranges(*g);
+  a.nf = g->nx1 + 1;
+  a.gammad = p->gammad;
+  a.riemann = p->riemann;
+  a.shock = p->shock_x1;
+  a.mirror_lo = p->bc[0] != SNAPY_BC_OUTFLOW;
+  a.mirror_hi = p->bc[1] != SNAPY_BC_OUTFLOW;
+  a.refl_lo = p->bc[0] == SNAPY_BC_REFLECTING;
+  a.refl_hi = p->bc[1] == SNAPY_BC_REFLECTING;
+  a.wall_lo = a.refl_lo;
+  a.wall_hi = a.refl_hi;
+  int64_t tot = int64_t(a.nf) * g->nx2 * g->nx3;
+  hipLaunchKernelGGL((sweep_x1_wb_kernel<double>), dim3(blocks_for(tot, 256)),
+                     dim3(256), 0, as_stream(stream),
+                     static_cast<double const*>(w), ref,
+                     static_cast<double*>(flux),
+                     static_cast<double*>(face_pressure), a);
+  return launch_status("sweep_x1_wb");
+}
+
+int snapy_update_gravity(const snapy_grid* g, const snapy_physics* p,
+                         int dtype, const void* const flux[3], const void* u0,
+                         void* u, const void* w, const double* x1tab,
+                         const void* du_src, double dt, const double* dt_dev,
+                         double w0, double w1, double w2, void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (!p || !flux || !flux[0] || !u0 || !u || !w || !x1tab)
+    return fail(-1, "update_gravity: NULL argument");
+  if (g->nx1 < 3) return fail(-9, "update_gravity: needs >= 3 x1 cells");
+  if (dtype != SNAPY_F64) return fail(-10, "update_gravity: FP64 state");
+  GUpdArgs a;
+  a.r = ranges(*g);
+  a.a1 = g->dx3 * g->dx2;
+  a.a2 = g->dx3 * g->dx1;
+  a.a3 = g->dx2 * g->dx1;
+  a.vol = g->dx3 * (g->dx2 * g->dx1);
+  a.mdt = -dt;
+  a.dt_dev = dt_dev;
+  a.w0 = w0;
+  a.w1 = w1;
+  a.w2 = w2;
+  a.grav1 = p->grav1;
+  int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
+  hipLaunchKernelGGL((update_gravity_kernel<double>),
+                     dim3(blocks_for(tot, 256)), dim3(256), 0,
+                     as_stream(stream), static_cast<double const*>(flux[0]),
+                     static_cast<double const*>(flux[1]),
+                     static_cast<double const*>(flux[2]),
+                     static_cast<double const*>(u0), static_cast<double*>(u),
+                     static_cast<double const*>(w), x1tab,
+                     static_cast<double const*>(du_src), a);
+  return launch_status("update_gravity");
+}
+
+}  // extern "C"

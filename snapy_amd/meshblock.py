@@ -198,6 +198,17 @@ class MeshBlock:
         if "outflow" in self.options.bcnames:
             self._scratch_w = torch.empty(self.shape(), dtype=self._dtype(), device=self.device)
         self._dt_dev = torch.zeros((), dtype=torch.float64, device=self.device)
+        h = self.options.hydro
+        self._ref = None
+        self._x1tab = None
+        if h.grav1 != 0.0:
+            from .geometry import x1_gravity_table
+
+            self._ref = torch.zeros((5, c.nc3(), c.nc2(), c.nc1()), dtype=torch.float64, device=self.device)
+            cb = self._coord
+            self._x1tab = x1_gravity_table(
+                cb.buffer("x1f"), cb.buffer("x1v"), c.nghost, c.nx1, h.grav1
+            ).to(self.device)
 
     def flux_register(self) -> FluxRegister:
         self._ensure_buffers()
@@ -275,14 +286,24 @@ class MeshBlock:
         dtp = _capi.ptr(self._dt_dev)
         if stage == 0:
             self._state.backup()
+        grav = self._ref is not None
         # (1) primitives
         _capi.check(lib.snapy_cons2prim(g, p, dtc, _capi.ptr(u), _capi.ptr(w), s), "cons2prim")
-        # (2)-(4) fused reconstruction + Riemann per active direction
+        # (2)-(4) fused reconstruction + Riemann per active direction; with
+        # x1 gravity the x1 sweep is the well-balanced one about the
+        # hydrostatic reference of every column
         for d in (1, 2, 3):
             t = freg.total(d)
             if t is None:
                 continue
             fp = freg.face_pressure1 if d == 1 else None
+            if d == 1 and grav:
+                _capi.check(lib.snapy_hydro_ref_x1(g, p, _capi.ptr(w), _capi.ptr(self._ref), s), "hydro_ref_x1")
+                _capi.check(
+                    lib.snapy_sweep_x1_wb(g, p, dtc, _capi.ptr(w), _capi.ptr(self._ref), _capi.ptr(t), _capi.ptr(fp), s),
+                    "sweep x1 (well-balanced)",
+                )
+                continue
             _capi.check(
                 lib.snapy_sweep(d, g, p, dtc, _capi.ptr(w), _capi.ptr(t), _capi.ptr(fp), s),
                 f"sweep x{d}",
@@ -298,13 +319,22 @@ class MeshBlock:
                 )
         # (5)-(6) + combine: divergence, du = -dt div, u <- w0 u0 + w1 u + w2 du
         fl = (ctypes.c_void_p * 3)(*[_capi.ptr(freg.total(d)) for d in (1, 2, 3)])
-        _capi.check(
-            lib.snapy_update(
-                g, dtc, fl, _capi.ptr(self._state.u0()), _capi.ptr(u), None, 0.0, dtp,
-                st.wght0(), st.wght1(), st.wght2(), s,
-            ),
-            "update",
-        )
+        if grav:
+            _capi.check(
+                lib.snapy_update_gravity(
+                    g, p, dtc, fl, _capi.ptr(self._state.u0()), _capi.ptr(u), _capi.ptr(w),
+                    _capi.ptr(self._x1tab), None, 0.0, dtp, st.wght0(), st.wght1(), st.wght2(), s,
+                ),
+                "update (gravity)",
+            )
+        else:
+            _capi.check(
+                lib.snapy_update(
+                    g, dtc, fl, _capi.ptr(self._state.u0()), _capi.ptr(u), None, 0.0, dtp,
+                    st.wght0(), st.wght1(), st.wght2(), s,
+                ),
+                "update",
+            )
         # boundaries on the combined state
         _capi.check(
             lib.snapy_apply_boundaries(

@@ -277,8 +277,19 @@ def _check_supported(op: MeshBlockOptions) -> None:
         raise ValueError(f"riemann-solver '{op.hydro.riemann}' is not implemented here")
     if op.hydro.precision not in ("float64", "float32"):
         raise ValueError("dynamics/precision must be float64 or float32")
+    c = op.coord
     if op.hydro.grav1 != 0.0:
-        raise ValueError("const-gravity: the gravity path is not built into this stage yet")
+        if c.nx1 < 6:
+            raise ValueError("const-gravity grav1 needs a resolved x1 (nx1 >= 6)")
+        if "periodic" in op.bcnames[0:2]:
+            raise ValueError("const-gravity grav1 with a periodic x1 column is not implemented")
+        if op.hydro.precision != "float64":
+            raise ValueError("const-gravity: the gravity path runs in FP64 storage here")
+    if op.hydro.flux_covariance and c.nx1 > 2 and (c.nx2 > 1 or c.nx3 > 1):
+        raise ValueError(
+            "dynamics/flux-covariance: true (the reference default) is not implemented; "
+            "set flux-covariance: false"
+        )
     if op.intg.type != "rk3":
         raise ValueError(f"integration/type '{op.intg.type}': rk3 only")
     if op.intg.implicit_scheme != 0:
