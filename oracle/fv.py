@@ -527,12 +527,20 @@ def apply_boundaries(q, bcs, gammad, bref=None, primitive=False, dfloor=1e-10, p
 # CFL (HydroImpl::max_time_step explicit branch, src/hydro/hydro.cpp:372-396;
 # MeshBlockImpl::global_time_step, src/mesh/meshblock_advance.cpp:65-72)
 # ---------------------------------------------------------------------------
-def max_time_step(grid, w, gammad):
+def max_time_step(grid, w, gammad, scheme=0, cfl=0.9, advection_cfl=1.0):
+    """Explicit branch, or with an implicit scheme the icorr branch
+    (hydro.cpp:326-370): an implicit direction is bounded by advection,
+    |v| * cfl / advection_cfl."""
     cs = np.sqrt(gammad * w[IPR] / w[IDN])
     sub = grid.interior()
     dt = [1.0e9, 1.0e9, 1.0e9]
+    adv = cfl / advection_cfl
     if cs.shape[2] > 1:
-        dt[0] = (grid.dx(1) / (np.abs(w[IVX]) + cs))[sub].min()
+        if scheme and (scheme & 1) and not (cs.shape[0] == 1 and cs.shape[1] == 1):
+            with np.errstate(divide="ignore"):
+                dt[0] = (grid.dx(1) / (np.abs(w[IVX]) * adv))[sub].min()
+        else:
+            dt[0] = (grid.dx(1) / (np.abs(w[IVX]) + cs))[sub].min()
     if cs.shape[1] > 1:
         dt[1] = (grid.dx(2) / (np.abs(w[IVY]) + cs))[sub].min()
     if cs.shape[0] > 1:
@@ -565,6 +573,7 @@ class Deck:
     dfloor: float = 1e-10
     pfloor: float = 1e-10
     grav1: float = 0.0
+    implicit_scheme: int = 0
 
 
 @dataclass
@@ -630,18 +639,45 @@ class Block:
             if 1 in flux:
                 gec = gravity.gravity_energy_correction(g, self.w, flux[1], dt, d.grav1)
                 du[IPR][..., g.nghost : g.nghost + g.nx1] += gec
+        if d.implicit_scheme == 9:
+            # the VIC with the stage weight inside the solve
+            # (hydro_forward.cpp:1137-1180)
+            du = self._implicit(du, dt * RK3_WEIGHTS[self._stage][2])
+        return du
+
+    def _implicit(self, du, dt_corr):
+        from . import implicit
+
+        g = self.deck.grid
+        ng = g.nghost
+        k, j, i = g.interior()
+        n = g.nx1
+        wc = self.w[:, k, j, ng - 1 : ng + n + 1]  # (5, nk, nj, n + 2)
+        ncol = wc.shape[1] * wc.shape[2]
+        wcol = np.moveaxis(wc, 0, -1).reshape(ncol, n + 2, 5)
+        dcol = np.moveaxis(du[:, k, j, ng : ng + n], 0, -1).reshape(ncol, n, 5)
+        x1f = g.faces(1)[ng : ng + n + 1]
+        x1v = g.centers(1)[ng : ng + n]
+        out = implicit.vic_full(
+            dcol, wcol, self.deck.gammad, self.deck.grav1, dt_corr, g.dx(1), x1f, x1v,
+            g.dx(3) * g.dx(2), g.dx(3) * (g.dx(2) * g.dx(1)),
+        )
+        du = du.copy()
+        du[:, k, j, ng : ng + n] = np.moveaxis(out.reshape(wc.shape[1], wc.shape[2], n, 5), -1, 0)
         return du
 
     def forward(self, dt, stage):
         d = self.deck
         if stage == 0:
             self.u0 = self.u.copy()
+        self._stage = stage
         du = self.tendency(dt)
         self.u = rk_combine(self.u0, self.u, du, stage)
         apply_boundaries(self.u, d.bcs, d.gammad, self.bref, dfloor=d.dfloor, pfloor=d.pfloor)
 
     def max_time_step(self):
-        return self.deck.cfl * max_time_step(self.deck.grid, self.w, self.deck.gammad)
+        d = self.deck
+        return d.cfl * max_time_step(d.grid, self.w, d.gammad, d.implicit_scheme, d.cfl)
 
 
 def sod_deck(nx=256):
