@@ -70,6 +70,9 @@ typedef struct snapy_physics {
   int32_t shock_x1;      /* reconstruct/vertical shock: weno5 on every row */
   int32_t shock_x23;     /* reconstruct/horizontal shock */
   int32_t bc[6];         /* x1-inner, x1-outer, x2-inner, ... x3-outer */
+  int32_t implicit_scheme; /* integration/implicit-scheme (0, 1, 9) */
+  double advection;      /* cfl / implicit-advection-cfl (CFL of an implicit
+                            direction, src/hydro/hydro.cpp:326-370) */
 } snapy_physics;
 
 /* ---- entry points ----------------------------------------------------- */
@@ -176,12 +179,37 @@ int snapy_sweep_x1_wb(const snapy_grid *g, const snapy_physics *p, int dtype,
  * (src/hydro/hydro_forward.cpp:1049-1124, src/hydro/gravity_work.hpp).
  * x1tab is an FP64 device table of 8 rows of nc1: x1v, the face-work
  * factors grav1*(x1f[i+1]-x1v[i]) and grav1*(x1f[i]-x1v[i]), grav1*dx1^2/12,
- * the three centroid-slope weights (interior cells), and a spare row. */
+ * the three centroid-slope weights (interior cells), and a spare row.
+ * With u0 NULL the kernel writes the stage increment du into u instead of
+ * combining it (the input of snapy_vic_full). */
 int snapy_update_gravity(const snapy_grid *g, const snapy_physics *p,
                          int dtype, const void *const flux[3], const void *u0,
                          void *u, const void *w, const double *x1tab,
                          const void *du_src, double dt, const double *dt_dev,
                          double w0, double w1, double w2, void *stream);
+
+/* Vertical implicit correction, full 5x5 matrix (implicit-scheme 9), FP64,
+ * one column per thread, then the RK combine u = w0*u0 + w1*u + w2*du'
+ * with du' the corrected increment (interior cells).  du is the explicit
+ * stage increment (snapy_update_gravity with u0 NULL).  The solve runs with
+ * dt_c = wstage * dt (the stage weight inside the solve,
+ * src/hydro/hydro_forward.cpp:1137-1175).  vtab: FP64 rows of nc1 (face
+ * work weights lo/hi, corrected-PE bands lo/mid/hi, cell potential,
+ * face-minus-cell potentials lo/hi); t0, t1 the wall third points; x1tab as
+ * for snapy_update_gravity.  ws: snapy_vic_workspace(g) doubles; status:
+ * nx2*nx3 ints, 0 = solved, 1 = singular (that column keeps its explicit
+ * increment).
+ * Replaces: vic_stage assemble/solve/redistribute (src/implicit/
+ * implicit_dispatch.hpp:50-57, src/implicit/vic_kernels.h:66-297) and
+ * ImplicitHydroImpl::forward_masked (src/implicit/implicit_hydro.cpp:
+ * 301-623), radial-exact coupling with Woodbury, no solids. */
+int64_t snapy_vic_workspace(const snapy_grid *g);
+int snapy_vic_full(const snapy_grid *g, const snapy_physics *p, const double *w,
+                   const double *du, const double *u0, double *u,
+                   const double *vtab, const double *x1tab, double t0,
+                   double t1, double dt, const double *dt_dev, double wstage,
+                   double w0, double w1, double w2, double *ws, int *status,
+                   void *stream);
 
 /* FP64 budget: out[v] = sum over interior cells of u[v] * volume, for
  * v < nvar, deterministic (fixed reduction tree).  out is device FP64 of

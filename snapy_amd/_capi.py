@@ -56,6 +56,8 @@ class Physics(ctypes.Structure):
         ("shock_x1", ctypes.c_int32),
         ("shock_x23", ctypes.c_int32),
         ("bc", ctypes.c_int32 * 6),
+        ("implicit_scheme", ctypes.c_int32),
+        ("advection", ctypes.c_double),
     ]
 
 
@@ -81,6 +83,8 @@ SIGNATURES = {
     "snapy_hydro_ref_x1": (_I, [_GP, _PP, _P, _P, _P]),
     "snapy_sweep_x1_wb": (_I, [_GP, _PP, _I, _P, _P, _P, _P, _P]),
     "snapy_update_gravity": (_I, [_GP, _PP, _I, _P * 3, _P, _P, _P, _P, _P, _D, _P, _D, _D, _D, _P]),
+    "snapy_vic_workspace": (ctypes.c_int64, [_GP]),
+    "snapy_vic_full": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P]),
     "snapy_budget_workspace": (ctypes.c_int64, [_GP]),
     "snapy_budget": (_I, [_GP, _I, _P, _P, _P, _P]),
 }

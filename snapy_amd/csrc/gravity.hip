@@ -266,7 +266,9 @@ struct GUpdArgs {
   double grav1;
 };
 
-template <typename T>
+// WRITE_DU: store the stage increment du (for the implicit solve to finish)
+// instead of combining it into u
+template <typename T, bool WRITE_DU>
 __global__ void __launch_bounds__(256)
     update_gravity_kernel(T const* __restrict__ f1, T const* __restrict__ f2,
                           T const* __restrict__ f3, T const* __restrict__ u0,
@@ -322,7 +324,10 @@ __global__ void __launch_bounds__(256)
   du[IPR] = du[IPR] + (fgw - ogw);
   for (int v = 0; v < 5; ++v) {
     int64_t o = v * r.sv + c;
-    u[o] = w0 * u0[o] + w1 * u[o] + w2 * du[v];
+    if (WRITE_DU)
+      u[o] = du[v];
+    else
+      u[o] = w0 * u0[o] + w1 * u[o] + w2 * du[v];
   }
 }
 
@@ -386,7 +391,7 @@ int snapy_update_gravity(const snapy_grid* g, const snapy_physics* p,
                          const void* du_src, double dt, const double* dt_dev,
                          double w0, double w1, double w2, void* stream) {
   if (int e = check_grid(g)) return e;
-  if (!p || !flux || !flux[0] || !u0 || !u || !w || !x1tab)
+  if (!p || !flux || !flux[0] || !u || !w || !x1tab)
     return fail(-1, "update_gravity: NULL argument");
   if (g->nx1 < 3) return fail(-9, "update_gravity: needs >= 3 x1 cells");
   if (dtype != SNAPY_F64) return fail(-10, "update_gravity: FP64 state");
@@ -403,7 +408,9 @@ int snapy_update_gravity(const snapy_grid* g, const snapy_physics* p,
   a.w2 = w2;
   a.grav1 = p->grav1;
   int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
-  hipLaunchKernelGGL((update_gravity_kernel<double>),
+  auto kern = u0 ? update_gravity_kernel<double, false>
+                 : update_gravity_kernel<double, true>;
+  hipLaunchKernelGGL(kern,
                      dim3(blocks_for(tot, 256)), dim3(256), 0,
                      as_stream(stream), static_cast<double const*>(flux[0]),
                      static_cast<double const*>(flux[1]),

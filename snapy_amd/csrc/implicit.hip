@@ -1,0 +1,467 @@
+// The vertical implicit correction (VIC), full 5x5 matrix (implicit-scheme
+// 9), in FP64 on gfx950: one thread owns one x1 column and runs the whole
+// column program -- block assembly, the corrected-PE coupling, the
+// block-tridiagonal forward sweep with two Woodbury right-hand sides, back
+// substitution, the 2x2 Woodbury correction, the mass redistribution, the
+// post-solve remainder -- and writes the RK-combined state.  The factored
+// blocks and partial solutions live in a column-interleaved FP64 workspace
+// (layer-major, columns fastest, so a wavefront's accesses are coalesced).
+//
+// Restates ImplicitHydroImpl::forward_masked (src/implicit/
+// implicit_hydro.cpp:301-623) for a Cartesian block owning its whole x1
+// column, no solids, dry ideal gas, no FluxRegister posts.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "common.hpp"
+#include "point.hpp"
+
+namespace snapy {
+namespace {
+
+constexpr int NW = 40;  // workspace doubles per layer: A' 25, d 5, z0 5, z1 5
+
+// v-table rows (nc1 each)
+enum { V_WLO = 0, V_WHI, V_KLO, V_KMID, V_KHI, V_PHI, V_DPLO, V_DPHI };
+// x1 table rows used here (see gravity.hip)
+enum { T_GVAR = 3, T_WT0 = 4, T_WT1 = 5, T_WT2 = 6 };
+
+struct VicArgs {
+  Range3 r;
+  int n;  // layers (nx1)
+  double gamma, grav, area, vol, t0, t1;
+  double const* dt_dev;
+  double dt_host;  // used when dt_dev is NULL
+  double wstage;   // the stage weight inside the solve: dt_c = wstage * dt
+  double w0, w1, w2;
+};
+
+using M5 = double[5][5];
+
+// FluxJacobian (flux_decomposition_impl.h:105-126), dir 0
+__device__ __forceinline__ void flux_jacobian(M5 f, double gm1, double const* w) {
+  double v1 = w[IVX], v2 = w[IVY], v3 = w[IVZ], rho = w[IDN], pres = w[IPR];
+  double s2 = v1 * v1 + v2 * v2 + v3 * v3;
+  double c1 = ((gm1 - 1) * s2 / 2 - (gm1 + 1) / gm1 * pres / rho) * v1;
+  double c2 = (gm1 + 1) / gm1 * pres / rho + s2 / 2 - gm1 * v1 * v1;
+  f[0][0] = 0.; f[0][1] = 1.; f[0][2] = 0.; f[0][3] = 0.; f[0][4] = 0.;
+  f[1][0] = gm1 * s2 / 2 - v1 * v1; f[1][1] = (2. - gm1) * v1;
+  f[1][2] = -gm1 * v2; f[1][3] = -gm1 * v3; f[1][4] = gm1;
+  f[2][0] = -v1 * v2; f[2][1] = v2; f[2][2] = v1; f[2][3] = 0.; f[2][4] = 0.;
+  f[3][0] = -v1 * v3; f[3][1] = v3; f[3][2] = 0.; f[3][3] = v1; f[3][4] = 0.;
+  f[4][0] = c1; f[4][1] = c2; f[4][2] = -gm1 * v2 * v1;
+  f[4][3] = -gm1 * v3 * v1; f[4][4] = (gm1 + 1) * v1;
+}
+
+// |A| = R Lambda R^-1 at the Roe average of wl, wr (RoeAverage, SoundSpeed,
+// Eigenvalue, Eigenvector: flux_decomposition_impl.h:13-103)
+__device__ __forceinline__ void roe_abs(M5 out, double gm1, double const* wl,
+                                        double const* wr) {
+  double sdl = sqrt(wl[IDN]), sdr = sqrt(wr[IDN]);
+  double is = 1.0 / (sdl + sdr);
+  double pr[5];
+  pr[IDN] = sdl * sdr;
+  for (int v = IVX; v <= IVZ; ++v) pr[v] = (sdl * wl[v] + sdr * wr[v]) * is;
+  double el = wl[IPR] / gm1 +
+              0.5 * wl[IDN] * (wl[IVX] * wl[IVX] + wl[IVY] * wl[IVY] + wl[IVZ] * wl[IVZ]);
+  double er = wr[IPR] / gm1 +
+              0.5 * wr[IDN] * (wr[IVX] * wr[IVX] + wr[IVY] * wr[IVY] + wr[IVZ] * wr[IVZ]);
+  double hbar = ((el + wl[IPR]) / sdl + (er + wr[IPR]) / sdr) * is;
+  pr[IPR] = (hbar - 0.5 * (pr[IVX] * pr[IVX] + pr[IVY] * pr[IVY] + pr[IVZ] * pr[IVZ])) *
+            gm1 / (gm1 + 1.) * pr[IDN];
+  double cs = sqrt(pr[IPR] * (gm1 + 1.) / pr[IDN]);
+  double r = pr[IDN], u = pr[IVX], v = pr[IVY], w = pr[IVZ], p = pr[IPR];
+  double ke = 0.5 * (u * u + v * v + w * w);
+  double hp = (gm1 + 1.) / gm1 * p / r;
+  double h = hp + ke;
+  double lam[5] = {fabs(u - cs), fabs(u), fabs(u + cs), fabs(u), fabs(u)};
+  double R[5][5] = {{1., 1., 1., 0., 0.},
+                    {u - cs, u, u + cs, 0., 0.},
+                    {v, v, v, 1., 0.},
+                    {w, w, w, 0., 1.},
+                    {h - u * cs, ke, h + u * cs, v, w}};
+  double Ri[5][5] = {
+      {(cs * ke + u * hp) / (2. * cs * hp), (-hp - cs * u) / (2. * cs * hp),
+       -v / (2. * hp), -w / (2. * hp), 1. / (2. * hp)},
+      {(hp - ke) / hp, u / hp, v / hp, w / hp, -1. / hp},
+      {(cs * ke - u * hp) / (2. * cs * hp), (hp - cs * u) / (2. * cs * hp),
+       -v / (2. * hp), -w / (2. * hp), 1. / (2. * hp)},
+      {-v, 0., 1., 0., 0.},
+      {-w, 0., 0., 1., 0.}};
+  for (int a = 0; a < 5; ++a)
+    for (int b = 0; b < 5; ++b) {
+      double s = 0.;
+      for (int k = 0; k < 5; ++k) s += (R[a][k] * lam[k]) * Ri[k][b];
+      out[a][b] = s;
+    }
+}
+
+// ludcmp with implicit row scaling and partial pivoting (Crout), then
+// lubksb on NRHS right-hand-side columns (src/math/ludcmp.h, lubksb.h).
+// Returns false on a singular or non-finite factorisation.
+template <int NRHS>
+__device__ __forceinline__ bool lu_solve(M5 a, double (*rhs)[NRHS]) {
+  int indx[5];
+  double vv[5], scale[5];
+  const double tol = 8. * 5. * 2.220446049250313e-16;
+  for (int i = 0; i < 5; ++i) {
+    double big = 0.;
+    for (int j = 0; j < 5; ++j) {
+      double t = fabs(a[i][j]);
+      if (t > big) big = t;
+    }
+    if (big == 0.) return false;
+    scale[i] = big;
+    vv[i] = 1.0 / big;
+    if (!isfinite(vv[i])) return false;
+  }
+  for (int j = 0; j < 5; ++j) {
+    for (int i = 0; i < j; ++i) {
+      double sum = a[i][j];
+      for (int k = 0; k < i; ++k) sum -= a[i][k] * a[k][j];
+      a[i][j] = sum;
+    }
+    double big = 0.;
+    int imax = j;
+    for (int i = j; i < 5; ++i) {
+      double sum = a[i][j];
+      for (int k = 0; k < j; ++k) sum -= a[i][k] * a[k][j];
+      a[i][j] = sum;
+      double dum = vv[i] * fabs(sum);
+      if (!isfinite(dum)) return false;
+      if (dum >= big) {
+        big = dum;
+        imax = i;
+      }
+    }
+    if (j != imax) {
+      for (int k = 0; k < 5; ++k) {
+        double d = a[imax][k];
+        a[imax][k] = a[j][k];
+        a[j][k] = d;
+      }
+      vv[imax] = vv[j];
+      double t = scale[imax];
+      scale[imax] = scale[j];
+      scale[j] = t;
+    }
+    indx[j] = imax;
+    if (fabs(a[j][j]) / scale[j] <= tol) return false;
+    if (j != 4) {
+      double dum = 1.0 / a[j][j];
+      for (int i = j + 1; i < 5; ++i) a[i][j] *= dum;
+    }
+  }
+  for (int i = 0; i < 5; ++i) {
+    int ip = indx[i];
+    if (ip != i)
+      for (int c = 0; c < NRHS; ++c) {
+        double t = rhs[ip][c];
+        rhs[ip][c] = rhs[i][c];
+        rhs[i][c] = t;
+      }
+    for (int j = 0; j < i; ++j)
+      for (int c = 0; c < NRHS; ++c) rhs[i][c] -= a[i][j] * rhs[j][c];
+  }
+  for (int i = 4; i >= 0; --i) {
+    for (int j = i + 1; j < 5; ++j)
+      for (int c = 0; c < NRHS; ++c) rhs[i][c] -= a[i][j] * rhs[j][c];
+    for (int c = 0; c < NRHS; ++c) rhs[i][c] /= a[i][i];
+  }
+  return true;
+}
+
+__global__ void __launch_bounds__(64)
+    vic_full_kernel(double const* __restrict__ w, double const* __restrict__ du,
+                    double const* __restrict__ u0, double* __restrict__ u,
+                    double const* __restrict__ vtab,
+                    double const* __restrict__ xtab, double* __restrict__ ws,
+                    int* __restrict__ status, VicArgs a) {
+  Range3 const& r = a.r;
+  int64_t col = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t ncol = int64_t(r.n2) * r.n3;
+  if (col >= ncol) return;
+  int j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
+  int64_t row = k * r.s3 + j * r.s2 + r.lo1;  // interior cell 0 of the column
+  int n = a.n, nc1 = r.nc1, lo1 = r.lo1;
+  double const gm1 = a.gamma - 1.;
+  double const grav = a.grav, A = a.area, V = a.vol;
+  double const dt = (a.dt_dev ? *a.dt_dev : a.dt_host) * a.wstage;
+  double const hiv = 0.5 / V;
+  auto W = [&](int v, int i) { return w[v * r.sv + row + i]; };      // i in [-1, n]
+  auto DU = [&](int v, int i) { return du[v * r.sv + row + i]; };    // i in [0, n)
+  auto WS = [&](int l, int e) -> double& { return ws[(int64_t(l) * NW + e) * ncol + col]; };
+  auto tab = [&](double const* t, int rowi, int i) { return t[rowi * nc1 + lo1 + i]; };
+  // the corrected-PE rows applied to the explicit mass column at cell i
+  auto rows_at = [&](int i, auto q) {
+    double s = tab(vtab, V_KMID, i) * q(i);
+    if (i > 0) s = s + tab(vtab, V_KLO, i) * q(i - 1);
+    if (i < n - 1) s = s + tab(vtab, V_KHI, i) * q(i + 1);
+    if (i == 0) s = s + a.t0 * q(2);
+    if (i == n - 1) s = s + a.t1 * q(n - 3);
+    return s;
+  };
+  auto mass0 = [&](int i) { return DU(IDN, i); };
+
+  // ---- forward sweep ------------------------------------------------------
+  double ap[5][5], dp[5], z0p[5], z1p[5];  // the previous layer's solution
+  bool ok = true;
+  for (int i = 0; i < n && ok; ++i) {
+    double wl[5], wc[5], wr[5];
+    for (int v = 0; v < 5; ++v) {
+      wl[v] = W(v, i - 1);
+      wc[v] = W(v, i);
+      wr[v] = W(v, i + 1);
+    }
+    double am[5][5], apm[5][5], dc[5][5];
+    roe_abs(am, 0.5 * (a.gamma + a.gamma) - 1., wl, wc);
+    roe_abs(apm, 0.5 * (a.gamma + a.gamma) - 1., wc, wr);
+    double bb[5][5], cc[5][5], aa[5][5];
+    {
+      double dprev[5][5];
+      flux_jacobian(dprev, gm1, wl);
+      for (int p = 0; p < 5; ++p)
+        for (int q = 0; q < 5; ++q) bb[p][q] = -(am[p][q] + dprev[p][q]) * A * hiv;
+    }
+    {
+      double dnext[5][5];
+      flux_jacobian(dnext, gm1, wr);
+      for (int p = 0; p < 5; ++p)
+        for (int q = 0; q < 5; ++q) cc[p][q] = -(apm[p][q] - dnext[p][q]) * A * hiv;
+    }
+    flux_jacobian(dc, gm1, wc);
+    for (int p = 0; p < 5; ++p)
+      for (int q = 0; q < 5; ++q)
+        aa[p][q] = (am[p][q] * A + apm[p][q] * A + (A - A) * dc[p][q]) * hiv;
+    aa[IVX][IDN] -= grav;
+    aa[IPR][IVX] -= grav;
+    for (int p = 0; p < 5; ++p) aa[p][p] += 1. / dt;
+    // gravity-work rows (vic_gravity_work_impl.h:24-36)
+    double lo = tab(vtab, V_WLO, i), up = tab(vtab, V_WHI, i);
+    for (int q = 0; q < 5; ++q) {
+      aa[IPR][q] -= grav * (up * apm[0][q] - lo * am[0][q]);
+      bb[IPR][q] -= grav * lo * am[0][q];
+      cc[IPR][q] += grav * up * apm[0][q];
+    }
+    aa[IPR][IVX] += grav * (1. - lo - up);
+    bb[IPR][IVX] -= grav * lo;
+    cc[IPR][IVX] -= grav * up;
+    // the wall mirror merges the boundary face: Bnd = (1, -1, 1, 1, 1)
+    if (i == 0)
+      for (int p = 0; p < 5; ++p)
+        for (int q = 0; q < 5; ++q) aa[p][q] += bb[p][q] * (q == IVX ? -1. : 1.);
+    if (i == n - 1)
+      for (int p = 0; p < 5; ++p)
+        for (int q = 0; q < 5; ++q) aa[p][q] += cc[p][q] * (q == IVX ? -1. : 1.);
+    // couple_radial_exact (implicit_hydro.cpp:408-446)
+    aa[IPR][IDN] -= tab(vtab, V_KMID, i) / dt;
+    bb[IPR][IDN] -= tab(vtab, V_KLO, i) / dt;
+    cc[IPR][IDN] -= tab(vtab, V_KHI, i) / dt;
+    // right-hand sides: du/dt (energy without the explicit corrected PE),
+    // and the unit E rows at the two wall cells
+    double rhs[5];
+    rhs[0] = DU(IDN, i) / dt;
+    rhs[1] = DU(IVX, i) / dt;
+    rhs[2] = DU(IVY, i) / dt;
+    rhs[3] = DU(IVZ, i) / dt;
+    rhs[4] = (DU(IPR, i) - rows_at(i, mass0)) / dt;
+    double sol[5][8];
+    for (int p = 0; p < 5; ++p) {
+      for (int q = 0; q < 5; ++q) sol[p][q] = cc[p][q];
+      sol[p][5] = rhs[p];
+      sol[p][6] = (i == 0 && p == 4) ? 1. : 0.;
+      sol[p][7] = (i == n - 1 && p == 4) ? 1. : 0.;
+    }
+    if (i > 0) {
+      // a_i -= b_i A'_{i-1}; rhs -= b_i d_{i-1} (and the Woodbury columns)
+      for (int p = 0; p < 5; ++p) {
+        for (int q = 0; q < 5; ++q) {
+          double s = 0.;
+          for (int m = 0; m < 5; ++m) s += bb[p][m] * ap[m][q];
+          aa[p][q] -= s;
+        }
+        double s5 = 0., s6 = 0., s7 = 0.;
+        for (int m = 0; m < 5; ++m) {
+          s5 += bb[p][m] * dp[m];
+          s6 += bb[p][m] * z0p[m];
+          s7 += bb[p][m] * z1p[m];
+        }
+        sol[p][5] -= s5;
+        sol[p][6] -= s6;
+        sol[p][7] -= s7;
+      }
+    }
+    if (!lu_solve<8>(aa, sol)) {
+      ok = false;
+      break;
+    }
+    for (int p = 0; p < 5; ++p) {
+      for (int q = 0; q < 5; ++q) {
+        ap[p][q] = sol[p][q];
+        WS(i, p * 5 + q) = sol[p][q];
+      }
+      dp[p] = sol[p][5];
+      z0p[p] = sol[p][6];
+      z1p[p] = sol[p][7];
+      WS(i, 25 + p) = dp[p];
+      WS(i, 30 + p) = z0p[p];
+      WS(i, 35 + p) = z1p[p];
+      if (!isfinite(dp[p])) ok = false;
+    }
+  }
+  if (!ok) {
+    // a failed column keeps its explicit increments (the reference undoes
+    // the column with its status bits and flags a retry)
+    status[col] = 1;
+    for (int i = 0; i < n; ++i)
+      for (int v = 0; v < 5; ++v) {
+        int64_t o = v * r.sv + row + i;
+        u[o] = a.w0 * u0[o] + a.w1 * u[o] + a.w2 * du[o];
+      }
+    return;
+  }
+  status[col] = 0;
+  // ---- back substitution (vic_backward_substitute) ------------------------
+  for (int i = n - 2; i >= 0; --i)
+    for (int p = 0; p < 5; ++p) {
+      double sd = 0., s0 = 0., s1 = 0.;
+      for (int q = 0; q < 5; ++q) {
+        double aq = WS(i, p * 5 + q);
+        sd += aq * WS(i + 1, 25 + q);
+        s0 += aq * WS(i + 1, 30 + q);
+        s1 += aq * WS(i + 1, 35 + q);
+      }
+      WS(i, 25 + p) -= sd;
+      WS(i, 30 + p) -= s0;
+      WS(i, 35 + p) -= s1;
+    }
+  // ---- Woodbury 2x2 (vic_woodbury_column) ---------------------------------
+  {
+    double k0 = -a.t0 / dt, k1 = -a.t1 / dt;
+    int i0 = 2, i1 = n - 3;
+    double m00 = 1. + k0 * WS(i0, 30), m01 = k0 * WS(i0, 35);
+    double m10 = k1 * WS(i1, 30), m11 = 1. + k1 * WS(i1, 35);
+    double r0 = k0 * WS(i0, 25), r1 = k1 * WS(i1, 25);
+    double det = m00 * m11 - m01 * m10;
+    double al0 = (m11 * r0 - m01 * r1) / det, al1 = (m00 * r1 - m10 * r0) / det;
+    for (int i = 0; i < n; ++i)
+      for (int p = 0; p < 5; ++p) {
+        double y = WS(i, 25 + p);
+        y -= WS(i, 30 + p) * al0;
+        y -= WS(i, 35 + p) * al1;
+        WS(i, 25 + p) = y;
+      }
+  }
+  // ---- mass redistribution (vic_constituent_column, dry) ------------------
+  // workspace reuse per layer: [0] phi, [1] M (face below), [2] mass change,
+  // [3] moved q (face below)
+  double R = 0., S = 0.;
+  for (int i = 0; i < n; ++i) {
+    double phi = (WS(i, 25) - DU(IDN, i)) * V;
+    WS(i, 0) = phi;
+    R += phi;
+    S += W(IDN, i) * V;
+  }
+  {
+    double M = 0.;
+    for (int i = 0; i < n; ++i) {
+      WS(i, 1) = M;
+      M -= WS(i, 0) - R * (W(IDN, i) * V / S);
+      WS(i, 2) = 0.;
+      WS(i, 3) = 0.;
+    }
+  }
+  {
+    double avail = (W(IDN, 0) * 1. + DU(IDN, 0)) * V;
+    if (avail < 0) avail = 0;
+    for (int i = 0; i + 1 < n; ++i) {
+      double avail_up = (W(IDN, i + 1) * 1. + DU(IDN, i + 1)) * V;
+      if (avail_up < 0) avail_up = 0;
+      double mf = WS(i + 1, 1);
+      double q = mf > 0 ? mf * 1. : mf * 1.;
+      if (q > avail) q = avail;
+      if (q < -avail_up) q = -avail_up;
+      WS(i, 2) -= q / V;
+      WS(i + 1, 2) += q / V;
+      WS(i + 1, 3) = q;
+      avail = avail_up + q;
+    }
+  }
+  // ---- outputs, remainder (book_remainder_) and the RK combine ------------
+  auto moved_rho = [&](int i) { return (DU(IDN, i) + WS(i, 2)) - DU(IDN, i); };
+  auto raw_minus_mass0 = [&](int i) { return WS(i, 25) - DU(IDN, i); };
+  for (int i = 0; i < n; ++i) {
+    double out[5];
+    out[IDN] = DU(IDN, i) + WS(i, 2);
+    out[IVX] = WS(i, 26);
+    out[IVY] = WS(i, 27);
+    out[IVZ] = WS(i, 28);
+    out[IPR] = WS(i, 29);
+    double req_lo = WS(i, 1), req_hi = i + 1 < n ? WS(i + 1, 1) : 0.;
+    double mv_lo = WS(i, 3) - req_lo;
+    double mv_hi = (i + 1 < n ? WS(i + 1, 3) : 0.) - req_hi;
+    double projected = (req_lo - req_hi) / V;
+    double pw = tab(vtab, V_PHI, i) * (WS(i, 25) - DU(IDN, i) - projected);
+    double cw = -(tab(vtab, V_DPHI, i) * mv_hi - tab(vtab, V_DPLO, i) * mv_lo) / V;
+    out[IPR] = out[IPR] + (pw + cw);
+    // corrected-PE work of the moved mass minus the operator's part
+    int jj = (i < 1 ? 1 : (i > n - 2 ? n - 2 : i)) - 1;
+    double s = tab(xtab, T_WT0, i) * moved_rho(jj);
+    s = s + tab(xtab, T_WT1, i) * moved_rho(jj + 1);
+    s = s + tab(xtab, T_WT2, i) * moved_rho(jj + 2);
+    double work = tab(xtab, T_GVAR, i) * s;
+    work = work - rows_at(i, raw_minus_mass0);
+    out[IPR] = out[IPR] + work;
+    for (int v = 0; v < 5; ++v) {
+      int64_t o = v * r.sv + row + i;
+      u[o] = a.w0 * u0[o] + a.w1 * u[o] + a.w2 * out[v];
+    }
+  }
+}
+
+}  // namespace
+}  // namespace snapy
+
+using namespace snapy;
+
+extern "C" {
+
+int64_t snapy_vic_workspace(const snapy_grid* g) {
+  if (!g) return 0;
+  return int64_t(NW) * g->nx1 * g->nx2 * g->nx3;
+}
+
+int snapy_vic_full(const snapy_grid* g, const snapy_physics* p, const double* w,
+                   const double* du, const double* u0, double* u,
+                   const double* vtab, const double* x1tab, double t0,
+                   double t1, double dt, const double* dt_dev, double wstage,
+                   double w0, double w1, double w2, double* ws, int* status,
+                   void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (!p || !w || !du || !u0 || !u || !vtab || !x1tab || !ws || !status)
+    return fail(-1, "vic_full: NULL argument");
+  if (g->nx1 < 5) return fail(-9, "vic_full: needs >= 5 x1 cells");
+  VicArgs a;
+  a.r = ranges(*g);
+  a.n = g->nx1;
+  a.gamma = p->gammad;
+  a.grav = p->grav1;
+  a.area = g->dx3 * g->dx2;
+  a.vol = g->dx3 * (g->dx2 * g->dx1);
+  a.t0 = t0;
+  a.t1 = t1;
+  a.dt_dev = dt_dev;
+  a.dt_host = dt;
+  a.wstage = wstage;
+  a.w0 = w0;
+  a.w1 = w1;
+  a.w2 = w2;
+  int64_t ncol = int64_t(g->nx2) * g->nx3;
+  hipLaunchKernelGGL(vic_full_kernel, dim3(blocks_for(ncol, 64)), dim3(64), 0,
+                     as_stream(stream), w, du, u0, u, vtab, x1tab, ws, status,
+                     a);
+  return launch_status("vic_full");
+}
+
+}  // extern "C"

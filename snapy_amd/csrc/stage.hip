@@ -257,7 +257,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     cfl_kernel(T const* __restrict__ w, unsigned long long* out, Range3 r,
                double gd, double dx1, double dx2, double dx3, int act1,
-               int act2, int act3) {
+               int act2, int act3, double adv1) {
   int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
   double best = 1.e9;
@@ -271,7 +271,9 @@ __global__ void __launch_bounds__(256)
     // sound_speed: sqrt(gamma * p / rho) (equation_of_state.cpp:235-238)
     T cs = sqrt(T(gd) * p / rho);
     if (act1) {
-      T t = T(dx1) / (fabs(w[r.sv + c]) + cs);
+      // adv1 > 0: the implicit x1 direction's advection bound
+      T t = adv1 > 0. ? T(dx1) / (fabs(w[r.sv + c]) * T(adv1))
+                      : T(dx1) / (fabs(w[r.sv + c]) + cs);
       best = fmin(best, double(t));
     }
     if (act2) {
@@ -519,7 +521,10 @@ int snapy_cfl(const snapy_grid* g, const snapy_physics* p, int dtype,
     using T = decltype(z);
     hipLaunchKernelGGL(cfl_kernel<T>, dim3(blocks_for(tot, 256)), dim3(256), 0,
                        s, static_cast<T const*>(w), out, r, p->gammad, g->dx1,
-                       g->dx2, g->dx3, g->nc1 > 1, g->nc2 > 1, g->nc3 > 1);
+                       g->dx2, g->dx3, g->nc1 > 1, g->nc2 > 1, g->nc3 > 1,
+                       ((p->implicit_scheme & 1) && (g->nc2 > 1 || g->nc3 > 1))
+                           ? p->advection
+                           : 0.);
     if (scale != 1.)
       hipLaunchKernelGGL(scale_f64_kernel, dim3(1), dim3(1), 0, s, dt_out,
                          scale);

@@ -48,3 +48,55 @@ def x1_gravity_table(x1f: torch.Tensor, x1v: torch.Tensor, nghost: int, nx1: int
         tab[5, is_:ie] = wt1
         tab[6, is_:ie] = wt2
     return tab
+
+
+def band_rows(x1v_int: torch.Tensor, gvar: torch.Tensor):
+    """GravityWork::update_rows (gravity_work.hpp:321-346): the corrected-PE
+    operator's bands (lo, mid, hi) and wall third points (t0, t1)."""
+    n = x1v_int.shape[0]
+    wt0, wt1, wt2 = centroid_slope_weights(x1v_int)
+    j = torch.arange(n).clamp(1, n - 2) - 1
+    idx = (j, j + 1, j + 2)
+    ar = torch.arange(n)
+
+    def band(off):
+        at = ar + off
+        s = wt0 * (idx[0] == at).to(wt0.dtype)
+        s = s + wt1 * (idx[1] == at).to(wt1.dtype)
+        s = s + wt2 * (idx[2] == at).to(wt2.dtype)
+        return s
+
+    mid = gvar * band(0)
+    lo = torch.zeros(n, dtype=torch.float64)
+    hi = torch.zeros(n, dtype=torch.float64)
+    lo[1:n] = band(-1)[1:n]
+    hi[0 : n - 1] = band(1)[0 : n - 1]
+    lo = lo * gvar
+    hi = hi * gvar
+    t0 = float(gvar[0] * band(2)[0])
+    t1 = float(gvar[n - 1] * band(-2)[n - 1])
+    return lo, mid, hi, t0, t1
+
+
+def vic_table(x1f, x1v, nghost, nx1, grav1, area1, vol):
+    """FP64 rows (nc1 each, interior filled) of the vertical implicit solve:
+    0/1 the face-work weights 0.5 A (x1v - x1f) / V and 0.5 A (x1f - x1v) / V
+    (GravityWork::update, gravity_work.hpp:245-262); 2/3/4 the corrected-PE
+    bands; 5 the cell potential -grav1 x1v; 6/7 the face-minus-cell
+    potentials.  Returns (table, t0, t1)."""
+    nc1 = x1v.shape[0]
+    tab = torch.zeros((8, nc1), dtype=torch.float64)
+    is_, ie = nghost, nghost + nx1
+    zc = x1v[is_:ie]
+    tab[0, is_:ie] = 0.5 * area1 * (zc - x1f[is_:ie]) / vol
+    tab[1, is_:ie] = 0.5 * area1 * (x1f[is_ + 1 : ie + 1] - zc) / vol
+    h = x1f[is_ + 1 : ie + 1] - x1f[is_:ie]
+    gvar = grav1 * (h * h / 12.0)
+    lo, mid, hi, t0, t1 = band_rows(zc, gvar)
+    tab[2, is_:ie] = lo
+    tab[3, is_:ie] = mid
+    tab[4, is_:ie] = hi
+    tab[5, is_:ie] = -grav1 * zc
+    tab[6, is_:ie] = -grav1 * (x1f[is_:ie] - zc)
+    tab[7, is_:ie] = -grav1 * (x1f[is_ + 1 : ie + 1] - zc)
+    return tab, t0, t1

@@ -85,6 +85,9 @@ class MeshBlock:
         self.device = torch.device("cpu")
         self._state = StateStore()
         self._freg = None
+        self._ref = None
+        self._x1tab = None
+        self._vic = None
         self._scratch_w = None
         self._dt_dev = None
         self._executor = StageExecutor(self, options.intg.execution)
@@ -116,6 +119,8 @@ class MeshBlock:
             int(h.recon1.shock),
             int(h.recon23.shock),
             bcs,
+            int(options.intg.implicit_scheme),
+            options.intg.cfl / options.intg.implicit_advection_cfl,
         )
         self._build_coord()
 
@@ -209,6 +214,24 @@ class MeshBlock:
             self._x1tab = x1_gravity_table(
                 cb.buffer("x1f"), cb.buffer("x1v"), c.nghost, c.nx1, h.grav1
             ).to(self.device)
+        self._vic = None
+        if self.options.intg.implicit_scheme == 9:
+            from .geometry import vic_table
+
+            cb = self._coord
+            a1 = c.dx(3) * c.dx(2)
+            vol = c.dx(3) * (c.dx(2) * c.dx(1))
+            tab, t0, t1 = vic_table(cb.buffer("x1f"), cb.buffer("x1v"), c.nghost, c.nx1, h.grav1, a1, vol)
+            lib = _capi.load()
+            nws = int(lib.snapy_vic_workspace(ctypes.byref(self._grid)))
+            self._vic = {
+                "tab": tab.to(self.device),
+                "t0": t0,
+                "t1": t1,
+                "ws": torch.empty(nws, dtype=torch.float64, device=self.device),
+                "status": torch.zeros(c.nx2 * c.nx3, dtype=torch.int32, device=self.device),
+                "du": torch.zeros(self.shape(), dtype=torch.float64, device=self.device),
+            }
 
     def flux_register(self) -> FluxRegister:
         self._ensure_buffers()
@@ -319,7 +342,27 @@ class MeshBlock:
                 )
         # (5)-(6) + combine: divergence, du = -dt div, u <- w0 u0 + w1 u + w2 du
         fl = (ctypes.c_void_p * 3)(*[_capi.ptr(freg.total(d)) for d in (1, 2, 3)])
-        if grav:
+        if self._vic is not None:
+            vic = self._vic
+            _capi.check(
+                lib.snapy_update_gravity(
+                    g, p, dtc, fl, None, _capi.ptr(vic["du"]), _capi.ptr(w),
+                    _capi.ptr(self._x1tab), None, 0.0, dtp, 1.0, 0.0, 1.0, s,
+                ),
+                "stage increment (gravity)",
+            )
+            # the implicit solve takes the stage weight inside its dt and
+            # finishes the RK combine
+            _capi.check(
+                lib.snapy_vic_full(
+                    g, p, _capi.ptr(w), _capi.ptr(vic["du"]), _capi.ptr(self._state.u0()), _capi.ptr(u),
+                    _capi.ptr(vic["tab"]), _capi.ptr(self._x1tab), vic["t0"], vic["t1"], 0.0, dtp,
+                    st.wght2(), st.wght0(), st.wght1(), st.wght2(), _capi.ptr(vic["ws"]),
+                    _capi.ptr(vic["status"]), s,
+                ),
+                "vic_full",
+            )
+        elif grav:
             _capi.check(
                 lib.snapy_update_gravity(
                     g, p, dtc, fl, _capi.ptr(self._state.u0()), _capi.ptr(u), _capi.ptr(w),
@@ -382,6 +425,10 @@ class MeshBlock:
         restores u0 and halves dt through current_redo."""
         u = vars["hydro_u"]
         bad = bool((~torch.isfinite(u[self.part(exterior=False)])).any().item())
+        if self._vic is not None:
+            # a column whose implicit solve failed asks for a redo
+            # (implicit.column retry flag, implicit_hydro.cpp:476-478)
+            bad = bad or bool(self._vic["status"].any().item())
         if not bad:
             self.intg.current_redo = 0
             return 0

@@ -118,6 +118,7 @@ class IntegratorOptions:
     tlim: float = 1.0
     nlim: int = -1
     implicit_scheme: int = 0
+    implicit_advection_cfl: float = 1.0
     execution: str = "auto"
 
 
@@ -238,7 +239,17 @@ def _read_integration(i) -> IntegratorOptions:
     _check_keys(
         i,
         "integration",
-        {"type", "cfl", "tlim", "nlim", "ncycle_out", "implicit-scheme", "execution", "verbose"},
+        {
+            "type",
+            "cfl",
+            "tlim",
+            "nlim",
+            "ncycle_out",
+            "implicit-scheme",
+            "implicit-advection-cfl",
+            "execution",
+            "verbose",
+        },
     )
     o = IntegratorOptions()
     o.type = i.get("type", o.type)
@@ -246,6 +257,7 @@ def _read_integration(i) -> IntegratorOptions:
     o.tlim = float(i.get("tlim", o.tlim))
     o.nlim = int(i.get("nlim", o.nlim))
     o.implicit_scheme = int(i.get("implicit-scheme", 0))
+    o.implicit_advection_cfl = float(i.get("implicit-advection-cfl", 1.0))
     o.execution = i.get("execution", o.execution)
     return o
 
@@ -292,8 +304,15 @@ def _check_supported(op: MeshBlockOptions) -> None:
         )
     if op.intg.type != "rk3":
         raise ValueError(f"integration/type '{op.intg.type}': rk3 only")
-    if op.intg.implicit_scheme != 0:
-        raise ValueError("integration/implicit-scheme: the vertical implicit solve is not built yet")
+    if op.intg.implicit_scheme not in (0, 9):
+        raise ValueError(
+            f"integration/implicit-scheme {op.intg.implicit_scheme}: scheme 9 (vic-full) is implemented"
+        )
+    if op.intg.implicit_scheme == 9:
+        if op.hydro.grav1 == 0.0:
+            raise ValueError("implicit-scheme 9 needs forcing/const-gravity grav1 (ImplicitHydro requires it)")
+        if c.nx1 < 5:
+            raise ValueError("implicit-scheme 9 needs nx1 >= 5")
     for f, n in zip(_FACES, op.bcnames):
         if n not in ("reflecting", "periodic", "extrapolation", "outflow", "custom", "none"):
             raise ValueError(f"boundary {f} '{n}' is not implemented")
