@@ -1,13 +1,14 @@
 #!/usr/bin/env python3
 """Benchmark of the finite-volume stage on one MI355X.
 
-Prints ONE JSON line: cell-updates/s per RK3 stage of the named workload,
-with the dominant kernel's roofline (HIP-event timed, algorithmic bytes) and
-the CPU oracle timed on this host as the baseline.
+Prints ONE JSON line: cell-updates/s per RK3 stage of the named workload
+(one interior cell advanced through one RK3 stage), with the dominant
+kernel's roofline (HIP-event timed, algorithmic bytes) and the CPU oracle
+timed on this host as the baseline.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-A "step" is one RK3 step (3 stages) of the workload, the CFL reduction
+A "step" is one RK3 step (3 stages) of the workload, its CFL reduction
 included, run as captured hipGraphs with a device dt (no host read inside the
 timed region).  Inputs are resident in HBM before timing starts.
 """
@@ -15,7 +16,6 @@ timed region).  Inputs are resident in HBM before timing starts.
 from __future__ import annotations
 
 import argparse
-import ctypes
 import json
 import os
 import sys
@@ -25,6 +25,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 vector peak
+
+WORKLOADS = ("convection3d", "sod")
 
 
 def parse():
@@ -32,99 +35,128 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="sod", choices=["sod"])
+    ap.add_argument("--workload", default="convection3d", choices=WORKLOADS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing")
     return ap.parse_args()
 
 
 # --------------------------------------------------------------------------
-# workloads
-# --------------------------------------------------------------------------
-def sod_workload():
-    import torch
+def make_block(deck, execution="hip-graph", **coord):
+    from snapy_amd import MeshBlock, MeshBlockOptions
 
-    from snapy_amd import MeshBlock, MeshBlockOptions, kIDN, kIPR
-
-    deck = os.path.join(ROOT, "snapy_amd", "decks", "sod.yaml")
-    op = MeshBlockOptions.from_yaml(deck)
-    op.intg.execution = "hip-graph"
+    op = MeshBlockOptions.from_yaml(os.path.join(ROOT, "snapy_amd", "decks", deck))
+    for k, v in coord.items():
+        setattr(op.coord, k, v)
+    op.intg.execution = execution
     block = MeshBlock(op)
-    block._executor.__init__(block, "hip-graph")
     block.to("cuda")
-    x1v = block.module("coord").buffer("x1v").view(1, 1, -1)
-    w = torch.zeros(block.shape(), dtype=torch.float64)
-    w[kIDN] = torch.where(x1v < 0.5, torch.tensor(1.0, dtype=torch.float64), torch.tensor(0.125, dtype=torch.float64))
-    w[kIPR] = torch.where(x1v < 0.5, torch.tensor(1.0, dtype=torch.float64), torch.tensor(0.1, dtype=torch.float64))
-    vars, _ = block.initialize({"hydro_w": w.cuda()})
-    cells = op.coord.nx1 * op.coord.nx2 * op.coord.nx3
+    return block
+
+
+def workload(name):
+    """(block, vars, cells, config, dtype, B_alg bytes per cell-update)."""
+    from snapy_amd import problems
+
+    if name == "sod":
+        block = make_block("sod.yaml")
+        vars, _ = block.initialize({"hydro_w": problems.sod(block).cuda()})
+        c = block.options.coord
+        cfg = {
+            "workload": "sod_1d_256",
+            "cells": [c.nx1, c.nx2, c.nx3],
+            "method": "weno5(shock)+hllc+ssp-rk3, outflow, cfl 0.9",
+            "form": "plain FP64",
+        }
+        return block, vars, c.nx1, cfg, "f64", 160.0
+    block = make_block("convection3d.yaml")
+    vars, _ = block.initialize({"hydro_w": problems.convection3d(block).cuda()})
+    c = block.options.coord
     cfg = {
-        "workload": "sod_1d_c1",
-        "cells": [op.coord.nx1, op.coord.nx2, op.coord.nx3],
-        "method": "weno5(shock)+hllc+ssp-rk3, outflow, cfl 0.9",
-        "form": "plain FP64",
-        "members": 1,
+        "workload": "convection3d_256x256x128",
+        "cells": [c.nx1, c.nx2, c.nx3],
+        "method": "weno5+lmars+ssp-rk3, well-balanced x1 gravity, flux covariance, bot-heat, "
+        "vertical implicit scheme 9 (full 5x5, Woodbury), periodic x2/x3, cfl 0.8",
+        "form": "plain FP64 (FP64 storage and compute, one block)",
     }
-    return block, vars, cells, cfg, "f64", 160.0
+    # B_alg (cell-update bytes): FP64 state read u, u0, write u; 3 face
+    # totals; the FP64 vertical-solve column I/O (3 rows x 5 x 8)
+    return block, vars, c.nx1 * c.nx2 * c.nx3, cfg, "f64", 5 * (3 * 8 + 3 * 8) + 3 * 5 * 8
 
 
-def sod_cpu_baseline(budget_s=10.0):
-    """The CPU FP64 oracle (numpy) on the same Sod case, repeated whole runs
-    until about budget_s seconds of CPU work."""
+# --------------------------------------------------------------------------
+def cpu_baseline(name, budget_s=10.0):
+    """The CPU FP64 oracle (numpy, one thread) on a bounded sample of the
+    same workload."""
+    import numpy as np
+
     from oracle import fv
 
     t0 = time.perf_counter()
-    stages = 0
-    runs = 0
-    while time.perf_counter() - t0 < budget_s:
-        d = fv.sod_deck(256)
-        b = fv.Block(d)
-        b.initialize(fv.sod_initial(d.grid))
-        _, n = fv.run(b, 0.25)
-        stages += 3 * n
-        runs += 1
+    stages = cells = runs = 0
+    if name == "sod":
+        while time.perf_counter() - t0 < budget_s:
+            d = fv.sod_deck(256)
+            b = fv.Block(d)
+            b.initialize(fv.sod_initial(d.grid))
+            _, n = fv.run(b, 0.25)
+            stages += 3 * n
+            runs += 1
+        el = time.perf_counter() - t0
+        return {
+            "value": 256 * stages / el,
+            "unit": "cell-updates/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{runs} full Sod runs (256 cells, t_end 0.25), numpy FP64 oracle, {el:.1f} s",
+        }
+    # convection3d: a 24 x 24 column sub-domain of the same 128-layer column
+    # physics (periodic x2/x3), whole RK3 steps
+    from snapy_amd import MeshBlock, MeshBlockOptions, problems
+
+    op = MeshBlockOptions.from_yaml(os.path.join(ROOT, "snapy_amd", "decks", "convection3d.yaml"))
+    op.coord.nx2 = op.coord.nx3 = 24
+    op.coord.x2max = op.coord.x3max = 2400.0
+    blk = MeshBlock(op)
+    w = problems.convection3d(blk).numpy()
+    c = op.coord
+    g = fv.Grid(c.nx1, c.nx2, c.nx3, c.x1min, c.x1max, c.x2min, c.x2max, c.x3min, c.x3max)
+    h = op.hydro
+    d = fv.Deck(grid=g, gammad=h.eos.gammad, riemann="lmars", shock_x1=False, shock_x23=False,
+                bcs=tuple(op.bcnames), cfl=op.intg.cfl, grav1=h.grav1, implicit_scheme=9,
+                flux_covariance=True, bot_heat_flux=h.bot_heat_flux, bot_heat_depth=h.bot_heat_depth)
+    b = fv.Block(d)
+    b.initialize(w)
+    steps = 0
+    while time.perf_counter() - t0 < budget_s or steps == 0:
+        dt = b.max_time_step()
+        for s in range(3):
+            b.forward(dt, s)
+        steps += 1
     el = time.perf_counter() - t0
+    n = c.nx1 * c.nx2 * c.nx3
+    assert np.isfinite(b.u).all()
     return {
-        "value": 256 * stages / el,
+        "value": n * 3 * steps / el,
         "unit": "cell-updates/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{runs} full Sod runs (256 cells, t_end 0.25) of the numpy FP64 oracle, {el:.1f} s",
+        "sample": f"{steps} RK3 steps of a 128x24x24 sub-domain of the same deck, numpy FP64 oracle, {el:.1f} s",
     }
 
 
 # --------------------------------------------------------------------------
-def kernel_times(block, vars, reps=200):
+def kernel_times(block, vars, reps=20):
     """Average launch duration (ms) of each stage kernel, timed alone with
-    HIP events on the stream the kernels are launched on."""
+    HIP events on the stream the kernels are launched on (stage-0 weights,
+    so repeated launches are idempotent)."""
     import torch
 
-    from snapy_amd import _capi
-
-    lib = _capi.load()
-    g, p = ctypes.byref(block._grid), ctypes.byref(block._phys)
-    u, w = vars["hydro_u"], vars["hydro_w"]
-    dtc = _capi.dtype_code(u.dtype)
-    s = _capi.stream_of(block.device)
-    freg = block._freg
-    calls = {}
-    calls["cons2prim"] = lambda: lib.snapy_cons2prim(g, p, dtc, u.data_ptr(), w.data_ptr(), s)
-    for d in (1, 2, 3):
-        t = freg.total(d)
-        if t is None:
-            continue
-        fp = freg.face_pressure1 if d == 1 else None
-        calls[f"sweep_x{d}"] = (
-            lambda d=d, t=t, fp=fp: lib.snapy_sweep(d, g, p, dtc, w.data_ptr(), t.data_ptr(), _capi.ptr(fp), s)
-        )
-    ucopy = u.clone()
-    fl = (ctypes.c_void_p * 3)(*[_capi.ptr(freg.total(d)) for d in (1, 2, 3)])
-    calls["update"] = lambda: lib.snapy_update(
-        g, dtc, fl, block._state.u0().data_ptr(), ucopy.data_ptr(), None, 1e-9, None, 1.0, 0.0, 1.0, s
-    )
-    out = {}
+    calls = block._stage_calls(vars, 0)
     stream = torch.cuda.current_stream()
-    for name, fn in calls.items():
-        for _ in range(5):
+    out = {}
+    for name, fn in calls:
+        for _ in range(2):
             fn()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
@@ -137,42 +169,45 @@ def kernel_times(block, vars, reps=200):
     return out
 
 
-def kernel_bytes(block, name, sbytes):
-    """Algorithmic HBM bytes of one launch: every cell value read once, every
-    output written once (no stencil re-reads)."""
+def kernel_bytes(block, name, sb=8):
+    """Algorithmic HBM bytes of one launch: each input value read once and
+    each output written once (stencil re-reads and solver workspace
+    excluded)."""
     c = block.options.coord
     ncell = c.nc1() * c.nc2() * c.nc3()
     nint = c.nx1 * c.nx2 * c.nx3
-    nv = 5
-    if name == "cons2prim":
-        return 2 * nv * ncell * sbytes
-    if name.startswith("sweep"):
-        return nv * ncell * sbytes + nv * nint * sbytes
-    if name == "update":
-        nd = sum(1 for n in (c.nx1, c.nx2, c.nx3) if n > 1)
-        return nv * nint * sbytes * (3 + nd)
-    return 0
+    nd = sum(1 for n in (c.nx1, c.nx2, c.nx3) if n > 1)
+    table = {
+        "cons2prim": 10 * ncell * sb,
+        "hydro_ref_x1": 7 * ncell * 8,
+        "sweep_x1_wb": (5 * sb + 4 * 8) * ncell + 6 * nint * sb,
+        "stage_increment": (5 * nd + 2 + 5) * nint * sb,
+        "vic_full": 25 * nint * 8,
+        "update": (3 + nd) * 5 * nint * sb,
+        "update_gravity": ((3 + nd) * 5 + 2) * nint * sb,
+    }
+    if name.startswith("sweep_x"):
+        return table.get(name, 5 * ncell * sb + 5 * nint * sb)
+    return table.get(name, 0)
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(
-            json.dumps({"error": "this build measures one GPU: every box here has one; "
-                        "run with --gpus 1 (multi-GPU sharding is described in DESIGN.md)"}),
-            flush=True,
-        )
+        print(json.dumps({"error": "this build measures one GPU: every GPU box here has one; run with "
+                          "--gpus 1 (how the path shards over more GPUs is described in DESIGN.md)"}), flush=True)
         sys.exit(2)
     import torch
 
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
-    steps = args.steps or 200
-    warmup = args.warmup if args.warmup is not None else 10
+    defaults = {"sod": (200, 10), "convection3d": (20, 3)}[args.workload]
+    steps = args.steps or defaults[0]
+    warmup = args.warmup if args.warmup is not None else defaults[1]
 
-    block, vars, cells, cfg, dtype, b_alg = sod_workload()
-    # warm-up: eager, capture, replays
+    block, vars, cells, cfg, dtype, b_alg = workload(args.workload)
+    print(f"# workload {cfg['workload']} ready", file=sys.stderr, flush=True)
     for _ in range(warmup):
         dt = block.device_time_step(vars)
         for s in range(3):
@@ -187,25 +222,28 @@ def main():
     el = time.perf_counter() - t0
     ms_per_step = el / steps * 1e3
     value = cells * 3 * steps / el
+    ok = bool(torch.isfinite(vars["hydro_u"]).all().item()) and block.check_redo(vars) == 0
+    print(f"# timed {steps} steps in {el:.3f} s, state finite: {ok}", file=sys.stderr, flush=True)
 
-    kt = kernel_times(block, vars)
-    dom = max(kt, key=kt.get)
-    sb = 8 if dtype == "f64" else 4
-    kb = kernel_bytes(block, dom, sb)
-    achieved = kb / (kt[dom] * 1e-3) / 1e9
-    roofline = {
-        "bound": "hbm",
-        "kernel": dom,
-        "achieved": achieved,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
-        "kernel_ms": kt,
-        "stage_b_alg_per_cell": b_alg,
-        "stage_frac": value * b_alg / (HBM_PEAK_GBS * 1e9),
-    }
-    cpu = None if args.no_cpu else sod_cpu_baseline()
+    roofline = None
+    if not args.no_kernels:
+        kt = kernel_times(block, vars)
+        dom = max(kt, key=kt.get)
+        kb = kernel_bytes(block, dom)
+        achieved = kb / (kt[dom] * 1e-3) / 1e9
+        roofline = {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel_ms": kt,
+            "stage_b_alg_per_cell": b_alg,
+            "stage_frac": value * b_alg / (HBM_PEAK_GBS * 1e9),
+        }
+    cpu = None if args.no_cpu else cpu_baseline(args.workload)
     line = {
         "metric": "cell-updates/s per RK3 stage",
         "value": value,
@@ -222,6 +260,7 @@ def main():
         "config": cfg,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "state_finite": ok,
         "execution": block.execution_report(),
     }
     print(json.dumps(line), flush=True)

@@ -73,6 +73,9 @@ typedef struct snapy_physics {
   int32_t implicit_scheme; /* integration/implicit-scheme (0, 1, 9) */
   double advection;      /* cfl / implicit-advection-cfl (CFL of an implicit
                             direction, src/hydro/hydro.cpp:326-370) */
+  double bot_heat_flux;  /* forcing/bot-heat flux (W/m^2), 0 = none */
+  int32_t bot_heat_depth; /* forcing/bot-heat depth (cells) */
+  int32_t flux_covariance; /* dynamics/flux-covariance on the x2/x3 faces */
 } snapy_physics;
 
 /* ---- entry points ----------------------------------------------------- */
@@ -107,6 +110,15 @@ int snapy_prim2cons(const snapy_grid *g, const snapy_physics *p, int dtype,
 int snapy_sweep(int dim, const snapy_grid *g, const snapy_physics *p,
                 int dtype, const void *w, void *flux, void *face_pressure,
                 void *stream);
+
+/* x2/x3 sweep (dim 2 or 3) with the x1 flux covariance of the energy flux
+ * when p->flux_covariance is set (HydroImpl::_flux_covariance, Cartesian,
+ * src/hydro/hydro_forward.cpp:58-231, added as add_cov :782-793), else the
+ * plain sweep.  x1tab row 0 holds x1v (nc1 doubles).  Interior faces only;
+ * wall planes prescribed as in snapy_sweep. */
+int snapy_sweep_cov(int dim, const snapy_grid *g, const snapy_physics *p,
+                    int dtype, const void *w, const double *x1tab, void *flux,
+                    void *stream);
 
 /* Fused flux divergence + RK stage combine on the interior cells:
  *   du = -dt * div(F) + du_src      (du_src may be NULL)

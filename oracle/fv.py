@@ -347,6 +347,28 @@ class Grid:
         return tuple(out)
 
 
+def flux_covariance(wl, wr, dim, grid, gammad):
+    """HydroImpl::_flux_covariance, Cartesian dry rows (hydro_forward.cpp:
+    58-231, :196-213): sigma1^2 rho d1[h] d1[u_n] on the x1 cells 1..n1-2 of
+    the x2/x3 faces, wbar = (wl + wr)/2, h = (W->I + p)/rho."""
+    n1 = wl.shape[-1]
+    wbar = 0.5 * (wl + wr)
+    p = wbar[IPR]
+    enth = p / (gammad - 1.0) + p
+    un = wbar[IVY if dim == 2 else IVZ]
+    x1v = grid.centers(1)
+    dx1f = np.full(n1, grid.dx(1))
+    s2 = dx1f * dx1f / 12.0
+    rho = wbar[IDN]
+    dx1 = x1v[2:n1] - x1v[0 : n1 - 2]
+    cell = lambda q: q[..., 1 : n1 - 1]  # noqa: E731
+    d1 = lambda q: (q[..., 2:n1] - q[..., 0 : n1 - 2]) / dx1  # noqa: E731
+    s2rho = cell(s2) * cell(rho)
+    dun = d1(un)
+    h = enth / rho
+    return s2rho * d1(h) * dun
+
+
 def prescribe_walls(flux, tdim, bcs, axis, grid):
     """boundary.prescribed on the wall planes of one direction
     (FluxRegister::apply_prescribed, src/hydro/flux_register.cpp:707-748;
@@ -574,6 +596,9 @@ class Deck:
     pfloor: float = 1e-10
     grav1: float = 0.0
     implicit_scheme: int = 0
+    flux_covariance: bool = False
+    bot_heat_flux: float = 0.0
+    bot_heat_depth: int = 1
 
 
 @dataclass
@@ -623,6 +648,8 @@ class Block:
                 wlr = reconstruct(self.w, tdim, shock)
             solver = hllc if d.riemann == "hllc" else lmars
             f, p = solver(wlr[ILT], wlr[IRT], d.gammad, tdim)
+            if axis > 1 and d.flux_covariance and g.nc(1) >= 3:
+                f[IPR][..., 1 : g.nc(1) - 1] += flux_covariance(wlr[ILT], wlr[IRT], tdim, g, d.gammad)
             prescribe_walls(f, tdim, d.bcs, axis, g)
             flux[axis] = f
             if axis == 1:
@@ -636,6 +663,10 @@ class Block:
             from . import gravity
 
             gravity.const_gravity(du, self.w, dt, d.grav1)
+            if d.bot_heat_flux != 0.0:
+                # LayerHeat bot-heat (layer_heat.cpp:78-85)
+                il = g.nghost
+                du[IPR][..., il : il + d.bot_heat_depth] += dt * d.bot_heat_flux / (g.dx(1) * d.bot_heat_depth)
             if 1 in flux:
                 gec = gravity.gravity_energy_correction(g, self.w, flux[1], dt, d.grav1)
                 du[IPR][..., g.nghost : g.nghost + g.nx1] += gec

@@ -58,6 +58,9 @@ class Physics(ctypes.Structure):
         ("bc", ctypes.c_int32 * 6),
         ("implicit_scheme", ctypes.c_int32),
         ("advection", ctypes.c_double),
+        ("bot_heat_flux", ctypes.c_double),
+        ("bot_heat_depth", ctypes.c_int32),
+        ("flux_covariance", ctypes.c_int32),
     ]
 
 
@@ -75,6 +78,7 @@ SIGNATURES = {
     "snapy_cons2prim": (_I, [_GP, _PP, _I, _P, _P, _P]),
     "snapy_prim2cons": (_I, [_GP, _PP, _I, _P, _P, _P]),
     "snapy_sweep": (_I, [_I, _GP, _PP, _I, _P, _P, _P, _P]),
+    "snapy_sweep_cov": (_I, [_I, _GP, _PP, _I, _P, _P, _P, _P]),
     "snapy_update": (_I, [_GP, _I, _P * 3, _P, _P, _P, _D, _P, _D, _D, _D, _P]),
     "snapy_step_field": (_I, [_GP, _I, _P, _P, _D, _D, _D, _P, _P]),
     "snapy_cfl": (_I, [_GP, _PP, _I, _P, _D, _P, _P]),

@@ -264,6 +264,8 @@ struct GUpdArgs {
   double const* dt_dev;
   double w0, w1, w2;
   double grav1;
+  double heat_flux, heat_dzd;  // bot-heat flux and dz * depth
+  int heat_depth;               // 0 = no bot-heat
 };
 
 // WRITE_DU: store the stage increment du (for the implicit solve to finish)
@@ -305,6 +307,9 @@ __global__ void __launch_bounds__(256)
   du[IPR] = du[IPR] + tdt * rho * vx * g1 * T(1.);
   if (dsrc)
     for (int v = 0; v < 5; ++v) du[v] = du[v] + dsrc[v * r.sv + c];
+  // LayerHeat bot-heat, an integer depth of bottom cells
+  // (src/forcing/layer_heat.cpp:78-85): du[IPR] += dt * flux / (dz * depth)
+  if (ii < a.heat_depth) du[IPR] = du[IPR] + T(dt * a.heat_flux / a.heat_dzd);
   // gravity_energy_correction (hydro_forward.cpp:1049-1124), Cartesian
   T const* m1 = f1 + row;  // the IDN row of F1 along this column
   auto drho = [&](int m) {  // -dt * x1_face_difference / V at cell lo1 + m
@@ -407,6 +412,9 @@ int snapy_update_gravity(const snapy_grid* g, const snapy_physics* p,
   a.w1 = w1;
   a.w2 = w2;
   a.grav1 = p->grav1;
+  a.heat_depth = p->bot_heat_flux != 0. ? p->bot_heat_depth : 0;
+  a.heat_flux = p->bot_heat_flux;
+  a.heat_dzd = g->dx1 * double(p->bot_heat_depth);
   int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
   auto kern = u0 ? update_gravity_kernel<double, false>
                  : update_gravity_kernel<double, true>;

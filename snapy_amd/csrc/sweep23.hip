@@ -1,0 +1,160 @@
+// x2/x3 face sweep with the x1 flux covariance (dynamics/flux-covariance,
+// the reference default): the O(dx1^2) covariance of the enthalpy flux is
+// added to the energy row of every x2/x3 face flux
+// (HydroImpl::_flux_covariance, src/hydro/hydro_forward.cpp:58-231, the
+// Cartesian branch :196-213; add_cov :782-793).
+//
+// The covariance of a face needs the face states of its two x1 neighbours,
+// so a workgroup of 64 lanes walks 62 consecutive x1 cells plus one halo
+// lane on each side: every lane reconstructs its own face states, the
+// enthalpy h and the normal velocity u_n go through LDS, and the 62 inner
+// lanes solve the Riemann problem and add the covariance.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "point.hpp"
+
+namespace snapy {
+namespace {
+
+constexpr int kLanes = 64;
+constexpr int kInner = kLanes - 2;
+
+struct CovArgs {
+  Range3 r;
+  int nf;  // faces along the sweep axis
+  double gammad;
+  int riemann, shock;
+  int wall_lo, wall_hi;
+  int cov;    // add the covariance
+  double s2;  // dx1^2 / 12 (face_moment2_x1, uniform x1)
+};
+
+template <typename T, int DIR>
+__global__ void __launch_bounds__(kLanes)
+    sweep23_kernel(T const* __restrict__ w, double const* __restrict__ x1v,
+                   T* __restrict__ flx, CovArgs a) {
+  Range3 const& r = a.r;
+  __shared__ double sh_h[kLanes], sh_un[kLanes];
+  int t = threadIdx.x;
+  int i = r.lo1 - 1 + int(blockIdx.y) * kInner + t;
+  // blockIdx.x: (other axis, face) -- DIR 2: (k, j face); DIR 3: (k face, j)
+  int64_t y = blockIdx.x;
+  int j, k;
+  int64_t stride;
+  if (DIR == 2) {
+    j = r.lo2 + int(y % a.nf);
+    k = r.lo3 + int(y / a.nf);
+    stride = r.s2;
+  } else {
+    j = r.lo2 + int(y % r.n2);
+    k = r.lo3 + int(y / r.n2);
+    stride = r.s3;
+  }
+  bool live = i >= r.lo1 - 1 && i <= r.lo1 + r.n1;
+  Prim5<T> L, R;
+  double h = 0., un = 0.;
+  if (live) {
+    int64_t base = k * r.s3 + j * r.s2 + i;
+    T* lv[5] = {&L.r, &L.v[0], &L.v[1], &L.v[2], &L.p};
+    T* rv[5] = {&R.r, &R.v[0], &R.v[1], &R.v[2], &R.p};
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      T const* p = w + v * r.sv + base;
+      T c0 = p[-3 * stride], c1 = p[-2 * stride], c2 = p[-stride], c3 = p[0],
+        c4 = p[stride], c5 = p[2 * stride];
+      if (v == IDN || a.shock) {
+        *lv[v] = weno5_plus(c0, c1, c2, c3, c4);
+        *rv[v] = weno5_minus(c1, c2, c3, c4, c5);
+      } else {
+        *lv[v] = cp5_plus(c0, c1, c2, c3, c4);
+        *rv[v] = cp5_minus(c1, c2, c3, c4, c5);
+      }
+    }
+    if (a.cov) {
+      // wbar = 0.5 (wl + wr); h = (W->I(wbar) + p) / rho; u_n
+      double rho = 0.5 * (double(L.r) + double(R.r));
+      double p = 0.5 * (double(L.p) + double(R.p));
+      double enth = p / (a.gammad - 1.) + p;
+      h = enth / rho;
+      un = 0.5 * (double(L.v[DIR - 1]) + double(R.v[DIR - 1]));
+    }
+  }
+  sh_h[t] = h;
+  sh_un[t] = un;
+  __syncthreads();
+  if (t == 0 || t == kLanes - 1) return;
+  if (!(i >= r.lo1 && i < r.lo1 + r.n1)) return;
+  T g = T(a.gammad);
+  T out[5];
+  T pface;
+  if (a.riemann == SNAPY_RIEMANN_LMARS)
+    lmars(L, R, g, g, DIR - 1, out, &pface);
+  else
+    hllc(L, R, g, g, DIR - 1, out, &pface);
+  if (a.cov) {
+    double rho = 0.5 * (double(L.r) + double(R.r));
+    double dx1 = x1v[i + 1] - x1v[i - 1];
+    double d1h = (sh_h[t + 1] - sh_h[t - 1]) / dx1;
+    double dun = (sh_un[t + 1] - sh_un[t - 1]) / dx1;
+    double ce = ((a.s2 * rho) * d1h) * dun;
+    out[IPR] = T(double(out[IPR]) + ce);
+  }
+  int f = DIR == 2 ? j - r.lo2 : k - r.lo3;
+  if ((f == 0 && a.wall_lo) || (f == a.nf - 1 && a.wall_hi)) {
+    out[IDN] = T(0);
+    out[IPR] = T(0);
+  }
+  int64_t base = k * r.s3 + j * r.s2 + i;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) flx[v * r.sv + base] = out[v];
+}
+
+}  // namespace
+}  // namespace snapy
+
+using namespace snapy;
+
+extern "C" {
+
+int snapy_sweep_cov(int dim, const snapy_grid* g, const snapy_physics* p,
+                    int dtype, const void* w, const double* x1tab, void* flux,
+                    void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (!p || !w || !x1tab || !flux) return fail(-1, "sweep_cov: NULL argument");
+  if (dim != 2 && dim != 3) return fail(-8, "sweep_cov: dim must be 2 or 3");
+  if (g->nvar != 5) return fail(-6, "sweep_cov: nvar 5");
+  if (g->nx1 < 3) return fail(-9, "sweep_cov: the covariance needs nx1 >= 3");
+  int nx[3] = {g->nx1, g->nx2, g->nx3};
+  if (nx[dim - 1] < 2) return fail(-9, "sweep_cov: inactive direction");
+  CovArgs a;
+  a.r = ranges(*g);
+  a.nf = nx[dim - 1] + 1;
+  a.gammad = p->gammad;
+  a.riemann = p->riemann;
+  a.shock = p->shock_x23;
+  a.wall_lo = p->bc[2 * (dim - 1)] == SNAPY_BC_REFLECTING;
+  a.wall_hi = p->bc[2 * (dim - 1) + 1] == SNAPY_BC_REFLECTING;
+  a.cov = p->flux_covariance;
+  a.s2 = g->dx1 * g->dx1 / 12.;
+  unsigned nseg = blocks_for(g->nx1, kInner);
+  int64_t ny = dim == 2 ? int64_t(a.nf) * g->nx3 : int64_t(g->nx2) * a.nf;
+  dim3 grid(unsigned(ny), nseg);
+  hipStream_t s = as_stream(stream);
+  if (dtype == SNAPY_F64) {
+    auto k = dim == 2 ? sweep23_kernel<double, 2> : sweep23_kernel<double, 3>;
+    hipLaunchKernelGGL(k, grid, dim3(kLanes), 0, s,
+                       static_cast<double const*>(w), x1tab,
+                       static_cast<double*>(flux), a);
+  } else if (dtype == SNAPY_F32) {
+    auto k = dim == 2 ? sweep23_kernel<float, 2> : sweep23_kernel<float, 3>;
+    hipLaunchKernelGGL(k, grid, dim3(kLanes), 0, s,
+                       static_cast<float const*>(w), x1tab,
+                       static_cast<float*>(flux), a);
+  } else {
+    return fail(-10, "unknown dtype");
+  }
+  return launch_status("sweep_cov");
+}
+
+}  // extern "C"

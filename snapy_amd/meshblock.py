@@ -121,6 +121,9 @@ class MeshBlock:
             bcs,
             int(options.intg.implicit_scheme),
             options.intg.cfl / options.intg.implicit_advection_cfl,
+            h.bot_heat_flux,
+            int(h.bot_heat_depth),
+            int(h.flux_covariance),
         )
         self._build_coord()
 
@@ -206,14 +209,16 @@ class MeshBlock:
         h = self.options.hydro
         self._ref = None
         self._x1tab = None
-        if h.grav1 != 0.0:
-            from .geometry import x1_gravity_table
+        from .geometry import x1_gravity_table
 
+        cb = self._coord
+        # the x1 table: x1v always (the covariance reads it), the gravity
+        # factors when grav1 != 0
+        self._x1tab = x1_gravity_table(cb.buffer("x1f"), cb.buffer("x1v"), c.nghost, c.nx1, h.grav1).to(
+            self.device
+        )
+        if h.grav1 != 0.0:
             self._ref = torch.zeros((5, c.nc3(), c.nc2(), c.nc1()), dtype=torch.float64, device=self.device)
-            cb = self._coord
-            self._x1tab = x1_gravity_table(
-                cb.buffer("x1f"), cb.buffer("x1v"), c.nghost, c.nx1, h.grav1
-            ).to(self.device)
         self._vic = None
         if self.options.intg.implicit_scheme == 9:
             from .geometry import vic_table
@@ -295,8 +300,17 @@ class MeshBlock:
 
     def _launch_stage(self, vars, stage: int) -> None:
         """The stage's launches on the current stream (the body a hipGraph
-        captures): backup at stage 0, EOS, sweeps, step fields, divergence +
-        combine, boundaries."""
+        captures)."""
+        if stage == 0:
+            self._state.backup()
+        for name, fn in self._stage_calls(vars, stage):
+            _capi.check(fn(), name)
+
+    def _stage_calls(self, vars, stage: int):
+        """The stage's kernel launches in order, as (name, launch) pairs:
+        EOS, the x1 reference and sweep, the x2/x3 sweeps, the step fields,
+        the stage increment with its sources and (with an implicit scheme)
+        the vertical solve, the RK combine, the boundaries."""
         lib = _capi.load()
         g, p = ctypes.byref(self._grid), ctypes.byref(self._phys)
         u = vars["hydro_u"]
@@ -307,11 +321,11 @@ class MeshBlock:
         st = self.intg.stages[stage]
         freg.begin_stage(stage, st.wght1(), st.wght2())
         dtp = _capi.ptr(self._dt_dev)
-        if stage == 0:
-            self._state.backup()
         grav = self._ref is not None
+        P = _capi.ptr
+        calls = []
         # (1) primitives
-        _capi.check(lib.snapy_cons2prim(g, p, dtc, _capi.ptr(u), _capi.ptr(w), s), "cons2prim")
+        calls.append(("cons2prim", lambda: lib.snapy_cons2prim(g, p, dtc, P(u), P(w), s)))
         # (2)-(4) fused reconstruction + Riemann per active direction; with
         # x1 gravity the x1 sweep is the well-balanced one about the
         # hydrostatic reference of every column
@@ -321,70 +335,81 @@ class MeshBlock:
                 continue
             fp = freg.face_pressure1 if d == 1 else None
             if d == 1 and grav:
-                _capi.check(lib.snapy_hydro_ref_x1(g, p, _capi.ptr(w), _capi.ptr(self._ref), s), "hydro_ref_x1")
-                _capi.check(
-                    lib.snapy_sweep_x1_wb(g, p, dtc, _capi.ptr(w), _capi.ptr(self._ref), _capi.ptr(t), _capi.ptr(fp), s),
-                    "sweep x1 (well-balanced)",
+                calls.append(("hydro_ref_x1", lambda: lib.snapy_hydro_ref_x1(g, p, P(w), P(self._ref), s)))
+                calls.append(
+                    ("sweep_x1_wb", lambda t=t, fp=fp: lib.snapy_sweep_x1_wb(g, p, dtc, P(w), P(self._ref), P(t), P(fp), s))
                 )
-                continue
-            _capi.check(
-                lib.snapy_sweep(d, g, p, dtc, _capi.ptr(w), _capi.ptr(t), _capi.ptr(fp), s),
-                f"sweep x{d}",
-            )
+            elif d > 1 and self.options.coord.nx1 >= 3:
+                # x2/x3 with the x1 flux covariance (on by default)
+                calls.append(
+                    (f"sweep_x{d}", lambda d=d, t=t: lib.snapy_sweep_cov(d, g, p, dtc, P(w), P(self._x1tab), P(t), s))
+                )
+            else:
+                calls.append((f"sweep_x{d}", lambda d=d, t=t, fp=fp: lib.snapy_sweep(d, g, p, dtc, P(w), P(t), P(fp), s)))
         # FluxRegister::end_stage: S_d <- w1 S_d + w2 dt T_d
         for d in (1, 2, 3):
             if freg.step(d) is not None:
-                _capi.check(
-                    lib.snapy_step_field(
-                        g, dtc, _capi.ptr(freg.total(d)), _capi.ptr(freg.step(d)), st.wght1(), st.wght2(), 0.0, dtp, s
-                    ),
-                    "step_field",
+                calls.append(
+                    (
+                        f"step_field_x{d}",
+                        lambda d=d: lib.snapy_step_field(
+                            g, dtc, P(freg.total(d)), P(freg.step(d)), st.wght1(), st.wght2(), 0.0, dtp, s
+                        ),
+                    )
                 )
-        # (5)-(6) + combine: divergence, du = -dt div, u <- w0 u0 + w1 u + w2 du
-        fl = (ctypes.c_void_p * 3)(*[_capi.ptr(freg.total(d)) for d in (1, 2, 3)])
+        # (5)-(7) + combine: divergence, sources, implicit, u <- w0 u0 + w1 u + w2 du
+        fl = (ctypes.c_void_p * 3)(*[P(freg.total(d)) for d in (1, 2, 3)])
+        u0 = self._state.u0()
         if self._vic is not None:
             vic = self._vic
-            _capi.check(
-                lib.snapy_update_gravity(
-                    g, p, dtc, fl, None, _capi.ptr(vic["du"]), _capi.ptr(w),
-                    _capi.ptr(self._x1tab), None, 0.0, dtp, 1.0, 0.0, 1.0, s,
-                ),
-                "stage increment (gravity)",
+            calls.append(
+                (
+                    "stage_increment",
+                    lambda: lib.snapy_update_gravity(
+                        g, p, dtc, fl, None, P(vic["du"]), P(w), P(self._x1tab), None, 0.0, dtp, 1.0, 0.0, 1.0, s
+                    ),
+                )
             )
             # the implicit solve takes the stage weight inside its dt and
             # finishes the RK combine
-            _capi.check(
-                lib.snapy_vic_full(
-                    g, p, _capi.ptr(w), _capi.ptr(vic["du"]), _capi.ptr(self._state.u0()), _capi.ptr(u),
-                    _capi.ptr(vic["tab"]), _capi.ptr(self._x1tab), vic["t0"], vic["t1"], 0.0, dtp,
-                    st.wght2(), st.wght0(), st.wght1(), st.wght2(), _capi.ptr(vic["ws"]),
-                    _capi.ptr(vic["status"]), s,
-                ),
-                "vic_full",
+            calls.append(
+                (
+                    "vic_full",
+                    lambda: lib.snapy_vic_full(
+                        g, p, P(w), P(vic["du"]), P(u0), P(u), P(vic["tab"]), P(self._x1tab), vic["t0"], vic["t1"],
+                        0.0, dtp, st.wght2(), st.wght0(), st.wght1(), st.wght2(), P(vic["ws"]), P(vic["status"]), s,
+                    ),
+                )
             )
         elif grav:
-            _capi.check(
-                lib.snapy_update_gravity(
-                    g, p, dtc, fl, _capi.ptr(self._state.u0()), _capi.ptr(u), _capi.ptr(w),
-                    _capi.ptr(self._x1tab), None, 0.0, dtp, st.wght0(), st.wght1(), st.wght2(), s,
-                ),
-                "update (gravity)",
+            calls.append(
+                (
+                    "update_gravity",
+                    lambda: lib.snapy_update_gravity(
+                        g, p, dtc, fl, P(u0), P(u), P(w), P(self._x1tab), None, 0.0, dtp,
+                        st.wght0(), st.wght1(), st.wght2(), s,
+                    ),
+                )
             )
         else:
-            _capi.check(
-                lib.snapy_update(
-                    g, dtc, fl, _capi.ptr(self._state.u0()), _capi.ptr(u), None, 0.0, dtp,
-                    st.wght0(), st.wght1(), st.wght2(), s,
-                ),
-                "update",
+            calls.append(
+                (
+                    "update",
+                    lambda: lib.snapy_update(
+                        g, dtc, fl, P(u0), P(u), None, 0.0, dtp, st.wght0(), st.wght1(), st.wght2(), s
+                    ),
+                )
             )
         # boundaries on the combined state
-        _capi.check(
-            lib.snapy_apply_boundaries(
-                g, p, dtc, _capi.ptr(u), _capi.ptr(vars.get("boundary_reference_w")), _capi.ptr(self._scratch_w), s
-            ),
-            "apply_boundaries",
+        calls.append(
+            (
+                "apply_boundaries",
+                lambda: lib.snapy_apply_boundaries(
+                    g, p, dtc, P(u), P(vars.get("boundary_reference_w")), P(self._scratch_w), s
+                ),
+            )
         )
+        return calls
 
     def device_time_step(self, vars) -> torch.Tensor:
         """max_time_step without the host read: the block's device dt

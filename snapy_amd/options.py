@@ -107,6 +107,8 @@ class HydroOptions:
     recon23: ReconstructOptions = field(default_factory=ReconstructOptions)
     riemann: str = "hllc"
     grav1: float = 0.0
+    bot_heat_flux: float = 0.0
+    bot_heat_depth: int = 1
     precision: str = "float64"
     flux_covariance: bool = True
 
@@ -227,7 +229,14 @@ def _read_dynamics(d, forcing) -> HydroOptions:
     h.riemann = rs.get("type", h.riemann)
     h.precision = d.get("precision", h.precision)
     h.flux_covariance = bool(d.get("flux-covariance", h.flux_covariance))
-    _check_keys(forcing, "forcing", {"const-gravity"})
+    _check_keys(forcing, "forcing", {"const-gravity", "bot-heat"})
+    if "bot-heat" in forcing:
+        bh = forcing["bot-heat"] or {}
+        _check_keys(bh, "forcing/bot-heat", {"flux", "depth", "thickness"})
+        if float(bh.get("thickness", 0.0)) != 0.0:
+            raise ValueError("forcing/bot-heat thickness > 0 is not implemented (use depth)")
+        h.bot_heat_flux = float(bh.get("flux", 0.0))
+        h.bot_heat_depth = int(bh.get("depth", 1))
     if "const-gravity" in forcing:
         cg = forcing["const-gravity"] or {}
         _check_keys(cg, "forcing/const-gravity", {"grav1"})
@@ -297,11 +306,8 @@ def _check_supported(op: MeshBlockOptions) -> None:
             raise ValueError("const-gravity grav1 with a periodic x1 column is not implemented")
         if op.hydro.precision != "float64":
             raise ValueError("const-gravity: the gravity path runs in FP64 storage here")
-    if op.hydro.flux_covariance and c.nx1 > 2 and (c.nx2 > 1 or c.nx3 > 1):
-        raise ValueError(
-            "dynamics/flux-covariance: true (the reference default) is not implemented; "
-            "set flux-covariance: false"
-        )
+    if op.hydro.bot_heat_flux != 0.0 and op.hydro.grav1 == 0.0:
+        raise ValueError("forcing/bot-heat is built on the gravity path here (needs grav1)")
     if op.intg.type != "rk3":
         raise ValueError(f"integration/type '{op.intg.type}': rk3 only")
     if op.intg.implicit_scheme not in (0, 9):

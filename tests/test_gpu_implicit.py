@@ -63,22 +63,29 @@ def test_vic_kernel_matches_oracle(shape):
         assert err < 1e-11
 
 
-@pytest.mark.parametrize("g", [fv.Grid(24, 16, 1, 0.0, 6400.0, -6400.0, 6400.0, 0.0, 100.0),
-                               fv.Grid(12, 8, 6, 0.0, 3200.0, -3200.0, 3200.0, -1600.0, 1600.0)])
-def test_whole_block_implicit(g):
+CASES = [
+    (fv.Grid(24, 16, 1, 0.0, 6400.0, -6400.0, 6400.0, 0.0, 100.0), ("reflecting",) * 6, False, 0.0),
+    (fv.Grid(12, 8, 6, 0.0, 3200.0, -3200.0, 3200.0, -1600.0, 1600.0), ("reflecting",) * 6, False, 0.0),
+    # the 3D convection physics: periodic x2/x3, covariance on, bottom heating
+    (fv.Grid(16, 8, 8, 0.0, 800.0, 0.0, 800.0, 0.0, 800.0),
+     ("reflecting", "reflecting", "periodic", "periodic", "periodic", "periodic"), True, 100.0),
+]
+
+
+@pytest.mark.parametrize("g,bcs,cov,heat", CASES)
+def test_whole_block_implicit(g, bcs, cov, heat):
     from snapy_amd import MeshBlock
     from snapy_amd.options import CoordinateOptions, HydroOptions, IntegratorOptions, MeshBlockOptions
 
-    bcs = ("reflecting",) * 6
     d = fv.Deck(grid=g, gammad=GAMMA, riemann="lmars", shock_x1=False, shock_x23=False, bcs=bcs,
-                grav1=-9.81, cfl=0.9, implicit_scheme=9)
+                grav1=-9.81, cfl=0.9, implicit_scheme=9, flux_covariance=cov, bot_heat_flux=heat)
     w = stratified(g, seed=4, noise=0.5)
     ref = fv.Block(d)
     ref.initialize(w)
     op = MeshBlockOptions()
     op.coord = CoordinateOptions("cartesian", g.x1min, g.x1max, g.x2min, g.x2max, g.x3min, g.x3max,
                                  g.nx1, g.nx2, g.nx3, 3)
-    op.hydro = HydroOptions(riemann="lmars", grav1=-9.81, flux_covariance=False)
+    op.hydro = HydroOptions(riemann="lmars", grav1=-9.81, flux_covariance=cov, bot_heat_flux=heat)
     op.hydro.eos.gammad = GAMMA
     op.hydro.recon1.shock = False
     op.hydro.recon23.shock = False
