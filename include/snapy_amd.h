@@ -201,7 +201,9 @@ int snapy_update_gravity(const snapy_grid *g, const snapy_physics *p,
                          double w0, double w1, double w2, void *stream);
 
 /* Vertical implicit correction, full 5x5 matrix (implicit-scheme 9), FP64,
- * one column per thread, then the RK combine u = w0*u0 + w1*u + w2*du'
+ * one column per 8-lane group (lane p owns row p of every 5x5 block; the
+ * blocks are eliminated by Gauss-Jordan with scaled partial pivoting across
+ * the row lanes), then the RK combine u = w0*u0 + w1*u + w2*du'
  * with du' the corrected increment (interior cells).  du is the explicit
  * stage increment (snapy_update_gravity with u0 NULL).  The solve runs with
  * dt_c = wstage * dt (the stage weight inside the solve,
@@ -222,6 +224,16 @@ int snapy_vic_full(const snapy_grid *g, const snapy_physics *p, const double *w,
                    double t1, double dt, const double *dt_dev, double wstage,
                    double w0, double w1, double w2, double *ws, int *status,
                    void *stream);
+/* The same solve with one column per thread and the reference's Crout LU
+ * (ludcmp/lubksb) on each block: the same arguments and outputs as
+ * snapy_vic_full, kept as the independent cross-check of the group kernel. */
+int snapy_vic_full_serial(const snapy_grid *g, const snapy_physics *p,
+                          const double *w, const double *du, const double *u0,
+                          double *u, const double *vtab, const double *x1tab,
+                          double t0, double t1, double dt,
+                          const double *dt_dev, double wstage, double w0,
+                          double w1, double w2, double *ws, int *status,
+                          void *stream);
 
 /* FP64 budget: out[v] = sum over interior cells of u[v] * volume, for
  * v < nvar, deterministic (fixed reduction tree).  out is device FP64 of

@@ -89,6 +89,7 @@ SIGNATURES = {
     "snapy_update_gravity": (_I, [_GP, _PP, _I, _P * 3, _P, _P, _P, _P, _P, _D, _P, _D, _D, _D, _P]),
     "snapy_vic_workspace": (ctypes.c_int64, [_GP]),
     "snapy_vic_full": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P]),
+    "snapy_vic_full_serial": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P]),
     "snapy_budget_workspace": (ctypes.c_int64, [_GP]),
     "snapy_budget": (_I, [_GP, _I, _P, _P, _P, _P]),
 }

@@ -420,6 +420,517 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// One column per 5 lanes, 12 columns per wavefront (lanes 60..63 idle).
+// Lane p owns row p of every 5x5 block and of every right-hand side.  The
+// face matrices |A| and the cell flux Jacobians are carried from one layer
+// to the next, so each layer builds one Roe average and one Jacobian row.
+// The block solve is Gauss-Jordan with the scaled partial pivoting of
+// ludcmp; rows move between lanes with ds_bpermute shuffles.
+// ---------------------------------------------------------------------------
+constexpr int kSlots = 12;  // columns per wavefront
+
+__device__ __forceinline__ double from_lane(double v, int src) { return __shfl(v, src); }
+
+// one x1 cell's Roe-average inputs: w, sqrt(rho), (E + p) / sqrt(rho)
+struct RoeCell {
+  double w[5];
+  double sd, hs;
+};
+
+__device__ __forceinline__ RoeCell roe_cell(double const* w, double gm1) {
+  RoeCell c;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) c.w[v] = w[v];
+  c.sd = sqrt(w[IDN]);
+  double e = w[IPR] / gm1 +
+             0.5 * w[IDN] * (w[IVX] * w[IVX] + w[IVY] * w[IVY] + w[IVZ] * w[IVZ]);
+  c.hs = (e + w[IPR]) / c.sd;
+  return c;
+}
+
+// rows p and 0 of |A| = R Lambda R^-1 at the Roe average of cells l, r
+// (RoeAverage, SoundSpeed, Eigenvalue, Eigenvector:
+// flux_decomposition_impl.h:13-103); the R^-1 entries use the reciprocals
+// of 2 cs hp and hp
+__device__ __forceinline__ void roe_rows(double* rowp, double* row0, int p,
+                                         double gm1, double gp1_gm1,
+                                         RoeCell const& l, RoeCell const& r) {
+  double is = 1.0 / (l.sd + r.sd);
+  double rho = l.sd * r.sd;
+  double u = (l.sd * l.w[IVX] + r.sd * r.w[IVX]) * is;
+  double v = (l.sd * l.w[IVY] + r.sd * r.w[IVY]) * is;
+  double w = (l.sd * l.w[IVZ] + r.sd * r.w[IVZ]) * is;
+  double hbar = (l.hs + r.hs) * is;
+  double pp = (hbar - 0.5 * (u * u + v * v + w * w)) * gm1 / (gm1 + 1.) * rho;
+  double cs = sqrt(pp * (gm1 + 1.) / rho);
+  double ke = 0.5 * (u * u + v * v + w * w);
+  double hp = gp1_gm1 * pp / rho;
+  double h = hp + ke;
+  double lam0 = fabs(u - cs), lam1 = fabs(u), lam2 = fabs(u + cs);
+  double ihp = 1. / hp, i2hp = 0.5 * ihp, i2c = 1. / (2. * cs * hp);
+  double Ri[5][5] = {{(cs * ke + u * hp) * i2c, (-hp - cs * u) * i2c, -v * i2hp, -w * i2hp, i2hp},
+                     {(hp - ke) * ihp, u * ihp, v * ihp, w * ihp, -ihp},
+                     {(cs * ke - u * hp) * i2c, (hp - cs * u) * i2c, -v * i2hp, -w * i2hp, i2hp},
+                     {-v, 0., 1., 0., 0.},
+                     {-w, 0., 0., 1., 0.}};
+  // row p of R (R = [1 1 1 0 0; u-c u u+c 0 0; v v v 1 0; w w w 0 1;
+  // h-uc ke h+uc v w])
+  double vt = p == 1 ? u : (p == 2 ? v : w);
+  double R0 = p == 0 ? 1. : (p == 4 ? h - u * cs : (p == 1 ? u - cs : vt));
+  double R1 = p == 0 ? 1. : (p == 4 ? ke : vt);
+  double R2 = p == 0 ? 1. : (p == 4 ? h + u * cs : (p == 1 ? u + cs : vt));
+  double R3 = p == 2 ? 1. : (p == 4 ? v : 0.);
+  double R4 = p == 3 ? 1. : (p == 4 ? w : 0.);
+  double a0 = R0 * lam0, a1 = R1 * lam1, a2 = R2 * lam2, a3 = R3 * lam1, a4 = R4 * lam1;
+#pragma unroll
+  for (int b = 0; b < 5; ++b) {
+    rowp[b] = (((a0 * Ri[0][b] + a1 * Ri[1][b]) + a2 * Ri[2][b]) + a3 * Ri[3][b]) + a4 * Ri[4][b];
+    row0[b] = (lam0 * Ri[0][b] + lam1 * Ri[1][b]) + lam2 * Ri[2][b];
+  }
+}
+
+// row p of FluxJacobian (flux_decomposition_impl.h:105-126), dir 0
+__device__ __forceinline__ void jacobian_row(double* f, int p, double gm1,
+                                             double gp1_gm1, double const* w) {
+  double v1 = w[IVX], v2 = w[IVY], v3 = w[IVZ];
+  double s2 = v1 * v1 + v2 * v2 + v3 * v3;
+  double hr = gp1_gm1 * w[IPR] / w[IDN];
+  double c1 = ((gm1 - 1) * s2 / 2 - hr) * v1;
+  double c2 = hr + s2 / 2 - gm1 * v1 * v1;
+  bool r1 = p == 1, r2 = p == 2, r3 = p == 3, r4 = p == 4;
+  f[0] = r1 ? gm1 * s2 / 2 - v1 * v1 : r2 ? -v1 * v2 : r3 ? -v1 * v3 : r4 ? c1 : 0.;
+  f[1] = r1 ? (2. - gm1) * v1 : r2 ? v2 : r3 ? v3 : r4 ? c2 : 1.;
+  f[2] = r1 ? -gm1 * v2 : r2 ? v1 : r4 ? -gm1 * v2 * v1 : 0.;
+  f[3] = r1 ? -gm1 * v3 : r3 ? v1 : r4 ? -gm1 * v3 * v1 : 0.;
+  f[4] = r1 ? gm1 : r4 ? (gm1 + 1) * v1 : 0.;
+}
+
+__global__ void __launch_bounds__(64)
+    vic_forward_kernel(double const* __restrict__ w, double const* __restrict__ du,
+                       double const* __restrict__ vtab, double* __restrict__ ws,
+                       int* __restrict__ status, VicArgs a) {
+  Range3 const& r = a.r;
+  int const lane = threadIdx.x;
+  int const slot = lane / 5, p = lane - 5 * slot, base = 5 * slot;
+  int64_t const col = int64_t(blockIdx.x) * kSlots + slot;
+  int64_t const ncol = int64_t(r.n2) * r.n3;
+  if (slot >= kSlots || col >= ncol) return;  // a column's five lanes leave together
+  int const j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
+  int64_t const row = k * r.s3 + j * r.s2 + r.lo1;
+  int const n = a.n, nc1 = r.nc1, lo1 = r.lo1;
+  double const gm1 = a.gamma - 1.;
+  double const gp1_gm1 = (gm1 + 1) / gm1;
+  double const grav = a.grav, A = a.area, V = a.vol;
+  double const dt = (a.dt_dev ? *a.dt_dev : a.dt_host) * a.wstage;
+  double const idt = 1. / dt;
+  double const hiv = 0.5 / V;
+  double const AH = A * hiv;
+  auto W = [&](int v, int i) { return w[v * r.sv + row + i]; };
+  auto DU = [&](int v, int i) { return du[v * r.sv + row + i]; };
+  // the wavefront's workspace is one contiguous run: [layer][element][slot]
+  double* wsb = ws + int64_t(blockIdx.x) * n * NW * kSlots + slot;
+  auto WS = [&](int l, int e) -> double& { return wsb[(l * NW + e) * kSlots]; };
+  auto tab = [&](double const* t, int rowi, int i) { return t[rowi * nc1 + lo1 + i]; };
+  auto rows_at = [&](int i, auto q) {
+    double s = tab(vtab, V_KMID, i) * q(i);
+    if (i > 0) s = s + tab(vtab, V_KLO, i) * q(i - 1);
+    if (i < n - 1) s = s + tab(vtab, V_KHI, i) * q(i + 1);
+    if (i == 0) s = s + a.t0 * q(2);
+    if (i == n - 1) s = s + a.t1 * q(n - 3);
+    return s;
+  };
+  auto mass0 = [&](int i) { return DU(IDN, i); };
+  auto load_w = [&](int i, double* o) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) o[v] = W(v, i);
+  };
+
+  // carried state: cell i, the face below it, the Jacobians of cells i-1, i
+  double wt[5];
+  load_w(-1, wt);
+  RoeCell cl = roe_cell(wt, gm1);
+  double fl[5], fc[5];
+  jacobian_row(fl, p, gm1, gp1_gm1, wt);
+  load_w(0, wt);
+  RoeCell cc = roe_cell(wt, gm1);
+  jacobian_row(fc, p, gm1, gp1_gm1, wt);
+  double am[5], am0[5];
+  roe_rows(am, am0, p, gm1, gp1_gm1, cl, cc);
+  double Ap[5] = {0, 0, 0, 0, 0}, dpv = 0., z0v = 0., z1v = 0.;
+  bool ok = true;
+  double const tol = 8. * 5. * 2.220446049250313e-16;  // ludcmp's singularity test
+  // the next layer's loads are issued one layer ahead (cells up to n + 1
+  // are ghost cells of the x1 row)
+  load_w(1, wt);
+  double du_next = DU(p, 0);
+
+  for (int i = 0; i < n; ++i) {
+    RoeCell cr = roe_cell(wt, gm1);
+    double fr[5], apm[5], ap0[5];
+    jacobian_row(fr, p, gm1, gp1_gm1, wt);
+    double du_p = du_next;
+    load_w(i + 2, wt);
+    du_next = DU(p, i + 1);
+    roe_rows(apm, ap0, p, gm1, gp1_gm1, cc, cr);
+    // block rows (the (A - A) * FluxJacobian(cell) term of the diagonal is
+    // zero on the uniform Cartesian x1 grid)
+    double ar[5], br[5], crw[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      br[q] = -(am[q] + fl[q]) * A * hiv;
+      crw[q] = -(apm[q] - fr[q]) * A * hiv;
+      ar[q] = (am[q] * A + apm[q] * A) * hiv;
+    }
+    ar[IDN] -= p == IVX ? grav : 0.;
+    ar[IVX] -= p == IPR ? grav : 0.;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) ar[q] += q == p ? idt : 0.;
+    double rhs = du_p * idt;
+    if (p == IPR) {
+      // gravity-work rows (vic_gravity_work_impl.h:24-36)
+      double lo = tab(vtab, V_WLO, i), up = tab(vtab, V_WHI, i);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        ar[q] -= grav * (up * ap0[q] - lo * am0[q]);
+        br[q] -= grav * lo * am0[q];
+        crw[q] += grav * up * ap0[q];
+      }
+      ar[IVX] += grav * (1. - lo - up);
+      br[IVX] -= grav * lo;
+      crw[IVX] -= grav * up;
+    }
+    // the wall mirror merges the boundary face: Bnd = (1, -1, 1, 1, 1)
+    if (i == 0)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) ar[q] += br[q] * (q == IVX ? -1. : 1.);
+    if (i == n - 1)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) ar[q] += crw[q] * (q == IVX ? -1. : 1.);
+    if (p == IPR) {
+      // couple_radial_exact (implicit_hydro.cpp:408-446)
+      ar[IDN] -= tab(vtab, V_KMID, i) / dt;
+      br[IDN] -= tab(vtab, V_KLO, i) / dt;
+      crw[IDN] -= tab(vtab, V_KHI, i) / dt;
+      rhs = (DU(IPR, i) - rows_at(i, mass0)) / dt;
+    }
+    // right-hand sides: c | du/dt | unit E at the bottom | unit E at the top
+    double x[8];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) x[q] = crw[q];
+    x[5] = rhs;
+    x[6] = (i == 0 && p == IPR) ? 1. : 0.;
+    x[7] = (i == n - 1 && p == IPR) ? 1. : 0.;
+    if (i > 0) {
+      // a_i -= b_i A'_{i-1}, rhs -= b_i d_{i-1}, e0 -= b_i z0_{i-1}; the
+      // top-wall column z1 is zero below the top layer
+      double s[5] = {0, 0, 0, 0, 0}, s5 = 0., s6 = 0.;
+#pragma unroll
+      for (int m = 0; m < 5; ++m) {
+        double bm = br[m];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) s[q] += bm * from_lane(Ap[q], base + m);
+        s5 += bm * from_lane(dpv, base + m);
+        s6 += bm * from_lane(z0v, base + m);
+      }
+#pragma unroll
+      for (int q = 0; q < 5; ++q) ar[q] -= s[q];
+      x[5] -= s5;
+      x[6] -= s6;
+    }
+    // Gauss-Jordan, scaled partial pivoting (the last maximum wins, as in
+    // ludcmp); columns left of the pivot are not needed again
+    double big = 0.;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) big = fmax(big, fabs(ar[q]));
+    double scale = big > 0. ? 1. / big : 0.;
+    unsigned used = 0;
+    int piv[5];
+#pragma unroll
+    for (int jc = 0; jc < 5; ++jc) {
+      double bv = (used >> p) & 1 ? -1. : scale * fabs(ar[jc]);
+      double best = -1.;
+      int bl = 0;
+#pragma unroll
+      for (int m = 0; m < 5; ++m) {
+        double c = from_lane(bv, base + m);
+        if (c >= best) {
+          best = c;
+          bl = m;
+        }
+      }
+      if (!(best > tol) || !isfinite(best)) ok = false;
+      used |= 1u << bl;
+      piv[jc] = bl;
+      int src = base + bl;
+      double pj = from_lane(ar[jc], src);
+      double fct = p == bl ? 0. : ar[jc] * (1. / pj);
+#pragma unroll
+      for (int q = jc + 1; q < 5; ++q) ar[q] -= fct * from_lane(ar[q], src);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) x[t] -= fct * from_lane(x[t], src);
+    }
+    // this lane pivoted the unknown whose column it still holds; lane p
+    // takes the solution row of unknown p from its pivot lane
+    int myj = 0;
+#pragma unroll
+    for (int jc = 0; jc < 5; ++jc) myj = piv[jc] == p ? jc : myj;
+    double dj = ar[0];
+#pragma unroll
+    for (int jc = 1; jc < 5; ++jc) dj = myj == jc ? ar[jc] : dj;
+    double idj = 1. / dj;
+    int srcp = piv[0];
+#pragma unroll
+    for (int jc = 1; jc < 5; ++jc) srcp = p == jc ? piv[jc] : srcp;
+    double X[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) X[t] = from_lane(x[t] * idj, base + srcp);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) Ap[q] = X[q];
+    dpv = X[5];
+    z0v = X[6];
+    z1v = X[7];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      if (!isfinite(X[t])) ok = false;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) WS(i, p * 5 + q) = X[q];
+    WS(i, 25 + p) = dpv;
+    WS(i, 30 + p) = z0v;
+    // slide the window up one layer
+    cl = cc;
+    cc = cr;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      fl[q] = fc[q];
+      fc[q] = fr[q];
+      am[q] = apm[q];
+      am0[q] = ap0[q];
+    }
+  }
+  // a column fails as a whole: its five lanes agree
+  int okv = ok ? 1 : 0;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) okv &= __shfl(okv, base + m);
+  if (p == 0) status[col] = okv ? 0 : 1;
+  WS(n - 1, 35 + p) = z1v;  // the top-wall column, zero below the top layer
+}
+
+// The back half of the VIC, with the lane and workspace layout of
+// vic_forward_kernel: back substitution, the Woodbury correction, the dry
+// mass redistribution, the post-solve remainder and the RK combine.  It
+// holds few registers, so many wavefronts share a SIMD and hide the latency
+// of the workspace round trips; loads are issued in batches of layers.
+__global__ void __launch_bounds__(64)
+    vic_back_kernel(double const* __restrict__ w, double const* __restrict__ du,
+                    double const* __restrict__ u0, double* __restrict__ u,
+                    double const* __restrict__ vtab,
+                    double const* __restrict__ xtab, double* __restrict__ ws,
+                    int const* __restrict__ status, VicArgs a) {
+  Range3 const& r = a.r;
+  int const lane = threadIdx.x;
+  int const slot = lane / 5, p = lane - 5 * slot, base = 5 * slot;
+  int64_t const col = int64_t(blockIdx.x) * kSlots + slot;
+  int64_t const ncol = int64_t(r.n2) * r.n3;
+  if (slot >= kSlots || col >= ncol) return;
+  int const j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
+  int64_t const row = k * r.s3 + j * r.s2 + r.lo1;
+  int const n = a.n, nc1 = r.nc1, lo1 = r.lo1;
+  double const V = a.vol;
+  double const dt = (a.dt_dev ? *a.dt_dev : a.dt_host) * a.wstage;
+  auto W = [&](int v, int i) { return w[v * r.sv + row + i]; };
+  auto DU = [&](int v, int i) { return du[v * r.sv + row + i]; };
+  double* wsb = ws + int64_t(blockIdx.x) * n * NW * kSlots + slot;
+  auto WS = [&](int l, int e) -> double& { return wsb[(l * NW + e) * kSlots]; };
+  auto tab = [&](double const* t, int rowi, int i) { return t[rowi * nc1 + lo1 + i]; };
+  auto rows_at = [&](int i, auto q) {
+    double s = tab(vtab, V_KMID, i) * q(i);
+    if (i > 0) s = s + tab(vtab, V_KLO, i) * q(i - 1);
+    if (i < n - 1) s = s + tab(vtab, V_KHI, i) * q(i + 1);
+    if (i == 0) s = s + a.t0 * q(2);
+    if (i == n - 1) s = s + a.t1 * q(n - 3);
+    return s;
+  };
+  auto combine = [&](int v, int i, double out) {
+    int64_t o = v * r.sv + row + i;
+    u[o] = a.w0 * u0[o] + a.w1 * u[o] + a.w2 * out;
+  };
+  if (status[col]) {
+    // a failed column keeps its explicit increments (the reference undoes
+    // the column with its status bits and flags a retry)
+    for (int i = 0; i < n; ++i) combine(p, i, DU(p, i));
+    return;
+  }
+  constexpr int B = 4;  // layers per batch of loads
+  // ---- back substitution (vic_backward_substitute) ------------------------
+  int const i0 = 2, i1 = n - 3;
+  double dpv = WS(n - 1, 25 + p), z0v = WS(n - 1, 30 + p), z1v = WS(n - 1, 35 + p);
+  double d_i0 = 0., z0_i0 = 0., z1_i0 = 0., d_i1 = 0., z0_i1 = 0., z1_i1 = 0.;
+  for (int ib = n - 2; ib >= 0; ib -= B) {
+    double Ab[B][5], db[B], zb[B];
+#pragma unroll
+    for (int t = 0; t < B; ++t) {
+      int i = ib - t > 0 ? ib - t : 0;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) Ab[t][q] = WS(i, p * 5 + q);
+      db[t] = WS(i, 25 + p);
+      zb[t] = WS(i, 30 + p);
+    }
+#pragma unroll
+    for (int t = 0; t < B; ++t) {
+      int i = ib - t;
+      if (i >= 0) {
+        double sd = 0., s0 = 0., s1 = 0.;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          double aq = Ab[t][q];
+          sd += aq * from_lane(dpv, base + q);
+          s0 += aq * from_lane(z0v, base + q);
+          s1 += aq * from_lane(z1v, base + q);
+        }
+        dpv = db[t] - sd;
+        z0v = zb[t] - s0;
+        z1v = 0. - s1;  // the forward z1 is zero below the top layer
+        WS(i, 25 + p) = dpv;
+        WS(i, 30 + p) = z0v;
+        WS(i, 35 + p) = z1v;
+        if (i == i0) {
+          d_i0 = from_lane(dpv, base);
+          z0_i0 = from_lane(z0v, base);
+          z1_i0 = from_lane(z1v, base);
+        }
+        if (i == i1) {
+          d_i1 = from_lane(dpv, base);
+          z0_i1 = from_lane(z0v, base);
+          z1_i1 = from_lane(z1v, base);
+        }
+      }
+    }
+  }
+  // ---- Woodbury 2x2 (vic_woodbury_column); every lane solves it -----------
+  double k0 = -a.t0 / dt, k1 = -a.t1 / dt;
+  double m00 = 1. + k0 * z0_i0, m01 = k0 * z1_i0;
+  double m10 = k1 * z0_i1, m11 = 1. + k1 * z1_i1;
+  double r0 = k0 * d_i0, r1 = k1 * d_i1;
+  double det = m00 * m11 - m01 * m10;
+  double al0 = (m11 * r0 - m01 * r1) / det, al1 = (m00 * r1 - m10 * r0) / det;
+  // the momentum rows are final here; the mass and energy rows wait for the
+  // redistribution
+  bool const direct = p != IDN && p != IPR;
+  for (int ib = 0; ib < n; ib += B) {
+    double y[B], c0[B], c1[B];
+#pragma unroll
+    for (int t = 0; t < B; ++t) {
+      int i = ib + t < n ? ib + t : n - 1;
+      y[t] = WS(i, 25 + p);
+      c0[t] = WS(i, 30 + p);
+      c1[t] = WS(i, 35 + p);
+    }
+#pragma unroll
+    for (int t = 0; t < B; ++t) {
+      int i = ib + t;
+      if (i < n) {
+        double yt = y[t];
+        yt -= c0[t] * al0;
+        yt -= c1[t] * al1;
+        if (direct)
+          combine(p, i, yt);
+        else
+          WS(i, 25 + p) = yt;
+      }
+    }
+  }
+  // ---- mass redistribution (vic_constituent_column, dry), lane 0 ----------
+  // per layer: [1] M at the face below, [2] mass change, [3] moved q below
+  if (p == 0) {
+    constexpr int C = 8;
+    double R = 0., S = 0.;
+    for (int ib = 0; ib < n; ib += C) {
+      double y0[C], m0[C], rho[C];
+#pragma unroll
+      for (int t = 0; t < C; ++t) {
+        int i = ib + t < n ? ib + t : n - 1;
+        y0[t] = WS(i, 25);
+        m0[t] = DU(IDN, i);
+        rho[t] = W(IDN, i);
+      }
+#pragma unroll
+      for (int t = 0; t < C; ++t)
+        if (ib + t < n) {
+          R += (y0[t] - m0[t]) * V;
+          S += rho[t] * V;
+        }
+    }
+    double M = 0.;
+    double avail = (W(IDN, 0) * 1. + DU(IDN, 0)) * V;
+    if (avail < 0) avail = 0;
+    double carry = 0.;  // mass change of the current cell from the face below
+    WS(0, 3) = 0.;
+    for (int ib = 0; ib < n; ib += C) {
+      double y0[C], m0[C], rho[C], mu[C], rhou[C];
+#pragma unroll
+      for (int t = 0; t < C; ++t) {
+        int i = ib + t < n ? ib + t : n - 1;
+        int iu = i + 1 < n ? i + 1 : n - 1;
+        y0[t] = WS(i, 25);
+        m0[t] = DU(IDN, i);
+        rho[t] = W(IDN, i);
+        mu[t] = DU(IDN, iu);
+        rhou[t] = W(IDN, iu);
+      }
+#pragma unroll
+      for (int t = 0; t < C; ++t) {
+        int i = ib + t;
+        if (i < n) {
+          WS(i, 1) = M;
+          double phi = (y0[t] - m0[t]) * V;
+          M -= phi - R * (rho[t] * V / S);
+          if (i + 1 < n) {
+            double avail_up = (rhou[t] * 1. + mu[t]) * V;
+            if (avail_up < 0) avail_up = 0;
+            double mf = M;
+            double q = mf > 0 ? mf * 1. : mf * 1.;
+            if (q > avail) q = avail;
+            if (q < -avail_up) q = -avail_up;
+            WS(i, 2) = carry - q / V;
+            carry = 0. + q / V;
+            WS(i + 1, 3) = q;
+            avail = avail_up + q;
+          } else {
+            WS(i, 2) = carry;
+          }
+        }
+      }
+    }
+  }
+  // lane 0's and lane 4's rows become visible to the column's other lanes
+  __threadfence();
+  // ---- mass and energy outputs, the remainder (book_remainder_) -----------
+  auto moved_rho = [&](int i) { return (DU(IDN, i) + WS(i, 2)) - DU(IDN, i); };
+  auto raw_minus_mass0 = [&](int i) { return WS(i, 25) - DU(IDN, i); };
+  for (int i = p; i < n; i += 5) {
+    combine(IDN, i, DU(IDN, i) + WS(i, 2));
+    double out = WS(i, 29);
+    double req_lo = WS(i, 1), req_hi = i + 1 < n ? WS(i + 1, 1) : 0.;
+    double mv_lo = WS(i, 3) - req_lo;
+    double mv_hi = (i + 1 < n ? WS(i + 1, 3) : 0.) - req_hi;
+    double projected = (req_lo - req_hi) / V;
+    double pw = tab(vtab, V_PHI, i) * (WS(i, 25) - DU(IDN, i) - projected);
+    double cw = -(tab(vtab, V_DPHI, i) * mv_hi - tab(vtab, V_DPLO, i) * mv_lo) / V;
+    out = out + (pw + cw);
+    // corrected-PE work of the moved mass minus the operator's part
+    int jj = (i < 1 ? 1 : (i > n - 2 ? n - 2 : i)) - 1;
+    double s = tab(xtab, T_WT0, i) * moved_rho(jj);
+    s = s + tab(xtab, T_WT1, i) * moved_rho(jj + 1);
+    s = s + tab(xtab, T_WT2, i) * moved_rho(jj + 2);
+    double work = tab(xtab, T_GVAR, i) * s;
+    work = work - rows_at(i, raw_minus_mass0);
+    out = out + work;
+    combine(IPR, i, out);
+  }
+}
+
 }  // namespace
 }  // namespace snapy
 
@@ -429,15 +940,18 @@ extern "C" {
 
 int64_t snapy_vic_workspace(const snapy_grid* g) {
   if (!g) return 0;
-  return int64_t(NW) * g->nx1 * g->nx2 * g->nx3;
+  // whole wavefronts of kSlots columns (the serial kernel needs less)
+  int64_t ncol = int64_t(g->nx2) * g->nx3;
+  int64_t padded = (ncol + kSlots - 1) / kSlots * kSlots;
+  return int64_t(NW) * g->nx1 * padded;
 }
 
-int snapy_vic_full(const snapy_grid* g, const snapy_physics* p, const double* w,
-                   const double* du, const double* u0, double* u,
-                   const double* vtab, const double* x1tab, double t0,
-                   double t1, double dt, const double* dt_dev, double wstage,
-                   double w0, double w1, double w2, double* ws, int* status,
-                   void* stream) {
+static int vic_launch(const snapy_grid* g, const snapy_physics* p,
+                      const double* w, const double* du, const double* u0,
+                      double* u, const double* vtab, const double* x1tab,
+                      double t0, double t1, double dt, const double* dt_dev,
+                      double wstage, double w0, double w1, double w2,
+                      double* ws, int* status, void* stream, bool serial) {
   if (int e = check_grid(g)) return e;
   if (!p || !w || !du || !u0 || !u || !vtab || !x1tab || !ws || !status)
     return fail(-1, "vic_full: NULL argument");
@@ -458,10 +972,41 @@ int snapy_vic_full(const snapy_grid* g, const snapy_physics* p, const double* w,
   a.w1 = w1;
   a.w2 = w2;
   int64_t ncol = int64_t(g->nx2) * g->nx3;
-  hipLaunchKernelGGL(vic_full_kernel, dim3(blocks_for(ncol, 64)), dim3(64), 0,
-                     as_stream(stream), w, du, u0, u, vtab, x1tab, ws, status,
-                     a);
+  if (serial)
+    hipLaunchKernelGGL(vic_full_kernel, dim3(blocks_for(ncol, 64)), dim3(64),
+                       0, as_stream(stream), w, du, u0, u, vtab, x1tab, ws,
+                       status, a);
+  else {
+    unsigned nb = blocks_for(ncol, kSlots);
+    hipLaunchKernelGGL(vic_forward_kernel, dim3(nb), dim3(64), 0,
+                       as_stream(stream), w, du, vtab, ws, status, a);
+    if (int e = launch_status("vic_forward")) return e;
+    hipLaunchKernelGGL(vic_back_kernel, dim3(nb), dim3(64), 0,
+                       as_stream(stream), w, du, u0, u, vtab, x1tab, ws,
+                       status, a);
+  }
   return launch_status("vic_full");
+}
+
+int snapy_vic_full(const snapy_grid* g, const snapy_physics* p, const double* w,
+                   const double* du, const double* u0, double* u,
+                   const double* vtab, const double* x1tab, double t0,
+                   double t1, double dt, const double* dt_dev, double wstage,
+                   double w0, double w1, double w2, double* ws, int* status,
+                   void* stream) {
+  return vic_launch(g, p, w, du, u0, u, vtab, x1tab, t0, t1, dt, dt_dev,
+                    wstage, w0, w1, w2, ws, status, stream, false);
+}
+
+int snapy_vic_full_serial(const snapy_grid* g, const snapy_physics* p,
+                          const double* w, const double* du, const double* u0,
+                          double* u, const double* vtab, const double* x1tab,
+                          double t0, double t1, double dt,
+                          const double* dt_dev, double wstage, double w0,
+                          double w1, double w2, double* ws, int* status,
+                          void* stream) {
+  return vic_launch(g, p, w, du, u0, u, vtab, x1tab, t0, t1, dt, dt_dev,
+                    wstage, w0, w1, w2, ws, status, stream, true);
 }
 
 }  // extern "C"

@@ -17,8 +17,9 @@ from tests.test_gpu_gravity import GAMMA, stratified
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("shape", [(16, 6, 1), (12, 4, 3)])
-def test_vic_kernel_matches_oracle(shape):
+@pytest.mark.parametrize("entry", ["snapy_vic_full", "snapy_vic_full_serial"])
+@pytest.mark.parametrize("shape", [(16, 6, 1), (12, 4, 3), (40, 37, 3)])
+def test_vic_kernel_matches_oracle(shape, entry):
     lib = _capi.load()
     g = fv.Grid(*shape, 0.0, 50.0 * shape[0], -400.0, 400.0, 0.0, 300.0)
     w = stratified(g, seed=5, noise=0.5)
@@ -40,8 +41,8 @@ def test_vic_kernel_matches_oracle(shape):
     u0 = torch.zeros(w.shape, dtype=torch.float64, device="cuda")
     u = torch.zeros_like(u0)
     tabd, xtd, wd, dud = tab.cuda(), xt.cuda(), dev(w), dev(du)
-    print(f"vic kernel {shape}", flush=True)
-    rc = lib.snapy_vic_full(ctypes.byref(gd), ctypes.byref(pd), wd.data_ptr(), dud.data_ptr(), u0.data_ptr(),
+    print(f"vic kernel {entry} {shape}", flush=True)
+    rc = getattr(lib, entry)(ctypes.byref(gd), ctypes.byref(pd), wd.data_ptr(), dud.data_ptr(), u0.data_ptr(),
                             u.data_ptr(), tabd.data_ptr(), xtd.data_ptr(), t0, t1, dt, None, 1.0, 0.0, 0.0, 1.0,
                             ws.data_ptr(), st.data_ptr(), None)
     assert rc == 0, lib.snapy_last_error()
