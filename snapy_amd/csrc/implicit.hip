@@ -507,18 +507,89 @@ __device__ __forceinline__ void jacobian_row(double* f, int p, double gm1,
   f[4] = r1 ? gm1 : r4 ? (gm1 + 1) * v1 : 0.;
 }
 
-__global__ void __launch_bounds__(64)
+constexpr int kQ = 8;          // workspace doubles per lane per layer: A' row, d, z0, z1
+constexpr int kLB = 8;         // layers per staged block of the forward sweep
+constexpr int kWin = kLB + 4;  // staged layers [l0 - 2, l0 + kLB + 2)
+constexpr int kOB = 16;        // layers per output block of the back kernel
+constexpr int kOWin = kOB + 4;
+
+// The lanes of a wavefront: 12 columns of 5 lanes.  Lanes 60..63 shadow
+// lanes 0..3 of the last column and never write; columns past the grid's
+// last are computed on the last column and never written.
+struct WaveLanes {
+  int slot, p, base;
+  bool shadow;  // lanes 60..63
+  bool live;    // owns a column of the grid
+  int64_t col;  // the column computed on
+};
+
+__device__ __forceinline__ WaveLanes wave_lanes(int64_t ncol) {
+  WaveLanes L;
+  int lane = threadIdx.x;
+  L.shadow = lane >= 5 * kSlots;
+  L.slot = L.shadow ? kSlots - 1 : lane / 5;
+  L.p = L.shadow ? lane - 5 * kSlots : lane - 5 * L.slot;
+  L.base = 5 * L.slot;
+  int64_t c = int64_t(blockIdx.x) * kSlots + L.slot;
+  L.live = !L.shadow && c < ncol;
+  L.col = c < ncol ? c : ncol - 1;
+  return L;
+}
+
+__device__ __forceinline__ int64_t column_row(Range3 const& r, int64_t col) {
+  int j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
+  return k * r.s3 + j * r.s2 + r.lo1;
+}
+
+// Coalesced cooperative copy of layers [i0, i0 + CNT) of NV rows (from row
+// v0) of a state array into LDS dst[v][slot][CNT] for the wavefront's
+// columns; layer indices are clamped to the x1 row's ghost range [-3, n+2].
+// All loads are issued before the first LDS store.
+template <int NV, int CNT>
+__device__ __forceinline__ void stage_layers(double* dst, double const* __restrict__ src,
+                                             int64_t sv, int v0, int64_t const* srow,
+                                             int i0, int n) {
+  constexpr int TOT = NV * kSlots * CNT;
+  constexpr int IT = (TOT + 63) / 64;
+  double val[IT];
+#pragma unroll
+  for (int t = 0; t < IT; ++t) {
+    int f = threadIdx.x + 64 * t;
+    if (f < TOT) {
+      int v = f / (kSlots * CNT);
+      int h = f - v * (kSlots * CNT);
+      int sl = h / CNT;
+      int i = i0 + (h - sl * CNT);
+      i = i < -3 ? -3 : (i > n + 2 ? n + 2 : i);
+      val[t] = src[(v0 + v) * sv + srow[sl] + i];
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < IT; ++t) {
+    int f = threadIdx.x + 64 * t;
+    if (f < TOT) dst[f] = val[t];
+  }
+}
+
+// The forward sweep of the VIC: block assembly, the corrected-PE coupling,
+// Gauss-Jordan on [a | c, rhs, e0, e1] and the forward elimination of the
+// block-tridiagonal system.  The column state is staged through LDS in
+// blocks of kLB layers; each lane writes its row of [A' | d | z0 | z1] to a
+// lane-contiguous workspace ws[blk][layer][q][64].
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     vic_forward_kernel(double const* __restrict__ w, double const* __restrict__ du,
                        double const* __restrict__ vtab, double* __restrict__ ws,
                        int* __restrict__ status, VicArgs a) {
   Range3 const& r = a.r;
-  int const lane = threadIdx.x;
-  int const slot = lane / 5, p = lane - 5 * slot, base = 5 * slot;
-  int64_t const col = int64_t(blockIdx.x) * kSlots + slot;
   int64_t const ncol = int64_t(r.n2) * r.n3;
-  if (slot >= kSlots || col >= ncol) return;  // a column's five lanes leave together
-  int const j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
-  int64_t const row = k * r.s3 + j * r.s2 + r.lo1;
+  WaveLanes const L = wave_lanes(ncol);
+  int const p = L.p, base = L.base, slot = L.slot;
+  __shared__ int64_t s_row[kSlots];
+  __shared__ double s_w[5 * kSlots * kWin], s_du[5 * kSlots * kWin];
+  if (threadIdx.x < kSlots) {
+    int64_t c = int64_t(blockIdx.x) * kSlots + threadIdx.x;
+    s_row[threadIdx.x] = column_row(r, c < ncol ? c : ncol - 1);
+  }
   int const n = a.n, nc1 = r.nc1, lo1 = r.lo1;
   double const gm1 = a.gamma - 1.;
   double const gp1_gm1 = (gm1 + 1) / gm1;
@@ -526,13 +597,19 @@ __global__ void __launch_bounds__(64)
   double const dt = (a.dt_dev ? *a.dt_dev : a.dt_host) * a.wstage;
   double const idt = 1. / dt;
   double const hiv = 0.5 / V;
-  double const AH = A * hiv;
-  auto W = [&](int v, int i) { return w[v * r.sv + row + i]; };
-  auto DU = [&](int v, int i) { return du[v * r.sv + row + i]; };
-  // the wavefront's workspace is one contiguous run: [layer][element][slot]
-  double* wsb = ws + int64_t(blockIdx.x) * n * NW * kSlots + slot;
-  auto WS = [&](int l, int e) -> double& { return wsb[(l * NW + e) * kSlots]; };
+  double* wsl = ws + int64_t(blockIdx.x) * n * kQ * 64 + threadIdx.x;
+  auto WS = [&](int l, int q) -> double& { return wsl[(l * kQ + q) * 64]; };
   auto tab = [&](double const* t, int rowi, int i) { return t[rowi * nc1 + lo1 + i]; };
+  int l0 = 0;  // first layer of the staged block
+  auto SW = [&](int v, int i) { return s_w[(v * kSlots + slot) * kWin + (i - l0 + 2)]; };
+  auto SD = [&](int v, int i) { return s_du[(v * kSlots + slot) * kWin + (i - l0 + 2)]; };
+  auto stage = [&](int start) {
+    __syncthreads();
+    l0 = start;
+    stage_layers<5, kWin>(s_w, w, r.sv, 0, s_row, start - 2, n);
+    stage_layers<5, kWin>(s_du, du, r.sv, 0, s_row, start - 2, n);
+    __syncthreads();
+  };
   auto rows_at = [&](int i, auto q) {
     double s = tab(vtab, V_KMID, i) * q(i);
     if (i > 0) s = s + tab(vtab, V_KLO, i) * q(i - 1);
@@ -541,19 +618,18 @@ __global__ void __launch_bounds__(64)
     if (i == n - 1) s = s + a.t1 * q(n - 3);
     return s;
   };
-  auto mass0 = [&](int i) { return DU(IDN, i); };
-  auto load_w = [&](int i, double* o) {
-#pragma unroll
-    for (int v = 0; v < 5; ++v) o[v] = W(v, i);
-  };
+  auto mass0 = [&](int i) { return SD(IDN, i); };
 
+  stage(0);
   // carried state: cell i, the face below it, the Jacobians of cells i-1, i
   double wt[5];
-  load_w(-1, wt);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) wt[v] = SW(v, -1);
   RoeCell cl = roe_cell(wt, gm1);
   double fl[5], fc[5];
   jacobian_row(fl, p, gm1, gp1_gm1, wt);
-  load_w(0, wt);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) wt[v] = SW(v, 0);
   RoeCell cc = roe_cell(wt, gm1);
   jacobian_row(fc, p, gm1, gp1_gm1, wt);
   double am[5], am0[5];
@@ -561,18 +637,14 @@ __global__ void __launch_bounds__(64)
   double Ap[5] = {0, 0, 0, 0, 0}, dpv = 0., z0v = 0., z1v = 0.;
   bool ok = true;
   double const tol = 8. * 5. * 2.220446049250313e-16;  // ludcmp's singularity test
-  // the next layer's loads are issued one layer ahead (cells up to n + 1
-  // are ghost cells of the x1 row)
-  load_w(1, wt);
-  double du_next = DU(p, 0);
 
   for (int i = 0; i < n; ++i) {
+    if (i > 0 && i % kLB == 0) stage(i);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) wt[v] = SW(v, i + 1);
     RoeCell cr = roe_cell(wt, gm1);
     double fr[5], apm[5], ap0[5];
     jacobian_row(fr, p, gm1, gp1_gm1, wt);
-    double du_p = du_next;
-    load_w(i + 2, wt);
-    du_next = DU(p, i + 1);
     roe_rows(apm, ap0, p, gm1, gp1_gm1, cc, cr);
     // block rows (the (A - A) * FluxJacobian(cell) term of the diagonal is
     // zero on the uniform Cartesian x1 grid)
@@ -587,7 +659,7 @@ __global__ void __launch_bounds__(64)
     ar[IVX] -= p == IPR ? grav : 0.;
 #pragma unroll
     for (int q = 0; q < 5; ++q) ar[q] += q == p ? idt : 0.;
-    double rhs = du_p * idt;
+    double rhs = SD(p, i) * idt;
     if (p == IPR) {
       // gravity-work rows (vic_gravity_work_impl.h:24-36)
       double lo = tab(vtab, V_WLO, i), up = tab(vtab, V_WHI, i);
@@ -613,7 +685,7 @@ __global__ void __launch_bounds__(64)
       ar[IDN] -= tab(vtab, V_KMID, i) / dt;
       br[IDN] -= tab(vtab, V_KLO, i) / dt;
       crw[IDN] -= tab(vtab, V_KHI, i) / dt;
-      rhs = (DU(IPR, i) - rows_at(i, mass0)) / dt;
+      rhs = (SD(IPR, i) - rows_at(i, mass0)) / dt;
     }
     // right-hand sides: c | du/dt | unit E at the bottom | unit E at the top
     double x[8];
@@ -695,9 +767,9 @@ __global__ void __launch_bounds__(64)
     for (int t = 0; t < 8; ++t)
       if (!isfinite(X[t])) ok = false;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) WS(i, p * 5 + q) = X[q];
-    WS(i, 25 + p) = dpv;
-    WS(i, 30 + p) = z0v;
+    for (int q = 0; q < 5; ++q) WS(i, q) = X[q];
+    WS(i, 5) = dpv;
+    WS(i, 6) = z0v;
     // slide the window up one layer
     cl = cc;
     cc = cr;
@@ -713,59 +785,52 @@ __global__ void __launch_bounds__(64)
   int okv = ok ? 1 : 0;
 #pragma unroll
   for (int m = 0; m < 5; ++m) okv &= __shfl(okv, base + m);
-  if (p == 0) status[col] = okv ? 0 : 1;
-  WS(n - 1, 35 + p) = z1v;  // the top-wall column, zero below the top layer
+  if (p == 0 && L.live) status[L.col] = okv ? 0 : 1;
+  WS(n - 1, 7) = z1v;  // the top-wall column, zero below the top layer
 }
 
-// The back half of the VIC, with the lane and workspace layout of
+// The back half of the VIC, with the lanes and workspace of
 // vic_forward_kernel: back substitution, the Woodbury correction, the dry
-// mass redistribution, the post-solve remainder and the RK combine.  It
-// holds few registers, so many wavefronts share a SIMD and hide the latency
-// of the workspace round trips; loads are issued in batches of layers.
-__global__ void __launch_bounds__(64)
+// mass redistribution, the post-solve remainder and the RK combine.  Every
+// global access is coalesced: the workspace is lane-contiguous, the column
+// state is staged through LDS and the outputs are transposed through LDS
+// before the combine.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
     vic_back_kernel(double const* __restrict__ w, double const* __restrict__ du,
                     double const* __restrict__ u0, double* __restrict__ u,
                     double const* __restrict__ vtab,
                     double const* __restrict__ xtab, double* __restrict__ ws,
                     int const* __restrict__ status, VicArgs a) {
   Range3 const& r = a.r;
-  int const lane = threadIdx.x;
-  int const slot = lane / 5, p = lane - 5 * slot, base = 5 * slot;
-  int64_t const col = int64_t(blockIdx.x) * kSlots + slot;
   int64_t const ncol = int64_t(r.n2) * r.n3;
-  if (slot >= kSlots || col >= ncol) return;
-  int const j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
-  int64_t const row = k * r.s3 + j * r.s2 + r.lo1;
+  WaveLanes const L = wave_lanes(ncol);
+  int const p = L.p, base = L.base, slot = L.slot;
+  __shared__ int64_t s_row[kSlots];
+  __shared__ int s_live[kSlots];
+  __shared__ double s_buf[5 * kSlots * kOB];  // staged mass/density, then outputs
+  __shared__ double s_mw[kSlots * kOWin];     // staged du row IDN
+  if (threadIdx.x < kSlots) {
+    int64_t c = int64_t(blockIdx.x) * kSlots + threadIdx.x;
+    s_row[threadIdx.x] = column_row(r, c < ncol ? c : ncol - 1);
+    s_live[threadIdx.x] = c < ncol;
+  }
+  __syncthreads();
   int const n = a.n, nc1 = r.nc1, lo1 = r.lo1;
+  int64_t const row = s_row[slot];
   double const V = a.vol;
   double const dt = (a.dt_dev ? *a.dt_dev : a.dt_host) * a.wstage;
-  auto W = [&](int v, int i) { return w[v * r.sv + row + i]; };
-  auto DU = [&](int v, int i) { return du[v * r.sv + row + i]; };
-  double* wsb = ws + int64_t(blockIdx.x) * n * NW * kSlots + slot;
-  auto WS = [&](int l, int e) -> double& { return wsb[(l * NW + e) * kSlots]; };
+  double* wsw = ws + int64_t(blockIdx.x) * n * kQ * 64;
+  double* wsl = wsw + threadIdx.x;
+  double* wsb = wsw + base;  // lane 0 of the column
+  auto WS = [&](int l, int q) -> double& { return wsl[(l * kQ + q) * 64]; };
+  auto WSB = [&](int l, int q) -> double& { return wsb[(l * kQ + q) * 64]; };
   auto tab = [&](double const* t, int rowi, int i) { return t[rowi * nc1 + lo1 + i]; };
-  auto rows_at = [&](int i, auto q) {
-    double s = tab(vtab, V_KMID, i) * q(i);
-    if (i > 0) s = s + tab(vtab, V_KLO, i) * q(i - 1);
-    if (i < n - 1) s = s + tab(vtab, V_KHI, i) * q(i + 1);
-    if (i == 0) s = s + a.t0 * q(2);
-    if (i == n - 1) s = s + a.t1 * q(n - 3);
-    return s;
-  };
-  auto combine = [&](int v, int i, double out) {
-    int64_t o = v * r.sv + row + i;
-    u[o] = a.w0 * u0[o] + a.w1 * u[o] + a.w2 * out;
-  };
-  if (status[col]) {
-    // a failed column keeps its explicit increments (the reference undoes
-    // the column with its status bits and flags a retry)
-    for (int i = 0; i < n; ++i) combine(p, i, DU(p, i));
-    return;
-  }
-  constexpr int B = 4;  // layers per batch of loads
+  bool const failed = status[L.col] != 0;
+  constexpr int B = 4;  // layers per batch of workspace loads
+
   // ---- back substitution (vic_backward_substitute) ------------------------
   int const i0 = 2, i1 = n - 3;
-  double dpv = WS(n - 1, 25 + p), z0v = WS(n - 1, 30 + p), z1v = WS(n - 1, 35 + p);
+  double dpv = WS(n - 1, 5), z0v = WS(n - 1, 6), z1v = WS(n - 1, 7);
   double d_i0 = 0., z0_i0 = 0., z1_i0 = 0., d_i1 = 0., z0_i1 = 0., z1_i1 = 0.;
   for (int ib = n - 2; ib >= 0; ib -= B) {
     double Ab[B][5], db[B], zb[B];
@@ -773,9 +838,9 @@ __global__ void __launch_bounds__(64)
     for (int t = 0; t < B; ++t) {
       int i = ib - t > 0 ? ib - t : 0;
 #pragma unroll
-      for (int q = 0; q < 5; ++q) Ab[t][q] = WS(i, p * 5 + q);
-      db[t] = WS(i, 25 + p);
-      zb[t] = WS(i, 30 + p);
+      for (int q = 0; q < 5; ++q) Ab[t][q] = WS(i, q);
+      db[t] = WS(i, 5);
+      zb[t] = WS(i, 6);
     }
 #pragma unroll
     for (int t = 0; t < B; ++t) {
@@ -792,9 +857,9 @@ __global__ void __launch_bounds__(64)
         dpv = db[t] - sd;
         z0v = zb[t] - s0;
         z1v = 0. - s1;  // the forward z1 is zero below the top layer
-        WS(i, 25 + p) = dpv;
-        WS(i, 30 + p) = z0v;
-        WS(i, 35 + p) = z1v;
+        WS(i, 5) = dpv;
+        WS(i, 6) = z0v;
+        WS(i, 7) = z1v;
         if (i == i0) {
           d_i0 = from_lane(dpv, base);
           z0_i0 = from_lane(z0v, base);
@@ -815,119 +880,152 @@ __global__ void __launch_bounds__(64)
   double r0 = k0 * d_i0, r1 = k1 * d_i1;
   double det = m00 * m11 - m01 * m10;
   double al0 = (m11 * r0 - m01 * r1) / det, al1 = (m00 * r1 - m10 * r0) / det;
-  // the momentum rows are final here; the mass and energy rows wait for the
-  // redistribution
-  bool const direct = p != IDN && p != IPR;
-  for (int ib = 0; ib < n; ib += B) {
-    double y[B], c0[B], c1[B];
+  // the corrected increment of this lane's row: y = d - z0 al0 - z1 al1
+  auto ywb = [&](int l) {
+    double y = WS(l, 5);
+    y -= WS(l, 6) * al0;
+    y -= WS(l, 7) * al1;
+    return y;
+  };
+  // ---- mass redistribution (vic_constituent_column, dry) ------------------
+  // lane 0 of each column scans; lane 0's dead A' row keeps per layer
+  // [0] M at the face below, [1] the mass change, [2] the moved q below,
+  // and its [5] becomes the corrected mass increment
+  constexpr int SB = kOB + 1;
+  double* s_m0 = s_buf;
+  double* s_rho = s_buf + kSlots * SB;
+  double R = 0., S = 0.;
+  for (int b0 = 0; b0 < n; b0 += kOB) {
+    __syncthreads();
+    stage_layers<1, SB>(s_m0, du, r.sv, IDN, s_row, b0, n);
+    stage_layers<1, SB>(s_rho, w, r.sv, IDN, s_row, b0, n);
+    __syncthreads();
+    if (p == 0) {
 #pragma unroll
-    for (int t = 0; t < B; ++t) {
-      int i = ib + t < n ? ib + t : n - 1;
-      y[t] = WS(i, 25 + p);
-      c0[t] = WS(i, 30 + p);
-      c1[t] = WS(i, 35 + p);
-    }
-#pragma unroll
-    for (int t = 0; t < B; ++t) {
-      int i = ib + t;
-      if (i < n) {
-        double yt = y[t];
-        yt -= c0[t] * al0;
-        yt -= c1[t] * al1;
-        if (direct)
-          combine(p, i, yt);
-        else
-          WS(i, 25 + p) = yt;
-      }
-    }
-  }
-  // ---- mass redistribution (vic_constituent_column, dry), lane 0 ----------
-  // per layer: [1] M at the face below, [2] mass change, [3] moved q below
-  if (p == 0) {
-    constexpr int C = 8;
-    double R = 0., S = 0.;
-    for (int ib = 0; ib < n; ib += C) {
-      double y0[C], m0[C], rho[C];
-#pragma unroll
-      for (int t = 0; t < C; ++t) {
-        int i = ib + t < n ? ib + t : n - 1;
-        y0[t] = WS(i, 25);
-        m0[t] = DU(IDN, i);
-        rho[t] = W(IDN, i);
-      }
-#pragma unroll
-      for (int t = 0; t < C; ++t)
-        if (ib + t < n) {
-          R += (y0[t] - m0[t]) * V;
-          S += rho[t] * V;
+      for (int t = 0; t < kOB; ++t)
+        if (b0 + t < n) {
+          R += (ywb(b0 + t) - s_m0[slot * SB + t]) * V;
+          S += s_rho[slot * SB + t] * V;
         }
     }
-    double M = 0.;
-    double avail = (W(IDN, 0) * 1. + DU(IDN, 0)) * V;
-    if (avail < 0) avail = 0;
-    double carry = 0.;  // mass change of the current cell from the face below
-    WS(0, 3) = 0.;
-    for (int ib = 0; ib < n; ib += C) {
-      double y0[C], m0[C], rho[C], mu[C], rhou[C];
-#pragma unroll
-      for (int t = 0; t < C; ++t) {
-        int i = ib + t < n ? ib + t : n - 1;
-        int iu = i + 1 < n ? i + 1 : n - 1;
-        y0[t] = WS(i, 25);
-        m0[t] = DU(IDN, i);
-        rho[t] = W(IDN, i);
-        mu[t] = DU(IDN, iu);
-        rhou[t] = W(IDN, iu);
+  }
+  double M = 0., avail = 0., carry = 0.;
+  for (int b0 = 0; b0 < n; b0 += kOB) {
+    __syncthreads();
+    stage_layers<1, SB>(s_m0, du, r.sv, IDN, s_row, b0, n);
+    stage_layers<1, SB>(s_rho, w, r.sv, IDN, s_row, b0, n);
+    __syncthreads();
+    if (p == 0) {
+      if (b0 == 0) {
+        avail = (s_rho[slot * SB] * 1. + s_m0[slot * SB]) * V;
+        if (avail < 0) avail = 0;
+        WS(0, 2) = 0.;
       }
-#pragma unroll
-      for (int t = 0; t < C; ++t) {
-        int i = ib + t;
+#pragma unroll 8
+      for (int t = 0; t < kOB; ++t) {
+        int i = b0 + t;
         if (i < n) {
-          WS(i, 1) = M;
-          double phi = (y0[t] - m0[t]) * V;
-          M -= phi - R * (rho[t] * V / S);
+          double y0 = ywb(i);
+          WS(i, 5) = y0;  // read back as the mass row by the other lanes
+          WS(i, 0) = M;
+          double phi = (y0 - s_m0[slot * SB + t]) * V;
+          M -= phi - R * (s_rho[slot * SB + t] * V / S);
           if (i + 1 < n) {
-            double avail_up = (rhou[t] * 1. + mu[t]) * V;
+            double avail_up = (s_rho[slot * SB + t + 1] * 1. + s_m0[slot * SB + t + 1]) * V;
             if (avail_up < 0) avail_up = 0;
             double mf = M;
             double q = mf > 0 ? mf * 1. : mf * 1.;
             if (q > avail) q = avail;
             if (q < -avail_up) q = -avail_up;
-            WS(i, 2) = carry - q / V;
+            WS(i, 1) = carry - q / V;
             carry = 0. + q / V;
-            WS(i + 1, 3) = q;
+            WS(i + 1, 2) = q;
             avail = avail_up + q;
           } else {
-            WS(i, 2) = carry;
+            WS(i, 1) = carry;
           }
         }
       }
     }
   }
-  // lane 0's and lane 4's rows become visible to the column's other lanes
+  // lane 0's rows become visible to the column's other lanes
   __threadfence();
-  // ---- mass and energy outputs, the remainder (book_remainder_) -----------
-  auto moved_rho = [&](int i) { return (DU(IDN, i) + WS(i, 2)) - DU(IDN, i); };
-  auto raw_minus_mass0 = [&](int i) { return WS(i, 25) - DU(IDN, i); };
-  for (int i = p; i < n; i += 5) {
-    combine(IDN, i, DU(IDN, i) + WS(i, 2));
-    double out = WS(i, 29);
-    double req_lo = WS(i, 1), req_hi = i + 1 < n ? WS(i + 1, 1) : 0.;
-    double mv_lo = WS(i, 3) - req_lo;
-    double mv_hi = (i + 1 < n ? WS(i + 1, 3) : 0.) - req_hi;
-    double projected = (req_lo - req_hi) / V;
-    double pw = tab(vtab, V_PHI, i) * (WS(i, 25) - DU(IDN, i) - projected);
-    double cw = -(tab(vtab, V_DPHI, i) * mv_hi - tab(vtab, V_DPLO, i) * mv_lo) / V;
-    out = out + (pw + cw);
-    // corrected-PE work of the moved mass minus the operator's part
-    int jj = (i < 1 ? 1 : (i > n - 2 ? n - 2 : i)) - 1;
-    double s = tab(xtab, T_WT0, i) * moved_rho(jj);
-    s = s + tab(xtab, T_WT1, i) * moved_rho(jj + 1);
-    s = s + tab(xtab, T_WT2, i) * moved_rho(jj + 2);
-    double work = tab(xtab, T_GVAR, i) * s;
-    work = work - rows_at(i, raw_minus_mass0);
-    out = out + work;
-    combine(IPR, i, out);
+  // ---- outputs, the remainder (book_remainder_), the RK combine -----------
+  double* s_out = s_buf;
+  for (int b0 = 0; b0 < n; b0 += kOB) {
+    __syncthreads();
+    stage_layers<1, kOWin>(s_mw, du, r.sv, IDN, s_row, b0 - 2, n);
+    __syncthreads();
+    auto M0 = [&](int i) { return s_mw[slot * kOWin + (i - b0 + 2)]; };
+    auto moved_rho = [&](int i) { return (M0(i) + WSB(i, 1)) - M0(i); };
+    auto raw_minus_mass0 = [&](int i) { return WSB(i, 5) - M0(i); };
+#pragma unroll 8
+    for (int t = 0; t < kOB; ++t) {
+      int i = b0 + t;
+      double out = 0.;
+      if (i < n) {
+        if (p == IDN) {
+          out = M0(i) + WSB(i, 1);
+        } else if (p == IPR) {
+          out = ywb(i);
+          double req_lo = WSB(i, 0), req_hi = i + 1 < n ? WSB(i + 1, 0) : 0.;
+          double mv_lo = WSB(i, 2) - req_lo;
+          double mv_hi = (i + 1 < n ? WSB(i + 1, 2) : 0.) - req_hi;
+          double projected = (req_lo - req_hi) / V;
+          double pw = tab(vtab, V_PHI, i) * (WSB(i, 5) - M0(i) - projected);
+          double cw = -(tab(vtab, V_DPHI, i) * mv_hi - tab(vtab, V_DPLO, i) * mv_lo) / V;
+          out = out + (pw + cw);
+          // corrected-PE work of the moved mass minus the operator's part
+          int jj = (i < 1 ? 1 : (i > n - 2 ? n - 2 : i)) - 1;
+          double s = tab(xtab, T_WT0, i) * moved_rho(jj);
+          s = s + tab(xtab, T_WT1, i) * moved_rho(jj + 1);
+          s = s + tab(xtab, T_WT2, i) * moved_rho(jj + 2);
+          double work = tab(xtab, T_GVAR, i) * s;
+          double rs = tab(vtab, V_KMID, i) * raw_minus_mass0(i);
+          if (i > 0) rs = rs + tab(vtab, V_KLO, i) * raw_minus_mass0(i - 1);
+          if (i < n - 1) rs = rs + tab(vtab, V_KHI, i) * raw_minus_mass0(i + 1);
+          if (i == 0) rs = rs + a.t0 * raw_minus_mass0(2);
+          if (i == n - 1) rs = rs + a.t1 * raw_minus_mass0(n - 3);
+          work = work - rs;
+          out = out + work;
+        } else {
+          out = ywb(i);
+        }
+        // a failed column keeps its explicit increments (the reference
+        // undoes the column with its status bits and flags a retry)
+        if (failed) out = du[p * r.sv + row + i];
+      }
+      if (!L.shadow) s_out[(p * kSlots + slot) * kOB + t] = out;
+    }
+    __syncthreads();
+    // the RK combine u = w0 u0 + w1 u + w2 du' over the block, coalesced,
+    // in chunks whose loads are all issued before their stores
+    constexpr int TOT = 5 * kSlots * kOB;
+    constexpr int CH = 5;  // elements per lane per chunk
+    for (int f0 = 0; f0 < TOT; f0 += 64 * CH) {
+      double x0[CH], x1[CH];
+      int64_t oo[CH];
+#pragma unroll
+      for (int t = 0; t < CH; ++t) {
+        int f = f0 + threadIdx.x + 64 * t;
+        oo[t] = -1;
+        x0[t] = 0.;
+        x1[t] = 0.;
+        if (f < TOT) {
+          int v = f / (kSlots * kOB), h = f - v * (kSlots * kOB);
+          int sl = h / kOB, i = b0 + (h - sl * kOB);
+          if (i < n && s_live[sl]) {
+            oo[t] = v * r.sv + s_row[sl] + i;
+            x0[t] = u0[oo[t]];
+            x1[t] = u[oo[t]];
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < CH; ++t)
+        if (oo[t] >= 0)
+          u[oo[t]] = a.w0 * x0[t] + a.w1 * x1[t] + a.w2 * s_out[f0 + threadIdx.x + 64 * t];
+    }
   }
 }
 
@@ -940,10 +1038,12 @@ extern "C" {
 
 int64_t snapy_vic_workspace(const snapy_grid* g) {
   if (!g) return 0;
-  // whole wavefronts of kSlots columns (the serial kernel needs less)
+  // the group kernels: kQ doubles per lane per layer for whole wavefronts;
+  // the serial kernel: NW per column per layer
   int64_t ncol = int64_t(g->nx2) * g->nx3;
-  int64_t padded = (ncol + kSlots - 1) / kSlots * kSlots;
-  return int64_t(NW) * g->nx1 * padded;
+  int64_t group = (ncol + kSlots - 1) / kSlots * 64 * kQ * g->nx1;
+  int64_t serial = int64_t(NW) * g->nx1 * ncol;
+  return group > serial ? group : serial;
 }
 
 static int vic_launch(const snapy_grid* g, const snapy_physics* p,
