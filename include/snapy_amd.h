@@ -76,6 +76,9 @@ typedef struct snapy_physics {
   double bot_heat_flux;  /* forcing/bot-heat flux (W/m^2), 0 = none */
   int32_t bot_heat_depth; /* forcing/bot-heat depth (cells) */
   int32_t flux_covariance; /* dynamics/flux-covariance on the x2/x3 faces */
+  double nu_iso;         /* forcing/diffusion kinematic viscosity (m^2/s) */
+  double kappa_iso;      /* forcing/diffusion thermal diffusivity (m^2/s) */
+  double gas_r;          /* dry gas constant Rgas / weight (J/kg/K) */
 } snapy_physics;
 
 /* ---- entry points ----------------------------------------------------- */
@@ -142,7 +145,8 @@ int snapy_step_field(const snapy_grid *g, int dtype, const void *total,
                      const double *dt_dev, void *stream);
 
 /* CFL reduction: *dt_out (device, FP64) = scale * min over interior cells
- * and active directions of dx_d / (|v_d| + c) (HydroImpl::max_time_step,
+ * and active directions of dx_d / (|v_d| + c), and of the kinematic
+ * diffusion bound when p->nu_iso or p->kappa_iso is set (HydroImpl::max_time_step,
  * explicit branch, src/hydro/hydro.cpp:372-392; scale = 2^-redo * cfl as in
  * global_time_step, src/mesh/meshblock_advance.cpp:65-72).  dt_out is
  * overwritten, not merged; no host read. */
@@ -234,6 +238,24 @@ int snapy_vic_full_serial(const snapy_grid *g, const snapy_physics *p,
                           const double *dt_dev, double wstage, double w0,
                           double w1, double w2, double *ws, int *status,
                           void *stream);
+
+/* Isotropic Cartesian diffusion, kinematic form (forcing/diffusion with
+ * nu_iso, kappa_iso; no x1 profiles; stress-free walls).  w is the FP64
+ * primitive state with ghosts filled.  For every active direction d the
+ * viscous momentum flux, its work and the Fourier flux -kappa (rho cv) dT/dn
+ * are formed on the faces (a reflecting x1 wall face extrapolates the face
+ * density from the two nearest active cells); the energy row is added to
+ * totals[d] (dtype) on every face that is not a reflecting wall, and the
+ * momentum rows become du_src = -dt div(F) (rows IDN and IPR zero) on the
+ * interior cells, FP64, the du_src of snapy_update / snapy_update_gravity.
+ * work: snapy_diffusion_workspace(g) FP64 doubles.
+ * Replaces: DiffusionImpl::cartesian_fluxes + post_fluxes + add
+ * (src/forcing/diffusion.cpp:532-706, diffusion_cubed_sphere.cpp:365-379). */
+int64_t snapy_diffusion_workspace(const snapy_grid *g);
+int snapy_diffusion(const snapy_grid *g, const snapy_physics *p, int dtype,
+                    const double *w, void *const totals[3], double *work,
+                    double *du_src, double dt, const double *dt_dev,
+                    void *stream);
 
 /* FP64 budget: out[v] = sum over interior cells of u[v] * volume, for
  * v < nvar, deterministic (fixed reduction tree).  out is device FP64 of

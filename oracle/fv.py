@@ -599,6 +599,10 @@ class Deck:
     flux_covariance: bool = False
     bot_heat_flux: float = 0.0
     bot_heat_depth: int = 1
+    # forcing/diffusion (kinematic, isotropic); weight is the EOS molar mass
+    nu_iso: float = 0.0
+    kappa_iso: float = 0.0
+    weight: float = 0.02897025302492418
 
 
 @dataclass
@@ -654,6 +658,14 @@ class Block:
             flux[axis] = f
             if axis == 1:
                 self.face_pressure1 = p
+        fdiff = None
+        if d.nu_iso > 0.0 or d.kappa_iso > 0.0:
+            # the diffusion posts: the energy row joins the totals before the
+            # divergence (hydro_forward.cpp:930-935)
+            from . import diffusion
+
+            fdiff = diffusion.cartesian_fluxes(self.w, g, d.nu_iso, d.kappa_iso, d.gammad, d.weight, d.bcs)
+            diffusion.post_fluxes(flux, fdiff, g, d.bcs)
         self.flux = flux
         div = divergence(g, flux)
         du = np.zeros_like(div)
@@ -663,6 +675,12 @@ class Block:
             from . import gravity
 
             gravity.const_gravity(du, self.w, dt, d.grav1)
+        if fdiff is not None:
+            # DiffusionImpl::add: the momentum rows (diffusion.cpp:532-547)
+            from . import diffusion
+
+            diffusion.add_momentum(du, fdiff, g, dt)
+        if d.grav1 != 0.0:
             if d.bot_heat_flux != 0.0:
                 # LayerHeat bot-heat (layer_heat.cpp:78-85)
                 il = g.nghost
@@ -708,7 +726,13 @@ class Block:
 
     def max_time_step(self):
         d = self.deck
-        return d.cfl * max_time_step(d.grid, self.w, d.gammad, d.implicit_scheme, d.cfl)
+        dt = max_time_step(d.grid, self.w, d.gammad, d.implicit_scheme, d.cfl)
+        if d.nu_iso > 0.0 or d.kappa_iso > 0.0:
+            # the sources' bound joins before the CFL factor (hydro.cpp:396)
+            from . import diffusion
+
+            dt = min(dt, diffusion.max_time_step(d.grid, d.nu_iso, d.kappa_iso))
+        return d.cfl * dt
 
 
 def sod_deck(nx=256):

@@ -61,6 +61,9 @@ class Physics(ctypes.Structure):
         ("bot_heat_flux", ctypes.c_double),
         ("bot_heat_depth", ctypes.c_int32),
         ("flux_covariance", ctypes.c_int32),
+        ("nu_iso", ctypes.c_double),
+        ("kappa_iso", ctypes.c_double),
+        ("gas_r", ctypes.c_double),
     ]
 
 
@@ -88,6 +91,8 @@ SIGNATURES = {
     "snapy_sweep_x1_wb": (_I, [_GP, _PP, _I, _P, _P, _P, _P, _P]),
     "snapy_update_gravity": (_I, [_GP, _PP, _I, _P * 3, _P, _P, _P, _P, _P, _D, _P, _D, _D, _D, _P]),
     "snapy_vic_workspace": (ctypes.c_int64, [_GP]),
+    "snapy_diffusion_workspace": (ctypes.c_int64, [_GP]),
+    "snapy_diffusion": (_I, [_GP, _PP, _I, _P, _P * 3, _P, _P, _D, _P, _P]),
     "snapy_vic_full": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P]),
     "snapy_vic_full_serial": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P]),
     "snapy_budget_workspace": (ctypes.c_int64, [_GP]),

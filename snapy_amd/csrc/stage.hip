@@ -513,6 +513,25 @@ int snapy_cfl(const snapy_grid* g, const snapy_physics* p, int dtype,
   int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
   auto out = reinterpret_cast<unsigned long long*>(dt_out);
   double init = 1.e9;
+  // the sources' bound joins before the CFL factor (hydro.cpp:396): the
+  // kinematic diffusion bound dx_min^2 / (2 ndim max(nu, kappa))
+  // (DiffusionImpl::max_time_step, src/forcing/diffusion.cpp:708-756)
+  double coeff = p->nu_iso > p->kappa_iso ? p->nu_iso : p->kappa_iso;
+  if (coeff > 0.) {
+    int ndim = 0;
+    double dxm = 1.e300;
+    double dxs[3] = {g->dx1, g->dx2, g->dx3};
+    int ncs[3] = {g->nc1, g->nc2, g->nc3};
+    for (int ax = 0; ax < 3; ++ax)
+      if (ncs[ax] > 1) {
+        ++ndim;
+        dxm = dxs[ax] < dxm ? dxs[ax] : dxm;
+      }
+    if (ndim > 0) {
+      double dtd = dxm * dxm / (2. * ndim * coeff);
+      init = dtd < init ? dtd : init;
+    }
+  }
   unsigned long long bits;
   std::memcpy(&bits, &init, sizeof(bits));
   hipStream_t s = as_stream(stream);

@@ -124,6 +124,9 @@ class MeshBlock:
             h.bot_heat_flux,
             int(h.bot_heat_depth),
             int(h.flux_covariance),
+            h.nu_iso,
+            h.kappa_iso,
+            h.eos.gas_constant_dry(),
         )
         self._build_coord()
 
@@ -219,6 +222,14 @@ class MeshBlock:
         )
         if h.grav1 != 0.0:
             self._ref = torch.zeros((5, c.nc3(), c.nc2(), c.nc1()), dtype=torch.float64, device=self.device)
+        self._diff = None
+        if h.nu_iso > 0.0 or h.kappa_iso > 0.0:
+            lib = _capi.load()
+            nwk = int(lib.snapy_diffusion_workspace(ctypes.byref(self._grid)))
+            self._diff = {
+                "work": torch.zeros(nwk, dtype=torch.float64, device=self.device),
+                "du": torch.zeros(self.shape(), dtype=torch.float64, device=self.device),
+            }
         self._vic = None
         if self.options.intg.implicit_scheme == 9:
             from .geometry import vic_table
@@ -346,6 +357,19 @@ class MeshBlock:
                 )
             else:
                 calls.append((f"sweep_x{d}", lambda d=d, t=t, fp=fp: lib.snapy_sweep(d, g, p, dtc, P(w), P(t), P(fp), s)))
+        # the diffusion posts: the energy row joins the totals, the momentum
+        # rows become the sources' du (before the step fields read the totals)
+        dsrc = None
+        if self._diff is not None:
+            dif = self._diff
+            tot = (ctypes.c_void_p * 3)(*[P(freg.total(d)) for d in (1, 2, 3)])
+            calls.append(
+                (
+                    "diffusion",
+                    lambda: lib.snapy_diffusion(g, p, dtc, P(w), tot, P(dif["work"]), P(dif["du"]), 0.0, dtp, s),
+                )
+            )
+            dsrc = P(dif["du"])
         # FluxRegister::end_stage: S_d <- w1 S_d + w2 dt T_d
         for d in (1, 2, 3):
             if freg.step(d) is not None:
@@ -366,7 +390,7 @@ class MeshBlock:
                 (
                     "stage_increment",
                     lambda: lib.snapy_update_gravity(
-                        g, p, dtc, fl, None, P(vic["du"]), P(w), P(self._x1tab), None, 0.0, dtp, 1.0, 0.0, 1.0, s
+                        g, p, dtc, fl, None, P(vic["du"]), P(w), P(self._x1tab), dsrc, 0.0, dtp, 1.0, 0.0, 1.0, s
                     ),
                 )
             )
@@ -386,7 +410,7 @@ class MeshBlock:
                 (
                     "update_gravity",
                     lambda: lib.snapy_update_gravity(
-                        g, p, dtc, fl, P(u0), P(u), P(w), P(self._x1tab), None, 0.0, dtp,
+                        g, p, dtc, fl, P(u0), P(u), P(w), P(self._x1tab), dsrc, 0.0, dtp,
                         st.wght0(), st.wght1(), st.wght2(), s,
                     ),
                 )
@@ -396,7 +420,7 @@ class MeshBlock:
                 (
                     "update",
                     lambda: lib.snapy_update(
-                        g, dtc, fl, P(u0), P(u), None, 0.0, dtp, st.wght0(), st.wght1(), st.wght2(), s
+                        g, dtc, fl, P(u0), P(u), dsrc, 0.0, dtp, st.wght0(), st.wght1(), st.wght2(), s
                     ),
                 )
             )

@@ -111,6 +111,9 @@ class HydroOptions:
     bot_heat_depth: int = 1
     precision: str = "float64"
     flux_covariance: bool = True
+    # forcing/diffusion, kinematic isotropic coefficients (m^2/s)
+    nu_iso: float = 0.0
+    kappa_iso: float = 0.0
 
 
 @dataclass
@@ -229,7 +232,21 @@ def _read_dynamics(d, forcing) -> HydroOptions:
     h.riemann = rs.get("type", h.riemann)
     h.precision = d.get("precision", h.precision)
     h.flux_covariance = bool(d.get("flux-covariance", h.flux_covariance))
-    _check_keys(forcing, "forcing", {"const-gravity", "bot-heat"})
+    _check_keys(forcing, "forcing", {"const-gravity", "bot-heat", "diffusion"})
+    if "diffusion" in forcing:
+        # DiffusionOptionsImpl::from_yaml (src/forcing/diffusion.cpp:280-366):
+        # the kinematic isotropic form with stress-free walls is built here
+        df = forcing["diffusion"] or {}
+        _check_keys(df, "forcing/diffusion", {"nu_iso", "kappa_iso", "dynamic", "wall_velocity"})
+        if str(df.get("dynamic", "false")).lower() not in ("false",):
+            raise ValueError("forcing/diffusion dynamic: true is not implemented")
+        if str(df.get("wall_velocity", "stress-free")) != "stress-free":
+            raise ValueError("forcing/diffusion wall_velocity must be stress-free on a Cartesian grid")
+        for key in ("nu_iso", "kappa_iso"):
+            v = float(df.get(key, 0.0))
+            if not (v >= 0.0) or v != v or v == float("inf"):
+                raise ValueError(f"DiffusionOptions: {key} must be a finite number >= 0")
+            setattr(h, key, v)
     if "bot-heat" in forcing:
         bh = forcing["bot-heat"] or {}
         _check_keys(bh, "forcing/bot-heat", {"flux", "depth", "thickness"})
@@ -306,6 +323,8 @@ def _check_supported(op: MeshBlockOptions) -> None:
             raise ValueError("const-gravity grav1 with a periodic x1 column is not implemented")
         if op.hydro.precision != "float64":
             raise ValueError("const-gravity: the gravity path runs in FP64 storage here")
+    if (op.hydro.nu_iso > 0.0 or op.hydro.kappa_iso > 0.0) and op.coord.nghost < 2:
+        raise ValueError("[Diffusion] Isotropic diffusion requires nghost >= 2.")
     if op.hydro.bot_heat_flux != 0.0 and op.hydro.grav1 == 0.0:
         raise ValueError("forcing/bot-heat is built on the gravity path here (needs grav1)")
     if op.intg.type != "rk3":
