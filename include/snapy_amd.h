@@ -103,8 +103,12 @@ int snapy_prim2cons(const snapy_grid *g, const snapy_physics *p, int dtype,
 /* Fused face sweep of direction dim (1 = x1, 2 = x2, 3 = x3): the weno5/cp5
  * face states of every interior face straight from w, then the Riemann flux,
  * written into the FluxRegister total `flux` (faces lo..hi+1 along dim,
- * interior cells across it).  face_pressure (x1 only, may be NULL) receives
- * the solver's face pressure.  At a reflecting (wall) boundary face the mass
+ * interior cells across it).  face_pressure (x1 only, may be NULL, FP64)
+ * receives the solver's face pressure.  w is the FP64 primitive state (the
+ * PrecisionPolicy keeps primitives FP64); dtype is the compute and flux
+ * precision.  The FP32 LMARS form reconstructs the pressure about the
+ * pressure g of each sweep line's first interior cell and stores a
+ * gauge-free momentum flux (g cancels in the divergence).  At a reflecting (wall) boundary face the mass
  * and energy rows are written as zero (FluxRegister boundary.prescribed,
  * src/layout/seam_register.cpp:184-220).
  * Replaces: call_recon (src/recon/recon_dispatch.hpp:13-16) twice per row
@@ -118,7 +122,8 @@ int snapy_sweep(int dim, const snapy_grid *g, const snapy_physics *p,
  * when p->flux_covariance is set (HydroImpl::_flux_covariance, Cartesian,
  * src/hydro/hydro_forward.cpp:58-231, added as add_cov :782-793), else the
  * plain sweep.  x1tab row 0 holds x1v (nc1 doubles).  Interior faces only;
- * wall planes prescribed as in snapy_sweep. */
+ * wall planes prescribed as in snapy_sweep; w FP64, dtype and the FP32
+ * gauge as in snapy_sweep. */
 int snapy_sweep_cov(int dim, const snapy_grid *g, const snapy_physics *p,
                     int dtype, const void *w, const double *x1tab, void *flux,
                     void *stream);
@@ -181,7 +186,9 @@ int snapy_hydro_ref_x1(const snapy_grid *g, const snapy_physics *p,
 
 /* Fused well-balanced x1 sweep: perturbation reconstruction about ref with
  * even wall ghosts, the reference face pressure/density restored with their
- * positivity fallbacks, then the Riemann flux into the x1 total (FP64).
+ * positivity fallbacks, then the Riemann flux into the x1 total, computed
+ * in FP64 from the FP64 w.  With dtype FP32 the stored x1 momentum row is
+ * the anomaly flux - psf_lo (the FP32 update restores psf_lo).
  * Replaces: WellBalancedX1::forward (src/recon/well_balanced_x1.cpp:16-154),
  * X1Reference::evaluate's face densities (src/recon/x1_reference.cpp:
  * 177-243) and the x1 call_riemann. */
@@ -256,6 +263,48 @@ int snapy_diffusion(const snapy_grid *g, const snapy_physics *p, int dtype,
                     const double *w, void *const totals[3], double *work,
                     double *du_src, double dt, const double *dt_dev,
                     void *stream);
+
+/* ---- the FP32 state form ----------------------------------------------
+ * Under dynamics/precision float32 the conserved state is an FP32 pair
+ * (u, c): the state is double(u) + double(c), c the compensation that keeps
+ * the RK3 accumulation exact to FP64 rounding (Kahan-compensated storage).
+ * The RK backup is the pair (u0, c0).  Primitives, the x1 reference, the
+ * implicit solve and the budgets stay FP64; face fluxes are FP32.  Replaces:
+ * PrecisionPolicy's float32 rows (src/mesh/precision_policy.hpp:31-71) with
+ * the compensated accumulation of the GPU form. */
+
+/* w (FP64) = prim(u + c) on every cell (ghosts included). */
+int snapy_cons2prim_f32(const snapy_grid *g, const snapy_physics *p,
+                        const float *u, const float *c, double *w,
+                        void *stream);
+/* (u, c) = the FP32 pair of cons(w) on every cell. */
+int snapy_prim2cons_f32(const snapy_grid *g, const snapy_physics *p,
+                        const double *w, float *u, float *c, void *stream);
+/* snapy_update on the pair: FP32 fluxes, FP64 divergence, du_src FP64 (may
+ * be NULL), (u, c) <- w0 (u0 + c0) + w1 (u + c) + w2 du in FP64. */
+int snapy_update_f32(const snapy_grid *g, const void *const flux[3],
+                     const float *u0, const float *c0, float *u, float *c,
+                     const double *du_src, double dt, const double *dt_dev,
+                     double w0, double w1, double w2, void *stream);
+/* snapy_update_gravity on the pair: FP32 fluxes whose x1 momentum row is
+ * the anomaly about the reference face pressure in ref (snapy_hydro_ref_x1);
+ * with du_out non-NULL the FP64 stage increment is written there instead of
+ * the combine (the input of snapy_vic_full_f32). */
+int snapy_update_gravity_f32(const snapy_grid *g, const snapy_physics *p,
+                             const void *const flux[3], const double *ref,
+                             const float *u0, const float *c0, float *u,
+                             float *c, double *du_out, const double *w,
+                             const double *x1tab, const double *du_src,
+                             double dt, const double *dt_dev, double w0,
+                             double w1, double w2, void *stream);
+/* snapy_vic_full finishing into the pair (u, c) from (u0, c0). */
+int snapy_vic_full_f32(const snapy_grid *g, const snapy_physics *p,
+                       const double *w, const double *du, const float *u0,
+                       const float *c0, float *u, float *c,
+                       const double *vtab, const double *x1tab, double t0,
+                       double t1, double dt, const double *dt_dev,
+                       double wstage, double w0, double w1, double w2,
+                       double *ws, int *status, void *stream);
 
 /* FP64 budget: out[v] = sum over interior cells of u[v] * volume, for
  * v < nvar, deterministic (fixed reduction tree).  out is device FP64 of

@@ -92,6 +92,17 @@ SIGNATURES = {
     "snapy_update_gravity": (_I, [_GP, _PP, _I, _P * 3, _P, _P, _P, _P, _P, _D, _P, _D, _D, _D, _P]),
     "snapy_vic_workspace": (ctypes.c_int64, [_GP]),
     "snapy_diffusion_workspace": (ctypes.c_int64, [_GP]),
+    "snapy_cons2prim_f32": (_I, [_GP, _PP, _P, _P, _P, _P]),
+    "snapy_prim2cons_f32": (_I, [_GP, _PP, _P, _P, _P, _P]),
+    "snapy_update_f32": (_I, [_GP, _P * 3, _P, _P, _P, _P, _P, _D, _P, _D, _D, _D, _P]),
+    "snapy_update_gravity_f32": (
+        _I,
+        [_GP, _PP, _P * 3, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _D, _D, _D, _P],
+    ),
+    "snapy_vic_full_f32": (
+        _I,
+        [_GP, _PP, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P],
+    ),
     "snapy_diffusion": (_I, [_GP, _PP, _I, _P, _P * 3, _P, _P, _D, _P, _P]),
     "snapy_vic_full": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P]),
     "snapy_vic_full_serial": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P]),

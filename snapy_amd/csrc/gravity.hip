@@ -152,10 +152,14 @@ struct WbArgs {
   int wall_lo, wall_hi;      // prescribed wall planes
 };
 
-template <typename T>
+// The x1 column path computes in FP64 (T); TF is the stored flux precision.
+// An FP32 flux keeps the x1 momentum row as the anomaly about the reference
+// face pressure psf (the FP64 update adds psf back), so the FP32 word holds
+// the small non-hydrostatic part of the flux.
+template <typename T, typename TF>
 __global__ void __launch_bounds__(256)
     sweep_x1_wb_kernel(T const* __restrict__ w, double const* __restrict__ ref,
-                       T* __restrict__ flx, T* __restrict__ fp, WbArgs a) {
+                       TF* __restrict__ flx, double* __restrict__ fp, WbArgs a) {
   Range3 const& r = a.r;
   int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t tot = int64_t(a.nf) * r.n2 * r.n3;
@@ -246,8 +250,9 @@ __global__ void __launch_bounds__(256)
     out[IDN] = T(0);
     out[IPR] = T(0);
   }
-  for (int v = 0; v < 5; ++v) flx[v * r.sv + row + i] = out[v];
-  if (fp) fp[row + i] = pface;
+  if constexpr (sizeof(TF) == 4) out[IVX] = out[IVX] - T(psf);
+  for (int v = 0; v < 5; ++v) flx[v * r.sv + row + i] = TF(out[v]);
+  if (fp) fp[row + i] = double(pface);
 }
 
 // ---------------------------------------------------------------------------
@@ -269,14 +274,20 @@ struct GUpdArgs {
 };
 
 // WRITE_DU: store the stage increment du (for the implicit solve to finish)
-// instead of combining it into u
-template <typename T, bool WRITE_DU>
+// instead of combining it into u.  TF is the flux precision; PAIR: the state
+// is the FP32 pair (u, c) (the FP32 state form), combined in FP64.  psf (may
+// be NULL) is the reference face pressure that an FP32 x1 momentum flux
+// carries as an anomaly.
+template <typename TF, bool WRITE_DU, bool PAIR>
 __global__ void __launch_bounds__(256)
-    update_gravity_kernel(T const* __restrict__ f1, T const* __restrict__ f2,
-                          T const* __restrict__ f3, T const* __restrict__ u0,
-                          T* __restrict__ u, T const* __restrict__ w,
+    update_gravity_kernel(TF const* __restrict__ f1, TF const* __restrict__ f2,
+                          TF const* __restrict__ f3, void const* __restrict__ u0v,
+                          float const* __restrict__ c0, void* __restrict__ uv,
+                          float* __restrict__ cc, double const* __restrict__ w,
                           double const* __restrict__ tab,
-                          T const* __restrict__ dsrc, GUpdArgs a) {
+                          double const* __restrict__ dsrc,
+                          double const* __restrict__ psf, GUpdArgs a) {
+  using T = double;
   Range3 const& r = a.r;
   int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
@@ -293,12 +304,19 @@ __global__ void __launch_bounds__(256)
   T mdt = T(-dt), tdt = T(dt);
   T w0 = T(a.w0), w1 = T(a.w1), w2 = T(a.w2), g1 = T(a.grav1);
   T rho = w[c], vx = w[r.sv + c];
+  // x1 face flux of row v at face q (the reference face pressure restored
+  // on an FP32 momentum anomaly)
+  auto F1 = [&](int v, int64_t q) {
+    T x = T(f1[v * r.sv + q]);
+    if (psf && v == IVX) x = x + psf[q];
+    return x;
+  };
   T du[5];
   for (int v = 0; v < 5; ++v) {
     int64_t o = v * r.sv + c;
-    T div = A1 * f1[o + 1] - A1 * f1[o];
-    if (f2) div = div + (A2 * f2[o + r.s2] - A2 * f2[o]);
-    if (f3) div = div + (A3 * f3[o + r.s3] - A3 * f3[o]);
+    T div = A1 * F1(v, c + 1) - A1 * F1(v, c);
+    if (f2) div = div + (A2 * T(f2[o + r.s2]) - A2 * T(f2[o]));
+    if (f3) div = div + (A3 * T(f3[o + r.s3]) - A3 * T(f3[o]));
     div = div / V;
     du[v] = mdt * div;
   }
@@ -311,13 +329,13 @@ __global__ void __launch_bounds__(256)
   // (src/forcing/layer_heat.cpp:78-85): du[IPR] += dt * flux / (dz * depth)
   if (ii < a.heat_depth) du[IPR] = du[IPR] + T(dt * a.heat_flux / a.heat_dzd);
   // gravity_energy_correction (hydro_forward.cpp:1049-1124), Cartesian
-  T const* m1 = f1 + row;  // the IDN row of F1 along this column
+  TF const* m1 = f1 + row;  // the IDN row of F1 along this column
   auto drho = [&](int m) {  // -dt * x1_face_difference / V at cell lo1 + m
     int q = r.lo1 + m;
-    return mdt * ((A1 * m1[q + 1] - A1 * m1[q]) / V);
+    return mdt * ((A1 * T(m1[q + 1]) - A1 * T(m1[q])) / V);
   };
-  T gw = (T(tab[T_GW_HI * r.nc1 + i]) * (A1 * m1[i + 1]) -
-          T(tab[T_GW_LO * r.nc1 + i]) * (A1 * m1[i])) /
+  T gw = (T(tab[T_GW_HI * r.nc1 + i]) * (A1 * T(m1[i + 1])) -
+          T(tab[T_GW_LO * r.nc1 + i]) * (A1 * T(m1[i]))) /
          V;
   T fgw = tdt * gw;
   int jj = (ii < 1 ? 1 : (ii > r.n1 - 2 ? r.n1 - 2 : ii)) - 1;
@@ -329,10 +347,21 @@ __global__ void __launch_bounds__(256)
   du[IPR] = du[IPR] + (fgw - ogw);
   for (int v = 0; v < 5; ++v) {
     int64_t o = v * r.sv + c;
-    if (WRITE_DU)
-      u[o] = du[v];
-    else
-      u[o] = w0 * u0[o] + w1 * u[o] + w2 * du[v];
+    if (WRITE_DU) {
+      static_cast<double*>(uv)[o] = du[v];
+    } else if (PAIR) {
+      float* uf = static_cast<float*>(uv);
+      float const* u0f = static_cast<float const*>(u0v);
+      double x = w0 * (double(u0f[o]) + double(c0[o])) +
+                 w1 * (double(uf[o]) + double(cc[o])) + w2 * du[v];
+      float h = float(x);
+      uf[o] = h;
+      cc[o] = float(x - double(h));
+    } else {
+      double* ud = static_cast<double*>(uv);
+      double const* u0d = static_cast<double const*>(u0v);
+      ud[o] = w0 * u0d[o] + w1 * ud[o] + w2 * du[v];
+    }
   }
 }
 
@@ -367,8 +396,7 @@ int snapy_sweep_x1_wb(const snapy_grid* g, const snapy_physics* p, int dtype,
   if (int e = check_grid(g)) return e;
   if (!p || !w || !ref || !flux) return fail(-1, "sweep_x1_wb: NULL argument");
   if (g->nx1 < 2) return fail(-9, "sweep_x1_wb: x1 is not resolved");
-  if (dtype != SNAPY_F64)
-    return fail(-10, "sweep_x1_wb: the total-state reference path is FP64");
+  if (dtype != SNAPY_F64 && dtype != SNAPY_F32) return fail(-10, "unknown dtype");
   WbArgs a;
   a.r = ranges(*g);
   a.nf = g->nx1 + 1;
@@ -382,11 +410,18 @@ int snapy_sweep_x1_wb(const snapy_grid* g, const snapy_physics* p, int dtype,
   a.wall_lo = a.refl_lo;
   a.wall_hi = a.refl_hi;
   int64_t tot = int64_t(a.nf) * g->nx2 * g->nx3;
-  hipLaunchKernelGGL((sweep_x1_wb_kernel<double>), dim3(blocks_for(tot, 256)),
-                     dim3(256), 0, as_stream(stream),
-                     static_cast<double const*>(w), ref,
-                     static_cast<double*>(flux),
-                     static_cast<double*>(face_pressure), a);
+  if (dtype == SNAPY_F64)
+    hipLaunchKernelGGL((sweep_x1_wb_kernel<double, double>),
+                       dim3(blocks_for(tot, 256)), dim3(256), 0,
+                       as_stream(stream), static_cast<double const*>(w), ref,
+                       static_cast<double*>(flux),
+                       static_cast<double*>(face_pressure), a);
+  else
+    hipLaunchKernelGGL((sweep_x1_wb_kernel<double, float>),
+                       dim3(blocks_for(tot, 256)), dim3(256), 0,
+                       as_stream(stream), static_cast<double const*>(w), ref,
+                       static_cast<float*>(flux),
+                       static_cast<double*>(face_pressure), a);
   return launch_status("sweep_x1_wb");
 }
 
@@ -416,17 +451,57 @@ int snapy_update_gravity(const snapy_grid* g, const snapy_physics* p,
   a.heat_flux = p->bot_heat_flux;
   a.heat_dzd = g->dx1 * double(p->bot_heat_depth);
   int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
-  auto kern = u0 ? update_gravity_kernel<double, false>
-                 : update_gravity_kernel<double, true>;
+  auto kern = u0 ? update_gravity_kernel<double, false, false>
+                 : update_gravity_kernel<double, true, false>;
   hipLaunchKernelGGL(kern,
                      dim3(blocks_for(tot, 256)), dim3(256), 0,
                      as_stream(stream), static_cast<double const*>(flux[0]),
                      static_cast<double const*>(flux[1]),
-                     static_cast<double const*>(flux[2]),
-                     static_cast<double const*>(u0), static_cast<double*>(u),
-                     static_cast<double const*>(w), x1tab,
-                     static_cast<double const*>(du_src), a);
+                     static_cast<double const*>(flux[2]), u0, nullptr, u,
+                     nullptr, static_cast<double const*>(w), x1tab,
+                     static_cast<double const*>(du_src), nullptr, a);
   return launch_status("update_gravity");
+}
+
+int snapy_update_gravity_f32(const snapy_grid* g, const snapy_physics* p,
+                             const void* const flux[3], const double* ref,
+                             const float* u0, const float* c0, float* u,
+                             float* c, double* du_out, const double* w,
+                             const double* x1tab, const double* du_src,
+                             double dt, const double* dt_dev, double w0,
+                             double w1, double w2, void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (!p || !flux || !flux[0] || !ref || !w || !x1tab)
+    return fail(-1, "update_gravity_f32: NULL argument");
+  if (!du_out && (!u0 || !c0 || !u || !c))
+    return fail(-1, "update_gravity_f32: NULL state");
+  if (g->nx1 < 3) return fail(-9, "update_gravity_f32: needs >= 3 x1 cells");
+  GUpdArgs a;
+  a.r = ranges(*g);
+  a.a1 = g->dx3 * g->dx2;
+  a.a2 = g->dx3 * g->dx1;
+  a.a3 = g->dx2 * g->dx1;
+  a.vol = g->dx3 * (g->dx2 * g->dx1);
+  a.mdt = -dt;
+  a.dt_dev = dt_dev;
+  a.w0 = w0;
+  a.w1 = w1;
+  a.w2 = w2;
+  a.grav1 = p->grav1;
+  a.heat_depth = p->bot_heat_flux != 0. ? p->bot_heat_depth : 0;
+  a.heat_flux = p->bot_heat_flux;
+  a.heat_dzd = g->dx1 * double(p->bot_heat_depth);
+  int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
+  double const* psf = ref + R_PSF_LO * ranges(*g).sv;
+  auto kern = du_out ? update_gravity_kernel<float, true, true>
+                     : update_gravity_kernel<float, false, true>;
+  void* uo = du_out ? static_cast<void*>(du_out) : static_cast<void*>(u);
+  hipLaunchKernelGGL(kern, dim3(blocks_for(tot, 256)), dim3(256), 0,
+                     as_stream(stream), static_cast<float const*>(flux[0]),
+                     static_cast<float const*>(flux[1]),
+                     static_cast<float const*>(flux[2]), u0, c0, uo, c, w,
+                     x1tab, du_src, psf, a);
+  return launch_status("update_gravity_f32");
 }
 
 }  // extern "C"

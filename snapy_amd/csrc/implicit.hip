@@ -795,9 +795,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 // global access is coalesced: the workspace is lane-contiguous, the column
 // state is staged through LDS and the outputs are transposed through LDS
 // before the combine.
+// PAIR: the state is the FP32 pair (u, c) of the FP32 state form, combined
+// in FP64; otherwise FP64 u0, u.
+template <bool PAIR>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
     vic_back_kernel(double const* __restrict__ w, double const* __restrict__ du,
-                    double const* __restrict__ u0, double* __restrict__ u,
+                    void const* __restrict__ u0v, float const* __restrict__ c0,
+                    void* __restrict__ uv, float* __restrict__ cc,
                     double const* __restrict__ vtab,
                     double const* __restrict__ xtab, double* __restrict__ ws,
                     int const* __restrict__ status, VicArgs a) {
@@ -1016,15 +1020,28 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
           int sl = h / kOB, i = b0 + (h - sl * kOB);
           if (i < n && s_live[sl]) {
             oo[t] = v * r.sv + s_row[sl] + i;
-            x0[t] = u0[oo[t]];
-            x1[t] = u[oo[t]];
+            if constexpr (PAIR) {
+              x0[t] = double(static_cast<float const*>(u0v)[oo[t]]) + double(c0[oo[t]]);
+              x1[t] = double(static_cast<float*>(uv)[oo[t]]) + double(cc[oo[t]]);
+            } else {
+              x0[t] = static_cast<double const*>(u0v)[oo[t]];
+              x1[t] = static_cast<double*>(uv)[oo[t]];
+            }
           }
         }
       }
 #pragma unroll
       for (int t = 0; t < CH; ++t)
-        if (oo[t] >= 0)
-          u[oo[t]] = a.w0 * x0[t] + a.w1 * x1[t] + a.w2 * s_out[f0 + threadIdx.x + 64 * t];
+        if (oo[t] >= 0) {
+          double x = a.w0 * x0[t] + a.w1 * x1[t] + a.w2 * s_out[f0 + threadIdx.x + 64 * t];
+          if constexpr (PAIR) {
+            float h = float(x);
+            static_cast<float*>(uv)[oo[t]] = h;
+            cc[oo[t]] = float(x - double(h));
+          } else {
+            static_cast<double*>(uv)[oo[t]] = x;
+          }
+        }
     }
   }
 }
@@ -1047,14 +1064,17 @@ int64_t snapy_vic_workspace(const snapy_grid* g) {
 }
 
 static int vic_launch(const snapy_grid* g, const snapy_physics* p,
-                      const double* w, const double* du, const double* u0,
-                      double* u, const double* vtab, const double* x1tab,
-                      double t0, double t1, double dt, const double* dt_dev,
-                      double wstage, double w0, double w1, double w2,
-                      double* ws, int* status, void* stream, bool serial) {
+                      const double* w, const double* du, const void* u0,
+                      const float* c0, void* u, float* c, const double* vtab,
+                      const double* x1tab, double t0, double t1, double dt,
+                      const double* dt_dev, double wstage, double w0,
+                      double w1, double w2, double* ws, int* status,
+                      void* stream, bool serial) {
   if (int e = check_grid(g)) return e;
   if (!p || !w || !du || !u0 || !u || !vtab || !x1tab || !ws || !status)
     return fail(-1, "vic_full: NULL argument");
+  bool pair = c0 != nullptr;
+  if (pair && (!c || serial)) return fail(-1, "vic_full: the FP32 pair needs c0 and c");
   if (g->nx1 < 5) return fail(-9, "vic_full: needs >= 5 x1 cells");
   VicArgs a;
   a.r = ranges(*g);
@@ -1074,16 +1094,22 @@ static int vic_launch(const snapy_grid* g, const snapy_physics* p,
   int64_t ncol = int64_t(g->nx2) * g->nx3;
   if (serial)
     hipLaunchKernelGGL(vic_full_kernel, dim3(blocks_for(ncol, 64)), dim3(64),
-                       0, as_stream(stream), w, du, u0, u, vtab, x1tab, ws,
-                       status, a);
+                       0, as_stream(stream), w, du,
+                       static_cast<double const*>(u0), static_cast<double*>(u),
+                       vtab, x1tab, ws, status, a);
   else {
     unsigned nb = blocks_for(ncol, kSlots);
     hipLaunchKernelGGL(vic_forward_kernel, dim3(nb), dim3(64), 0,
                        as_stream(stream), w, du, vtab, ws, status, a);
     if (int e = launch_status("vic_forward")) return e;
-    hipLaunchKernelGGL(vic_back_kernel, dim3(nb), dim3(64), 0,
-                       as_stream(stream), w, du, u0, u, vtab, x1tab, ws,
-                       status, a);
+    if (pair)
+      hipLaunchKernelGGL(vic_back_kernel<true>, dim3(nb), dim3(64), 0,
+                         as_stream(stream), w, du, u0, c0, u, c, vtab, x1tab,
+                         ws, status, a);
+    else
+      hipLaunchKernelGGL(vic_back_kernel<false>, dim3(nb), dim3(64), 0,
+                         as_stream(stream), w, du, u0, c0, u, c, vtab, x1tab,
+                         ws, status, a);
   }
   return launch_status("vic_full");
 }
@@ -1094,8 +1120,21 @@ int snapy_vic_full(const snapy_grid* g, const snapy_physics* p, const double* w,
                    double t1, double dt, const double* dt_dev, double wstage,
                    double w0, double w1, double w2, double* ws, int* status,
                    void* stream) {
-  return vic_launch(g, p, w, du, u0, u, vtab, x1tab, t0, t1, dt, dt_dev,
-                    wstage, w0, w1, w2, ws, status, stream, false);
+  return vic_launch(g, p, w, du, u0, nullptr, u, nullptr, vtab, x1tab, t0,
+                    t1, dt, dt_dev, wstage, w0, w1, w2, ws, status, stream,
+                    false);
+}
+
+int snapy_vic_full_f32(const snapy_grid* g, const snapy_physics* p,
+                       const double* w, const double* du, const float* u0,
+                       const float* c0, float* u, float* c,
+                       const double* vtab, const double* x1tab, double t0,
+                       double t1, double dt, const double* dt_dev,
+                       double wstage, double w0, double w1, double w2,
+                       double* ws, int* status, void* stream) {
+  if (!c0 || !c) return fail(-1, "vic_full_f32: NULL compensation");
+  return vic_launch(g, p, w, du, u0, c0, u, c, vtab, x1tab, t0, t1, dt,
+                    dt_dev, wstage, w0, w1, w2, ws, status, stream, false);
 }
 
 int snapy_vic_full_serial(const snapy_grid* g, const snapy_physics* p,
@@ -1105,8 +1144,9 @@ int snapy_vic_full_serial(const snapy_grid* g, const snapy_physics* p,
                           const double* dt_dev, double wstage, double w0,
                           double w1, double w2, double* ws, int* status,
                           void* stream) {
-  return vic_launch(g, p, w, du, u0, u, vtab, x1tab, t0, t1, dt, dt_dev,
-                    wstage, w0, w1, w2, ws, status, stream, true);
+  return vic_launch(g, p, w, du, u0, nullptr, u, nullptr, vtab, x1tab, t0,
+                    t1, dt, dt_dev, wstage, w0, w1, w2, ws, status, stream,
+                    true);
 }
 
 }  // extern "C"

@@ -215,22 +215,29 @@ __device__ __forceinline__ void hllc(const Prim5<T>& L, const Prim5<T>& R,
 // LMARS (lmars_impl, riemann_kernels.h:197-255), dry.  The specific internal
 // energy e/rho = p/((gamma-1) rho) enters the enthalpy
 // (src/riemann/riemann.cpp:262-268).
+//
+// With a gauge g the face states carry p - g (a constant along the sweep
+// line): the enthalpy and the sound speed take the total p' + g, the
+// momentum flux and the face pressure stay gauge-free (their constant g
+// cancels in the flux difference).  g = 0 is the plain solver exactly.
 template <typename T>
 __device__ __forceinline__ void lmars(const Prim5<T>& L, const Prim5<T>& R,
-                                      T gl, T gr, int n, T out[5], T* fp) {
+                                      T gl, T gr, int n, T out[5], T* fp,
+                                      T gauge = T(0)) {
   const int t1 = (n + 1) % 3, t2 = (n + 2) % 3;
-  T hl = (L.p / (gl - T(1))) / L.r;
-  T hr = (R.p / (gr - T(1))) / R.r;
+  T pl = L.p + gauge, pr = R.p + gauge;  // total pressures
+  T hl = (pl / (gl - T(1))) / L.r;
+  T hr = (pr / (gr - T(1))) / R.r;
   // wli = {rho, v_n, v_t1, v_t2, p}
   T lx = L.v[n], ly = L.v[t1], lz = L.v[t2];
   T rx = R.v[n], ry = R.v[t1], rz = R.v[t2];
   // hl += a + b is hl + (a + b)
-  hl = hl + (T(0.5) * (sqr(lx) + sqr(ly) + sqr(lz)) + L.p / L.r);
-  hr = hr + (T(0.5) * (sqr(rx) + sqr(ry) + sqr(rz)) + R.p / R.r);
+  hl = hl + (T(0.5) * (sqr(lx) + sqr(ly) + sqr(lz)) + pl / L.r);
+  hr = hr + (T(0.5) * (sqr(rx) + sqr(ry) + sqr(rz)) + pr / R.r);
 
   T rhobar = T(0.5) * (L.r + R.r);
   T gamma_bar = T(0.5) * (gl + gr);
-  T cbar = sqrt(T(0.5) * gamma_bar * (L.p + R.p) / rhobar);
+  T cbar = sqrt(T(0.5) * gamma_bar * (pl + pr) / rhobar);
   T pbar = T(0.5) * (L.p + R.p) + T(0.5) * (rhobar * cbar) * (lx - rx);
   if (fp) *fp = pbar;
   T ubar = T(0.5) * (lx + rx) + T(0.5) / (rhobar * cbar) * (L.p - R.p);

@@ -94,15 +94,63 @@ struct SweepArgs {
   int riemann;
   int shock;
   int wall_lo, wall_hi;  // the sweep's lower/upper boundary face is a wall
+  int gauge;             // FP32 LMARS: subtract the line's pressure gauge
 };
+
+// ---------------------------------------------------------------------------
+// the FP32 state form: the conserved state is an FP32 pair (u, c) whose sum
+// in FP64 is the state (c the compensation of the RK accumulation)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double pair_value(float hi, float lo) {
+  return double(hi) + double(lo);
+}
+
+__device__ __forceinline__ void pair_store(float* hi, float* lo, double x) {
+  float h = float(x);
+  *hi = h;
+  *lo = float(x - double(h));
+}
+
+__global__ void __launch_bounds__(256)
+    cons2prim_pair_kernel(float const* __restrict__ u, float const* __restrict__ c,
+                          double* __restrict__ w, int64_t n, int64_t sv,
+                          double gd) {
+  int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double uu[5], ww[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) uu[v] = pair_value(u[v * sv + q], c[v * sv + q]);
+  cons2prim5(uu, ww, gd);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) w[v * sv + q] = ww[v];
+}
+
+__global__ void __launch_bounds__(256)
+    prim2cons_pair_kernel(double const* __restrict__ w, float* __restrict__ u,
+                          float* __restrict__ c, int64_t n, int64_t sv,
+                          double gm1) {
+  int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double uu[5], ww[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) ww[v] = w[v * sv + q];
+  prim2cons5(ww, uu, gm1);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) pair_store(u + v * sv + q, c + v * sv + q, uu[v]);
+}
 
 // one thread per interior face.  DIR 1: the lane walks the faces along x1;
 // DIR 2/3: the lane walks x1 across the faces, so each of the six stencil
 // loads of a wavefront is one coalesced row.
+// The primitive state is FP64 (PrecisionPolicy keeps primitives FP64); T is
+// the compute and flux precision.  With a.gauge the pressure row enters the
+// reconstruction as p - g, g the pressure of the sweep line's first interior
+// cell, so an FP32 reconstruction keeps the small pressure differences; the
+// stored momentum flux is then gauge-free (g cancels in its divergence).
 template <typename T, int DIR>
 __global__ void __launch_bounds__(256)
-    sweep_kernel(T const* __restrict__ w, T* __restrict__ flx,
-                 T* __restrict__ fp, SweepArgs a) {
+    sweep_kernel(double const* __restrict__ w, T* __restrict__ flx,
+                 double* __restrict__ fp, SweepArgs a) {
   Range3 const& r = a.r;
   int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   int i, j, k;
@@ -134,14 +182,23 @@ __global__ void __launch_bounds__(256)
   }
   int64_t base = k * r.s3 + j * r.s2 + i;  // cell f: face f is its lower face
 
+  double gauge = 0.;
+  if (a.gauge) {
+    int64_t first = DIR == 1 ? k * r.s3 + j * r.s2 + r.lo1
+                  : DIR == 2 ? k * r.s3 + r.lo2 * r.s2 + i
+                             : r.lo3 * r.s3 + j * r.s2 + i;
+    gauge = w[IPR * r.sv + first];
+  }
   Prim5<T> L, R;
   T* lv[5] = {&L.r, &L.v[0], &L.v[1], &L.v[2], &L.p};
   T* rv[5] = {&R.r, &R.v[0], &R.v[1], &R.v[2], &R.p};
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    T const* p = w + v * r.sv + base;
-    T c0 = p[-3 * stride], c1 = p[-2 * stride], c2 = p[-stride], c3 = p[0],
-      c4 = p[stride], c5 = p[2 * stride];
+    double const* p = w + v * r.sv + base;
+    double gv = v == IPR ? gauge : 0.;
+    T c0 = T(p[-3 * stride] - gv), c1 = T(p[-2 * stride] - gv),
+      c2 = T(p[-stride] - gv), c3 = T(p[0] - gv), c4 = T(p[stride] - gv),
+      c5 = T(p[2 * stride] - gv);
     // left state: right edge of cell f-1 (window c0..c4); right state: left
     // edge of cell f (window c1..c5).  Density always weno5; velocity and
     // pressure cp5 unless shock (reconstruct.cpp:303-322).
@@ -157,7 +214,7 @@ __global__ void __launch_bounds__(256)
   T out[5];
   T pface;
   if (a.riemann == SNAPY_RIEMANN_LMARS)
-    lmars(L, R, g, g, DIR - 1, out, &pface);
+    lmars(L, R, g, g, DIR - 1, out, &pface, T(gauge));
   else
     hllc(L, R, g, g, DIR - 1, out, &pface);
   // boundary.prescribed at a wall plane (FluxRegister::apply_prescribed,
@@ -171,7 +228,7 @@ __global__ void __launch_bounds__(256)
   }
 #pragma unroll
   for (int v = 0; v < 5; ++v) flx[v * r.sv + base] = out[v];
-  if (DIR == 1 && fp != nullptr) fp[base] = pface;
+  if (DIR == 1 && fp != nullptr) fp[base] = double(pface);
 }
 
 // ---------------------------------------------------------------------------
@@ -228,6 +285,53 @@ __global__ void __launch_bounds__(256)
     T du = mdt * div;
     if (dsrc) du = du + dsrc[o];
     u[o] = w0 * u0[o] + w1 * u[o] + w2 * du;
+  }
+}
+
+// The FP32 state form of update_kernel: FP32 face fluxes, the divergence and
+// the combine in FP64 on the (u, c) pairs, du_src FP64.
+__global__ void __launch_bounds__(256)
+    update_pair_kernel(float const* __restrict__ f1, float const* __restrict__ f2,
+                       float const* __restrict__ f3, float const* __restrict__ u0,
+                       float const* __restrict__ c0, float* __restrict__ u,
+                       float* __restrict__ cc, double const* __restrict__ dsrc,
+                       UpdateArgs a) {
+  Range3 const& r = a.r;
+  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
+  if (idx >= tot) return;
+  int i = r.lo1 + int(idx % r.n1);
+  int64_t rest = idx / r.n1;
+  int j = r.lo2 + int(rest % r.n2);
+  int k = r.lo3 + int(rest / r.n2);
+  int64_t c = k * r.s3 + j * r.s2 + i;
+  double A1 = a.a1, A2 = a.a2, A3 = a.a3, V = a.vol;
+  double mdt = a.dt_dev ? -*a.dt_dev : a.mdt;
+  for (int v = 0; v < a.nvar; ++v) {
+    int64_t o = v * r.sv + c;
+    double div = 0.;
+    bool first = true;
+    if (f1) {
+      double d = A1 * double(f1[o + 1]) - A1 * double(f1[o]);
+      div = first ? d : div + d;
+      first = false;
+    }
+    if (f2) {
+      double d = A2 * double(f2[o + r.s2]) - A2 * double(f2[o]);
+      div = first ? d : div + d;
+      first = false;
+    }
+    if (f3) {
+      double d = A3 * double(f3[o + r.s3]) - A3 * double(f3[o]);
+      div = first ? d : div + d;
+      first = false;
+    }
+    div = div / V;
+    double du = mdt * div;
+    if (dsrc) du = du + dsrc[o];
+    double x = a.w0 * pair_value(u0[o], c0[o]) + a.w1 * pair_value(u[o], cc[o]) +
+               a.w2 * du;
+    pair_store(u + o, cc + o, x);
   }
 }
 
@@ -415,6 +519,60 @@ int snapy_prim2cons(const snapy_grid* g, const snapy_physics* p, int dtype,
   });
 }
 
+int snapy_cons2prim_f32(const snapy_grid* g, const snapy_physics* p,
+                        const float* u, const float* c, double* w,
+                        void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (int e = check_phys(p)) return e;
+  if (!u || !c || !w) return fail(-1, "cons2prim_f32: NULL buffer");
+  if (g->nvar != 5) return fail(-6, "cons2prim_f32: ideal gas needs nvar 5");
+  Range3 r = ranges(*g);
+  hipLaunchKernelGGL(cons2prim_pair_kernel, dim3(blocks_for(r.sv, 256)),
+                     dim3(256), 0, as_stream(stream), u, c, w, r.sv, r.sv,
+                     p->gammad);
+  return launch_status("cons2prim_f32");
+}
+
+int snapy_prim2cons_f32(const snapy_grid* g, const snapy_physics* p,
+                        const double* w, float* u, float* c, void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (int e = check_phys(p)) return e;
+  if (!u || !c || !w) return fail(-1, "prim2cons_f32: NULL buffer");
+  if (g->nvar != 5) return fail(-6, "prim2cons_f32: ideal gas needs nvar 5");
+  Range3 r = ranges(*g);
+  hipLaunchKernelGGL(prim2cons_pair_kernel, dim3(blocks_for(r.sv, 256)),
+                     dim3(256), 0, as_stream(stream), w, u, c, r.sv, r.sv,
+                     p->gammad - 1.);
+  return launch_status("prim2cons_f32");
+}
+
+int snapy_update_f32(const snapy_grid* g, const void* const flux[3],
+                     const float* u0, const float* c0, float* u, float* c,
+                     const double* du_src, double dt, const double* dt_dev,
+                     double w0, double w1, double w2, void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (!u0 || !c0 || !u || !c || !flux) return fail(-1, "update_f32: NULL buffer");
+  UpdateArgs a;
+  a.r = ranges(*g);
+  a.nvar = g->nvar;
+  a.a1 = g->dx3 * g->dx2;
+  a.a2 = g->dx3 * g->dx1;
+  a.a3 = g->dx2 * g->dx1;
+  a.vol = g->dx3 * (g->dx2 * g->dx1);
+  a.mdt = -dt;
+  a.dt_dev = dt_dev;
+  a.w0 = w0;
+  a.w1 = w1;
+  a.w2 = w2;
+  int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
+  hipLaunchKernelGGL(update_pair_kernel, dim3(blocks_for(tot, 256)), dim3(256),
+                     0, as_stream(stream), static_cast<float const*>(flux[0]),
+                     static_cast<float const*>(flux[1]),
+                     static_cast<float const*>(flux[2]), u0, c0, u, c, du_src,
+                     a);
+  return launch_status("update_f32");
+}
+
 int snapy_sweep(int dim, const snapy_grid* g, const snapy_physics* p,
                 int dtype, const void* w, void* flux, void* face_pressure,
                 void* stream) {
@@ -440,11 +598,14 @@ int snapy_sweep(int dim, const snapy_grid* g, const snapy_physics* p,
   int64_t tot = int64_t(a.nf);
   for (int d = 0; d < 3; ++d)
     if (d != dim - 1) tot *= nx[d];
+  // FP32 LMARS sweeps carry the line gauge (HLLC's clamps and shock
+  // switches need total pressures: its FP32 form reconstructs them as they are)
+  a.gauge = dtype == SNAPY_F32 && p->riemann == SNAPY_RIEMANN_LMARS;
   return dispatch(dtype, [&](auto z) {
     using T = decltype(z);
-    auto wp = static_cast<T const*>(w);
+    auto wp = static_cast<double const*>(w);
     auto fl = static_cast<T*>(flux);
-    auto fp = static_cast<T*>(face_pressure);
+    auto fp = static_cast<double*>(face_pressure);
     dim3 grid(blocks_for(tot, 256)), blk(256);
     hipStream_t s = as_stream(stream);
     if (dim == 1)

@@ -28,11 +28,16 @@ struct CovArgs {
   int wall_lo, wall_hi;
   int cov;    // add the covariance
   double s2;  // dx1^2 / 12 (face_moment2_x1, uniform x1)
+  int gauge;  // FP32 LMARS: subtract the line's pressure gauge
 };
 
+// The primitive state is FP64; T is the compute and flux precision.  With
+// a.gauge the pressure row is reconstructed as p - g, g the pressure of the
+// line's first interior cell along the sweep (the momentum flux stays
+// gauge-free; the enthalpy and the sound speed take p - g + g).
 template <typename T, int DIR>
 __global__ void __launch_bounds__(kLanes)
-    sweep23_kernel(T const* __restrict__ w, double const* __restrict__ x1v,
+    sweep23_kernel(double const* __restrict__ w, double const* __restrict__ x1v,
                    T* __restrict__ flx, CovArgs a) {
   Range3 const& r = a.r;
   __shared__ double sh_h[kLanes], sh_un[kLanes];
@@ -54,15 +59,21 @@ __global__ void __launch_bounds__(kLanes)
   bool live = i >= r.lo1 - 1 && i <= r.lo1 + r.n1;
   Prim5<T> L, R;
   double h = 0., un = 0.;
+  double gauge = 0.;
+  if (live && a.gauge)
+    gauge = w[IPR * r.sv + (DIR == 2 ? k * r.s3 + r.lo2 * r.s2 + i
+                                     : r.lo3 * r.s3 + j * r.s2 + i)];
   if (live) {
     int64_t base = k * r.s3 + j * r.s2 + i;
     T* lv[5] = {&L.r, &L.v[0], &L.v[1], &L.v[2], &L.p};
     T* rv[5] = {&R.r, &R.v[0], &R.v[1], &R.v[2], &R.p};
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      T const* p = w + v * r.sv + base;
-      T c0 = p[-3 * stride], c1 = p[-2 * stride], c2 = p[-stride], c3 = p[0],
-        c4 = p[stride], c5 = p[2 * stride];
+      double const* p = w + v * r.sv + base;
+      double gv = v == IPR ? gauge : 0.;
+      T c0 = T(p[-3 * stride] - gv), c1 = T(p[-2 * stride] - gv),
+        c2 = T(p[-stride] - gv), c3 = T(p[0] - gv), c4 = T(p[stride] - gv),
+        c5 = T(p[2 * stride] - gv);
       if (v == IDN || a.shock) {
         *lv[v] = weno5_plus(c0, c1, c2, c3, c4);
         *rv[v] = weno5_minus(c1, c2, c3, c4, c5);
@@ -74,7 +85,7 @@ __global__ void __launch_bounds__(kLanes)
     if (a.cov) {
       // wbar = 0.5 (wl + wr); h = (W->I(wbar) + p) / rho; u_n
       double rho = 0.5 * (double(L.r) + double(R.r));
-      double p = 0.5 * (double(L.p) + double(R.p));
+      double p = 0.5 * (double(L.p) + double(R.p)) + gauge;
       double enth = p / (a.gammad - 1.) + p;
       h = enth / rho;
       un = 0.5 * (double(L.v[DIR - 1]) + double(R.v[DIR - 1]));
@@ -89,7 +100,7 @@ __global__ void __launch_bounds__(kLanes)
   T out[5];
   T pface;
   if (a.riemann == SNAPY_RIEMANN_LMARS)
-    lmars(L, R, g, g, DIR - 1, out, &pface);
+    lmars(L, R, g, g, DIR - 1, out, &pface, T(gauge));
   else
     hllc(L, R, g, g, DIR - 1, out, &pface);
   if (a.cov) {
@@ -137,6 +148,7 @@ int snapy_sweep_cov(int dim, const snapy_grid* g, const snapy_physics* p,
   a.wall_hi = p->bc[2 * (dim - 1) + 1] == SNAPY_BC_REFLECTING;
   a.cov = p->flux_covariance;
   a.s2 = g->dx1 * g->dx1 / 12.;
+  a.gauge = dtype == SNAPY_F32 && p->riemann == SNAPY_RIEMANN_LMARS;
   unsigned nseg = blocks_for(g->nx1, kInner);
   int64_t ny = dim == 2 ? int64_t(a.nf) * g->nx3 : int64_t(g->nx2) * a.nf;
   dim3 grid(unsigned(ny), nseg);
@@ -149,7 +161,7 @@ int snapy_sweep_cov(int dim, const snapy_grid* g, const snapy_physics* p,
   } else if (dtype == SNAPY_F32) {
     auto k = dim == 2 ? sweep23_kernel<float, 2> : sweep23_kernel<float, 3>;
     hipLaunchKernelGGL(k, grid, dim3(kLanes), 0, s,
-                       static_cast<float const*>(w), x1tab,
+                       static_cast<double const*>(w), x1tab,
                        static_cast<float*>(flux), a);
   } else {
     return fail(-10, "unknown dtype");

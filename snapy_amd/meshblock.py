@@ -205,9 +205,9 @@ class MeshBlock:
         c = self.options.coord
         active = (c.nx1 > 1, c.nx2 > 1, c.nx3 > 1)
         shape = (c.nc3(), c.nc2(), c.nc1())
-        self._freg = FluxRegister(5, shape, active, self._dtype(), self.device)
+        self._freg = FluxRegister(5, shape, active, self.policy.row("flux").storage, self.device)
         if "outflow" in self.options.bcnames:
-            self._scratch_w = torch.empty(self.shape(), dtype=self._dtype(), device=self.device)
+            self._scratch_w = torch.empty(self.shape(), dtype=torch.float64, device=self.device)
         self._dt_dev = torch.zeros((), dtype=torch.float64, device=self.device)
         h = self.options.hydro
         self._ref = None
@@ -264,7 +264,8 @@ class MeshBlock:
         w = vars["hydro_w"]
         if tuple(w.shape) != self.shape():
             raise ValueError(f"initialize: hydro_w has shape {tuple(w.shape)}, expected {self.shape()}")
-        w = w.to(device=self.device, dtype=self._dtype()).contiguous()
+        # primitives are FP64 in both precision forms
+        w = w.to(device=self.device, dtype=torch.float64).contiguous()
         vars["hydro_w"] = w
         if "outflow" in self.options.bcnames:
             vars["boundary_reference_w"] = w.clone()
@@ -278,6 +279,20 @@ class MeshBlock:
             ),
             "apply_boundaries(prim)",
         )
+        if self.policy.fp32():
+            # the FP32 state form: the pair (u, c) of the FP64 conserved state
+            u = torch.empty(w.shape, dtype=torch.float32, device=self.device)
+            comp = torch.empty_like(u)
+            _capi.check(
+                lib.snapy_prim2cons_f32(
+                    ctypes.byref(self._grid), ctypes.byref(self._phys), _capi.ptr(w), _capi.ptr(u),
+                    _capi.ptr(comp), s,
+                ),
+                "prim2cons_f32",
+            )
+            vars["hydro_u"] = u
+            self._state.adopt(u, comp)
+            return vars, 0.0
         u = torch.empty_like(w)
         _capi.check(
             lib.snapy_prim2cons(
@@ -326,7 +341,9 @@ class MeshBlock:
         g, p = ctypes.byref(self._grid), ctypes.byref(self._phys)
         u = vars["hydro_u"]
         w = vars["hydro_w"]
-        dtc = _capi.dtype_code(u.dtype)
+        pair = self.policy.fp32()
+        # dtc: the compute and flux precision (the primitives are FP64)
+        dtc = _capi.dtype_code(self.policy.row("flux").storage)
         s = _capi.stream_of(self.device)
         freg = self._freg
         st = self.intg.stages[stage]
@@ -335,8 +352,16 @@ class MeshBlock:
         grav = self._ref is not None
         P = _capi.ptr
         calls = []
+        c, c0 = self._state.comp(), self._state.comp0()
+        bref = P(vars.get("boundary_reference_w"))
         # (1) primitives
-        calls.append(("cons2prim", lambda: lib.snapy_cons2prim(g, p, dtc, P(u), P(w), s)))
+        if pair:
+            # the FP32 pair form fills the ghosts on the FP64 primitives (a
+            # mirror, copy or radiating fill commutes with prim(u + c))
+            calls.append(("cons2prim", lambda: lib.snapy_cons2prim_f32(g, p, P(u), P(c), P(w), s)))
+            calls.append(("apply_boundaries", lambda: lib.snapy_apply_boundaries_prim(g, p, 0, P(w), bref, s)))
+        else:
+            calls.append(("cons2prim", lambda: lib.snapy_cons2prim(g, p, dtc, P(u), P(w), s)))
         # (2)-(4) fused reconstruction + Riemann per active direction; with
         # x1 gravity the x1 sweep is the well-balanced one about the
         # hydrostatic reference of every column
@@ -384,6 +409,49 @@ class MeshBlock:
         # (5)-(7) + combine: divergence, sources, implicit, u <- w0 u0 + w1 u + w2 du
         fl = (ctypes.c_void_p * 3)(*[P(freg.total(d)) for d in (1, 2, 3)])
         u0 = self._state.u0()
+        if pair:
+            ref = P(self._ref) if grav else None
+            if self._vic is not None:
+                vic = self._vic
+                calls.append(
+                    (
+                        "stage_increment",
+                        lambda: lib.snapy_update_gravity_f32(
+                            g, p, fl, ref, None, None, None, None, P(vic["du"]), P(w), P(self._x1tab), dsrc,
+                            0.0, dtp, 1.0, 0.0, 1.0, s,
+                        ),
+                    )
+                )
+                calls.append(
+                    (
+                        "vic_full",
+                        lambda: lib.snapy_vic_full_f32(
+                            g, p, P(w), P(vic["du"]), P(u0), P(c0), P(u), P(c), P(vic["tab"]), P(self._x1tab),
+                            vic["t0"], vic["t1"], 0.0, dtp, st.wght2(), st.wght0(), st.wght1(), st.wght2(),
+                            P(vic["ws"]), P(vic["status"]), s,
+                        ),
+                    )
+                )
+            elif grav:
+                calls.append(
+                    (
+                        "update_gravity",
+                        lambda: lib.snapy_update_gravity_f32(
+                            g, p, fl, ref, P(u0), P(c0), P(u), P(c), None, P(w), P(self._x1tab), dsrc, 0.0, dtp,
+                            st.wght0(), st.wght1(), st.wght2(), s,
+                        ),
+                    )
+                )
+            else:
+                calls.append(
+                    (
+                        "update",
+                        lambda: lib.snapy_update_f32(
+                            g, fl, P(u0), P(c0), P(u), P(c), dsrc, 0.0, dtp, st.wght0(), st.wght1(), st.wght2(), s
+                        ),
+                    )
+                )
+            return calls
         if self._vic is not None:
             vic = self._vic
             calls.append(
@@ -488,7 +556,12 @@ class MeshBlock:
         return 1
 
     def total_u(self, vars):
+        """The conserved state; in the FP32 state form the FP64 value of the
+        pair, u + c."""
         u = vars["hydro_u"] if isinstance(vars, dict) else vars
+        c = self._state.comp()
+        if c is not None and u.data_ptr() == self._state.u().data_ptr():
+            return u.double() + c.double()
         return u.clone()
 
     def budget(self, vars) -> torch.Tensor:
@@ -498,18 +571,22 @@ class MeshBlock:
         n = lib.snapy_budget_workspace(ctypes.byref(self._grid))
         ws = torch.empty(int(n), dtype=torch.float64, device=self.device)
         out = torch.empty(5, dtype=torch.float64, device=self.device)
-        _capi.check(
-            lib.snapy_budget(
-                ctypes.byref(self._grid),
-                _capi.dtype_code(u.dtype),
-                _capi.ptr(u),
-                _capi.ptr(ws),
-                _capi.ptr(out),
-                _capi.stream_of(self.device),
-            ),
-            "budget",
-        )
-        return out
+        parts = [u] + ([self._state.comp()] if self._state.comp() is not None else [])
+        total = None
+        for q in parts:
+            _capi.check(
+                lib.snapy_budget(
+                    ctypes.byref(self._grid),
+                    _capi.dtype_code(q.dtype),
+                    _capi.ptr(q),
+                    _capi.ptr(ws),
+                    _capi.ptr(out),
+                    _capi.stream_of(self.device),
+                ),
+                "budget",
+            )
+            total = out.clone() if total is None else total + out
+        return total
 
     def execution_report(self) -> dict:
         return self._executor.report()

@@ -321,8 +321,6 @@ def _check_supported(op: MeshBlockOptions) -> None:
             raise ValueError("const-gravity grav1 needs a resolved x1 (nx1 >= 6)")
         if "periodic" in op.bcnames[0:2]:
             raise ValueError("const-gravity grav1 with a periodic x1 column is not implemented")
-        if op.hydro.precision != "float64":
-            raise ValueError("const-gravity: the gravity path runs in FP64 storage here")
     if (op.hydro.nu_iso > 0.0 or op.hydro.kappa_iso > 0.0) and op.coord.nghost < 2:
         raise ValueError("[Diffusion] Isotropic diffusion requires nghost >= 2.")
     if op.hydro.bot_heat_flux != 0.0 and op.hydro.grav1 == 0.0:

@@ -20,20 +20,33 @@ import torch
 
 
 class StateStore:
-    """Fixed-address conserved state and its RK backup."""
+    """Fixed-address conserved state and its RK backup.  In the FP32 state
+    form the state is the FP32 pair (u, c), c the compensation of the RK
+    accumulation, and the backup is the pair (u0, c0)."""
 
     def __init__(self):
         self._u = None
         self._u0 = None
+        self._c = None
+        self._c0 = None
         self.generation = 0
         self._storage = {}
 
-    def adopt(self, u: torch.Tensor) -> None:
-        """Take u as hydro_u (its address is then fixed) and size u0."""
+    def adopt(self, u: torch.Tensor, comp: torch.Tensor | None = None) -> None:
+        """Take u as hydro_u (its address is then fixed) and size u0; with
+        comp, the pair (u, comp) and its backup (u0, c0)."""
         if self._u is None or self._u.data_ptr() != u.data_ptr():
             self._u = u
             self._u0 = torch.empty_like(u)
+            self._c = comp
+            self._c0 = torch.empty_like(comp) if comp is not None else None
             self.generation += 1
+
+    def comp(self):
+        return self._c
+
+    def comp0(self):
+        return self._c0
 
     def register_storage(self, name: str, t: torch.Tensor) -> None:
         self._storage[name] = t
@@ -47,9 +60,13 @@ class StateStore:
 
     def backup(self) -> None:
         self._u0.copy_(self._u)
+        if self._c is not None:
+            self._c0.copy_(self._c)
 
     def restore(self) -> None:
         self._u.copy_(self._u0)
+        if self._c is not None:
+            self._c.copy_(self._c0)
 
 
 class FluxRegister:
@@ -63,8 +80,9 @@ class FluxRegister:
         for d in range(3):
             if active[d]:
                 self._total[d] = torch.zeros((nvar,) + tuple(shape), dtype=dtype, device=device)
+        # the x1 face pressure is a reference quantity: FP64 in both forms
         self.face_pressure1 = (
-            torch.zeros(tuple(shape), dtype=dtype, device=device) if active[0] else None
+            torch.zeros(tuple(shape), dtype=torch.float64, device=device) if active[0] else None
         )
         self._w1 = 0.0
         self._w2 = 0.0
@@ -98,7 +116,9 @@ class PrecisionRow:
 
 class PrecisionPolicy:
     """dtype rows: the state, backup, flux narrow under float32; the
-    primitive and the reference stay FP64 (precision_policy.hpp:31-71)."""
+    primitive and the reference stay FP64 (precision_policy.hpp:31-71).  The
+    GPU form keeps a compensation pair for the narrowed state and FP64
+    implicit and budget rows."""
 
     QUANTITIES = ("hydro_state", "backup", "flux", "primitive", "implicit", "budget")
 
@@ -111,7 +131,7 @@ class PrecisionPolicy:
             "hydro_state": PrecisionRow(narrow, narrow),
             "backup": PrecisionRow(narrow, narrow),
             "flux": PrecisionRow(narrow, narrow),
-            "primitive": PrecisionRow(narrow, narrow),
+            "primitive": PrecisionRow(torch.float64, torch.float64),
             "implicit": PrecisionRow(torch.float64, torch.float64),
             "budget": PrecisionRow(torch.float64, torch.float64),
         }
