@@ -42,19 +42,20 @@ def parse():
 
 
 # --------------------------------------------------------------------------
-def make_block(deck, execution="hip-graph", **coord):
+def make_block(deck, execution="hip-graph", precision="float64", **coord):
     from snapy_amd import MeshBlock, MeshBlockOptions
 
     op = MeshBlockOptions.from_yaml(os.path.join(ROOT, "snapy_amd", "decks", deck))
     for k, v in coord.items():
         setattr(op.coord, k, v)
     op.intg.execution = execution
+    op.hydro.precision = precision
     block = MeshBlock(op)
     block.to("cuda")
     return block
 
 
-def workload(name):
+def workload(name, precision="float64", execution="hip-graph"):
     """(block, vars, cells, config, dtype, B_alg bytes per cell-update)."""
     from snapy_amd import problems
 
@@ -69,7 +70,7 @@ def workload(name):
             "form": "plain FP64",
         }
         return block, vars, c.nx1, cfg, "f64", 160.0
-    block = make_block("convection3d.yaml")
+    block = make_block("convection3d.yaml", execution=execution, precision=precision)
     vars, _ = block.initialize({"hydro_w": problems.convection3d(block).cuda()})
     c = block.options.coord
     cfg = {
@@ -77,8 +78,14 @@ def workload(name):
         "cells": [c.nx1, c.nx2, c.nx3],
         "method": "weno5+lmars+ssp-rk3, well-balanced x1 gravity, flux covariance, bot-heat, "
         "vertical implicit scheme 9 (full 5x5, Woodbury), periodic x2/x3, cfl 0.8",
-        "form": "plain FP64 (FP64 storage and compute, one block)",
     }
+    if precision == "float32":
+        cfg["form"] = ("FP32 state form: FP32 sweeps and face fluxes, compensated FP32 state pair, "
+                       "FP64 primitives, x1 reference, vertical implicit solve and budgets; one block")
+        # B_alg: FP32 state read u, u0, write u; Kahan compensation read and
+        # write; 3 FP32 face totals; the FP64 vertical-solve column I/O
+        return block, vars, c.nx1 * c.nx2 * c.nx3, cfg, "f32", 5 * (3 * 4 + 3 * 4) + 2 * 5 * 4 + 3 * 5 * 8
+    cfg["form"] = "plain FP64 (FP64 storage and compute, one block)"
     # B_alg (cell-update bytes): FP64 state read u, u0, write u; 3 face
     # totals; the FP64 vertical-solve column I/O (3 rows x 5 x 8)
     return block, vars, c.nx1 * c.nx2 * c.nx3, cfg, "f64", 5 * (3 * 8 + 3 * 8) + 3 * 5 * 8
@@ -169,7 +176,7 @@ def kernel_times(block, vars, reps=20):
     return out
 
 
-def kernel_bytes(block, name, sb=8):
+def kernel_bytes(block, name, sb=8, pair=False):
     """Algorithmic HBM bytes of one launch: each input value read once and
     each output written once (stencil re-reads and solver workspace
     excluded)."""
@@ -182,13 +189,63 @@ def kernel_bytes(block, name, sb=8):
         "hydro_ref_x1": 7 * ncell * 8,
         "sweep_x1_wb": (5 * sb + 4 * 8) * ncell + 6 * nint * sb,
         "stage_increment": (5 * nd + 2 + 5) * nint * sb,
-        "vic_full": 25 * nint * 8,
+        # du, w (FP64) read; u0, u read and u written: FP64, or FP32 pairs
+        "vic_full": (10 * 8 + 15 * (2 * 4 if pair else 8)) * nint,
         "update": (3 + nd) * 5 * nint * sb,
         "update_gravity": ((3 + nd) * 5 + 2) * nint * sb,
     }
     if name.startswith("sweep_x"):
         return table.get(name, 5 * ncell * sb + 5 * nint * sb)
     return table.get(name, 0)
+
+
+def timed_run(block, vars, steps, warmup):
+    """Warm up, then time exactly `steps` RK3 steps (device dt, captured
+    stages) between two synchronizations; returns seconds."""
+    import torch
+
+    for _ in range(warmup):
+        dt = block.device_time_step(vars)
+        for s in range(3):
+            block.forward(vars, dt, s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dt = block.device_time_step(vars)
+        for s in range(3):
+            block.forward(vars, dt, s)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def fp32_accuracy(name, nsteps=3):
+    """Relative L2 of the FP32 form against the FP64 GPU form (itself within
+    1e-12 of the FP64 oracle) after `nsteps` steps from the same state with
+    the same host dt, per row group; and the FP32 form's relative mass
+    drift."""
+    import numpy as np
+
+    b64, v64, _, _, _, _ = workload(name, "float64", execution="eager")
+    b32, v32, _, _, _, _ = workload(name, "float32", execution="eager")
+    m0 = float(b32.budget(v32)[0].item())
+    for _ in range(nsteps):
+        dt = b64.max_time_step(v64)
+        for s in range(3):
+            b64.forward(v64, dt, s)
+            b32.forward(v32, dt, s)
+    m1 = float(b32.budget(v32)[0].item())
+    sl = tuple(b64.part(exterior=False))
+    u64 = b64.total_u(v64)[sl].double()
+    u32 = b32.total_u(v32)[sl].double()
+    out = {}
+    for key, rows in (("density", [0]), ("momentum", [1, 2, 3]), ("energy", [4])):
+        d = (u32[rows] - u64[rows]).square().sum().sqrt()
+        n = u64[rows].square().sum().sqrt()
+        out[key] = float((d / n).item())
+    out["mass_drift"] = abs(m1 - m0) / abs(m0)
+    out["steps"] = nsteps
+    del b64, v64, b32, v32
+    return out
 
 
 def main():
@@ -206,30 +263,38 @@ def main():
     steps = args.steps or defaults[0]
     warmup = args.warmup if args.warmup is not None else defaults[1]
 
-    block, vars, cells, cfg, dtype, b_alg = workload(args.workload)
-    print(f"# workload {cfg['workload']} ready", file=sys.stderr, flush=True)
-    for _ in range(warmup):
-        dt = block.device_time_step(vars)
-        for s in range(3):
-            block.forward(vars, dt, s)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        dt = block.device_time_step(vars)
-        for s in range(3):
-            block.forward(vars, dt, s)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    # the FP64-plain form first (the comparison point), then the headline
+    # FP32 form of the same workload
+    plain = None
+    precision = "float32" if args.workload == "convection3d" else "float64"
+    if precision == "float32":
+        b64, v64, cells64, cfg64, _, balg64 = workload(args.workload, "float64")
+        print(f"# workload {cfg64['workload']} FP64-plain ready", file=sys.stderr, flush=True)
+        el64 = timed_run(b64, v64, steps, warmup)
+        plain = {
+            "value": cells64 * 3 * steps / el64,
+            "ms_per_step": el64 / steps * 1e3,
+            "stage_b_alg_per_cell": balg64,
+            "stage_frac": cells64 * 3 * steps / el64 * balg64 / (HBM_PEAK_GBS * 1e9),
+            "form": cfg64["form"],
+        }
+        print(f"# FP64-plain {steps} steps in {el64:.3f} s", file=sys.stderr, flush=True)
+        del b64, v64
+        torch.cuda.empty_cache()
+    block, vars, cells, cfg, dtype, b_alg = workload(args.workload, precision)
+    print(f"# workload {cfg['workload']} {precision} ready", file=sys.stderr, flush=True)
+    el = timed_run(block, vars, steps, warmup)
     ms_per_step = el / steps * 1e3
     value = cells * 3 * steps / el
     ok = bool(torch.isfinite(vars["hydro_u"]).all().item()) and block.check_redo(vars) == 0
     print(f"# timed {steps} steps in {el:.3f} s, state finite: {ok}", file=sys.stderr, flush=True)
+    execution = block.execution_report()
 
     roofline = None
     if not args.no_kernels:
         kt = kernel_times(block, vars)
         dom = max(kt, key=kt.get)
-        kb = kernel_bytes(block, dom)
+        kb = kernel_bytes(block, dom, sb=4 if dtype == "f32" else 8, pair=dtype == "f32")
         achieved = kb / (kt[dom] * 1e-3) / 1e9
         roofline = {
             "bound": "hbm",
@@ -244,8 +309,17 @@ def main():
             "stage_frac": value * b_alg / (HBM_PEAK_GBS * 1e9),
         }
     cpu = None if args.no_cpu else cpu_baseline(args.workload)
+    accuracy = None
+    if precision == "float32":
+        del block, vars
+        torch.cuda.empty_cache()
+        accuracy = fp32_accuracy(args.workload)
+        print(f"# FP32 vs FP64 {accuracy}", file=sys.stderr, flush=True)
+    # the metric as the baseline names it: the FP32 form is `value`, the
+    # FP64-plain form sits beside it
+    metric = "cell-updates/s per RK3 stage, FP32-batched vs FP64-plain; p99 request latency"
     line = {
-        "metric": "cell-updates/s per RK3 stage",
+        "metric": metric,
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -261,8 +335,12 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "state_finite": ok,
-        "execution": block.execution_report(),
+        "execution": execution,
     }
+    if plain is not None:
+        line["fp64_plain"] = plain
+        line["speedup_vs_fp64_plain"] = value / plain["value"]
+        line["fp32_vs_fp64_rel_l2"] = accuracy
     print(json.dumps(line), flush=True)
 
 
