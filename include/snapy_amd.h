@@ -39,6 +39,9 @@ enum snapy_riemann {
   SNAPY_RIEMANN_SHALLOW_ROE = 2
 };
 
+/* equation-of-state/type (src/eos/equation_of_state.cpp) */
+enum snapy_eos { SNAPY_EOS_IDEAL_GAS = 0, SNAPY_EOS_SHALLOW_WATER = 1 };
+
 /* src/bc/bc_func.cpp: boundary functions by name */
 enum snapy_bc {
   SNAPY_BC_NONE = 0, /* custom / no-op */
@@ -79,6 +82,9 @@ typedef struct snapy_physics {
   double nu_iso;         /* forcing/diffusion kinematic viscosity (m^2/s) */
   double kappa_iso;      /* forcing/diffusion thermal diffusivity (m^2/s) */
   double gas_r;          /* dry gas constant Rgas / weight (J/kg/K) */
+  int32_t eos;           /* enum snapy_eos; shallow water: nvar 4, rows
+                            (phi, phi v1, phi v2, phi v3), Riemann
+                            shallow-roe, dir xy (x1 and x2 sweeps) */
 } snapy_physics;
 
 /* ---- entry points ----------------------------------------------------- */

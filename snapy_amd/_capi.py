@@ -64,6 +64,7 @@ class Physics(ctypes.Structure):
         ("nu_iso", ctypes.c_double),
         ("kappa_iso", ctypes.c_double),
         ("gas_r", ctypes.c_double),
+        ("eos", ctypes.c_int32),
     ]
 
 

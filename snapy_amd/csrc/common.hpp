@@ -48,4 +48,15 @@ inline unsigned blocks_for(int64_t n, int tpb) {
   return static_cast<unsigned>((n + tpb - 1) / tpb);
 }
 
+// the shallow-water stage (swe.hip), dispatched from the generic entry
+// points when p->eos is SNAPY_EOS_SHALLOW_WATER
+int swe_cons2prim(const snapy_grid* g, int dtype, const void* u, void* w,
+                  void* stream);
+int swe_prim2cons(const snapy_grid* g, int dtype, const void* w, void* u,
+                  void* stream);
+int swe_sweep(int dim, const snapy_grid* g, const snapy_physics* p, int dtype,
+              const void* w, void* flux, void* stream);
+int swe_cfl(const snapy_grid* g, int dtype, const void* w, double scale,
+            double* dt_out, void* stream);
+
 }  // namespace snapy

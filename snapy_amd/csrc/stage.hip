@@ -490,6 +490,7 @@ int snapy_cons2prim(const snapy_grid* g, const snapy_physics* p, int dtype,
   if (int e = check_grid(g)) return e;
   if (int e = check_phys(p)) return e;
   if (!u || !w) return fail(-1, "cons2prim: NULL buffer");
+  if (p->eos == SNAPY_EOS_SHALLOW_WATER) return swe_cons2prim(g, dtype, u, w, stream);
   if (g->nvar != 5) return fail(-6, "cons2prim: ideal gas needs nvar 5");
   Range3 r = ranges(*g);
   return dispatch(dtype, [&](auto z) {
@@ -507,6 +508,7 @@ int snapy_prim2cons(const snapy_grid* g, const snapy_physics* p, int dtype,
   if (int e = check_grid(g)) return e;
   if (int e = check_phys(p)) return e;
   if (!u || !w) return fail(-1, "prim2cons: NULL buffer");
+  if (p->eos == SNAPY_EOS_SHALLOW_WATER) return swe_prim2cons(g, dtype, w, u, stream);
   if (g->nvar != 5) return fail(-6, "prim2cons: ideal gas needs nvar 5");
   Range3 r = ranges(*g);
   return dispatch(dtype, [&](auto z) {
@@ -580,9 +582,12 @@ int snapy_sweep(int dim, const snapy_grid* g, const snapy_physics* p,
   if (int e = check_phys(p)) return e;
   if (dim < 1 || dim > 3) return fail(-8, "sweep: dim must be 1, 2 or 3");
   if (!w || !flux) return fail(-1, "sweep: NULL buffer");
+  if (p->eos == SNAPY_EOS_SHALLOW_WATER || p->riemann == SNAPY_RIEMANN_SHALLOW_ROE) {
+    if (p->eos != SNAPY_EOS_SHALLOW_WATER || p->riemann != SNAPY_RIEMANN_SHALLOW_ROE)
+      return fail(-7, "sweep: shallow water pairs with shallow-roe");
+    return swe_sweep(dim, g, p, dtype, w, flux, stream);
+  }
   if (g->nvar != 5) return fail(-6, "sweep: the Euler sweep needs nvar 5");
-  if (p->riemann == SNAPY_RIEMANN_SHALLOW_ROE)
-    return fail(-7, "sweep: shallow-roe runs through the cubed-sphere path");
   int nx[3] = {g->nx1, g->nx2, g->nx3};
   if (nx[dim - 1] < 2) return fail(-9, "sweep: inactive direction");
   SweepArgs a;
@@ -670,6 +675,7 @@ int snapy_cfl(const snapy_grid* g, const snapy_physics* p, int dtype,
   if (int e = check_grid(g)) return e;
   if (int e = check_phys(p)) return e;
   if (!w || !dt_out) return fail(-1, "cfl: NULL buffer");
+  if (p->eos == SNAPY_EOS_SHALLOW_WATER) return swe_cfl(g, dtype, w, scale, dt_out, stream);
   Range3 r = ranges(*g);
   int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
   auto out = reinterpret_cast<unsigned long long*>(dt_out);

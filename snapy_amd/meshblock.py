@@ -92,8 +92,10 @@ class MeshBlock:
         self._dt_dev = None
         self._executor = StageExecutor(self, options.intg.execution)
         c = options.coord
+        # ideal gas: (rho, m1, m2, m3, E); shallow water: (phi, phi v1..v3)
+        self.nvar = 4 if options.hydro.eos.type == "shallow-water" else 5
         self._grid = _capi.Grid(
-            5,
+            self.nvar,
             c.nghost,
             c.nx1,
             c.nx2,
@@ -127,6 +129,7 @@ class MeshBlock:
             h.nu_iso,
             h.kappa_iso,
             h.eos.gas_constant_dry(),
+            1 if h.eos.type == "shallow-water" else 0,
         )
         self._build_coord()
 
@@ -166,7 +169,7 @@ class MeshBlock:
 
     def shape(self):
         c = self.options.coord
-        return (5, c.nc3(), c.nc2(), c.nc1())
+        return (self.nvar, c.nc3(), c.nc2(), c.nc1())
 
     def part(self, offset=(0, 0, 0), exterior=True, extend_x1=0, extend_x2=0, extend_x3=0):
         """The slices of the block's interior (exterior=False) or of the
@@ -205,7 +208,7 @@ class MeshBlock:
         c = self.options.coord
         active = (c.nx1 > 1, c.nx2 > 1, c.nx3 > 1)
         shape = (c.nc3(), c.nc2(), c.nc1())
-        self._freg = FluxRegister(5, shape, active, self.policy.row("flux").storage, self.device)
+        self._freg = FluxRegister(self.nvar, shape, active, self.policy.row("flux").storage, self.device)
         if "outflow" in self.options.bcnames:
             self._scratch_w = torch.empty(self.shape(), dtype=torch.float64, device=self.device)
         self._dt_dev = torch.zeros((), dtype=torch.float64, device=self.device)
@@ -375,7 +378,7 @@ class MeshBlock:
                 calls.append(
                     ("sweep_x1_wb", lambda t=t, fp=fp: lib.snapy_sweep_x1_wb(g, p, dtc, P(w), P(self._ref), P(t), P(fp), s))
                 )
-            elif d > 1 and self.options.coord.nx1 >= 3:
+            elif d > 1 and self.options.coord.nx1 >= 3 and self.nvar == 5:
                 # x2/x3 with the x1 flux covariance (on by default)
                 calls.append(
                     (f"sweep_x{d}", lambda d=d, t=t: lib.snapy_sweep_cov(d, g, p, dtc, P(w), P(self._x1tab), P(t), s))
@@ -570,7 +573,7 @@ class MeshBlock:
         u = vars["hydro_u"]
         n = lib.snapy_budget_workspace(ctypes.byref(self._grid))
         ws = torch.empty(int(n), dtype=torch.float64, device=self.device)
-        out = torch.empty(5, dtype=torch.float64, device=self.device)
+        out = torch.empty(self.nvar, dtype=torch.float64, device=self.device)
         parts = [u] + ([self._state.comp()] if self._state.comp() is not None else [])
         total = None
         for q in parts:

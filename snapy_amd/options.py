@@ -106,6 +106,7 @@ class HydroOptions:
     recon1: ReconstructOptions = field(default_factory=ReconstructOptions)
     recon23: ReconstructOptions = field(default_factory=ReconstructOptions)
     riemann: str = "hllc"
+    riemann_dir: str = "xy"  # riemann-solver/dir of shallow-roe
     grav1: float = 0.0
     bot_heat_flux: float = 0.0
     bot_heat_depth: int = 1
@@ -230,6 +231,7 @@ def _read_dynamics(d, forcing) -> HydroOptions:
     rs = d.get("riemann-solver") or {}
     _check_keys(rs, "dynamics/riemann-solver", {"type", "dir"})
     h.riemann = rs.get("type", h.riemann)
+    h.riemann_dir = str(rs.get("dir", h.riemann_dir))
     h.precision = d.get("precision", h.precision)
     h.flux_covariance = bool(d.get("flux-covariance", h.flux_covariance))
     _check_keys(forcing, "forcing", {"const-gravity", "bot-heat", "diffusion"})
@@ -302,8 +304,25 @@ def _check_supported(op: MeshBlockOptions) -> None:
     """Refuse deck features outside the implemented stage, loudly."""
     if op.coord.type != "cartesian":
         raise ValueError(f"geometry/type '{op.coord.type}': this stage implements cartesian")
-    if op.hydro.eos.type != "ideal-gas":
+    if op.hydro.eos.type not in ("ideal-gas", "shallow-water"):
         raise ValueError(f"equation-of-state type '{op.hydro.eos.type}' is not implemented")
+    if op.hydro.eos.type == "shallow-water":
+        # the Cartesian shallow-water stage (rows phi, phi v1, phi v2, phi v3)
+        if op.hydro.riemann != "shallow-roe" or op.hydro.riemann_dir != "xy":
+            raise ValueError("shallow-water pairs with riemann-solver shallow-roe, dir: xy, on a Cartesian grid")
+        if op.coord.nx3 > 1:
+            raise ValueError("shallow-water dir xy: x3 must be a single cell")
+        h = op.hydro
+        if h.grav1 != 0.0 or h.nu_iso > 0.0 or h.kappa_iso > 0.0 or h.bot_heat_flux != 0.0:
+            raise ValueError("shallow-water: forcing is not implemented for the shallow-water stage")
+        if op.intg.implicit_scheme != 0:
+            raise ValueError("shallow-water: no vertical implicit correction")
+        if h.precision != "float64":
+            raise ValueError("shallow-water: the FP32 state form is built for the ideal-gas stage")
+        if "outflow" in op.bcnames:
+            raise ValueError("shallow-water: outflow (radiating) boundaries are ideal-gas only")
+    elif op.hydro.riemann == "shallow-roe":
+        raise ValueError("riemann-solver shallow-roe needs equation-of-state shallow-water")
     if op.hydro.eos.limiter:
         raise ValueError("equation-of-state/limiter: true is not implemented")
     for r, name in ((op.hydro.recon1, "vertical"), (op.hydro.recon23, "horizontal")):
@@ -311,7 +330,7 @@ def _check_supported(op: MeshBlockOptions) -> None:
             raise ValueError(f"reconstruct/{name}/type '{r.type}': weno5 only")
         if r.scale:
             raise ValueError(f"reconstruct/{name}/scale: true is not implemented")
-    if op.hydro.riemann not in ("hllc", "lmars"):
+    if op.hydro.riemann not in ("hllc", "lmars", "shallow-roe"):
         raise ValueError(f"riemann-solver '{op.hydro.riemann}' is not implemented here")
     if op.hydro.precision not in ("float64", "float32"):
         raise ValueError("dynamics/precision must be float64 or float32")
