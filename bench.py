@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--workload", default="convection3d", choices=WORKLOADS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream leg")
+    ap.add_argument("--serving-requests", type=int, default=24)
     return ap.parse_args()
 
 
@@ -248,6 +250,25 @@ def fp32_accuracy(name, nsteps=3):
     return out
 
 
+def serving_leg(nreq, seed=2025):
+    """A bounded serving stream: Poisson arrivals (20/s, seeded) of the
+    Sod / density-current / shallow-water mix, continuous batching with
+    admission by HBM footprint (200 GB budget); p50/p99 request latency."""
+    from snapy_amd import serving
+
+    reqs = serving.sample_requests(nreq, seed=seed, rate=20.0)
+    srv = serving.Server(budget_bytes=200 * 10**9, max_slots=16, realtime=True)
+    t0 = time.perf_counter()
+    res = srv.run(reqs)
+    wall = time.perf_counter() - t0
+    out = serving.summarize(res, wall)
+    out["arrival_rate_per_s"] = 20.0
+    out["seed"] = seed
+    out["note"] = ("shallow-water requests run on a periodic Cartesian plane (the cubed-sphere "
+                   "panels are not built)")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -317,6 +338,11 @@ def main():
         print(f"# FP32 vs FP64 {accuracy}", file=sys.stderr, flush=True)
     # the metric as the baseline names it: the FP32 form is `value`, the
     # FP64-plain form sits beside it
+    serving_summary = None
+    if not args.no_serving and args.workload == "convection3d":
+        print("# serving stream", file=sys.stderr, flush=True)
+        serving_summary = serving_leg(args.serving_requests)
+        print(f"# serving {serving_summary}", file=sys.stderr, flush=True)
     metric = "cell-updates/s per RK3 stage, FP32-batched vs FP64-plain; p99 request latency"
     line = {
         "metric": metric,
@@ -337,6 +363,8 @@ def main():
         "state_finite": ok,
         "execution": execution,
     }
+    if serving_summary is not None:
+        line["serving"] = serving_summary
     if plain is not None:
         line["fp64_plain"] = plain
         line["speedup_vs_fp64_plain"] = value / plain["value"]

@@ -15,7 +15,9 @@ fallback: a block on a CPU device, or a missing kernel library, raises.
 from __future__ import annotations
 
 import ctypes
+import gc
 import math
+import weakref
 
 import torch
 
@@ -201,6 +203,30 @@ class MeshBlock:
 
     def _dtype(self):
         return self.policy.row("hydro_state").storage
+
+    def hbm_footprint(self) -> int:
+        """Device bytes the block holds once initialized: the state and its
+        RK backup (with the compensation pair in the FP32 form), the FP64
+        primitives, the face totals and face pressure, and the gravity,
+        diffusion and implicit buffers its options call for."""
+        c = self.options.coord
+        h = self.options.hydro
+        cells = c.nc1() * c.nc2() * c.nc3()
+        ncell = self.nvar * cells
+        fp32 = self.policy.fp32()
+        total = ncell * ((4 + 4) * 2 if fp32 else 8 * 2) + ncell * 8  # u, u0 (+c, c0); w
+        ndim = sum(1 for n in (c.nx1, c.nx2, c.nx3) if n > 1)
+        total += ndim * ncell * (4 if fp32 else 8) + (cells * 8 if c.nx1 > 1 else 0)
+        total += cells * 8 * 8  # x1 table (nc1 rows, bounded by cells)
+        if h.grav1 != 0.0:
+            total += 5 * cells * 8
+        if h.nu_iso > 0.0 or h.kappa_iso > 0.0:
+            total += 12 * cells * 8 + ncell * 8
+        if self.options.intg.implicit_scheme == 9:
+            lib = _capi.load()
+            total += int(lib.snapy_vic_workspace(ctypes.byref(self._grid))) * 8 + ncell * 8
+            total += c.nx2 * c.nx3 * 4
+        return int(total)
 
     def _ensure_buffers(self):
         if self._freg is not None:
@@ -594,6 +620,18 @@ class MeshBlock:
     def execution_report(self) -> dict:
         return self._executor.report()
 
+    def release(self, sync: bool = True) -> None:
+        """Free the block's graphs and device buffers (a serving loop calls
+        it when a request leaves).  sync=False: the caller has already
+        synchronized the stream the block ran on."""
+        if sync and self.device.type == "cuda":
+            torch.cuda.synchronize(self.device)
+        self._executor.release()
+        self._freg = None
+        self._ref = self._x1tab = self._vic = self._diff = None
+        self._scratch_w = self._dt_dev = None
+        self._state = StateStore()
+
 
 class StageExecutor:
     """Runs a block's stages eagerly or as captured hipGraphs
@@ -612,7 +650,10 @@ class StageExecutor:
             execution = "hip-graph"
         if execution not in ("auto", "eager", "hip-graph"):
             raise ValueError(f"integration/execution '{execution}'")
-        self.block = block
+        # a weak reference: a block and its executor form no cycle, so a block
+        # (and its captured graphs) is freed when its last reference goes,
+        # never by the cyclic collector in the middle of another capture
+        self._block = weakref.ref(block)
         self.mode = "eager" if execution == "auto" else execution
         self._graphs = {}
         self._warm = set()
@@ -627,6 +668,15 @@ class StageExecutor:
             vars["hydro_w"].data_ptr(),
             b._state.u0().data_ptr(),
         )
+
+    @property
+    def block(self) -> "MeshBlock":
+        return self._block()
+
+    def release(self) -> None:
+        """Drop the captured graphs (after the device has finished them)."""
+        self._graphs.clear()
+        self._warm.clear()
 
     def run_stage(self, vars, stage: int) -> None:
         if self.mode == "eager":
@@ -644,8 +694,16 @@ class StageExecutor:
                 return
             g = torch.cuda.CUDAGraph()
             torch.cuda.synchronize(self.block.device)
-            with torch.cuda.graph(g):
-                self.block._launch_stage(vars, stage)
+            # no collector run inside the capture: a finalizer that destroys a
+            # graph or a stream is not permitted while a stream captures
+            was = gc.isenabled()
+            gc.disable()
+            try:
+                with torch.cuda.graph(g):
+                    self.block._launch_stage(vars, stage)
+            finally:
+                if was:
+                    gc.enable()
             self._graphs[key] = g
             self._counts["captures"] += 1
         g.replay()

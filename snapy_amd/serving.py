@@ -1,0 +1,283 @@
+"""A request loop over resident blocks: continuous batching of simulation
+requests on one GPU.
+
+A request is (kind, parameters, seed, steps).  The server admits a waiting
+request once it has arrived, a slot is free and its HBM footprint
+(MeshBlock.hbm_footprint: state, RK backups, primitives, face totals,
+implicit workspace) fits in the free part of the budget.  Admitted requests
+keep their state on the device between steps; every round advances each
+active request by one RK3 step, each request on its own HIP stream (all at
+the default priority), so the stages of different requests overlap on the
+device while each request's own stages stay ordered.  A request leaves when
+its step count is reached, freeing its slot and memory for the next
+arrival; its latency is completion time minus arrival time.
+
+Request kinds (the serving mix of the benchmark):
+  * ``sod``              -- a 1D Sod-type tube with seeded left/right states;
+  * ``density_current``  -- the 2D density current (gravity, diffusion, the
+    vertical implicit solve) at a seeded grid and bubble amplitude, in the
+    FP64 or the FP32 state form;
+  * ``swe``              -- shallow water on a periodic Cartesian plane with
+    a seeded Gaussian hill.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import problems
+from .meshblock import MeshBlock
+from .options import CoordinateOptions, HydroOptions, IntegratorOptions, MeshBlockOptions
+
+RD, CP, GRAV = 287.0, 1004.0, 9.81
+GAMMA_DRY = CP / (CP - RD)
+WEIGHT_DRY = 8.31446261815324 / RD
+
+
+@dataclass
+class Request:
+    rid: int
+    kind: str
+    params: dict
+    seed: int
+    steps: int
+    arrival: float = 0.0  # seconds after the stream's start
+
+
+@dataclass
+class Result:
+    rid: int
+    kind: str
+    steps: int
+    cells: int
+    arrival: float
+    admitted: float
+    done: float
+    state: np.ndarray  # the final conserved state (FP64 value), interior
+    finite: bool
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+# ----------------------------------------------------------------------------
+# building blocks for requests
+# ----------------------------------------------------------------------------
+def _options(kind: str, p: dict) -> MeshBlockOptions:
+    op = MeshBlockOptions()
+    op.intg = IntegratorOptions(cfl=0.9, execution=p.get("execution", "hip-graph"))
+    if kind == "sod":
+        n = int(p["n"])
+        op.coord = CoordinateOptions("cartesian", 0.0, 1.0, -0.5, 0.5, -0.5, 0.5, n, 1, 1, 3)
+        op.hydro = HydroOptions(riemann="hllc")
+        op.hydro.eos.gammad = 1.4
+        op.bcnames = ["outflow"] * 6
+    elif kind == "density_current":
+        nx, nz = int(p["nx"]), int(p["nz"])
+        op.coord = CoordinateOptions("cartesian", 0.0, 6400.0, -25600.0, 25600.0, -50.0, 50.0, nz, nx, 1, 3)
+        op.hydro = HydroOptions(riemann="lmars", grav1=-GRAV, nu_iso=75.0, kappa_iso=75.0,
+                                precision=p.get("precision", "float64"))
+        op.hydro.eos.gammad = GAMMA_DRY
+        op.hydro.eos.weight = WEIGHT_DRY
+        op.hydro.recon1.shock = False
+        op.hydro.recon23.shock = False
+        op.intg.implicit_scheme = 9
+        op.bcnames = ["reflecting"] * 6
+    elif kind == "swe":
+        n = int(p["n"])
+        half = 3.0e6
+        op.coord = CoordinateOptions("cartesian", -half, half, -half, half, -0.5, 0.5, n, n, 1, 3)
+        op.hydro = HydroOptions(riemann="shallow-roe")
+        op.hydro.eos.type = "shallow-water"
+        op.hydro.recon1.shock = False
+        op.hydro.recon23.shock = False
+        op.bcnames = ["periodic"] * 6
+    else:
+        raise ValueError(f"unknown request kind '{kind}'")
+    return op
+
+
+def _initial_state(kind: str, block: MeshBlock, p: dict, seed: int) -> torch.Tensor:
+    if kind == "sod":
+        rng = np.random.default_rng(seed)
+        left = (rng.uniform(0.1, 1.0), 0.0, rng.uniform(0.1, 1.0))
+        right = (rng.uniform(0.1, 1.0), 0.0, rng.uniform(0.1, 1.0))
+        return problems.sod(block, left=left, right=right)
+    if kind == "density_current":
+        return problems.density_current(block, dT=float(p["dT"]), grav=GRAV)
+    # swe: phi = g (1000 + A exp(-(d/R)^2)), at rest
+    rng = np.random.default_rng(seed)
+    amp = rng.uniform(50.0, 200.0)
+    rad = rng.uniform(3.0e5, 1.0e6)
+    cx, cy = rng.uniform(-1.5e6, 1.5e6, size=2)
+    x = block.module("coord").buffer("x1v").numpy()[None, None, :]
+    y = block.module("coord").buffer("x2v").numpy()[None, :, None]
+    w = np.zeros(block.shape())
+    d2 = (x - cx) ** 2 + (y - cy) ** 2
+    w[0] = GRAV * (1000.0 + amp * np.exp(-d2 / (rad * rad)))
+    return torch.from_numpy(w)
+
+
+def make_block(req: Request, device="cuda") -> MeshBlock:
+    block = MeshBlock(_options(req.kind, req.params))
+    block.to(device)
+    return block
+
+
+def request_cells(req: Request) -> int:
+    p = req.params
+    if req.kind == "sod":
+        return int(p["n"])
+    if req.kind == "density_current":
+        return int(p["nx"]) * int(p["nz"])
+    return int(p["n"]) ** 2
+
+
+# ----------------------------------------------------------------------------
+# the mix and the arrival process
+# ----------------------------------------------------------------------------
+def sample_requests(n: int, seed: int = 7, rate: float = 20.0, kmin: int = 50, kmax: int = 500,
+                    sizes: str = "full") -> list[Request]:
+    """Poisson arrivals at `rate` per second; 40% Sod (N ~ logU[128, 8192]),
+    40% density current (nx in {256, 512, 1024}, nz in {32, 64, 128},
+    dT ~ U(-20, -5), FP64 or FP32 state form), 20% shallow water (N in
+    {32, 64, 96, 128, 192}); K ~ U[kmin, kmax] steps.  One generator for the
+    arrivals and the mix, one seed per request for its content.  sizes
+    'small' scales the grids down for a quick stream."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(n):
+        t += rng.exponential(1.0 / rate)
+        u = rng.random()
+        k = int(rng.integers(kmin, kmax + 1))
+        s = int(rng.integers(0, 2**31 - 1))
+        small = sizes == "small"
+        if u < 0.4:
+            n1 = int(round(math.exp(rng.uniform(math.log(128), math.log(8192)))))
+            if small:
+                n1 = max(32, n1 // 16)
+            out.append(Request(rid, "sod", {"n": n1}, s, k, t))
+        elif u < 0.8:
+            nx = int(rng.choice([256, 512, 1024]))
+            nz = int(rng.choice([32, 64, 128]))
+            if small:
+                nx, nz = nx // 8, max(8, nz // 4)
+            prec = "float32" if rng.random() < 0.5 else "float64"
+            out.append(Request(rid, "density_current", {"nx": nx, "nz": nz, "dT": float(rng.uniform(-20.0, -5.0)),
+                                                        "precision": prec}, s, k, t))
+        else:
+            n2 = int(rng.choice([32, 64, 96, 128, 192]))
+            if small:
+                n2 = max(16, n2 // 4)
+            out.append(Request(rid, "swe", {"n": n2}, s, k, t))
+    return out
+
+
+# ----------------------------------------------------------------------------
+# the server
+# ----------------------------------------------------------------------------
+@dataclass
+class _Slot:
+    req: Request
+    block: MeshBlock
+    vars: dict
+    stream: torch.cuda.Stream
+    bytes: int
+    admitted: float
+    done_steps: int = 0
+
+
+@dataclass
+class Server:
+    budget_bytes: int = 200 * 10**9
+    max_slots: int = 16
+    realtime: bool = True  # honour arrival times against the wall clock
+    log: list = field(default_factory=list)
+
+    def run(self, requests: list[Request]) -> list[Result]:
+        """Serve every request; return their results in completion order."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        streams = [torch.cuda.Stream() for _ in range(self.max_slots)]
+        free_streams = list(range(self.max_slots))
+        active: list[_Slot] = []
+        used = 0
+        results = []
+        t0 = time.perf_counter()
+
+        def now():
+            return time.perf_counter() - t0
+
+        while pending or active:
+            # admit the arrived requests that fit (first come, first served;
+            # a request too big for the free memory waits for departures)
+            while pending and free_streams and (not self.realtime or pending[0].arrival <= now()):
+                req = pending[0]
+                block = make_block(req)
+                need = block.hbm_footprint()
+                if need > self.budget_bytes:
+                    raise RuntimeError(f"request {req.rid} needs {need} bytes, above the budget")
+                if used + need > self.budget_bytes:
+                    break
+                pending.pop(0)
+                si = free_streams.pop(0)
+                with torch.cuda.stream(streams[si]):
+                    w = _initial_state(req.kind, block, req.params, req.seed)
+                    vars, _ = block.initialize({"hydro_w": w.to("cuda")})
+                used += need
+                active.append(_Slot(req, block, vars, streams[si], need, now()))
+                self.log.append(("admit", req.rid, now(), used))
+            if not active:
+                # nothing to run until the next arrival
+                if pending and self.realtime:
+                    time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
+                continue
+            # one RK3 step of every active request, each on its own stream
+            for sl in active:
+                with torch.cuda.stream(sl.stream):
+                    dt = sl.block.device_time_step(sl.vars)
+                    for s in range(3):
+                        sl.block.forward(sl.vars, dt, s)
+                sl.done_steps += 1
+            # departures
+            keep = []
+            for sl in active:
+                if sl.done_steps < sl.req.steps:
+                    keep.append(sl)
+                    continue
+                sl.stream.synchronize()
+                b = sl.block
+                sub = tuple(b.part(exterior=False))
+                u = b.total_u(sl.vars)[sub].double().cpu().numpy()
+                # without real-time arrivals every request is there at the start
+                arrival = sl.req.arrival if self.realtime else 0.0
+                results.append(Result(sl.req.rid, sl.req.kind, sl.req.steps, request_cells(sl.req),
+                                      arrival, sl.admitted, now(), u, bool(np.isfinite(u).all())))
+                b.release(sync=False)  # its stream was synchronized above
+                sl.vars.clear()
+                used -= sl.bytes
+                free_streams.append(streams.index(sl.stream))
+                self.log.append(("done", sl.req.rid, now(), used))
+            active = keep
+        torch.cuda.synchronize()
+        return results
+
+
+def summarize(results: list[Result], wall: float) -> dict:
+    lat = np.array([r.latency for r in results])
+    cell_updates = sum(r.cells * r.steps * 3 for r in results)
+    return {
+        "requests": len(results),
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "cell_updates_per_s": cell_updates / wall,
+        "wall_s": wall,
+        "all_finite": all(r.finite for r in results),
+        "mix": {k: sum(1 for r in results if r.kind == k) for k in ("sod", "density_current", "swe")},
+    }

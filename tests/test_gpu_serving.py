@@ -1,0 +1,46 @@
+"""The serving loop on the GPU: every request's result is the same alone and
+in the mix (bitwise: each request's kernels read only its own state), the
+admission keeps the resident footprint within the budget, and the latency
+summary is well formed."""
+
+import numpy as np
+import pytest
+
+from snapy_amd import serving
+
+pytestmark = pytest.mark.gpu
+
+
+def small_stream():
+    reqs = serving.sample_requests(8, seed=5, rate=50.0, kmin=6, kmax=14, sizes="small")
+    kinds = {r.kind for r in reqs}
+    assert kinds == {"sod", "density_current", "swe"}, kinds
+    return reqs
+
+
+def test_same_alone_and_in_the_mix():
+    reqs = small_stream()
+    print("serving mix of", [(r.kind, r.params) for r in reqs], flush=True)
+    mixed = {r.rid: r for r in serving.Server(realtime=False, max_slots=8).run(reqs)}
+    assert len(mixed) == len(reqs)
+    for req in reqs:
+        alone = serving.Server(realtime=False, max_slots=1).run([req])[0]
+        print("request", req.rid, req.kind, flush=True)
+        assert alone.finite
+        assert np.array_equal(alone.state, mixed[req.rid].state), req
+
+
+def test_admission_respects_the_budget():
+    reqs = small_stream()
+    blocks = [serving.make_block(r) for r in reqs]
+    need = [b.hbm_footprint() for b in blocks]
+    del blocks
+    budget = max(need) + min(need)  # room for the largest and one small one
+    srv = serving.Server(budget_bytes=budget, realtime=False, max_slots=8)
+    res = srv.run(reqs)
+    assert len(res) == len(reqs)
+    assert all(used <= budget for _, _, _, used in srv.log)
+    summary = serving.summarize(res, max(r.done for r in res))
+    print(summary, flush=True)
+    assert summary["all_finite"]
+    assert 0.0 <= summary["p50_latency_s"] <= summary["p99_latency_s"]
