@@ -66,38 +66,51 @@ __device__ __forceinline__ double rop_smooth(double const* rho,
   return (v[0] + 4. * v[1] + 6. * v[2] + 4. * v[3] + v[4]) / 16.;
 }
 
+// One wavefront per column: the column's density and pressure are staged in
+// LDS with coalesced loads, lane 0 runs the two face scans in their serial
+// order (hydro_ref_x1_scan_impl), and the lanes share the cell pass
+// (hydro_ref_x1_cell_impl), whose cells are independent given the faces.
+// LDS: 5 rows of nc1 doubles (rho, p, face lo, face hi, smoothed rho/p).
 __global__ void __launch_bounds__(64)
     hydro_ref_x1_kernel(double const* __restrict__ w, double* __restrict__ ref,
                         RefArgs a) {
+  extern __shared__ double smem[];
   Range3 const& r = a.r;
-  int64_t col = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t ncol = int64_t(r.nc2) * r.nc3;
-  if (col >= ncol) return;
+  int64_t col = blockIdx.x;
+  int const lane = threadIdx.x;
   int nc1 = r.nc1, iu = a.iu;
   int64_t base = col * nc1;
-  double const* rho = w + base;
-  double const* pr = w + 4 * r.sv + base;
-  double* lo_ = ref + R_PSF_LO * r.sv + base;
-  double* hi_ = ref + R_PSF_HI * r.sv + base;
-  double const g = a.grav, dx = a.dx1, mp = DBL_MIN;
-  // scan from the top anchor (hydro_ref_x1_scan_impl)
-  double rdt_top = pr[iu] / rho[iu];
-  double top = pr[iu] * exp(-g * 0.5 * dx / rdt_top);
-  double face = top;
-  for (int i = iu; i >= 0; --i) {
-    double dp = g * rho[i] * dx;
-    double lo = face + dp, hi = face;
-    lo_[i] = lo > mp ? lo : mp;
-    hi_[i] = hi > mp ? hi : mp;
-    face = lo;
+  double* s_rho = smem;
+  double* s_pr = smem + nc1;
+  double* s_lo = smem + 2 * nc1;
+  double* s_hi = smem + 3 * nc1;
+  double* s_rs = smem + 4 * nc1;
+  for (int i = lane; i < nc1; i += 64) {
+    s_rho[i] = w[base + i];
+    s_pr[i] = w[4 * r.sv + base + i];
   }
-  face = top;
-  for (int i = iu + 1; i < nc1; ++i) {
-    double dp = g * rho[i] * dx;
-    double lo = face, hi = lo - dp;
-    lo_[i] = lo > mp ? lo : mp;
-    hi_[i] = hi > mp ? hi : mp;
-    face = hi;
+  __syncthreads();
+  double const g = a.grav, dx = a.dx1, mp = DBL_MIN;
+  if (lane == 0) {
+    // scan from the top anchor (hydro_ref_x1_scan_impl)
+    double rdt_top = s_pr[iu] / s_rho[iu];
+    double top = s_pr[iu] * exp(-g * 0.5 * dx / rdt_top);
+    double face = top;
+    for (int i = iu; i >= 0; --i) {
+      double dp = g * s_rho[i] * dx;
+      double lo = face + dp, hi = face;
+      s_lo[i] = lo > mp ? lo : mp;
+      s_hi[i] = hi > mp ? hi : mp;
+      face = lo;
+    }
+    face = top;
+    for (int i = iu + 1; i < nc1; ++i) {
+      double dp = g * s_rho[i] * dx;
+      double lo = face, hi = lo - dp;
+      s_lo[i] = lo > mp ? lo : mp;
+      s_hi[i] = hi > mp ? hi : mp;
+      face = hi;
+    }
   }
   // cells (hydro_ref_x1_cell_impl), phys_in = phys_out = true
   int il = nc1 - 1 - iu;
@@ -105,14 +118,16 @@ __global__ void __launch_bounds__(64)
   bool thin = (iu - il + 1) < 5, fits = iu >= 4;
   int jlo = wc ? il : 0, jhi = wc ? iu : nc1 - 1;
   bool ext_lo = wc && il + 1 <= iu, ext_hi = wc && iu - 1 >= il;
-  auto facev = [&](int f) { return f < nc1 ? lo_[f] : hi_[nc1 - 1]; };
-  double rs_prev = 0.;
-  for (int i = 0; i < nc1; ++i) {
+  for (int i = lane; i < nc1; i += 64)
+    s_rs[i] = rop_smooth(s_rho, s_pr, i, jlo, jhi, ext_lo, ext_hi);
+  __syncthreads();
+  auto facev = [&](int f) { return f < nc1 ? s_lo[f] : s_hi[nc1 - 1]; };
+  for (int i = lane; i < nc1; i += 64) {
     bool wall_in = wc && i >= il && i < il + 2;
     bool wall_out = wc && i > iu - 2 && i <= iu;
     bool clamp_in = wall_in && fits && !thin;
     bool clamp_out = wall_out && fits && !thin;
-    double lo = lo_[i], hi = hi_[i];
+    double lo = s_lo[i], hi = s_hi[i];
     double cell = 0.5 * (lo + hi);
     double lower = lo < hi ? lo : hi, upper = lo > hi ? lo : hi;
     auto six = [&](int start, double const* wt, bool rev) {
@@ -127,9 +142,10 @@ __global__ void __launch_bounds__(64)
       int s = iu + 1 - 5;
       six(s, kW6E[4 - (i - s)], true);
     }
-    double rs = rop_smooth(rho, pr, i, jlo, jhi, ext_lo, ext_hi);
-    double rf = i > 0 ? 0.5 * (rs_prev + rs) : rs;
-    rs_prev = rs;
+    double rs = s_rs[i];
+    double rf = i > 0 ? 0.5 * (s_rs[i - 1] + rs) : rs;
+    ref[R_PSF_LO * r.sv + base + i] = lo;
+    ref[R_PSF_HI * r.sv + base + i] = hi;
     ref[R_PREF * r.sv + base + i] = cell;
     ref[R_DREF * r.sv + base + i] = cell * rs;
     ref[R_DSF * r.sv + base + i] = lo * rf;
@@ -385,8 +401,10 @@ int snapy_hydro_ref_x1(const snapy_grid* g, const snapy_physics* p,
   a.dx1 = g->dx1;
   a.wall_clamp = 1;
   int64_t ncol = int64_t(g->nc2) * g->nc3;
-  hipLaunchKernelGGL(hydro_ref_x1_kernel, dim3(blocks_for(ncol, 64)), dim3(64),
-                     0, as_stream(stream), w, ref, a);
+  size_t lds = size_t(5) * g->nc1 * sizeof(double);
+  if (lds > 64 * 1024) return fail(-9, "hydro_ref_x1: an x1 column above 1638 cells");
+  hipLaunchKernelGGL(hydro_ref_x1_kernel, dim3(unsigned(ncol)), dim3(64), lds,
+                     as_stream(stream), w, ref, a);
   return launch_status("hydro_ref_x1");
 }
 

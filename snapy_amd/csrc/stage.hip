@@ -362,10 +362,11 @@ __global__ void __launch_bounds__(256)
     cfl_kernel(T const* __restrict__ w, unsigned long long* out, Range3 r,
                double gd, double dx1, double dx2, double dx3, int act1,
                int act2, int act3, double adv1) {
-  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
   double best = 1.e9;
-  if (idx < tot) {
+  // grid-stride: a bounded grid keeps the final atomics few
+  for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < tot;
+       idx += int64_t(gridDim.x) * blockDim.x) {
     int i = r.lo1 + int(idx % r.n1);
     int64_t rest = idx / r.n1;
     int j = r.lo2 + int(rest % r.n2);
@@ -705,7 +706,9 @@ int snapy_cfl(const snapy_grid* g, const snapy_physics* p, int dtype,
   hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, s, out, bits);
   return dispatch(dtype, [&](auto z) {
     using T = decltype(z);
-    hipLaunchKernelGGL(cfl_kernel<T>, dim3(blocks_for(tot, 256)), dim3(256), 0,
+    unsigned nb = blocks_for(tot, 256);
+    nb = nb < 2048u ? nb : 2048u;
+    hipLaunchKernelGGL(cfl_kernel<T>, dim3(nb), dim3(256), 0,
                        s, static_cast<T const*>(w), out, r, p->gammad, g->dx1,
                        g->dx2, g->dx3, g->nc1 > 1, g->nc2 > 1, g->nc3 > 1,
                        ((p->implicit_scheme & 1) && (g->nc2 > 1 || g->nc3 > 1))
