@@ -272,8 +272,8 @@ def serving_leg(nreq, seed=2025):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU: every GPU box here has one; run with "
-                          "--gpus 1 (how the path shards over more GPUs is described in DESIGN.md)"}), flush=True)
+        print(json.dumps({"error": "this benchmark measures one GPU; run with "
+                          "--gpus 1 (the path shards by whole members or x2/x3 column groups; not measured here)"}), flush=True)
         sys.exit(2)
     import torch
 

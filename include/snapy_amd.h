@@ -321,6 +321,103 @@ int64_t snapy_budget_workspace(const snapy_grid *g);
 int snapy_budget(const snapy_grid *g, int dtype, const void *u, double *ws,
                  double *out, void *stream);
 
+/* ---- the cubed-sphere shallow-water stage -------------------------------
+ *
+ * Six gnomonic-equiangle panels per member (the reference's cubed-sphere
+ * layout with one block per panel, all on one device), shallow water with
+ * shallow-roe in the y-z plane.  A state tensor holds members * 6 blocks of
+ * (4, nc, nc, 1) FP64, x2 fastest, block b = member * 6 + panel, nc = n + 6.
+ * Faces of axis d are stored at the index of the cell above them (as the
+ * flux registers).  The descriptor's tables are grid-only, built once on the
+ * host (snapy_amd/cubed.py) and kept on the device:
+ *   geom         7 planes (nc x nc): cos_c, sin_c, volume, x_ov_rD, y_ov_rC,
+ *                width2, width3; then nc x (nc + 1): cos_f2, sin_f2, area2;
+ *                then (nc + 1) x nc: cos_f3, sin_f3, area3
+ *                (snapy_cs_geom_size doubles);
+ *   ghost_*      per ghost entry (4 sides x 3 layers x n per panel, plus the
+ *                corner cells): int32 [dst panel, dst cell, src panel,
+ *                src cells 0..3, npts], FP64 weights [3], FP64 matrices
+ *                [5][3][3] (the sender's frame matrix at each source cell,
+ *                then the receiver's), for conserved (_c) and primitive (_p)
+ *                variables;
+ *   seam_*       per seam face (12 n): int32 [owner panel, side, index,
+ *                other panel, side, index, sign, 0], FP64 [30]: the other
+ *                panel's to-geographic matrix, the owner's from-geographic
+ *                matrix, then (x, y, delta, C, D, face cosine) at the owner's
+ *                face and at the other panel's. */
+typedef struct snapy_cs_mesh {
+  int32_t n;            /* cells along a panel edge (nx2 = nx3) */
+  int32_t nghost;       /* 3 */
+  int32_t members;      /* independent six-panel meshes in the tensor */
+  int32_t interp_order; /* geometry/cells/interp_order: 2 or 4 */
+  int32_t shock;        /* reconstruct/horizontal shock */
+  int32_t pad;
+  double radius;        /* x1v of the shell (the x1 mass source) */
+  const double *geom;
+  const int32_t *ghost_idx;
+  const double *ghost_w;
+  const double *ghost_mat_c;
+  const double *ghost_mat_p;
+  int64_t nghost_entries;
+  const int32_t *seam_idx;
+  const double *seam_d;
+} snapy_cs_mesh;
+
+/* Doubles of the packed grid tables for n cells per edge. */
+int64_t snapy_cs_geom_size(int32_t n, int32_t nghost);
+/* Doubles of the edge-state buffer the sweeps fill and the seams read. */
+int64_t snapy_cs_edge_size(const snapy_cs_mesh *m);
+
+/* u -> w over every cell: v = m / phi, raised with the cell cosine.
+ * Replaces: ShallowWaterImpl::_cons2prim (src/eos/shallow_water.cpp:31-41). */
+int snapy_cs_cons2prim(const snapy_cs_mesh *m, const double *u, double *w,
+                       void *stream);
+/* w -> u: m = phi v, lowered with the cell cosine.
+ * Replaces: ShallowWaterImpl::_prim2cons + _mom_from_vel_
+ * (shallow_water.cpp:43-55, equation_of_state.cpp:290-295). */
+int snapy_cs_prim2cons(const snapy_cs_mesh *m, const double *w, double *u,
+                       void *stream);
+/* The panel-edge ghost fill of q (conserved != 0: hydro_u, else hydro_w),
+ * in place: each edge ghost from the neighbour panel's interior,
+ * interpolated along the edge in the geographic frame; corner cells copy the
+ * nearest interior cell.  Bit-exact with the host restatement.
+ * Replaces: MeshBlockImpl::exchange_state over the cubed-sphere layout
+ * (meshblock_exchange.cpp:103-117; cubed_sphere_layout.cpp:563-800;
+ * gnomonic_equiangle.cpp:286-305,537-605). */
+int snapy_cs_ghosts(const snapy_cs_mesh *m, int conserved, double *q,
+                    void *stream);
+/* The x2 (dim 2) or x3 (dim 3) sweep of every panel: weno5/cp5 face states,
+ * the face-local frame, shallow-roe (dir yz), covariant global rows into
+ * flux; the interior-side states at the panel's edge faces into edge.
+ * Replaces: reconstruct + riemann_solve + to_face_frame_ / to_global_frame_
+ * on a gnomonic grid (sweep.cpp:105-157, riemann.cpp:16-51,219-346). */
+int snapy_cs_sweep(const snapy_cs_mesh *m, int dim, const double *w,
+                   double *flux, double *edge, void *stream);
+/* The twelve panel seams: the owner's face flux from the two panels' face
+ * states (the face-state sync), and the other panel's copy through the
+ * global Cartesian frame, written into flux2 / flux3.
+ * Replaces: the cross-panel face-state exchange (hydro_forward.cpp:707-757)
+ * and reconcile_panel_seams (seam_register.cpp:35-186). */
+int snapy_cs_seams(const snapy_cs_mesh *m, double *flux2, double *flux3,
+                   const double *edge, void *stream);
+/* Divergence with the gnomonic areas and volume, the shallow-water metric
+ * sources and the x1 mass source, du = -dt div, and the RK combine
+ * u = w0 u0 + w1 u + w2 du on the interior cells.  dt_dev (per member) if
+ * non-null, else dt.
+ * Replaces: CoordinateImpl::divergence + GnomonicEquiangleImpl::forward
+ * (coordinate.cpp:618-662, gnomonic_equiangle.cpp:472-535) + harp's
+ * combine. */
+int snapy_cs_update(const snapy_cs_mesh *m, const double *u0, double *u,
+                    const double *w, const double *flux2, const double *flux3,
+                    double dt, const double *dt_dev, double w0, double w1,
+                    double w2, void *stream);
+/* dt_out[member] = scale * min over the member's six panels of
+ * width_d / (|v_d| + sqrt(phi)), d = x2, x3.
+ * Replaces: HydroImpl::max_time_step (hydro.cpp:372-397) on each block and
+ * the mesh minimum. */
+int snapy_cs_cfl(const snapy_cs_mesh *m, const double *w, double scale,
+                 double *dt_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

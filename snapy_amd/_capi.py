@@ -68,11 +68,34 @@ class Physics(ctypes.Structure):
     ]
 
 
+class CsMesh(ctypes.Structure):
+    """snapy_cs_mesh: the cubed-sphere descriptor and its device tables."""
+
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("nghost", ctypes.c_int32),
+        ("members", ctypes.c_int32),
+        ("interp_order", ctypes.c_int32),
+        ("shock", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("radius", ctypes.c_double),
+        ("geom", ctypes.c_void_p),
+        ("ghost_idx", ctypes.c_void_p),
+        ("ghost_w", ctypes.c_void_p),
+        ("ghost_mat_c", ctypes.c_void_p),
+        ("ghost_mat_p", ctypes.c_void_p),
+        ("nghost_entries", ctypes.c_int64),
+        ("seam_idx", ctypes.c_void_p),
+        ("seam_d", ctypes.c_void_p),
+    ]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
 _GP = ctypes.POINTER(Grid)
 _PP = ctypes.POINTER(Physics)
+_MP = ctypes.POINTER(CsMesh)
 
 # name -> (restype, argtypes); every name here must be declared in
 # include/snapy_amd.h (tests/test_capi_load.py checks both ways)
@@ -109,6 +132,15 @@ SIGNATURES = {
     "snapy_vic_full_serial": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P]),
     "snapy_budget_workspace": (ctypes.c_int64, [_GP]),
     "snapy_budget": (_I, [_GP, _I, _P, _P, _P, _P]),
+    "snapy_cs_geom_size": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32]),
+    "snapy_cs_edge_size": (ctypes.c_int64, [_MP]),
+    "snapy_cs_cons2prim": (_I, [_MP, _P, _P, _P]),
+    "snapy_cs_prim2cons": (_I, [_MP, _P, _P, _P]),
+    "snapy_cs_ghosts": (_I, [_MP, _I, _P, _P]),
+    "snapy_cs_sweep": (_I, [_MP, _I, _P, _P, _P, _P]),
+    "snapy_cs_seams": (_I, [_MP, _P, _P, _P, _P]),
+    "snapy_cs_update": (_I, [_MP, _P, _P, _P, _P, _P, _D, _P, _D, _D, _D, _P]),
+    "snapy_cs_cfl": (_I, [_MP, _P, _D, _P, _P]),
 }
 
 _lib = None

@@ -1,0 +1,644 @@
+// The cubed-sphere shallow-water stage: six gnomonic-equiangle panels per
+// member, every member and panel resident in one tensor (B = members * 6
+// blocks of (4, nc, nc, 1), block b = member * 6 + panel), each kernel one
+// launch over all of them.
+//
+//   * cons <-> prim with the metric raise / lower
+//     (ShallowWaterImpl::_cons2prim/_prim2cons, src/eos/shallow_water.cpp:
+//     31-55; coord_vec_raise/lower_impl, src/coord/coord_utils_impl.h:9-25);
+//   * the panel-edge ghost fill as a gather over precomputed index maps: per
+//     ghost cell the source cells on the neighbour panel, the along-edge
+//     weights and the pointwise frame matrices (CubedSphereLayoutImpl::
+//     serialize/deserialize, src/layout/cubed_sphere_layout.cpp:563-800;
+//     _interp_ghost, src/coord/gnomonic_equiangle.cpp:537-605;
+//     call_cs_matrix_vec, src/coord/coord_dispatch.cpp:66-90);
+//   * the x2 / x3 sweeps: weno5 / cp5, the face-local frame, shallow-roe in
+//     the y-z plane, the covariant global rows (prim2local2_/3_,
+//     flux2global2_/3_, src/coord/gnomonic_equiangle.cpp:342-470); each
+//     panel's interior-side states at its edge faces go to an edge buffer;
+//   * the seams: per seam face the owner's (lower panel's) flux from its own
+//     state and the neighbour's, through the frame matrices, and the other
+//     panel's copy through the global Cartesian frame (the face-state sync
+//     of HydroImpl::forward, src/hydro/hydro_forward.cpp:707-757;
+//     reconcile_panel_seams, src/layout/seam_register.cpp:35-186);
+//   * divergence with the gnomonic areas and volumes, the shallow-water
+//     metric sources, the x1 mass source, the RK3 combine
+//     (CoordinateImpl::divergence, src/coord/coordinate.cpp:618-662;
+//     GnomonicEquiangleImpl::forward / x1_mass_source_,
+//     src/coord/gnomonic_equiangle.cpp:472-535);
+//   * the CFL bound per member (HydroImpl::max_time_step,
+//     src/hydro/hydro.cpp:372-397).
+// Every expression keeps the reference's operation order (FP64, no
+// contraction), so the result is the CPU path's bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "common.hpp"
+#include "point.hpp"
+
+namespace snapy {
+namespace {
+
+constexpr int kRows = 4;  // phi, phi v1, phi v2, phi v3
+
+// cs_local_to_cart_vel / cs_cart_to_local_vel
+// (src/layout/cubed_sphere_constants.h:22-37)
+__constant__ int kL2CIdx[6][3] = {{1, 2, 0}, {2, 1, 0}, {1, 2, 0}, {0, 2, 1}, {2, 1, 0}, {0, 2, 1}};
+__constant__ int kL2CSgn[6][3] = {{1, 1, 1}, {1, -1, 1}, {-1, -1, 1}, {1, 1, -1}, {-1, 1, 1}, {-1, 1, 1}};
+__constant__ int kC2LIdx[6][3] = {{2, 0, 1}, {2, 1, 0}, {2, 0, 1}, {0, 2, 1}, {2, 1, 0}, {0, 2, 1}};
+__constant__ int kC2LSgn[6][3] = {{1, 1, 1}, {1, -1, 1}, {1, -1, -1}, {1, -1, 1}, {1, 1, -1}, {-1, 1, 1}};
+
+// the packed grid tables (snapy_cs_mesh.geom)
+struct Geom {
+  double const *cos_c, *sin_c, *vol, *xr, *yr, *w2, *w3;
+  double const *cos_f2, *sin_f2, *area2;  // nc x (nc + 1)
+  double const *cos_f3, *sin_f3, *area3;  // (nc + 1) x nc
+};
+
+struct Dims {
+  int n, ng, nc, members;
+  int64_t plane, block;  // nc * nc, kRows * plane
+};
+
+Geom geom_of(double const* g, int nc) {
+  int64_t p = int64_t(nc) * nc, f = int64_t(nc) * (nc + 1);
+  Geom o;
+  o.cos_c = g;
+  o.sin_c = g + p;
+  o.vol = g + 2 * p;
+  o.xr = g + 3 * p;
+  o.yr = g + 4 * p;
+  o.w2 = g + 5 * p;
+  o.w3 = g + 6 * p;
+  double const* h = g + 7 * p;
+  o.cos_f2 = h;
+  o.sin_f2 = h + f;
+  o.area2 = h + 2 * f;
+  o.cos_f3 = h + 3 * f;
+  o.sin_f3 = h + 4 * f;
+  o.area3 = h + 5 * f;
+  return o;
+}
+
+Dims dims_of(snapy_cs_mesh const& m) {
+  Dims d;
+  d.n = m.n;
+  d.ng = m.nghost;
+  d.nc = m.n + 2 * m.nghost;
+  d.members = m.members;
+  d.plane = int64_t(d.nc) * d.nc;
+  d.block = kRows * d.plane;
+  return d;
+}
+
+// ---------------------------------------------------------------------------
+// EOS
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+    cs_cons2prim_kernel(double const* __restrict__ u, double* __restrict__ w,
+                        double const* __restrict__ cos_c, Dims d, int64_t ncell) {
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= ncell) return;
+  int64_t b = t / d.plane, c = t % d.plane;
+  double const* ub = u + b * d.block + c;
+  double* wb = w + b * d.block + c;
+  double h = ub[0];
+  double v1 = ub[d.plane] / h, v2 = ub[2 * d.plane] / h, v3 = ub[3 * d.plane] / h;
+  double cth = cos_c[c];
+  double sth2 = 1. - cth * cth;
+  wb[0] = h;
+  wb[d.plane] = v1;
+  wb[2 * d.plane] = v2 / sth2 - v3 * cth / sth2;
+  wb[3 * d.plane] = -v2 * cth / sth2 + v3 / sth2;
+}
+
+__global__ void __launch_bounds__(256)
+    cs_prim2cons_kernel(double const* __restrict__ w, double* __restrict__ u,
+                        double const* __restrict__ cos_c, Dims d, int64_t ncell) {
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= ncell) return;
+  int64_t b = t / d.plane, c = t % d.plane;
+  double const* wb = w + b * d.block + c;
+  double* ub = u + b * d.block + c;
+  double h = wb[0];
+  double m1 = wb[d.plane] * h, m2 = wb[2 * d.plane] * h, m3 = wb[3 * d.plane] * h;
+  double cth = cos_c[c];
+  ub[0] = h;
+  ub[d.plane] = m1;
+  ub[2 * d.plane] = m2 + m3 * cth;
+  ub[3 * d.plane] = m3 + m2 * cth;
+}
+
+// ---------------------------------------------------------------------------
+// ghost fill
+// ---------------------------------------------------------------------------
+// out_i = m_i0 v0 + m_i1 v1 + m_i2 v2
+__device__ __forceinline__ void matvec(double const* m, double const* v, double* o) {
+  o[0] = m[0] * v[0] + m[1] * v[1] + m[2] * v[2];
+  o[1] = m[3] * v[0] + m[4] * v[1] + m[5] * v[2];
+  o[2] = m[6] * v[0] + m[7] * v[1] + m[8] * v[2];
+}
+
+// one thread per (member, ghost entry): npts 2 (linear), 4 (cubic) or 1 (a
+// corner cell: a plain copy of the nearest interior cell)
+__global__ void __launch_bounds__(256)
+    cs_ghost_kernel(double* __restrict__ u, int32_t const* __restrict__ idx,
+                    double const* __restrict__ wt, double const* __restrict__ mat,
+                    int64_t nent, Dims d) {
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nent * d.members) return;
+  int64_t e = t % nent;
+  int64_t m = t / nent;
+  int32_t const* ix = idx + 8 * e;
+  int npts = ix[7];
+  double* dst = u + (m * 6 + ix[0]) * d.block + ix[1];
+  double const* src = u + (m * 6 + ix[2]) * d.block;
+  if (npts == 1) {
+    for (int v = 0; v < kRows; ++v) dst[v * d.plane] = src[v * d.plane + ix[3]];
+    return;
+  }
+  double const* M = mat + 45 * e;
+  // each source point: phi, and its momentum to the geographic frame
+  double b[4][kRows];
+  for (int q = 0; q < npts; ++q) {
+    double const* s = src + ix[3 + q];
+    double v[3] = {s[d.plane], s[2 * d.plane], s[3 * d.plane]};
+    b[q][0] = s[0];
+    matvec(M + 9 * q, v, &b[q][1]);
+  }
+  double g[kRows];
+  double const* wq = wt + 3 * e;
+  if (npts == 2) {
+    double x = wq[0];
+    for (int v = 0; v < kRows; ++v) g[v] = b[0][v] + x * (b[1][v] - b[0][v]);
+  } else {
+    double c0 = wq[0], c2 = wq[1], c3 = wq[2];
+    for (int v = 0; v < kRows; ++v)
+      g[v] = b[1][v] + c0 * (b[0][v] - b[1][v]) + c2 * (b[2][v] - b[1][v]) +
+             c3 * (b[3][v] - b[1][v]);
+  }
+  double o[3];
+  matvec(M + 36, &g[1], o);
+  dst[0] = g[0];
+  dst[d.plane] = o[0];
+  dst[2 * d.plane] = o[1];
+  dst[3 * d.plane] = o[2];
+}
+
+// ---------------------------------------------------------------------------
+// the face frame and the solver
+// ---------------------------------------------------------------------------
+// shallow_roe_impl (riemann_kernels.h:447-495) with shallow_axes dir yz
+// (:44-47): normal row ivx, tangential ivy, the radial row ivz gets no flux
+__device__ __forceinline__ void roe_yz(double const* wl, double const* wr, int ivx,
+                                       int ivy, int ivz, double* flx) {
+  double sqrtdl = sqrt(wl[0]);
+  double sqrtdr = sqrt(wr[0]);
+  double isdlpdr = 1. / (sqrtdl + sqrtdr);
+  double ubar = (wl[ivx] * sqrtdl + wr[ivx] * sqrtdr) * isdlpdr;
+  double vbar = (wl[ivy] * sqrtdl + wr[ivy] * sqrtdr) * isdlpdr;
+  double cbar = sqrt(0.5 * (wl[0] + wr[0]));
+  double del0 = wr[0] - wl[0];
+  double delx = wr[ivx] - wl[ivx];
+  double dely = wr[ivy] - wl[ivy];
+  double hbar = sqrt(wl[0] * wr[0]);
+  double a1 = 0.5 * (cbar * del0 - hbar * delx) / cbar;
+  double a2 = hbar * dely;
+  double a3 = 0.5 * (cbar * del0 + hbar * delx) / cbar;
+  double wave0[4] = {0, 0, 0, 0}, wave1[4] = {0, 0, 0, 0}, wave2[4] = {0, 0, 0, 0};
+  wave0[0] = a1;
+  wave0[ivx] = a1 * (ubar - cbar);
+  wave0[ivy] = a1 * vbar;
+  wave1[ivy] = a2;
+  wave2[0] = a3;
+  wave2[ivx] = a3 * (ubar + cbar);
+  wave2[ivy] = a3 * vbar;
+  double speed0 = fabs(ubar - cbar);
+  double speed1 = fabs(ubar);
+  double speed2 = fabs(ubar + cbar);
+  flx[0] = 0.5 * (wl[0] * wl[ivx] + wr[0] * wr[ivx]);
+  flx[ivx] = 0.5 * (wl[0] * wl[ivx] * wl[ivx] + 0.5 * wl[0] * wl[0] +
+                    wr[0] * wr[ivx] * wr[ivx] + 0.5 * wr[0] * wr[0]);
+  flx[ivy] = 0.5 * (wl[0] * wl[ivx] * wl[ivy] + wr[0] * wr[ivx] * wr[ivy]);
+  flx[ivz] = 0.;
+  for (int v = 0; v < 4; ++v)
+    flx[v] = flx[v] - 0.5 * (speed0 * wave0[v] + speed1 * wave1[v] + speed2 * wave2[v]);
+}
+
+// the solve at one face of axis AX (2 = x2, 3 = x3) with the face's metric
+// cosine c and sine s: prim2local, the solver, flux2global
+template <int AX>
+__device__ __forceinline__ void face_flux(double* wl, double* wr, double c, double s,
+                                          double* f) {
+  double gi = 1.0 / (s * s);
+  if (AX == 2) {
+    double t22 = 1.0 / sqrt(gi);
+    for (double* w : {wl, wr}) {
+      double uu2 = w[2], uu3 = w[3];
+      w[3] = c * uu2 + 1.0 * uu3;
+      w[2] = t22 * uu2;
+    }
+    roe_yz(wl, wr, 2, 3, 1, f);
+    double sg = sqrt(gi);
+    double t32 = -sg * c / 1.0;
+    double txy = f[2], txz = f[3];
+    f[3] = t32 * txy + 1.0 * txz;
+    f[2] = sg * txy;
+  } else {
+    double t33 = 1.0 / sqrt(gi);
+    for (double* w : {wl, wr}) {
+      double uu2 = w[2], uu3 = w[3];
+      w[2] = 1.0 * uu2 + c * uu3;
+      w[3] = t33 * uu3;
+    }
+    roe_yz(wl, wr, 3, 2, 1, f);
+    double sg = sqrt(gi);
+    double t23 = -c / 1.0 * sg;
+    double txy = f[2], txz = f[3];
+    f[2] = 1.0 * txy + t23 * txz;
+    f[3] = sg * txz;
+  }
+  double ty = f[2], tz = f[3];
+  f[2] = ty + tz * c;
+  f[3] = tz + ty * c;
+}
+
+// one thread per interior face (x2: faces ng..ng+n of the interior rows; x3:
+// the transpose); edge faces also store the interior-side state
+template <int AX>
+__global__ void __launch_bounds__(256)
+    cs_sweep_kernel(double const* __restrict__ w, double* __restrict__ flx,
+                    double* __restrict__ edge, Geom gm, Dims d, int shock) {
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int nf = d.n + 1;
+  int64_t per = int64_t(d.n) * nf;
+  if (t >= per * 6 * d.members) return;
+  int64_t b = t / per;
+  int r = int(t % per);
+  int f = r % nf, a = r / nf;  // face along the sweep, interior line across
+  int k, j;
+  int64_t stride;
+  if (AX == 2) {
+    k = d.ng + a;
+    j = d.ng + f;
+    stride = 1;
+  } else {
+    k = d.ng + f;
+    j = d.ng + a;
+    stride = d.nc;
+  }
+  int64_t c = int64_t(k) * d.nc + j;
+  double const* wb = w + b * d.block + c;
+  double wl[kRows], wr[kRows];
+  for (int v = 0; v < kRows; ++v) {
+    double const* p = wb + v * d.plane;
+    double c0 = p[-3 * stride], c1 = p[-2 * stride], c2 = p[-stride], c3 = p[0],
+           c4 = p[stride], c5 = p[2 * stride];
+    if (v == 0 || shock) {
+      wl[v] = weno5_plus(c0, c1, c2, c3, c4);
+      wr[v] = weno5_minus(c1, c2, c3, c4, c5);
+    } else {
+      wl[v] = cp5_plus(c0, c1, c2, c3, c4);
+      wr[v] = cp5_minus(c1, c2, c3, c4, c5);
+    }
+  }
+  if (f == 0 || f == d.n) {
+    // sides L/R (x2) or B/T (x3): the state on the panel's own side
+    int side = (AX == 2 ? 0 : 2) + (f == d.n ? 1 : 0);
+    double* e = edge + ((b * 4 + side) * int64_t(d.n) + a) * kRows;
+    double const* own = f == d.n ? wl : wr;
+    for (int v = 0; v < kRows; ++v) e[v] = own[v];
+  }
+  double cf, sf;
+  if (AX == 2) {
+    int64_t q = int64_t(k) * (d.nc + 1) + j;
+    cf = gm.cos_f2[q];
+    sf = gm.sin_f2[q];
+  } else {
+    int64_t q = int64_t(k) * d.nc + j;
+    cf = gm.cos_f3[q];
+    sf = gm.sin_f3[q];
+  }
+  double out[kRows];
+  face_flux<AX>(wl, wr, cf, sf, out);
+  double* fb = flx + b * d.block + c;
+  for (int v = 0; v < kRows; ++v) fb[v * d.plane] = out[v];
+}
+
+// ---------------------------------------------------------------------------
+// seams
+// ---------------------------------------------------------------------------
+// one thread per (member, seam face): the owner's flux from its own state and
+// the neighbour's (through the two frame matrices), then the neighbour's copy
+// through the global Cartesian frame
+__global__ void __launch_bounds__(256)
+    cs_seam_kernel(double* __restrict__ f2, double* __restrict__ f3,
+                   double const* __restrict__ edge, int32_t const* __restrict__ idx,
+                   double const* __restrict__ sd, int64_t nseam, Geom gm, Dims d) {
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nseam * d.members) return;
+  int64_t e = t % nseam, m = t / nseam;
+  int32_t const* ix = idx + 8 * e;
+  int pa = ix[0], sa = ix[1], ia = ix[2], pb = ix[3], sb = ix[4], ib = ix[5];
+  double sign = double(ix[6]);
+  double const* D = sd + 30 * e;
+  int64_t ba = m * 6 + pa, bb = m * 6 + pb;
+  double const* own = edge + ((ba * 4 + sa) * int64_t(d.n) + ia) * kRows;
+  double const* oth = edge + ((bb * 4 + sb) * int64_t(d.n) + ib) * kRows;
+  double mine[kRows], other[kRows];
+  for (int v = 0; v < kRows; ++v) mine[v] = own[v];
+  {
+    double v0[3] = {oth[1], oth[2], oth[3]}, g[3], o[3];
+    matvec(D, v0, g);      // the neighbour's components to the geographic frame
+    matvec(D + 9, g, o);   // and into the owner's
+    other[0] = oth[0];
+    other[1] = o[0];
+    other[2] = o[1];
+    other[3] = o[2];
+  }
+  bool up = sa == 1 || sa == 3;
+  double* wl = up ? mine : other;
+  double* wr = up ? other : mine;
+  int plane_a = sa & 1 ? d.ng + d.n : d.ng;
+  double F[kRows];
+  int64_t ca;
+  if (sa < 2) {
+    int k = d.ng + ia;
+    int64_t q = int64_t(k) * (d.nc + 1) + plane_a;
+    face_flux<2>(wl, wr, gm.cos_f2[q], gm.sin_f2[q], F);
+    ca = int64_t(k) * d.nc + plane_a;
+  } else {
+    int j = d.ng + ia;
+    int64_t q = int64_t(plane_a) * d.nc + j;
+    face_flux<3>(wl, wr, gm.cos_f3[q], gm.sin_f3[q], F);
+    ca = int64_t(plane_a) * d.nc + j;
+  }
+  double* fa = (sa < 2 ? f2 : f3) + ba * d.block + ca;
+  for (int v = 0; v < kRows; ++v) fa[v * d.plane] = F[v];
+
+  // the owner's rows: raise with its face cosine, to Cartesian (z, x, y)
+  double const* A = D + 18;  // x, y, delta, C, D, cos at the owner's face
+  double const* Bm = D + 24;
+  double cth = A[5];
+  double sth2 = 1. - cth * cth;
+  double r1 = F[1];
+  double r2 = F[2] / sth2 - F[3] * cth / sth2;
+  double r3 = -F[2] * cth / sth2 + F[3] / sth2;
+  double cart[3];
+  {
+    double x = A[0], y = A[1], delta = A[2], C = A[3], Dd = A[4];
+    double vz = r1, vx = r2, vy = r3;
+    cart[kL2CIdx[pa][0]] = double(kL2CSgn[pa][0]) * (vz - vx * x / Dd - vy * y / C) / delta;
+    cart[kL2CIdx[pa][1]] = double(kL2CSgn[pa][1]) * (vz * x + vx * Dd - (vy * x * y) / C) / delta;
+    cart[kL2CIdx[pa][2]] = double(kL2CSgn[pa][2]) * (vz * y + vy * C - (vx * x * y) / Dd) / delta;
+  }
+  double g0 = F[0];
+  if (sign < 0.) {
+    g0 = -g0;
+    cart[0] = -cart[0];
+    cart[1] = -cart[1];
+    cart[2] = -cart[2];
+  }
+  // into the other panel's contravariant, then lower with its face cosine
+  double o1, o2, o3;
+  {
+    double x = Bm[0], y = Bm[1], delta = Bm[2], C = Bm[3], Dd = Bm[4];
+    double loc[3];
+    loc[kC2LIdx[pb][0]] = double(kC2LSgn[pb][0]) * cart[0];
+    loc[kC2LIdx[pb][1]] = double(kC2LSgn[pb][1]) * cart[1];
+    loc[kC2LIdx[pb][2]] = double(kC2LSgn[pb][2]) * cart[2];
+    double vz = loc[0], vx = loc[1], vy = loc[2];
+    o1 = (vz + x * vx + y * vy) / delta;
+    o2 = Dd / delta * (vx - x * vz);
+    o3 = C / delta * (vy - y * vz);
+  }
+  double cb = Bm[5];
+  double l2 = o2 + o3 * cb;
+  double l3 = o3 + o2 * cb;
+  int plane_b = sb & 1 ? d.ng + d.n : d.ng;
+  int64_t cbi = sb < 2 ? int64_t(d.ng + ib) * d.nc + plane_b
+                       : int64_t(plane_b) * d.nc + (d.ng + ib);
+  double* fb = (sb < 2 ? f2 : f3) + bb * d.block + cbi;
+  fb[0] = g0;
+  fb[d.plane] = o1;
+  fb[2 * d.plane] = l2;
+  fb[3 * d.plane] = l3;
+}
+
+// ---------------------------------------------------------------------------
+// divergence, sources, combine
+// ---------------------------------------------------------------------------
+struct CsUpdArgs {
+  double w0, w1, w2;
+  double dt;
+  double const* dt_dev;  // per member, or null
+  double radius;         // x1v
+};
+
+__global__ void __launch_bounds__(256)
+    cs_update_kernel(double const* __restrict__ u0, double* __restrict__ u,
+                     double const* __restrict__ w, double const* __restrict__ f2,
+                     double const* __restrict__ f3, Geom gm, Dims d, CsUpdArgs a) {
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t per = int64_t(d.n) * d.n;
+  if (t >= per * 6 * d.members) return;
+  int64_t b = t / per;
+  int r = int(t % per);
+  int k = d.ng + r / d.n, j = d.ng + r % d.n;
+  int64_t c = int64_t(k) * d.nc + j;
+  int64_t c2 = int64_t(k) * (d.nc + 1) + j;  // x2 face j of row k in area2
+  double dt = a.dt_dev ? a.dt_dev[b / 6] : a.dt;
+  double vol = gm.vol[c];
+  double const* wb = w + b * d.block + c;
+  double phi = wb[0], v1 = wb[d.plane], v2 = wb[2 * d.plane], v3 = wb[3 * d.plane];
+  double cth = gm.cos_c[c];
+  double sine2 = gm.sin_c[c] * gm.sin_c[c];
+  double pr = 0.5 * (phi * phi);
+  double mdt = (-dt) * ((phi * v1) / a.radius);
+  double a2l = gm.area2[c2], a2r = gm.area2[c2 + 1];
+  double a3l = gm.area3[c], a3r = gm.area3[c + d.nc];
+  for (int v = 0; v < kRows; ++v) {
+    int64_t o = b * d.block + v * d.plane + c;
+    double dv = 0. + (a2r * f2[o + 1] - a2l * f2[o]);
+    dv = dv + (a3r * f3[o + d.nc] - a3l * f3[o]);
+    dv = dv / vol;
+    if (v == 2) dv = dv - gm.xr[c] * (pr + phi * v3 * v3 * sine2);
+    if (v == 3) dv = dv - gm.yr[c] * (pr + phi * v2 * v2 * sine2);
+    double du = -dt * dv;
+    // the x1 mass source is added to every row (zero on phi and phi v1)
+    if (v == 2)
+      du = du + (0. + mdt * (v2 + v3 * cth));
+    else if (v == 3)
+      du = du + (0. + mdt * (v3 + v2 * cth));
+    else
+      du = du + 0.;
+    u[o] = a.w0 * u0[o] + a.w1 * u[o] + a.w2 * du;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CFL, per member
+// ---------------------------------------------------------------------------
+__global__ void cs_cfl_init_kernel(unsigned long long* out, int members) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < members) {
+    double big = 1.e9;
+    unsigned long long bits;
+    memcpy(&bits, &big, sizeof(bits));
+    out[m] = bits;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    cs_cfl_kernel(double const* __restrict__ w, Geom gm, Dims d,
+                  unsigned long long* __restrict__ out) {
+  // one block row of 256 threads per (member, chunk); blockIdx.y = member
+  int m = blockIdx.y;
+  int64_t per = int64_t(d.n) * d.n * 6;
+  double best = 1.e9;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < per;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    int64_t p = t / (int64_t(d.n) * d.n);
+    int r = int(t % (int64_t(d.n) * d.n));
+    int64_t c = int64_t(d.ng + r / d.n) * d.nc + (d.ng + r % d.n);
+    double const* wb = w + (int64_t(m) * 6 + p) * d.block + c;
+    double cs = sqrt(wb[0]);
+    best = fmin(best, gm.w2[c] / (fabs(wb[2 * d.plane]) + cs));
+    best = fmin(best, gm.w3[c] / (fabs(wb[3 * d.plane]) + cs));
+  }
+  for (int off = 32; off > 0; off >>= 1) best = fmin(best, __shfl_down(best, off, 64));
+  if ((threadIdx.x & 63) == 0) {
+    unsigned long long bits;
+    memcpy(&bits, &best, sizeof(bits));
+    atomicMin(out + m, bits);
+  }
+}
+
+__global__ void cs_cfl_scale_kernel(double* out, double s, int members) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < members) out[m] = out[m] * s;
+}
+
+int check_mesh(snapy_cs_mesh const* m) {
+  if (!m) return fail(-1, "null mesh descriptor");
+  if (m->n < 4 || m->nghost != 3 || m->members < 1)
+    return fail(-2, "cubed sphere: n >= 4, nghost 3, members >= 1");
+  if (m->interp_order != 2 && m->interp_order != 4)
+    return fail(-3, "cubed sphere: interp_order 2 or 4");
+  if (!m->geom) return fail(-4, "cubed sphere: null grid tables");
+  return 0;
+}
+
+}  // namespace
+}  // namespace snapy
+
+using namespace snapy;
+
+extern "C" {
+
+int64_t snapy_cs_geom_size(int32_t n, int32_t nghost) {
+  int64_t nc = n + 2 * nghost;
+  return 7 * nc * nc + 6 * nc * (nc + 1);
+}
+
+int64_t snapy_cs_edge_size(const snapy_cs_mesh* m) {
+  if (!m) return -1;
+  return int64_t(m->members) * 6 * 4 * m->n * kRows;
+}
+
+int snapy_cs_cons2prim(const snapy_cs_mesh* m, const double* u, double* w, void* stream) {
+  if (int rc = check_mesh(m)) return rc;
+  Dims d = dims_of(*m);
+  int64_t n = d.plane * 6 * d.members;
+  hipLaunchKernelGGL(cs_cons2prim_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+                     as_stream(stream), u, w, m->geom, d, n);
+  return launch_status("cs_cons2prim");
+}
+
+int snapy_cs_prim2cons(const snapy_cs_mesh* m, const double* w, double* u, void* stream) {
+  if (int rc = check_mesh(m)) return rc;
+  Dims d = dims_of(*m);
+  int64_t n = d.plane * 6 * d.members;
+  hipLaunchKernelGGL(cs_prim2cons_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+                     as_stream(stream), w, u, m->geom, d, n);
+  return launch_status("cs_prim2cons");
+}
+
+int snapy_cs_ghosts(const snapy_cs_mesh* m, int conserved, double* q, void* stream) {
+  if (int rc = check_mesh(m)) return rc;
+  if (!m->ghost_idx || !m->ghost_w || !m->ghost_mat_c || !m->ghost_mat_p)
+    return fail(-4, "cubed sphere: null ghost tables");
+  Dims d = dims_of(*m);
+  int64_t n = m->nghost_entries * d.members;
+  hipLaunchKernelGGL(cs_ghost_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+                     as_stream(stream), q, m->ghost_idx, m->ghost_w,
+                     conserved ? m->ghost_mat_c : m->ghost_mat_p, m->nghost_entries, d);
+  return launch_status("cs_ghosts");
+}
+
+int snapy_cs_sweep(const snapy_cs_mesh* m, int dim, const double* w, double* flux,
+                   double* edge, void* stream) {
+  if (int rc = check_mesh(m)) return rc;
+  if (dim != 2 && dim != 3) return fail(-8, "cubed sphere: sweeps are x2 and x3");
+  if (!edge) return fail(-4, "cubed sphere: null edge buffer");
+  Dims d = dims_of(*m);
+  Geom gm = geom_of(m->geom, d.nc);
+  int64_t n = int64_t(d.n) * (d.n + 1) * 6 * d.members;
+  if (dim == 2)
+    hipLaunchKernelGGL(cs_sweep_kernel<2>, dim3(blocks_for(n, 256)), dim3(256), 0,
+                       as_stream(stream), w, flux, edge, gm, d, m->shock);
+  else
+    hipLaunchKernelGGL(cs_sweep_kernel<3>, dim3(blocks_for(n, 256)), dim3(256), 0,
+                       as_stream(stream), w, flux, edge, gm, d, m->shock);
+  return launch_status("cs_sweep");
+}
+
+int snapy_cs_seams(const snapy_cs_mesh* m, double* flux2, double* flux3,
+                   const double* edge, void* stream) {
+  if (int rc = check_mesh(m)) return rc;
+  if (!m->seam_idx || !m->seam_d) return fail(-4, "cubed sphere: null seam tables");
+  Dims d = dims_of(*m);
+  Geom gm = geom_of(m->geom, d.nc);
+  int64_t nseam = int64_t(12) * d.n;
+  int64_t n = nseam * d.members;
+  hipLaunchKernelGGL(cs_seam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+                     as_stream(stream), flux2, flux3, edge, m->seam_idx, m->seam_d, nseam,
+                     gm, d);
+  return launch_status("cs_seams");
+}
+
+int snapy_cs_update(const snapy_cs_mesh* m, const double* u0, double* u, const double* w,
+                    const double* flux2, const double* flux3, double dt,
+                    const double* dt_dev, double w0, double w1, double w2, void* stream) {
+  if (int rc = check_mesh(m)) return rc;
+  Dims d = dims_of(*m);
+  Geom gm = geom_of(m->geom, d.nc);
+  CsUpdArgs a{w0, w1, w2, dt, dt_dev, m->radius};
+  int64_t n = int64_t(d.n) * d.n * 6 * d.members;
+  hipLaunchKernelGGL(cs_update_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+                     as_stream(stream), u0, u, w, flux2, flux3, gm, d, a);
+  return launch_status("cs_update");
+}
+
+int snapy_cs_cfl(const snapy_cs_mesh* m, const double* w, double scale, double* dt_out,
+                 void* stream) {
+  if (int rc = check_mesh(m)) return rc;
+  if (!dt_out) return fail(-4, "cubed sphere: null dt output");
+  Dims d = dims_of(*m);
+  Geom gm = geom_of(m->geom, d.nc);
+  auto* bits = reinterpret_cast<unsigned long long*>(dt_out);
+  unsigned mb = blocks_for(d.members, 64);
+  hipLaunchKernelGGL(cs_cfl_init_kernel, dim3(mb), dim3(64), 0, as_stream(stream), bits,
+                     d.members);
+  int64_t per = int64_t(d.n) * d.n * 6;
+  unsigned gx = blocks_for(per, 256);
+  if (gx > 64) gx = 64;
+  hipLaunchKernelGGL(cs_cfl_kernel, dim3(gx, d.members), dim3(256), 0, as_stream(stream), w,
+                     gm, d, bits);
+  hipLaunchKernelGGL(cs_cfl_scale_kernel, dim3(mb), dim3(64), 0, as_stream(stream), dt_out,
+                     scale, d.members);
+  return launch_status("cs_cfl");
+}
+
+}  // extern "C"

@@ -56,6 +56,7 @@ class CoordinateOptions:
     nx2: int = 1
     nx3: int = 1
     nghost: int = 3
+    interp_order: int = 2  # cells/interp_order: panel-edge ghost interpolation
 
     def nc(self, d: int) -> int:
         nx = (self.nx1, self.nx2, self.nx3)[d - 1]
@@ -171,17 +172,21 @@ class MeshBlockOptions:
 
 
 def _read_geometry(g) -> CoordinateOptions:
-    _check_keys(g, "geometry", {"type", "bounds", "cells", "interp_order"})
+    _check_keys(g, "geometry", {"type", "bounds", "cells"})
     c = CoordinateOptions()
     c.type = g.get("type", "cartesian")
     b = g.get("bounds") or {}
-    _check_keys(b, "geometry/bounds", {"x1min", "x1max", "x2min", "x2max", "x3min", "x3max"})
-    for k in ("x1min", "x1max", "x2min", "x2max", "x3min", "x3max"):
+    names = ("x1min", "x1max", "x2min", "x2max", "x3min", "x3max")
+    _check_keys(b, "geometry/bounds", set(names) | {k + "_pi" for k in names})
+    for k in names:
         if k in b:
             setattr(c, k, _pi(b[k]))
+        # the _pi alias, scaled by pi; given both, the alias wins
+        if k + "_pi" in b:
+            setattr(c, k, float(b[k + "_pi"]) * math.pi)
     cells = g.get("cells") or {}
-    _check_keys(cells, "geometry/cells", {"nx1", "nx2", "nx3", "nghost"})
-    for k in ("nx1", "nx2", "nx3", "nghost"):
+    _check_keys(cells, "geometry/cells", {"nx1", "nx2", "nx3", "nghost", "interp_order"})
+    for k in ("nx1", "nx2", "nx3", "nghost", "interp_order"):
         if k in cells:
             setattr(c, k, int(cells[k]))
     return c
@@ -234,7 +239,12 @@ def _read_dynamics(d, forcing) -> HydroOptions:
     h.riemann_dir = str(rs.get("dir", h.riemann_dir))
     h.precision = d.get("precision", h.precision)
     h.flux_covariance = bool(d.get("flux-covariance", h.flux_covariance))
-    _check_keys(forcing, "forcing", {"const-gravity", "bot-heat", "diffusion"})
+    _check_keys(forcing, "forcing", {"const-gravity", "bot-heat", "diffusion", "coriolis"})
+    if "coriolis" in forcing:
+        co = forcing["coriolis"] or {}
+        _check_keys(co, "forcing/coriolis", {"omega1", "omega2", "omega3", "omegax", "omegay", "omegaz"})
+        if any(float(v) != 0.0 for v in co.values()):
+            raise ValueError("forcing/coriolis with a nonzero rotation rate is not implemented")
     if "diffusion" in forcing:
         # DiffusionOptionsImpl::from_yaml (src/forcing/diffusion.cpp:280-366):
         # the kinematic isotropic form with stress-free walls is built here
@@ -300,10 +310,13 @@ def _read_bcs(bc) -> list:
     return [ext.get(f, "reflecting") for f in _FACES]
 
 
-def _check_supported(op: MeshBlockOptions) -> None:
+def _check_supported(op: MeshBlockOptions, layout: str = "") -> None:
     """Refuse deck features outside the implemented stage, loudly."""
+    if op.coord.type == "gnomonic-equiangle":
+        _check_cubed(op, layout)
+        return
     if op.coord.type != "cartesian":
-        raise ValueError(f"geometry/type '{op.coord.type}': this stage implements cartesian")
+        raise ValueError(f"geometry/type '{op.coord.type}': cartesian or gnomonic-equiangle")
     if op.hydro.eos.type not in ("ideal-gas", "shallow-water"):
         raise ValueError(f"equation-of-state type '{op.hydro.eos.type}' is not implemented")
     if op.hydro.eos.type == "shallow-water":
@@ -358,3 +371,67 @@ def _check_supported(op: MeshBlockOptions) -> None:
     for f, n in zip(_FACES, op.bcnames):
         if n not in ("reflecting", "periodic", "extrapolation", "outflow", "custom", "none"):
             raise ValueError(f"boundary {f} '{n}' is not implemented")
+
+
+def _check_cubed(op: MeshBlockOptions, layout: str) -> None:
+    """The cubed-sphere shallow-water stage: six panels, one block each."""
+    c, h = op.coord, op.hydro
+    if layout != "cubed-sphere":
+        raise ValueError("geometry/type gnomonic-equiangle needs distribute/layout: cubed-sphere (use MeshOptions)")
+    if h.eos.type != "shallow-water" or h.riemann != "shallow-roe" or h.riemann_dir != "yz":
+        raise ValueError("the cubed sphere is built for shallow-water with riemann-solver shallow-roe, dir: yz")
+    if c.nx1 != 1 or c.nx2 != c.nx3 or c.nx2 < 4:
+        raise ValueError("cubed sphere: nx1 = 1 and nx2 = nx3 >= 4")
+    if c.nghost != 3:
+        raise ValueError("cubed sphere: nghost 3")
+    if c.interp_order not in (2, 4):
+        raise ValueError("GnomonicEquiangle: interp_order must be 2 (linear panel-edge ghosts) or 4 (cubic)")
+    quarter = 0.25 * math.pi
+    for v, want in ((c.x2min, -quarter), (c.x2max, quarter), (c.x3min, -quarter), (c.x3max, quarter)):
+        if abs(v - want) > 1e-12:
+            raise ValueError("cubed sphere: x2 and x3 span [-pi/4, pi/4] (one block per panel)")
+    if h.precision != "float64":
+        raise ValueError("cubed sphere: the FP64 form is built")
+    if h.grav1 != 0.0 or h.nu_iso > 0.0 or h.kappa_iso > 0.0 or h.bot_heat_flux != 0.0:
+        raise ValueError("cubed sphere: no forcing is implemented")
+    if h.eos.limiter:
+        raise ValueError("equation-of-state/limiter: true is not implemented")
+    if op.intg.type != "rk3" or op.intg.implicit_scheme != 0:
+        raise ValueError("cubed sphere: rk3, explicit")
+    for f, n in zip(_FACES[2:], op.bcnames[2:]):
+        if n not in ("custom", "none"):
+            raise ValueError(f"cubed sphere: boundary {f} is a panel seam (custom), got '{n}'")
+
+
+@dataclass
+class MeshOptions:
+    """MeshOptionsImpl::from_yaml (src/mesh/mesh.hpp): the block options and
+    the layout that places the blocks (distribute/)."""
+
+    block: MeshBlockOptions = field(default_factory=MeshBlockOptions)
+    layout: str = "slab"
+    nb2: int = 1
+    nb3: int = 1
+    blocks_per_process: int = 1
+
+    @staticmethod
+    def from_yaml(filename: str, verbose: bool = False) -> "MeshOptions":
+        with open(filename) as f:
+            cfg = yaml.safe_load(f) or {}
+        dist = cfg.get("distribute") or {}
+        _check_keys(dist, "distribute", {"layout", "nb2", "nb3", "blocks_per_process", "verbose"})
+        mo = MeshOptions()
+        mo.layout = str(dist.get("layout", "slab"))
+        mo.nb2 = int(dist.get("nb2", 1))
+        mo.nb3 = int(dist.get("nb3", 1))
+        mo.blocks_per_process = int(dist.get("blocks_per_process", 1))
+        if mo.layout == "cubed-sphere" and (mo.nb2 != 1 or mo.nb3 != 1):
+            raise ValueError("cubed sphere: one block per panel (nb2 = nb3 = 1)")
+        op = MeshBlockOptions(verbose=verbose, filename=os.fspath(filename))
+        op.coord = _read_geometry(cfg.get("geometry") or {})
+        op.hydro = _read_dynamics(cfg.get("dynamics") or {}, cfg.get("forcing") or {})
+        op.intg = _read_integration(cfg.get("integration") or {})
+        op.bcnames = _read_bcs(cfg.get("boundary-condition") or {})
+        _check_supported(op, mo.layout)
+        mo.block = op
+        return mo

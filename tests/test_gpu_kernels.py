@@ -1,6 +1,6 @@
 """Kernel-level parity of the HIP stage against the CPU FP64 oracle, through
-the C-ABI (include/snapy_amd.h).  FP64 bar: 1e-12 relative L2 (the north-star
-tolerance for the plain FP64 form); copies (boundaries) bit-exact."""
+the C-ABI (include/snapy_amd.h).  FP64 bar: 1e-12 relative L2 (the tolerance
+for the plain FP64 form); copies (boundaries) bit-exact."""
 
 import ctypes
 import itertools
