@@ -276,7 +276,16 @@ __global__ void __launch_bounds__(256)
   if (t >= per * 6 * d.members) return;
   int64_t b = t / per;
   int r = int(t % per);
-  int f = r % nf, a = r / nf;  // face along the sweep, interior line across
+  // consecutive threads step along x2 (the contiguous axis) in both sweeps:
+  // along the faces of an x2 sweep, across the lines of an x3 sweep
+  int f, a;  // face along the sweep, interior line across
+  if (AX == 2) {
+    f = r % nf;
+    a = r / nf;
+  } else {
+    a = r % d.n;
+    f = r / d.n;
+  }
   int k, j;
   int64_t stride;
   if (AX == 2) {

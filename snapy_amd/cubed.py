@@ -399,6 +399,17 @@ def build_seam_tables(grid: PanelGrid):
     return np.array(idx, dtype=np.int32), np.array(dat, dtype=np.float64)
 
 
+def sphere_points(grid: PanelGrid, panel: int) -> np.ndarray:
+    """(3, nc, nc) unit positions (x, y, z) of a panel's cell centres
+    (cs_ab_to_xyz, src/coord/cubed_sphere_utils.cpp:47-52)."""
+    a = np.broadcast_to(grid.tx[None, :], (grid.nc, grid.nc))
+    b = np.broadcast_to(grid.ty[:, None], (grid.nc, grid.nc))
+    one = np.ones_like(a)
+    x, y, z = ((one, a, b), (-a, one, b), (-one, -a, b), (-b, a, one), (a, -one, b), (b, a, -one))[panel]
+    norm = np.sqrt(x * x + y * y + z * z)
+    return np.stack([x / norm, y / norm, z / norm])
+
+
 # ---------------------------------------------------------------------------
 # the mesh
 # ---------------------------------------------------------------------------
@@ -612,6 +623,14 @@ class CubedSphereMesh:
     def execution_report(self) -> dict:
         return self._executor.report()
 
-    def release(self) -> None:
-        torch.cuda.synchronize(self.device)
+    def interior_state(self) -> torch.Tensor:
+        """The conserved state of every block's interior, (blocks, 4, n, n)."""
+        i = slice(self.ng, self.ng + self.n)
+        return self._u[:, :, i, i, 0]
+
+    def release(self, sync: bool = True) -> None:
+        """Drop the captured graphs (sync=False: the caller has synchronized
+        the stream the stages ran on)."""
+        if sync:
+            torch.cuda.synchronize(self.device)
         self._executor.release()

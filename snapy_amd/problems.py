@@ -57,3 +57,29 @@ def density_current(block, dT=-15.0, theta=300.0, psfc=1.0e5, grav=9.81, xc=0.0,
     w[4] = p
     w[0] = p / (RD * t)
     return torch.from_numpy(w)
+
+
+def swe_hill(mesh, seed, radius=6.371e6, grav=9.81, h0=1000.0):
+    """The batched shallow-water initial state: phi = g (h0 + A exp(-(d/R)^2))
+    at rest, A ~ U(50, 200) m, R ~ U(300, 1000) km, d the great-circle
+    distance to a centre with latitude asin(U(-1, 1)) and longitude
+    U(0, 2 pi), all drawn from `seed`.  Returns one hydro_w per panel."""
+    import math
+
+    from .cubed import sphere_points
+
+    rng = np.random.default_rng(seed)
+    amp = rng.uniform(50.0, 200.0)
+    rad = rng.uniform(3.0e5, 1.0e6)
+    lat = math.asin(rng.uniform(-1.0, 1.0))
+    lon = rng.uniform(0.0, 2.0 * math.pi)
+    c = np.array([math.cos(lat) * math.cos(lon), math.cos(lat) * math.sin(lon), math.sin(lat)])
+    out = []
+    for panel in range(6):
+        xyz = sphere_points(mesh.grid, panel)
+        d = radius * np.arccos(np.clip(np.tensordot(c, xyz, axes=1), -1.0, 1.0))
+        w = np.zeros(mesh.block_shape())
+        w[0, :, :, 0] = grav * (h0 + amp * np.exp(-((d / rad) ** 2)))
+        out.append(torch.from_numpy(w))
+    return out
+

@@ -17,8 +17,9 @@ Request kinds (the serving mix of the benchmark):
   * ``density_current``  -- the 2D density current (gravity, diffusion, the
     vertical implicit solve) at a seeded grid and bubble amplitude, in the
     FP64 or the FP32 state form;
-  * ``swe``              -- shallow water on a periodic Cartesian plane with
-    a seeded Gaussian hill.
+  * ``swe``              -- cubed-sphere shallow water (six gnomonic panels
+    of N x N) with a seeded Gaussian hill of h at a seeded point of the
+    sphere, at rest.
 """
 
 from __future__ import annotations
@@ -31,10 +32,12 @@ import numpy as np
 import torch
 
 from . import problems
+from .cubed import CubedSphereMesh
 from .meshblock import MeshBlock
-from .options import CoordinateOptions, HydroOptions, IntegratorOptions, MeshBlockOptions
+from .options import CoordinateOptions, HydroOptions, IntegratorOptions, MeshBlockOptions, MeshOptions
 
 RD, CP, GRAV = 287.0, 1004.0, 9.81
+RADIUS = 6.371e6
 GAMMA_DRY = CP / (CP - RD)
 WEIGHT_DRY = 8.31446261815324 / RD
 
@@ -91,19 +94,20 @@ def _options(kind: str, p: dict) -> MeshBlockOptions:
         op.bcnames = ["reflecting"] * 6
     elif kind == "swe":
         n = int(p["n"])
-        half = 3.0e6
-        op.coord = CoordinateOptions("cartesian", -half, half, -half, half, -0.5, 0.5, n, n, 1, 3)
-        op.hydro = HydroOptions(riemann="shallow-roe")
+        q = 0.25 * math.pi
+        op.coord = CoordinateOptions("gnomonic-equiangle", RADIUS, RADIUS + 1.0, -q, q, -q, q, 1, n, n, 3)
+        op.hydro = HydroOptions(riemann="shallow-roe", riemann_dir="yz")
         op.hydro.eos.type = "shallow-water"
         op.hydro.recon1.shock = False
         op.hydro.recon23.shock = False
-        op.bcnames = ["periodic"] * 6
+        op.bcnames = ["reflecting"] * 2 + ["custom"] * 4
+        return MeshOptions(block=op, layout="cubed-sphere")
     else:
         raise ValueError(f"unknown request kind '{kind}'")
     return op
 
 
-def _initial_state(kind: str, block: MeshBlock, p: dict, seed: int) -> torch.Tensor:
+def _initial_state(kind: str, block, p: dict, seed: int):
     if kind == "sod":
         rng = np.random.default_rng(seed)
         left = (rng.uniform(0.1, 1.0), 0.0, rng.uniform(0.1, 1.0))
@@ -111,23 +115,34 @@ def _initial_state(kind: str, block: MeshBlock, p: dict, seed: int) -> torch.Ten
         return problems.sod(block, left=left, right=right)
     if kind == "density_current":
         return problems.density_current(block, dT=float(p["dT"]), grav=GRAV)
-    # swe: phi = g (1000 + A exp(-(d/R)^2)), at rest
-    rng = np.random.default_rng(seed)
-    amp = rng.uniform(50.0, 200.0)
-    rad = rng.uniform(3.0e5, 1.0e6)
-    cx, cy = rng.uniform(-1.5e6, 1.5e6, size=2)
-    x = block.module("coord").buffer("x1v").numpy()[None, None, :]
-    y = block.module("coord").buffer("x2v").numpy()[None, :, None]
-    w = np.zeros(block.shape())
-    d2 = (x - cx) ** 2 + (y - cy) ** 2
-    w[0] = GRAV * (1000.0 + amp * np.exp(-d2 / (rad * rad)))
-    return torch.from_numpy(w)
+    # swe: a seeded Gaussian hill of h on the sphere, at rest; one w per panel
+    return problems.swe_hill(block, seed, radius=RADIUS, grav=GRAV)
 
 
-def make_block(req: Request, device="cuda") -> MeshBlock:
-    block = MeshBlock(_options(req.kind, req.params))
+def make_block(req: Request, device="cuda"):
+    """A MeshBlock, or for shallow water a six-panel CubedSphereMesh."""
+    op = _options(req.kind, req.params)
+    if isinstance(op, MeshOptions):
+        op.block.intg = IntegratorOptions(cfl=0.9, execution=req.params.get("execution", "hip-graph"))
+        return CubedSphereMesh(op).to(device)
+    block = MeshBlock(op)
     block.to(device)
     return block
+
+
+def _initialize(block, w):
+    if isinstance(block, CubedSphereMesh):
+        vars, _ = block.initialize([{"hydro_w": x} for x in w])
+        return vars
+    vars, _ = block.initialize({"hydro_w": w.to("cuda")})
+    return vars
+
+
+def _final_state(block, vars) -> np.ndarray:
+    if isinstance(block, CubedSphereMesh):
+        return block.interior_state().cpu().numpy()
+    sub = tuple(block.part(exterior=False))
+    return block.total_u(vars)[sub].double().cpu().numpy()
 
 
 def request_cells(req: Request) -> int:
@@ -136,7 +151,7 @@ def request_cells(req: Request) -> int:
         return int(p["n"])
     if req.kind == "density_current":
         return int(p["nx"]) * int(p["nz"])
-    return int(p["n"]) ** 2
+    return 6 * int(p["n"]) ** 2
 
 
 # ----------------------------------------------------------------------------
@@ -186,8 +201,8 @@ def sample_requests(n: int, seed: int = 7, rate: float = 20.0, kmin: int = 50, k
 @dataclass
 class _Slot:
     req: Request
-    block: MeshBlock
-    vars: dict
+    block: object
+    vars: object
     stream: torch.cuda.Stream
     bytes: int
     admitted: float
@@ -229,7 +244,7 @@ class Server:
                 si = free_streams.pop(0)
                 with torch.cuda.stream(streams[si]):
                     w = _initial_state(req.kind, block, req.params, req.seed)
-                    vars, _ = block.initialize({"hydro_w": w.to("cuda")})
+                    vars = _initialize(block, w)
                 used += need
                 active.append(_Slot(req, block, vars, streams[si], need, now()))
                 self.log.append(("admit", req.rid, now(), used))
@@ -253,8 +268,7 @@ class Server:
                     continue
                 sl.stream.synchronize()
                 b = sl.block
-                sub = tuple(b.part(exterior=False))
-                u = b.total_u(sl.vars)[sub].double().cpu().numpy()
+                u = _final_state(b, sl.vars)
                 # without real-time arrivals every request is there at the start
                 arrival = sl.req.arrival if self.realtime else 0.0
                 results.append(Result(sl.req.rid, sl.req.kind, sl.req.steps, request_cells(sl.req),
