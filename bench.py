@@ -41,11 +41,16 @@ def parse():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream leg")
     ap.add_argument("--serving-requests", type=int, default=24)
     ap.add_argument("--no-cubed", action="store_true", help="skip the batched cubed-sphere leg")
+    ap.add_argument("--implicit-scheme", type=int, default=None, choices=(1, 9),
+                    help="override the convection deck's implicit scheme")
     ap.add_argument("--cubed-steps", type=int, default=50)
     return ap.parse_args()
 
 
 # --------------------------------------------------------------------------
+SCHEME = {"value": None}  # --implicit-scheme
+
+
 def make_block(deck, execution="hip-graph", precision="float64", **coord):
     from snapy_amd import MeshBlock, MeshBlockOptions
 
@@ -54,6 +59,8 @@ def make_block(deck, execution="hip-graph", precision="float64", **coord):
         setattr(op.coord, k, v)
     op.intg.execution = execution
     op.hydro.precision = precision
+    if SCHEME["value"] is not None and op.intg.implicit_scheme:
+        op.intg.implicit_scheme = SCHEME["value"]
     block = MeshBlock(op)
     block.to("cuda")
     return block
@@ -77,11 +84,14 @@ def workload(name, precision="float64", execution="hip-graph"):
     block = make_block("convection3d.yaml", execution=execution, precision=precision)
     vars, _ = block.initialize({"hydro_w": problems.convection3d(block).cuda()})
     c = block.options.coord
+    scheme = block.options.intg.implicit_scheme
+    solve = "9 (full 5x5" if scheme == 9 else "1 (partial 3x3 with the tangential fold"
     cfg = {
         "workload": "convection3d_256x256x128",
         "cells": [c.nx1, c.nx2, c.nx3],
         "method": "weno5+lmars+ssp-rk3, well-balanced x1 gravity, flux covariance, bot-heat, "
-        "vertical implicit scheme 9 (full 5x5, Woodbury), periodic x2/x3, cfl 0.8",
+        f"vertical implicit scheme {solve}, Woodbury), periodic x2/x3, cfl 0.8",
+        "implicit_scheme": scheme,
     }
     if precision == "float32":
         cfg["form"] = ("FP32 state form: FP32 sweeps and face fluxes, compensated FP32 state pair, "
@@ -195,6 +205,7 @@ def kernel_bytes(block, name, sb=8, pair=False):
         "stage_increment": (5 * nd + 2 + 5) * nint * sb,
         # du, w (FP64) read; u0, u read and u written: FP64, or FP32 pairs
         "vic_full": (10 * 8 + 15 * (2 * 4 if pair else 8)) * nint,
+        "vic_partial": (10 * 8 + 15 * (2 * 4 if pair else 8)) * nint,
         "update": (3 + nd) * 5 * nint * sb,
         "update_gravity": ((3 + nd) * 5 + 2) * nint * sb,
     }
@@ -350,6 +361,7 @@ def serving_leg(nreq, seed=2025):
 
 def main():
     args = parse()
+    SCHEME["value"] = args.implicit_scheme
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark measures one GPU; run with "
                           "--gpus 1 (the path shards by whole members or x2/x3 column groups; not measured here)"}), flush=True)

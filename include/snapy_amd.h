@@ -303,6 +303,35 @@ int snapy_update_gravity_f32(const snapy_grid *g, const snapy_physics *p,
                              const double *x1tab, const double *du_src,
                              double dt, const double *dt_dev, double w0,
                              double w1, double w2, void *stream);
+/* The vertical implicit correction with the partial matrix
+ * (implicit-scheme 1): 3x3 blocks in (rho, m1, E) with the tangential
+ * momenta folded in as dm_t = v_t drho + q_t, the corrected-PE coupling and
+ * its Woodbury wall points, the mass redistribution with the tangential
+ * momentum each moved mass carries (the donor cell's velocity), the
+ * post-solve remainder and the RK combine; arguments as snapy_vic_full.
+ * One kernel assembles each cell's blocks into a 32-double record of ws
+ * (snapy_vic_partial_workspace(g) doubles), one thread per column then
+ * sweeps, back-substitutes and redistributes.
+ * Replaces: vic_stage for VicMatrix::partial (vic_kernels.h:66-297 with
+ * N = 3; vic_assemble_impl.h:45-218; forward_sweep_impl.h:104-121;
+ * vic_redistribute_impl.h:177-205) and ImplicitHydroImpl::forward_masked
+ * (implicit_hydro.cpp:301-623). */
+int64_t snapy_vic_partial_workspace(const snapy_grid *g);
+int snapy_vic_partial(const snapy_grid *g, const snapy_physics *p,
+                      const double *w, const double *du, const double *u0,
+                      double *u, const double *vtab, const double *x1tab,
+                      double t0, double t1, double dt, const double *dt_dev,
+                      double wstage, double w0, double w1, double w2,
+                      double *ws, int *status, void *stream);
+/* snapy_vic_partial finishing into the FP32 pair (u, c) from (u0, c0). */
+int snapy_vic_partial_f32(const snapy_grid *g, const snapy_physics *p,
+                          const double *w, const double *du, const float *u0,
+                          const float *c0, float *u, float *c,
+                          const double *vtab, const double *x1tab, double t0,
+                          double t1, double dt, const double *dt_dev,
+                          double wstage, double w0, double w1, double w2,
+                          double *ws, int *status, void *stream);
+
 /* snapy_vic_full finishing into the pair (u, c) from (u0, c0). */
 int snapy_vic_full_f32(const snapy_grid *g, const snapy_physics *p,
                        const double *w, const double *du, const float *u0,

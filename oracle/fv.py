@@ -688,7 +688,7 @@ class Block:
             if 1 in flux:
                 gec = gravity.gravity_energy_correction(g, self.w, flux[1], dt, d.grav1)
                 du[IPR][..., g.nghost : g.nghost + g.nx1] += gec
-        if d.implicit_scheme == 9:
+        if d.implicit_scheme in (1, 9):
             # the VIC with the stage weight inside the solve
             # (hydro_forward.cpp:1137-1180)
             du = self._implicit(du, dt * RK3_WEIGHTS[self._stage][2])
@@ -707,7 +707,8 @@ class Block:
         dcol = np.moveaxis(du[:, k, j, ng : ng + n], 0, -1).reshape(ncol, n, 5)
         x1f = g.faces(1)[ng : ng + n + 1]
         x1v = g.centers(1)[ng : ng + n]
-        out = implicit.vic_full(
+        solve = implicit.vic_partial if self.deck.implicit_scheme == 1 else implicit.vic_full
+        out = solve(
             dcol, wcol, self.deck.gammad, self.deck.grav1, dt_corr, g.dx(1), x1f, x1v,
             g.dx(3) * g.dx(2), g.dx(3) * (g.dx(2) * g.dx(1)),
         )

@@ -1,0 +1,697 @@
+// The vertical implicit correction with the partial matrix (implicit-scheme
+// 1), FP64 on gfx950: 3x3 blocks in (rho, m1, E), the tangential momenta
+// written dm_t = v_t drho + q_t and folded into the rho columns and the
+// right-hand side, the corrected-PE coupling with its two Woodbury wall
+// points, the mass redistribution with the tangential momentum each moved
+// mass carries, the post-solve remainder and the RK combine.
+//
+// Two kernels:
+//   * vicp_assemble_kernel: one thread per cell (x1 fastest, so the state
+//     reads are coalesced), writing the cell's blocks and right-hand side
+//     as one 32-double record (the assembly is cell-local:
+//     vic_assemble_cell, src/implicit/vic_kernels.h:66-105);
+//   * vicp_sweep_kernel: one thread per column, the serial sweep over the
+//     records (rewriting each in place with the factored row and the three
+//     partial solutions) and the back substitution;
+//   * vicp_finish_kernel<PAIR>: one wavefront per column, layers across the
+//     lanes: the Woodbury 2x2, the redistribution (its serial chains on one
+//     lane, in the column's order), the outputs and the combine.
+//
+// Restates ImplicitHydroImpl::forward_masked (src/implicit/
+// implicit_hydro.cpp:301-623, scheme 1) for a Cartesian block owning its
+// whole x1 column, no solids, dry ideal gas, no FluxRegister posts:
+// vic_assemble_impl<3> (vic_assemble_impl.h:45-218), ForwardSweep's small
+// matrix path (forward_sweep_impl.h:104-121), vic_backward_substitute,
+// vic_woodbury_column, vic_constituent_column and
+// vic_tangential_momentum_column (vic_redistribute_impl.h:31-205),
+// book_remainder_ (implicit_hydro.cpp:713-794).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "common.hpp"
+#include "point.hpp"
+
+namespace snapy {
+namespace {
+
+constexpr int kRec = 32;  // doubles per cell record
+constexpr int kTileC = 32, kTileL = 8;  // the assembly's block tile
+// record slots: assembly a 0..8, b 9..17, c 18..26, rhs 27..29; the sweep
+// rewrites 0..8 A' = a^-1 c, 9..11 y, 12..14 z0, 15..17 z1; the
+// redistribution uses 18 phi, 19 M (face below), 20 mass change, 21 moved
+enum { R_A = 0, R_B = 9, R_C = 18, R_RHS = 27, R_Y = 9, R_Z0 = 12, R_Z1 = 15,
+       R_PHI = 18, R_M = 19, R_DM = 20, R_MV = 21 };
+
+enum { V_WLO = 0, V_WHI, V_KLO, V_KMID, V_KHI, V_PHI, V_DPLO, V_DPHI };
+enum { T_GVAR = 3, T_WT0 = 4, T_WT1 = 5, T_WT2 = 6 };
+
+struct PartArgs {
+  Range3 r;
+  int n;
+  double gamma, grav, area, vol, t0, t1;
+  double const* dt_dev;
+  double dt_host;
+  double wstage;
+  double w0, w1, w2;
+};
+
+__device__ __forceinline__ double stage_dt(PartArgs const& a) {
+  return (a.dt_dev ? *a.dt_dev : a.dt_host) * a.wstage;
+}
+
+// rows {rho, m1, E} of FluxJacobian (flux_decomposition_impl.h:105-126), dir 0
+__device__ __forceinline__ void jac_rows(double f[3][5], double gm1, double const* w) {
+  double v1 = w[IVX], v2 = w[IVY], v3 = w[IVZ], rho = w[IDN], pres = w[IPR];
+  double s2 = v1 * v1 + v2 * v2 + v3 * v3;
+  double c1 = ((gm1 - 1) * s2 / 2 - (gm1 + 1) / gm1 * pres / rho) * v1;
+  double c2 = (gm1 + 1) / gm1 * pres / rho + s2 / 2 - gm1 * v1 * v1;
+  f[0][0] = 0.; f[0][1] = 1.; f[0][2] = 0.; f[0][3] = 0.; f[0][4] = 0.;
+  f[1][0] = gm1 * s2 / 2 - v1 * v1; f[1][1] = (2. - gm1) * v1;
+  f[1][2] = -gm1 * v2; f[1][3] = -gm1 * v3; f[1][4] = gm1;
+  f[2][0] = c1; f[2][1] = c2; f[2][2] = -gm1 * v2 * v1;
+  f[2][3] = -gm1 * v3 * v1; f[2][4] = (gm1 + 1) * v1;
+}
+
+// rows {rho, m1, E} of |A| = R Lambda R^-1 at the Roe average of wl, wr
+// (flux_decomposition_impl.h:13-103)
+__device__ __forceinline__ void roe_rows3(double out[3][5], double gm1, double const* wl,
+                                          double const* wr) {
+  double sdl = sqrt(wl[IDN]), sdr = sqrt(wr[IDN]);
+  double is = 1.0 / (sdl + sdr);
+  double pr[5];
+  pr[IDN] = sdl * sdr;
+  for (int v = IVX; v <= IVZ; ++v) pr[v] = (sdl * wl[v] + sdr * wr[v]) * is;
+  double el = wl[IPR] / gm1 +
+              0.5 * wl[IDN] * (wl[IVX] * wl[IVX] + wl[IVY] * wl[IVY] + wl[IVZ] * wl[IVZ]);
+  double er = wr[IPR] / gm1 +
+              0.5 * wr[IDN] * (wr[IVX] * wr[IVX] + wr[IVY] * wr[IVY] + wr[IVZ] * wr[IVZ]);
+  double hbar = ((el + wl[IPR]) / sdl + (er + wr[IPR]) / sdr) * is;
+  pr[IPR] = (hbar - 0.5 * (pr[IVX] * pr[IVX] + pr[IVY] * pr[IVY] + pr[IVZ] * pr[IVZ])) *
+            gm1 / (gm1 + 1.) * pr[IDN];
+  double cs = sqrt(pr[IPR] * (gm1 + 1.) / pr[IDN]);
+  double r = pr[IDN], u = pr[IVX], v = pr[IVY], w = pr[IVZ], p = pr[IPR];
+  double ke = 0.5 * (u * u + v * v + w * w);
+  double hp = (gm1 + 1.) / gm1 * p / r;
+  double h = hp + ke;
+  double lam[5] = {fabs(u - cs), fabs(u), fabs(u + cs), fabs(u), fabs(u)};
+  double R[3][5] = {{1., 1., 1., 0., 0.},
+                    {u - cs, u, u + cs, 0., 0.},
+                    {h - u * cs, ke, h + u * cs, v, w}};
+  double Ri[5][5] = {
+      {(cs * ke + u * hp) / (2. * cs * hp), (-hp - cs * u) / (2. * cs * hp),
+       -v / (2. * hp), -w / (2. * hp), 1. / (2. * hp)},
+      {(hp - ke) / hp, u / hp, v / hp, w / hp, -1. / hp},
+      {(cs * ke - u * hp) / (2. * cs * hp), (hp - cs * u) / (2. * cs * hp),
+       -v / (2. * hp), -w / (2. * hp), 1. / (2. * hp)},
+      {-v, 0., 1., 0., 0.},
+      {-w, 0., 0., 1., 0.}};
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 5; ++b) {
+      double s = 0.;
+      for (int k = 0; k < 5; ++k) s += (R[a][k] * lam[k]) * Ri[k][b];
+      out[a][b] = s;
+    }
+}
+
+// the 5x5 row/column indices the partial blocks keep: rho, m1, E
+__device__ __forceinline__ int full_col(int s) { return s == 2 ? IPR : s; }
+
+// ---------------------------------------------------------------------------
+// assembly: one thread per cell
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+    vicp_assemble_kernel(double const* __restrict__ w, double const* __restrict__ du,
+                         double const* __restrict__ vtab, double* __restrict__ ws,
+                         PartArgs a) {
+  Range3 const& r = a.r;
+  int n = a.n;
+  int64_t ncol = int64_t(r.n2) * r.n3;
+  // a block covers kTileC columns x kTileL layers, layers fastest: the state
+  // reads of 8 lanes are one 64-byte run along x1, the record writes of a
+  // layer are kTileC consecutive columns
+  int li = threadIdx.x % kTileL, ci = threadIdx.x / kTileL;
+  int64_t col = int64_t(blockIdx.x) * kTileC + ci;
+  int i = int(blockIdx.y) * kTileL + li;
+  if (col >= ncol || i >= n) return;
+  int j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
+  int64_t row = k * r.s3 + j * r.s2 + r.lo1;
+  int nc1 = r.nc1, lo1 = r.lo1;
+  double const gm1 = a.gamma - 1.;
+  double const grav = a.grav, A = a.area, V = a.vol;
+  double const dt = stage_dt(a);
+  double const hiv = 0.5 / V;
+  auto W = [&](int v, int l) { return w[v * r.sv + row + l]; };
+  auto DU = [&](int v, int l) { return du[v * r.sv + row + l]; };
+  auto tab = [&](int rowi, int l) { return vtab[rowi * nc1 + lo1 + l]; };
+
+  double wl[5], wc[5], wr[5];
+  for (int v = 0; v < 5; ++v) {
+    wl[v] = W(v, i - 1);
+    wc[v] = W(v, i);
+    wr[v] = W(v, i + 1);
+  }
+  double am[3][5], ap[3][5], dp[3][5], dc[3][5], dn[3][5];
+  double gme = 0.5 * (a.gamma + a.gamma) - 1.;
+  roe_rows3(am, gme, wl, wc);
+  roe_rows3(ap, gme, wc, wr);
+  jac_rows(dp, gm1, wl);
+  jac_rows(dc, gm1, wc);
+  jac_rows(dn, gm1, wr);
+  double aa[3][3], bb[3][3], cc[3][3];
+  for (int p = 0; p < 3; ++p)
+    for (int s = 0; s < 3; ++s) {
+      int q = full_col(s);
+      aa[p][s] = (am[p][q] * A + ap[p][q] * A + (A - A) * dc[p][q]) * hiv;
+      bb[p][s] = -(am[p][q] + dp[p][q]) * A * hiv;
+      cc[p][s] = -(ap[p][q] - dn[p][q]) * A * hiv;
+    }
+  aa[1][0] -= grav;  // - Phi (Phi(m1, rho) = Phi(E, m1) = grav)
+  aa[2][1] -= grav;
+  for (int p = 0; p < 3; ++p) aa[p][p] += 1. / dt;
+  double lo = tab(V_WLO, i), up = tab(V_WHI, i);
+  for (int s = 0; s < 3; ++s) {
+    int q = full_col(s);
+    aa[2][s] -= grav * (up * ap[0][q] - lo * am[0][q]);
+    bb[2][s] -= grav * lo * am[0][q];
+    cc[2][s] += grav * up * ap[0][q];
+  }
+  aa[2][1] += grav * (1. - lo - up);
+  bb[2][1] -= grav * lo;
+  cc[2][1] -= grav * up;
+  bool merged_lo = i == 0, merged_hi = i == n - 1;
+  if (merged_lo)
+    for (int p = 0; p < 3; ++p)
+      for (int s = 0; s < 3; ++s) aa[p][s] += bb[p][s] * (s == 1 ? -1. : 1.);
+  if (merged_hi)
+    for (int p = 0; p < 3; ++p)
+      for (int s = 0; s < 3; ++s) aa[p][s] += cc[p][s] * (s == 1 ? -1. : 1.);
+  // the tangential columns (m2, m3) of the same rows, and the fold
+  double aT[3][2], bT[3][2], cT[3][2];
+  for (int p = 0; p < 3; ++p)
+    for (int kk = 0; kk < 2; ++kk) {
+      int q = IVY + kk;
+      aT[p][kk] = (am[p][q] * A + ap[p][q] * A + (A - A) * dc[p][q]) * hiv;
+      bT[p][kk] = -(am[p][q] + dp[p][q]) * A * hiv;
+      cT[p][kk] = -(ap[p][q] - dn[p][q]) * A * hiv;
+    }
+  for (int kk = 0; kk < 2; ++kk) {
+    int q = IVY + kk;
+    aT[2][kk] -= grav * (up * ap[0][q] - lo * am[0][q]);
+    bT[2][kk] -= grav * lo * am[0][q];
+    cT[2][kk] += grav * up * ap[0][q];
+  }
+  if (merged_lo)
+    for (int p = 0; p < 3; ++p)
+      for (int kk = 0; kk < 2; ++kk) aT[p][kk] += bT[p][kk];
+  if (merged_hi)
+    for (int p = 0; p < 3; ++p)
+      for (int kk = 0; kk < 2; ++kk) aT[p][kk] += cT[p][kk];
+  double corr[3] = {0., 0., 0.};
+  auto fold = [&](double (*blk)[3], double (*tb)[2], int jcell) {
+    double drho = DU(IDN, jcell);
+    for (int kk = 0; kk < 2; ++kk) {
+      double v = W(IVY + kk, jcell);
+      double q = DU(IVY + kk, jcell) - v * drho;
+      if (v != 0.)
+        for (int p = 0; p < 3; ++p) blk[p][0] += tb[p][kk] * v;
+      if (q != 0.)
+        for (int p = 0; p < 3; ++p) corr[p] += tb[p][kk] * q;
+    }
+  };
+  fold(aa, aT, i);
+  if (!merged_lo) fold(bb, bT, i - 1);
+  if (!merged_hi) fold(cc, cT, i + 1);
+  // couple_radial_exact (implicit_hydro.cpp:408-446): E row, rho column
+  aa[2][0] -= tab(V_KMID, i) / dt;
+  bb[2][0] -= tab(V_KLO, i) / dt;
+  cc[2][0] -= tab(V_KHI, i) / dt;
+  // right-hand side: du / dt, the energy without the explicit corrected PE
+  double rows = tab(V_KMID, i) * DU(IDN, i);
+  if (i > 0) rows = rows + tab(V_KLO, i) * DU(IDN, i - 1);
+  if (i < n - 1) rows = rows + tab(V_KHI, i) * DU(IDN, i + 1);
+  if (i == 0) rows = rows + a.t0 * DU(IDN, 2);
+  if (i == n - 1) rows = rows + a.t1 * DU(IDN, n - 3);
+  double rhs[3] = {DU(IDN, i) / dt, DU(IVX, i) / dt, (DU(IPR, i) - rows) / dt};
+  for (int p = 0; p < 3; ++p)
+    if (corr[p] != 0.) rhs[p] -= corr[p];
+  // value e of layer i of column col at ws[(i * kRec + e) * ncol + col]
+  double* rec = ws + int64_t(i) * kRec * ncol + col;
+  for (int p = 0; p < 3; ++p)
+    for (int s = 0; s < 3; ++s) {
+      rec[(R_A + 3 * p + s) * ncol] = aa[p][s];
+      rec[(R_B + 3 * p + s) * ncol] = bb[p][s];
+      rec[(R_C + 3 * p + s) * ncol] = cc[p][s];
+    }
+  for (int p = 0; p < 3; ++p) rec[(R_RHS + p) * ncol] = rhs[p];
+}
+
+// ---------------------------------------------------------------------------
+// the column program: one thread per column
+// ---------------------------------------------------------------------------
+// ludcmp's singularity test for a 3x3 block with implicit row scaling
+// (src/math/ludcmp.h:31-95): false if the matrix is singular or non-finite
+__device__ __forceinline__ bool ludcmp3_ok(double const (&m)[3][3]) {
+  double a[3][3], vv[3], scale[3];
+  const double tol = 8. * 3. * 2.220446049250313e-16;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      a[i][j] = m[i][j];
+      if (!isfinite(a[i][j])) return false;
+    }
+  for (int i = 0; i < 3; ++i) {
+    double big = 0.;
+    for (int j = 0; j < 3; ++j) big = fabs(a[i][j]) > big ? fabs(a[i][j]) : big;
+    if (big == 0.) return false;
+    scale[i] = big;
+    vv[i] = 1.0 / big;
+  }
+  for (int j = 0; j < 3; ++j) {
+    for (int i = 0; i < j; ++i) {
+      double sum = a[i][j];
+      for (int k = 0; k < i; ++k) sum -= a[i][k] * a[k][j];
+      a[i][j] = sum;
+    }
+    double big = 0.;
+    int imax = j;
+    for (int i = j; i < 3; ++i) {
+      double sum = a[i][j];
+      for (int k = 0; k < j; ++k) sum -= a[i][k] * a[k][j];
+      a[i][j] = sum;
+      double dum = vv[i] * fabs(sum);
+      if (!isfinite(dum)) return false;
+      if (dum >= big) {
+        big = dum;
+        imax = i;
+      }
+    }
+    if (j != imax) {
+      for (int k = 0; k < 3; ++k) {
+        double d = a[imax][k];
+        a[imax][k] = a[j][k];
+        a[j][k] = d;
+      }
+      vv[imax] = vv[j];
+      double t = scale[imax];
+      scale[imax] = scale[j];
+      scale[j] = t;
+    }
+    if (fabs(a[j][j]) / scale[j] <= tol) return false;
+    if (j != 2) {
+      double dum = 1.0 / a[j][j];
+      for (int i = j + 1; i < 3; ++i) a[i][j] *= dum;
+    }
+  }
+  return true;
+}
+
+// Eigen's 3x3 inverse by cofactors (Eigen 3.4, InverseImpl.h)
+__device__ __forceinline__ void inverse3(double const (&m)[3][3], double (&o)[3][3]) {
+  double cof00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+  double cof10 = m[2][1] * m[0][2] - m[2][2] * m[0][1];
+  double cof20 = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+  double det = cof00 * m[0][0] + cof10 * m[1][0] + cof20 * m[2][0];
+  double inv = 1.0 / det;
+  o[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) * inv;
+  o[1][1] = (m[2][2] * m[0][0] - m[2][0] * m[0][2]) * inv;
+  o[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * inv;
+  o[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * inv;
+  o[2][1] = (m[2][0] * m[0][1] - m[2][1] * m[0][0]) * inv;
+  o[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * inv;
+  o[0][0] = cof00 * inv;
+  o[0][1] = cof10 * inv;
+  o[0][2] = cof20 * inv;
+}
+
+__device__ __forceinline__ void mv3(double const (&m)[3][3], double const* v, double* o) {
+  for (int r = 0; r < 3; ++r) o[r] = m[r][0] * v[0] + m[r][1] * v[1] + m[r][2] * v[2];
+}
+
+// record value e of layer l of column col: ws[(l * kRec + e) * ncol + col], so
+// a wavefront of consecutive columns reads one value as 64 consecutive doubles
+struct Rec {
+  double* p;
+  int64_t s;
+  __device__ double& operator[](int e) const { return p[e * s]; }
+};
+
+// the serial part of the column program, one thread per column: the forward
+// sweep over the assembled records (each rewritten in place with A' = a^-1 c
+// and the three partial solutions) and the back substitution
+__global__ void __launch_bounds__(64)
+    vicp_sweep_kernel(double* __restrict__ ws, int* __restrict__ status, PartArgs a) {
+  Range3 const& r = a.r;
+  int64_t col = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t ncol = int64_t(r.n2) * r.n3;
+  if (col >= ncol) return;
+  int n = a.n;
+  auto REC = [&](int l) { return Rec{ws + int64_t(l) * kRec * ncol + col, ncol}; };
+
+  // ---- forward sweep (ForwardSweep<2>, the small-matrix path) --------------
+  double ap[3][3], yp[3], z0p[3], z1p[3];
+  bool ok = true;
+  for (int i = 0; i < n && ok; ++i) {
+    Rec rec = REC(i);
+    double m[3][3], bb[3][3], c3[3][3], rhs[3];
+    for (int p = 0; p < 3; ++p) {
+      for (int s = 0; s < 3; ++s) {
+        m[p][s] = rec[R_A + 3 * p + s];
+        bb[p][s] = rec[R_B + 3 * p + s];
+        c3[p][s] = rec[R_C + 3 * p + s];
+      }
+      rhs[p] = rec[R_RHS + p];
+    }
+    double e0[3] = {0., 0., i == 0 ? 1. : 0.};
+    double e1[3] = {0., 0., i == n - 1 ? 1. : 0.};
+    if (i > 0)
+      for (int p = 0; p < 3; ++p)
+        for (int s = 0; s < 3; ++s)
+          m[p][s] -= bb[p][0] * ap[0][s] + bb[p][1] * ap[1][s] + bb[p][2] * ap[2][s];
+    if (!ludcmp3_ok(m)) {
+      ok = false;
+      break;
+    }
+    double inv[3][3];
+    inverse3(m, inv);
+    double y[3], z0[3], z1[3];
+    if (i > 0) {
+      double t[3], s0[3], s1[3], by[3], b0[3], b1[3];
+      mv3(bb, yp, by);
+      mv3(bb, z0p, b0);
+      mv3(bb, z1p, b1);
+      for (int p = 0; p < 3; ++p) {
+        t[p] = rhs[p] - by[p];
+        s0[p] = e0[p] - b0[p];
+        s1[p] = e1[p] - b1[p];
+      }
+      mv3(inv, t, y);
+      mv3(inv, s0, z0);
+      mv3(inv, s1, z1);
+    } else {
+      mv3(inv, rhs, y);
+      mv3(inv, e0, z0);
+      mv3(inv, e1, z1);
+    }
+    for (int p = 0; p < 3; ++p)
+      for (int s = 0; s < 3; ++s)
+        ap[p][s] = inv[p][0] * c3[0][s] + inv[p][1] * c3[1][s] + inv[p][2] * c3[2][s];
+    for (int p = 0; p < 3; ++p) {
+      if (!isfinite(y[p])) ok = false;
+      for (int s = 0; s < 3; ++s)
+        if (!isfinite(ap[p][s])) ok = false;
+      yp[p] = y[p];
+      z0p[p] = z0[p];
+      z1p[p] = z1[p];
+    }
+    for (int p = 0; p < 3; ++p) {
+      for (int s = 0; s < 3; ++s) rec[R_A + 3 * p + s] = ap[p][s];
+      rec[R_Y + p] = y[p];
+      rec[R_Z0 + p] = z0[p];
+      rec[R_Z1 + p] = z1[p];
+    }
+  }
+  // a failed column keeps its explicit increments (the finish kernel reads
+  // the flag; the reference undoes the column with its status bits and
+  // flags a retry)
+  status[col] = ok ? 0 : 1;
+  if (!ok) return;
+  // ---- back substitution (vic_backward_substitute) -------------------------
+  {
+    double yn[3], z0n[3], z1n[3];
+    Rec last = REC(n - 1);
+    for (int p = 0; p < 3; ++p) {
+      yn[p] = last[R_Y + p];
+      z0n[p] = last[R_Z0 + p];
+      z1n[p] = last[R_Z1 + p];
+    }
+    for (int i = n - 2; i >= 0; --i) {
+      Rec rec = REC(i);
+      double A3[3][3];
+      for (int p = 0; p < 3; ++p)
+        for (int s = 0; s < 3; ++s) A3[p][s] = rec[R_A + 3 * p + s];
+      double sy[3], s0[3], s1[3];
+      mv3(A3, yn, sy);
+      mv3(A3, z0n, s0);
+      mv3(A3, z1n, s1);
+      for (int p = 0; p < 3; ++p) {
+        yn[p] = rec[R_Y + p] - sy[p];
+        z0n[p] = rec[R_Z0 + p] - s0[p];
+        z1n[p] = rec[R_Z1 + p] - s1[p];
+        rec[R_Y + p] = yn[p];
+        rec[R_Z0 + p] = z0n[p];
+        rec[R_Z1 + p] = z1n[p];
+      }
+    }
+  }
+}
+
+// the rest of the column program, one wavefront per column with the layers
+// across the lanes (the state reads and writes run along x1, coalesced):
+// the Woodbury 2x2, the redistribution's serial chains on lane 0 in the
+// column's order (the sums, the prefix of the face transfers, the
+// availability clamps), then every cell's outputs, the tangential
+// transfers, the remainder and the RK combine.  Dynamic LDS: 9 (n + 1)
+// doubles.
+template <bool PAIR>
+__global__ void __launch_bounds__(64)
+    vicp_finish_kernel(double const* __restrict__ w, double const* __restrict__ du,
+                       void const* __restrict__ u0v, float const* __restrict__ c0,
+                       void* __restrict__ uv, float* __restrict__ cc,
+                       double const* __restrict__ vtab, double const* __restrict__ xtab,
+                       double* __restrict__ ws, int const* __restrict__ status, PartArgs a) {
+  extern __shared__ double lds[];
+  Range3 const& r = a.r;
+  int64_t ncol = int64_t(r.n2) * r.n3;
+  int64_t col = blockIdx.x;
+  int lane = threadIdx.x;
+  int j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
+  int64_t row = k * r.s3 + j * r.s2 + r.lo1;
+  int n = a.n, nc1 = r.nc1, lo1 = r.lo1;
+  double const V = a.vol;
+  double const dt = stage_dt(a);
+  auto W = [&](int v, int l) { return w[v * r.sv + row + l]; };
+  auto DU = [&](int v, int l) { return du[v * r.sv + row + l]; };
+  auto REC = [&](int l) { return Rec{ws + int64_t(l) * kRec * ncol + col, ncol}; };
+  auto tab = [&](double const* t, int rowi, int l) { return t[rowi * nc1 + lo1 + l]; };
+  auto combine = [&](int v, int l, double x) {
+    int64_t o = v * r.sv + row + l;
+    if constexpr (PAIR) {
+      double y = a.w0 * (double(static_cast<float const*>(u0v)[o]) + double(c0[o])) +
+                 a.w1 * (double(static_cast<float*>(uv)[o]) + double(cc[o])) + a.w2 * x;
+      float h = float(y);
+      static_cast<float*>(uv)[o] = h;
+      cc[o] = float(y - double(h));
+    } else {
+      double* u = static_cast<double*>(uv);
+      u[o] = a.w0 * static_cast<double const*>(u0v)[o] + a.w1 * u[o] + a.w2 * x;
+    }
+  };
+  if (status[col] != 0) {
+    for (int l = lane; l < n; l += 64)
+      for (int v = 0; v < 5; ++v) combine(v, l, DU(v, l));
+    return;
+  }
+  int const L = n + 1;
+  double* s_phi = lds;          // phi_i
+  double* s_wv = lds + L;       // w_rho V
+  double* s_av = lds + 2 * L;   // (w_rho + du_rho) V
+  double* s_M = lds + 3 * L;    // requested transfer through the face below i
+  double* s_q = lds + 4 * L;    // moved transfer through the face below i
+  double* s_y0 = lds + 5 * L;   // the solved total density increment
+  double* s_d0 = lds + 6 * L;   // du_rho
+  double* s_vy = lds + 7 * L;   // w_v2, w_v3 (the donor velocities)
+  double* s_vz = lds + 8 * L;
+  // ---- Woodbury 2x2 (vic_woodbury_column) ----------------------------------
+  double al0, al1;
+  {
+    double k0 = -a.t0 / dt, k1 = -a.t1 / dt;
+    int i0 = 2, i1 = n - 3;
+    Rec r0p = REC(i0);
+    Rec r1p = REC(i1);
+    double m00 = 1. + k0 * r0p[R_Z0], m01 = k0 * r0p[R_Z1];
+    double m10 = k1 * r1p[R_Z0], m11 = 1. + k1 * r1p[R_Z1];
+    double r0 = k0 * r0p[R_Y], r1 = k1 * r1p[R_Y];
+    double det = m00 * m11 - m01 * m10;
+    al0 = (m11 * r0 - m01 * r1) / det;
+    al1 = (m00 * r1 - m10 * r0) / det;
+  }
+  auto Y = [&](int l, int p) {
+    Rec rec = REC(l);
+    double y = rec[R_Y + p];
+    y -= rec[R_Z0 + p] * al0;
+    y -= rec[R_Z1 + p] * al1;
+    return y;
+  };
+  for (int l = lane; l < n; l += 64) {
+    double y0 = Y(l, 0), d0 = DU(IDN, l), wr = W(IDN, l);
+    s_y0[l] = y0;
+    s_d0[l] = d0;
+    s_phi[l] = (y0 - d0) * V;
+    s_wv[l] = wr * V;
+    s_av[l] = (wr * 1. + d0) * V;
+    s_vy[l] = W(IVY, l);
+    s_vz[l] = W(IVZ, l);
+  }
+  __syncthreads();
+  // ---- the serial chains of vic_constituent_column, in the column's order --
+  if (lane == 0) {
+    double R = 0., S = 0.;
+    for (int i = 0; i < n; ++i) {
+      R += s_phi[i];
+      S += s_wv[i];
+    }
+    double M = 0.;
+    for (int i = 0; i < n; ++i) {
+      s_M[i] = M;
+      M -= s_phi[i] - R * (s_wv[i] / S);
+    }
+    s_M[n] = 0.;
+    s_q[0] = 0.;
+    s_q[n] = 0.;
+    double avail = s_av[0];
+    if (avail < 0) avail = 0;
+    for (int i = 0; i + 1 < n; ++i) {
+      double avail_up = s_av[i + 1];
+      if (avail_up < 0) avail_up = 0;
+      double mf = s_M[i + 1];
+      double q = mf > 0 ? mf * 1. : mf * 1.;
+      if (q > avail) q = avail;
+      if (q < -avail_up) q = -avail_up;
+      s_q[i + 1] = q;
+      avail = avail_up + q;
+    }
+  }
+  __syncthreads();
+  // ---- outputs, the tangential transfers, remainder, RK combine ------------
+  // the mass change of cell l: the face below adds, the face above removes
+  auto dm = [&](int l) { return (0. + s_q[l] / V) - s_q[l + 1] / V; };
+  auto moved_rho = [&](int l) { return (s_d0[l] + dm(l)) - s_d0[l]; };
+  auto raw_minus_mass0 = [&](int l) { return s_y0[l] - s_d0[l]; };
+  auto rows_at = [&](int l, auto q) {
+    double s = tab(vtab, V_KMID, l) * q(l);
+    if (l > 0) s = s + tab(vtab, V_KLO, l) * q(l - 1);
+    if (l < n - 1) s = s + tab(vtab, V_KHI, l) * q(l + 1);
+    if (l == 0) s = s + a.t0 * q(2);
+    if (l == n - 1) s = s + a.t1 * q(n - 3);
+    return s;
+  };
+  for (int i = lane; i < n; i += 64) {
+    double out[5];
+    out[IDN] = s_d0[i] + dm(i);
+    out[IVX] = Y(i, 1);
+    out[IPR] = Y(i, 2);
+    // vic_tangential_momentum_column: the face below adds (its upper cell),
+    // then the face above removes (its lower cell), in the column's order
+    double q_lo = s_q[i], q_hi = s_q[i + 1];
+    for (int m = IVY; m <= IVZ; ++m) {
+      double const* vm = m == IVY ? s_vy : s_vz;
+      double x = DU(m, i);
+      if (i > 0) {
+        double pv = q_lo * vm[q_lo > 0 ? i - 1 : i];
+        if (pv != 0.) x = x + pv / V;
+      }
+      if (i + 1 < n) {
+        double pv = q_hi * vm[q_hi > 0 ? i : i + 1];
+        if (pv != 0.) x = x - pv / V;
+      }
+      out[m] = x;
+    }
+    double req_lo = s_M[i], req_hi = s_M[i + 1];
+    double mv_lo = s_q[i] - req_lo;
+    double mv_hi = s_q[i + 1] - req_hi;
+    double projected = (req_lo - req_hi) / V;
+    double pw = tab(vtab, V_PHI, i) * (s_y0[i] - s_d0[i] - projected);
+    double cw = -(tab(vtab, V_DPHI, i) * mv_hi - tab(vtab, V_DPLO, i) * mv_lo) / V;
+    out[IPR] = out[IPR] + (pw + cw);
+    int jj = (i < 1 ? 1 : (i > n - 2 ? n - 2 : i)) - 1;
+    double s = tab(xtab, T_WT0, i) * moved_rho(jj);
+    s = s + tab(xtab, T_WT1, i) * moved_rho(jj + 1);
+    s = s + tab(xtab, T_WT2, i) * moved_rho(jj + 2);
+    double work = tab(xtab, T_GVAR, i) * s;
+    work = work - rows_at(i, raw_minus_mass0);
+    out[IPR] = out[IPR] + work;
+    for (int v = 0; v < 5; ++v) combine(v, i, out[v]);
+  }
+}
+
+}  // namespace
+}  // namespace snapy
+
+using namespace snapy;
+
+extern "C" {
+
+static int vicp_launch(const snapy_grid* g, const snapy_physics* p, const double* w,
+                       const double* du, const void* u0, const float* c0, void* u,
+                       float* c, const double* vtab, const double* x1tab, double t0,
+                       double t1, double dt, const double* dt_dev, double wstage,
+                       double w0, double w1, double w2, double* ws, int* status,
+                       void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (!p || !w || !du || !u0 || !u || !vtab || !x1tab || !ws || !status)
+    return fail(-1, "vic_partial: NULL argument");
+  bool pair = c0 != nullptr;
+  if (pair && !c) return fail(-1, "vic_partial: the FP32 pair needs c0 and c");
+  if (g->nx1 < 5) return fail(-9, "vic_partial: needs >= 5 x1 cells");
+  PartArgs a;
+  a.r = ranges(*g);
+  a.n = g->nx1;
+  a.gamma = p->gammad;
+  a.grav = p->grav1;
+  a.area = g->dx3 * g->dx2;
+  a.vol = g->dx3 * (g->dx2 * g->dx1);
+  a.t0 = t0;
+  a.t1 = t1;
+  a.dt_dev = dt_dev;
+  a.dt_host = dt;
+  a.wstage = wstage;
+  a.w0 = w0;
+  a.w1 = w1;
+  a.w2 = w2;
+  int64_t ncol = int64_t(g->nx2) * g->nx3;
+  int64_t ncell = ncol * g->nx1;
+  (void)ncell;
+  dim3 agrid(blocks_for(ncol, kTileC), blocks_for(g->nx1, kTileL));
+  hipLaunchKernelGGL(vicp_assemble_kernel, agrid, dim3(kTileC * kTileL), 0,
+                     as_stream(stream), w, du, vtab, ws, a);
+  if (int e = launch_status("vic_partial_assemble")) return e;
+  hipLaunchKernelGGL(vicp_sweep_kernel, dim3(blocks_for(ncol, 64)), dim3(64), 0,
+                     as_stream(stream), ws, status, a);
+  if (int e = launch_status("vic_partial_sweep")) return e;
+  size_t lds = size_t(9) * (g->nx1 + 1) * sizeof(double);
+  if (lds > 64 * 1024) return fail(-9, "vic_partial: the column does not fit the LDS");
+  if (pair)
+    hipLaunchKernelGGL(vicp_finish_kernel<true>, dim3(ncol), dim3(64), lds, as_stream(stream), w,
+                       du, u0, c0, u, c, vtab, x1tab, ws, status, a);
+  else
+    hipLaunchKernelGGL(vicp_finish_kernel<false>, dim3(ncol), dim3(64), lds, as_stream(stream),
+                       w, du, u0, c0, u, c, vtab, x1tab, ws, status, a);
+  return launch_status("vic_partial");
+}
+
+int64_t snapy_vic_partial_workspace(const snapy_grid* g) {
+  if (!g) return 0;
+  return int64_t(kRec) * g->nx1 * g->nx2 * g->nx3;
+}
+
+int snapy_vic_partial(const snapy_grid* g, const snapy_physics* p, const double* w,
+                      const double* du, const double* u0, double* u, const double* vtab,
+                      const double* x1tab, double t0, double t1, double dt,
+                      const double* dt_dev, double wstage, double w0, double w1,
+                      double w2, double* ws, int* status, void* stream) {
+  return vicp_launch(g, p, w, du, u0, nullptr, u, nullptr, vtab, x1tab, t0, t1, dt, dt_dev,
+                     wstage, w0, w1, w2, ws, status, stream);
+}
+
+int snapy_vic_partial_f32(const snapy_grid* g, const snapy_physics* p, const double* w,
+                          const double* du, const float* u0, const float* c0, float* u,
+                          float* c, const double* vtab, const double* x1tab, double t0,
+                          double t1, double dt, const double* dt_dev, double wstage,
+                          double w0, double w1, double w2, double* ws, int* status,
+                          void* stream) {
+  if (!c0 || !c) return fail(-1, "vic_partial_f32: NULL compensation");
+  return vicp_launch(g, p, w, du, u0, c0, u, c, vtab, x1tab, t0, t1, dt, dt_dev, wstage, w0,
+                     w1, w2, ws, status, stream);
+}
+
+}  // extern "C"

@@ -190,6 +190,12 @@ class MeshBlock:
                 out.append(slice(lo, lo + nx + ext))
         return tuple(out)
 
+    def _vic_workspace(self, lib) -> int:
+        """FP64 workspace doubles of the deck's implicit scheme."""
+        if self.options.intg.implicit_scheme == 1:
+            return lib.snapy_vic_partial_workspace(ctypes.byref(self._grid))
+        return lib.snapy_vic_workspace(ctypes.byref(self._grid))
+
     # -------------------------------------------------------------- devices
     def to(self, device, dtype=None):
         device = torch.device(device)
@@ -222,9 +228,9 @@ class MeshBlock:
             total += 5 * cells * 8
         if h.nu_iso > 0.0 or h.kappa_iso > 0.0:
             total += 12 * cells * 8 + ncell * 8
-        if self.options.intg.implicit_scheme == 9:
+        if self.options.intg.implicit_scheme in (1, 9):
             lib = _capi.load()
-            total += int(lib.snapy_vic_workspace(ctypes.byref(self._grid))) * 8 + ncell * 8
+            total += int(self._vic_workspace(lib)) * 8 + ncell * 8
             total += c.nx2 * c.nx3 * 4
         return int(total)
 
@@ -260,7 +266,7 @@ class MeshBlock:
                 "du": torch.zeros(self.shape(), dtype=torch.float64, device=self.device),
             }
         self._vic = None
-        if self.options.intg.implicit_scheme == 9:
+        if self.options.intg.implicit_scheme in (1, 9):
             from .geometry import vic_table
 
             cb = self._coord
@@ -268,7 +274,7 @@ class MeshBlock:
             vol = c.dx(3) * (c.dx(2) * c.dx(1))
             tab, t0, t1 = vic_table(cb.buffer("x1f"), cb.buffer("x1v"), c.nghost, c.nx1, h.grav1, a1, vol)
             lib = _capi.load()
-            nws = int(lib.snapy_vic_workspace(ctypes.byref(self._grid)))
+            nws = int(self._vic_workspace(lib))
             self._vic = {
                 "tab": tab.to(self.device),
                 "t0": t0,
@@ -276,6 +282,8 @@ class MeshBlock:
                 "ws": torch.empty(nws, dtype=torch.float64, device=self.device),
                 "status": torch.zeros(c.nx2 * c.nx3, dtype=torch.int32, device=self.device),
                 "du": torch.zeros(self.shape(), dtype=torch.float64, device=self.device),
+                "scheme": int(self.options.intg.implicit_scheme),
+                "name": "vic_partial" if self.options.intg.implicit_scheme == 1 else "vic_full",
             }
 
     def flux_register(self) -> FluxRegister:
@@ -451,10 +459,11 @@ class MeshBlock:
                         ),
                     )
                 )
+                vic_fn = lib.snapy_vic_partial_f32 if vic["scheme"] == 1 else lib.snapy_vic_full_f32
                 calls.append(
                     (
-                        "vic_full",
-                        lambda: lib.snapy_vic_full_f32(
+                        vic["name"],
+                        lambda: vic_fn(
                             g, p, P(w), P(vic["du"]), P(u0), P(c0), P(u), P(c), P(vic["tab"]), P(self._x1tab),
                             vic["t0"], vic["t1"], 0.0, dtp, st.wght2(), st.wght0(), st.wght1(), st.wght2(),
                             P(vic["ws"]), P(vic["status"]), s,
@@ -493,10 +502,11 @@ class MeshBlock:
             )
             # the implicit solve takes the stage weight inside its dt and
             # finishes the RK combine
+            vic_fn = lib.snapy_vic_partial if vic["scheme"] == 1 else lib.snapy_vic_full
             calls.append(
                 (
-                    "vic_full",
-                    lambda: lib.snapy_vic_full(
+                    vic["name"],
+                    lambda: vic_fn(
                         g, p, P(w), P(vic["du"]), P(u0), P(u), P(vic["tab"]), P(self._x1tab), vic["t0"], vic["t1"],
                         0.0, dtp, st.wght2(), st.wght0(), st.wght1(), st.wght2(), P(vic["ws"]), P(vic["status"]), s,
                     ),

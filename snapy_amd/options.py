@@ -359,11 +359,12 @@ def _check_supported(op: MeshBlockOptions, layout: str = "") -> None:
         raise ValueError("forcing/bot-heat is built on the gravity path here (needs grav1)")
     if op.intg.type != "rk3":
         raise ValueError(f"integration/type '{op.intg.type}': rk3 only")
-    if op.intg.implicit_scheme not in (0, 9):
+    if op.intg.implicit_scheme not in (0, 1, 9):
         raise ValueError(
-            f"integration/implicit-scheme {op.intg.implicit_scheme}: scheme 9 (vic-full) is implemented"
+            f"integration/implicit-scheme {op.intg.implicit_scheme}: schemes 1 (vic-partial) and 9 (vic-full) "
+            "are implemented"
         )
-    if op.intg.implicit_scheme == 9:
+    if op.intg.implicit_scheme in (1, 9):
         if op.hydro.grav1 == 0.0:
             raise ValueError("implicit-scheme 9 needs forcing/const-gravity grav1 (ImplicitHydro requires it)")
         if c.nx1 < 5:

@@ -93,6 +93,10 @@ CASES = [
      dict(grav1=-9.81, scheme=9, nu=75.0)),
     # gravity, explicit
     (fv.Grid(24, 20, 1, 0.0, 6400.0, -6400.0, 6400.0, 0.0, 100.0), ("reflecting",) * 6, dict(grav1=-9.81)),
+    # the 3D convection physics with the partial implicit matrix (scheme 1)
+    (fv.Grid(16, 8, 8, 0.0, 800.0, 0.0, 800.0, 0.0, 800.0),
+     ("reflecting", "reflecting", "periodic", "periodic", "periodic", "periodic"),
+     dict(grav1=-9.81, scheme=1, heat=100.0)),
 ]
 
 

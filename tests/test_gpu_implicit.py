@@ -1,5 +1,5 @@
-"""The vertical implicit correction (implicit-scheme 9) on the GPU against
-the CPU FP64 oracle: one kernel call on random columns, then whole 2D/3D
+"""The vertical implicit correction (implicit-schemes 9 and 1) on the GPU
+against the CPU FP64 oracle: one kernel call on random columns, then whole 2D/3D
 blocks with gravity and the implicit x1 over several RK3 steps."""
 
 import ctypes
@@ -17,7 +17,7 @@ from tests.test_gpu_gravity import GAMMA, stratified
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("entry", ["snapy_vic_full", "snapy_vic_full_serial"])
+@pytest.mark.parametrize("entry", ["snapy_vic_full", "snapy_vic_full_serial", "snapy_vic_partial"])
 @pytest.mark.parametrize("shape", [(16, 6, 1), (12, 4, 3), (40, 37, 3)])
 def test_vic_kernel_matches_oracle(shape, entry):
     lib = _capi.load()
@@ -30,13 +30,15 @@ def test_vic_kernel_matches_oracle(shape, entry):
     dt = 0.7
     pd = phys_desc(gammad=GAMMA, riemann="lmars", shock_x1=False, shock_x23=False, bcs=("reflecting",) * 6,
                    grav1=-9.81)
-    pd.implicit_scheme = 9
+    scheme = 1 if entry == "snapy_vic_partial" else 9
+    pd.implicit_scheme = scheme
     gd = grid_desc(g)
     x1f, x1v = torch.from_numpy(g.faces(1)), torch.from_numpy(g.centers(1))
     a1, vol = g.dx(3) * g.dx(2), g.dx(3) * (g.dx(2) * g.dx(1))
     tab, t0, t1 = vic_table(x1f, x1v, 3, g.nx1, -9.81, a1, vol)
     xt = x1_gravity_table(x1f, x1v, 3, g.nx1, -9.81)
-    ws = torch.empty(int(lib.snapy_vic_workspace(ctypes.byref(gd))), dtype=torch.float64, device="cuda")
+    wsize = lib.snapy_vic_partial_workspace if scheme == 1 else lib.snapy_vic_workspace
+    ws = torch.empty(int(wsize(ctypes.byref(gd))), dtype=torch.float64, device="cuda")
     st = torch.zeros(g.nx2 * g.nx3, dtype=torch.int32, device="cuda")
     u0 = torch.zeros(w.shape, dtype=torch.float64, device="cuda")
     u = torch.zeros_like(u0)
@@ -55,8 +57,8 @@ def test_vic_kernel_matches_oracle(shape, entry):
     ncol = wc.shape[1] * wc.shape[2]
     wcol = np.moveaxis(wc, 0, -1).reshape(ncol, n + 2, 5)
     dcol = np.moveaxis(du[:, k, j, 3 : 3 + n], 0, -1).reshape(ncol, n, 5)
-    ref = implicit.vic_full(dcol, wcol, GAMMA, -9.81, dt, g.dx(1), g.faces(1)[3 : 3 + n + 1],
-                            g.centers(1)[3 : 3 + n], a1, vol)
+    solve = implicit.vic_partial if scheme == 1 else implicit.vic_full
+    ref = solve(dcol, wcol, GAMMA, -9.81, dt, g.dx(1), g.faces(1)[3 : 3 + n + 1], g.centers(1)[3 : 3 + n], a1, vol)
     gc = np.moveaxis(got[:, k, j, 3 : 3 + n], 0, -1).reshape(ncol, n, 5)
     for v in range(5):
         err = rel_l2(gc[..., v], ref[..., v])
@@ -73,13 +75,14 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("scheme", [9, 1])
 @pytest.mark.parametrize("g,bcs,cov,heat", CASES)
-def test_whole_block_implicit(g, bcs, cov, heat):
+def test_whole_block_implicit(g, bcs, cov, heat, scheme):
     from snapy_amd import MeshBlock
     from snapy_amd.options import CoordinateOptions, HydroOptions, IntegratorOptions, MeshBlockOptions
 
     d = fv.Deck(grid=g, gammad=GAMMA, riemann="lmars", shock_x1=False, shock_x23=False, bcs=bcs,
-                grav1=-9.81, cfl=0.9, implicit_scheme=9, flux_covariance=cov, bot_heat_flux=heat)
+                grav1=-9.81, cfl=0.9, implicit_scheme=scheme, flux_covariance=cov, bot_heat_flux=heat)
     w = stratified(g, seed=4, noise=0.5)
     ref = fv.Block(d)
     ref.initialize(w)
@@ -90,9 +93,10 @@ def test_whole_block_implicit(g, bcs, cov, heat):
     op.hydro.eos.gammad = GAMMA
     op.hydro.recon1.shock = False
     op.hydro.recon23.shock = False
-    op.intg = IntegratorOptions(cfl=0.9, implicit_scheme=9)
+    op.intg = IntegratorOptions(cfl=0.9, implicit_scheme=scheme)
     op.bcnames = list(bcs)
     block = MeshBlock(op)
+    print(f"whole block scheme {scheme} {g.nx1}x{g.nx2}x{g.nx3}", flush=True)
     block.to("cuda")
     vars, _ = block.initialize({"hydro_w": torch.from_numpy(w).cuda()})
     for n in range(3):
