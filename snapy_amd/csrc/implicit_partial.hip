@@ -6,13 +6,12 @@
 // mass carries, the post-solve remainder and the RK combine.
 //
 // Two kernels:
-//   * vicp_assemble_kernel: one thread per cell (x1 fastest, so the state
-//     reads are coalesced), writing the cell's blocks and right-hand side
-//     as one 32-double record (the assembly is cell-local:
-//     vic_assemble_cell, src/implicit/vic_kernels.h:66-105);
-//   * vicp_sweep_kernel: one thread per column, the serial sweep over the
-//     records (rewriting each in place with the factored row and the three
-//     partial solutions) and the back substitution;
+//   * vicp_forward_kernel: one thread per column, 64 columns per block, the
+//     state staged through LDS a few layers at a time with coalesced loads:
+//     each layer's blocks assembled in registers (vic_assemble_cell,
+//     src/implicit/vic_kernels.h:66-105, the face matrices and Jacobians
+//     carried up from the layer below), the forward sweep and the back
+//     substitution;
 //   * vicp_finish_kernel<PAIR>: one wavefront per column, layers across the
 //     lanes: the Woodbury 2x2, the redistribution (its serial chains on one
 //     lane, in the column's order), the outputs and the combine.
@@ -35,13 +34,10 @@
 namespace snapy {
 namespace {
 
-constexpr int kRec = 32;  // doubles per cell record
-constexpr int kTileC = 32, kTileL = 8;  // the assembly's block tile
-// record slots: assembly a 0..8, b 9..17, c 18..26, rhs 27..29; the sweep
-// rewrites 0..8 A' = a^-1 c, 9..11 y, 12..14 z0, 15..17 z1; the
-// redistribution uses 18 phi, 19 M (face below), 20 mass change, 21 moved
-enum { R_A = 0, R_B = 9, R_C = 18, R_RHS = 27, R_Y = 9, R_Z0 = 12, R_Z1 = 15,
-       R_PHI = 18, R_M = 19, R_DM = 20, R_MV = 21 };
+constexpr int kRec = 18;  // doubles per layer record: A' 9, y 3, z0 3, z1 3
+constexpr int kF = kRec;
+// record slots: A' = a^-1 c 0..8, y 9..11, z0 12..14, z1 15..17
+enum { R_A = 0, R_Y = 9, R_Z0 = 12, R_Z1 = 15 };
 
 enum { V_WLO = 0, V_WHI, V_KLO, V_KMID, V_KHI, V_PHI, V_DPLO, V_DPHI };
 enum { T_GVAR = 3, T_WT0 = 4, T_WT1 = 5, T_WT2 = 6 };
@@ -60,50 +56,61 @@ __device__ __forceinline__ double stage_dt(PartArgs const& a) {
   return (a.dt_dev ? *a.dt_dev : a.dt_host) * a.wstage;
 }
 
-// rows {rho, m1, E} of FluxJacobian (flux_decomposition_impl.h:105-126), dir 0
-__device__ __forceinline__ void jac_rows(double f[3][5], double gm1, double const* w) {
-  double v1 = w[IVX], v2 = w[IVY], v3 = w[IVZ], rho = w[IDN], pres = w[IPR];
+// rows {rho, m1, E} of FluxJacobian (flux_decomposition_impl.h:105-126), dir
+// 0; gp_g = (gm1 + 1) / gm1, and p / rho divided once
+__device__ __forceinline__ void jac_rows(double f[3][5], double gm1, double gp_g,
+                                         double const* w) {
+  double v1 = w[IVX], v2 = w[IVY], v3 = w[IVZ];
+  double pr = w[IPR] / w[IDN];
   double s2 = v1 * v1 + v2 * v2 + v3 * v3;
-  double c1 = ((gm1 - 1) * s2 / 2 - (gm1 + 1) / gm1 * pres / rho) * v1;
-  double c2 = (gm1 + 1) / gm1 * pres / rho + s2 / 2 - gm1 * v1 * v1;
+  double c1 = ((gm1 - 1) * s2 * 0.5 - gp_g * pr) * v1;
+  double c2 = gp_g * pr + s2 * 0.5 - gm1 * v1 * v1;
   f[0][0] = 0.; f[0][1] = 1.; f[0][2] = 0.; f[0][3] = 0.; f[0][4] = 0.;
-  f[1][0] = gm1 * s2 / 2 - v1 * v1; f[1][1] = (2. - gm1) * v1;
+  f[1][0] = gm1 * s2 * 0.5 - v1 * v1; f[1][1] = (2. - gm1) * v1;
   f[1][2] = -gm1 * v2; f[1][3] = -gm1 * v3; f[1][4] = gm1;
   f[2][0] = c1; f[2][1] = c2; f[2][2] = -gm1 * v2 * v1;
   f[2][3] = -gm1 * v3 * v1; f[2][4] = (gm1 + 1) * v1;
 }
 
 // rows {rho, m1, E} of |A| = R Lambda R^-1 at the Roe average of wl, wr
-// (flux_decomposition_impl.h:13-103)
-__device__ __forceinline__ void roe_rows3(double out[3][5], double gm1, double const* wl,
+// (flux_decomposition_impl.h:13-103).  The divisions by the constants of
+// the gas and by 2 c hp and hp are taken as products with reciprocals
+// formed once (seven divisions per face instead of twenty); the rows agree
+// with the reference's to rounding.
+struct GasK {
+  double gm1, igm1, gp_g, g_gp;  // gm1, 1 / gm1, (gm1 + 1) / gm1, gm1 / (gm1 + 1)
+};
+
+__device__ __forceinline__ void roe_rows3(double out[3][5], GasK const& K, double const* wl,
                                           double const* wr) {
   double sdl = sqrt(wl[IDN]), sdr = sqrt(wr[IDN]);
   double is = 1.0 / (sdl + sdr);
   double pr[5];
   pr[IDN] = sdl * sdr;
   for (int v = IVX; v <= IVZ; ++v) pr[v] = (sdl * wl[v] + sdr * wr[v]) * is;
-  double el = wl[IPR] / gm1 +
+  double el = wl[IPR] * K.igm1 +
               0.5 * wl[IDN] * (wl[IVX] * wl[IVX] + wl[IVY] * wl[IVY] + wl[IVZ] * wl[IVZ]);
-  double er = wr[IPR] / gm1 +
+  double er = wr[IPR] * K.igm1 +
               0.5 * wr[IDN] * (wr[IVX] * wr[IVX] + wr[IVY] * wr[IVY] + wr[IVZ] * wr[IVZ]);
   double hbar = ((el + wl[IPR]) / sdl + (er + wr[IPR]) / sdr) * is;
   pr[IPR] = (hbar - 0.5 * (pr[IVX] * pr[IVX] + pr[IVY] * pr[IVY] + pr[IVZ] * pr[IVZ])) *
-            gm1 / (gm1 + 1.) * pr[IDN];
-  double cs = sqrt(pr[IPR] * (gm1 + 1.) / pr[IDN]);
+            K.g_gp * pr[IDN];
   double r = pr[IDN], u = pr[IVX], v = pr[IVY], w = pr[IVZ], p = pr[IPR];
+  double hp = K.gp_g * p / r;
+  double cs = sqrt(hp * K.gm1);  // p (gm1 + 1) / rho
   double ke = 0.5 * (u * u + v * v + w * w);
-  double hp = (gm1 + 1.) / gm1 * p / r;
   double h = hp + ke;
+  double ihp = 1. / hp;
+  double i2hp = 0.5 * ihp;
+  double i2cshp = i2hp / cs;
   double lam[5] = {fabs(u - cs), fabs(u), fabs(u + cs), fabs(u), fabs(u)};
   double R[3][5] = {{1., 1., 1., 0., 0.},
                     {u - cs, u, u + cs, 0., 0.},
                     {h - u * cs, ke, h + u * cs, v, w}};
   double Ri[5][5] = {
-      {(cs * ke + u * hp) / (2. * cs * hp), (-hp - cs * u) / (2. * cs * hp),
-       -v / (2. * hp), -w / (2. * hp), 1. / (2. * hp)},
-      {(hp - ke) / hp, u / hp, v / hp, w / hp, -1. / hp},
-      {(cs * ke - u * hp) / (2. * cs * hp), (hp - cs * u) / (2. * cs * hp),
-       -v / (2. * hp), -w / (2. * hp), 1. / (2. * hp)},
+      {(cs * ke + u * hp) * i2cshp, (-hp - cs * u) * i2cshp, -v * i2hp, -w * i2hp, i2hp},
+      {(hp - ke) * ihp, u * ihp, v * ihp, w * ihp, -ihp},
+      {(cs * ke - u * hp) * i2cshp, (hp - cs * u) * i2cshp, -v * i2hp, -w * i2hp, i2hp},
       {-v, 0., 1., 0., 0.},
       {-w, 0., 0., 1., 0.}};
   for (int a = 0; a < 3; ++a)
@@ -118,47 +125,25 @@ __device__ __forceinline__ void roe_rows3(double out[3][5], double gm1, double c
 __device__ __forceinline__ int full_col(int s) { return s == 2 ? IPR : s; }
 
 // ---------------------------------------------------------------------------
-// assembly: one thread per cell
+// assembly of one cell's blocks (vic_assemble_impl<3>)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-    vicp_assemble_kernel(double const* __restrict__ w, double const* __restrict__ du,
-                         double const* __restrict__ vtab, double* __restrict__ ws,
-                         PartArgs a) {
-  Range3 const& r = a.r;
-  int n = a.n;
-  int64_t ncol = int64_t(r.n2) * r.n3;
-  // a block covers kTileC columns x kTileL layers, layers fastest: the state
-  // reads of 8 lanes are one 64-byte run along x1, the record writes of a
-  // layer are kTileC consecutive columns
-  int li = threadIdx.x % kTileL, ci = threadIdx.x / kTileL;
-  int64_t col = int64_t(blockIdx.x) * kTileC + ci;
-  int i = int(blockIdx.y) * kTileL + li;
-  if (col >= ncol || i >= n) return;
-  int j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
-  int64_t row = k * r.s3 + j * r.s2 + r.lo1;
-  int nc1 = r.nc1, lo1 = r.lo1;
-  double const gm1 = a.gamma - 1.;
-  double const grav = a.grav, A = a.area, V = a.vol;
-  double const dt = stage_dt(a);
-  double const hiv = 0.5 / V;
-  auto W = [&](int v, int l) { return w[v * r.sv + row + l]; };
-  auto DU = [&](int v, int l) { return du[v * r.sv + row + l]; };
-  auto tab = [&](int rowi, int l) { return vtab[rowi * nc1 + lo1 + l]; };
+// the grid-only factors of cell i and the two explicit masses the wall
+// cells' corrected-PE rows read
+struct CellTab {
+  double wlo, whi, klo, kmid, khi;
+  double d2, dn3;  // du_rho at cells 2 and n - 3
+};
 
-  double wl[5], wc[5], wr[5];
-  for (int v = 0; v < 5; ++v) {
-    wl[v] = W(v, i - 1);
-    wc[v] = W(v, i);
-    wr[v] = W(v, i + 1);
-  }
-  double am[3][5], ap[3][5], dp[3][5], dc[3][5], dn[3][5];
-  double gme = 0.5 * (a.gamma + a.gamma) - 1.;
-  roe_rows3(am, gme, wl, wc);
-  roe_rows3(ap, gme, wc, wr);
-  jac_rows(dp, gm1, wl);
-  jac_rows(dc, gm1, wc);
-  jac_rows(dn, gm1, wr);
-  double aa[3][3], bb[3][3], cc[3][3];
+// am, ap: rows {rho, m1, E} of |A| at the lower / upper face; dp, dc, dn:
+// the flux Jacobian rows of cells i - 1, i, i + 1; wl, wc, wr and dl, dm, dr
+// their primitives and explicit increments.  Writes the blocks a, b, c and
+// the right-hand side with the fold's correction subtracted.
+__device__ __forceinline__ void assemble_cell(
+    double const (&am)[3][5], double const (&ap)[3][5], double const (&dp)[3][5],
+    double const (&dc)[3][5], double const (&dn)[3][5], double const* wl, double const* wc,
+    double const* wr, double const* dl, double const* dm, double const* dr, CellTab const& t,
+    int i, int n, double A, double hiv, double grav, double dt, double t0, double t1,
+    double (&aa)[3][3], double (&bb)[3][3], double (&cc)[3][3], double (&rhs)[3]) {
   for (int p = 0; p < 3; ++p)
     for (int s = 0; s < 3; ++s) {
       int q = full_col(s);
@@ -168,8 +153,9 @@ __global__ void __launch_bounds__(256)
     }
   aa[1][0] -= grav;  // - Phi (Phi(m1, rho) = Phi(E, m1) = grav)
   aa[2][1] -= grav;
-  for (int p = 0; p < 3; ++p) aa[p][p] += 1. / dt;
-  double lo = tab(V_WLO, i), up = tab(V_WHI, i);
+  double const idt = 1. / dt;
+  for (int p = 0; p < 3; ++p) aa[p][p] += idt;
+  double lo = t.wlo, up = t.whi;
   for (int s = 0; s < 3; ++s) {
     int q = full_col(s);
     aa[2][s] -= grav * (up * ap[0][q] - lo * am[0][q]);
@@ -208,42 +194,38 @@ __global__ void __launch_bounds__(256)
     for (int p = 0; p < 3; ++p)
       for (int kk = 0; kk < 2; ++kk) aT[p][kk] += cT[p][kk];
   double corr[3] = {0., 0., 0.};
-  auto fold = [&](double (*blk)[3], double (*tb)[2], int jcell) {
-    double drho = DU(IDN, jcell);
+  // cell jc = 0, 1, 2: i - 1, i, i + 1
+  auto fold = [&](double (*blk)[3], double (*tb)[2], int jc) {
+    double const* dj = jc == 0 ? dl : (jc == 1 ? dm : dr);
+    double const* wj = jc == 0 ? wl : (jc == 1 ? wc : wr);
+    double drho = dj[IDN];
     for (int kk = 0; kk < 2; ++kk) {
-      double v = W(IVY + kk, jcell);
-      double q = DU(IVY + kk, jcell) - v * drho;
+      double v = wj[IVY + kk];
+      double q = dj[IVY + kk] - v * drho;
       if (v != 0.)
         for (int p = 0; p < 3; ++p) blk[p][0] += tb[p][kk] * v;
       if (q != 0.)
         for (int p = 0; p < 3; ++p) corr[p] += tb[p][kk] * q;
     }
   };
-  fold(aa, aT, i);
-  if (!merged_lo) fold(bb, bT, i - 1);
-  if (!merged_hi) fold(cc, cT, i + 1);
+  fold(aa, aT, 1);
+  if (!merged_lo) fold(bb, bT, 0);
+  if (!merged_hi) fold(cc, cT, 2);
   // couple_radial_exact (implicit_hydro.cpp:408-446): E row, rho column
-  aa[2][0] -= tab(V_KMID, i) / dt;
-  bb[2][0] -= tab(V_KLO, i) / dt;
-  cc[2][0] -= tab(V_KHI, i) / dt;
+  aa[2][0] -= t.kmid * idt;
+  bb[2][0] -= t.klo * idt;
+  cc[2][0] -= t.khi * idt;
   // right-hand side: du / dt, the energy without the explicit corrected PE
-  double rows = tab(V_KMID, i) * DU(IDN, i);
-  if (i > 0) rows = rows + tab(V_KLO, i) * DU(IDN, i - 1);
-  if (i < n - 1) rows = rows + tab(V_KHI, i) * DU(IDN, i + 1);
-  if (i == 0) rows = rows + a.t0 * DU(IDN, 2);
-  if (i == n - 1) rows = rows + a.t1 * DU(IDN, n - 3);
-  double rhs[3] = {DU(IDN, i) / dt, DU(IVX, i) / dt, (DU(IPR, i) - rows) / dt};
+  double rows = t.kmid * dm[IDN];
+  if (i > 0) rows = rows + t.klo * dl[IDN];
+  if (i < n - 1) rows = rows + t.khi * dr[IDN];
+  if (i == 0) rows = rows + t0 * t.d2;
+  if (i == n - 1) rows = rows + t1 * t.dn3;
+  rhs[0] = dm[IDN] * idt;
+  rhs[1] = dm[IVX] * idt;
+  rhs[2] = (dm[IPR] - rows) * idt;
   for (int p = 0; p < 3; ++p)
     if (corr[p] != 0.) rhs[p] -= corr[p];
-  // value e of layer i of column col at ws[(i * kRec + e) * ncol + col]
-  double* rec = ws + int64_t(i) * kRec * ncol + col;
-  for (int p = 0; p < 3; ++p)
-    for (int s = 0; s < 3; ++s) {
-      rec[(R_A + 3 * p + s) * ncol] = aa[p][s];
-      rec[(R_B + 3 * p + s) * ncol] = bb[p][s];
-      rec[(R_C + 3 * p + s) * ncol] = cc[p][s];
-    }
-  for (int p = 0; p < 3; ++p) rec[(R_RHS + p) * ncol] = rhs[p];
 }
 
 // ---------------------------------------------------------------------------
@@ -335,32 +317,119 @@ struct Rec {
   __device__ double& operator[](int e) const { return p[e * s]; }
 };
 
-// the serial part of the column program, one thread per column: the forward
-// sweep over the assembled records (each rewritten in place with A' = a^-1 c
-// and the three partial solutions) and the back substitution
+constexpr int kLB = 4;           // layers staged per chunk
+constexpr int kStage = kLB + 2;  // with the layer below and the layer above
+
+// the column program's serial part, one thread per column (64 per block):
+// each layer's blocks assembled in registers from the staged state (the
+// faces and Jacobians carried up from the layer below), the forward sweep
+// (ForwardSweep<2>, the small-matrix path) writing A' = a^-1 c and the three
+// partial solutions to the lane-contiguous records, then the back
+// substitution, which leaves y, z0, z1 of every layer in yc, column-major
+// (yc[(col * 9 + e) * n + l]) for the finish kernel's per-layer lanes
 __global__ void __launch_bounds__(64)
-    vicp_sweep_kernel(double* __restrict__ ws, int* __restrict__ status, PartArgs a) {
+    vicp_forward_kernel(double const* __restrict__ w, double const* __restrict__ du,
+                        double const* __restrict__ vtab, double* __restrict__ ws,
+                        double* __restrict__ yc, int* __restrict__ status, PartArgs a) {
+  __shared__ double s_w[5][kStage][64];
+  __shared__ double s_du[5][kStage][64];
   Range3 const& r = a.r;
-  int64_t col = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t ncol = int64_t(r.n2) * r.n3;
-  if (col >= ncol) return;
-  int n = a.n;
+  int lane = threadIdx.x;
+  int64_t col0 = int64_t(blockIdx.x) * 64;
+  int64_t col = col0 + lane;
+  bool live = col < ncol;
+  int n = a.n, nc1 = r.nc1, lo1 = r.lo1;
+  auto row_of = [&](int64_t c) {
+    int jj = r.lo2 + int(c % r.n2), kk = r.lo3 + int(c / r.n2);
+    return kk * r.s3 + jj * r.s2 + r.lo1;
+  };
+  double const gm1 = a.gamma - 1.;
+  double const gme = 0.5 * (a.gamma + a.gamma) - 1.;
+  GasK const K{gme, 1. / gme, (gme + 1.) / gme, gme / (gme + 1.)};
+  double const gp_g = (gm1 + 1.) / gm1;
+  double const A = a.area, V = a.vol;
+  double const hiv = 0.5 / V;
+  double const dt = stage_dt(a);
   auto REC = [&](int l) { return Rec{ws + int64_t(l) * kRec * ncol + col, ncol}; };
+  auto tab = [&](int rowi, int l) { return vtab[rowi * nc1 + lo1 + l]; };
+  auto put_yz = [&](int l, double const* y, double const* z0, double const* z1) {
+    double* o = yc + (col * 9) * n + l;
+    for (int p = 0; p < 3; ++p) {
+      o[(p) * n] = y[p];
+      o[(3 + p) * n] = z0[p];
+      o[(6 + p) * n] = z1[p];
+    }
+  };
+  double d2 = 0., dn3 = 0.;
+  if (live) {
+    int64_t row = row_of(col);
+    d2 = du[IDN * r.sv + row + 2];
+    dn3 = du[IDN * r.sv + row + n - 3];
+  }
 
   // ---- forward sweep (ForwardSweep<2>, the small-matrix path) --------------
   double ap[3][3], yp[3], z0p[3], z1p[3];
   bool ok = true;
-  for (int i = 0; i < n && ok; ++i) {
-    Rec rec = REC(i);
-    double m[3][3], bb[3][3], c3[3][3], rhs[3];
-    for (int p = 0; p < 3; ++p) {
-      for (int s = 0; s < 3; ++s) {
-        m[p][s] = rec[R_A + 3 * p + s];
-        bb[p][s] = rec[R_B + 3 * p + s];
-        c3[p][s] = rec[R_C + 3 * p + s];
+  // the lower face's |A| rows and the Jacobian rows of cells i - 1, i,
+  // carried from the layer below
+  double fa_lo[3][5], jp[3][5], jc[3][5];
+  // every lane stages its share of every chunk (even after its own column
+  // has failed), so the loop runs to the top for all lanes
+  for (int c0l = 0; c0l < n; c0l += kLB) {
+    // stage layers c0l - 1 .. c0l + kLB of w and du for the wavefront's 64
+    // columns: lanes run along x1 within a column (coalesced runs)
+    __syncthreads();
+    for (int e = lane; e < 64 * kStage; e += 64) {
+      int cc = e / kStage, ll = e % kStage, layer = c0l - 1 + ll;
+      int64_t gcol = col0 + cc;
+      bool in = gcol < ncol;
+      int64_t grow = in ? row_of(gcol) : 0;
+      for (int v = 0; v < 5; ++v) {
+        s_w[v][ll][cc] = in && layer <= n ? w[v * r.sv + grow + layer] : 1.;
+        s_du[v][ll][cc] = in && layer >= 0 && layer < n ? du[v * r.sv + grow + layer] : 0.;
       }
-      rhs[p] = rec[R_RHS + p];
     }
+    __syncthreads();
+    if (!live || !ok) continue;
+    for (int ll = 0; ll < kLB && c0l + ll < n && ok; ++ll) {
+    int i = c0l + ll;
+    Rec rec = REC(i);
+    double wl[5], wc[5], wr[5], dl[5], dmv[5], dr[5];
+    for (int v = 0; v < 5; ++v) {
+      wl[v] = s_w[v][ll][lane];
+      wc[v] = s_w[v][ll + 1][lane];
+      wr[v] = s_w[v][ll + 2][lane];
+      dl[v] = s_du[v][ll][lane];
+      dmv[v] = s_du[v][ll + 1][lane];
+      dr[v] = s_du[v][ll + 2][lane];
+    }
+    if (i == 0) {
+      roe_rows3(fa_lo, K, wl, wc);
+      jac_rows(jp, gm1, gp_g, wl);
+      jac_rows(jc, gm1, gp_g, wc);
+    }
+    double fa_hi[3][5], jn[3][5];
+    roe_rows3(fa_hi, K, wc, wr);
+    jac_rows(jn, gm1, gp_g, wr);
+    CellTab ct;
+    ct.wlo = tab(V_WLO, i);
+    ct.whi = tab(V_WHI, i);
+    ct.klo = tab(V_KLO, i);
+    ct.kmid = tab(V_KMID, i);
+    ct.khi = tab(V_KHI, i);
+    ct.d2 = d2;
+    ct.dn3 = dn3;
+    double m[3][3], bb[3][3], c3[3][3], rhs[3];
+    assemble_cell(fa_lo, fa_hi, jp, jc, jn, wl, wc, wr, dl, dmv, dr, ct, i, n, A, hiv, a.grav, dt,
+                  a.t0, a.t1, m, bb, c3, rhs);
+    // the faces and Jacobians carried to the layer above
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 5; ++q) {
+        fa_lo[p][q] = fa_hi[p][q];
+        jp[p][q] = jc[p][q];
+        jc[p][q] = jn[p][q];
+      }
     double e0[3] = {0., 0., i == 0 ? 1. : 0.};
     double e1[3] = {0., 0., i == n - 1 ? 1. : 0.};
     if (i > 0)
@@ -410,6 +479,8 @@ __global__ void __launch_bounds__(64)
       rec[R_Z1 + p] = z1[p];
     }
   }
+  }
+  if (!live) return;
   // a failed column keeps its explicit increments (the finish kernel reads
   // the flag; the reference undoes the column with its status bits and
   // flags a retry)
@@ -424,6 +495,7 @@ __global__ void __launch_bounds__(64)
       z0n[p] = last[R_Z0 + p];
       z1n[p] = last[R_Z1 + p];
     }
+    put_yz(n - 1, yn, z0n, z1n);
     for (int i = n - 2; i >= 0; --i) {
       Rec rec = REC(i);
       double A3[3][3];
@@ -437,10 +509,8 @@ __global__ void __launch_bounds__(64)
         yn[p] = rec[R_Y + p] - sy[p];
         z0n[p] = rec[R_Z0 + p] - s0[p];
         z1n[p] = rec[R_Z1 + p] - s1[p];
-        rec[R_Y + p] = yn[p];
-        rec[R_Z0 + p] = z0n[p];
-        rec[R_Z1 + p] = z1n[p];
       }
+      put_yz(i, yn, z0n, z1n);
     }
   }
 }
@@ -458,7 +528,7 @@ __global__ void __launch_bounds__(64)
                        void const* __restrict__ u0v, float const* __restrict__ c0,
                        void* __restrict__ uv, float* __restrict__ cc,
                        double const* __restrict__ vtab, double const* __restrict__ xtab,
-                       double* __restrict__ ws, int const* __restrict__ status, PartArgs a) {
+                       double const* __restrict__ yc, int const* __restrict__ status, PartArgs a) {
   extern __shared__ double lds[];
   Range3 const& r = a.r;
   int64_t ncol = int64_t(r.n2) * r.n3;
@@ -471,7 +541,6 @@ __global__ void __launch_bounds__(64)
   double const dt = stage_dt(a);
   auto W = [&](int v, int l) { return w[v * r.sv + row + l]; };
   auto DU = [&](int v, int l) { return du[v * r.sv + row + l]; };
-  auto REC = [&](int l) { return Rec{ws + int64_t(l) * kRec * ncol + col, ncol}; };
   auto tab = [&](double const* t, int rowi, int l) { return t[rowi * nc1 + lo1 + l]; };
   auto combine = [&](int v, int l, double x) {
     int64_t o = v * r.sv + row + l;
@@ -506,20 +575,19 @@ __global__ void __launch_bounds__(64)
   {
     double k0 = -a.t0 / dt, k1 = -a.t1 / dt;
     int i0 = 2, i1 = n - 3;
-    Rec r0p = REC(i0);
-    Rec r1p = REC(i1);
-    double m00 = 1. + k0 * r0p[R_Z0], m01 = k0 * r0p[R_Z1];
-    double m10 = k1 * r1p[R_Z0], m11 = 1. + k1 * r1p[R_Z1];
-    double r0 = k0 * r0p[R_Y], r1 = k1 * r1p[R_Y];
+    double const* yv = yc + (col * 9) * n;
+    double m00 = 1. + k0 * yv[3 * n + i0], m01 = k0 * yv[6 * n + i0];
+    double m10 = k1 * yv[3 * n + i1], m11 = 1. + k1 * yv[6 * n + i1];
+    double r0 = k0 * yv[i0], r1 = k1 * yv[i1];
     double det = m00 * m11 - m01 * m10;
     al0 = (m11 * r0 - m01 * r1) / det;
     al1 = (m00 * r1 - m10 * r0) / det;
   }
+  double const* ycol = yc + (col * 9) * n;
   auto Y = [&](int l, int p) {
-    Rec rec = REC(l);
-    double y = rec[R_Y + p];
-    y -= rec[R_Z0 + p] * al0;
-    y -= rec[R_Z1 + p] * al1;
+    double y = ycol[p * n + l];
+    y -= ycol[(3 + p) * n + l] * al0;
+    y -= ycol[(6 + p) * n + l] * al1;
     return y;
   };
   for (int l = lane; l < n; l += 64) {
@@ -651,27 +719,26 @@ static int vicp_launch(const snapy_grid* g, const snapy_physics* p, const double
   int64_t ncol = int64_t(g->nx2) * g->nx3;
   int64_t ncell = ncol * g->nx1;
   (void)ncell;
-  dim3 agrid(blocks_for(ncol, kTileC), blocks_for(g->nx1, kTileL));
-  hipLaunchKernelGGL(vicp_assemble_kernel, agrid, dim3(kTileC * kTileL), 0,
-                     as_stream(stream), w, du, vtab, ws, a);
-  if (int e = launch_status("vic_partial_assemble")) return e;
-  hipLaunchKernelGGL(vicp_sweep_kernel, dim3(blocks_for(ncol, 64)), dim3(64), 0,
-                     as_stream(stream), ws, status, a);
-  if (int e = launch_status("vic_partial_sweep")) return e;
+  // the records (kF per layer, lane-contiguous) then yc (9 per layer)
+  double* yc = ws + int64_t(kF) * g->nx1 * ncol;
+  hipLaunchKernelGGL(vicp_forward_kernel, dim3(blocks_for(ncol, 64)), dim3(64), 0,
+                     as_stream(stream), w, du, vtab, ws, yc, status, a);
+  if (int e = launch_status("vic_partial_forward")) return e;
   size_t lds = size_t(9) * (g->nx1 + 1) * sizeof(double);
   if (lds > 64 * 1024) return fail(-9, "vic_partial: the column does not fit the LDS");
   if (pair)
     hipLaunchKernelGGL(vicp_finish_kernel<true>, dim3(ncol), dim3(64), lds, as_stream(stream), w,
-                       du, u0, c0, u, c, vtab, x1tab, ws, status, a);
+                       du, u0, c0, u, c, vtab, x1tab, yc, status, a);
   else
     hipLaunchKernelGGL(vicp_finish_kernel<false>, dim3(ncol), dim3(64), lds, as_stream(stream),
-                       w, du, u0, c0, u, c, vtab, x1tab, ws, status, a);
+                       w, du, u0, c0, u, c, vtab, x1tab, yc, status, a);
   return launch_status("vic_partial");
 }
 
 int64_t snapy_vic_partial_workspace(const snapy_grid* g) {
   if (!g) return 0;
-  return int64_t(kRec) * g->nx1 * g->nx2 * g->nx3;
+  // the lane-contiguous records and the column-major y, z0, z1
+  return int64_t(kRec + 9) * g->nx1 * g->nx2 * g->nx3;
 }
 
 int snapy_vic_partial(const snapy_grid* g, const snapy_physics* p, const double* w,
