@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--implicit-scheme", type=int, default=None, choices=(1, 9),
                     help="override the convection deck's implicit scheme")
     ap.add_argument("--cubed-steps", type=int, default=50)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain elementwise copy of 1 GiB (the HBM counter calibration)")
+    ap.add_argument("--no-other-scheme", action="store_true",
+                    help="skip the FP32 leg with the other vertical implicit scheme")
     return ap.parse_args()
 
 
@@ -144,7 +148,8 @@ def cpu_baseline(name, budget_s=10.0):
     g = fv.Grid(c.nx1, c.nx2, c.nx3, c.x1min, c.x1max, c.x2min, c.x2max, c.x3min, c.x3max)
     h = op.hydro
     d = fv.Deck(grid=g, gammad=h.eos.gammad, riemann="lmars", shock_x1=False, shock_x23=False,
-                bcs=tuple(op.bcnames), cfl=op.intg.cfl, grav1=h.grav1, implicit_scheme=9,
+                bcs=tuple(op.bcnames), cfl=op.intg.cfl, grav1=h.grav1,
+                implicit_scheme=SCHEME["value"] or op.intg.implicit_scheme,
                 flux_covariance=True, bot_heat_flux=h.bot_heat_flux, bot_heat_depth=h.bot_heat_depth)
     b = fv.Block(d)
     b.initialize(w)
@@ -342,6 +347,56 @@ def cubed_leg(steps=50, warmup=3, members=32, seed0=1000):
     }
 
 
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain elementwise copy of `nbytes` FP64 (y = -x, one streaming
+    kernel per repetition that reads x once and writes y once): the
+    known-size reference that the HBM counters FETCH_SIZE / WRITE_SIZE are
+    scaled by, and the achievable streaming rate."""
+    import torch
+
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        torch.neg(x, out=y)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": reps + 1,
+            "ms_per_copy": ms, "copy_GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
+
+
+def other_scheme_leg(steps, warmup):
+    """The FP32 form of the same workload with the other vertical implicit
+    scheme (9, the full 5x5 blocks, when the deck runs 1; and the reverse):
+    throughput and the implicit kernel's HIP-event time."""
+    import torch
+
+    from snapy_amd import MeshBlockOptions
+
+    keep = SCHEME["value"]
+    deck = MeshBlockOptions.from_yaml(os.path.join(ROOT, "snapy_amd", "decks", "convection3d.yaml"))
+    scheme = 9 if (keep or deck.intg.implicit_scheme) == 1 else 1
+    SCHEME["value"] = scheme
+    try:
+        block, vars, cells, cfg, _, _ = workload("convection3d", "float32")
+        el = timed_run(block, vars, steps, warmup)
+        kt = kernel_times(block, vars)
+        ok = bool(torch.isfinite(vars["hydro_u"]).all().item()) and block.check_redo(vars) == 0
+    finally:
+        SCHEME["value"] = keep
+    vic = "vic_full" if scheme == 9 else "vic_partial"
+    out = {"implicit_scheme": scheme, "value": cells * 3 * steps / el, "unit": "cell-updates/s",
+           "ms_per_step": el / steps * 1e3, "form": "FP32 state form, as the headline",
+           "vic_ms": kt.get(vic), "state_finite": ok}
+    del block, vars
+    torch.cuda.empty_cache()
+    return out
+
+
 def serving_leg(nreq, seed=2025):
     """A bounded serving stream: Poisson arrivals (20/s, seeded) of the
     Sod / density-current / shallow-water mix, continuous batching with
@@ -371,6 +426,9 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
+    if args.copy_calibration:
+        print(json.dumps(copy_calibration()), flush=True)
+        return
     defaults = {"sod": (200, 10), "convection3d": (20, 3)}[args.workload]
     steps = args.steps or defaults[0]
     warmup = args.warmup if args.warmup is not None else defaults[1]
@@ -434,6 +492,10 @@ def main():
         print("# serving stream", file=sys.stderr, flush=True)
         serving_summary = serving_leg(args.serving_requests)
         print(f"# serving {serving_summary}", file=sys.stderr, flush=True)
+    other = None
+    if not args.no_other_scheme and args.workload == "convection3d":
+        other = other_scheme_leg(steps, warmup)
+        print(f"# other scheme {other}", file=sys.stderr, flush=True)
     cubed = None
     if not args.no_cubed and args.workload == "convection3d":
         print("# batched cubed-sphere shallow water", file=sys.stderr, flush=True)
@@ -464,6 +526,8 @@ def main():
         line["serving"] = serving_summary
     if cubed is not None:
         line["cubed_sphere"] = cubed
+    if other is not None:
+        line["other_implicit_scheme"] = other
     if plain is not None:
         line["fp64_plain"] = plain
         line["speedup_vs_fp64_plain"] = value / plain["value"]

@@ -309,9 +309,10 @@ int snapy_update_gravity_f32(const snapy_grid *g, const snapy_physics *p,
  * its Woodbury wall points, the mass redistribution with the tangential
  * momentum each moved mass carries (the donor cell's velocity), the
  * post-solve remainder and the RK combine; arguments as snapy_vic_full.
- * One kernel assembles each cell's blocks into a 32-double record of ws
- * (snapy_vic_partial_workspace(g) doubles), one thread per column then
- * sweeps, back-substitutes and redistributes.
+ * One thread per column assembles, sweeps and back-substitutes (records of
+ * the factored rows and the partial solutions in ws,
+ * snapy_vic_partial_workspace(g) doubles); one wavefront per column then
+ * applies the Woodbury correction, redistributes and combines.
  * Replaces: vic_stage for VicMatrix::partial (vic_kernels.h:66-297 with
  * N = 3; vic_assemble_impl.h:45-218; forward_sweep_impl.h:104-121;
  * vic_redistribute_impl.h:177-205) and ImplicitHydroImpl::forward_masked

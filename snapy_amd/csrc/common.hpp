@@ -48,6 +48,17 @@ inline unsigned blocks_for(int64_t n, int tpb) {
   return static_cast<unsigned>((n + tpb - 1) / tpb);
 }
 
+// MI355X dispatches workgroup b of a launch to XCD b mod 8, and each XCD has
+// its own L2.  Kernels whose neighbouring tiles share input (stencil halos)
+// take their tile index through this map, so that each XCD works on one
+// contiguous range of tiles: b -> (b mod 8) * (nb / 8) + b / 8 for the
+// first 8 * (nb / 8) workgroups, the identity for the rest.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
+  unsigned q = nb / kXcds;
+  return b < q * kXcds ? (b % kXcds) * q + b / kXcds : b;
+}
+
 // the shallow-water stage (swe.hip), dispatched from the generic entry
 // points when p->eos is SNAPY_EOS_SHALLOW_WATER
 int swe_cons2prim(const snapy_grid* g, int dtype, const void* u, void* w,

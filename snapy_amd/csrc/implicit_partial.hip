@@ -319,6 +319,11 @@ struct Rec {
 
 constexpr int kLB = 4;           // layers staged per chunk
 constexpr int kStage = kLB + 2;  // with the layer below and the layer above
+constexpr int kYL = 8;           // layers of y, z0, z1 gathered before a write
+constexpr int kYS = 9 * kYL + 1;  // LDS row of one column (padded)
+// the forward staging (w and du, 5 x kStage x 64 each) and the back
+// substitution's gather (64 columns x kYS) share one LDS buffer
+constexpr int kLds = 64 * kYS > 2 * 5 * kStage * 64 ? 64 * kYS : 2 * 5 * kStage * 64;
 
 // the column program's serial part, one thread per column (64 per block):
 // each layer's blocks assembled in registers from the staged state (the
@@ -326,24 +331,29 @@ constexpr int kStage = kLB + 2;  // with the layer below and the layer above
 // (ForwardSweep<2>, the small-matrix path) writing A' = a^-1 c and the three
 // partial solutions to the lane-contiguous records, then the back
 // substitution, which leaves y, z0, z1 of every layer in yc, column-major
-// (yc[(col * 9 + e) * n + l]) for the finish kernel's per-layer lanes
+// (yc[(col * 9 + e) * n + l]) for the finish kernel's per-layer lanes: kYL
+// layers are gathered in LDS and written as runs along x1
 __global__ void __launch_bounds__(64)
     vicp_forward_kernel(double const* __restrict__ w, double const* __restrict__ du,
                         double const* __restrict__ vtab, double* __restrict__ ws,
                         double* __restrict__ yc, int* __restrict__ status, PartArgs a) {
-  __shared__ double s_w[5][kStage][64];
-  __shared__ double s_du[5][kStage][64];
+  __shared__ double s_buf[kLds];
+  __shared__ int64_t s_row[64];
+  __shared__ int s_act[64];
+  double* s_w = s_buf;                    // [v][ll][cc]
+  double* s_du = s_buf + 5 * kStage * 64;
   Range3 const& r = a.r;
-  int64_t ncol = int64_t(r.n2) * r.n3;
+  int ncol = r.n2 * r.n3;  // < 2^31 (checked by the launcher)
   int lane = threadIdx.x;
-  int64_t col0 = int64_t(blockIdx.x) * 64;
-  int64_t col = col0 + lane;
+  int col0 = blockIdx.x * 64;
+  int col = col0 + lane;
   bool live = col < ncol;
   int n = a.n, nc1 = r.nc1, lo1 = r.lo1;
-  auto row_of = [&](int64_t c) {
-    int jj = r.lo2 + int(c % r.n2), kk = r.lo3 + int(c / r.n2);
-    return kk * r.s3 + jj * r.s2 + r.lo1;
-  };
+  // the first interior cell of each of the block's columns
+  {
+    int jj = r.lo2 + col % r.n2, kk = r.lo3 + col / r.n2;
+    s_row[lane] = live ? kk * r.s3 + jj * r.s2 + r.lo1 : 0;
+  }
   double const gm1 = a.gamma - 1.;
   double const gme = 0.5 * (a.gamma + a.gamma) - 1.;
   GasK const K{gme, 1. / gme, (gme + 1.) / gme, gme / (gme + 1.)};
@@ -353,17 +363,10 @@ __global__ void __launch_bounds__(64)
   double const dt = stage_dt(a);
   auto REC = [&](int l) { return Rec{ws + int64_t(l) * kRec * ncol + col, ncol}; };
   auto tab = [&](int rowi, int l) { return vtab[rowi * nc1 + lo1 + l]; };
-  auto put_yz = [&](int l, double const* y, double const* z0, double const* z1) {
-    double* o = yc + (col * 9) * n + l;
-    for (int p = 0; p < 3; ++p) {
-      o[(p) * n] = y[p];
-      o[(3 + p) * n] = z0[p];
-      o[(6 + p) * n] = z1[p];
-    }
-  };
+  __syncthreads();
   double d2 = 0., dn3 = 0.;
   if (live) {
-    int64_t row = row_of(col);
+    int64_t row = s_row[lane];
     d2 = du[IDN * r.sv + row + 2];
     dn3 = du[IDN * r.sv + row + n - 3];
   }
@@ -382,12 +385,12 @@ __global__ void __launch_bounds__(64)
     __syncthreads();
     for (int e = lane; e < 64 * kStage; e += 64) {
       int cc = e / kStage, ll = e % kStage, layer = c0l - 1 + ll;
-      int64_t gcol = col0 + cc;
-      bool in = gcol < ncol;
-      int64_t grow = in ? row_of(gcol) : 0;
+      bool in = col0 + cc < ncol;
+      int64_t grow = s_row[cc];
       for (int v = 0; v < 5; ++v) {
-        s_w[v][ll][cc] = in && layer <= n ? w[v * r.sv + grow + layer] : 1.;
-        s_du[v][ll][cc] = in && layer >= 0 && layer < n ? du[v * r.sv + grow + layer] : 0.;
+        s_w[(v * kStage + ll) * 64 + cc] = in && layer <= n ? w[v * r.sv + grow + layer] : 1.;
+        s_du[(v * kStage + ll) * 64 + cc] =
+            in && layer >= 0 && layer < n ? du[v * r.sv + grow + layer] : 0.;
       }
     }
     __syncthreads();
@@ -397,12 +400,12 @@ __global__ void __launch_bounds__(64)
     Rec rec = REC(i);
     double wl[5], wc[5], wr[5], dl[5], dmv[5], dr[5];
     for (int v = 0; v < 5; ++v) {
-      wl[v] = s_w[v][ll][lane];
-      wc[v] = s_w[v][ll + 1][lane];
-      wr[v] = s_w[v][ll + 2][lane];
-      dl[v] = s_du[v][ll][lane];
-      dmv[v] = s_du[v][ll + 1][lane];
-      dr[v] = s_du[v][ll + 2][lane];
+      wl[v] = s_w[(v * kStage + ll) * 64 + lane];
+      wc[v] = s_w[(v * kStage + ll + 1) * 64 + lane];
+      wr[v] = s_w[(v * kStage + ll + 2) * 64 + lane];
+      dl[v] = s_du[(v * kStage + ll) * 64 + lane];
+      dmv[v] = s_du[(v * kStage + ll + 1) * 64 + lane];
+      dr[v] = s_du[(v * kStage + ll + 2) * 64 + lane];
     }
     if (i == 0) {
       roe_rows3(fa_lo, K, wl, wc);
@@ -480,38 +483,74 @@ __global__ void __launch_bounds__(64)
     }
   }
   }
-  if (!live) return;
   // a failed column keeps its explicit increments (the finish kernel reads
   // the flag; the reference undoes the column with its status bits and
   // flags a retry)
-  status[col] = ok ? 0 : 1;
-  if (!ok) return;
+  if (live) status[col] = ok ? 0 : 1;
+  bool act = live && ok;
+  __syncthreads();  // the staging buffer becomes the gather buffer
+  s_act[lane] = act;
   // ---- back substitution (vic_backward_substitute) -------------------------
-  {
-    double yn[3], z0n[3], z1n[3];
+  // every lane runs the loop (the gather is shared); a lane without a solved
+  // column carries zeros and its values are not written
+  double* s_y = s_buf;  // [cc][e * kYL + layer % kYL], row kYS
+  auto gather = [&](int l, double const* y, double const* z0, double const* z1) {
+    double* o = s_y + lane * kYS + (l % kYL);
+    for (int p = 0; p < 3; ++p) {
+      o[p * kYL] = y[p];
+      o[(3 + p) * kYL] = z0[p];
+      o[(6 + p) * kYL] = z1[p];
+    }
+  };
+  // layers l0 .. l0 + cnt - 1 of the block's columns, lanes along x1
+  auto flush = [&](int l0, int cnt) {
+    __syncthreads();
+    for (int t = lane; t < 64 * 9 * kYL; t += 64) {
+      int ll = t % kYL, pe = t / kYL, cc = pe / 9, e = pe % 9;
+      int c = col0 + cc;
+      if (ll < cnt && s_act[cc])
+        yc[(int64_t(c) * 9 + e) * n + l0 + ll] = s_y[cc * kYS + e * kYL + ll];
+    }
+    __syncthreads();
+  };
+  double yn[3] = {0., 0., 0.}, z0n[3] = {0., 0., 0.}, z1n[3] = {0., 0., 0.};
+  double nx[18];
+  if (act) {
     Rec last = REC(n - 1);
     for (int p = 0; p < 3; ++p) {
       yn[p] = last[R_Y + p];
       z0n[p] = last[R_Z0 + p];
       z1n[p] = last[R_Z1 + p];
     }
-    put_yz(n - 1, yn, z0n, z1n);
-    for (int i = n - 2; i >= 0; --i) {
-      Rec rec = REC(i);
-      double A3[3][3];
-      for (int p = 0; p < 3; ++p)
-        for (int s = 0; s < 3; ++s) A3[p][s] = rec[R_A + 3 * p + s];
-      double sy[3], s0[3], s1[3];
-      mv3(A3, yn, sy);
-      mv3(A3, z0n, s0);
-      mv3(A3, z1n, s1);
-      for (int p = 0; p < 3; ++p) {
-        yn[p] = rec[R_Y + p] - sy[p];
-        z0n[p] = rec[R_Z0 + p] - s0[p];
-        z1n[p] = rec[R_Z1 + p] - s1[p];
-      }
-      put_yz(i, yn, z0n, z1n);
+    Rec rec = REC(n - 2);
+    for (int e = 0; e < 18; ++e) nx[e] = rec[e];
+  } else {
+    for (int e = 0; e < 18; ++e) nx[e] = 0.;
+  }
+  gather(n - 1, yn, z0n, z1n);
+  if ((n - 1) % kYL == 0) flush(n - 1, 1);
+  for (int i = n - 2; i >= 0; --i) {
+    double cur[18];
+    for (int e = 0; e < 18; ++e) cur[e] = nx[e];
+    // the record of the layer below, loaded while this layer is solved
+    if (act && i > 0) {
+      Rec rec = REC(i - 1);
+      for (int e = 0; e < 18; ++e) nx[e] = rec[e];
     }
+    double A3[3][3];
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 3; ++q) A3[p][q] = cur[R_A + 3 * p + q];
+    double sy[3], s0[3], s1[3];
+    mv3(A3, yn, sy);
+    mv3(A3, z0n, s0);
+    mv3(A3, z1n, s1);
+    for (int p = 0; p < 3; ++p) {
+      yn[p] = cur[R_Y + p] - sy[p];
+      z0n[p] = cur[R_Z0 + p] - s0[p];
+      z1n[p] = cur[R_Z1 + p] - s1[p];
+    }
+    gather(i, yn, z0n, z1n);
+    if (i % kYL == 0) flush(i, (n - i) < kYL ? (n - i) : kYL);
   }
 }
 
@@ -717,8 +756,7 @@ static int vicp_launch(const snapy_grid* g, const snapy_physics* p, const double
   a.w1 = w1;
   a.w2 = w2;
   int64_t ncol = int64_t(g->nx2) * g->nx3;
-  int64_t ncell = ncol * g->nx1;
-  (void)ncell;
+  if (ncol > (int64_t(1) << 30)) return fail(-9, "vic_partial: too many columns");
   // the records (kF per layer, lane-contiguous) then yc (9 per layer)
   double* yc = ws + int64_t(kF) * g->nx1 * ncol;
   hipLaunchKernelGGL(vicp_forward_kernel, dim3(blocks_for(ncol, 64)), dim3(64), 0,

@@ -43,8 +43,13 @@ __global__ void __launch_bounds__(kLanes)
   __shared__ double sh_h[kLanes], sh_un[kLanes];
   int t = threadIdx.x;
   int i = r.lo1 - 1 + int(blockIdx.y) * kInner + t;
-  // blockIdx.x: (other axis, face) -- DIR 2: (k, j face); DIR 3: (k face, j)
-  int64_t y = blockIdx.x;
+  // blockIdx.x: (other axis, face) -- DIR 2: (k, j face); DIR 3: (k face, j).
+  // DIR 2 goes through the XCD map, so the neighbouring faces that share
+  // stencil lines share an L2; in DIR 3 the faces that share a line are n2
+  // workgroups apart (the same XCD when 8 divides n2), and the natural order
+  // keeps each XCD on an eighth of the lines, whose reuse distance then fits
+  // its L2
+  int64_t y = DIR == 2 ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   int j, k;
   int64_t stride;
   if (DIR == 2) {
