@@ -317,240 +317,304 @@ struct Rec {
   __device__ double& operator[](int e) const { return p[e * s]; }
 };
 
-constexpr int kLB = 4;           // layers staged per chunk
+constexpr int kLB = 2;           // layers the producer stages per chunk
 constexpr int kStage = kLB + 2;  // with the layer below and the layer above
+constexpr int kH = 30;           // hand-off per column and layer: m 9, b 9, c 9, rhs 3
 constexpr int kYL = 8;           // layers of y, z0, z1 gathered before a write
 constexpr int kYS = 9 * kYL + 1;  // LDS row of one column (padded)
-// the forward staging (w and du, 5 x kStage x 64 each) and the back
-// substitution's gather (64 columns x kYS) share one LDS buffer
-constexpr int kLds = 64 * kYS > 2 * 5 * kStage * 64 ? 64 * kYS : 2 * 5 * kStage * 64;
+// LDS: during the sweep, the producer's staging (w and du, 5 x kStage x 64
+// each) and the hand-off (kH x 64); during the back substitution, the
+// gather (64 x kYS)
+constexpr int kFwdLds = 2 * 5 * kStage * 64 + kH * 64;
+constexpr int kLds = 64 * kYS > kFwdLds ? 64 * kYS : kFwdLds;
 
-// the column program's serial part, one thread per column (64 per block):
-// each layer's blocks assembled in registers from the staged state (the
-// faces and Jacobians carried up from the layer below), the forward sweep
-// (ForwardSweep<2>, the small-matrix path) writing A' = a^-1 c and the three
-// partial solutions to the lane-contiguous records, then the back
-// substitution, which leaves y, z0, z1 of every layer in yc, column-major
-// (yc[(col * 9 + e) * n + l]) for the finish kernel's per-layer lanes: kYL
-// layers are gathered in LDS and written as runs along x1
-__global__ void __launch_bounds__(64)
+// The column program's serial part for 64 columns, two wavefronts:
+//   * the producer (wavefront 0), one lane per column, stages the state
+//     through LDS (coalesced along x1) and assembles layer i + 1's blocks
+//     (vic_assemble_cell, the upper face's |A| rows and the Jacobian rows
+//     carried up from the layer below, so each face is assembled once);
+//   * the consumer (wavefront 1), one lane per column, meanwhile runs the
+//     forward sweep of layer i (ForwardSweep<2>, the small-matrix path),
+//     writing A' = a^-1 c and the three partial solutions to the
+//     lane-contiguous records.
+// Layer i's blocks pass from producer to consumer through LDS; two
+// workgroup barriers per layer order the hand-off.  Both wavefronts then
+// run the back substitution's loop: the consumer solves, the gather of y,
+// z0, z1 (kYL layers in LDS) is written by both as runs along x1 into yc,
+// column-major (yc[(col * 9 + e) * n + l]) for the finish kernel.  The role
+// is a scalar value, so each wavefront takes one branch, and both branches
+// pass the same barriers (1 + 2 n in the sweep, 2 per gather flush).
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
     vicp_forward_kernel(double const* __restrict__ w, double const* __restrict__ du,
                         double const* __restrict__ vtab, double* __restrict__ ws,
                         double* __restrict__ yc, int* __restrict__ status, PartArgs a) {
   __shared__ double s_buf[kLds];
   __shared__ int64_t s_row[64];
   __shared__ int s_act[64];
-  double* s_w = s_buf;                    // [v][ll][cc]
+  double* s_w = s_buf;  // [v][ll][cc]
   double* s_du = s_buf + 5 * kStage * 64;
+  double* s_h = s_buf + 2 * 5 * kStage * 64;  // [e][cc]
   Range3 const& r = a.r;
-  int ncol = r.n2 * r.n3;  // < 2^31 (checked by the launcher)
-  int lane = threadIdx.x;
+  int ncol = r.n2 * r.n3;  // < 2^30 (checked by the launcher)
+  int const lane = threadIdx.x & 63;
+  bool const producer = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0;
   int col0 = blockIdx.x * 64;
   int col = col0 + lane;
   bool live = col < ncol;
   int n = a.n, nc1 = r.nc1, lo1 = r.lo1;
-  // the first interior cell of each of the block's columns
-  {
-    int jj = r.lo2 + col % r.n2, kk = r.lo3 + col / r.n2;
-    s_row[lane] = live ? kk * r.s3 + jj * r.s2 + r.lo1 : 0;
-  }
-  double const gm1 = a.gamma - 1.;
-  double const gme = 0.5 * (a.gamma + a.gamma) - 1.;
-  GasK const K{gme, 1. / gme, (gme + 1.) / gme, gme / (gme + 1.)};
-  double const gp_g = (gm1 + 1.) / gm1;
   double const A = a.area, V = a.vol;
-  double const hiv = 0.5 / V;
   double const dt = stage_dt(a);
   auto REC = [&](int l) { return Rec{ws + int64_t(l) * kRec * ncol + col, ncol}; };
-  auto tab = [&](int rowi, int l) { return vtab[rowi * nc1 + lo1 + l]; };
+  if (producer) {
+    // the first interior cell of each of the block's columns
+    // (column 0's for a lane past the grid, so its staging loads stay in
+    // bounds)
+    int cl = live ? col : 0;
+    int jj = r.lo2 + cl % r.n2, kk = r.lo3 + cl / r.n2;
+    s_row[lane] = kk * r.s3 + jj * r.s2 + r.lo1;
+  }
   __syncthreads();
-  double d2 = 0., dn3 = 0.;
-  if (live) {
-    int64_t row = s_row[lane];
-    d2 = du[IDN * r.sv + row + 2];
-    dn3 = du[IDN * r.sv + row + n - 3];
+
+  if (producer) {
+    // ---- producer: staging and assembly ------------------------------------
+    double const gm1 = a.gamma - 1.;
+    double const gme = 0.5 * (a.gamma + a.gamma) - 1.;
+    GasK const K{gme, 1. / gme, (gme + 1.) / gme, gme / (gme + 1.)};
+    double const gp_g = (gm1 + 1.) / gm1;
+    double const hiv = 0.5 / V;
+    auto tab = [&](int rowi, int l) { return vtab[rowi * nc1 + lo1 + l]; };
+    double d2 = 0., dn3 = 0.;
+    if (live) {
+      int64_t row = s_row[lane];
+      d2 = du[IDN * r.sv + row + 2];
+      dn3 = du[IDN * r.sv + row + n - 3];
+    }
+    // stage layers c0 - 1 .. c0 + kLB of w and du for the 64 columns, lanes
+    // along x1 within a column (coalesced runs); only this wavefront reads
+    // the staging area
+    // Every load is unconditional and in bounds (the layer index clamped to
+    // the ghost layers of w and the interior of du; a column past the grid
+    // reads column 0), so a chunk's 10 kStage loads are in flight together;
+    // the out-of-range values are then replaced.
+    auto stage = [&](int c0) {
+      double xw[kStage][5], xd[kStage][5];
+#pragma unroll
+      for (int t = 0; t < kStage; ++t) {
+        int e = lane + 64 * t;
+        int cc = e / kStage, ll = e % kStage, layer = c0 - 1 + ll;
+        int64_t grow = s_row[cc];
+        int lw = layer > n ? n : layer;
+        int ld = layer < 0 ? 0 : (layer > n - 1 ? n - 1 : layer);
+        for (int v = 0; v < 5; ++v) {
+          xw[t][v] = w[v * r.sv + grow + lw];
+          xd[t][v] = du[v * r.sv + grow + ld];
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int t = 0; t < kStage; ++t) {
+        int e = lane + 64 * t;
+        int cc = e / kStage, ll = e % kStage, layer = c0 - 1 + ll;
+        bool in = col0 + cc < ncol;
+        for (int v = 0; v < 5; ++v) {
+          s_w[(v * kStage + ll) * 64 + cc] = in && layer <= n ? xw[t][v] : 1.;
+          s_du[(v * kStage + ll) * 64 + cc] = in && layer >= 0 && layer < n ? xd[t][v] : 0.;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    };
+    // the lower face's |A| rows and the Jacobian rows of cells i - 1, i,
+    // carried from the layer below
+    double fa_lo[3][5], jp[3][5], jc[3][5];
+    // assemble layer i from the staged chunk and hand it to the consumer
+    auto assemble = [&](int i) {
+      int ll = i % kLB;
+      if (ll == 0) stage(i);
+      double wl[5], wc[5], wr[5], dl[5], dmv[5], dr[5];
+      for (int v = 0; v < 5; ++v) {
+        wl[v] = s_w[(v * kStage + ll) * 64 + lane];
+        wc[v] = s_w[(v * kStage + ll + 1) * 64 + lane];
+        wr[v] = s_w[(v * kStage + ll + 2) * 64 + lane];
+        dl[v] = s_du[(v * kStage + ll) * 64 + lane];
+        dmv[v] = s_du[(v * kStage + ll + 1) * 64 + lane];
+        dr[v] = s_du[(v * kStage + ll + 2) * 64 + lane];
+      }
+      if (i == 0) {
+        roe_rows3(fa_lo, K, wl, wc);
+        jac_rows(jp, gm1, gp_g, wl);
+        jac_rows(jc, gm1, gp_g, wc);
+      }
+      double fa_hi[3][5], jn[3][5];
+      roe_rows3(fa_hi, K, wc, wr);
+      jac_rows(jn, gm1, gp_g, wr);
+      CellTab ct;
+      ct.wlo = tab(V_WLO, i);
+      ct.whi = tab(V_WHI, i);
+      ct.klo = tab(V_KLO, i);
+      ct.kmid = tab(V_KMID, i);
+      ct.khi = tab(V_KHI, i);
+      ct.d2 = d2;
+      ct.dn3 = dn3;
+      double m[3][3], bb[3][3], c3[3][3], rhs[3];
+      assemble_cell(fa_lo, fa_hi, jp, jc, jn, wl, wc, wr, dl, dmv, dr, ct, i, n, A, hiv, a.grav,
+                    dt, a.t0, a.t1, m, bb, c3, rhs);
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 5; ++q) {
+          fa_lo[p][q] = fa_hi[p][q];
+          jp[p][q] = jc[p][q];
+          jc[p][q] = jn[p][q];
+        }
+      for (int p = 0; p < 3; ++p) {
+        for (int q = 0; q < 3; ++q) {
+          s_h[(3 * p + q) * 64 + lane] = m[p][q];
+          s_h[(9 + 3 * p + q) * 64 + lane] = bb[p][q];
+          s_h[(18 + 3 * p + q) * 64 + lane] = c3[p][q];
+        }
+        s_h[(27 + p) * 64 + lane] = rhs[p];
+      }
+    };
+    assemble(0);
+    __syncthreads();  // layer 0 handed off
+    for (int i = 0; i < n; ++i) {
+      __syncthreads();  // the consumer has taken layer i
+      if (i + 1 < n) assemble(i + 1);
+      __syncthreads();  // layer i + 1 handed off
+    }
+  } else {
+    // ---- consumer: the forward sweep ---------------------------------------
+    double ap[3][3], yp[3], z0p[3], z1p[3];
+    bool ok = live;
+    __syncthreads();  // layer 0 handed off
+    for (int i = 0; i < n; ++i) {
+      double m[3][3], bb[3][3], c3[3][3], rhs[3];
+      for (int p = 0; p < 3; ++p) {
+        for (int q = 0; q < 3; ++q) {
+          m[p][q] = s_h[(3 * p + q) * 64 + lane];
+          bb[p][q] = s_h[(9 + 3 * p + q) * 64 + lane];
+          c3[p][q] = s_h[(18 + 3 * p + q) * 64 + lane];
+        }
+        rhs[p] = s_h[(27 + p) * 64 + lane];
+      }
+      __syncthreads();  // layer i taken
+      if (ok) {
+        double e0[3] = {0., 0., i == 0 ? 1. : 0.};
+        double e1[3] = {0., 0., i == n - 1 ? 1. : 0.};
+        if (i > 0)
+          for (int p = 0; p < 3; ++p)
+            for (int s = 0; s < 3; ++s)
+              m[p][s] -= bb[p][0] * ap[0][s] + bb[p][1] * ap[1][s] + bb[p][2] * ap[2][s];
+        if (!ludcmp3_ok(m)) ok = false;
+        if (ok) {
+          double inv[3][3];
+          inverse3(m, inv);
+          double y[3], z0[3], z1[3];
+          if (i > 0) {
+            double t[3], s0[3], s1[3], by[3], b0[3], b1[3];
+            mv3(bb, yp, by);
+            mv3(bb, z0p, b0);
+            mv3(bb, z1p, b1);
+            for (int p = 0; p < 3; ++p) {
+              t[p] = rhs[p] - by[p];
+              s0[p] = e0[p] - b0[p];
+              s1[p] = e1[p] - b1[p];
+            }
+            mv3(inv, t, y);
+            mv3(inv, s0, z0);
+            mv3(inv, s1, z1);
+          } else {
+            mv3(inv, rhs, y);
+            mv3(inv, e0, z0);
+            mv3(inv, e1, z1);
+          }
+          for (int p = 0; p < 3; ++p)
+            for (int s = 0; s < 3; ++s)
+              ap[p][s] = inv[p][0] * c3[0][s] + inv[p][1] * c3[1][s] + inv[p][2] * c3[2][s];
+          for (int p = 0; p < 3; ++p) {
+            if (!isfinite(y[p])) ok = false;
+            for (int s = 0; s < 3; ++s)
+              if (!isfinite(ap[p][s])) ok = false;
+            yp[p] = y[p];
+            z0p[p] = z0[p];
+            z1p[p] = z1[p];
+          }
+          Rec rec = REC(i);
+          for (int p = 0; p < 3; ++p) {
+            for (int s = 0; s < 3; ++s) rec[R_A + 3 * p + s] = ap[p][s];
+            rec[R_Y + p] = y[p];
+            rec[R_Z0 + p] = z0[p];
+            rec[R_Z1 + p] = z1[p];
+          }
+        }
+      }
+      __syncthreads();  // layer i + 1 handed off
+    }
+    // a failed column keeps its explicit increments (the finish kernel
+    // reads the flag; the reference undoes the column with its status bits
+    // and flags a retry)
+    if (live) status[col] = ok ? 0 : 1;
+    s_act[lane] = ok;
   }
 
-  // ---- forward sweep (ForwardSweep<2>, the small-matrix path) --------------
-  double ap[3][3], yp[3], z0p[3], z1p[3];
-  bool ok = true;
-  // the lower face's |A| rows and the Jacobian rows of cells i - 1, i,
-  // carried from the layer below
-  double fa_lo[3][5], jp[3][5], jc[3][5];
-  // every lane stages its share of every chunk (even after its own column
-  // has failed), so the loop runs to the top for all lanes
-  for (int c0l = 0; c0l < n; c0l += kLB) {
-    // stage layers c0l - 1 .. c0l + kLB of w and du for the wavefront's 64
-    // columns: lanes run along x1 within a column (coalesced runs)
-    __syncthreads();
-    for (int e = lane; e < 64 * kStage; e += 64) {
-      int cc = e / kStage, ll = e % kStage, layer = c0l - 1 + ll;
-      bool in = col0 + cc < ncol;
-      int64_t grow = s_row[cc];
-      for (int v = 0; v < 5; ++v) {
-        s_w[(v * kStage + ll) * 64 + cc] = in && layer <= n ? w[v * r.sv + grow + layer] : 1.;
-        s_du[(v * kStage + ll) * 64 + cc] =
-            in && layer >= 0 && layer < n ? du[v * r.sv + grow + layer] : 0.;
-      }
-    }
-    __syncthreads();
-    if (!live || !ok) continue;
-    for (int ll = 0; ll < kLB && c0l + ll < n && ok; ++ll) {
-    int i = c0l + ll;
-    Rec rec = REC(i);
-    double wl[5], wc[5], wr[5], dl[5], dmv[5], dr[5];
-    for (int v = 0; v < 5; ++v) {
-      wl[v] = s_w[(v * kStage + ll) * 64 + lane];
-      wc[v] = s_w[(v * kStage + ll + 1) * 64 + lane];
-      wr[v] = s_w[(v * kStage + ll + 2) * 64 + lane];
-      dl[v] = s_du[(v * kStage + ll) * 64 + lane];
-      dmv[v] = s_du[(v * kStage + ll + 1) * 64 + lane];
-      dr[v] = s_du[(v * kStage + ll + 2) * 64 + lane];
-    }
-    if (i == 0) {
-      roe_rows3(fa_lo, K, wl, wc);
-      jac_rows(jp, gm1, gp_g, wl);
-      jac_rows(jc, gm1, gp_g, wc);
-    }
-    double fa_hi[3][5], jn[3][5];
-    roe_rows3(fa_hi, K, wc, wr);
-    jac_rows(jn, gm1, gp_g, wr);
-    CellTab ct;
-    ct.wlo = tab(V_WLO, i);
-    ct.whi = tab(V_WHI, i);
-    ct.klo = tab(V_KLO, i);
-    ct.kmid = tab(V_KMID, i);
-    ct.khi = tab(V_KHI, i);
-    ct.d2 = d2;
-    ct.dn3 = dn3;
-    double m[3][3], bb[3][3], c3[3][3], rhs[3];
-    assemble_cell(fa_lo, fa_hi, jp, jc, jn, wl, wc, wr, dl, dmv, dr, ct, i, n, A, hiv, a.grav, dt,
-                  a.t0, a.t1, m, bb, c3, rhs);
-    // the faces and Jacobians carried to the layer above
-    for (int p = 0; p < 3; ++p)
-      for (int q = 0; q < 5; ++q) {
-        fa_lo[p][q] = fa_hi[p][q];
-        jp[p][q] = jc[p][q];
-        jc[p][q] = jn[p][q];
-      }
-    double e0[3] = {0., 0., i == 0 ? 1. : 0.};
-    double e1[3] = {0., 0., i == n - 1 ? 1. : 0.};
-    if (i > 0)
-      for (int p = 0; p < 3; ++p)
-        for (int s = 0; s < 3; ++s)
-          m[p][s] -= bb[p][0] * ap[0][s] + bb[p][1] * ap[1][s] + bb[p][2] * ap[2][s];
-    if (!ludcmp3_ok(m)) {
-      ok = false;
-      break;
-    }
-    double inv[3][3];
-    inverse3(m, inv);
-    double y[3], z0[3], z1[3];
-    if (i > 0) {
-      double t[3], s0[3], s1[3], by[3], b0[3], b1[3];
-      mv3(bb, yp, by);
-      mv3(bb, z0p, b0);
-      mv3(bb, z1p, b1);
-      for (int p = 0; p < 3; ++p) {
-        t[p] = rhs[p] - by[p];
-        s0[p] = e0[p] - b0[p];
-        s1[p] = e1[p] - b1[p];
-      }
-      mv3(inv, t, y);
-      mv3(inv, s0, z0);
-      mv3(inv, s1, z1);
-    } else {
-      mv3(inv, rhs, y);
-      mv3(inv, e0, z0);
-      mv3(inv, e1, z1);
-    }
-    for (int p = 0; p < 3; ++p)
-      for (int s = 0; s < 3; ++s)
-        ap[p][s] = inv[p][0] * c3[0][s] + inv[p][1] * c3[1][s] + inv[p][2] * c3[2][s];
-    for (int p = 0; p < 3; ++p) {
-      if (!isfinite(y[p])) ok = false;
-      for (int s = 0; s < 3; ++s)
-        if (!isfinite(ap[p][s])) ok = false;
-      yp[p] = y[p];
-      z0p[p] = z0[p];
-      z1p[p] = z1[p];
-    }
-    for (int p = 0; p < 3; ++p) {
-      for (int s = 0; s < 3; ++s) rec[R_A + 3 * p + s] = ap[p][s];
-      rec[R_Y + p] = y[p];
-      rec[R_Z0 + p] = z0[p];
-      rec[R_Z1 + p] = z1[p];
-    }
-  }
-  }
-  // a failed column keeps its explicit increments (the finish kernel reads
-  // the flag; the reference undoes the column with its status bits and
-  // flags a retry)
-  if (live) status[col] = ok ? 0 : 1;
-  bool act = live && ok;
-  __syncthreads();  // the staging buffer becomes the gather buffer
-  s_act[lane] = act;
   // ---- back substitution (vic_backward_substitute) -------------------------
-  // every lane runs the loop (the gather is shared); a lane without a solved
-  // column carries zeros and its values are not written
+  // the staging and hand-off areas become the gather (the last hand-off was
+  // read before the sweep's last barrier); both wavefronts run the loop, the
+  // consumer solves, both write the gathered layers
+  bool act = !producer && live && s_act[lane];
   double* s_y = s_buf;  // [cc][e * kYL + layer % kYL], row kYS
-  auto gather = [&](int l, double const* y, double const* z0, double const* z1) {
-    double* o = s_y + lane * kYS + (l % kYL);
-    for (int p = 0; p < 3; ++p) {
-      o[p * kYL] = y[p];
-      o[(3 + p) * kYL] = z0[p];
-      o[(6 + p) * kYL] = z1[p];
-    }
-  };
-  // layers l0 .. l0 + cnt - 1 of the block's columns, lanes along x1
-  auto flush = [&](int l0, int cnt) {
-    __syncthreads();
-    for (int t = lane; t < 64 * 9 * kYL; t += 64) {
-      int ll = t % kYL, pe = t / kYL, cc = pe / 9, e = pe % 9;
-      int c = col0 + cc;
-      if (ll < cnt && s_act[cc])
-        yc[(int64_t(c) * 9 + e) * n + l0 + ll] = s_y[cc * kYS + e * kYL + ll];
-    }
-    __syncthreads();
-  };
   double yn[3] = {0., 0., 0.}, z0n[3] = {0., 0., 0.}, z1n[3] = {0., 0., 0.};
   double nx[18];
+  for (int e = 0; e < 18; ++e) nx[e] = 0.;
   if (act) {
-    Rec last = REC(n - 1);
-    for (int p = 0; p < 3; ++p) {
-      yn[p] = last[R_Y + p];
-      z0n[p] = last[R_Z0 + p];
-      z1n[p] = last[R_Z1 + p];
-    }
-    Rec rec = REC(n - 2);
+    Rec rec = REC(n - 1);
     for (int e = 0; e < 18; ++e) nx[e] = rec[e];
-  } else {
-    for (int e = 0; e < 18; ++e) nx[e] = 0.;
   }
-  gather(n - 1, yn, z0n, z1n);
-  if ((n - 1) % kYL == 0) flush(n - 1, 1);
-  for (int i = n - 2; i >= 0; --i) {
-    double cur[18];
-    for (int e = 0; e < 18; ++e) cur[e] = nx[e];
-    // the record of the layer below, loaded while this layer is solved
-    if (act && i > 0) {
-      Rec rec = REC(i - 1);
-      for (int e = 0; e < 18; ++e) nx[e] = rec[e];
+  for (int i = n - 1; i >= 0; --i) {
+    if (!producer) {
+      double cur[18];
+      for (int e = 0; e < 18; ++e) cur[e] = nx[e];
+      // the record of the layer below, loaded while this layer is solved
+      if (act && i > 0) {
+        Rec rec = REC(i - 1);
+        for (int e = 0; e < 18; ++e) nx[e] = rec[e];
+      }
+      if (i == n - 1) {
+        for (int p = 0; p < 3; ++p) {
+          yn[p] = cur[R_Y + p];
+          z0n[p] = cur[R_Z0 + p];
+          z1n[p] = cur[R_Z1 + p];
+        }
+      } else {
+        double A3[3][3];
+        for (int p = 0; p < 3; ++p)
+          for (int q = 0; q < 3; ++q) A3[p][q] = cur[R_A + 3 * p + q];
+        double sy[3], s0[3], s1[3];
+        mv3(A3, yn, sy);
+        mv3(A3, z0n, s0);
+        mv3(A3, z1n, s1);
+        for (int p = 0; p < 3; ++p) {
+          yn[p] = cur[R_Y + p] - sy[p];
+          z0n[p] = cur[R_Z0 + p] - s0[p];
+          z1n[p] = cur[R_Z1 + p] - s1[p];
+        }
+      }
+      double* o = s_y + lane * kYS + (i % kYL);
+      for (int p = 0; p < 3; ++p) {
+        o[p * kYL] = yn[p];
+        o[(3 + p) * kYL] = z0n[p];
+        o[(6 + p) * kYL] = z1n[p];
+      }
     }
-    double A3[3][3];
-    for (int p = 0; p < 3; ++p)
-      for (int q = 0; q < 3; ++q) A3[p][q] = cur[R_A + 3 * p + q];
-    double sy[3], s0[3], s1[3];
-    mv3(A3, yn, sy);
-    mv3(A3, z0n, s0);
-    mv3(A3, z1n, s1);
-    for (int p = 0; p < 3; ++p) {
-      yn[p] = cur[R_Y + p] - sy[p];
-      z0n[p] = cur[R_Z0 + p] - s0[p];
-      z1n[p] = cur[R_Z1 + p] - s1[p];
+    if (i % kYL == 0) {
+      // layers i .. i + cnt - 1 of the block's columns, lanes along x1
+      int cnt = n - i < kYL ? n - i : kYL;
+      __syncthreads();
+      for (int t = threadIdx.x; t < 64 * 9 * kYL; t += 128) {
+        int ll = t % kYL, pe = t / kYL, cc = pe / 9, e = pe % 9;
+        if (ll < cnt && col0 + cc < ncol && s_act[cc])
+          yc[(int64_t(col0 + cc) * 9 + e) * n + i + ll] = s_y[cc * kYS + e * kYL + ll];
+      }
+      __syncthreads();
     }
-    gather(i, yn, z0n, z1n);
-    if (i % kYL == 0) flush(i, (n - i) < kYL ? (n - i) : kYL);
   }
 }
 
@@ -759,7 +823,7 @@ static int vicp_launch(const snapy_grid* g, const snapy_physics* p, const double
   if (ncol > (int64_t(1) << 30)) return fail(-9, "vic_partial: too many columns");
   // the records (kF per layer, lane-contiguous) then yc (9 per layer)
   double* yc = ws + int64_t(kF) * g->nx1 * ncol;
-  hipLaunchKernelGGL(vicp_forward_kernel, dim3(blocks_for(ncol, 64)), dim3(64), 0,
+  hipLaunchKernelGGL(vicp_forward_kernel, dim3(blocks_for(ncol, 64)), dim3(128), 0,
                      as_stream(stream), w, du, vtab, ws, yc, status, a);
   if (int e = launch_status("vic_partial_forward")) return e;
   size_t lds = size_t(9) * (g->nx1 + 1) * sizeof(double);
