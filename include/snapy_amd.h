@@ -351,6 +351,71 @@ int64_t snapy_budget_workspace(const snapy_grid *g);
 int snapy_budget(const snapy_grid *g, int dtype, const void *u, double *ws,
                  double *out, void *stream);
 
+/* ---- the work list: many x1 blocks of uneven size per launch ------------
+ *
+ * A batch of independent 1D (x1) ideal-gas blocks without sources (the
+ * serving mix's shock tubes), FP64.  Each block keeps its own buffers and its
+ * own device dt; every phase of the RK stage is one persistent launch over
+ * the concatenated items of all blocks (cells, faces, interior cells, ghost
+ * cells), a grid of at most 2048 workgroups striding over them.  The
+ * per-item arithmetic is the single-block kernels', so a block's state is
+ * bit-identical alone and in any batch.  The descriptor array and the prefix
+ * counts live in device memory; snapy_wl_counts fills the prefix counts on
+ * the host.
+ * Replaces: the per-block launches of MeshBlockImpl::forward
+ * (meshblock_advance.cpp:88-105) for many blocks, as the reference's Mesh
+ * runs its blocks one after another (mesh.cpp:74-110). */
+typedef struct snapy_wl_block {
+  int32_t nx1;          /* interior cells; nc1 = nx1 + 2 nghost */
+  int32_t bc_lo, bc_hi; /* SNAPY_BC_* of the two x1 faces */
+  int32_t pad;
+  double dx1, dx2, dx3; /* cell widths (x2, x3: the block's unit extents) */
+  double scale;         /* CFL factor 2^-redo * cfl */
+  double *u, *u0, *w;   /* (5, nc1) FP64: state, RK backup, primitives */
+  double *flux;         /* (5, nc1) FP64: the x1 face totals */
+  const double *bref;   /* outflow background w (5, nc1), or NULL */
+  double *dt;           /* the block's device dt */
+} snapy_wl_block;
+
+typedef struct snapy_work_list {
+  const snapy_wl_block *blocks; /* device array of nblocks descriptors */
+  const int32_t *prefix;        /* device: 4 x (nblocks + 1) prefix counts of
+                                   cells, faces, interior cells, ghost cells */
+  int32_t nblocks;
+  int32_t nghost;               /* 3 */
+  int64_t total[4];             /* the four item totals */
+} snapy_work_list;
+
+/* Host: the prefix counts (4 x (nblocks + 1)) and totals of a batch. */
+int snapy_wl_counts(const snapy_wl_block *blocks, int32_t nblocks,
+                    int32_t nghost, int32_t *prefix, int64_t *total);
+/* u0 <- u for every block (StateStore::backup at stage 0). */
+int snapy_wl_backup(const snapy_work_list *wl, const snapy_physics *p,
+                    void *stream);
+/* w <- cons2prim(u) over every cell of every block. */
+int snapy_wl_cons2prim(const snapy_work_list *wl, const snapy_physics *p,
+                       void *stream);
+/* The x1 face totals of every block (weno5/cp5 + HLLC/LMARS, walls).
+ * Replaces: call_recon x2 + call_riemann on x1 (sweep.cpp:105-157). */
+int snapy_wl_sweep_x1(const snapy_work_list *wl, const snapy_physics *p,
+                      void *stream);
+/* u <- w0 u0 + w1 u + w2 (-dt div F) on the interior cells, each block with
+ * its own dt.  Replaces: divergence + harp's combine for each block. */
+int snapy_wl_update(const snapy_work_list *wl, const snapy_physics *p,
+                    double w0, double w1, double w2, void *stream);
+/* The x1 ghost cells of u (outflow, reflecting, periodic, extrapolation).
+ * Replaces: apply_boundaries on x1 (bc_func.cpp:10-227). */
+int snapy_wl_boundaries(const snapy_work_list *wl, const snapy_physics *p,
+                        void *stream);
+/* Each block's dt = scale * min dx1 / (|v1| + c), from w.
+ * Replaces: HydroImpl::max_time_step + global_time_step per block. */
+int snapy_wl_cfl(const snapy_work_list *wl, const snapy_physics *p,
+                 void *stream);
+/* One RK stage of every block: backup (stage 0), cons2prim, sweep, update,
+ * boundaries. */
+int snapy_wl_stage(const snapy_work_list *wl, const snapy_physics *p,
+                   int stage, double w0, double w1, double w2, void *stream);
+
 /* ---- the cubed-sphere shallow-water stage -------------------------------
  *
  * Six gnomonic-equiangle panels per member (the reference's cubed-sphere

@@ -90,10 +90,44 @@ class CsMesh(ctypes.Structure):
     ]
 
 
+class WlBlock(ctypes.Structure):
+    """snapy_wl_block: one x1 block of a work list (sizes and device buffers)."""
+
+    _fields_ = [
+        ("nx1", ctypes.c_int32),
+        ("bc_lo", ctypes.c_int32),
+        ("bc_hi", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("dx1", ctypes.c_double),
+        ("dx2", ctypes.c_double),
+        ("dx3", ctypes.c_double),
+        ("scale", ctypes.c_double),
+        ("u", ctypes.c_void_p),
+        ("u0", ctypes.c_void_p),
+        ("w", ctypes.c_void_p),
+        ("flux", ctypes.c_void_p),
+        ("bref", ctypes.c_void_p),
+        ("dt", ctypes.c_void_p),
+    ]
+
+
+class WorkList(ctypes.Structure):
+    """snapy_work_list: the device descriptor array and prefix counts."""
+
+    _fields_ = [
+        ("blocks", ctypes.c_void_p),
+        ("prefix", ctypes.c_void_p),
+        ("nblocks", ctypes.c_int32),
+        ("nghost", ctypes.c_int32),
+        ("total", ctypes.c_int64 * 4),
+    ]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
 _GP = ctypes.POINTER(Grid)
+_WP = ctypes.POINTER(WorkList)
 _PP = ctypes.POINTER(Physics)
 _MP = ctypes.POINTER(CsMesh)
 
@@ -150,6 +184,14 @@ SIGNATURES = {
     "snapy_cs_seams": (_I, [_MP, _P, _P, _P, _P]),
     "snapy_cs_update": (_I, [_MP, _P, _P, _P, _P, _P, _D, _P, _D, _D, _D, _P]),
     "snapy_cs_cfl": (_I, [_MP, _P, _D, _P, _P]),
+    "snapy_wl_counts": (_I, [ctypes.POINTER(WlBlock), ctypes.c_int32, ctypes.c_int32, _P, _P]),
+    "snapy_wl_backup": (_I, [_WP, _PP, _P]),
+    "snapy_wl_cons2prim": (_I, [_WP, _PP, _P]),
+    "snapy_wl_sweep_x1": (_I, [_WP, _PP, _P]),
+    "snapy_wl_update": (_I, [_WP, _PP, _D, _D, _D, _P]),
+    "snapy_wl_boundaries": (_I, [_WP, _PP, _P]),
+    "snapy_wl_cfl": (_I, [_WP, _PP, _P]),
+    "snapy_wl_stage": (_I, [_WP, _PP, _I, _D, _D, _D, _P]),
 }
 
 _lib = None

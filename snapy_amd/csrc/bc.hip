@@ -101,45 +101,15 @@ __global__ void __launch_bounds__(256)
   int64_t off = row_offset(a, row);
   int src = side ? a.nc - a.ng - 1 : a.ng;
   int dst = (side ? a.nc - a.ng : 0) + g;
-  double sign = side ? 1. : -1.;
   int vn = a.axis;
-  T wi[5], bi[5], bg[5], d[5];
+  T wi[5], bi[5], bg[5], out[5];
   for (int v = 0; v < 5; ++v) {
     wi[v] = w[v * a.r.sv + off + src * a.st];
     bi[v] = b[v * a.r.sv + off + src * a.st];
     bg[v] = b[v * a.r.sv + off + dst * a.st];
   }
-  T gm = T(gd);
-  T c0 = sqrt(gm * bi[IPR] / bi[IDN]);
-  T c = sqrt(gm * wi[IPR] / wi[IDN]);
-  bool ok = isfinite(c) && c > T(0);
-  if (!ok) {
-    for (int v = 0; v < 5; ++v) wi[v] = bi[v];
-    c = c0;
-  }
-  for (int v = 0; v < 5; ++v) d[v] = wi[v] - bi[v];
-  T un = T(sign) * wi[vn];
-  bool advect = un >= T(0);
-  T impedance = bi[IDN] * c0;
-  T plus = (un + c >= T(0)) ? T(sign) * d[vn] + d[IPR] / impedance : T(0);
-  T minus = (un - c >= T(0)) ? T(sign) * d[vn] - d[IPR] / impedance : T(0);
-  T c02 = c0 * c0;
-  T entropy = advect ? d[IDN] - d[IPR] / c02 : T(0);
-  for (int v = 0; v < 5; ++v) d[v] = d[v] * T(advect ? 1 : 0);
-  d[vn] = T(sign) * T(0.5) * (plus + minus);
-  d[IPR] = T(0.5) * impedance * (plus - minus);
-  d[IDN] = entropy + d[IPR] / c02;
-  T alpha = T(1);
-  auto lower = [&](T base, T delta, T floor) {
-    T bound = delta < T(0) ? (base - floor) / (-delta) : T(1);
-    T cand = bound < T(1) ? T(0.99) * bound : bound;
-    alpha = cand < alpha ? cand : alpha;
-  };
-  lower(bg[IDN], d[IDN], T(dfloor));
-  lower(bg[IPR], d[IPR], T(pfloor));
-  alpha = alpha < T(0) ? T(0) : (alpha > T(1) ? T(1) : alpha);
-  for (int v = 0; v < 5; ++v)
-    w[v * a.r.sv + off + dst * a.st] = bg[v] + alpha * d[v];
+  radiating_ghost(wi, bi, bg, side, vn, gd, dfloor, pfloor, out);
+  for (int v = 0; v < 5; ++v) w[v * a.r.sv + off + dst * a.st] = out[v];
 }
 
 // W->U of the ghost layers of the outflow sides of one axis

@@ -257,6 +257,91 @@ __device__ __forceinline__ void lmars(const Prim5<T>& L, const Prim5<T>& R,
 }
 
 // ---------------------------------------------------------------------------
+// one face of a sweep: reconstruction + Riemann solver
+// ---------------------------------------------------------------------------
+// The face below cell f of a sweep line: p points at row IDN of cell f, sv
+// is the row stride and stride the cell stride along the sweep.  Density is
+// always weno5; velocity and pressure cp5 unless shock
+// (reconstruct.cpp:303-322).  With a gauge g the pressure row enters the
+// reconstruction as p - g (LMARS only).  n is the sweep's normal row
+// (0, 1, 2 for x1, x2, x3).
+template <typename T>
+__device__ __forceinline__ void face_flux_line(double const* p, int64_t sv, int64_t stride,
+                                               double gauge, double gammad, int riemann,
+                                               int shock, int n, T out[5], T* pface) {
+  Prim5<T> L, R;
+  T* lv[5] = {&L.r, &L.v[0], &L.v[1], &L.v[2], &L.p};
+  T* rv[5] = {&R.r, &R.v[0], &R.v[1], &R.v[2], &R.p};
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    double const* q = p + v * sv;
+    double gv = v == IPR ? gauge : 0.;
+    T c0 = T(q[-3 * stride] - gv), c1 = T(q[-2 * stride] - gv),
+      c2 = T(q[-stride] - gv), c3 = T(q[0] - gv), c4 = T(q[stride] - gv),
+      c5 = T(q[2 * stride] - gv);
+    // left state: right edge of cell f-1 (window c0..c4); right state: left
+    // edge of cell f (window c1..c5)
+    if (v == IDN || shock) {
+      *lv[v] = weno5_plus(c0, c1, c2, c3, c4);
+      *rv[v] = weno5_minus(c1, c2, c3, c4, c5);
+    } else {
+      *lv[v] = cp5_plus(c0, c1, c2, c3, c4);
+      *rv[v] = cp5_minus(c1, c2, c3, c4, c5);
+    }
+  }
+  T g = T(gammad);
+  if (riemann == SNAPY_RIEMANN_LMARS)
+    lmars(L, R, g, g, n, out, pface, T(gauge));
+  else
+    hllc(L, R, g, g, n, out, pface);
+}
+
+// ---------------------------------------------------------------------------
+// the outflow (radiating) ghost about the saved background (bc_func.cpp:
+// 110-220), ideal gas, no tracers, Cartesian: wi and bi the interior
+// boundary cell's state and background, bg the ghost cell's background,
+// side 0 / 1 the lower / upper face, vn the normal velocity row.  Writes
+// the ghost's primitive state.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void radiating_ghost(T wi[5], T const bi[5], T const bg[5], int side,
+                                                int vn, double gd, double dfloor,
+                                                double pfloor, T out[5]) {
+  double sign = side ? 1. : -1.;
+  T d[5];
+  T gm = T(gd);
+  T c0 = sqrt(gm * bi[IPR] / bi[IDN]);
+  T c = sqrt(gm * wi[IPR] / wi[IDN]);
+  bool ok = isfinite(c) && c > T(0);
+  if (!ok) {
+    for (int v = 0; v < 5; ++v) wi[v] = bi[v];
+    c = c0;
+  }
+  for (int v = 0; v < 5; ++v) d[v] = wi[v] - bi[v];
+  T un = T(sign) * wi[vn];
+  bool advect = un >= T(0);
+  T impedance = bi[IDN] * c0;
+  T plus = (un + c >= T(0)) ? T(sign) * d[vn] + d[IPR] / impedance : T(0);
+  T minus = (un - c >= T(0)) ? T(sign) * d[vn] - d[IPR] / impedance : T(0);
+  T c02 = c0 * c0;
+  T entropy = advect ? d[IDN] - d[IPR] / c02 : T(0);
+  for (int v = 0; v < 5; ++v) d[v] = d[v] * T(advect ? 1 : 0);
+  d[vn] = T(sign) * T(0.5) * (plus + minus);
+  d[IPR] = T(0.5) * impedance * (plus - minus);
+  d[IDN] = entropy + d[IPR] / c02;
+  T alpha = T(1);
+  auto lower = [&](T base, T delta, T floor) {
+    T bound = delta < T(0) ? (base - floor) / (-delta) : T(1);
+    T cand = bound < T(1) ? T(0.99) * bound : bound;
+    alpha = cand < alpha ? cand : alpha;
+  };
+  lower(bg[IDN], d[IDN], T(dfloor));
+  lower(bg[IPR], d[IPR], T(pfloor));
+  alpha = alpha < T(0) ? T(0) : (alpha > T(1) ? T(1) : alpha);
+  for (int v = 0; v < 5; ++v) out[v] = bg[v] + alpha * d[v];
+}
+
+// ---------------------------------------------------------------------------
 // Equation of state (src/eos/ideal_gas_impl.h, src/eos/ideal_gas.cpp)
 // ---------------------------------------------------------------------------
 template <typename T>

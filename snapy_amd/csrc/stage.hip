@@ -189,34 +189,10 @@ __global__ void __launch_bounds__(256)
                              : r.lo3 * r.s3 + j * r.s2 + i;
     gauge = w[IPR * r.sv + first];
   }
-  Prim5<T> L, R;
-  T* lv[5] = {&L.r, &L.v[0], &L.v[1], &L.v[2], &L.p};
-  T* rv[5] = {&R.r, &R.v[0], &R.v[1], &R.v[2], &R.p};
-#pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    double const* p = w + v * r.sv + base;
-    double gv = v == IPR ? gauge : 0.;
-    T c0 = T(p[-3 * stride] - gv), c1 = T(p[-2 * stride] - gv),
-      c2 = T(p[-stride] - gv), c3 = T(p[0] - gv), c4 = T(p[stride] - gv),
-      c5 = T(p[2 * stride] - gv);
-    // left state: right edge of cell f-1 (window c0..c4); right state: left
-    // edge of cell f (window c1..c5).  Density always weno5; velocity and
-    // pressure cp5 unless shock (reconstruct.cpp:303-322).
-    if (v == IDN || a.shock) {
-      *lv[v] = weno5_plus(c0, c1, c2, c3, c4);
-      *rv[v] = weno5_minus(c1, c2, c3, c4, c5);
-    } else {
-      *lv[v] = cp5_plus(c0, c1, c2, c3, c4);
-      *rv[v] = cp5_minus(c1, c2, c3, c4, c5);
-    }
-  }
-  T g = T(a.gammad);
   T out[5];
   T pface;
-  if (a.riemann == SNAPY_RIEMANN_LMARS)
-    lmars(L, R, g, g, DIR - 1, out, &pface, T(gauge));
-  else
-    hllc(L, R, g, g, DIR - 1, out, &pface);
+  face_flux_line(w + base, r.sv, stride, gauge, a.gammad, a.riemann, a.shock, DIR - 1, out,
+                 &pface);
   // boundary.prescribed at a wall plane (FluxRegister::apply_prescribed,
   // flux_register.cpp:707-748, planes and free rows from supply_prescribed,
   // seam_register.cpp:184-220): the momentum rows stay free, the mass and

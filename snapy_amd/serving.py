@@ -8,7 +8,10 @@ implicit workspace) fits in the free part of the budget.  Admitted requests
 keep their state on the device between steps; every round advances each
 active request by one RK3 step, each request on its own HIP stream (all at
 the default priority), so the stages of different requests overlap on the
-device while each request's own stages stay ordered.  A request leaves when
+device while each request's own stages stay ordered.  The shock tubes of a
+round are batched: one work list (`worklist.BlockBatch`) advances all of
+them with one launch per phase on a stream of their own, rebuilt when a
+tube joins or leaves; a tube's result is bit-identical batched or alone.  A request leaves when
 its step count is reached, freeing its slot and memory for the next
 arrival; its latency is completion time minus arrival time.
 
@@ -34,6 +37,7 @@ import torch
 from . import problems
 from .cubed import CubedSphereMesh
 from .meshblock import MeshBlock
+from .worklist import BlockBatch, eligible
 from .options import CoordinateOptions, HydroOptions, IntegratorOptions, MeshBlockOptions, MeshOptions
 
 RD, CP, GRAV = 287.0, 1004.0, 9.81
@@ -207,6 +211,8 @@ class _Slot:
     bytes: int
     admitted: float
     done_steps: int = 0
+    si: int = 0  # the index of the stream the request was admitted on
+    batched: bool = False  # stepped by the round's work list
 
 
 @dataclass
@@ -214,12 +220,15 @@ class Server:
     budget_bytes: int = 200 * 10**9
     max_slots: int = 16
     realtime: bool = True  # honour arrival times against the wall clock
+    batch_tubes: bool = True  # advance the shock tubes as one work list
     log: list = field(default_factory=list)
 
     def run(self, requests: list[Request]) -> list[Result]:
         """Serve every request; return their results in completion order."""
         pending = sorted(requests, key=lambda r: r.arrival)
         streams = [torch.cuda.Stream() for _ in range(self.max_slots)]
+        tube_stream = torch.cuda.Stream()
+        batch, batch_ids = None, ()
         free_streams = list(range(self.max_slots))
         active: list[_Slot] = []
         used = 0
@@ -246,15 +255,35 @@ class Server:
                     w = _initial_state(req.kind, block, req.params, req.seed)
                     vars = _initialize(block, w)
                 used += need
-                active.append(_Slot(req, block, vars, streams[si], need, now()))
+                sl = _Slot(req, block, vars, streams[si], need, now(), si=si)
+                if self.batch_tubes and isinstance(block, MeshBlock) and eligible(block):
+                    # the tube's steps run on the work list's stream, after
+                    # its initialization
+                    tube_stream.wait_stream(streams[si])
+                    sl.stream = tube_stream
+                    sl.batched = True
+                active.append(sl)
                 self.log.append(("admit", req.rid, now(), used))
             if not active:
                 # nothing to run until the next arrival
                 if pending and self.realtime:
                     time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
                 continue
-            # one RK3 step of every active request, each on its own stream
+            # one RK3 step of every active request: the tubes as one work
+            # list, every other request on its own stream
+            tubes = [sl for sl in active if sl.batched]
+            ids = tuple(sl.req.rid for sl in tubes)
+            if ids != batch_ids:
+                batch = BlockBatch([(sl.block, sl.vars) for sl in tubes]) if tubes else None
+                batch_ids = ids
+            if batch is not None:
+                with torch.cuda.stream(tube_stream):
+                    batch.step()
+                for sl in tubes:
+                    sl.done_steps += 1
             for sl in active:
+                if sl.batched:
+                    continue
                 with torch.cuda.stream(sl.stream):
                     dt = sl.block.device_time_step(sl.vars)
                     for s in range(3):
@@ -276,7 +305,7 @@ class Server:
                 b.release(sync=False)  # its stream was synchronized above
                 sl.vars.clear()
                 used -= sl.bytes
-                free_streams.append(streams.index(sl.stream))
+                free_streams.append(sl.si)
                 self.log.append(("done", sl.req.rid, now(), used))
             active = keep
         torch.cuda.synchronize()

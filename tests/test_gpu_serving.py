@@ -44,3 +44,17 @@ def test_admission_respects_the_budget():
     print(summary, flush=True)
     assert summary["all_finite"]
     assert 0.0 <= summary["p50_latency_s"] <= summary["p99_latency_s"]
+
+
+def test_tubes_batched_equal_their_own_launches():
+    """The shock tubes of the stream as one work list give the same states,
+    bit for bit, as each tube on its own captured stages."""
+    reqs = [r for r in serving.sample_requests(24, seed=9, rate=50.0, kmin=5, kmax=12, sizes="small")
+            if r.kind == "sod"][:5]
+    assert len(reqs) >= 3
+    print("tubes", [r.params["n"] for r in reqs], flush=True)
+    a = {r.rid: r for r in serving.Server(realtime=False, max_slots=8, batch_tubes=True).run(reqs)}
+    b = {r.rid: r for r in serving.Server(realtime=False, max_slots=8, batch_tubes=False).run(reqs)}
+    for r in reqs:
+        assert a[r.rid].finite
+        assert np.array_equal(a[r.rid].state, b[r.rid].state), r
