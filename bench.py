@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--implicit-scheme", type=int, default=None, choices=(1, 9),
                     help="override the convection deck's implicit scheme")
     ap.add_argument("--cubed-steps", type=int, default=50)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default workgroup sizes instead of tuning them for the workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain elementwise copy of 1 GiB (the HBM counter calibration)")
     ap.add_argument("--no-other-scheme", action="store_true",
@@ -453,6 +455,12 @@ def main():
         torch.cuda.empty_cache()
     block, vars, cells, cfg, dtype, b_alg = workload(args.workload, precision)
     print(f"# workload {cfg['workload']} {precision} ready", file=sys.stderr, flush=True)
+    if not args.no_autotune:
+        from snapy_amd import autotune
+
+        tuned = autotune.tune_block(block, vars, reps=5)
+        cfg["launch_shapes"] = {f: r["choice"] for f, r in tuned.items()}
+        print(f"# launch shapes {cfg['launch_shapes']}", file=sys.stderr, flush=True)
     el = timed_run(block, vars, steps, warmup)
     ms_per_step = el / steps * 1e3
     value = cells * 3 * steps / el

@@ -351,6 +351,26 @@ int64_t snapy_budget_workspace(const snapy_grid *g);
 int snapy_budget(const snapy_grid *g, int dtype, const void *u, double *ws,
                  double *out, void *stream);
 
+/* ---- launch-shape tuning --------------------------------------------------
+ *
+ * The workgroup size of a kernel family for blocks of one shape (nx1, nx2,
+ * nx3) and dtype, or the persistent grid of the work list (any shape): set
+ * by the project's autotuner, read at every launch.  A tuned shape only
+ * changes how items are grouped into workgroups, so results are bit-identical
+ * for every value.  Workgroup sizes: 64, 128 or 256 (default 256); the work
+ * list's grid: 64 .. 65536 workgroups (default 2048).  value 0 restores the
+ * default.  No counterpart in the reference (its CUDA launches use fixed
+ * shapes). */
+enum snapy_tunable {
+  SNAPY_TUNE_EOS = 0,     /* cons2prim / prim2cons (FP64 and the FP32 pair) */
+  SNAPY_TUNE_SWEEP = 1,   /* snapy_sweep (x1, x2, x3) and snapy_sweep_x1_wb */
+  SNAPY_TUNE_UPDATE = 2,  /* snapy_update(_f32), snapy_update_gravity(_f32) */
+  SNAPY_TUNE_WL_GRID = 3, /* the work list's persistent grid */
+  SNAPY_TUNE_COUNT = 4
+};
+int snapy_tune_set(int tunable, const snapy_grid *g, int dtype, int value);
+int snapy_tune_get(int tunable, const snapy_grid *g, int dtype);
+
 /* ---- the work list: many x1 blocks of uneven size per launch ------------
  *
  * A batch of independent 1D (x1) ideal-gas blocks without sources (the

@@ -184,6 +184,8 @@ SIGNATURES = {
     "snapy_cs_seams": (_I, [_MP, _P, _P, _P, _P]),
     "snapy_cs_update": (_I, [_MP, _P, _P, _P, _P, _P, _D, _P, _D, _D, _D, _P]),
     "snapy_cs_cfl": (_I, [_MP, _P, _D, _P, _P]),
+    "snapy_tune_set": (_I, [_I, _GP, _I, _I]),
+    "snapy_tune_get": (_I, [_I, _GP, _I]),
     "snapy_wl_counts": (_I, [ctypes.POINTER(WlBlock), ctypes.c_int32, ctypes.c_int32, _P, _P]),
     "snapy_wl_backup": (_I, [_WP, _PP, _P]),
     "snapy_wl_cons2prim": (_I, [_WP, _PP, _P]),

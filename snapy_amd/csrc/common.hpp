@@ -48,6 +48,10 @@ inline unsigned blocks_for(int64_t n, int tpb) {
   return static_cast<unsigned>((n + tpb - 1) / tpb);
 }
 
+// the tuned workgroup size (or work-list grid) of a kernel family for a
+// block of this shape and dtype, dflt when none is set (tune.hip)
+unsigned tuned(int tunable, snapy_grid const* g, int dtype, unsigned dflt);
+
 // MI355X dispatches workgroup b of a launch to XCD b mod 8, and each XCD has
 // its own L2.  Kernels whose neighbouring tiles share input (stencil halos)
 // take their tile index through this map, so that each XCD works on one

@@ -428,15 +428,16 @@ int snapy_sweep_x1_wb(const snapy_grid* g, const snapy_physics* p, int dtype,
   a.wall_lo = a.refl_lo;
   a.wall_hi = a.refl_hi;
   int64_t tot = int64_t(a.nf) * g->nx2 * g->nx3;
+  unsigned tpb = tuned(SNAPY_TUNE_SWEEP, g, dtype, 256);
   if (dtype == SNAPY_F64)
     hipLaunchKernelGGL((sweep_x1_wb_kernel<double, double>),
-                       dim3(blocks_for(tot, 256)), dim3(256), 0,
+                       dim3(blocks_for(tot, tpb)), dim3(tpb), 0,
                        as_stream(stream), static_cast<double const*>(w), ref,
                        static_cast<double*>(flux),
                        static_cast<double*>(face_pressure), a);
   else
     hipLaunchKernelGGL((sweep_x1_wb_kernel<double, float>),
-                       dim3(blocks_for(tot, 256)), dim3(256), 0,
+                       dim3(blocks_for(tot, tpb)), dim3(tpb), 0,
                        as_stream(stream), static_cast<double const*>(w), ref,
                        static_cast<float*>(flux),
                        static_cast<double*>(face_pressure), a);
@@ -471,8 +472,9 @@ int snapy_update_gravity(const snapy_grid* g, const snapy_physics* p,
   int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
   auto kern = u0 ? update_gravity_kernel<double, false, false>
                  : update_gravity_kernel<double, true, false>;
+  unsigned tpb = tuned(SNAPY_TUNE_UPDATE, g, SNAPY_F64, 256);
   hipLaunchKernelGGL(kern,
-                     dim3(blocks_for(tot, 256)), dim3(256), 0,
+                     dim3(blocks_for(tot, tpb)), dim3(tpb), 0,
                      as_stream(stream), static_cast<double const*>(flux[0]),
                      static_cast<double const*>(flux[1]),
                      static_cast<double const*>(flux[2]), u0, nullptr, u,
@@ -514,7 +516,8 @@ int snapy_update_gravity_f32(const snapy_grid* g, const snapy_physics* p,
   auto kern = du_out ? update_gravity_kernel<float, true, true>
                      : update_gravity_kernel<float, false, true>;
   void* uo = du_out ? static_cast<void*>(du_out) : static_cast<void*>(u);
-  hipLaunchKernelGGL(kern, dim3(blocks_for(tot, 256)), dim3(256), 0,
+  unsigned tpb = tuned(SNAPY_TUNE_UPDATE, g, SNAPY_F32, 256);
+  hipLaunchKernelGGL(kern, dim3(blocks_for(tot, tpb)), dim3(tpb), 0,
                      as_stream(stream), static_cast<float const*>(flux[0]),
                      static_cast<float const*>(flux[1]),
                      static_cast<float const*>(flux[2]), u0, c0, uo, c, w,

@@ -472,8 +472,9 @@ int snapy_cons2prim(const snapy_grid* g, const snapy_physics* p, int dtype,
   Range3 r = ranges(*g);
   return dispatch(dtype, [&](auto z) {
     using T = decltype(z);
-    hipLaunchKernelGGL(cons2prim_kernel<T>, dim3(blocks_for(r.sv, 256)),
-                       dim3(256), 0, as_stream(stream),
+    unsigned tpb = tuned(SNAPY_TUNE_EOS, g, dtype, 256);
+    hipLaunchKernelGGL(cons2prim_kernel<T>, dim3(blocks_for(r.sv, tpb)),
+                       dim3(tpb), 0, as_stream(stream),
                        static_cast<T const*>(u), static_cast<T*>(w), r.sv,
                        r.sv, p->gammad);
     return launch_status("cons2prim");
@@ -490,8 +491,9 @@ int snapy_prim2cons(const snapy_grid* g, const snapy_physics* p, int dtype,
   Range3 r = ranges(*g);
   return dispatch(dtype, [&](auto z) {
     using T = decltype(z);
-    hipLaunchKernelGGL(prim2cons_kernel<T>, dim3(blocks_for(r.sv, 256)),
-                       dim3(256), 0, as_stream(stream),
+    unsigned tpb = tuned(SNAPY_TUNE_EOS, g, dtype, 256);
+    hipLaunchKernelGGL(prim2cons_kernel<T>, dim3(blocks_for(r.sv, tpb)),
+                       dim3(tpb), 0, as_stream(stream),
                        static_cast<T const*>(w), static_cast<T*>(u), r.sv,
                        r.sv, T(p->gammad - 1.));
     return launch_status("prim2cons");
@@ -506,8 +508,9 @@ int snapy_cons2prim_f32(const snapy_grid* g, const snapy_physics* p,
   if (!u || !c || !w) return fail(-1, "cons2prim_f32: NULL buffer");
   if (g->nvar != 5) return fail(-6, "cons2prim_f32: ideal gas needs nvar 5");
   Range3 r = ranges(*g);
-  hipLaunchKernelGGL(cons2prim_pair_kernel, dim3(blocks_for(r.sv, 256)),
-                     dim3(256), 0, as_stream(stream), u, c, w, r.sv, r.sv,
+  unsigned tpb = tuned(SNAPY_TUNE_EOS, g, SNAPY_F32, 256);
+  hipLaunchKernelGGL(cons2prim_pair_kernel, dim3(blocks_for(r.sv, tpb)),
+                     dim3(tpb), 0, as_stream(stream), u, c, w, r.sv, r.sv,
                      p->gammad);
   return launch_status("cons2prim_f32");
 }
@@ -519,8 +522,9 @@ int snapy_prim2cons_f32(const snapy_grid* g, const snapy_physics* p,
   if (!u || !c || !w) return fail(-1, "prim2cons_f32: NULL buffer");
   if (g->nvar != 5) return fail(-6, "prim2cons_f32: ideal gas needs nvar 5");
   Range3 r = ranges(*g);
-  hipLaunchKernelGGL(prim2cons_pair_kernel, dim3(blocks_for(r.sv, 256)),
-                     dim3(256), 0, as_stream(stream), w, u, c, r.sv, r.sv,
+  unsigned tpb = tuned(SNAPY_TUNE_EOS, g, SNAPY_F32, 256);
+  hipLaunchKernelGGL(prim2cons_pair_kernel, dim3(blocks_for(r.sv, tpb)),
+                     dim3(tpb), 0, as_stream(stream), w, u, c, r.sv, r.sv,
                      p->gammad - 1.);
   return launch_status("prim2cons_f32");
 }
@@ -544,7 +548,8 @@ int snapy_update_f32(const snapy_grid* g, const void* const flux[3],
   a.w1 = w1;
   a.w2 = w2;
   int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
-  hipLaunchKernelGGL(update_pair_kernel, dim3(blocks_for(tot, 256)), dim3(256),
+  unsigned tpb = tuned(SNAPY_TUNE_UPDATE, g, SNAPY_F32, 256);
+  hipLaunchKernelGGL(update_pair_kernel, dim3(blocks_for(tot, tpb)), dim3(tpb),
                      0, as_stream(stream), static_cast<float const*>(flux[0]),
                      static_cast<float const*>(flux[1]),
                      static_cast<float const*>(flux[2]), u0, c0, u, c, du_src,
@@ -588,7 +593,8 @@ int snapy_sweep(int dim, const snapy_grid* g, const snapy_physics* p,
     auto wp = static_cast<double const*>(w);
     auto fl = static_cast<T*>(flux);
     auto fp = static_cast<double*>(face_pressure);
-    dim3 grid(blocks_for(tot, 256)), blk(256);
+    unsigned tpb = tuned(SNAPY_TUNE_SWEEP, g, dtype, 256);
+    dim3 grid(blocks_for(tot, tpb)), blk(tpb);
     hipStream_t s = as_stream(stream);
     if (dim == 1)
       hipLaunchKernelGGL((sweep_kernel<T, 1>), grid, blk, 0, s, wp, fl, fp, a);
@@ -621,7 +627,8 @@ int snapy_update(const snapy_grid* g, int dtype, const void* const flux[3],
   int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
   return dispatch(dtype, [&](auto z) {
     using T = decltype(z);
-    hipLaunchKernelGGL(update_kernel<T>, dim3(blocks_for(tot, 256)), dim3(256),
+    unsigned tpb = tuned(SNAPY_TUNE_UPDATE, g, dtype, 256);
+    hipLaunchKernelGGL(update_kernel<T>, dim3(blocks_for(tot, tpb)), dim3(tpb),
                        0, as_stream(stream), static_cast<T const*>(flux[0]),
                        static_cast<T const*>(flux[1]),
                        static_cast<T const*>(flux[2]),

@@ -7,8 +7,8 @@
 // descriptors in device memory (each block keeps its own buffers and its own
 // device dt), and every phase of the RK stage is one persistent launch over
 // the concatenation of the blocks' items (cells, faces, interior cells,
-// ghost cells): a grid of at most kWlGrid workgroups strides over the
-// items, and a thread finds its block by a binary search of the phase's
+// ghost cells): a grid of at most kWlGrid workgroups (or the tuned cap,
+// SNAPY_TUNE_WL_GRID) strides over the items, and a thread finds its block by a binary search of the phase's
 // prefix counts.
 //
 // The per-item arithmetic is the single-block kernels' (the same device
@@ -30,7 +30,7 @@ namespace snapy {
 namespace {
 
 constexpr int kWlTpb = 256;
-constexpr unsigned kWlGrid = 2048;  // the persistent grid's workgroups (8 per CU)
+constexpr unsigned kWlGrid = 2048;  // the persistent grid's default cap (8 per CU)
 enum { WL_CELLS = 0, WL_FACES = 1, WL_INTERIOR = 2, WL_GHOSTS = 3 };
 
 struct WlArgs {
@@ -219,8 +219,9 @@ int wl_args(snapy_work_list const* wl, snapy_physics const* p, int phase, WlArgs
 }
 
 unsigned wl_grid(int64_t total) {
+  unsigned cap = tuned(SNAPY_TUNE_WL_GRID, nullptr, 0, kWlGrid);
   unsigned nb = blocks_for(total, kWlTpb);
-  return nb < kWlGrid ? (nb > 0 ? nb : 1) : kWlGrid;
+  return nb < cap ? (nb > 0 ? nb : 1) : cap;
 }
 
 template <typename K>
