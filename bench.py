@@ -221,6 +221,29 @@ def kernel_bytes(block, name, sb=8, pair=False):
     return table.get(name, 0)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "convection3d_hbm_traffic.md")
+# stage launch -> the kernels it runs (FP64 form, FP32 pair form)
+STAGE_KERNELS = {
+    "vic_partial": (["vicp_forward_kernel", "vicp_finish_kernel<false>"],
+                    ["vicp_forward_kernel", "vicp_finish_kernel<true>"]),
+    "sweep_x1_wb": (["sweep_x1_wb_kernel<double, double>"], ["sweep_x1_wb_kernel<double, float>"]),
+    "stage_increment": (["update_gravity_kernel<double, true, false>"], ["update_gravity_kernel<float, true, true>"]),
+}
+
+
+def hbm_traffic(name, pair):
+    """HBM bytes per launch of a stage launch, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (calibrated against a plain copy kernel);
+    None when the launch was not profiled."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+        table = json.loads(text.split("```json", 1)[1].split("```", 1)[0])["kernels"]
+        kernels = STAGE_KERNELS[name][1 if pair else 0]
+        return sum(table[k]["read_bytes"] + table[k]["written_bytes"] for k in kernels)
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def timed_run(block, vars, steps, warmup):
     """Warm up, then time exactly `steps` RK3 steps (device dt, captured
     stages) between two synchronizations; returns seconds."""
@@ -481,7 +504,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": hbm_traffic(dom, dtype == "f32"),
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/convection3d_hbm_traffic.md)",
             "kernel_ms": kt,
             "stage_b_alg_per_cell": b_alg,
             "stage_frac": value * b_alg / (HBM_PEAK_GBS * 1e9),
