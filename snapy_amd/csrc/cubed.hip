@@ -270,7 +270,11 @@ template <int AX>
 __global__ void __launch_bounds__(256)
     cs_sweep_kernel(double const* __restrict__ w, double* __restrict__ flx,
                     double* __restrict__ edge, Geom gm, Dims d, int shock) {
-  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  // x3: neighbouring face lines share five of their six stencil rows, so the
+  // workgroups go through the XCD map (each XCD then sweeps a contiguous
+  // range of lines and the shared rows stay in its L2)
+  unsigned wg = AX == 3 ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+  int64_t t = int64_t(wg) * blockDim.x + threadIdx.x;
   int nf = d.n + 1;
   int64_t per = int64_t(d.n) * nf;
   if (t >= per * 6 * d.members) return;
