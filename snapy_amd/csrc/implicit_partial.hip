@@ -317,15 +317,15 @@ struct Rec {
   __device__ double& operator[](int e) const { return p[e * s]; }
 };
 
-constexpr int kLB = 2;           // layers the producer stages per chunk
-constexpr int kStage = kLB + 2;  // with the layer below and the layer above
+constexpr int kLB = 4;  // layers the producer stages per chunk (the layer
+                        // above each of the chunk's layers)
 constexpr int kH = 30;           // hand-off per column and layer: m 9, b 9, c 9, rhs 3
 constexpr int kYL = 8;           // layers of y, z0, z1 gathered before a write
 constexpr int kYS = 9 * kYL + 1;  // LDS row of one column (padded)
-// LDS: during the sweep, the producer's staging (w and du, 5 x kStage x 64
+// LDS: during the sweep, the producer's staging (w and du, 5 x kLB x 64
 // each) and the hand-off (kH x 64); during the back substitution, the
 // gather (64 x kYS)
-constexpr int kFwdLds = 2 * 5 * kStage * 64 + kH * 64;
+constexpr int kFwdLds = 2 * 5 * kLB * 64 + kH * 64;
 constexpr int kLds = 64 * kYS > kFwdLds ? 64 * kYS : kFwdLds;
 
 // The column program's serial part for 64 columns, two wavefronts:
@@ -352,8 +352,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
   __shared__ int64_t s_row[64];
   __shared__ int s_act[64];
   double* s_w = s_buf;  // [v][ll][cc]
-  double* s_du = s_buf + 5 * kStage * 64;
-  double* s_h = s_buf + 2 * 5 * kStage * 64;  // [e][cc]
+  double* s_du = s_buf + 5 * kLB * 64;
+  double* s_h = s_buf + 2 * 5 * kLB * 64;  // [e][cc]
   Range3 const& r = a.r;
   int ncol = r.n2 * r.n3;  // < 2^30 (checked by the launcher)
   int const lane = threadIdx.x & 63;
@@ -389,22 +389,23 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
       d2 = du[IDN * r.sv + row + 2];
       dn3 = du[IDN * r.sv + row + n - 3];
     }
-    // stage layers c0 - 1 .. c0 + kLB of w and du for the 64 columns, lanes
-    // along x1 within a column (coalesced runs); only this wavefront reads
-    // the staging area
-    // Every load is unconditional and in bounds (the layer index clamped to
-    // the ghost layers of w and the interior of du; a column past the grid
-    // reads column 0), so a chunk's 10 kStage loads are in flight together;
-    // the out-of-range values are then replaced.
+    // stage layers c0 + 1 .. c0 + kLB of w and du (the layer above each of
+    // the chunk's layers; the layer itself and the one below are carried in
+    // registers) for the 64 columns, lanes along x1 within a column
+    // (coalesced runs); only this wavefront reads the staging area.  Every
+    // load is unconditional and in bounds (the layer index clamped to the
+    // ghost layers of w and the interior of du; a column past the grid reads
+    // column 0), so a chunk's loads are in flight together; the out-of-range
+    // values are then replaced.
     auto stage = [&](int c0) {
-      double xw[kStage][5], xd[kStage][5];
+      double xw[kLB][5], xd[kLB][5];
 #pragma unroll
-      for (int t = 0; t < kStage; ++t) {
+      for (int t = 0; t < kLB; ++t) {
         int e = lane + 64 * t;
-        int cc = e / kStage, ll = e % kStage, layer = c0 - 1 + ll;
+        int cc = e / kLB, layer = c0 + 1 + e % kLB;
         int64_t grow = s_row[cc];
         int lw = layer > n ? n : layer;
-        int ld = layer < 0 ? 0 : (layer > n - 1 ? n - 1 : layer);
+        int ld = layer > n - 1 ? n - 1 : layer;
         for (int v = 0; v < 5; ++v) {
           xw[t][v] = w[v * r.sv + grow + lw];
           xd[t][v] = du[v * r.sv + grow + ld];
@@ -412,40 +413,46 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
       }
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
-      for (int t = 0; t < kStage; ++t) {
+      for (int t = 0; t < kLB; ++t) {
         int e = lane + 64 * t;
-        int cc = e / kStage, ll = e % kStage, layer = c0 - 1 + ll;
+        int cc = e / kLB, ll = e % kLB, layer = c0 + 1 + ll;
         bool in = col0 + cc < ncol;
         for (int v = 0; v < 5; ++v) {
-          s_w[(v * kStage + ll) * 64 + cc] = in && layer <= n ? xw[t][v] : 1.;
-          s_du[(v * kStage + ll) * 64 + cc] = in && layer >= 0 && layer < n ? xd[t][v] : 0.;
+          s_w[(v * kLB + ll) * 64 + cc] = in && layer <= n ? xw[t][v] : 1.;
+          s_du[(v * kLB + ll) * 64 + cc] = in && layer < n ? xd[t][v] : 0.;
         }
       }
       __builtin_amdgcn_wave_barrier();
     };
-    // the lower face's |A| rows and the Jacobian rows of cells i - 1, i,
-    // carried from the layer below
-    double fa_lo[3][5], jp[3][5], jc[3][5];
-    // assemble layer i from the staged chunk and hand it to the consumer
+    // carried from the layer below: the state and increments of cells i - 1
+    // and i, and the lower face's |A| rows (the Jacobian rows, cheap, are
+    // formed again each layer)
+    double wl[5], wc[5], dl[5], dmv[5], fa_lo[3][5];
+    {
+      int64_t row = s_row[lane];
+      for (int v = 0; v < 5; ++v) {
+        double x0 = w[v * r.sv + row - 1], x1 = w[v * r.sv + row], d0 = du[v * r.sv + row];
+        wl[v] = live ? x0 : 1.;
+        wc[v] = live ? x1 : 1.;
+        dl[v] = 0.;
+        dmv[v] = live ? d0 : 0.;
+      }
+      roe_rows3(fa_lo, K, wl, wc);
+    }
+    // assemble layer i from the carried cells and the staged layer above,
+    // and hand it to the consumer
     auto assemble = [&](int i) {
       int ll = i % kLB;
       if (ll == 0) stage(i);
-      double wl[5], wc[5], wr[5], dl[5], dmv[5], dr[5];
+      double wr[5], dr[5];
       for (int v = 0; v < 5; ++v) {
-        wl[v] = s_w[(v * kStage + ll) * 64 + lane];
-        wc[v] = s_w[(v * kStage + ll + 1) * 64 + lane];
-        wr[v] = s_w[(v * kStage + ll + 2) * 64 + lane];
-        dl[v] = s_du[(v * kStage + ll) * 64 + lane];
-        dmv[v] = s_du[(v * kStage + ll + 1) * 64 + lane];
-        dr[v] = s_du[(v * kStage + ll + 2) * 64 + lane];
+        wr[v] = s_w[(v * kLB + ll) * 64 + lane];
+        dr[v] = s_du[(v * kLB + ll) * 64 + lane];
       }
-      if (i == 0) {
-        roe_rows3(fa_lo, K, wl, wc);
-        jac_rows(jp, gm1, gp_g, wl);
-        jac_rows(jc, gm1, gp_g, wc);
-      }
-      double fa_hi[3][5], jn[3][5];
+      double fa_hi[3][5], jp[3][5], jc[3][5], jn[3][5];
       roe_rows3(fa_hi, K, wc, wr);
+      jac_rows(jp, gm1, gp_g, wl);
+      jac_rows(jc, gm1, gp_g, wc);
       jac_rows(jn, gm1, gp_g, wr);
       CellTab ct;
       ct.wlo = tab(V_WLO, i);
@@ -458,12 +465,14 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
       double m[3][3], bb[3][3], c3[3][3], rhs[3];
       assemble_cell(fa_lo, fa_hi, jp, jc, jn, wl, wc, wr, dl, dmv, dr, ct, i, n, A, hiv, a.grav,
                     dt, a.t0, a.t1, m, bb, c3, rhs);
+      for (int v = 0; v < 5; ++v) {
+        wl[v] = wc[v];
+        wc[v] = wr[v];
+        dl[v] = dmv[v];
+        dmv[v] = dr[v];
+      }
       for (int p = 0; p < 3; ++p)
-        for (int q = 0; q < 5; ++q) {
-          fa_lo[p][q] = fa_hi[p][q];
-          jp[p][q] = jc[p][q];
-          jc[p][q] = jn[p][q];
-        }
+        for (int q = 0; q < 5; ++q) fa_lo[p][q] = fa_hi[p][q];
       for (int p = 0; p < 3; ++p) {
         for (int q = 0; q < 3; ++q) {
           s_h[(3 * p + q) * 64 + lane] = m[p][q];
