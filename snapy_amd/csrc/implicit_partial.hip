@@ -632,7 +632,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
 // the Woodbury 2x2, the redistribution's serial chains on lane 0 in the
 // column's order (the sums, the prefix of the face transfers, the
 // availability clamps), then every cell's outputs, the tangential
-// transfers, the remainder and the RK combine.  Dynamic LDS: 9 (n + 1)
+// transfers, the remainder and the RK combine.  Dynamic LDS: 9 (n + 1) + 2
 // doubles.
 template <bool PAIR>
 __global__ void __launch_bounds__(64)
@@ -674,7 +674,7 @@ __global__ void __launch_bounds__(64)
   }
   int const L = n + 1;
   double* s_phi = lds;          // phi_i
-  double* s_wv = lds + L;       // w_rho V
+  double* s_wv = lds + L;       // w_rho V, then each cell's request term
   double* s_av = lds + 2 * L;   // (w_rho + du_rho) V
   double* s_M = lds + 3 * L;    // requested transfer through the face below i
   double* s_q = lds + 4 * L;    // moved transfer through the face below i
@@ -714,31 +714,67 @@ __global__ void __launch_bounds__(64)
   }
   __syncthreads();
   // ---- the serial chains of vic_constituent_column, in the column's order --
+  // the column sums on lane 0, in order; then every cell's term of the
+  // requested transfer on its own lane; then the prefix of the requests and
+  // the availability clamps on lane 0, reading their inputs in register
+  // batches (the same operations in the same order as the serial loops)
+  double* s_sum = lds + 9 * L;  // R, S
   if (lane == 0) {
     double R = 0., S = 0.;
     for (int i = 0; i < n; ++i) {
       R += s_phi[i];
       S += s_wv[i];
     }
+    s_sum[0] = R;
+    s_sum[1] = S;
+  }
+  __syncthreads();
+  {
+    double R = s_sum[0], S = s_sum[1];
+    // s_wv[i] becomes phi_i - R (w_rho V)_i / S
+    for (int l = lane; l < n; l += 64) s_wv[l] = s_phi[l] - R * (s_wv[l] / S);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    constexpr int kB = 16;
     double M = 0.;
-    for (int i = 0; i < n; ++i) {
-      s_M[i] = M;
-      M -= s_phi[i] - R * (s_wv[i] / S);
+    for (int i0 = 0; i0 < n; i0 += kB) {
+      double tt[kB];
+#pragma unroll
+      for (int t = 0; t < kB; ++t) tt[t] = s_wv[i0 + t < n ? i0 + t : n - 1];
+#pragma unroll
+      for (int t = 0; t < kB; ++t) {
+        if (i0 + t >= n) break;
+        s_M[i0 + t] = M;
+        M -= tt[t];
+      }
     }
     s_M[n] = 0.;
     s_q[0] = 0.;
     s_q[n] = 0.;
     double avail = s_av[0];
     if (avail < 0) avail = 0;
-    for (int i = 0; i + 1 < n; ++i) {
-      double avail_up = s_av[i + 1];
-      if (avail_up < 0) avail_up = 0;
-      double mf = s_M[i + 1];
-      double q = mf > 0 ? mf * 1. : mf * 1.;
-      if (q > avail) q = avail;
-      if (q < -avail_up) q = -avail_up;
-      s_q[i + 1] = q;
-      avail = avail_up + q;
+    for (int i0 = 0; i0 + 1 < n; i0 += kB) {
+      double av[kB], mm[kB];
+#pragma unroll
+      for (int t = 0; t < kB; ++t) {
+        int k = i0 + t + 1 < n ? i0 + t + 1 : n - 1;
+        av[t] = s_av[k];
+        mm[t] = s_M[k];
+      }
+#pragma unroll
+      for (int t = 0; t < kB; ++t) {
+        int i = i0 + t;
+        if (i + 1 >= n) break;
+        double avail_up = av[t];
+        if (avail_up < 0) avail_up = 0;
+        double mf = mm[t];
+        double q = mf > 0 ? mf * 1. : mf * 1.;
+        if (q > avail) q = avail;
+        if (q < -avail_up) q = -avail_up;
+        s_q[i + 1] = q;
+        avail = avail_up + q;
+      }
     }
   }
   __syncthreads();
@@ -835,7 +871,7 @@ static int vicp_launch(const snapy_grid* g, const snapy_physics* p, const double
   hipLaunchKernelGGL(vicp_forward_kernel, dim3(blocks_for(ncol, 64)), dim3(128), 0,
                      as_stream(stream), w, du, vtab, ws, yc, status, a);
   if (int e = launch_status("vic_partial_forward")) return e;
-  size_t lds = size_t(9) * (g->nx1 + 1) * sizeof(double);
+  size_t lds = (size_t(9) * (g->nx1 + 1) + 2) * sizeof(double);
   if (lds > 64 * 1024) return fail(-9, "vic_partial: the column does not fit the LDS");
   if (pair)
     hipLaunchKernelGGL(vicp_finish_kernel<true>, dim3(ncol), dim3(64), lds, as_stream(stream), w,
