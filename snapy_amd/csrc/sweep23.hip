@@ -4,11 +4,13 @@
 // (HydroImpl::_flux_covariance, src/hydro/hydro_forward.cpp:58-231, the
 // Cartesian branch :196-213; add_cov :782-793).
 //
-// The covariance of a face needs the face states of its two x1 neighbours,
-// so a workgroup of 64 lanes walks 62 consecutive x1 cells plus one halo
-// lane on each side: every lane reconstructs its own face states, the
-// enthalpy h and the normal velocity u_n go through LDS, and the 62 inner
-// lanes solve the Riemann problem and add the covariance.
+// The covariance of a face needs the face states of its two x1 neighbours:
+// every lane reconstructs its own face states, the enthalpy h and the normal
+// velocity u_n go through LDS, and the inner lanes solve the Riemann problem
+// and add the covariance.  When a whole x1 line (nx1 + 2 cells with the two
+// halo cells) fits, a workgroup packs several whole lines; otherwise a
+// workgroup of 64 lanes walks 62 consecutive x1 cells plus one halo lane on
+// each side.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -35,6 +37,85 @@ struct CovArgs {
 // a.gauge the pressure row is reconstructed as p - g, g the pressure of the
 // line's first interior cell along the sweep (the momentum flux stays
 // gauge-free; the enthalpy and the sound speed take p - g + g).
+//
+// The face states of the face below cell (i, j, k) along the sweep, and the
+// h and u_n the covariance of the neighbouring x1 faces reads.
+template <typename T, int DIR>
+__device__ __forceinline__ void cov_states(double const* __restrict__ w, CovArgs const& a, int i,
+                                           int j, int k, bool live, Prim5<T>& L, Prim5<T>& R,
+                                           double& h, double& un, double& gauge) {
+  Range3 const& r = a.r;
+  int64_t stride = DIR == 2 ? r.s2 : r.s3;
+  h = 0.;
+  un = 0.;
+  gauge = 0.;
+  if (live && a.gauge)
+    gauge = w[IPR * r.sv + (DIR == 2 ? k * r.s3 + r.lo2 * r.s2 + i
+                                     : r.lo3 * r.s3 + j * r.s2 + i)];
+  if (!live) return;
+  int64_t base = k * r.s3 + j * r.s2 + i;
+  T* lv[5] = {&L.r, &L.v[0], &L.v[1], &L.v[2], &L.p};
+  T* rv[5] = {&R.r, &R.v[0], &R.v[1], &R.v[2], &R.p};
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    double const* p = w + v * r.sv + base;
+    double gv = v == IPR ? gauge : 0.;
+    T c0 = T(p[-3 * stride] - gv), c1 = T(p[-2 * stride] - gv),
+      c2 = T(p[-stride] - gv), c3 = T(p[0] - gv), c4 = T(p[stride] - gv),
+      c5 = T(p[2 * stride] - gv);
+    if (v == IDN || a.shock) {
+      *lv[v] = weno5_plus(c0, c1, c2, c3, c4);
+      *rv[v] = weno5_minus(c1, c2, c3, c4, c5);
+    } else {
+      *lv[v] = cp5_plus(c0, c1, c2, c3, c4);
+      *rv[v] = cp5_minus(c1, c2, c3, c4, c5);
+    }
+  }
+  if (a.cov) {
+    // wbar = 0.5 (wl + wr); h = (W->I(wbar) + p) / rho; u_n
+    double rho = 0.5 * (double(L.r) + double(R.r));
+    double p = 0.5 * (double(L.p) + double(R.p)) + gauge;
+    double enth = p / (a.gammad - 1.) + p;
+    h = enth / rho;
+    un = 0.5 * (double(L.v[DIR - 1]) + double(R.v[DIR - 1]));
+  }
+}
+
+// The flux of an interior face from its states, the covariance from the
+// x1 neighbours' h and u_n (hm, hp, um, up), the wall prescription, stored.
+template <typename T, int DIR>
+__device__ __forceinline__ void cov_flux(double const* __restrict__ x1v, T* __restrict__ flx,
+                                         CovArgs const& a, int i, int j, int k,
+                                         Prim5<T> const& L, Prim5<T> const& R, double gauge,
+                                         double hm, double hp, double um, double up) {
+  Range3 const& r = a.r;
+  T g = T(a.gammad);
+  T out[5];
+  T pface;
+  if (a.riemann == SNAPY_RIEMANN_LMARS)
+    lmars(L, R, g, g, DIR - 1, out, &pface, T(gauge));
+  else
+    hllc(L, R, g, g, DIR - 1, out, &pface);
+  if (a.cov) {
+    double rho = 0.5 * (double(L.r) + double(R.r));
+    double dx1 = x1v[i + 1] - x1v[i - 1];
+    double d1h = (hp - hm) / dx1;
+    double dun = (up - um) / dx1;
+    double ce = ((a.s2 * rho) * d1h) * dun;
+    out[IPR] = T(double(out[IPR]) + ce);
+  }
+  int f = DIR == 2 ? j - r.lo2 : k - r.lo3;
+  if ((f == 0 && a.wall_lo) || (f == a.nf - 1 && a.wall_hi)) {
+    out[IDN] = T(0);
+    out[IPR] = T(0);
+  }
+  int64_t base = k * r.s3 + j * r.s2 + i;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) flx[v * r.sv + base] = out[v];
+}
+
+// Segmented lines: a workgroup of 64 lanes walks 62 consecutive x1 cells
+// plus one halo lane on each side (any nx1).
 template <typename T, int DIR>
 __global__ void __launch_bounds__(kLanes)
     sweep23_kernel(double const* __restrict__ w, double const* __restrict__ x1v,
@@ -51,79 +132,80 @@ __global__ void __launch_bounds__(kLanes)
   // its L2
   int64_t y = DIR == 2 ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   int j, k;
-  int64_t stride;
   if (DIR == 2) {
     j = r.lo2 + int(y % a.nf);
     k = r.lo3 + int(y / a.nf);
-    stride = r.s2;
   } else {
     j = r.lo2 + int(y % r.n2);
     k = r.lo3 + int(y / r.n2);
-    stride = r.s3;
   }
   bool live = i >= r.lo1 - 1 && i <= r.lo1 + r.n1;
   Prim5<T> L, R;
-  double h = 0., un = 0.;
-  double gauge = 0.;
-  if (live && a.gauge)
-    gauge = w[IPR * r.sv + (DIR == 2 ? k * r.s3 + r.lo2 * r.s2 + i
-                                     : r.lo3 * r.s3 + j * r.s2 + i)];
-  if (live) {
-    int64_t base = k * r.s3 + j * r.s2 + i;
-    T* lv[5] = {&L.r, &L.v[0], &L.v[1], &L.v[2], &L.p};
-    T* rv[5] = {&R.r, &R.v[0], &R.v[1], &R.v[2], &R.p};
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      double const* p = w + v * r.sv + base;
-      double gv = v == IPR ? gauge : 0.;
-      T c0 = T(p[-3 * stride] - gv), c1 = T(p[-2 * stride] - gv),
-        c2 = T(p[-stride] - gv), c3 = T(p[0] - gv), c4 = T(p[stride] - gv),
-        c5 = T(p[2 * stride] - gv);
-      if (v == IDN || a.shock) {
-        *lv[v] = weno5_plus(c0, c1, c2, c3, c4);
-        *rv[v] = weno5_minus(c1, c2, c3, c4, c5);
-      } else {
-        *lv[v] = cp5_plus(c0, c1, c2, c3, c4);
-        *rv[v] = cp5_minus(c1, c2, c3, c4, c5);
-      }
-    }
-    if (a.cov) {
-      // wbar = 0.5 (wl + wr); h = (W->I(wbar) + p) / rho; u_n
-      double rho = 0.5 * (double(L.r) + double(R.r));
-      double p = 0.5 * (double(L.p) + double(R.p)) + gauge;
-      double enth = p / (a.gammad - 1.) + p;
-      h = enth / rho;
-      un = 0.5 * (double(L.v[DIR - 1]) + double(R.v[DIR - 1]));
-    }
-  }
+  double h, un, gauge;
+  cov_states<T, DIR>(w, a, i, j, k, live, L, R, h, un, gauge);
   sh_h[t] = h;
   sh_un[t] = un;
   __syncthreads();
   if (t == 0 || t == kLanes - 1) return;
   if (!(i >= r.lo1 && i < r.lo1 + r.n1)) return;
-  T g = T(a.gammad);
-  T out[5];
-  T pface;
-  if (a.riemann == SNAPY_RIEMANN_LMARS)
-    lmars(L, R, g, g, DIR - 1, out, &pface, T(gauge));
-  else
-    hllc(L, R, g, g, DIR - 1, out, &pface);
-  if (a.cov) {
-    double rho = 0.5 * (double(L.r) + double(R.r));
-    double dx1 = x1v[i + 1] - x1v[i - 1];
-    double d1h = (sh_h[t + 1] - sh_h[t - 1]) / dx1;
-    double dun = (sh_un[t + 1] - sh_un[t - 1]) / dx1;
-    double ce = ((a.s2 * rho) * d1h) * dun;
-    out[IPR] = T(double(out[IPR]) + ce);
+  cov_flux<T, DIR>(x1v, flx, a, i, j, k, L, R, gauge, sh_h[t - 1], sh_h[t + 1], sh_un[t - 1],
+                   sh_un[t + 1]);
+}
+
+// Packed lines (nx1 + 2 <= kPackMax): a workgroup holds `lines` whole x1
+// lines of nx1 + 2 lanes each (the two halo cells included), so few lanes
+// idle; its lines are consecutive faces along the sweep (the face index
+// fastest), which share five of their six stencil rows, and the workgroups
+// go through the XCD map so neighbouring workgroups share an L2.
+constexpr int kPackThreads = 1024;
+constexpr int kPackMax = 512;
+
+template <typename T, int DIR>
+__global__ void __launch_bounds__(kPackThreads)
+    sweep23_packed_kernel(double const* __restrict__ w, double const* __restrict__ x1v,
+                          T* __restrict__ flx, CovArgs a, int lines) {
+  Range3 const& r = a.r;
+  __shared__ double sh_h[kPackThreads], sh_un[kPackThreads];
+  int t = threadIdx.x;
+  int len = r.n1 + 2;
+  int q = t / len, p = t - q * len;
+  int64_t nline = int64_t(a.nf) * (DIR == 2 ? r.n3 : r.n2);
+  int64_t y = int64_t(xcd_tile(blockIdx.x, gridDim.x)) * lines + q;
+  bool in = q < lines && y < nline;
+  int i = r.lo1 - 1 + p;
+  int j, k;
+  if (DIR == 2) {
+    j = r.lo2 + int(y % a.nf);
+    k = r.lo3 + int(y / a.nf);
+  } else {
+    k = r.lo3 + int(y % a.nf);
+    j = r.lo2 + int(y / a.nf);
   }
-  int f = DIR == 2 ? j - r.lo2 : k - r.lo3;
-  if ((f == 0 && a.wall_lo) || (f == a.nf - 1 && a.wall_hi)) {
-    out[IDN] = T(0);
-    out[IPR] = T(0);
+  Prim5<T> L, R;
+  double h, un, gauge;
+  cov_states<T, DIR>(w, a, i, j, k, in, L, R, h, un, gauge);
+  sh_h[t] = h;
+  sh_un[t] = un;
+  __syncthreads();
+  if (!in || p == 0 || p == len - 1) return;
+  cov_flux<T, DIR>(x1v, flx, a, i, j, k, L, R, gauge, sh_h[t - 1], sh_h[t + 1], sh_un[t - 1],
+                   sh_un[t + 1]);
+}
+
+// lines per packed workgroup: the count whose lanes waste the least of
+// their whole wavefronts (ties: fewer lines)
+int packed_lines(int len) {
+  int best = 1;
+  double eff = 0.;
+  for (int P = 1; P * len <= kPackThreads; ++P) {
+    int lanes = P * len;
+    double e = double(lanes) / double((lanes + 63) / 64 * 64);
+    if (e > eff + 1e-12) {
+      eff = e;
+      best = P;
+    }
   }
-  int64_t base = k * r.s3 + j * r.s2 + i;
-#pragma unroll
-  for (int v = 0; v < 5; ++v) flx[v * r.sv + base] = out[v];
+  return best;
 }
 
 }  // namespace
@@ -158,6 +240,24 @@ int snapy_sweep_cov(int dim, const snapy_grid* g, const snapy_physics* p,
   int64_t ny = dim == 2 ? int64_t(a.nf) * g->nx3 : int64_t(g->nx2) * a.nf;
   dim3 grid(unsigned(ny), nseg);
   hipStream_t s = as_stream(stream);
+  int len = g->nx1 + 2;
+  if (len <= kPackMax) {
+    int lines = packed_lines(len);
+    unsigned threads = unsigned((lines * len + 63) / 64 * 64);
+    unsigned blocks = unsigned((ny + lines - 1) / lines);
+    if (dtype == SNAPY_F64) {
+      auto k = dim == 2 ? sweep23_packed_kernel<double, 2> : sweep23_packed_kernel<double, 3>;
+      hipLaunchKernelGGL(k, dim3(blocks), dim3(threads), 0, s, static_cast<double const*>(w), x1tab,
+                         static_cast<double*>(flux), a, lines);
+    } else if (dtype == SNAPY_F32) {
+      auto k = dim == 2 ? sweep23_packed_kernel<float, 2> : sweep23_packed_kernel<float, 3>;
+      hipLaunchKernelGGL(k, dim3(blocks), dim3(threads), 0, s, static_cast<double const*>(w), x1tab,
+                         static_cast<float*>(flux), a, lines);
+    } else {
+      return fail(-10, "unknown dtype");
+    }
+    return launch_status("sweep_cov");
+  }
   if (dtype == SNAPY_F64) {
     auto k = dim == 2 ? sweep23_kernel<double, 2> : sweep23_kernel<double, 3>;
     hipLaunchKernelGGL(k, grid, dim3(kLanes), 0, s,
