@@ -214,6 +214,10 @@ def kernel_bytes(block, name, sb=8, pair=False):
         "vic_full": (10 * 8 + 15 * (2 * 4 if pair else 8)) * nint,
         "vic_partial": (10 * 8 + 15 * (2 * 4 if pair else 8)) * nint,
         "update": (3 + nd) * 5 * nint * sb,
+        # outflow: the primitive view of the block (u read, w written), then
+        # the ghost layers; the gathers read and write the ghost layers only
+        "apply_boundaries": (10 * ncell * sb if "outflow" in block.options.bcnames else 0)
+        + 2 * 2 * 3 * 5 * (ncell // max(c.nc1(), 1)) * sb * (nd > 0),
         "update_gravity": ((3 + nd) * 5 + 2) * nint * sb,
     }
     if name.startswith("sweep_x"):
