@@ -57,11 +57,11 @@ __device__ __forceinline__ double stage_dt(PartArgs const& a) {
 }
 
 // rows {rho, m1, E} of FluxJacobian (flux_decomposition_impl.h:105-126), dir
-// 0; gp_g = (gm1 + 1) / gm1, and p / rho divided once
+// 0; gp_g = (gm1 + 1) / gm1, pr = p / rho (the caller's, formed once per
+// cell)
 __device__ __forceinline__ void jac_rows(double f[3][5], double gm1, double gp_g,
-                                         double const* w) {
+                                         double const* w, double pr) {
   double v1 = w[IVX], v2 = w[IVY], v3 = w[IVZ];
-  double pr = w[IPR] / w[IDN];
   double s2 = v1 * v1 + v2 * v2 + v3 * v3;
   double c1 = ((gm1 - 1) * s2 * 0.5 - gp_g * pr) * v1;
   double c2 = gp_g * pr + s2 * 0.5 - gm1 * v1 * v1;
@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
     // carried from the layer below: the state and increments of cells i - 1
     // and i, and the lower face's |A| rows (the Jacobian rows, cheap, are
     // formed again each layer)
-    double wl[5], wc[5], dl[5], dmv[5], fa_lo[3][5];
+    double wl[5], wc[5], dl[5], dmv[5], fa_lo[3][5], prl, prc;
     {
       int64_t row = s_row[lane];
       for (int v = 0; v < 5; ++v) {
@@ -438,6 +438,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
         dmv[v] = live ? d0 : 0.;
       }
       roe_rows3(fa_lo, K, wl, wc);
+      prl = wl[IPR] / wl[IDN];
+      prc = wc[IPR] / wc[IDN];
     }
     // assemble layer i from the carried cells and the staged layer above,
     // and hand it to the consumer
@@ -451,9 +453,10 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
       }
       double fa_hi[3][5], jp[3][5], jc[3][5], jn[3][5];
       roe_rows3(fa_hi, K, wc, wr);
-      jac_rows(jp, gm1, gp_g, wl);
-      jac_rows(jc, gm1, gp_g, wc);
-      jac_rows(jn, gm1, gp_g, wr);
+      double prr = wr[IPR] / wr[IDN];
+      jac_rows(jp, gm1, gp_g, wl, prl);
+      jac_rows(jc, gm1, gp_g, wc, prc);
+      jac_rows(jn, gm1, gp_g, wr, prr);
       CellTab ct;
       ct.wlo = tab(V_WLO, i);
       ct.whi = tab(V_WHI, i);
@@ -471,6 +474,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
         dl[v] = dmv[v];
         dmv[v] = dr[v];
       }
+      prl = prc;
+      prc = prr;
       for (int p = 0; p < 3; ++p)
         for (int q = 0; q < 5; ++q) fa_lo[p][q] = fa_hi[p][q];
       for (int p = 0; p < 3; ++p) {
@@ -505,47 +510,52 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
         rhs[p] = s_h[(27 + p) * 64 + lane];
       }
       __syncthreads();  // layer i taken
-      if (ok) {
+      // the factorisation and the singularity test of the pivot block are
+      // independent work (the test only decides the column's status), so
+      // both run every layer; a failed column's values are never read
+      {
         double e0[3] = {0., 0., i == 0 ? 1. : 0.};
         double e1[3] = {0., 0., i == n - 1 ? 1. : 0.};
         if (i > 0)
           for (int p = 0; p < 3; ++p)
             for (int s = 0; s < 3; ++s)
               m[p][s] -= bb[p][0] * ap[0][s] + bb[p][1] * ap[1][s] + bb[p][2] * ap[2][s];
-        if (!ludcmp3_ok(m)) ok = false;
-        if (ok) {
-          double inv[3][3];
-          inverse3(m, inv);
-          double y[3], z0[3], z1[3];
-          if (i > 0) {
-            double t[3], s0[3], s1[3], by[3], b0[3], b1[3];
-            mv3(bb, yp, by);
-            mv3(bb, z0p, b0);
-            mv3(bb, z1p, b1);
-            for (int p = 0; p < 3; ++p) {
-              t[p] = rhs[p] - by[p];
-              s0[p] = e0[p] - b0[p];
-              s1[p] = e1[p] - b1[p];
-            }
-            mv3(inv, t, y);
-            mv3(inv, s0, z0);
-            mv3(inv, s1, z1);
-          } else {
-            mv3(inv, rhs, y);
-            mv3(inv, e0, z0);
-            mv3(inv, e1, z1);
-          }
-          for (int p = 0; p < 3; ++p)
-            for (int s = 0; s < 3; ++s)
-              ap[p][s] = inv[p][0] * c3[0][s] + inv[p][1] * c3[1][s] + inv[p][2] * c3[2][s];
+        bool piv = ludcmp3_ok(m);
+        double inv[3][3];
+        inverse3(m, inv);
+        double y[3], z0[3], z1[3];
+        if (i > 0) {
+          double t[3], s0[3], s1[3], by[3], b0[3], b1[3];
+          mv3(bb, yp, by);
+          mv3(bb, z0p, b0);
+          mv3(bb, z1p, b1);
           for (int p = 0; p < 3; ++p) {
-            if (!isfinite(y[p])) ok = false;
-            for (int s = 0; s < 3; ++s)
-              if (!isfinite(ap[p][s])) ok = false;
-            yp[p] = y[p];
-            z0p[p] = z0[p];
-            z1p[p] = z1[p];
+            t[p] = rhs[p] - by[p];
+            s0[p] = e0[p] - b0[p];
+            s1[p] = e1[p] - b1[p];
           }
+          mv3(inv, t, y);
+          mv3(inv, s0, z0);
+          mv3(inv, s1, z1);
+        } else {
+          mv3(inv, rhs, y);
+          mv3(inv, e0, z0);
+          mv3(inv, e1, z1);
+        }
+        for (int p = 0; p < 3; ++p)
+          for (int s = 0; s < 3; ++s)
+            ap[p][s] = inv[p][0] * c3[0][s] + inv[p][1] * c3[1][s] + inv[p][2] * c3[2][s];
+        bool fin = true;
+        for (int p = 0; p < 3; ++p) {
+          if (!isfinite(y[p])) fin = false;
+          for (int s = 0; s < 3; ++s)
+            if (!isfinite(ap[p][s])) fin = false;
+          yp[p] = y[p];
+          z0p[p] = z0[p];
+          z1p[p] = z1[p];
+        }
+        ok = ok && piv && fin;
+        if (ok) {
           Rec rec = REC(i);
           for (int p = 0; p < 3; ++p) {
             for (int s = 0; s < 3; ++s) rec[R_A + 3 * p + s] = ap[p][s];
