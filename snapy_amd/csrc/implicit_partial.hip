@@ -10,11 +10,11 @@
 //     state staged through LDS a few layers at a time with coalesced loads:
 //     each layer's blocks assembled in registers (vic_assemble_cell,
 //     src/implicit/vic_kernels.h:66-105, the face matrices and Jacobians
-//     carried up from the layer below), the forward sweep and the back
-//     substitution;
-//   * vicp_finish_kernel<PAIR>: one wavefront per column, layers across the
-//     lanes: the Woodbury 2x2, the redistribution (its serial chains on one
-//     lane, in the column's order), the outputs and the combine.
+//     carried up from the layer below), the forward sweep, the Woodbury
+//     2x2 and the back substitution;
+//   * vicp_finish_kernel<PAIR>: a few columns per workgroup, cells across
+//     the threads: the redistribution (its serial chains one thread per
+//     column, in the column's order), the outputs and the combine.
 //
 // Restates ImplicitHydroImpl::forward_masked (src/implicit/
 // implicit_hydro.cpp:301-623, scheme 1) for a Cartesian block owning its
@@ -34,10 +34,14 @@
 namespace snapy {
 namespace {
 
-constexpr int kRec = 18;  // doubles per layer record: A' 9, y 3, z0 3, z1 3
+// doubles per layer record: A' 9, y 3, z0 3.  The second Woodbury column's
+// right-hand side is nonzero only in the top layer, so its forward-sweep
+// vector is zero below the top layer and only the top layer's (kept in
+// registers) is needed.
+constexpr int kRec = 15;
 constexpr int kF = kRec;
-// record slots: A' = a^-1 c 0..8, y 9..11, z0 12..14, z1 15..17
-enum { R_A = 0, R_Y = 9, R_Z0 = 12, R_Z1 = 15 };
+// record slots: A' = a^-1 c 0..8, y 9..11, z0 12..14
+enum { R_A = 0, R_Y = 9, R_Z0 = 12 };
 
 enum { V_WLO = 0, V_WHI, V_KLO, V_KMID, V_KHI, V_PHI, V_DPLO, V_DPHI };
 enum { T_GVAR = 3, T_WT0 = 4, T_WT1 = 5, T_WT2 = 6 };
@@ -320,8 +324,8 @@ struct Rec {
 constexpr int kLB = 4;  // layers the producer stages per chunk (the layer
                         // above each of the chunk's layers)
 constexpr int kH = 30;           // hand-off per column and layer: m 9, b 9, c 9, rhs 3
-constexpr int kYL = 8;           // layers of y, z0, z1 gathered before a write
-constexpr int kYS = 9 * kYL + 1;  // LDS row of one column (padded)
+constexpr int kYL = 16;           // layers of the solution gathered before a write
+constexpr int kYS = 3 * kYL + 1;  // LDS row of one column (padded)
 // LDS: during the sweep, the producer's staging (w and du, 5 x kLB x 64
 // each) and the hand-off (kH x 64); during the back substitution, the
 // gather (64 x kYS)
@@ -335,15 +339,18 @@ constexpr int kLds = 64 * kYS > kFwdLds ? 64 * kYS : kFwdLds;
 //     carried up from the layer below, so each face is assembled once);
 //   * the consumer (wavefront 1), one lane per column, meanwhile runs the
 //     forward sweep of layer i (ForwardSweep<2>, the small-matrix path),
-//     writing A' = a^-1 c and the three partial solutions to the
-//     lane-contiguous records.
+//     writing A' = a^-1 c and the partial solutions y, z0 to the
+//     lane-contiguous records, and accumulating row 0 of the back
+//     substitution's layer 2 (the Woodbury row at the lower wall).
 // Layer i's blocks pass from producer to consumer through LDS; two
 // workgroup barriers per layer order the hand-off.  Both wavefronts then
-// run the back substitution's loop: the consumer solves, the gather of y,
-// z0, z1 (kYL layers in LDS) is written by both as runs along x1 into yc,
-// column-major (yc[(col * 9 + e) * n + l]) for the finish kernel.  The role
-// is a scalar value, so each wavefront takes one branch, and both branches
-// pass the same barriers (1 + 2 n in the sweep, 2 per gather flush).
+// run the back substitution's loop: the consumer solves the Woodbury 2x2
+// after the top three layers and then the one combined column, the gather
+// of the solution (kYL layers in LDS) is written by both as runs along x1
+// into yc, column-major (yc[(col * 3 + p) * n + l]) for the finish kernel.
+// The role is a scalar value, so each wavefront takes one branch, and both
+// branches pass the same barriers (1 + 2 n in the sweep, 2 per gather
+// flush).
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
     vicp_forward_kernel(double const* __restrict__ w, double const* __restrict__ du,
                         double const* __restrict__ vtab, double* __restrict__ ws,
@@ -375,6 +382,9 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
   }
   __syncthreads();
 
+  // the consumer's Woodbury inputs for the back substitution: row 0 of
+  // layer 2's solution for y and z0, and for z1 with its top-layer vector
+  double acc[2] = {0., 0.}, acc_z1 = 0., z1top[3] = {0., 0., 0.};
   if (producer) {
     // ---- producer: staging and assembly ------------------------------------
     double const gm1 = a.gamma - 1.;
@@ -496,8 +506,15 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
     }
   } else {
     // ---- consumer: the forward sweep ---------------------------------------
-    double ap[3][3], yp[3], z0p[3], z1p[3];
+    double ap[3][3], yp[3], z0p[3];
     bool ok = live;
+    // row 0 of the back substitution's layer 2 for y, z0, z1, accumulated
+    // over the sweep: x_2 = sum_{j >= 2} r_j zeta_j with r_2 = e0 and
+    // r_{j+1} = -r_j A'_j (the recursion x_j = zeta_j - A'_j x_{j+1}
+    // unrolled), so the Woodbury rows are known when the back substitution
+    // starts
+    double rv[3] = {1., 0., 0.};
+    double acc_y = 0., acc_z0 = 0.;
     __syncthreads();  // layer 0 handed off
     for (int i = 0; i < n; ++i) {
       double m[3][3], bb[3][3], c3[3][3], rhs[3];
@@ -515,7 +532,6 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
       // both run every layer; a failed column's values are never read
       {
         double e0[3] = {0., 0., i == 0 ? 1. : 0.};
-        double e1[3] = {0., 0., i == n - 1 ? 1. : 0.};
         if (i > 0)
           for (int p = 0; p < 3; ++p)
             for (int s = 0; s < 3; ++s)
@@ -523,24 +539,20 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
         bool piv = ludcmp3_ok(m);
         double inv[3][3];
         inverse3(m, inv);
-        double y[3], z0[3], z1[3];
+        double y[3], z0[3];
         if (i > 0) {
-          double t[3], s0[3], s1[3], by[3], b0[3], b1[3];
+          double t[3], s0[3], by[3], b0[3];
           mv3(bb, yp, by);
           mv3(bb, z0p, b0);
-          mv3(bb, z1p, b1);
           for (int p = 0; p < 3; ++p) {
             t[p] = rhs[p] - by[p];
             s0[p] = e0[p] - b0[p];
-            s1[p] = e1[p] - b1[p];
           }
           mv3(inv, t, y);
           mv3(inv, s0, z0);
-          mv3(inv, s1, z1);
         } else {
           mv3(inv, rhs, y);
           mv3(inv, e0, z0);
-          mv3(inv, e1, z1);
         }
         for (int p = 0; p < 3; ++p)
           for (int s = 0; s < 3; ++s)
@@ -552,7 +564,20 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
             if (!isfinite(ap[p][s])) fin = false;
           yp[p] = y[p];
           z0p[p] = z0[p];
-          z1p[p] = z1[p];
+        }
+        if (i >= 2) {
+          acc_y += rv[0] * y[0] + rv[1] * y[1] + rv[2] * y[2];
+          acc_z0 += rv[0] * z0[0] + rv[1] * z0[1] + rv[2] * z0[2];
+          if (i == n - 1) {
+            // the second Woodbury column: right-hand side e2 in the top
+            // layer only, so its sweep vector is inv e2 there
+            for (int p = 0; p < 3; ++p) z1top[p] = inv[p][2];
+            acc_z1 = rv[0] * z1top[0] + rv[1] * z1top[1] + rv[2] * z1top[2];
+          }
+          double rn[3];
+          for (int s = 0; s < 3; ++s)
+            rn[s] = -(rv[0] * ap[0][s] + rv[1] * ap[1][s] + rv[2] * ap[2][s]);
+          for (int s = 0; s < 3; ++s) rv[s] = rn[s];
         }
         ok = ok && piv && fin;
         if (ok) {
@@ -561,12 +586,13 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
             for (int s = 0; s < 3; ++s) rec[R_A + 3 * p + s] = ap[p][s];
             rec[R_Y + p] = y[p];
             rec[R_Z0 + p] = z0[p];
-            rec[R_Z1 + p] = z1[p];
           }
         }
       }
       __syncthreads();  // layer i + 1 handed off
     }
+    acc[0] = acc_y;
+    acc[1] = acc_z0;
     // a failed column keeps its explicit increments (the finish kernel
     // reads the flag; the reference undoes the column with its status bits
     // and flags a retry)
@@ -574,250 +600,298 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
     s_act[lane] = ok;
   }
 
-  // ---- back substitution (vic_backward_substitute) -------------------------
-  // the staging and hand-off areas become the gather (the last hand-off was
+  // ---- back substitution (vic_backward_substitute) + Woodbury -------------
+  // (vic_woodbury_column).  The top three layers are solved for the three
+  // columns y, z0, z1; row 0 of layer n - 3 and the swept row 0 of layer 2
+  // give the 2x2 and its coefficients al0, al1; every lower layer then
+  // solves the one combined column, zeta = zeta_y - al0 zeta_z0 (the z1
+  // vector is zero below the top layer), since the substitution is linear.
+  // The staging and hand-off areas become the gather (the last hand-off was
   // read before the sweep's last barrier); both wavefronts run the loop, the
-  // consumer solves, both write the gathered layers
+  // consumer solves, both write the gathered layers.
   bool act = !producer && live && s_act[lane];
-  double* s_y = s_buf;  // [cc][e * kYL + layer % kYL], row kYS
-  double yn[3] = {0., 0., 0.}, z0n[3] = {0., 0., 0.}, z1n[3] = {0., 0., 0.};
-  double nx[18];
-  for (int e = 0; e < 18; ++e) nx[e] = 0.;
+  double* s_y = s_buf;  // [cc][p * kYL + layer % kYL], row kYS
+  double ytop[3][3];    // the combined solution of layers n - 1, n - 2, n - 3
+  double xc[3] = {0., 0., 0.};
+  double al0 = 0., al1 = 0.;
+  for (int t = 0; t < 3; ++t)
+    for (int p = 0; p < 3; ++p) ytop[t][p] = 0.;
+  double nx[kRec];
+  for (int e = 0; e < kRec; ++e) nx[e] = 0.;
   if (act) {
-    Rec rec = REC(n - 1);
-    for (int e = 0; e < 18; ++e) nx[e] = rec[e];
-  }
-  for (int i = n - 1; i >= 0; --i) {
-    if (!producer) {
-      double cur[18];
-      for (int e = 0; e < 18; ++e) cur[e] = nx[e];
-      // the record of the layer below, loaded while this layer is solved
-      if (act && i > 0) {
-        Rec rec = REC(i - 1);
-        for (int e = 0; e < 18; ++e) nx[e] = rec[e];
+    double xy[3][3], x0[3][3], x1[3][3];
+    for (int t = 0; t < 3; ++t) {
+      Rec rec = REC(n - 1 - t);
+      double cy[3], cz[3];
+      for (int p = 0; p < 3; ++p) {
+        cy[p] = rec[R_Y + p];
+        cz[p] = rec[R_Z0 + p];
       }
-      if (i == n - 1) {
+      if (t == 0) {
         for (int p = 0; p < 3; ++p) {
-          yn[p] = cur[R_Y + p];
-          z0n[p] = cur[R_Z0 + p];
-          z1n[p] = cur[R_Z1 + p];
+          xy[0][p] = cy[p];
+          x0[0][p] = cz[p];
+          x1[0][p] = z1top[p];
         }
       } else {
         double A3[3][3];
         for (int p = 0; p < 3; ++p)
-          for (int q = 0; q < 3; ++q) A3[p][q] = cur[R_A + 3 * p + q];
+          for (int q = 0; q < 3; ++q) A3[p][q] = rec[R_A + 3 * p + q];
         double sy[3], s0[3], s1[3];
-        mv3(A3, yn, sy);
-        mv3(A3, z0n, s0);
-        mv3(A3, z1n, s1);
+        mv3(A3, xy[t - 1], sy);
+        mv3(A3, x0[t - 1], s0);
+        mv3(A3, x1[t - 1], s1);
         for (int p = 0; p < 3; ++p) {
-          yn[p] = cur[R_Y + p] - sy[p];
-          z0n[p] = cur[R_Z0 + p] - s0[p];
-          z1n[p] = cur[R_Z1 + p] - s1[p];
+          xy[t][p] = cy[p] - sy[p];
+          x0[t][p] = cz[p] - s0[p];
+          x1[t][p] = 0. - s1[p];
+        }
+      }
+    }
+    // the record of layer n - 4, loaded while the 2x2 is solved
+    {
+      Rec rec = REC(n - 4);
+      for (int e = 0; e < kRec; ++e) nx[e] = rec[e];
+    }
+    double k0 = -a.t0 / dt, k1 = -a.t1 / dt;
+    double m00 = 1. + k0 * acc[1], m01 = k0 * acc_z1;
+    double m10 = k1 * x0[2][0], m11 = 1. + k1 * x1[2][0];
+    double r0 = k0 * acc[0], r1 = k1 * xy[2][0];
+    double det = m00 * m11 - m01 * m10;
+    al0 = (m11 * r0 - m01 * r1) / det;
+    al1 = (m00 * r1 - m10 * r0) / det;
+    for (int t = 0; t < 3; ++t)
+      for (int p = 0; p < 3; ++p) {
+        double y = xy[t][p];
+        y -= x0[t][p] * al0;
+        y -= x1[t][p] * al1;
+        ytop[t][p] = y;
+      }
+    for (int p = 0; p < 3; ++p) xc[p] = ytop[2][p];
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    if (!producer) {
+      double yv[3];
+      if (i >= n - 3) {
+        int t = n - 1 - i;
+        for (int p = 0; p < 3; ++p) yv[p] = t == 0 ? ytop[0][p] : (t == 1 ? ytop[1][p] : ytop[2][p]);
+      } else {
+        double cur[kRec];
+        for (int e = 0; e < kRec; ++e) cur[e] = nx[e];
+        // the record of the layer below, loaded while this layer is solved
+        if (act && i > 0) {
+          Rec rec = REC(i - 1);
+          for (int e = 0; e < kRec; ++e) nx[e] = rec[e];
+        }
+        double A3[3][3];
+        for (int p = 0; p < 3; ++p)
+          for (int q = 0; q < 3; ++q) A3[p][q] = cur[R_A + 3 * p + q];
+        double sx[3];
+        mv3(A3, xc, sx);
+        for (int p = 0; p < 3; ++p) {
+          double z = cur[R_Y + p] - cur[R_Z0 + p] * al0;
+          xc[p] = z - sx[p];
+          yv[p] = xc[p];
         }
       }
       double* o = s_y + lane * kYS + (i % kYL);
-      for (int p = 0; p < 3; ++p) {
-        o[p * kYL] = yn[p];
-        o[(3 + p) * kYL] = z0n[p];
-        o[(6 + p) * kYL] = z1n[p];
-      }
+      for (int p = 0; p < 3; ++p) o[p * kYL] = yv[p];
     }
     if (i % kYL == 0) {
       // layers i .. i + cnt - 1 of the block's columns, lanes along x1
       int cnt = n - i < kYL ? n - i : kYL;
       __syncthreads();
-      for (int t = threadIdx.x; t < 64 * 9 * kYL; t += 128) {
-        int ll = t % kYL, pe = t / kYL, cc = pe / 9, e = pe % 9;
+      for (int t = threadIdx.x; t < 64 * 3 * kYL; t += 128) {
+        int ll = t % kYL, pe = t / kYL, cc = pe / 3, e = pe % 3;
         if (ll < cnt && col0 + cc < ncol && s_act[cc])
-          yc[(int64_t(col0 + cc) * 9 + e) * n + i + ll] = s_y[cc * kYS + e * kYL + ll];
+          yc[(int64_t(col0 + cc) * 3 + e) * n + i + ll] = s_y[cc * kYS + e * kYL + ll];
       }
       __syncthreads();
     }
   }
 }
 
-// the rest of the column program, one wavefront per column with the layers
-// across the lanes (the state reads and writes run along x1, coalesced):
-// the Woodbury 2x2, the redistribution's serial chains on lane 0 in the
-// column's order (the sums, the prefix of the face transfers, the
-// availability clamps), then every cell's outputs, the tangential
-// transfers, the remainder and the RK combine.  Dynamic LDS: 9 (n + 1) + 2
-// doubles.
+// the rest of the column program for kFinCols columns per workgroup
+// (kFinThreads threads): every cell's inputs of the redistribution formed
+// across the threads (the state reads and writes run along x1, coalesced),
+// the serial chains of vic_constituent_column one thread per column in the
+// column's order (the sums, the prefix of the requested face transfers,
+// the availability clamps), then every cell's outputs, the tangential
+// transfers, the remainder and the RK combine.  Dynamic LDS: 3 nc (n + 1)
+// doubles for nc <= kFinCols columns.
+constexpr int kFinCols = 8;
+constexpr int kFinThreads = 256;
+
 template <bool PAIR>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(kFinThreads)
     vicp_finish_kernel(double const* __restrict__ w, double const* __restrict__ du,
                        void const* __restrict__ u0v, float const* __restrict__ c0,
                        void* __restrict__ uv, float* __restrict__ cc,
                        double const* __restrict__ vtab, double const* __restrict__ xtab,
-                       double const* __restrict__ yc, int const* __restrict__ status, PartArgs a) {
+                       double const* __restrict__ yc, int const* __restrict__ status, PartArgs a,
+                       int nc) {
   extern __shared__ double lds[];
+  __shared__ int64_t s_row[kFinCols];
+  __shared__ int s_in[kFinCols], s_ok[kFinCols];
   Range3 const& r = a.r;
   int64_t ncol = int64_t(r.n2) * r.n3;
-  int64_t col = blockIdx.x;
-  int lane = threadIdx.x;
-  int j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
-  int64_t row = k * r.s3 + j * r.s2 + r.lo1;
-  int n = a.n, nc1 = r.nc1, lo1 = r.lo1;
+  int64_t col0 = int64_t(blockIdx.x) * nc;
+  int const tid = threadIdx.x;
+  int n = a.n, nc1 = r.nc1, lo1 = r.lo1, L = n + 1;
   double const V = a.vol;
-  double const dt = stage_dt(a);
-  auto W = [&](int v, int l) { return w[v * r.sv + row + l]; };
-  auto DU = [&](int v, int l) { return du[v * r.sv + row + l]; };
-  auto tab = [&](double const* t, int rowi, int l) { return t[rowi * nc1 + lo1 + l]; };
-  auto combine = [&](int v, int l, double x) {
-    int64_t o = v * r.sv + row + l;
-    if constexpr (PAIR) {
-      double y = a.w0 * (double(static_cast<float const*>(u0v)[o]) + double(c0[o])) +
-                 a.w1 * (double(static_cast<float*>(uv)[o]) + double(cc[o])) + a.w2 * x;
-      float h = float(y);
-      static_cast<float*>(uv)[o] = h;
-      cc[o] = float(y - double(h));
-    } else {
-      double* u = static_cast<double*>(uv);
-      u[o] = a.w0 * static_cast<double const*>(u0v)[o] + a.w1 * u[o] + a.w2 * x;
-    }
-  };
-  if (status[col] != 0) {
-    for (int l = lane; l < n; l += 64)
-      for (int v = 0; v < 5; ++v) combine(v, l, DU(v, l));
-    return;
-  }
-  int const L = n + 1;
-  double* s_phi = lds;          // phi_i
-  double* s_wv = lds + L;       // w_rho V, then each cell's request term
-  double* s_av = lds + 2 * L;   // (w_rho + du_rho) V
-  double* s_M = lds + 3 * L;    // requested transfer through the face below i
-  double* s_q = lds + 4 * L;    // moved transfer through the face below i
-  double* s_y0 = lds + 5 * L;   // the solved total density increment
-  double* s_d0 = lds + 6 * L;   // du_rho
-  double* s_vy = lds + 7 * L;   // w_v2, w_v3 (the donor velocities)
-  double* s_vz = lds + 8 * L;
-  // ---- Woodbury 2x2 (vic_woodbury_column) ----------------------------------
-  double al0, al1;
-  {
-    double k0 = -a.t0 / dt, k1 = -a.t1 / dt;
-    int i0 = 2, i1 = n - 3;
-    double const* yv = yc + (col * 9) * n;
-    double m00 = 1. + k0 * yv[3 * n + i0], m01 = k0 * yv[6 * n + i0];
-    double m10 = k1 * yv[3 * n + i1], m11 = 1. + k1 * yv[6 * n + i1];
-    double r0 = k0 * yv[i0], r1 = k1 * yv[i1];
-    double det = m00 * m11 - m01 * m10;
-    al0 = (m11 * r0 - m01 * r1) / det;
-    al1 = (m00 * r1 - m10 * r0) / det;
-  }
-  double const* ycol = yc + (col * 9) * n;
-  auto Y = [&](int l, int p) {
-    double y = ycol[p * n + l];
-    y -= ycol[(3 + p) * n + l] * al0;
-    y -= ycol[(6 + p) * n + l] * al1;
-    return y;
-  };
-  for (int l = lane; l < n; l += 64) {
-    double y0 = Y(l, 0), d0 = DU(IDN, l), wr = W(IDN, l);
-    s_y0[l] = y0;
-    s_d0[l] = d0;
-    s_phi[l] = (y0 - d0) * V;
-    s_wv[l] = wr * V;
-    s_av[l] = (wr * 1. + d0) * V;
-    s_vy[l] = W(IVY, l);
-    s_vz[l] = W(IVZ, l);
+  // per column c: phi_i, then M_i (the requested transfer through the face
+  // below cell i); (w_rho V)_i; (w_rho + du_rho)_i V, then q_i (the moved
+  // transfer through the face below cell i)
+  double* s_A = lds;
+  double* s_B = lds + nc * L;
+  double* s_C = lds + 2 * nc * L;
+  if (tid < nc) {
+    int64_t col = col0 + tid;
+    bool in = col < ncol;
+    int64_t cl = in ? col : 0;
+    int j = r.lo2 + int(cl % r.n2), k = r.lo3 + int(cl / r.n2);
+    s_row[tid] = k * r.s3 + j * r.s2 + r.lo1;
+    s_in[tid] = in;
+    s_ok[tid] = in && status[col] == 0;
   }
   __syncthreads();
-  // ---- the serial chains of vic_constituent_column, in the column's order --
-  // the column sums on lane 0, in order; then every cell's term of the
-  // requested transfer on its own lane; then the prefix of the requests and
-  // the availability clamps on lane 0, reading their inputs in register
-  // batches (the same operations in the same order as the serial loops)
-  double* s_sum = lds + 9 * L;  // R, S
-  if (lane == 0) {
+  int const cells = nc * n;
+  for (int e = tid; e < cells; e += kFinThreads) {
+    int c = e / n, l = e - c * n;
+    if (!s_ok[c]) continue;
+    int64_t row = s_row[c];
+    double y0 = yc[((col0 + c) * 3) * n + l];
+    double d0 = du[IDN * r.sv + row + l], wr = w[IDN * r.sv + row + l];
+    s_A[c * L + l] = (y0 - d0) * V;
+    s_B[c * L + l] = wr * V;
+    s_C[c * L + l] = (wr * 1. + d0) * V;
+  }
+  __syncthreads();
+  // ---- the serial chains of vic_constituent_column, one thread per column,
+  // in the column's order: the column sums R and S, then, face by face, the
+  // requested transfer M (the prefix of phi_i - R (w_rho V)_i / S) and the
+  // moved transfer q clamped by what the cells on either side hold
+  if (tid < nc && s_ok[tid]) {
+    double* A_ = s_A + tid * L;
+    double* B_ = s_B + tid * L;
+    double* C_ = s_C + tid * L;
+    constexpr int kB = 8;
     double R = 0., S = 0.;
-    for (int i = 0; i < n; ++i) {
-      R += s_phi[i];
-      S += s_wv[i];
+    int i0 = 0;
+    for (; i0 + kB <= n; i0 += kB) {
+      double ta[kB], tb[kB];
+#pragma unroll
+      for (int t = 0; t < kB; ++t) {
+        ta[t] = A_[i0 + t];
+        tb[t] = B_[i0 + t];
+      }
+#pragma unroll
+      for (int t = 0; t < kB; ++t) {
+        R += ta[t];
+        S += tb[t];
+      }
     }
-    s_sum[0] = R;
-    s_sum[1] = S;
-  }
-  __syncthreads();
-  {
-    double R = s_sum[0], S = s_sum[1];
-    // s_wv[i] becomes phi_i - R (w_rho V)_i / S
-    for (int l = lane; l < n; l += 64) s_wv[l] = s_phi[l] - R * (s_wv[l] / S);
-  }
-  __syncthreads();
-  if (lane == 0) {
-    constexpr int kB = 16;
+    for (; i0 < n; ++i0) {
+      R += A_[i0];
+      S += B_[i0];
+    }
     double M = 0.;
-    for (int i0 = 0; i0 < n; i0 += kB) {
-      double tt[kB];
-#pragma unroll
-      for (int t = 0; t < kB; ++t) tt[t] = s_wv[i0 + t < n ? i0 + t : n - 1];
-#pragma unroll
-      for (int t = 0; t < kB; ++t) {
-        if (i0 + t >= n) break;
-        s_M[i0 + t] = M;
-        M -= tt[t];
-      }
-    }
-    s_M[n] = 0.;
-    s_q[0] = 0.;
-    s_q[n] = 0.;
-    double avail = s_av[0];
+    double avail = C_[0];
     if (avail < 0) avail = 0;
-    for (int i0 = 0; i0 + 1 < n; i0 += kB) {
-      double av[kB], mm[kB];
-#pragma unroll
-      for (int t = 0; t < kB; ++t) {
-        int k = i0 + t + 1 < n ? i0 + t + 1 : n - 1;
-        av[t] = s_av[k];
-        mm[t] = s_M[k];
-      }
-#pragma unroll
-      for (int t = 0; t < kB; ++t) {
-        int i = i0 + t;
-        if (i + 1 >= n) break;
-        double avail_up = av[t];
+    C_[0] = 0.;
+    // one face: M_i stored, M_{i+1} formed, then the face above cell i
+    // (i + 1 < n) clamped
+    auto face = [&](int i, double phi, double wv, double av_up) {
+      A_[i] = M;
+      double wvp = phi - R * (wv / S);
+      M -= wvp;
+      if (i + 1 < n) {
+        double avail_up = av_up;
         if (avail_up < 0) avail_up = 0;
-        double mf = mm[t];
+        double mf = M;
         double q = mf > 0 ? mf * 1. : mf * 1.;
         if (q > avail) q = avail;
         if (q < -avail_up) q = -avail_up;
-        s_q[i + 1] = q;
+        C_[i + 1] = q;
         avail = avail_up + q;
       }
+    };
+    i0 = 0;
+    for (; i0 + kB < n; i0 += kB) {
+      double ta[kB], tb[kB], tc[kB];
+#pragma unroll
+      for (int t = 0; t < kB; ++t) {
+        ta[t] = A_[i0 + t];
+        tb[t] = B_[i0 + t];
+        tc[t] = C_[i0 + t + 1];
+      }
+#pragma unroll
+      for (int t = 0; t < kB; ++t) face(i0 + t, ta[t], tb[t], tc[t]);
     }
+    for (; i0 < n; ++i0) face(i0, A_[i0], B_[i0], i0 + 1 < n ? C_[i0 + 1] : 0.);
+    A_[n] = 0.;
+    C_[n] = 0.;
   }
   __syncthreads();
   // ---- outputs, the tangential transfers, remainder, RK combine ------------
-  // the mass change of cell l: the face below adds, the face above removes
-  auto dm = [&](int l) { return (0. + s_q[l] / V) - s_q[l + 1] / V; };
-  auto moved_rho = [&](int l) { return (s_d0[l] + dm(l)) - s_d0[l]; };
-  auto raw_minus_mass0 = [&](int l) { return s_y0[l] - s_d0[l]; };
-  auto rows_at = [&](int l, auto q) {
-    double s = tab(vtab, V_KMID, l) * q(l);
-    if (l > 0) s = s + tab(vtab, V_KLO, l) * q(l - 1);
-    if (l < n - 1) s = s + tab(vtab, V_KHI, l) * q(l + 1);
-    if (l == 0) s = s + a.t0 * q(2);
-    if (l == n - 1) s = s + a.t1 * q(n - 3);
-    return s;
-  };
-  for (int i = lane; i < n; i += 64) {
+  for (int e = tid; e < cells; e += kFinThreads) {
+    int c = e / n, i = e - c * n;
+    if (!s_in[c]) continue;
+    int64_t row = s_row[c];
+    auto W = [&](int v, int l) { return w[v * r.sv + row + l]; };
+    auto DU = [&](int v, int l) { return du[v * r.sv + row + l]; };
+    auto tab = [&](double const* t, int rowi, int l) { return t[rowi * nc1 + lo1 + l]; };
+    auto combine = [&](int v, double x) {
+      int64_t o = v * r.sv + row + i;
+      if constexpr (PAIR) {
+        double y = a.w0 * (double(static_cast<float const*>(u0v)[o]) + double(c0[o])) +
+                   a.w1 * (double(static_cast<float*>(uv)[o]) + double(cc[o])) + a.w2 * x;
+        float h = float(y);
+        static_cast<float*>(uv)[o] = h;
+        cc[o] = float(y - double(h));
+      } else {
+        double* u = static_cast<double*>(uv);
+        u[o] = a.w0 * static_cast<double const*>(u0v)[o] + a.w1 * u[o] + a.w2 * x;
+      }
+    };
+    if (!s_ok[c]) {
+      for (int v = 0; v < 5; ++v) combine(v, DU(v, i));
+      continue;
+    }
+    double const* ycol = yc + ((col0 + c) * 3) * n;
+    double const* s_M = s_A + c * L;
+    double const* s_q = s_C + c * L;
+    // the mass change of cell l: the face below adds, the face above removes
+    auto dm = [&](int l) { return (0. + s_q[l] / V) - s_q[l + 1] / V; };
+    auto moved_rho = [&](int l) {
+      double d0 = DU(IDN, l);
+      return (d0 + dm(l)) - d0;
+    };
+    auto raw_minus_mass0 = [&](int l) { return ycol[l] - DU(IDN, l); };
+    auto rows_at = [&](int l, auto q) {
+      double s = tab(vtab, V_KMID, l) * q(l);
+      if (l > 0) s = s + tab(vtab, V_KLO, l) * q(l - 1);
+      if (l < n - 1) s = s + tab(vtab, V_KHI, l) * q(l + 1);
+      if (l == 0) s = s + a.t0 * q(2);
+      if (l == n - 1) s = s + a.t1 * q(n - 3);
+      return s;
+    };
     double out[5];
-    out[IDN] = s_d0[i] + dm(i);
-    out[IVX] = Y(i, 1);
-    out[IPR] = Y(i, 2);
+    double y0 = ycol[i], d0 = DU(IDN, i);
+    out[IDN] = d0 + dm(i);
+    out[IVX] = ycol[n + i];
+    out[IPR] = ycol[2 * n + i];
     // vic_tangential_momentum_column: the face below adds (its upper cell),
     // then the face above removes (its lower cell), in the column's order
     double q_lo = s_q[i], q_hi = s_q[i + 1];
     for (int m = IVY; m <= IVZ; ++m) {
-      double const* vm = m == IVY ? s_vy : s_vz;
       double x = DU(m, i);
       if (i > 0) {
-        double pv = q_lo * vm[q_lo > 0 ? i - 1 : i];
+        double pv = q_lo * W(m, q_lo > 0 ? i - 1 : i);
         if (pv != 0.) x = x + pv / V;
       }
       if (i + 1 < n) {
-        double pv = q_hi * vm[q_hi > 0 ? i : i + 1];
+        double pv = q_hi * W(m, q_hi > 0 ? i : i + 1);
         if (pv != 0.) x = x - pv / V;
       }
       out[m] = x;
@@ -826,7 +900,7 @@ __global__ void __launch_bounds__(64)
     double mv_lo = s_q[i] - req_lo;
     double mv_hi = s_q[i + 1] - req_hi;
     double projected = (req_lo - req_hi) / V;
-    double pw = tab(vtab, V_PHI, i) * (s_y0[i] - s_d0[i] - projected);
+    double pw = tab(vtab, V_PHI, i) * (y0 - d0 - projected);
     double cw = -(tab(vtab, V_DPHI, i) * mv_hi - tab(vtab, V_DPLO, i) * mv_lo) / V;
     out[IPR] = out[IPR] + (pw + cw);
     int jj = (i < 1 ? 1 : (i > n - 2 ? n - 2 : i)) - 1;
@@ -836,7 +910,7 @@ __global__ void __launch_bounds__(64)
     double work = tab(xtab, T_GVAR, i) * s;
     work = work - rows_at(i, raw_minus_mass0);
     out[IPR] = out[IPR] + work;
-    for (int v = 0; v < 5; ++v) combine(v, i, out[v]);
+    for (int v = 0; v < 5; ++v) combine(v, out[v]);
   }
 }
 
@@ -876,26 +950,30 @@ static int vicp_launch(const snapy_grid* g, const snapy_physics* p, const double
   a.w2 = w2;
   int64_t ncol = int64_t(g->nx2) * g->nx3;
   if (ncol > (int64_t(1) << 30)) return fail(-9, "vic_partial: too many columns");
-  // the records (kF per layer, lane-contiguous) then yc (9 per layer)
+  // the records (kF per layer, lane-contiguous) then yc (3 per layer)
   double* yc = ws + int64_t(kF) * g->nx1 * ncol;
   hipLaunchKernelGGL(vicp_forward_kernel, dim3(blocks_for(ncol, 64)), dim3(128), 0,
                      as_stream(stream), w, du, vtab, ws, yc, status, a);
   if (int e = launch_status("vic_partial_forward")) return e;
-  size_t lds = (size_t(9) * (g->nx1 + 1) + 2) * sizeof(double);
+  // columns per finish workgroup: kFinCols, fewer for a tall column
+  int nc = kFinCols;
+  while (nc > 1 && size_t(3) * nc * (g->nx1 + 1) * sizeof(double) > 64 * 1024) nc /= 2;
+  size_t lds = size_t(3) * nc * (g->nx1 + 1) * sizeof(double);
   if (lds > 64 * 1024) return fail(-9, "vic_partial: the column does not fit the LDS");
+  dim3 fgrid(unsigned((ncol + nc - 1) / nc));
   if (pair)
-    hipLaunchKernelGGL(vicp_finish_kernel<true>, dim3(ncol), dim3(64), lds, as_stream(stream), w,
-                       du, u0, c0, u, c, vtab, x1tab, yc, status, a);
+    hipLaunchKernelGGL(vicp_finish_kernel<true>, fgrid, dim3(kFinThreads), lds,
+                       as_stream(stream), w, du, u0, c0, u, c, vtab, x1tab, yc, status, a, nc);
   else
-    hipLaunchKernelGGL(vicp_finish_kernel<false>, dim3(ncol), dim3(64), lds, as_stream(stream),
-                       w, du, u0, c0, u, c, vtab, x1tab, yc, status, a);
+    hipLaunchKernelGGL(vicp_finish_kernel<false>, fgrid, dim3(kFinThreads), lds,
+                       as_stream(stream), w, du, u0, c0, u, c, vtab, x1tab, yc, status, a, nc);
   return launch_status("vic_partial");
 }
 
 int64_t snapy_vic_partial_workspace(const snapy_grid* g) {
   if (!g) return 0;
-  // the lane-contiguous records and the column-major y, z0, z1
-  return int64_t(kRec + 9) * g->nx1 * g->nx2 * g->nx3;
+  // the lane-contiguous records and the column-major solution
+  return int64_t(kRec + 3) * g->nx1 * g->nx2 * g->nx3;
 }
 
 int snapy_vic_partial(const snapy_grid* g, const snapy_physics* p, const double* w,
