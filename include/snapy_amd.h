@@ -309,10 +309,11 @@ int snapy_update_gravity_f32(const snapy_grid *g, const snapy_physics *p,
  * its Woodbury wall points, the mass redistribution with the tangential
  * momentum each moved mass carries (the donor cell's velocity), the
  * post-solve remainder and the RK combine; arguments as snapy_vic_full.
- * One thread per column assembles, sweeps and back-substitutes (records of
- * the factored rows and the partial solutions in ws,
- * snapy_vic_partial_workspace(g) doubles); one wavefront per column then
- * applies the Woodbury correction, redistributes and combines.
+ * One thread per column assembles, sweeps, solves the Woodbury 2x2 and
+ * back-substitutes (records of the factored rows and the partial solutions
+ * in ws, snapy_vic_partial_workspace(g) doubles); a few columns per
+ * workgroup then redistribute (the serial chains one thread per column)
+ * and combine.
  * Replaces: vic_stage for VicMatrix::partial (vic_kernels.h:66-297 with
  * N = 3; vic_assemble_impl.h:45-218; forward_sweep_impl.h:104-121;
  * vic_redistribute_impl.h:177-205) and ImplicitHydroImpl::forward_masked
