@@ -46,11 +46,11 @@ struct RefArgs {
   int wall_clamp;
 };
 
-__device__ __forceinline__ double rop_smooth(double const* rho,
-                                             double const* pr, int i, int jlo,
+// ro = rho / p per cell
+__device__ __forceinline__ double rop_smooth(double const* ro, int i, int jlo,
                                              int jhi, bool ext_lo,
                                              bool ext_hi) {
-  auto rop = [&](int j) { return rho[j] / pr[j]; };
+  auto rop = [&](int j) { return ro[j]; };
   if (!(ext_lo && i - 2 < jlo) && !(ext_hi && i + 2 > jhi))
     return rop(i < jlo ? jlo : (i > jhi ? jhi : i));
   double v[5];
@@ -66,74 +66,105 @@ __device__ __forceinline__ double rop_smooth(double const* rho,
   return (v[0] + 4. * v[1] + 6. * v[2] + 4. * v[3] + v[4]) / 16.;
 }
 
-// One wavefront per column: the column's density and pressure are staged in
-// LDS with coalesced loads, lane 0 runs the two face scans in their serial
-// order (hydro_ref_x1_scan_impl), and the lanes share the cell pass
+// kRefCols columns per workgroup of kRefThreads threads.  Every cell's
+// pressure step g rho dx and rho / p are formed across the threads
+// (coalesced along x1) into LDS; one thread per column then runs the two
+// face scans in their serial order (hydro_ref_x1_scan_impl) while the
+// others smooth rho / p; then the threads share the cell pass
 // (hydro_ref_x1_cell_impl), whose cells are independent given the faces.
-// LDS: 5 rows of nc1 doubles (rho, p, face lo, face hi, smoothed rho/p).
-__global__ void __launch_bounds__(64)
+// The scans keep the raw face values F (F[iu + 1] = the top anchor, F[i] =
+// F[i + 1] + dp_i below it and F[i + 1] = F[i] - dp_i above it, as the
+// reference's running face); cell i's faces are F[i] and F[i + 1], floored
+// at DBL_MIN where read, as the reference stores them.
+// LDS per column: dp, rho / p, F (nc1 + 1), smoothed rho / p.
+constexpr int kRefCols = 8;
+constexpr int kRefThreads = 256;
+
+__global__ void __launch_bounds__(kRefThreads)
     hydro_ref_x1_kernel(double const* __restrict__ w, double* __restrict__ ref,
-                        RefArgs a) {
+                        RefArgs a, int nc) {
   extern __shared__ double smem[];
   Range3 const& r = a.r;
-  int64_t col = blockIdx.x;
-  int const lane = threadIdx.x;
+  int64_t ncol = int64_t(r.nc2) * r.nc3;
+  int64_t col0 = int64_t(blockIdx.x) * nc;
+  int const tid = threadIdx.x;
   int nc1 = r.nc1, iu = a.iu;
-  int64_t base = col * nc1;
-  double* s_rho = smem;
-  double* s_pr = smem + nc1;
-  double* s_lo = smem + 2 * nc1;
-  double* s_hi = smem + 3 * nc1;
-  double* s_rs = smem + 4 * nc1;
-  for (int i = lane; i < nc1; i += 64) {
-    s_rho[i] = w[base + i];
-    s_pr[i] = w[4 * r.sv + base + i];
+  int const stride = 4 * nc1 + 1;  // doubles per column
+  int ncl = ncol - col0 < nc ? int(ncol - col0) : nc;
+  double const g = a.grav, dx = a.dx1, mp = DBL_MIN;
+  auto floor_mp = [&](double v) { return v > mp ? v : mp; };
+  int const cells = ncl * nc1;
+  for (int e = tid; e < cells; e += kRefThreads) {
+    int c = e / nc1, i = e - c * nc1;
+    int64_t base = (col0 + c) * nc1;
+    double rho = w[base + i], pr = w[4 * r.sv + base + i];
+    double* sc = smem + c * stride;
+    sc[i] = g * rho * dx;
+    sc[nc1 + i] = rho / pr;
   }
   __syncthreads();
-  double const g = a.grav, dx = a.dx1, mp = DBL_MIN;
-  if (lane == 0) {
-    // scan from the top anchor (hydro_ref_x1_scan_impl)
-    double rdt_top = s_pr[iu] / s_rho[iu];
-    double top = s_pr[iu] * exp(-g * 0.5 * dx / rdt_top);
-    double face = top;
-    for (int i = iu; i >= 0; --i) {
-      double dp = g * s_rho[i] * dx;
-      double lo = face + dp, hi = face;
-      s_lo[i] = lo > mp ? lo : mp;
-      s_hi[i] = hi > mp ? hi : mp;
-      face = lo;
-    }
-    face = top;
-    for (int i = iu + 1; i < nc1; ++i) {
-      double dp = g * s_rho[i] * dx;
-      double lo = face, hi = lo - dp;
-      s_lo[i] = lo > mp ? lo : mp;
-      s_hi[i] = hi > mp ? hi : mp;
-      face = hi;
-    }
-  }
-  // cells (hydro_ref_x1_cell_impl), phys_in = phys_out = true
   int il = nc1 - 1 - iu;
   bool wc = a.wall_clamp;
   bool thin = (iu - il + 1) < 5, fits = iu >= 4;
   int jlo = wc ? il : 0, jhi = wc ? iu : nc1 - 1;
   bool ext_lo = wc && il + 1 <= iu, ext_hi = wc && iu - 1 >= il;
-  for (int i = lane; i < nc1; i += 64)
-    s_rs[i] = rop_smooth(s_rho, s_pr, i, jlo, jhi, ext_lo, ext_hi);
+  if (tid < ncl) {
+    // the scans from the top anchor, one column per thread
+    double* sc = smem + tid * stride;
+    double const* s_dp = sc;
+    double* s_F = sc + 2 * nc1;
+    int64_t base = (col0 + tid) * nc1;
+    double rho_u = w[base + iu], pr_u = w[4 * r.sv + base + iu];
+    double rdt_top = pr_u / rho_u;
+    double top = pr_u * exp(-g * 0.5 * dx / rdt_top);
+    s_F[iu + 1] = top;
+    double face = top;
+    constexpr int kB = 8;
+    int i = iu;
+    for (; i - kB + 1 >= 0; i -= kB) {
+      double t[kB];
+#pragma unroll
+      for (int m = 0; m < kB; ++m) t[m] = s_dp[i - m];
+#pragma unroll
+      for (int m = 0; m < kB; ++m) {
+        face = face + t[m];
+        s_F[i - m] = face;
+      }
+    }
+    for (; i >= 0; --i) {
+      face = face + s_dp[i];
+      s_F[i] = face;
+    }
+    face = top;
+    for (int k = iu + 1; k < nc1; ++k) {
+      face = face - s_dp[k];
+      s_F[k + 1] = face;
+    }
+  }
+  // the smoothed rho / p, meanwhile on the other threads
+  for (int e = tid; e < cells; e += kRefThreads) {
+    int c = e / nc1, i = e - c * nc1;
+    double* sc = smem + c * stride;
+    sc[3 * nc1 + 1 + i] = rop_smooth(sc + nc1, i, jlo, jhi, ext_lo, ext_hi);
+  }
   __syncthreads();
-  auto facev = [&](int f) { return f < nc1 ? s_lo[f] : s_hi[nc1 - 1]; };
-  for (int i = lane; i < nc1; i += 64) {
+  for (int e = tid; e < cells; e += kRefThreads) {
+    int c = e / nc1, i = e - c * nc1;
+    double const* sc = smem + c * stride;
+    double const* s_F = sc + 2 * nc1;
+    double const* s_rs = sc + 3 * nc1 + 1;
+    int64_t base = (col0 + c) * nc1;
     bool wall_in = wc && i >= il && i < il + 2;
     bool wall_out = wc && i > iu - 2 && i <= iu;
     bool clamp_in = wall_in && fits && !thin;
     bool clamp_out = wall_out && fits && !thin;
-    double lo = s_lo[i], hi = s_hi[i];
+    double lo = floor_mp(s_F[i]), hi = floor_mp(s_F[i + 1]);
     double cell = 0.5 * (lo + hi);
     double lower = lo < hi ? lo : hi, upper = lo > hi ? lo : hi;
     auto six = [&](int start, double const* wt, bool rev) {
       double value = 0.;
       for (int m = 0; m < 6; ++m)
-        value += wt[rev ? 5 - m : m] * facev(start + m);
+        value += wt[rev ? 5 - m : m] * floor_mp(s_F[start + m]);
       if (value >= lower && value <= upper) cell = value;
     };
     if (i >= 2 && i < nc1 - 2 && !wall_in && !wall_out) six(i - 2, kW6, false);
@@ -401,10 +432,13 @@ int snapy_hydro_ref_x1(const snapy_grid* g, const snapy_physics* p,
   a.dx1 = g->dx1;
   a.wall_clamp = 1;
   int64_t ncol = int64_t(g->nc2) * g->nc3;
-  size_t lds = size_t(5) * g->nc1 * sizeof(double);
-  if (lds > 64 * 1024) return fail(-9, "hydro_ref_x1: an x1 column above 1638 cells");
-  hipLaunchKernelGGL(hydro_ref_x1_kernel, dim3(unsigned(ncol)), dim3(64), lds,
-                     as_stream(stream), w, ref, a);
+  // columns per workgroup: kRefCols, fewer for a tall column
+  int nc = kRefCols;
+  auto lds_of = [&](int c) { return size_t(c) * (4 * g->nc1 + 1) * sizeof(double); };
+  while (nc > 1 && lds_of(nc) > 64 * 1024) nc /= 2;
+  if (lds_of(nc) > 64 * 1024) return fail(-9, "hydro_ref_x1: an x1 column above 2047 cells");
+  hipLaunchKernelGGL(hydro_ref_x1_kernel, dim3(unsigned((ncol + nc - 1) / nc)),
+                     dim3(kRefThreads), lds_of(nc), as_stream(stream), w, ref, a, nc);
   return launch_status("hydro_ref_x1");
 }
 
