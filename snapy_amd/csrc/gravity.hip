@@ -77,7 +77,7 @@ __device__ __forceinline__ double rop_smooth(double const* ro, int i, int jlo,
 // reference's running face); cell i's faces are F[i] and F[i + 1], floored
 // at DBL_MIN where read, as the reference stores them.
 // LDS per column: dp, rho / p, F (nc1 + 1), smoothed rho / p.
-constexpr int kRefCols = 8;
+constexpr int kRefCols = 4;
 constexpr int kRefThreads = 256;
 
 __global__ void __launch_bounds__(kRefThreads)
