@@ -208,7 +208,9 @@ def kernel_bytes(block, name, sb=8, pair=False):
     table = {
         "cons2prim": 10 * ncell * sb,
         "hydro_ref_x1": 7 * ncell * 8,
-        "sweep_x1_wb": (5 * sb + 4 * 8) * ncell + 6 * nint * sb,
+        # the reference and the well-balanced sweep in one launch: w read,
+        # psf_lo written; five flux rows and the face pressure written
+        "sweep_x1_wb": (5 * 8 + 8) * ncell + (5 * sb + 8) * nint,
         "stage_increment": (5 * nd + 2 + 5) * nint * sb,
         # du, w (FP64) read; u0, u read and u written: FP64, or FP32 pairs
         "vic_full": (10 * 8 + 15 * (2 * 4 if pair else 8)) * nint,
@@ -230,7 +232,7 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "convection3d_hbm_traffic.md")
 STAGE_KERNELS = {
     "vic_partial": (["vicp_forward_kernel", "vicp_finish_kernel<false>"],
                     ["vicp_forward_kernel", "vicp_finish_kernel<true>"]),
-    "sweep_x1_wb": (["sweep_x1_wb_kernel<double, double>"], ["sweep_x1_wb_kernel<double, float>"]),
+    "sweep_x1_wb": (["sweep_x1_wb_ref_kernel<double, double>"], ["sweep_x1_wb_ref_kernel<double, float>"]),
     "stage_increment": (["update_gravity_kernel<double, true, false>"], ["update_gravity_kernel<float, true, true>"]),
 }
 

@@ -202,6 +202,18 @@ int snapy_sweep_x1_wb(const snapy_grid *g, const snapy_physics *p, int dtype,
                       const void *w, const double *ref, void *flux,
                       void *face_pressure, void *stream);
 
+/* snapy_hydro_ref_x1 and snapy_sweep_x1_wb in one launch, for the interior
+ * columns: the reference stays on chip (LDS) between the two, and only its
+ * psf_lo plane is written to ref (the other four planes are left as they
+ * are; snapy_update_gravity reads psf_lo).  Fluxes and face pressure are
+ * the same bits as the two entries' (the same arithmetic).
+ * Replaces: the x1 reference evaluation and WellBalancedX1::forward of one
+ * stage (src/hydro/hydro_forward.cpp:479-546, src/hydro/hydro_ref_x1_impl.h:
+ * 24-272, src/recon/well_balanced_x1.cpp:16-154). */
+int snapy_sweep_x1_wb_ref(const snapy_grid *g, const snapy_physics *p, int dtype,
+                          const double *w, double *ref, void *flux,
+                          void *face_pressure, void *stream);
+
 /* snapy_update with x1 gravity: du also receives the const-gravity source
  * (src/forcing/const_gravity.cpp:30-55) and the face-based gravity work plus
  * the corrected potential-energy work in place of the cell-centred work

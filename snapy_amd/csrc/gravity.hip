@@ -203,32 +203,25 @@ struct WbArgs {
 // An FP32 flux keeps the x1 momentum row as the anomaly about the reference
 // face pressure psf (the FP64 update adds psf back), so the FP32 word holds
 // the small non-hydrostatic part of the flux.
-template <typename T, typename TF>
-__global__ void __launch_bounds__(256)
-    sweep_x1_wb_kernel(T const* __restrict__ w, double const* __restrict__ ref,
-                       TF* __restrict__ flx, double* __restrict__ fp, WbArgs a) {
+// One x1 face of the well-balanced sweep: face fi (x1 index i) of the
+// column at row; pref(m) and dref(m) give the reference pressure and
+// density of cell m of the column, psf and dsf the scan's face pressure and
+// density at the face.
+template <typename T, typename TF, typename PREF, typename DREF>
+__device__ __forceinline__ void wb_face(T const* __restrict__ w, int64_t row, int fi, double psf,
+                                        double dsf, PREF pref, DREF dref,
+                                        TF* __restrict__ flx, double* __restrict__ fp,
+                                        WbArgs const& a) {
   Range3 const& r = a.r;
-  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t tot = int64_t(a.nf) * r.n2 * r.n3;
-  if (idx >= tot) return;
-  int fi = int(idx % a.nf);
   int i = r.lo1 + fi;
-  int64_t rest = idx / a.nf;
-  int j = r.lo2 + int(rest % r.n2);
-  int k = r.lo3 + int(rest / r.n2);
-  int64_t row = k * r.s3 + j * r.s2;
   int il = r.lo1, iu = r.lo1 + r.n1 - 1;
-  double const* pref = ref + R_PREF * r.sv + row;
-  double const* dref = ref + R_DREF * r.sv + row;
-  double psf = ref[R_PSF_LO * r.sv + row + i];
-  double dsf = ref[R_DSF * r.sv + row + i];
 
   // perturbation of row c at cell m, with the even mirror at the walls
   auto pert = [&](int c, int m) -> T {
     int src = m;
     if (m < il && a.mirror_lo) src = 2 * il - 1 - m;
     if (m > iu && a.mirror_hi) src = 2 * iu + 1 - m;
-    double rf = c == IPR ? pref[src] : dref[src];
+    double rf = c == IPR ? pref(src) : dref(src);
     return T(double(w[c * r.sv + row + src]) - rf);
   };
 
@@ -270,8 +263,8 @@ __global__ void __launch_bounds__(256)
     T drp = weno5_minus(c1, c2, c3, c4, c5);
     // face densities of the reference: cp5 rows on dref guarded by the two
     // cells' range, else the scan's dsf; one mean value on a reflecting wall
-    double d0 = dref[i - 3], d1 = dref[i - 2], d2 = dref[i - 1], d3 = dref[i],
-           d4 = dref[i + 1], d5 = dref[i + 2];
+    double d0 = dref(i - 3), d1 = dref(i - 2), d2 = dref(i - 1), d3 = dref(i),
+           d4 = dref(i + 1), d5 = dref(i + 2);
     double fl = cp5_plus(d0, d1, d2, d3, d4);
     double fr = cp5_minus(d1, d2, d3, d4, d5);
     double lo = d2 < d3 ? d2 : d3, hi = d2 > d3 ? d2 : d3;
@@ -300,6 +293,160 @@ __global__ void __launch_bounds__(256)
   if constexpr (sizeof(TF) == 4) out[IVX] = out[IVX] - T(psf);
   for (int v = 0; v < 5; ++v) flx[v * r.sv + row + i] = TF(out[v]);
   if (fp) fp[row + i] = double(pface);
+}
+
+template <typename T, typename TF>
+__global__ void __launch_bounds__(256)
+    sweep_x1_wb_kernel(T const* __restrict__ w, double const* __restrict__ ref,
+                       TF* __restrict__ flx, double* __restrict__ fp, WbArgs a) {
+  Range3 const& r = a.r;
+  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t tot = int64_t(a.nf) * r.n2 * r.n3;
+  if (idx >= tot) return;
+  int fi = int(idx % a.nf);
+  int i = r.lo1 + fi;
+  int64_t rest = idx / a.nf;
+  int j = r.lo2 + int(rest % r.n2);
+  int k = r.lo3 + int(rest / r.n2);
+  int64_t row = k * r.s3 + j * r.s2;
+  double const* pref = ref + R_PREF * r.sv + row;
+  double const* dref = ref + R_DREF * r.sv + row;
+  double psf = ref[R_PSF_LO * r.sv + row + i];
+  double dsf = ref[R_DSF * r.sv + row + i];
+  wb_face<T, TF>(
+      w, row, fi, psf, dsf, [&](int m) { return pref[m]; }, [&](int m) { return dref[m]; },
+      flx, fp, a);
+}
+
+// The reference and the well-balanced sweep fused, for the interior
+// columns: kRefCols columns per workgroup run hydro_ref_x1_kernel's phases
+// (the pressure steps and rho / p, the scans one thread per column, the
+// smoothing, the cell pass), keeping the reference pressure and density of
+// every cell and the raw faces in LDS; only the face pressure psf_lo goes to
+// HBM (the divergence kernel's face gravity reads it).  Then the threads
+// share the column's faces, each as wb_face with the reference read from
+// LDS.  The arithmetic is that of the two kernels, so the fluxes are the
+// same bits.
+template <typename T, typename TF>
+__global__ void __launch_bounds__(kRefThreads)
+    sweep_x1_wb_ref_kernel(double const* __restrict__ w, double* __restrict__ ref,
+                           TF* __restrict__ flx, double* __restrict__ fp, RefArgs ra,
+                           WbArgs a, int nc) {
+  extern __shared__ double smem[];
+  __shared__ int64_t s_row[kRefCols];
+  Range3 const& r = a.r;
+  int64_t ncol = int64_t(r.n2) * r.n3;
+  int64_t col0 = int64_t(blockIdx.x) * nc;
+  int const tid = threadIdx.x;
+  int nc1 = r.nc1, iu = ra.iu;
+  int const stride = 4 * nc1 + 1;  // doubles per column
+  int ncl = ncol - col0 < nc ? int(ncol - col0) : nc;
+  double const g = ra.grav, dx = ra.dx1, mp = DBL_MIN;
+  auto floor_mp = [&](double v) { return v > mp ? v : mp; };
+  if (tid < ncl) {
+    int64_t col = col0 + tid;
+    int j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
+    s_row[tid] = k * r.s3 + j * r.s2;
+  }
+  __syncthreads();
+  int const cells = ncl * nc1;
+  for (int e = tid; e < cells; e += kRefThreads) {
+    int c = e / nc1, i = e - c * nc1;
+    int64_t base = s_row[c];
+    double rho = w[base + i], pr = w[4 * r.sv + base + i];
+    double* sc = smem + c * stride;
+    sc[i] = g * rho * dx;
+    sc[nc1 + i] = rho / pr;
+  }
+  __syncthreads();
+  int il = nc1 - 1 - iu;
+  bool wc = ra.wall_clamp;
+  bool thin = (iu - il + 1) < 5, fits = iu >= 4;
+  int jlo = wc ? il : 0, jhi = wc ? iu : nc1 - 1;
+  bool ext_lo = wc && il + 1 <= iu, ext_hi = wc && iu - 1 >= il;
+  if (tid < ncl) {
+    double* sc = smem + tid * stride;
+    double const* s_dp = sc;
+    double* s_F = sc + 2 * nc1;
+    int64_t base = s_row[tid];
+    double rho_u = w[base + iu], pr_u = w[4 * r.sv + base + iu];
+    double rdt_top = pr_u / rho_u;
+    double top = pr_u * exp(-g * 0.5 * dx / rdt_top);
+    s_F[iu + 1] = top;
+    double face = top;
+    constexpr int kB = 8;
+    int i = iu;
+    for (; i - kB + 1 >= 0; i -= kB) {
+      double t[kB];
+#pragma unroll
+      for (int m = 0; m < kB; ++m) t[m] = s_dp[i - m];
+#pragma unroll
+      for (int m = 0; m < kB; ++m) {
+        face = face + t[m];
+        s_F[i - m] = face;
+      }
+    }
+    for (; i >= 0; --i) {
+      face = face + s_dp[i];
+      s_F[i] = face;
+    }
+    face = top;
+    for (int k = iu + 1; k < nc1; ++k) {
+      face = face - s_dp[k];
+      s_F[k + 1] = face;
+    }
+  }
+  for (int e = tid; e < cells; e += kRefThreads) {
+    int c = e / nc1, i = e - c * nc1;
+    double* sc = smem + c * stride;
+    sc[3 * nc1 + 1 + i] = rop_smooth(sc + nc1, i, jlo, jhi, ext_lo, ext_hi);
+  }
+  __syncthreads();
+  // the cell pass: pref into the pressure-step slot, dref into the rho / p
+  // slot (neither is read again)
+  for (int e = tid; e < cells; e += kRefThreads) {
+    int c = e / nc1, i = e - c * nc1;
+    double* sc = smem + c * stride;
+    double const* s_F = sc + 2 * nc1;
+    double const* s_rs = sc + 3 * nc1 + 1;
+    bool wall_in = wc && i >= il && i < il + 2;
+    bool wall_out = wc && i > iu - 2 && i <= iu;
+    bool clamp_in = wall_in && fits && !thin;
+    bool clamp_out = wall_out && fits && !thin;
+    double lo = floor_mp(s_F[i]), hi = floor_mp(s_F[i + 1]);
+    double cell = 0.5 * (lo + hi);
+    double lower = lo < hi ? lo : hi, upper = lo > hi ? lo : hi;
+    auto six = [&](int start, double const* wt, bool rev) {
+      double value = 0.;
+      for (int m = 0; m < 6; ++m)
+        value += wt[rev ? 5 - m : m] * floor_mp(s_F[start + m]);
+      if (value >= lower && value <= upper) cell = value;
+    };
+    if (i >= 2 && i < nc1 - 2 && !wall_in && !wall_out) six(i - 2, kW6, false);
+    if (clamp_in) six(il, kW6E[i - il], false);
+    if (clamp_out) {
+      int s = iu + 1 - 5;
+      six(s, kW6E[4 - (i - s)], true);
+    }
+    sc[i] = cell;
+    sc[nc1 + i] = cell * s_rs[i];
+    ref[R_PSF_LO * r.sv + s_row[c] + i] = lo;
+  }
+  __syncthreads();
+  int const nf = a.nf;
+  int const faces = ncl * nf;
+  for (int e = tid; e < faces; e += kRefThreads) {
+    int c = e / nf, fi = e - c * nf;
+    int i = r.lo1 + fi;
+    double const* sc = smem + c * stride;
+    double const* s_F = sc + 2 * nc1;
+    double const* s_rs = sc + 3 * nc1 + 1;
+    double lo = floor_mp(s_F[i]);
+    double rf = i > 0 ? 0.5 * (s_rs[i - 1] + s_rs[i]) : s_rs[i];
+    wb_face<double, TF>(
+        w, s_row[c], fi, lo, lo * rf, [&](int m) { return sc[m]; },
+        [&](int m) { return sc[nc1 + m]; }, flx, fp, a);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -417,6 +564,32 @@ __global__ void __launch_bounds__(256)
 
 using namespace snapy;
 
+static WbArgs wb_args(const snapy_grid* g, const snapy_physics* p) {
+  WbArgs a;
+  a.r = ranges(*g);
+  a.nf = g->nx1 + 1;
+  a.gammad = p->gammad;
+  a.riemann = p->riemann;
+  a.shock = p->shock_x1;
+  a.mirror_lo = p->bc[0] != SNAPY_BC_OUTFLOW;
+  a.mirror_hi = p->bc[1] != SNAPY_BC_OUTFLOW;
+  a.refl_lo = p->bc[0] == SNAPY_BC_REFLECTING;
+  a.refl_hi = p->bc[1] == SNAPY_BC_REFLECTING;
+  a.wall_lo = a.refl_lo;
+  a.wall_hi = a.refl_hi;
+  return a;
+}
+
+static RefArgs ref_args(const snapy_grid* g, const snapy_physics* p) {
+  RefArgs a;
+  a.r = ranges(*g);
+  a.iu = a.r.lo1 + g->nx1 - 1;
+  a.grav = -p->grav1;
+  a.dx1 = g->dx1;
+  a.wall_clamp = 1;
+  return a;
+}
+
 extern "C" {
 
 int snapy_hydro_ref_x1(const snapy_grid* g, const snapy_physics* p,
@@ -425,12 +598,7 @@ int snapy_hydro_ref_x1(const snapy_grid* g, const snapy_physics* p,
   if (!p || !w || !ref) return fail(-1, "hydro_ref_x1: NULL argument");
   if (g->nc1 < 6) return fail(-9, "hydro_ref_x1: x1 is not resolved");
   if (p->grav1 == 0.) return fail(-12, "hydro_ref_x1: no x1 gravity");
-  RefArgs a;
-  a.r = ranges(*g);
-  a.iu = a.r.lo1 + g->nx1 - 1;
-  a.grav = -p->grav1;
-  a.dx1 = g->dx1;
-  a.wall_clamp = 1;
+  RefArgs a = ref_args(g, p);
   int64_t ncol = int64_t(g->nc2) * g->nc3;
   // columns per workgroup: kRefCols, fewer for a tall column
   int nc = kRefCols;
@@ -449,18 +617,7 @@ int snapy_sweep_x1_wb(const snapy_grid* g, const snapy_physics* p, int dtype,
   if (!p || !w || !ref || !flux) return fail(-1, "sweep_x1_wb: NULL argument");
   if (g->nx1 < 2) return fail(-9, "sweep_x1_wb: x1 is not resolved");
   if (dtype != SNAPY_F64 && dtype != SNAPY_F32) return fail(-10, "unknown dtype");
-  WbArgs a;
-  a.r = ranges(*g);
-  a.nf = g->nx1 + 1;
-  a.gammad = p->gammad;
-  a.riemann = p->riemann;
-  a.shock = p->shock_x1;
-  a.mirror_lo = p->bc[0] != SNAPY_BC_OUTFLOW;
-  a.mirror_hi = p->bc[1] != SNAPY_BC_OUTFLOW;
-  a.refl_lo = p->bc[0] == SNAPY_BC_REFLECTING;
-  a.refl_hi = p->bc[1] == SNAPY_BC_REFLECTING;
-  a.wall_lo = a.refl_lo;
-  a.wall_hi = a.refl_hi;
+  WbArgs a = wb_args(g, p);
   int64_t tot = int64_t(a.nf) * g->nx2 * g->nx3;
   unsigned tpb = tuned(SNAPY_TUNE_SWEEP, g, dtype, 256);
   if (dtype == SNAPY_F64)
@@ -476,6 +633,33 @@ int snapy_sweep_x1_wb(const snapy_grid* g, const snapy_physics* p, int dtype,
                        static_cast<float*>(flux),
                        static_cast<double*>(face_pressure), a);
   return launch_status("sweep_x1_wb");
+}
+
+int snapy_sweep_x1_wb_ref(const snapy_grid* g, const snapy_physics* p, int dtype,
+                          const double* w, double* ref, void* flux, void* face_pressure,
+                          void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (!p || !w || !ref || !flux) return fail(-1, "sweep_x1_wb_ref: NULL argument");
+  if (g->nc1 < 6) return fail(-9, "sweep_x1_wb_ref: x1 is not resolved");
+  if (p->grav1 == 0.) return fail(-12, "sweep_x1_wb_ref: no x1 gravity");
+  if (dtype != SNAPY_F64 && dtype != SNAPY_F32) return fail(-10, "unknown dtype");
+  WbArgs a = wb_args(g, p);
+  RefArgs ra = ref_args(g, p);
+  int64_t ncol = int64_t(g->nx2) * g->nx3;
+  int nc = kRefCols;
+  auto lds_of = [&](int c) { return size_t(c) * (4 * g->nc1 + 1) * sizeof(double); };
+  while (nc > 1 && lds_of(nc) > 64 * 1024) nc /= 2;
+  if (lds_of(nc) > 64 * 1024) return fail(-9, "sweep_x1_wb_ref: an x1 column above 2047 cells");
+  dim3 grid(unsigned((ncol + nc - 1) / nc));
+  if (dtype == SNAPY_F64)
+    hipLaunchKernelGGL((sweep_x1_wb_ref_kernel<double, double>), grid, dim3(kRefThreads),
+                       lds_of(nc), as_stream(stream), w, ref, static_cast<double*>(flux),
+                       static_cast<double*>(face_pressure), ra, a, nc);
+  else
+    hipLaunchKernelGGL((sweep_x1_wb_ref_kernel<double, float>), grid, dim3(kRefThreads),
+                       lds_of(nc), as_stream(stream), w, ref, static_cast<float*>(flux),
+                       static_cast<double*>(face_pressure), ra, a, nc);
+  return launch_status("sweep_x1_wb_ref");
 }
 
 int snapy_update_gravity(const snapy_grid* g, const snapy_physics* p,

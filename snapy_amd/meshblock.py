@@ -408,9 +408,13 @@ class MeshBlock:
                 continue
             fp = freg.face_pressure1 if d == 1 else None
             if d == 1 and grav:
-                calls.append(("hydro_ref_x1", lambda: lib.snapy_hydro_ref_x1(g, p, P(w), P(self._ref), s)))
+                # the reference (kept on chip but for its face pressure) and
+                # the well-balanced sweep in one launch
                 calls.append(
-                    ("sweep_x1_wb", lambda t=t, fp=fp: lib.snapy_sweep_x1_wb(g, p, dtc, P(w), P(self._ref), P(t), P(fp), s))
+                    (
+                        "sweep_x1_wb",
+                        lambda t=t, fp=fp: lib.snapy_sweep_x1_wb_ref(g, p, dtc, P(w), P(self._ref), P(t), P(fp), s),
+                    )
                 )
             elif d > 1 and self.options.coord.nx1 >= 3 and self.nvar == 5:
                 # x2/x3 with the x1 flux covariance (on by default)

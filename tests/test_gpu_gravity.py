@@ -119,3 +119,33 @@ def test_gravity_table_matches_oracle_weights():
     idx, wt = gravity.centroid_slope_weights(g.centers(1)[3:23])
     for r in range(3):
         assert np.array_equal(tab[4 + r, 3:23], wt[r])
+
+
+@pytest.mark.parametrize("g", GRIDS + [fv.Grid(20, 7, 5, 0.0, 4000.0, -700.0, 700.0, 0.0, 500.0)])
+@pytest.mark.parametrize("dtype", [0, 1])
+@pytest.mark.parametrize("bcs", [("reflecting",) * 6, ("outflow", "outflow") + ("periodic",) * 4])
+def test_fused_reference_sweep_is_the_two_launches(g, dtype, bcs):
+    """snapy_sweep_x1_wb_ref gives the same bits as snapy_hydro_ref_x1 then
+    snapy_sweep_x1_wb: the fluxes, the face pressure and the psf_lo plane of
+    every interior column (a partial last workgroup included)."""
+    lib = _capi.load()
+    w = stratified(g, seed=11, noise=0.3)
+    pd = phys_desc(gammad=GAMMA, riemann="lmars", shock_x1=False, shock_x23=False, bcs=bcs, grav1=-9.81)
+    gd = grid_desc(g)
+    wd = dev(w)
+    fdt = torch.float64 if dtype == 0 else torch.float32
+    ref_a = torch.zeros((5,) + g.shape, dtype=torch.float64, device="cuda")
+    ref_b = torch.zeros_like(ref_a)
+    fa, fb = torch.zeros(w.shape, dtype=fdt, device="cuda"), torch.zeros(w.shape, dtype=fdt, device="cuda")
+    pa, pb = torch.zeros_like(wd[0]), torch.zeros_like(wd[0])
+    print(f"fused x1 reference sweep {g.nx1}x{g.nx2}x{g.nx3} dtype {dtype} {bcs[0]}", flush=True)
+    assert lib.snapy_hydro_ref_x1(ctypes.byref(gd), ctypes.byref(pd), wd.data_ptr(), ref_a.data_ptr(), None) == 0
+    assert lib.snapy_sweep_x1_wb(ctypes.byref(gd), ctypes.byref(pd), dtype, wd.data_ptr(), ref_a.data_ptr(),
+                                 fa.data_ptr(), pa.data_ptr(), None) == 0
+    assert lib.snapy_sweep_x1_wb_ref(ctypes.byref(gd), ctypes.byref(pd), dtype, wd.data_ptr(), ref_b.data_ptr(),
+                                     fb.data_ptr(), pb.data_ptr(), None) == 0
+    k, j, i = g.interior()
+    cols = (slice(None), k, j)
+    assert torch.equal(fa[cols], fb[cols])
+    assert torch.equal(pa[k, j], pb[k, j])
+    assert torch.equal(ref_a[0][k, j], ref_b[0][k, j])
