@@ -233,3 +233,40 @@ def test_sod_converges_to_the_exact_solution():
         ex = fv.sod_exact(d.grid.centers(1)[3:-3], 0.1)
         errs.append(np.abs(w[0] - ex[0]).mean())
     assert errs[1] < 0.7 * errs[0]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_flux_covariance_is_the_face_average_of_linear_profiles(dim):
+    """The covariance term makes the energy flux of an x2/x3 face the average
+    over the face's x1 extent: with uniform density and h, u_n linear in x1,
+    rho <u_n h> - rho u_n(x) h(x) = (dx1^2 / 12) rho h' u_n' exactly.  The
+    average is taken by 2-point Gauss-Legendre quadrature, exact for the
+    quadratic product.  The face states enter as their mean (wl != wr), and
+    the other tangential velocity does not enter."""
+    gamma = 1.4
+    g = fv.Grid(10, 6, 4, 0.0, 500.0, 0.0, 600.0, 0.0, 400.0)
+    x1 = g.centers(1)
+    dx1 = g.dx(1)
+    rho0, p0, a, u0, b = 1.2, 9.0e4, -11.0, 3.0, 0.02
+    w = np.zeros((5,) + g.shape)
+    w[fv.IDN] = rho0
+    w[fv.IPR] = p0 + a * x1
+    un_row = fv.IVY if dim == 2 else fv.IVZ
+    other = fv.IVZ if dim == 2 else fv.IVY
+    w[un_row] = u0 + b * x1
+    w[other] = np.sin(x1 / 37.0)  # must not enter
+    rng = np.random.default_rng(3)
+    d = 1e-3 * rng.standard_normal(w.shape)
+    d[fv.IDN] = 0.0
+    cov = fv.flux_covariance(w + d, w - d, dim, g, gamma)
+    k = gamma / (gamma - 1.0)
+    h = lambda x: k * (p0 + a * x) / rho0  # noqa: E731
+    un = lambda x: u0 + b * x  # noqa: E731
+    xi = dx1 / (2.0 * np.sqrt(3.0))
+    xc = x1[1:-1]
+    avg = 0.5 * (un(xc - xi) * h(xc - xi) + un(xc + xi) * h(xc + xi))
+    expected = rho0 * (avg - un(xc) * h(xc))
+    got = cov[0, 0]
+    assert np.allclose(got, np.broadcast_to(expected, got.shape), rtol=1e-9, atol=0.0)
+    # and the closed form
+    assert np.allclose(got, dx1 * dx1 / 12.0 * rho0 * (k * a / rho0) * b, rtol=1e-9, atol=0.0)
