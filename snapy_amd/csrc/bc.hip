@@ -53,10 +53,21 @@ __global__ void __launch_bounds__(256)
   int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t tot = int64_t(2) * a.ng * a.nrow;
   if (idx >= tot) return;
-  int row = int(idx % a.nrow);
-  int64_t rest = idx / a.nrow;
-  int g = int(rest % a.ng);
-  int side = int(rest / a.ng);
+  int row, g, side;
+  if (a.st == 1) {
+    // x1: the ghost layers of one column are adjacent in memory, so the
+    // ghost index runs fastest (a wavefront touches ~11 columns' lines, not
+    // 64)
+    int gs = int(idx % (2 * a.ng));
+    row = int(idx / (2 * a.ng));
+    side = gs / a.ng;
+    g = gs - side * a.ng;
+  } else {
+    row = int(idx % a.nrow);
+    int64_t rest = idx / a.nrow;
+    g = int(rest % a.ng);
+    side = int(rest / a.ng);
+  }
   int bc = side ? a.bc_hi : a.bc_lo;
   int dst, src;
   bool flip = false;
