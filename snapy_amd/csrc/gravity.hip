@@ -1,6 +1,7 @@
 // The gravity path of the x1 sweep on gfx950: the hydrostatic x1 reference
-// scan, the fused well-balanced x1 reconstruction + Riemann sweep, and the
-// fused divergence + const gravity + face gravity work + RK combine.
+// scan, the fused well-balanced x1 reconstruction + Riemann sweep (alone,
+// and with the reference fused in, the stage's launch), and the fused
+// divergence + const gravity + face gravity work + RK combine.
 //
 // Uniform Cartesian block owning its whole x1 column (nb1 = 1), conversion
 // order 2, dry ideal gas, non-hydrostatic factor 1 (the const-gravity
@@ -17,7 +18,7 @@ namespace snapy {
 namespace {
 
 // ---------------------------------------------------------------------------
-// hydrostatic x1 reference, one thread per column
+// hydrostatic x1 reference, its scans one thread per column
 // (hydro_ref_x1_impl, src/hydro/hydro_ref_x1_impl.h:24-272, uniform branch)
 // ---------------------------------------------------------------------------
 // ref layout: 5 planes of (nc3, nc2, nc1): psf_lo, psf_hi, pref, dsf, dref
