@@ -121,7 +121,9 @@ def test_gravity_table_matches_oracle_weights():
         assert np.array_equal(tab[4 + r, 3:23], wt[r])
 
 
-@pytest.mark.parametrize("g", GRIDS + [fv.Grid(20, 7, 5, 0.0, 4000.0, -700.0, 700.0, 0.0, 500.0)])
+@pytest.mark.parametrize("g", GRIDS + [fv.Grid(20, 7, 5, 0.0, 4000.0, -700.0, 700.0, 0.0, 500.0),
+                                       # a tall column: fewer columns per workgroup
+                                       fv.Grid(640, 3, 1, 0.0, 16000.0, -300.0, 300.0, 0.0, 100.0)])
 @pytest.mark.parametrize("dtype", [0, 1])
 @pytest.mark.parametrize("bcs", [("reflecting",) * 6, ("outflow", "outflow") + ("periodic",) * 4])
 def test_fused_reference_sweep_is_the_two_launches(g, dtype, bcs):

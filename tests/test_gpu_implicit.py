@@ -18,8 +18,13 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("entry", ["snapy_vic_full", "snapy_vic_full_serial", "snapy_vic_partial"])
-@pytest.mark.parametrize("shape", [(16, 6, 1), (12, 4, 3), (40, 37, 3)])
+@pytest.mark.parametrize("shape", [(16, 6, 1), (12, 4, 3), (40, 37, 3), (400, 6, 1)])
 def test_vic_kernel_matches_oracle(shape, entry):
+    """One call on random columns against the oracle; the 400-layer columns
+    take the finish kernel's fallback to fewer columns per workgroup (its
+    LDS holds 3 (n + 1) doubles per column)."""
+    if shape[0] > 100 and entry != "snapy_vic_partial":
+        pytest.skip("the tall column exercises the scheme-1 finish kernel's launch fallback")
     lib = _capi.load()
     g = fv.Grid(*shape, 0.0, 50.0 * shape[0], -400.0, 400.0, 0.0, 300.0)
     w = stratified(g, seed=5, noise=0.5)
