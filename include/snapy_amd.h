@@ -367,19 +367,24 @@ int snapy_budget(const snapy_grid *g, int dtype, const void *u, double *ws,
 /* ---- launch-shape tuning --------------------------------------------------
  *
  * The workgroup size of a kernel family for blocks of one shape (nx1, nx2,
- * nx3) and dtype, or the persistent grid of the work list (any shape): set
- * by the project's autotuner, read at every launch.  A tuned shape only
- * changes how items are grouped into workgroups, so results are bit-identical
- * for every value.  Workgroup sizes: 64, 128 or 256 (default 256); the work
- * list's grid: 64 .. 65536 workgroups (default 2048).  value 0 restores the
- * default.  No counterpart in the reference (its CUDA launches use fixed
+ * nx3) and dtype, the columns per workgroup of the column kernels, or the
+ * persistent grid of the work list (any shape): set by the project's
+ * autotuner, read at every launch.  A tuned shape only changes how items are
+ * grouped into workgroups, so results are bit-identical for every value.
+ * Workgroup sizes: 64, 128 or 256 (default 256); columns per workgroup: 1,
+ * 2, 4 or 8 for the scheme-1 finish (default 8), 1, 2 or 4 for the fused x1
+ * reference sweep (default 4), fewer where a tall column's LDS needs it; the
+ * work list's grid: 64 .. 65536 workgroups (default 2048).  value 0 restores
+ * the default.  No counterpart in the reference (its CUDA launches use fixed
  * shapes). */
 enum snapy_tunable {
   SNAPY_TUNE_EOS = 0,     /* cons2prim / prim2cons (FP64 and the FP32 pair) */
   SNAPY_TUNE_SWEEP = 1,   /* snapy_sweep (x1, x2, x3) and snapy_sweep_x1_wb */
   SNAPY_TUNE_UPDATE = 2,  /* snapy_update(_f32), snapy_update_gravity(_f32) */
   SNAPY_TUNE_WL_GRID = 3, /* the work list's persistent grid */
-  SNAPY_TUNE_COUNT = 4
+  SNAPY_TUNE_VIC_COLS = 4, /* columns per workgroup, scheme-1 finish */
+  SNAPY_TUNE_REF_COLS = 5, /* columns per workgroup, snapy_sweep_x1_wb_ref */
+  SNAPY_TUNE_COUNT = 6
 };
 int snapy_tune_set(int tunable, const snapy_grid *g, int dtype, int value);
 int snapy_tune_get(int tunable, const snapy_grid *g, int dtype);

@@ -2,16 +2,18 @@
 
 For a block's shape and dtype it chooses the workgroup size of three kernel
 families -- the EOS (cons2prim), the face sweeps (x1, and the plain x2/x3
-sweeps) and the divergence + combine updates -- by timing every candidate
-on the device with HIP events, and it chooses the persistent grid of the
-work list (`worklist.BlockBatch`).  A choice goes into the library's tuning
+sweeps) and the divergence + combine updates -- and the columns per
+workgroup of two column kernels -- the scheme-1 vertical implicit finish
+and the fused x1 reference sweep -- by timing every candidate on the device
+with HIP events, and it chooses the persistent grid of the work list
+(`worklist.BlockBatch`).  A choice goes into the library's tuning
 table (snapy_tune_set), which every later launch of that family for that
 shape and dtype reads, and into a JSON cache, so a later process applies it
 without timing again (`apply_cache`).
 
-A workgroup size changes only how the cells are grouped into workgroups,
-never the arithmetic of a cell, so tuned and untuned runs give the same
-bits (tests/test_gpu_autotune.py).
+A workgroup size or a column count changes only how the cells (columns)
+are grouped into workgroups, never the arithmetic of a cell, so tuned and
+untuned runs give the same bits (tests/test_gpu_autotune.py).
 """
 
 from __future__ import annotations
@@ -25,15 +27,17 @@ import torch
 
 from . import _capi
 
-TUNE_EOS, TUNE_SWEEP, TUNE_UPDATE, TUNE_WL_GRID = 0, 1, 2, 3
+TUNE_EOS, TUNE_SWEEP, TUNE_UPDATE, TUNE_WL_GRID, TUNE_VIC_COLS, TUNE_REF_COLS = 0, 1, 2, 3, 4, 5
 WORKGROUPS = (64, 128, 256)
 WL_GRIDS = (256, 512, 1024, 2048, 4096)
 
-# family -> (tunable, the stage launches it covers)
+# family -> (tunable, the stage launches it covers, candidates, default)
 FAMILIES = {
-    "eos": (TUNE_EOS, ("cons2prim",)),
-    "sweep": (TUNE_SWEEP, ("sweep_x1", "sweep_x1_wb", "sweep_x2", "sweep_x3")),
-    "update": (TUNE_UPDATE, ("update", "update_gravity", "stage_increment")),
+    "eos": (TUNE_EOS, ("cons2prim",), WORKGROUPS, 256),
+    "sweep": (TUNE_SWEEP, ("sweep_x1", "sweep_x2", "sweep_x3"), WORKGROUPS, 256),
+    "update": (TUNE_UPDATE, ("update", "update_gravity", "stage_increment"), WORKGROUPS, 256),
+    "vic_cols": (TUNE_VIC_COLS, ("vic_partial",), (2, 4, 8), 8),
+    "ref_cols": (TUNE_REF_COLS, ("sweep_x1_wb",), (1, 2, 4), 4),
 }
 
 
@@ -55,8 +59,8 @@ def _dtype_code(block) -> int:
 
 
 def set_choice(block, family: str, value: int) -> None:
-    """Put one family's workgroup size for this block's shape in the table
-    (0 restores the default)."""
+    """Put one family's workgroup size (column count) for this block's shape
+    in the table (0 restores the default)."""
     lib = _capi.load()
     t = FAMILIES[family][0]
     _capi.check(lib.snapy_tune_set(t, ctypes.byref(block._grid), _dtype_code(block), int(value)), "tune_set")
@@ -82,28 +86,33 @@ def _time(fns, reps: int) -> float:
     return e0.elapsed_time(e1) / reps
 
 
-def tune_block(block, vars, reps: int = 10, cache: Path | str | None = None) -> dict:
-    """Time every workgroup size of every family on this block (its stage-0
+def tune_block(block, vars, reps: int = 10, cache: Path | str | None = None,
+               rounds: int = 3, tol: float = 0.02) -> dict:
+    """Time every candidate of every family on this block (its stage-0
     launches; the block's state is restored afterwards), keep the fastest in
     the library's table and, when `cache` is given (or True: the default
-    path), in the JSON cache.  Returns {family: {"choice": size, "ms":
-    {size: ms}}}."""
+    path), in the JSON cache.  The candidates are timed in `rounds` passes
+    in turn and each keeps its best pass; the default stays unless another
+    candidate beats it by more than `tol`, so timing noise does not move a
+    shape.  Returns {family: {"choice": value, "ms": {value: ms}}}."""
     out = {}
     # the stage-0 launches advance the state: keep it and restore it after
     keep = [vars["hydro_u"], vars["hydro_w"], block._state.u0()]
     keep += [t for t in (block._state.comp(), block._state.comp0()) if t is not None]
     saved = [t.clone() for t in keep]
     calls = dict(block._stage_calls(vars, 0))
-    for fam, (_, names) in FAMILIES.items():
+    for fam, (_, names, cands, dflt) in FAMILIES.items():
         fns = [calls[n] for n in names if n in calls]
         if not fns:
             continue
-        ms = {}
-        for v in WORKGROUPS:
-            set_choice(block, fam, v)
-            ms[v] = _time(fns, reps)
-        # the default wins ties
-        best = min(WORKGROUPS, key=lambda v: (ms[v], v != 256))
+        ms = {v: float("inf") for v in cands}
+        for _ in range(max(1, rounds)):
+            for v in cands:
+                set_choice(block, fam, v)
+                ms[v] = min(ms[v], _time(fns, reps))
+        best = min(cands, key=lambda v: ms[v])
+        if ms[dflt] <= ms[best] * (1.0 + tol):
+            best = dflt
         set_choice(block, fam, best)
         out[fam] = {"choice": best, "ms": ms}
     for t, c in zip(keep, saved):

@@ -647,7 +647,10 @@ int snapy_sweep_x1_wb_ref(const snapy_grid* g, const snapy_physics* p, int dtype
   WbArgs a = wb_args(g, p);
   RefArgs ra = ref_args(g, p);
   int64_t ncol = int64_t(g->nx2) * g->nx3;
-  int nc = kRefCols;
+  // columns per workgroup: the tuned value (default kRefCols), fewer for a
+  // tall column
+  int nc = int(tuned(SNAPY_TUNE_REF_COLS, g, dtype, kRefCols));
+  if (nc > kRefCols) nc = kRefCols;
   auto lds_of = [&](int c) { return size_t(c) * (4 * g->nc1 + 1) * sizeof(double); };
   while (nc > 1 && lds_of(nc) > 64 * 1024) nc /= 2;
   if (lds_of(nc) > 64 * 1024) return fail(-9, "sweep_x1_wb_ref: an x1 column above 2047 cells");

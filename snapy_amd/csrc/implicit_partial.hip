@@ -955,8 +955,10 @@ static int vicp_launch(const snapy_grid* g, const snapy_physics* p, const double
   hipLaunchKernelGGL(vicp_forward_kernel, dim3(blocks_for(ncol, 64)), dim3(128), 0,
                      as_stream(stream), w, du, vtab, ws, yc, status, a);
   if (int e = launch_status("vic_partial_forward")) return e;
-  // columns per finish workgroup: kFinCols, fewer for a tall column
-  int nc = kFinCols;
+  // columns per finish workgroup: the tuned value (default kFinCols), fewer
+  // for a tall column
+  int nc = int(tuned(SNAPY_TUNE_VIC_COLS, g, pair ? SNAPY_F32 : SNAPY_F64, kFinCols));
+  if (nc > kFinCols) nc = kFinCols;
   while (nc > 1 && size_t(3) * nc * (g->nx1 + 1) * sizeof(double) > 64 * 1024) nc /= 2;
   size_t lds = size_t(3) * nc * (g->nx1 + 1) * sizeof(double);
   if (lds > 64 * 1024) return fail(-9, "vic_partial: the column does not fit the LDS");

@@ -1,5 +1,6 @@
 // Launch-shape tuning: the workgroup size a kernel family uses for blocks of
-// one shape and dtype (or, for the work list, the persistent grid), set by
+// one shape and dtype (the columns per workgroup of the column kernels; for
+// the work list, the persistent grid), set by
 // the autotuner (snapy_amd/autotune.py) and read by the launchers.  A tuned
 // shape changes only how the items are split into workgroups, never the
 // arithmetic of an item, so the results are the same bits for every value.
@@ -25,6 +26,8 @@ Key key_of(int t, snapy_grid const* g, int dtype) {
 
 bool allowed(int t, int v) {
   if (t == SNAPY_TUNE_WL_GRID) return v >= 64 && v <= 65536;
+  if (t == SNAPY_TUNE_VIC_COLS) return v == 1 || v == 2 || v == 4 || v == 8;
+  if (t == SNAPY_TUNE_REF_COLS) return v == 1 || v == 2 || v == 4;
   return v == 64 || v == 128 || v == 256;
 }
 
@@ -58,7 +61,10 @@ int snapy_tune_set(int tunable, const snapy_grid* g, int dtype, int value) {
 
 int snapy_tune_get(int tunable, const snapy_grid* g, int dtype) {
   if (tunable < 0 || tunable >= SNAPY_TUNE_COUNT) return fail(-1, "tune_get: unknown tunable");
-  unsigned dflt = tunable == SNAPY_TUNE_WL_GRID ? 2048u : 256u;
+  unsigned dflt = tunable == SNAPY_TUNE_WL_GRID    ? 2048u
+                  : tunable == SNAPY_TUNE_VIC_COLS ? 8u
+                  : tunable == SNAPY_TUNE_REF_COLS ? 4u
+                                                   : 256u;
   return int(tuned(tunable, tunable == SNAPY_TUNE_WL_GRID ? nullptr : g, dtype, dflt));
 }
 

@@ -42,12 +42,23 @@ def test_tuning_table():
     def bad():
         out.append(lib.snapy_tune_set(autotune.TUNE_EOS, gp, 0, 96))
         out.append(lib.snapy_tune_set(7, gp, 0, 64))
+        out.append(lib.snapy_tune_set(autotune.TUNE_VIC_COLS, gp, 0, 3))
+        out.append(lib.snapy_tune_set(autotune.TUNE_REF_COLS, gp, 0, 8))
 
     t = threading.Thread(target=bad)
     t.start()
     t.join()
-    assert out[0] < 0 and out[1] < 0
+    assert all(r < 0 for r in out) and len(out) == 4
     assert lib.snapy_tune_get(autotune.TUNE_EOS, gp, 0) == 256
+    # the column kernels' columns per workgroup, with their own defaults
+    assert lib.snapy_tune_get(autotune.TUNE_VIC_COLS, gp, 1) == 8
+    assert lib.snapy_tune_get(autotune.TUNE_REF_COLS, gp, 1) == 4
+    assert lib.snapy_tune_set(autotune.TUNE_VIC_COLS, gp, 1, 2) == 0
+    assert lib.snapy_tune_get(autotune.TUNE_VIC_COLS, gp, 1) == 2
+    assert lib.snapy_tune_set(autotune.TUNE_VIC_COLS, gp, 1, 0) == 0
+    for fam, (t_, names, cands, dflt) in autotune.FAMILIES.items():
+        assert dflt in cands
+        assert lib.snapy_tune_get(t_, gp, 0) == dflt
 
 
 def test_cache_round_trip(tmp_path):

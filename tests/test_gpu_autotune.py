@@ -1,7 +1,7 @@
-"""The autotuner on the GPU: it times every workgroup size of every kernel
-family on a block, keeps a valid choice, and a block run with the tuned
-shapes gives the same bits as one run with the default shapes (the
-workgroup size never changes a cell's arithmetic)."""
+"""The autotuner on the GPU: it times every candidate (workgroup size,
+columns per workgroup) of every kernel family on a block, keeps a valid
+choice, and a block run with the tuned shapes gives the same bits as one
+run with the default shapes (a shape never changes a cell's arithmetic)."""
 
 import os
 
@@ -44,13 +44,14 @@ def test_tuned_shapes_give_the_same_bits(precision, tmp_path):
     u_ref = _steps(ref_blk, ref_vars)
     blk, vars = _block(precision)
     res = autotune.tune_block(blk, vars, reps=3, cache=tmp_path / "tune.json")
-    assert set(res) == {"eos", "sweep", "update"}
+    assert set(res) == {"eos", "sweep", "update", "vic_cols", "ref_cols"}
     for fam, r in res.items():
-        assert r["choice"] in autotune.WORKGROUPS
+        assert r["choice"] in autotune.FAMILIES[fam][2]
         assert autotune.get_choice(blk, fam) == r["choice"]
-    # force a non-default shape on every family, then step from a fresh state
+    # force a non-default shape on every family (the smallest candidate),
+    # then step from a fresh state
     for fam in res:
-        autotune.set_choice(blk, fam, 64)
+        autotune.set_choice(blk, fam, autotune.FAMILIES[fam][2][0])
     blk2, vars2 = _block(precision)
     u = _steps(blk2, vars2)
     assert torch.equal(u, u_ref)
