@@ -72,11 +72,18 @@ __device__ __forceinline__ void cov_states(double const* __restrict__ w, CovArgs
     }
   }
   if (a.cov) {
-    // wbar = 0.5 (wl + wr); h = (W->I(wbar) + p) / rho; u_n
+    // wbar = 0.5 (wl + wr); h = (W->I(wbar) + p) / rho; u_n.  FP64 in the
+    // FP64 form; in the FP32 form the FP32 quotients of qdiv (the term is a
+    // small correction to the energy flux)
     double rho = 0.5 * (double(L.r) + double(R.r));
     double p = 0.5 * (double(L.p) + double(R.p)) + gauge;
-    double enth = p / (a.gammad - 1.) + p;
-    h = enth / rho;
+    if constexpr (sizeof(T) == 8) {
+      double enth = p / (a.gammad - 1.) + p;
+      h = enth / rho;
+    } else {
+      float enth = qdiv(float(p), float(a.gammad - 1.)) + float(p);
+      h = double(qdiv(enth, float(rho)));
+    }
     un = 0.5 * (double(L.v[DIR - 1]) + double(R.v[DIR - 1]));
   }
 }
@@ -99,9 +106,16 @@ __device__ __forceinline__ void cov_flux(double const* __restrict__ x1v, T* __re
   if (a.cov) {
     double rho = 0.5 * (double(L.r) + double(R.r));
     double dx1 = x1v[i + 1] - x1v[i - 1];
-    double d1h = (hp - hm) / dx1;
-    double dun = (up - um) / dx1;
-    double ce = ((a.s2 * rho) * d1h) * dun;
+    double ce;
+    if constexpr (sizeof(T) == 8) {
+      double d1h = (hp - hm) / dx1;
+      double dun = (up - um) / dx1;
+      ce = ((a.s2 * rho) * d1h) * dun;
+    } else {
+      float idx = qdiv(1.f, float(dx1));
+      float d1h = float(hp - hm) * idx, dun = float(up - um) * idx;
+      ce = double((float(a.s2 * rho) * d1h) * dun);
+    }
     out[IPR] = T(double(out[IPR]) + ce);
   }
   int f = DIR == 2 ? j - r.lo2 : k - r.lo3;
