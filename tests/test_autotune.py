@@ -56,7 +56,7 @@ def test_tuning_table():
     assert lib.snapy_tune_set(autotune.TUNE_VIC_COLS, gp, 1, 2) == 0
     assert lib.snapy_tune_get(autotune.TUNE_VIC_COLS, gp, 1) == 2
     assert lib.snapy_tune_set(autotune.TUNE_VIC_COLS, gp, 1, 0) == 0
-    for fam, (t_, names, cands, dflt) in autotune.FAMILIES.items():
+    for fam, (t_, names, cands, dflt) in {**autotune.FAMILIES, **autotune.MANUAL}.items():
         assert dflt in cands
         assert lib.snapy_tune_get(t_, gp, 0) == dflt
 
