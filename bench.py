@@ -516,6 +516,12 @@ def main():
             "stage_b_alg_per_cell": b_alg,
             "stage_frac": value * b_alg / (HBM_PEAK_GBS * 1e9),
         }
+        # the measured streaming rate of this box (a plain 1 GiB copy), beside
+        # the nominal peak
+        cp = copy_calibration()
+        roofline["copy_GBps"] = cp["copy_GBps"]
+        roofline["frac_of_copy"] = achieved / cp["copy_GBps"]
+        torch.cuda.empty_cache()
     cpu = None if args.no_cpu else cpu_baseline(args.workload)
     accuracy = None
     if precision == "float32":
