@@ -341,6 +341,56 @@ class MeshBlock:
         self._state.adopt(u)
         return vars, 0.0
 
+    def write_restart(self, vars, path: str, time: float) -> str:
+        """Write this block's restart part to `path` (RestartOutput::
+        write_output_file, src/output/restart.cpp:42-147); see
+        snapy_amd.restart for what it holds."""
+        from . import restart
+
+        torch.cuda.synchronize(self.device)
+        return restart.write_restart(self, vars, path, time)
+
+    def initialize_from_restart(self, path_or_data):
+        """initialize from a restart part (MeshBlockImpl::_init_from_restart,
+        src/mesh/meshblock_restart.cpp:32-60): the metadata and grid checked,
+        the stored state taken back with its own bits (the FP32 form's pair
+        when the part holds one, else the FP64 totals split into the pair),
+        the primitives and the outflow background as saved, the cycle
+        restored.  Returns (vars, time)."""
+        from . import restart
+
+        if self.device.type != "cuda":
+            raise _capi.SnapyError("initialize_from_restart: call block.to('cuda') first")
+        data = restart.load_restart(path_or_data) if isinstance(path_or_data, str) else path_or_data
+        restart.check_restart_metadata(data, restart.grid_faces(self))
+        if tuple(data["hydro_u"].shape) != self.shape():
+            raise restart.RestartError(f"restart: hydro_u has shape {tuple(data['hydro_u'].shape)}, expected {self.shape()}")
+        if "outflow" in self.options.bcnames and "boundary_reference_w" not in data:
+            raise restart.RestartError("restart: an outflow block needs its boundary_reference_w")
+        self._ensure_buffers()
+        dev = self.device
+        vars = {"hydro_w": data["hydro_w"].to(device=dev, dtype=torch.float64).contiguous()}
+        if "boundary_reference_w" in data:
+            vars["boundary_reference_w"] = data["boundary_reference_w"].to(device=dev, dtype=torch.float64).contiguous()
+        stored = data["hydro_u_stored"]
+        if self.policy.fp32():
+            if stored.dtype == torch.float32:
+                u = stored.to(dev).contiguous()
+                comp = data["hydro_u_comp"].to(dev).contiguous()
+            else:
+                # an FP64 part resumed in the FP32 form: the pair of the totals
+                t = data["hydro_u"].to(dev)
+                u = t.float().contiguous()
+                comp = (t - u.double()).float().contiguous()
+            vars["hydro_u"] = u
+            self._state.adopt(u, comp)
+        else:
+            u = data["hydro_u"].to(device=dev, dtype=torch.float64).contiguous()
+            vars["hydro_u"] = u
+            self._state.adopt(u)
+        self.cycle = int(data["last_cycle"][0])
+        return vars, float(data["last_time"][0])
+
     def _adopt(self, vars):
         u = vars["hydro_u"]
         if self._state.u() is None or self._state.u().data_ptr() != u.data_ptr():

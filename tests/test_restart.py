@@ -1,0 +1,98 @@
+"""Restart parts on the CPU: the part name, the archive round trip, and the
+metadata checks of check_restart_metadata (restart.cpp:323-358)."""
+
+import pytest
+import torch
+
+from snapy_amd import restart
+
+
+class _Coord:
+    def __init__(self, n=(8, 6, 1)):
+        self.f = {k: torch.linspace(0.0, 1.0, m + 1, dtype=torch.float64) for k, m in zip(("x1f", "x2f", "x3f"), n)}
+
+    def buffer(self, k):
+        return self.f[k]
+
+
+class _State:
+    def __init__(self, comp):
+        self._c = comp
+
+    def comp(self):
+        return self._c
+
+
+class _Block:
+    """What restart_variables reads of a block."""
+
+    def __init__(self, fp32):
+        torch.manual_seed(0)
+        self.cycle = 17
+        self._coord = _Coord()
+        dt = torch.float32 if fp32 else torch.float64
+        self.u = torch.rand((5, 1, 12, 14), dtype=torch.float64).to(dt)
+        self._state = _State(torch.rand_like(self.u) * 1e-8 if fp32 else None)
+
+    def total_u(self, vars):
+        c = self._state.comp()
+        return self.u.double() + c.double() if c is not None else self.u.clone()
+
+
+def test_part_name():
+    assert restart.part_name("run", 3, 12) == "run.block3.00012.part"
+    assert restart.part_name("run") == "run.block0.final.part"
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_round_trip_and_checks(tmp_path, fp32):
+    blk = _Block(fp32)
+    w = torch.rand((5, 1, 12, 14), dtype=torch.float64)
+    vars = {"hydro_u": blk.u, "hydro_w": w}
+    path = str(tmp_path / restart.part_name("run", 0, 4))
+    restart.write_restart(blk, vars, path, 12.5)
+    data = restart.load_restart(path)
+    assert torch.equal(data["hydro_w"], w)
+    assert torch.equal(data["hydro_u_stored"], blk.u)
+    assert torch.equal(data["hydro_u"], blk.total_u(vars))
+    assert data["hydro_u"].dtype == torch.float64
+    assert float(data["last_time"][0]) == 12.5 and int(data["last_cycle"][0]) == 17
+    assert ("hydro_u_comp" in data) == fp32
+    faces = tuple(blk._coord.buffer(k) for k in ("x1f", "x2f", "x3f"))
+    restart.check_restart_metadata(data, faces)
+    # each check refuses its own mistake
+    bad = dict(data)
+    del bad["last_cycle"]
+    with pytest.raises(restart.RestartError, match="last_cycle"):
+        restart.check_restart_metadata(bad, faces)
+    bad = dict(data)
+    bad[restart.METADATA] = torch.tensor([restart.VERSION + 1, int(data[restart.METADATA][1])])
+    with pytest.raises(restart.RestartError, match="version"):
+        restart.check_restart_metadata(bad, faces)
+    bad = dict(data)
+    bad[restart.METADATA] = torch.tensor([restart.VERSION, 96 - int(data[restart.METADATA][1])])
+    with pytest.raises(restart.RestartError, match="dtype/shape"):
+        restart.check_restart_metadata(bad, faces)
+    other = (faces[0] * 2.0, faces[1], faces[2])
+    with pytest.raises(restart.RestartError, match="grid mismatch at hydro_ref_x1f"):
+        restart.check_restart_metadata(data, other)
+    if fp32:
+        bad = dict(data)
+        del bad["hydro_u_comp"]
+        with pytest.raises(restart.RestartError, match="compensation"):
+            restart.check_restart_metadata(bad, faces)
+
+
+def test_refuses_missing_and_foreign_files(tmp_path):
+    with pytest.raises(restart.RestartError, match="failed to open"):
+        restart.load_restart(str(tmp_path / "nope.part"))
+    p = tmp_path / "junk.part"
+    p.write_bytes(b"not an archive")
+    with pytest.raises(restart.RestartError):
+        restart.load_restart(str(p))
+    from safetensors.torch import save_file
+
+    q = str(tmp_path / "other.part")
+    save_file({"x": torch.zeros(2)}, q)
+    with pytest.raises(restart.RestartError, match="not a restart part"):
+        restart.load_restart(q)
