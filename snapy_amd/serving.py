@@ -13,7 +13,10 @@ round are batched: one work list (`worklist.BlockBatch`) advances all of
 them with one launch per phase on a stream of their own, rebuilt when a
 tube joins or leaves; a tube's result is bit-identical batched or alone.  A request leaves when
 its step count is reached, freeing its slot and memory for the next
-arrival; its latency is completion time minus arrival time.
+arrival; its latency is completion time minus arrival time.  With
+`preempt`, a request that does not fit may instead park the active block
+request with the most steps left in a host restart part; the parked request
+resumes from it when memory frees, bit for bit.
 
 Request kinds (the serving mix of the benchmark):
   * ``sod``              -- a 1D Sod-type tube with seeded left/right states;
@@ -221,6 +224,11 @@ class Server:
     max_slots: int = 16
     realtime: bool = True  # honour arrival times against the wall clock
     batch_tubes: bool = True  # advance the shock tubes as one work list
+    # a request that does not fit may preempt the active block request with
+    # the most steps left (more than the newcomer's): that request is written
+    # to a host restart part, its memory freed, and it resumes later from
+    # the part, bit for bit (snapy_amd.restart)
+    preempt: bool = False
     log: list = field(default_factory=list)
 
     def run(self, requests: list[Request]) -> list[Result]:
@@ -233,6 +241,7 @@ class Server:
         active: list[_Slot] = []
         used = 0
         results = []
+        parked = {}  # rid -> (restart part, steps done, first admission)
         t0 = time.perf_counter()
 
         def now():
@@ -248,14 +257,26 @@ class Server:
                 if need > self.budget_bytes:
                     raise RuntimeError(f"request {req.rid} needs {need} bytes, above the budget")
                 if used + need > self.budget_bytes:
-                    break
+                    victim = self._victim(active, req, used + need - self.budget_bytes) if self.preempt else None
+                    if victim is None:
+                        break
+                    used -= self._park(victim, parked, now())
+                    active.remove(victim)
+                    free_streams.append(victim.si)
+                    pending.insert(1, victim.req)
+                    self.log.append(("preempt", victim.req.rid, now(), used))
                 pending.pop(0)
                 si = free_streams.pop(0)
+                done0, admitted = 0, now()
                 with torch.cuda.stream(streams[si]):
-                    w = _initial_state(req.kind, block, req.params, req.seed)
-                    vars = _initialize(block, w)
+                    if req.rid in parked:
+                        part, done0, admitted = parked.pop(req.rid)
+                        vars, _ = block.initialize_from_restart(part)
+                    else:
+                        w = _initial_state(req.kind, block, req.params, req.seed)
+                        vars = _initialize(block, w)
                 used += need
-                sl = _Slot(req, block, vars, streams[si], need, now(), si=si)
+                sl = _Slot(req, block, vars, streams[si], need, admitted, done_steps=done0, si=si)
                 if self.batch_tubes and isinstance(block, MeshBlock) and eligible(block):
                     # the tube's steps run on the work list's stream, after
                     # its initialization
@@ -310,6 +331,32 @@ class Server:
             active = keep
         torch.cuda.synchronize()
         return results
+
+    @staticmethod
+    def _victim(active, req, short):
+        """The active block request with the most steps left, if it has more
+        left than `req` needs and freeing it covers the `short` bytes."""
+        best = None
+        for sl in active:
+            if not isinstance(sl.block, MeshBlock) or sl.bytes < short:
+                continue
+            left = sl.req.steps - sl.done_steps
+            if left > req.steps and (best is None or left > best.req.steps - best.done_steps):
+                best = sl
+        return best
+
+    @staticmethod
+    def _park(sl, parked, t):
+        """Write the slot's request to a host restart part and free its
+        block; returns the bytes freed."""
+        from .restart import restart_variables
+
+        sl.stream.synchronize()
+        part = restart_variables(sl.block, sl.vars, t)
+        parked[sl.req.rid] = (part, sl.done_steps, sl.admitted)
+        sl.block.release(sync=False)
+        sl.vars.clear()
+        return sl.bytes
 
 
 def summarize(results: list[Result], wall: float) -> dict:

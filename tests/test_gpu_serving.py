@@ -58,3 +58,24 @@ def test_tubes_batched_equal_their_own_launches():
     for r in reqs:
         assert a[r.rid].finite
         assert np.array_equal(a[r.rid].state, b[r.rid].state), r
+
+
+def test_preempted_requests_resume_bit_for_bit():
+    """A long tube is parked (written to a host restart part, its memory
+    freed) when a short request arrives that does not fit beside it; it
+    resumes from the part later and ends with the same bits as without the
+    preemption."""
+    long_req = serving.Request(0, "sod", {"n": 2000}, 3, 5000, arrival=0.0)
+    short_req = serving.Request(1, "sod", {"n": 600}, 4, 40, arrival=0.1)
+    other = serving.Request(2, "density_current", {"nx": 64, "nz": 16, "dT": -10.0}, 5, 30, arrival=0.12)
+    reqs = [long_req, short_req, other]
+    need = [serving.make_block(r).hbm_footprint() for r in reqs]
+    print("preemption footprints", need, flush=True)
+    srv = serving.Server(budget_bytes=need[0] + max(need[1:]) - 1, realtime=True, max_slots=4, preempt=True)
+    got = {r.rid: r for r in srv.run(reqs)}
+    assert any(e[0] == "preempt" and e[1] == 0 for e in srv.log), srv.log
+    ref = {r.rid: r for r in serving.Server(realtime=False, max_slots=4).run(reqs)}
+    for r in reqs:
+        print("request", r.rid, r.kind, flush=True)
+        assert got[r.rid].finite
+        assert np.array_equal(got[r.rid].state, ref[r.rid].state), r
