@@ -18,13 +18,14 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("entry", ["snapy_vic_full", "snapy_vic_full_serial", "snapy_vic_partial"])
-@pytest.mark.parametrize("shape", [(16, 6, 1), (12, 4, 3), (40, 37, 3), (400, 6, 1)])
+@pytest.mark.parametrize("shape", [(16, 6, 1), (12, 4, 3), (40, 37, 3), (400, 6, 1), (5, 9, 1)])
 def test_vic_kernel_matches_oracle(shape, entry):
     """One call on random columns against the oracle; the 400-layer columns
     take the finish kernel's fallback to fewer columns per workgroup (its
-    LDS holds 3 (n + 1) doubles per column)."""
-    if shape[0] > 100 and entry != "snapy_vic_partial":
-        pytest.skip("the tall column exercises the scheme-1 finish kernel's launch fallback")
+    LDS holds 3 (n + 1) doubles per column); the 5-layer columns (the
+    fewest scheme 1 takes) put both Woodbury rows on layer 2."""
+    if (shape[0] > 100 or shape[0] < 8) and entry != "snapy_vic_partial":
+        pytest.skip("the tall and the 5-layer columns exercise the scheme-1 kernels' edge cases")
     lib = _capi.load()
     g = fv.Grid(*shape, 0.0, 50.0 * shape[0], -400.0, 400.0, 0.0, 300.0)
     w = stratified(g, seed=5, noise=0.5)
