@@ -299,6 +299,52 @@ def fp32_accuracy(name, nsteps=3):
     return out
 
 
+def density_current_leg(steps=50, warmup=3):
+    """The density current deck (512 x 64, dx = dz = 100 m, LMARS, isotropic
+    diffusion, the vertical implicit scheme the deck states) in the FP32
+    state form, one block, captured stages: the throughput against its
+    B_alg (FP32 + compensation + FP64 implicit solve, 2 active directions:
+    260 B per cell-update)."""
+    import torch
+
+    from snapy_amd import problems
+
+    block = make_block("density_current.yaml", precision="float32")
+    vars, _ = block.initialize({"hydro_w": problems.density_current(block).cuda()})
+    for _ in range(warmup):
+        dt = block.device_time_step(vars)
+        for s in range(3):
+            block.forward(vars, dt, s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dt = block.device_time_step(vars)
+        for s in range(3):
+            block.forward(vars, dt, s)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    c = block.options.coord
+    cells = c.nx1 * c.nx2 * c.nx3
+    value = cells * 3 * steps / el
+    b_alg = 260
+    finite = bool(torch.isfinite(block.total_u(vars)).all())
+    out = {
+        "workload": f"density_current_{c.nx2}x{c.nx1}",
+        "value": value,
+        "unit": "cell-updates/s",
+        "steps": steps,
+        "ms_per_step": el / steps * 1e3,
+        "dtype": "f32",
+        "implicit_scheme": block.options.intg.implicit_scheme,
+        "form": "FP32 state form, diffusion, FP64 vertical implicit solve, captured stages",
+        "stage_b_alg_per_cell": b_alg,
+        "stage_frac": value * b_alg / (HBM_PEAK_GBS * 1e9),
+        "state_finite": finite,
+    }
+    block.release()
+    return out
+
+
 def cubed_leg(steps=50, warmup=3, members=32, seed0=1000):
     """The batched cubed-sphere shallow-water workload: `members` six-panel
     meshes of 192 x 192 cells per panel (FP64), each seeded with its own
@@ -540,6 +586,12 @@ def main():
     if not args.no_other_scheme and args.workload == "convection3d":
         other = other_scheme_leg(steps, warmup)
         print(f"# other scheme {other}", file=sys.stderr, flush=True)
+    c2 = None
+    if not args.no_other_scheme and args.workload == "convection3d":
+        print("# density current", file=sys.stderr, flush=True)
+        c2 = density_current_leg()
+        print(f"# density current {c2['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
     cubed = None
     if not args.no_cubed and args.workload == "convection3d":
         print("# batched cubed-sphere shallow water", file=sys.stderr, flush=True)
@@ -572,6 +624,8 @@ def main():
         line["cubed_sphere"] = cubed
     if other is not None:
         line["other_implicit_scheme"] = other
+    if c2 is not None:
+        line["density_current"] = c2
     if plain is not None:
         line["fp64_plain"] = plain
         line["speedup_vs_fp64_plain"] = value / plain["value"]
