@@ -20,7 +20,14 @@ Pinning (tests/test_oracle_*.py):
     tests/test_fp32_typed_constants.cpp:264-270;
   * divergence: telescoping sums and exact linear fluxes;
   * SSP-RK3: the stability polynomial 1 + z + z^2/2 + z^3/6;
-  * boundary functions and the CFL: brute-force loops.
+  * boundary functions and the CFL: brute-force loops;
+  * the outflow (radiating) ghost: linear acoustics on moving states (a
+    supersonic outflow copies the interior, a supersonic inflow keeps the
+    background, a subsonic face keeps exactly the leaving acoustic, entropy
+    and vortical waves), the mirror symmetry of its two faces and the
+    0.99 floor scaling;
+  * LayerHeat (integer depth): flux * area * dt of energy into the wall
+    cells, evenly, nothing elsewhere.
 """
 
 from __future__ import annotations
@@ -570,6 +577,17 @@ def max_time_step(grid, w, gammad, scheme=0, cfl=0.9, advection_cfl=1.0):
     return min(dt)
 
 
+def layer_heat(du, grid, dt, flux, depth, top=False):
+    """LayerHeat with an integer depth on a uniform x1 (layer_heat.cpp:78-85):
+    the energy rows of the `depth` wall cells (bottom: the first interior
+    cells; top: the last) gain dt * flux / (dx1 * depth), in place."""
+    il = grid.nghost
+    iu = grid.nghost + grid.nx1 - 1
+    i0 = iu + 1 - depth if top else il
+    du[IPR][..., i0 : i0 + depth] += dt * flux / (grid.dx(1) * depth)
+    return du
+
+
 # SSP-RK3 (harp "rk3"): u <- w0*u0 + w1*u + w2*du per stage; the weights
 # wght2 = {1, 1/4, 2/3} are pinned by tests/test_forcing.cpp:1317-1321
 RK3_WEIGHTS = ((1.0, 0.0, 1.0), (0.75, 0.25, 0.25), (1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0))
@@ -682,9 +700,7 @@ class Block:
             diffusion.add_momentum(du, fdiff, g, dt)
         if d.grav1 != 0.0:
             if d.bot_heat_flux != 0.0:
-                # LayerHeat bot-heat (layer_heat.cpp:78-85)
-                il = g.nghost
-                du[IPR][..., il : il + d.bot_heat_depth] += dt * d.bot_heat_flux / (g.dx(1) * d.bot_heat_depth)
+                layer_heat(du, g, dt, d.bot_heat_flux, d.bot_heat_depth)
             if 1 in flux:
                 gec = gravity.gravity_energy_correction(g, self.w, flux[1], dt, d.grav1)
                 du[IPR][..., g.nghost : g.nghost + g.nx1] += gec

@@ -19,8 +19,10 @@ discrete hydrostatic identity psf_lo - psf_hi = g rho dx, the six-face cell
 reference against the closed form of an isothermal column, the rest state
 (zero flux divergence plus source at hydrostatic equilibrium: the
 well-balanced property the reference asserts, tests/test_fp32_policy.py:65),
-the slope stencil against exact derivatives of quadratics, and the face work
-against its telescoping sum.
+the slope stencil against exact derivatives of quadratics, the face work
+against its telescoping sum, and the face work plus the corrected
+potential-energy term against the exact gravity work of a cubic mass flux
+(Gauss quadrature).
 """
 
 from __future__ import annotations

@@ -24,7 +24,14 @@ The column arrays carry one ghost row at each end: w has shape
 tests/test_oracle_implicit.py solves the same assembled system densely
 (numpy.linalg.solve on the (5n)x(5n) matrix, Woodbury points included),
 checks mass conservation of the redistribution, the rest state (zero du in,
-zero du out), and the small-dt limit delta -> du.
+zero du out), and the small-dt limit delta -> du.  The assembly pieces are
+pinned by tests/test_oracle_vic_assembly.py from the mathematics of the
+Euler equations: the flux Jacobian against finite differences of the flux,
+the Roe eigen-system (R R^-1 = I, R diag(lambda) R^-1 = A, Roe's property),
+|A| (|A|^2 = A^2, |A| r_k = |lambda_k| r_k), the assembled a, b, c as the
+exact linearisation of the Roe-type residual with the gravity source, the
+face gravity work and the wall mirrors, and the corrected-PE rows as the
+slope operator.
 """
 
 from __future__ import annotations

@@ -54,20 +54,41 @@ def sources() -> list[Path]:
     return sorted(CSRC.glob("*.hip"))
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile every kernel source into LIB; raises on a compiler error."""
-    srcs = [str(s) for s in sources()]
-    cmd = [HIPCC, *FLAGS, "-I", str(ROOT / "include"), *srcs, "-o", str(LIB) + ".tmp"]
-    tl = _torch_lib_dir()
-    if tl:
-        cmd += [f"-Wl,-rpath,{tl}"]
-    cmd += ["-Wl,-rpath,/opt/rocm/lib"]
+def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError(f"hipcc failed with status {res.returncode}")
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    """Compile every kernel source into LIB; raises on a compiler error.
+
+    Each source compiles to its own object in a private temporary folder
+    (in parallel, at most ``jobs`` at once), then one link makes the
+    library.  Every source is recompiled on each call."""
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
+
+    srcs = sources()
+    if jobs is None:
+        jobs = min(len(srcs), max(1, min(16, len(os.sched_getaffinity(0)))))
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory(prefix="snapy_amd_build_") as tmp:
+        objs = [str(Path(tmp) / (s.stem + ".o")) for s in srcs]
+        cmds = [[HIPCC, *compile_flags, "-I", str(ROOT / "include"), "-c", str(s), "-o", o]
+                for s, o in zip(srcs, objs)]
+        with ThreadPoolExecutor(max_workers=jobs) as ex:
+            for f in [ex.submit(_run, c, verbose) for c in cmds]:
+                f.result()
+        link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(LIB) + ".tmp"]
+        tl = _torch_lib_dir()
+        if tl:
+            link += [f"-Wl,-rpath,{tl}"]
+        link += ["-Wl,-rpath,/opt/rocm/lib"]
+        _run(link, verbose)
     os.replace(str(LIB) + ".tmp", LIB)
     return LIB
 

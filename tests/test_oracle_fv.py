@@ -270,3 +270,143 @@ def test_flux_covariance_is_the_face_average_of_linear_profiles(dim):
     assert np.allclose(got, np.broadcast_to(expected, got.shape), rtol=1e-9, atol=0.0)
     # and the closed form
     assert np.allclose(got, dx1 * dx1 / 12.0 * rho0 * (k * a / rho0) * b, rtol=1e-9, atol=0.0)
+
+
+# ---------------------------------------------------------------------------
+# the outflow (radiating) ghost on moving states: linear acoustics fixes it.
+# About the background (rho0, p0, c0) a perturbation d splits into the
+# outgoing/incoming acoustic waves W+- = s*dv_n +- dp/(rho0 c0), the entropy
+# wave S = drho - dp/c0^2 and the tangential (vortical) waves; the ghost
+# keeps the waves that leave the domain and drops the ones that enter.
+# ---------------------------------------------------------------------------
+def _radiate(wi, outer, gamma=1.4, rho0=1.1, p0=0.9, floor=1e-10):
+    g = fv.Grid(10, 1, 1)
+    b = np.zeros((5,) + g.shape)
+    b[fv.IDN] = rho0
+    b[fv.IPR] = p0
+    w = b.copy()
+    w[:, 0, 0, 3:-3] = np.asarray(wi)[:, None]
+    fv.radiating(w, 3, b, gamma, floor, floor, outer)
+    ghost = w[:, 0, 0, -3:] if outer else w[:, 0, 0, :3]
+    return ghost, b[:, 0, 0, 0], np.sqrt(gamma * p0 / rho0)
+
+
+def _waves(d, sign, rho0, c0):
+    wp = sign * d[fv.IVX] + d[fv.IPR] / (rho0 * c0)
+    wm = sign * d[fv.IVX] - d[fv.IPR] / (rho0 * c0)
+    s = d[fv.IDN] - d[fv.IPR] / (c0 * c0)
+    return wp, wm, s
+
+
+@pytest.mark.parametrize("outer", [True, False])
+def test_outflow_supersonic_outflow_copies_the_interior(outer):
+    sign = 1.0 if outer else -1.0
+    c0 = np.sqrt(1.4 * 0.9 / 1.1)
+    wi = [1.13, sign * 3.0 * c0, 0.21, -0.17, 0.95]
+    ghost, _, _ = _radiate(wi, outer)
+    np.testing.assert_allclose(ghost, np.broadcast_to(np.array(wi)[:, None], ghost.shape), rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("outer", [True, False])
+def test_outflow_supersonic_inflow_keeps_the_background(outer):
+    sign = 1.0 if outer else -1.0
+    c0 = np.sqrt(1.4 * 0.9 / 1.1)
+    wi = [1.13, -sign * 3.0 * c0, 0.21, -0.17, 0.95]
+    ghost, bg, _ = _radiate(wi, outer)
+    np.testing.assert_array_equal(ghost, np.broadcast_to(bg[:, None], ghost.shape))
+
+
+@pytest.mark.parametrize("outer", [True, False])
+@pytest.mark.parametrize("mach", [0.4, -0.4])
+def test_outflow_subsonic_keeps_only_the_leaving_waves(outer, mach):
+    sign = 1.0 if outer else -1.0
+    rho0, p0 = 1.1, 0.9
+    c0 = np.sqrt(1.4 * p0 / rho0)
+    wi = np.array([1.13, sign * mach * c0, 0.21, -0.17, 0.95])
+    ghost, bg, _ = _radiate(wi, outer, rho0=rho0, p0=p0)
+    d_int = wi - bg
+    d_gh = ghost - bg[:, None]
+    wp_i, wm_i, s_i = _waves(d_int, sign, rho0, c0)
+    wp_g, wm_g, s_g = _waves(d_gh, sign, rho0, c0)
+    # the wave moving at u_n + c leaves at either sign of a subsonic u_n
+    np.testing.assert_allclose(wp_g, wp_i, rtol=1e-12, atol=1e-14)
+    # the wave moving at u_n - c enters
+    np.testing.assert_allclose(wm_g, 0.0, atol=1e-14)
+    if mach > 0:
+        # entropy and vorticity are carried out with the flow
+        np.testing.assert_allclose(s_g, s_i, rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(d_gh[fv.IVY], d_int[fv.IVY], rtol=1e-13)
+        np.testing.assert_allclose(d_gh[fv.IVZ], d_int[fv.IVZ], rtol=1e-13)
+    else:
+        # an inflow brings the background's entropy and tangential motion
+        np.testing.assert_allclose(s_g, 0.0, atol=1e-14)
+        np.testing.assert_array_equal(d_gh[fv.IVY], 0.0)
+        np.testing.assert_array_equal(d_gh[fv.IVZ], 0.0)
+
+
+def test_outflow_inner_face_mirrors_the_outer_face():
+    """Reversing x1 and the normal velocity swaps the two faces."""
+    g = fv.Grid(10, 1, 1)
+    rng = np.random.default_rng(12)
+    b = np.zeros((5,) + g.shape)
+    b[fv.IDN] = 1.0 + 0.2 * rng.random(g.shape)
+    b[fv.IPR] = 0.8 + 0.2 * rng.random(g.shape)
+    w = b + 0.05 * (rng.random((5,) + g.shape) - 0.5)
+    flip = lambda q: np.concatenate([q[:1], -q[1:2], q[2:]])[..., ::-1].copy()  # noqa: E731
+    a = w.copy()
+    fv.radiating(a, 3, b, 1.4, 1e-10, 1e-10, True)
+    m = flip(w)
+    fv.radiating(m, 3, flip(b), 1.4, 1e-10, 1e-10, False)
+    np.testing.assert_allclose(flip(m)[..., -3:], a[..., -3:], rtol=1e-14, atol=1e-15)
+
+
+def test_outflow_ghost_respects_the_floors():
+    """An outgoing expansion deep enough to drive the ghost's pressure
+    below the floor is scaled back to 0.99 of the distance to it."""
+    rho0, p0, floor = 1.0, 1.0, 1e-3
+    c0 = np.sqrt(1.4 * p0 / rho0)
+    wi = [1.0, 3.0 * c0, 0.0, 0.0, 0.5]
+    g = fv.Grid(10, 1, 1)
+    b = np.zeros((5,) + g.shape)
+    b[fv.IDN] = rho0
+    b[fv.IPR] = p0
+    w = b.copy()
+    w[:, 0, 0, 3:-3] = np.array(wi)[:, None]
+    w[fv.IPR][..., 3:-3] = 0.5  # the interior itself stays a valid gas
+    # the perturbation d = w - b of the source cell is kept whole (all
+    # waves leave); a thinner background in the ghosts makes bg + d fall
+    # below the floor there
+    b2 = b.copy()
+    b2[fv.IPR][..., -3:] = 0.4  # a thinner background in the ghosts
+    fv.radiating(w, 3, b2, 1.4, floor, floor, True)
+    ghost = w[:, 0, 0, -3:]
+    dp = 0.5 - p0
+    alpha = 0.99 * (0.4 - floor) / -dp
+    np.testing.assert_allclose(ghost[fv.IPR], 0.4 + alpha * dp, rtol=1e-13)
+    assert (ghost[fv.IPR] > floor).all()
+    # one factor for every characteristic: the normal velocity too
+    np.testing.assert_allclose(ghost[fv.IVX], alpha * 3.0 * c0, rtol=1e-13)
+
+
+def test_layer_heat_adds_flux_times_area_of_energy_to_the_wall_cells():
+    """bot-heat (top-cool) with an integer depth puts flux * A * dt of energy
+    into the bottom (top) `depth` cells of every column, evenly, and nothing
+    elsewhere (layer_heat.cpp:78-85)."""
+    g = fv.Grid(12, 3, 2, 0.0, 600.0, 0.0, 300.0, 0.0, 200.0)
+    area = g.dx(2) * g.dx(3)
+    vol = area * g.dx(1)
+    for depth in (1, 2, 5):
+        for top in (False, True):
+            du = np.zeros((5,) + g.shape)
+            fv.layer_heat(du, g, 0.7, 100.0, depth, top)
+            e = du[fv.IPR][g.interior()] * vol
+            np.testing.assert_allclose(e.sum(-1), 0.7 * 100.0 * area, rtol=1e-14)
+            cells = np.arange(g.nx1)
+            hot = cells >= g.nx1 - depth if top else cells < depth
+            assert (e[..., hot] > 0).all() and not e[..., ~hot].any()
+            np.testing.assert_allclose(e[..., hot], 0.7 * 100.0 * area / depth, rtol=1e-14)
+            # nothing in other rows, nor in the x1 ghosts (the x2/x3 ghost
+            # columns take the same heating, as the reference's narrow along
+            # x1 does; their du is never read)
+            assert not du[[fv.IDN, fv.IVX, fv.IVY, fv.IVZ]].any()
+            assert not du[fv.IPR][..., : g.nghost].any() and not du[fv.IPR][..., g.nghost + g.nx1 :].any()
