@@ -240,8 +240,13 @@ int snapy_update_gravity(const snapy_grid *g, const snapy_physics *p,
  * work weights lo/hi, corrected-PE bands lo/mid/hi, cell potential,
  * face-minus-cell potentials lo/hi); t0, t1 the wall third points; x1tab as
  * for snapy_update_gravity.  ws: snapy_vic_workspace(g) doubles; status:
- * nx2*nx3 ints, 0 = solved, 1 = singular (that column keeps its explicit
- * increment).
+ * nx2*nx3 ints, one word per column: bit 0 is set when this call's solve of
+ * the column failed (singular pivot or a non-finite solution; the column
+ * keeps its explicit increment) and cleared when it succeeded; bit 1 is set
+ * with bit 0 and kept by later calls, a step-level latch the caller clears
+ * at the step's first stage (the reference ORs every stage's bits into the
+ * step status, src/implicit/implicit_hydro.cpp:610-617, and resets it at
+ * step start, src/mesh/meshblock_advance.cpp:186-188).
  * Replaces: vic_stage assemble/solve/redistribute (src/implicit/
  * implicit_dispatch.hpp:50-57, src/implicit/vic_kernels.h:66-297) and
  * ImplicitHydroImpl::forward_masked (src/implicit/implicit_hydro.cpp:

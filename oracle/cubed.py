@@ -779,8 +779,18 @@ class Mesh:
             du[sub] += src[sub]
             self.u[p] = fv.rk_combine(self.u0[p], self.u[p], du, stage)
         exchange_ghosts(g, self.u, conserved=True)
+        self._w_stale = True
+
+    def check_redo(self):
+        """w = prim(u) of the accepted state on every panel
+        (meshblock_redo.cpp:107-112)."""
+        self.w = [cons2prim(u, self.grid) for u in self.u]
+        self._w_stale = False
+        return 0
 
     def max_time_step(self):
+        if getattr(self, "_w_stale", False):
+            self.check_redo()
         return self.cfl * min(max_time_step(self.grid, w) for w in self.w)
 
     def mass(self):

@@ -593,6 +593,14 @@ def layer_heat(du, grid, dt, flux, depth, top=False):
 RK3_WEIGHTS = ((1.0, 0.0, 1.0), (0.75, 0.25, 0.25), (1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0))
 
 
+def step_field_fold(S, T, w1, w2, dt):
+    """FluxRegister::end_stage (flux_register.cpp:761-777): the FP64 step
+    field S <- w1 S + w2 dt T, as S.mul_(w1) then the axpy with
+    w2dt = dt.scaled(w2), the stage total T widened to FP64."""
+    w2dt = dt * w2
+    return S * w1 + np.asarray(T, dtype=np.float64) * w2dt
+
+
 def rk_combine(u0, u, du, stage):
     w0, w1, w2 = RK3_WEIGHTS[stage]
     return w0 * u0 + w1 * u + w2 * du
@@ -645,6 +653,7 @@ class Block:
             self.bref = w.copy()
         apply_boundaries(w, d.bcs, d.gammad, self.bref, primitive=True, dfloor=d.dfloor, pfloor=d.pfloor)
         self.w = w
+        self._w_stale = False
         self.u = prim2cons(w, d.gammad)
         return self.u
 
@@ -740,9 +749,23 @@ class Block:
         du = self.tendency(dt)
         self.u = rk_combine(self.u0, self.u, du, stage)
         apply_boundaries(self.u, d.bcs, d.gammad, self.bref, dfloor=d.dfloor, pfloor=d.pfloor)
+        self._w_stale = True
+
+    def check_redo(self):
+        """The primitives of the accepted state, w = prim(u) on every cell
+        (MeshBlockImpl::local_redo_flag_tensor, meshblock_redo.cpp:107-112);
+        the oracle models no retry causes."""
+        self.w = cons2prim(self.u, self.deck.gammad)
+        self._w_stale = False
+        return 0
 
     def max_time_step(self):
+        """From prim(u): the driver's check_redo refreshes the primitives
+        after every step (examples/shock.cpp:60-72); a step not followed by
+        one refreshes them here."""
         d = self.deck
+        if getattr(self, "_w_stale", False):
+            self.check_redo()
         dt = max_time_step(d.grid, self.w, d.gammad, d.implicit_scheme, d.cfl)
         if d.nu_iso > 0.0 or d.kappa_iso > 0.0:
             # the sources' bound joins before the CFL factor (hydro.cpp:396)

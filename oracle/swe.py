@@ -176,8 +176,17 @@ class Block:
         du = self.tendency(dt)
         self.u = fv.rk_combine(self.u0, self.u, du, stage)
         fv.apply_boundaries(self.u, self.deck.bcs, 1.4)
+        self._w_stale = True
+
+    def check_redo(self):
+        """w = prim(u) of the accepted state (meshblock_redo.cpp:107-112)."""
+        self.w = cons2prim(self.u)
+        self._w_stale = False
+        return 0
 
     def max_time_step(self):
-        """From the primitives of the last stage's start, as the block's
-        vars['hydro_w'] (meshblock_advance.cpp:74-85)."""
+        """From the primitives of the accepted state, as the block's
+        vars['hydro_w'] after check_redo (meshblock_advance.cpp:74-85)."""
+        if getattr(self, "_w_stale", False):
+            self.check_redo()
         return self.deck.cfl * max_time_step(self.deck.grid, self.w)

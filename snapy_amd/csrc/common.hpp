@@ -63,6 +63,19 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
   return b < q * kXcds ? (b % kXcds) * q + b / kXcds : b;
 }
 
+// The vertical implicit solve's column status word.  Bit 0: this call's
+// solve of the column failed (the column keeps its explicit increment; the
+// finish kernels read it).  Bit 1: a call since the caller last cleared the
+// word failed (a step-level latch: the host clears the words at a step's
+// first stage and check_redo reads them, as the reference ORs each stage's
+// bits into its step status, src/implicit/implicit_hydro.cpp:610-617).
+constexpr int kVicStageFailed = 1;
+constexpr int kVicStepFailed = 2;
+__device__ __forceinline__ void vic_mark(int* status, int64_t col, bool ok) {
+  int const s = status[col];
+  status[col] = ok ? (s & kVicStepFailed) : (kVicStageFailed | kVicStepFailed);
+}
+
 // the shallow-water stage (swe.hip), dispatched from the generic entry
 // points when p->eos is SNAPY_EOS_SHALLOW_WATER
 int swe_cons2prim(const snapy_grid* g, int dtype, const void* u, void* w,

@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(64)
   if (!ok) {
     // a failed column keeps its explicit increments (the reference undoes
     // the column with its status bits and flags a retry)
-    status[col] = 1;
+    vic_mark(status, col, false);
     for (int i = 0; i < n; ++i)
       for (int v = 0; v < 5; ++v) {
         int64_t o = v * r.sv + row + i;
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(64)
       }
     return;
   }
-  status[col] = 0;
+  vic_mark(status, col, true);
   // ---- back substitution (vic_backward_substitute) ------------------------
   for (int i = n - 2; i >= 0; --i)
     for (int p = 0; p < 5; ++p) {
@@ -785,7 +785,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
   int okv = ok ? 1 : 0;
 #pragma unroll
   for (int m = 0; m < 5; ++m) okv &= __shfl(okv, base + m);
-  if (p == 0 && L.live) status[L.col] = okv ? 0 : 1;
+  if (p == 0 && L.live) vic_mark(status, L.col, okv != 0);
   WS(n - 1, 7) = z1v;  // the top-wall column, zero below the top layer
 }
 
@@ -829,7 +829,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
   auto WS = [&](int l, int q) -> double& { return wsl[(l * kQ + q) * 64]; };
   auto WSB = [&](int l, int q) -> double& { return wsb[(l * kQ + q) * 64]; };
   auto tab = [&](double const* t, int rowi, int i) { return t[rowi * nc1 + lo1 + i]; };
-  bool const failed = status[L.col] != 0;
+  bool const failed = (status[L.col] & kVicStageFailed) != 0;
   constexpr int B = 4;  // layers per batch of workspace loads
 
   // ---- back substitution (vic_backward_substitute) ------------------------

@@ -596,7 +596,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
     // a failed column keeps its explicit increments (the finish kernel
     // reads the flag; the reference undoes the column with its status bits
     // and flags a retry)
-    if (live) status[col] = ok ? 0 : 1;
+    if (live) vic_mark(status, col, ok);
     s_act[lane] = ok;
   }
 
@@ -752,7 +752,7 @@ __global__ void __launch_bounds__(kFinThreads)
     int j = r.lo2 + int(cl % r.n2), k = r.lo3 + int(cl / r.n2);
     s_row[tid] = k * r.s3 + j * r.s2 + r.lo1;
     s_in[tid] = in;
-    s_ok[tid] = in && status[col] == 0;
+    s_ok[tid] = in && (status[col] & kVicStageFailed) == 0;
   }
   __syncthreads();
   int const cells = nc * n;

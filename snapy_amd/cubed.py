@@ -462,9 +462,11 @@ class CubedSphereMesh:
         self._state = _MeshState()
         self._dt_dev = None
         self._executor = _MeshExecutor(self, op.intg.execution)
-        self.cycle = 0
+        self._cycle = 0
         self.current_redo = 0
         self.max_redo = 5
+        self.density_floor = op.hydro.eos.density_floor
+        self._w_stale = False
 
     # -- device set-up ------------------------------------------------------
     def to(self, device):
@@ -516,6 +518,14 @@ class CubedSphereMesh:
         return _capi.stream_of(self.device)
 
     # -- the reference's Mesh surface ---------------------------------------
+    def cycle(self) -> int:
+        return self._cycle
+
+    def inc_cycle(self) -> int:
+        v = self._cycle
+        self._cycle += 1
+        return v
+
     def block_shape(self):
         return (NROWS, self.nc, self.nc, 1)
 
@@ -551,6 +561,7 @@ class CubedSphereMesh:
         for b, v in enumerate(vars):
             v["hydro_w"] = self._w[b]
             v["hydro_u"] = self._u[b]
+        self._w_stale = False
         return vars, 0.0
 
     def _stage_calls(self, stage: int):
@@ -582,10 +593,22 @@ class CubedSphereMesh:
         else:
             self._dt_dev.fill_(float(dt))
         self._executor.run_stage(vars, stage)
+        self._w_stale = True
+
+    def _refresh_primitives(self) -> None:
+        """w <- prim(u) on every cell of every panel (the reference's
+        check_redo refreshes the primitives, meshblock_redo.cpp:107-112)."""
+        lib = _capi.load()
+        _capi.check(lib.snapy_cs_cons2prim(ctypes.byref(self._desc), _capi.ptr(self._u), _capi.ptr(self._w),
+                                           self._stream()), "cs_cons2prim")
+        self._w_stale = False
 
     def device_time_step(self, vars=None) -> torch.Tensor:
         """The CFL dt of each member on the device (2^-redo * cfl * min over
-        its six panels), with no host read."""
+        its six panels), with no host read; from the primitives of the
+        current state."""
+        if self._w_stale:
+            self._refresh_primitives()
         lib = _capi.load()
         scale = math.pow(2.0, -self.current_redo) * self.cfl
         _capi.check(lib.snapy_cs_cfl(ctypes.byref(self._desc), _capi.ptr(self._w), scale,
@@ -599,17 +622,23 @@ class CubedSphereMesh:
         return dt[0] if self.members == 1 else dt
 
     def check_redo(self, vars=None) -> int:
-        """>0 redo (a non-finite interior: restore u0, halve dt), 0 ok, <0
-        stop."""
+        """>0 redo, 0 ok, <0 stop (meshblock_redo.cpp:32-89): the primitives
+        are refreshed from u; a non-finite interior or a geopotential at or
+        within 0.1% of the floor asks for a redo, which restores u0 (and
+        its primitives) and halves dt; past max_redo the mesh stops."""
+        self._refresh_primitives()
         i = slice(self.ng, self.ng + self.n)
-        bad = bool((~torch.isfinite(self._u[:, :, i, i])).any().item())
-        if not bad:
+        nan = ~torch.isfinite(self._u[:, :, i, i]).all()
+        floor = ~(self._w[:, IDN, i, i].min() > 1.001 * self.density_floor)
+        flags = torch.stack([floor, nan]).cpu().tolist()
+        if not any(flags):
             self.current_redo = 0
             return 0
-        if self.current_redo >= self.max_redo:
+        self.current_redo += 1
+        if self.current_redo > self.max_redo:
             return -1
         self._u.copy_(self._u0)
-        self.current_redo += 1
+        self._refresh_primitives()
         return 1
 
     def mass(self) -> torch.Tensor:

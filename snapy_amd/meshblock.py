@@ -80,7 +80,8 @@ class _Buffers:
 class MeshBlock:
     def __init__(self, options: MeshBlockOptions):
         self.options = options
-        self.cycle = 0
+        self._cycle = 0
+        self.last_redo_causes = 0
         self.intg = Integrator(options.intg)
         self.pintg = self.intg
         self.policy = PrecisionPolicy(options.hydro.precision)
@@ -92,6 +93,10 @@ class MeshBlock:
         self._vic = None
         self._scratch_w = None
         self._dt_dev = None
+        # hydro_w holds prim(hydro_u) (False) or the primitives of the last
+        # stage's input (True, after forward until check_redo or a time step
+        # refreshes them)
+        self._w_stale = False
         self._executor = StageExecutor(self, options.intg.execution)
         c = options.coord
         # ideal gas: (rho, m1, m2, m3, E); shallow water: (phi, phi v1..v3)
@@ -150,6 +155,17 @@ class MeshBlock:
             bufs[f"x{d}v"] = 0.5 * (f[:-1] + f[1:])
             bufs[f"dx{d}f"] = torch.ones(nx + 2 * ng, dtype=torch.float64) * dx
         self._coord = _Buffers(**bufs)
+
+    def cycle(self) -> int:
+        """The accepted-step count (pymesh.cpp:161)."""
+        return self._cycle
+
+    def inc_cycle(self) -> int:
+        """Advance the cycle after an accepted step; returns the cycle before
+        the increment (pymesh.cpp:162-167)."""
+        v = self._cycle
+        self._cycle += 1
+        return v
 
     def module(self, name: str):
         if name == "coord":
@@ -329,6 +345,7 @@ class MeshBlock:
             )
             vars["hydro_u"] = u
             self._state.adopt(u, comp)
+            self._w_stale = False
             return vars, 0.0
         u = torch.empty_like(w)
         _capi.check(
@@ -339,6 +356,7 @@ class MeshBlock:
         )
         vars["hydro_u"] = u
         self._state.adopt(u)
+        self._w_stale = False
         return vars, 0.0
 
     def write_restart(self, vars, path: str, time: float) -> str:
@@ -388,7 +406,8 @@ class MeshBlock:
             u = data["hydro_u"].to(device=dev, dtype=torch.float64).contiguous()
             vars["hydro_u"] = u
             self._state.adopt(u)
-        self.cycle = int(data["last_cycle"][0])
+        self._cycle = int(data["last_cycle"][0])
+        self._w_stale = False
         return vars, float(data["last_time"][0])
 
     def _adopt(self, vars):
@@ -410,6 +429,29 @@ class MeshBlock:
         else:
             self._dt_dev.fill_(float(dt))
         self._executor.run_stage(vars, stage)
+        self._w_stale = True
+
+    def _refresh_primitives(self, vars) -> None:
+        """hydro_w <- prim(hydro_u) on every cell, on the current stream
+        (the reference's check_redo does it, meshblock_redo.cpp:107-112, so
+        that the next time step reads the accepted state's primitives).  The
+        FP64 form converts the boundary-filled state; the FP32 pair form
+        converts u + c and fills the ghosts on the primitives, as its stages
+        do."""
+        lib = _capi.load()
+        g, p = ctypes.byref(self._grid), ctypes.byref(self._phys)
+        s = _capi.stream_of(self.device)
+        P = _capi.ptr
+        u, w = vars["hydro_u"], vars["hydro_w"]
+        if self.policy.fp32():
+            _capi.check(lib.snapy_cons2prim_f32(g, p, P(u), P(self._state.comp()), P(w), s), "cons2prim")
+            _capi.check(
+                lib.snapy_apply_boundaries_prim(g, p, 0, P(w), P(vars.get("boundary_reference_w")), s),
+                "apply_boundaries(prim)",
+            )
+        else:
+            _capi.check(lib.snapy_cons2prim(g, p, _capi.dtype_code(u.dtype), P(u), P(w), s), "cons2prim")
+        self._w_stale = False
 
     def _launch_stage(self, vars, stage: int) -> None:
         """The stage's launches on the current stream (the body a hipGraph
@@ -441,6 +483,11 @@ class MeshBlock:
         calls = []
         c, c0 = self._state.comp(), self._state.comp0()
         bref = P(vars.get("boundary_reference_w"))
+        if stage == 0 and self._vic is not None:
+            # the implicit solve's step-level failure latch starts clear (a
+            # fill kernel, so the clear replays inside a captured stage)
+            st_words = self._vic["status"]
+            calls.append(("vic_status_clear", lambda: (st_words.zero_(), 0)[1]))
         # (1) primitives
         if pair:
             # the FP32 pair form fills the ghosts on the FP64 primitives (a
@@ -600,6 +647,8 @@ class MeshBlock:
         """max_time_step without the host read: the block's device dt
         (2^-redo * cfl * min dx/(|v|+c)), for a driver that never syncs."""
         self._ensure_buffers()
+        if self._w_stale:
+            self._refresh_primitives(vars)
         lib = _capi.load()
         w = vars["hydro_w"]
         scale = math.pow(2.0, -self.intg.current_redo) * self.intg.options.cfl
@@ -617,6 +666,8 @@ class MeshBlock:
         (meshblock_advance.cpp:61-85): 2^-redo * cfl * min dx/(|v|+c) over
         the interior of vars['hydro_w'] (one host read)."""
         self._ensure_buffers()
+        if self._w_stale:
+            self._refresh_primitives(vars)
         lib = _capi.load()
         w = vars["hydro_w"]
         out = torch.empty((), dtype=torch.float64, device=self.device)
@@ -630,22 +681,59 @@ class MeshBlock:
         dt = float(out.item())
         return math.pow(2.0, -self.intg.current_redo) * self.intg.options.cfl * dt
 
-    def check_redo(self, vars) -> int:
-        """>0 redo, 0 ok, <0 stop (meshblock.hpp:311): a non-finite state
-        restores u0 and halves dt through current_redo."""
-        u = vars["hydro_u"]
-        bad = bool((~torch.isfinite(u[self.part(exterior=False)])).any().item())
+    # retry causes, the reference's bits (meshblock_redo.cpp:44-49)
+    REDO_FLOOR, REDO_NAN, REDO_VIC = 1, 8, 32
+
+    def redo_causes(self, vars) -> int:
+        """The step's retry causes from one host read
+        (MeshBlockImpl::local_redo_flag_tensor, meshblock_redo.cpp:98-160):
+        density (and pressure) of hydro_w at or within 0.1% of the floor or
+        not a number (1), a non-finite conserved interior (8), a column whose
+        implicit solve failed in a stage of this step (32).  Reads hydro_w
+        as it stands (check_redo refreshes it first)."""
+        inner = self.part(exterior=False)
+        w = vars["hydro_w"][inner]
+        eos = self.options.hydro.eos
+        floor = ~(w[IDN].min() > 1.001 * eos.density_floor)
+        if self.nvar > IPR:
+            floor = floor | ~(w[IPR].min() > 1.001 * eos.pressure_floor)
+        u = vars["hydro_u"][inner]
+        nan = ~torch.isfinite(u).all()
+        c = self._state.comp()
+        if c is not None:
+            nan = nan | ~torch.isfinite(c[inner]).all()
+        vic = torch.zeros((), dtype=torch.bool, device=self.device)
         if self._vic is not None:
-            # a column whose implicit solve failed asks for a redo
-            # (implicit.column retry flag, implicit_hydro.cpp:476-478)
-            bad = bad or bool(self._vic["status"].any().item())
-        if not bad:
+            vic = (self._vic["status"] & 2).any()
+        flags = torch.stack([floor, nan, vic]).cpu().tolist()
+        return (self.REDO_FLOOR if flags[0] else 0) | (self.REDO_NAN if flags[1] else 0) | (
+            self.REDO_VIC if flags[2] else 0
+        )
+
+    def check_redo(self, vars) -> int:
+        """>0 redo, 0 ok, <0 stop (meshblock.hpp:311; check_redo ->
+        decide_retry -> apply_redo, meshblock_redo.cpp:32-89,170-206).
+        hydro_w is first refreshed from hydro_u.  On a retry cause the redo
+        count goes up; past max_redo the block stops, at once for a floor or
+        NaN cause and after restoring the state for a failed implicit
+        solve.  A redo restores u (and the pair's compensation) from the
+        step's backup and recomputes hydro_w from it, so the next time step
+        (2^-redo of the CFL dt) reads the restored state."""
+        self._adopt(vars)
+        self._ensure_buffers()
+        self._refresh_primitives(vars)
+        causes = self.redo_causes(vars)
+        self.last_redo_causes = causes
+        if not causes:
             self.intg.current_redo = 0
             return 0
-        if self.intg.current_redo >= self.intg.max_redo:
+        self.intg.current_redo += 1
+        if self.intg.current_redo > self.intg.max_redo and not (causes & self.REDO_VIC):
             return -1
         self._state.restore()
-        self.intg.current_redo += 1
+        self._refresh_primitives(vars)
+        if self.intg.current_redo > self.intg.max_redo:
+            return -1
         return 1
 
     def total_u(self, vars):

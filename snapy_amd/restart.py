@@ -48,7 +48,11 @@ def grid_faces(block) -> tuple:
 
 
 def restart_variables(block, vars, time: float) -> dict:
-    """The part's tensors (on the CPU) for `block` at `time`."""
+    """The part's tensors (on the CPU) for `block` at `time`.  The
+    primitives saved are those of the saved state (refreshed first when the
+    block has stepped since, as the reference's check_redo leaves them)."""
+    if getattr(block, "_w_stale", False):
+        block._refresh_primitives(vars)
     out = {}
     for k in ("hydro_w", "boundary_reference_w"):
         if vars.get(k) is not None:
@@ -64,7 +68,7 @@ def restart_variables(block, vars, time: float) -> dict:
     for key, f in zip(REF_GRID, grid_faces(block)):
         out[key] = f
     out["last_time"] = torch.tensor([float(time)], dtype=torch.float64)
-    out["last_cycle"] = torch.tensor([int(block.cycle)], dtype=torch.int64)
+    out["last_cycle"] = torch.tensor([int(block.cycle())], dtype=torch.int64)
     return out
 
 

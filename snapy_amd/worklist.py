@@ -119,8 +119,10 @@ class BlockBatch:
         self.cells = sum(int(blk.options.coord.nx1) for blk, _ in self.members)
 
     def launches_per_step(self) -> int:
-        """Kernel launches of one RK3 step of the whole batch."""
-        return 1 + 3 * 4 + 1
+        """Kernel launches of one RK3 step of the whole batch (after its
+        first step: the primitives' refresh, the CFL, then per stage
+        cons2prim, sweep, update and boundaries, and the stage-0 backup)."""
+        return 1 + 1 + 3 * 4 + 1
 
     def step(self) -> None:
         """One RK3 step of every block on the current stream: the device CFL
@@ -128,7 +130,19 @@ class BlockBatch:
         lib = _capi.load()
         wl, p = ctypes.byref(self._wl), ctypes.byref(self._phys)
         s = _capi.stream_of(self.device)
+        # the CFL reads prim(u) of each block's current state, as a block
+        # alone does (MeshBlock.device_time_step): one launch when every
+        # block stepped since, each stale block alone when only some did
+        stale = [blk._w_stale for blk, _ in self.members]
+        if all(stale):
+            _capi.check(lib.snapy_wl_cons2prim(wl, p, s), "wl_cons2prim")
+        else:
+            for (blk, vars), st in zip(self.members, stale):
+                if st:
+                    blk._refresh_primitives(vars)
         _capi.check(lib.snapy_wl_cfl(wl, p, s), "wl_cfl")
         stages = self.members[0][0].intg.stages
         for k, st in enumerate(stages):
             _capi.check(lib.snapy_wl_stage(wl, p, k, st.wght0(), st.wght1(), st.wght2(), s), "wl_stage")
+        for blk, _ in self.members:
+            blk._w_stale = True

@@ -149,10 +149,11 @@ def test_cfl_and_eos(shape):
     assert lib.snapy_cons2prim(ctypes.byref(gd), ctypes.byref(pd), 0, ud.data_ptr(), wd.data_ptr(), None) == 0
     assert rel_l2(host(wd), fv.cons2prim(u, 1.4)) < TOL
     u2 = torch.zeros_like(ud)
-    assert lib.snapy_prim2cons(ctypes.byref(gd), ctypes.byref(pd), 0, dev(w).data_ptr(), u2.data_ptr(), None) == 0
+    wd2 = dev(w)  # referenced until the results are read back
+    assert lib.snapy_prim2cons(ctypes.byref(gd), ctypes.byref(pd), 0, wd2.data_ptr(), u2.data_ptr(), None) == 0
     assert rel_l2(host(u2), u) < TOL
     out = torch.zeros((), dtype=torch.float64, device="cuda")
-    assert lib.snapy_cfl(ctypes.byref(gd), ctypes.byref(pd), 0, dev(w).data_ptr(), 1.0, out.data_ptr(), None) == 0
+    assert lib.snapy_cfl(ctypes.byref(gd), ctypes.byref(pd), 0, wd2.data_ptr(), 1.0, out.data_ptr(), None) == 0
     # the min of the same per-cell quotients: exact
     assert float(host(out)) == fv.max_time_step(g, w, 1.4)
 
@@ -166,13 +167,14 @@ def test_budget_fp64():
     n = lib.snapy_budget_workspace(ctypes.byref(gd))
     ws = torch.empty(int(n), dtype=torch.float64, device="cuda")
     out = torch.empty(5, dtype=torch.float64, device="cuda")
-    assert lib.snapy_budget(ctypes.byref(gd), 0, dev(u).data_ptr(), ws.data_ptr(), out.data_ptr(), None) == 0
+    ud = dev(u)  # referenced until the results are read back
+    assert lib.snapy_budget(ctypes.byref(gd), 0, ud.data_ptr(), ws.data_ptr(), out.data_ptr(), None) == 0
     vol = g.dx(3) * (g.dx(2) * g.dx(1))
     ref = (u[(slice(None),) + g.interior()] * vol).reshape(5, -1).sum(1)
     np.testing.assert_allclose(host(out), ref, rtol=1e-13)
     # deterministic: the same bits twice
     out2 = torch.empty_like(out)
-    assert lib.snapy_budget(ctypes.byref(gd), 0, dev(u).data_ptr(), ws.data_ptr(), out2.data_ptr(), None) == 0
+    assert lib.snapy_budget(ctypes.byref(gd), 0, ud.data_ptr(), ws.data_ptr(), out2.data_ptr(), None) == 0
     assert np.array_equal(host(out), host(out2))
 
 

@@ -43,7 +43,8 @@ def test_swe_sweep_matches_oracle(dim):
     pd = swe_phys(bcs)
     flux = torch.zeros((4,) + g.shape, dtype=torch.float64, device="cuda")
     print(f"swe sweep x{dim}", flush=True)
-    rc = lib.snapy_sweep(dim, ctypes.byref(gd), ctypes.byref(pd), 0, dev(w).data_ptr(), flux.data_ptr(), None, None)
+    wd = dev(w)  # referenced until the results are read back
+    rc = lib.snapy_sweep(dim, ctypes.byref(gd), ctypes.byref(pd), 0, wd.data_ptr(), flux.data_ptr(), None, None)
     assert rc == 0, lib.snapy_last_error()
     tdim = 4 - dim
     wlr = fv.reconstruct(w, tdim, False)

@@ -47,7 +47,7 @@ def test_batch_is_bitwise_the_solo_blocks():
     solo = [_tube(n, 100 + i, bc) for i, (n, bc) in enumerate(TUBES)]
     batched = [_tube(n, 100 + i, bc) for i, (n, bc) in enumerate(TUBES)]
     batch = BlockBatch(batched)
-    assert batch.launches_per_step() == 14
+    assert batch.launches_per_step() == 15
     for _ in range(8):
         for blk, vars in solo:
             _solo_step(blk, vars)

@@ -410,3 +410,19 @@ def test_layer_heat_adds_flux_times_area_of_energy_to_the_wall_cells():
             # x1 does; their du is never read)
             assert not du[[fv.IDN, fv.IVX, fv.IVY, fv.IVZ]].any()
             assert not du[fv.IPR][..., : g.nghost].any() and not du[fv.IPR][..., g.nghost + g.nx1 :].any()
+
+
+def test_step_field_fold_is_the_weighted_stage_sum():
+    """Three stages of S <- w1 S + w2 dt T with the SSP-RK3 weights give
+    sum_s (w2_s prod_{t > s} w1_t) dt T_s (tests/test_flux_register.cpp:
+    211-248), whatever S held before the step (w1 = 0 at stage 0)."""
+    rng = np.random.default_rng(8)
+    T = [rng.standard_normal((5, 3, 4)) for _ in range(3)]
+    S = rng.standard_normal((5, 3, 4))
+    dt = 0.73
+    for s in range(3):
+        _, w1, w2 = fv.RK3_WEIGHTS[s]
+        S = fv.step_field_fold(S, T[s], w1, w2, dt)
+    c = [1.0 * 0.25 * (2.0 / 3.0), 0.25 * (2.0 / 3.0), 2.0 / 3.0]
+    ref = sum(c[s] * dt * T[s] for s in range(3))
+    np.testing.assert_allclose(S, ref, rtol=1e-14, atol=1e-15)

@@ -26,9 +26,12 @@ class _State:
 class _Block:
     """What restart_variables reads of a block."""
 
+    def cycle(self):
+        return self._cycle
+
     def __init__(self, fp32):
         torch.manual_seed(0)
-        self.cycle = 17
+        self._cycle = 17
         self._coord = _Coord()
         dt = torch.float32 if fp32 else torch.float64
         self.u = torch.rand((5, 1, 12, 14), dtype=torch.float64).to(dt)
