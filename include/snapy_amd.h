@@ -368,6 +368,19 @@ int snapy_vic_full_f32(const snapy_grid *g, const snapy_physics *p,
 int64_t snapy_budget_workspace(const snapy_grid *g);
 int snapy_budget(const snapy_grid *g, int dtype, const void *u, double *ws,
                  double *out, void *stream);
+/* snapy_budget without a workspace argument and without any allocation: the
+ * partial sums live in a static device array of the library (40 KiB per GPU,
+ * held until the process ends), whose address on the current device is
+ * looked up once per device.  The same kernels, so the same bits as
+ * snapy_budget.  The array holds 5 rows: a larger nvar returns an error
+ * before any launch.  Calls on one GPU share that array: calls that overlap in
+ * time on one GPU (different streams or threads) race and must be
+ * serialised by the caller.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under hipGraph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). */
+int snapy_budget_noalloc(const snapy_grid *g, int dtype, const void *u,
+                         double *out, void *stream);
 
 /* ---- launch-shape tuning --------------------------------------------------
  *

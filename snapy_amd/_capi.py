@@ -176,6 +176,7 @@ SIGNATURES = {
     "snapy_vic_full_serial": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P]),
     "snapy_budget_workspace": (ctypes.c_int64, [_GP]),
     "snapy_budget": (_I, [_GP, _I, _P, _P, _P, _P]),
+    "snapy_budget_noalloc": (_I, [_GP, _I, _P, _P, _P]),
     "snapy_cs_geom_size": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32]),
     "snapy_cs_edge_size": (ctypes.c_int64, [_MP]),
     "snapy_cs_cons2prim": (_I, [_MP, _P, _P, _P]),

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "common.hpp"
@@ -430,6 +431,16 @@ __global__ void __launch_bounds__(kBudgetTpb)
 
 constexpr int64_t kBudgetBlocks = 1024;
 
+// The allocation-free budget's partial sums (snapy_budget_noalloc): one FP64
+// slot per (row, first-pass workgroup) for up to kScratchRows rows (the five
+// conserved rows of the ideal gas), 40 KiB.
+// A file-scope device array: every GPU the library runs on holds its own
+// copy until the process ends.  Host code never hands this name to a kernel
+// (on the host it only stands for the symbol); it takes the address of the
+// current device's copy from hipGetSymbolAddress (scratch_address below).
+constexpr int kScratchRows = 5;
+static __device__ __align__(256) double scratch[kScratchRows * kBudgetBlocks];
+
 }  // namespace snapy
 
 // ===========================================================================
@@ -444,6 +455,48 @@ int dispatch(int dtype, F&& f) {
   if (dtype == SNAPY_F64) return f(double(0));
   if (dtype == SNAPY_F32) return f(float(0));
   return fail(-10, "unknown dtype");
+}
+
+// The device address of `scratch` on each device, looked up once per device
+// with that device current (each GPU holds its own copy at its own address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+int scratch_address(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0)
+    return fail(-12, "budget_noalloc: no current HIP device");
+  if (dev >= kMaxDevices)
+    return fail(-12, "budget_noalloc: device index " + std::to_string(dev) + " beyond " +
+                         std::to_string(kMaxDevices));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(snapy::scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(-12, std::string("budget_noalloc: hipGetSymbolAddress failed: ") +
+                           hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+
+// the two passes of the FP64 budget, partial sums in ws
+int budget_launch(snapy_grid const* g, int dtype, void const* u, double* ws, double* out,
+                  hipStream_t s) {
+  Range3 r = ranges(*g);
+  double vol = g->dx3 * (g->dx2 * g->dx1);
+  return dispatch(dtype, [&](auto z) {
+    using T = decltype(z);
+    hipLaunchKernelGGL(budget_partial_kernel<T>, dim3(unsigned(kBudgetBlocks), unsigned(g->nvar)),
+                       dim3(kBudgetTpb), 0, s, static_cast<T const*>(u), ws, r, g->nvar, vol,
+                       kBudgetBlocks);
+    hipLaunchKernelGGL(budget_final_kernel, dim3(unsigned(g->nvar)), dim3(kBudgetTpb), 0, s, ws, out,
+                       kBudgetBlocks);
+    return launch_status("budget");
+  });
 }
 
 int check_phys(snapy_physics const* p) {
@@ -713,19 +766,23 @@ int snapy_budget(const snapy_grid* g, int dtype, const void* u, double* ws,
                  double* out, void* stream) {
   if (int e = check_grid(g)) return e;
   if (!u || !ws || !out) return fail(-1, "budget: NULL buffer");
-  Range3 r = ranges(*g);
-  double vol = g->dx3 * (g->dx2 * g->dx1);
-  hipStream_t s = as_stream(stream);
-  return dispatch(dtype, [&](auto z) {
-    using T = decltype(z);
-    hipLaunchKernelGGL(budget_partial_kernel<T>,
-                       dim3(unsigned(kBudgetBlocks), unsigned(g->nvar)),
-                       dim3(kBudgetTpb), 0, s, static_cast<T const*>(u), ws, r,
-                       g->nvar, vol, kBudgetBlocks);
-    hipLaunchKernelGGL(budget_final_kernel, dim3(unsigned(g->nvar)),
-                       dim3(kBudgetTpb), 0, s, ws, out, kBudgetBlocks);
-    return launch_status("budget");
-  });
+  return budget_launch(g, dtype, u, ws, out, as_stream(stream));
+}
+
+int snapy_budget_noalloc(const snapy_grid* g, int dtype, const void* u,
+                         double* out, void* stream) {
+  if (!g) return fail(-1, "grid descriptor is NULL");
+  if (g->nvar > kScratchRows || g->nvar < 1)
+    return fail(-13, "budget_noalloc: nvar " + std::to_string(g->nvar) +
+                         " exceeds the " + std::to_string(kScratchRows) +
+                         " rows its device scratch holds");
+  if (int e = check_grid(g)) return e;
+  if (!u || !out) return fail(-1, "budget_noalloc: NULL buffer");
+  double* ws = nullptr;
+  if (int e = scratch_address(&ws)) return e;
+  // the first pass writes every slot the second pass reads: nothing of the
+  // previous call's contents is read, so no clear is needed
+  return budget_launch(g, dtype, u, ws, out, as_stream(stream));
 }
 
 }  // extern "C"
