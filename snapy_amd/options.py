@@ -4,9 +4,12 @@ Restates the YAML surface the reference reads for this path
 (MeshBlockOptionsImpl::from_yaml, src/mesh/meshblock_options.cpp:18-77, and
 the per-module readers it calls).  Keys the stage does not implement raise a
 ValueError naming the key, as the reference's strict key checks do
-(src/input/check_keys.cpp); sections that belong to subsystems outside the
-stage (outputs, species, reference-state, distribute, problem, scalar) are
-accepted and ignored.
+(src/input/check_keys.cpp).  Sections that would add state rows or physics
+the stage does not carry are refused by name: `scalar` with nvar > 0,
+`species` beyond the single dry gas, `reactions`, and a `distribute` that
+splits the domain over blocks.  Sections read by the driver rather than the
+stage (`outputs`, `problem`) and the dry gas's inert `reference-state` are
+accepted.
 """
 
 from __future__ import annotations
@@ -87,6 +90,7 @@ class EquationOfStateOptions:
     weight: float = 29.0e-3
     density_floor: float = 1.0e-10
     pressure_floor: float = 1.0e-10
+    temperature_floor: float = 20.0
     limiter: bool = False
 
     def gas_constant_dry(self) -> float:
@@ -163,12 +167,47 @@ class MeshBlockOptions:
                 "reactions",
             },
         )
+        _check_sections(cfg)
+        dist = cfg.get("distribute") or {}
+        _check_keys(dist, "distribute", _DIST_KEYS)
+        for k in ("nb1", "nb2", "nb3"):
+            if int(dist.get(k, 1)) != 1:
+                raise ValueError(
+                    f"distribute/{k} = {dist[k]}: a MeshBlock here covers the whole domain "
+                    "(nb1 = nb2 = nb3 = 1; the cubed sphere goes through MeshOptions)"
+                )
         op.coord = _read_geometry(cfg.get("geometry") or {})
         op.hydro = _read_dynamics(cfg.get("dynamics") or {}, cfg.get("forcing") or {})
         op.intg = _read_integration(cfg.get("integration") or {})
         op.bcnames = _read_bcs(cfg.get("boundary-condition") or {})
         _check_supported(op)
         return op
+
+
+_DIST_KEYS = {"layout", "nb1", "nb2", "nb3", "blocks_per_process", "verbose"}
+
+
+def _check_sections(cfg) -> None:
+    """The deck sections that would add state rows or physics this stage
+    does not carry are refused by name: transported scalars (tracers),
+    species beyond the single dry gas, reactions.  `outputs` (I/O) and
+    `problem` (driver parameters) are read by the caller, not the stage;
+    `reference-state` is the thermodynamic reference of moist species, inert
+    for the dry ideal gas."""
+    sc = cfg.get("scalar")
+    if sc:
+        _check_keys(sc, "scalar", {"nvar", "names"})
+        if int(sc.get("nvar", 0)) > 0:
+            raise ValueError(f"scalar: nvar {int(sc['nvar'])} (transported tracers) is not implemented")
+    sp = cfg.get("species")
+    if sp is not None:
+        if not isinstance(sp, list):
+            raise ValueError("species: a list of species")
+        if len(sp) > 1:
+            names = [str((x or {}).get("name", "?")) for x in sp]
+            raise ValueError(f"species: {names}: only the single dry species is implemented")
+    if cfg.get("reactions"):
+        raise ValueError("reactions: chemistry is not implemented")
 
 
 def _read_geometry(g) -> CoordinateOptions:
@@ -221,7 +260,7 @@ def _read_dynamics(d, forcing) -> HydroOptions:
     _check_keys(
         e,
         "dynamics/equation-of-state",
-        {"type", "gammad", "weight", "density-floor", "pressure-floor", "limiter"},
+        {"type", "gammad", "weight", "density-floor", "pressure-floor", "temperature-floor", "limiter"},
     )
     h.eos.type = e.get("type", h.eos.type)
     h.eos.gammad = float(e.get("gammad", h.eos.gammad))
@@ -229,6 +268,10 @@ def _read_dynamics(d, forcing) -> HydroOptions:
     h.eos.density_floor = float(e.get("density-floor", h.eos.density_floor))
     h.eos.pressure_floor = float(e.get("pressure-floor", h.eos.pressure_floor))
     h.eos.limiter = bool(e.get("limiter", h.eos.limiter))
+    # read for the record: the floor acts only inside the conserved limiter
+    # (EquationOfStateImpl::apply_conserved_limiter_, equation_of_state.cpp:
+    # 305-360), and limiter: true is refused here
+    h.eos.temperature_floor = float(e.get("temperature-floor", h.eos.temperature_floor))
     rc = d.get("reconstruct") or {}
     _check_keys(rc, "dynamics/reconstruct", {"vertical", "horizontal"})
     h.recon1 = _read_recon(rc.get("vertical"), "dynamics/reconstruct/vertical")
@@ -419,8 +462,11 @@ class MeshOptions:
     def from_yaml(filename: str, verbose: bool = False) -> "MeshOptions":
         with open(filename) as f:
             cfg = yaml.safe_load(f) or {}
+        _check_sections(cfg)
         dist = cfg.get("distribute") or {}
-        _check_keys(dist, "distribute", {"layout", "nb2", "nb3", "blocks_per_process", "verbose"})
+        _check_keys(dist, "distribute", _DIST_KEYS)
+        if int(dist.get("nb1", 1)) != 1:
+            raise ValueError("distribute/nb1: the vertical implicit solve needs one block along x1")
         mo = MeshOptions()
         mo.layout = str(dist.get("layout", "slab"))
         mo.nb2 = int(dist.get("nb2", 1))
