@@ -258,9 +258,11 @@ int snapy_vic_full(const snapy_grid *g, const snapy_physics *p, const double *w,
                    double t1, double dt, const double *dt_dev, double wstage,
                    double w0, double w1, double w2, double *ws, int *status,
                    void *stream);
-/* The same solve with one column per thread and the reference's Crout LU
- * (ludcmp/lubksb) on each block: the same arguments and outputs as
- * snapy_vic_full, kept as the independent cross-check of the group kernel. */
+/* The same solve with one column per thread and textbook Gaussian
+ * elimination with scaled partial pivoting on each block: the same
+ * arguments and outputs as snapy_vic_full, kept as the independent
+ * cross-check of the lane-group kernel (no counterpart of its own in the
+ * reference; it replaces the same vic_stage seam). */
 int snapy_vic_full_serial(const snapy_grid *g, const snapy_physics *p,
                           const double *w, const double *du, const double *u0,
                           double *u, const double *vtab, const double *x1tab,

@@ -97,78 +97,66 @@ __device__ __forceinline__ void roe_abs(M5 out, double gm1, double const* wl,
     }
 }
 
-// ludcmp with implicit row scaling and partial pivoting (Crout), then
-// lubksb on NRHS right-hand-side columns (src/math/ludcmp.h, lubksb.h).
-// Returns false on a singular or non-finite factorisation.
+// Gaussian elimination of the augmented block [A | B] (NRHS columns of B)
+// with scaled partial pivoting: each row is measured against its largest
+// original entry, the pivot of step k is the row i >= k of largest
+// |a_ik| / s_i, the rows below are eliminated, then x is back-substituted
+// (the textbook algorithm, e.g. Golub & Van Loan, Matrix Computations,
+// sec. 3.4).  A pivot whose scaled size is within 8 N eps of zero, or any
+// non-finite value, fails the solve (the reference's singularity threshold).
+// This is the serial cross-check of the lane-parallel elimination in
+// vic_forward_kernel; the solution is returned in B.
 template <int NRHS>
-__device__ __forceinline__ bool lu_solve(M5 a, double (*rhs)[NRHS]) {
-  int indx[5];
-  double vv[5], scale[5];
-  const double tol = 8. * 5. * 2.220446049250313e-16;
-  for (int i = 0; i < 5; ++i) {
-    double big = 0.;
-    for (int j = 0; j < 5; ++j) {
-      double t = fabs(a[i][j]);
-      if (t > big) big = t;
-    }
-    if (big == 0.) return false;
-    scale[i] = big;
-    vv[i] = 1.0 / big;
-    if (!isfinite(vv[i])) return false;
+__device__ __forceinline__ bool lu_solve(M5 a, double (*b)[NRHS]) {
+  constexpr int N = 5;
+  double const eps_n = 8. * N * 2.220446049250313e-16;
+  double rs[N];  // the rows' largest original magnitudes, swapped with them
+  for (int r = 0; r < N; ++r) {
+    double m = 0.;
+    for (int c = 0; c < N; ++c) m = fmax(m, fabs(a[r][c]));
+    if (!(m > 0.) || !isfinite(m)) return false;
+    rs[r] = m;
   }
-  for (int j = 0; j < 5; ++j) {
-    for (int i = 0; i < j; ++i) {
-      double sum = a[i][j];
-      for (int k = 0; k < i; ++k) sum -= a[i][k] * a[k][j];
-      a[i][j] = sum;
-    }
-    double big = 0.;
-    int imax = j;
-    for (int i = j; i < 5; ++i) {
-      double sum = a[i][j];
-      for (int k = 0; k < j; ++k) sum -= a[i][k] * a[k][j];
-      a[i][j] = sum;
-      double dum = vv[i] * fabs(sum);
-      if (!isfinite(dum)) return false;
-      if (dum >= big) {
-        big = dum;
-        imax = i;
+  for (int k = 0; k < N; ++k) {
+    int p = k;
+    double best = fabs(a[k][k]) / rs[k];
+    for (int r = k + 1; r < N; ++r) {
+      double q = fabs(a[r][k]) / rs[r];
+      if (q > best) {
+        best = q;
+        p = r;
       }
     }
-    if (j != imax) {
-      for (int k = 0; k < 5; ++k) {
-        double d = a[imax][k];
-        a[imax][k] = a[j][k];
-        a[j][k] = d;
+    if (!(best > eps_n) || !isfinite(best)) return false;
+    if (p != k) {
+      for (int c = 0; c < N; ++c) {
+        double t = a[k][c];
+        a[k][c] = a[p][c];
+        a[p][c] = t;
       }
-      vv[imax] = vv[j];
-      double t = scale[imax];
-      scale[imax] = scale[j];
-      scale[j] = t;
-    }
-    indx[j] = imax;
-    if (fabs(a[j][j]) / scale[j] <= tol) return false;
-    if (j != 4) {
-      double dum = 1.0 / a[j][j];
-      for (int i = j + 1; i < 5; ++i) a[i][j] *= dum;
-    }
-  }
-  for (int i = 0; i < 5; ++i) {
-    int ip = indx[i];
-    if (ip != i)
       for (int c = 0; c < NRHS; ++c) {
-        double t = rhs[ip][c];
-        rhs[ip][c] = rhs[i][c];
-        rhs[i][c] = t;
+        double t = b[k][c];
+        b[k][c] = b[p][c];
+        b[p][c] = t;
       }
-    for (int j = 0; j < i; ++j)
-      for (int c = 0; c < NRHS; ++c) rhs[i][c] -= a[i][j] * rhs[j][c];
+      double t = rs[k];
+      rs[k] = rs[p];
+      rs[p] = t;
+    }
+    double inv = 1. / a[k][k];
+    for (int r = k + 1; r < N; ++r) {
+      double m = a[r][k] * inv;
+      for (int c = k + 1; c < N; ++c) a[r][c] -= m * a[k][c];
+      for (int c = 0; c < NRHS; ++c) b[r][c] -= m * b[k][c];
+    }
   }
-  for (int i = 4; i >= 0; --i) {
-    for (int j = i + 1; j < 5; ++j)
-      for (int c = 0; c < NRHS; ++c) rhs[i][c] -= a[i][j] * rhs[j][c];
-    for (int c = 0; c < NRHS; ++c) rhs[i][c] /= a[i][i];
-  }
+  for (int k = N - 1; k >= 0; --k)
+    for (int c = 0; c < NRHS; ++c) {
+      double x = b[k][c];
+      for (int j2 = k + 1; j2 < N; ++j2) x -= a[k][j2] * b[j2][c];
+      b[k][c] = x / a[k][k];
+      if (!isfinite(b[k][c])) return false;
+    }
   return true;
 }
 
