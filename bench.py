@@ -112,64 +112,106 @@ def workload(name, precision="float64", execution="hip-graph"):
 
 
 # --------------------------------------------------------------------------
-def cpu_baseline(name, budget_s=10.0):
-    """The CPU FP64 oracle (numpy, one thread) on a bounded sample of the
-    same workload."""
+def _oracle_leg(name, budget_s):
+    """The CPU FP64 oracle (numpy) on a bounded sample of the workload, run
+    for about `budget_s` seconds in this process with its BLAS pools held to
+    one thread; returns (cell-updates, seconds, sample description)."""
     import numpy as np
+    from threadpoolctl import threadpool_limits
 
     from oracle import fv
 
-    t0 = time.perf_counter()
-    stages = cells = runs = 0
-    if name == "sod":
-        while time.perf_counter() - t0 < budget_s:
-            d = fv.sod_deck(256)
-            b = fv.Block(d)
-            b.initialize(fv.sod_initial(d.grid))
-            _, n = fv.run(b, 0.25)
-            stages += 3 * n
-            runs += 1
-        el = time.perf_counter() - t0
-        return {
-            "value": 256 * stages / el,
-            "unit": "cell-updates/s",
-            "cores": 1,
-            "kind": "port",
-            "sample": f"{runs} full Sod runs (256 cells, t_end 0.25), numpy FP64 oracle, {el:.1f} s",
-        }
-    # convection3d: a 24 x 24 column sub-domain of the same 128-layer column
-    # physics (periodic x2/x3), whole RK3 steps
-    from snapy_amd import MeshBlock, MeshBlockOptions, problems
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        stages = runs = 0
+        if name == "sod":
+            while time.perf_counter() - t0 < budget_s:
+                d = fv.sod_deck(256)
+                b = fv.Block(d)
+                b.initialize(fv.sod_initial(d.grid))
+                _, n = fv.run(b, 0.25)
+                stages += 3 * n
+                runs += 1
+            el = time.perf_counter() - t0
+            return 256 * stages, el, f"{runs} full Sod runs (256 cells, t_end 0.25) per worker"
+        # convection3d: a 24 x 24 column sub-domain of the same 128-layer
+        # column physics (periodic x2/x3), whole RK3 steps
+        from snapy_amd import MeshBlock, MeshBlockOptions, problems
 
-    op = MeshBlockOptions.from_yaml(os.path.join(ROOT, "snapy_amd", "decks", "convection3d.yaml"))
-    op.coord.nx2 = op.coord.nx3 = 24
-    op.coord.x2max = op.coord.x3max = 2400.0
-    blk = MeshBlock(op)
-    w = problems.convection3d(blk).numpy()
-    c = op.coord
-    g = fv.Grid(c.nx1, c.nx2, c.nx3, c.x1min, c.x1max, c.x2min, c.x2max, c.x3min, c.x3max)
-    h = op.hydro
-    d = fv.Deck(grid=g, gammad=h.eos.gammad, riemann="lmars", shock_x1=False, shock_x23=False,
-                bcs=tuple(op.bcnames), cfl=op.intg.cfl, grav1=h.grav1,
-                implicit_scheme=SCHEME["value"] or op.intg.implicit_scheme,
-                flux_covariance=True, bot_heat_flux=h.bot_heat_flux, bot_heat_depth=h.bot_heat_depth)
-    b = fv.Block(d)
-    b.initialize(w)
-    steps = 0
-    while time.perf_counter() - t0 < budget_s or steps == 0:
-        dt = b.max_time_step()
-        for s in range(3):
-            b.forward(dt, s)
-        steps += 1
-    el = time.perf_counter() - t0
-    n = c.nx1 * c.nx2 * c.nx3
-    assert np.isfinite(b.u).all()
+        op = MeshBlockOptions.from_yaml(os.path.join(ROOT, "snapy_amd", "decks", "convection3d.yaml"))
+        op.coord.nx2 = op.coord.nx3 = 24
+        op.coord.x2max = op.coord.x3max = 2400.0
+        blk = MeshBlock(op)
+        w = problems.convection3d(blk).numpy()
+        c = op.coord
+        g = fv.Grid(c.nx1, c.nx2, c.nx3, c.x1min, c.x1max, c.x2min, c.x2max, c.x3min, c.x3max)
+        h = op.hydro
+        d = fv.Deck(grid=g, gammad=h.eos.gammad, riemann="lmars", shock_x1=False, shock_x23=False,
+                    bcs=tuple(op.bcnames), cfl=op.intg.cfl, grav1=h.grav1,
+                    implicit_scheme=SCHEME["value"] or op.intg.implicit_scheme,
+                    flux_covariance=True, bot_heat_flux=h.bot_heat_flux, bot_heat_depth=h.bot_heat_depth)
+        b = fv.Block(d)
+        b.initialize(w)
+        steps = 0
+        while time.perf_counter() - t0 < budget_s or steps == 0:
+            dt = b.max_time_step()
+            for s in range(3):
+                b.forward(dt, s)
+            steps += 1
+        el = time.perf_counter() - t0
+        assert np.isfinite(b.u).all()
+        n = c.nx1 * c.nx2 * c.nx3
+        return n * 3 * steps, el, f"{steps} RK3 steps of a 128x24x24 sub-domain of the same deck per worker"
+
+
+def _oracle_worker(job):
+    name, budget_s, scheme = job
+    SCHEME["value"] = scheme
+    return _oracle_leg(name, budget_s)
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(name, budget_s=10.0, max_workers=16):
+    """The CPU FP64 oracle timed on this host: one worker alone, then one
+    worker per core of the process's affinity mask (at most `max_workers`,
+    the host share of one GPU) together, each on its own copy of the sample
+    with one thread.  Run before the GPU is touched: the workers are forked
+    from this process.  `value` is the all-cores aggregate."""
+    import multiprocessing as mp
+
+    import torch
+
+    ncores = len(os.sched_getaffinity(0))
+    n = max(1, min(ncores, max_workers))
+    ctx = mp.get_context("fork")
+    job = (name, budget_s, SCHEME["value"])
+    with ctx.Pool(1) as pool:
+        cu1, el1, sample = pool.map(_oracle_worker, [job])[0]
+    with ctx.Pool(n) as pool:
+        res = pool.map(_oracle_worker, [job] * n)
+    agg = sum(cu / el for cu, el, _ in res)
     return {
-        "value": n * 3 * steps / el,
+        "value": agg,
         "unit": "cell-updates/s",
-        "cores": 1,
+        "cores": n,
         "kind": "port",
-        "sample": f"{steps} RK3 steps of a 128x24x24 sub-domain of the same deck, numpy FP64 oracle, {el:.1f} s",
+        "value_1_core": cu1 / el1,
+        "cores_in_affinity_mask": ncores,
+        "cpu_model": _cpu_model(),
+        "torch_num_threads": torch.get_num_threads(),
+        "sample": f"numpy FP64 oracle, {sample}, about {budget_s:.0f} s; {n} single-threaded workers "
+                  f"together (each its own copy), and one alone",
     }
 
 
@@ -498,6 +540,12 @@ def main():
         print(json.dumps({"error": "this benchmark measures one GPU; run with "
                           "--gpus 1 (the path shards by whole members or x2/x3 column groups; not measured here)"}), flush=True)
         sys.exit(2)
+    # the CPU baseline first, from worker processes forked before this
+    # process touches the GPU
+    cpu = None
+    if not (args.no_cpu or args.copy_calibration):
+        cpu = cpu_baseline(args.workload)
+        print(f"# cpu baseline {cpu}", file=sys.stderr, flush=True)
     import torch
 
     if not torch.cuda.is_available():
@@ -568,7 +616,6 @@ def main():
         roofline["copy_GBps"] = cp["copy_GBps"]
         roofline["frac_of_copy"] = achieved / cp["copy_GBps"]
         torch.cuda.empty_cache()
-    cpu = None if args.no_cpu else cpu_baseline(args.workload)
     accuracy = None
     if precision == "float32":
         del block, vars
