@@ -200,16 +200,22 @@ struct WbArgs {
   int wall_lo, wall_hi;      // prescribed wall planes
 };
 
-// The x1 column path computes in FP64 (T); TF is the stored flux precision.
-// An FP32 flux keeps the x1 momentum row as the anomaly about the reference
-// face pressure psf (the FP64 update adds psf back), so the FP32 word holds
-// the small non-hydrostatic part of the flux.
+// T is the compute precision of the face (the primitives w, the reference
+// and the restored face densities are FP64); TF is the stored flux
+// precision.  An FP32 flux keeps the x1 momentum row as the anomaly about
+// the reference face pressure psf (the FP64 update adds psf back), so the
+// FP32 word holds the small non-hydrostatic part of the flux.  With FP32
+// compute (LMARS only) the face states carry the pressure perturbation
+// itself, p - psf, and LMARS takes psf as its gauge: the enthalpy and the
+// sound speed see the total pressure, the momentum flux and the face
+// pressure come out as anomalies about psf, so FP32 keeps the precision of
+// the non-hydrostatic part.
 // One x1 face of the well-balanced sweep: face fi (x1 index i) of the
 // column at row; pref(m) and dref(m) give the reference pressure and
 // density of cell m of the column, psf and dsf the scan's face pressure and
 // density at the face.
 template <typename T, typename TF, typename PREF, typename DREF>
-__device__ __forceinline__ void wb_face(T const* __restrict__ w, int64_t row, int fi, double psf,
+__device__ __forceinline__ void wb_face(double const* __restrict__ w, int64_t row, int fi, double psf,
                                         double dsf, PREF pref, DREF dref,
                                         TF* __restrict__ flx, double* __restrict__ fp,
                                         WbArgs const& a) {
@@ -231,8 +237,8 @@ __device__ __forceinline__ void wb_face(T const* __restrict__ w, int64_t row, in
   T* lv[3] = {&L.v[0], &L.v[1], &L.v[2]};
   T* rv[3] = {&R.v[0], &R.v[1], &R.v[2]};
   for (int v = IVX; v <= IVZ; ++v) {
-    T const* p = w + v * r.sv + row + i;
-    T c0 = p[-3], c1 = p[-2], c2 = p[-1], c3 = p[0], c4 = p[1], c5 = p[2];
+    double const* p = w + v * r.sv + row + i;
+    T c0 = T(p[-3]), c1 = T(p[-2]), c2 = T(p[-1]), c3 = T(p[0]), c4 = T(p[1]), c5 = T(p[2]);
     if (a.shock) {
       *lv[v - 1] = weno5_plus(c0, c1, c2, c3, c4);
       *rv[v - 1] = weno5_minus(c1, c2, c3, c4, c5);
@@ -254,8 +260,14 @@ __device__ __forceinline__ void wb_face(T const* __restrict__ w, int64_t row, in
       pr = cp5_minus(c1, c2, c3, c4, c5);
     }
     double tl = double(pl) + psf, tr = double(pr) + psf;
-    L.p = T(tl > 0. ? tl : psf);
-    R.p = T(tr > 0. ? tr : psf);
+    if constexpr (sizeof(T) == 4) {
+      // the perturbation itself (zero where the total is floored to psf)
+      L.p = tl > 0. ? pl : T(0);
+      R.p = tr > 0. ? pr : T(0);
+    } else {
+      L.p = T(tl > 0. ? tl : psf);
+      R.p = T(tr > 0. ? tr : psf);
+    }
   }
   {
     T c0 = pert(IDN, i - 3), c1 = pert(IDN, i - 2), c2 = pert(IDN, i - 1),
@@ -283,22 +295,27 @@ __device__ __forceinline__ void wb_face(T const* __restrict__ w, int64_t row, in
   T g = T(a.gammad);
   T out[5];
   T pface;
-  if (a.riemann == SNAPY_RIEMANN_LMARS)
-    lmars(L, R, g, g, 0, out, &pface);
-  else
-    hllc(L, R, g, g, 0, out, &pface);
+  if constexpr (sizeof(T) == 4) {
+    // anomaly form: the momentum flux and pface are about psf already
+    lmars(L, R, g, g, 0, out, &pface, T(psf));
+  } else {
+    if (a.riemann == SNAPY_RIEMANN_LMARS)
+      lmars(L, R, g, g, 0, out, &pface);
+    else
+      hllc(L, R, g, g, 0, out, &pface);
+  }
   if ((fi == 0 && a.wall_lo) || (fi == a.nf - 1 && a.wall_hi)) {
     out[IDN] = T(0);
     out[IPR] = T(0);
   }
-  if constexpr (sizeof(TF) == 4) out[IVX] = out[IVX] - T(psf);
+  if constexpr (sizeof(TF) == 4 && sizeof(T) == 8) out[IVX] = out[IVX] - T(psf);
   for (int v = 0; v < 5; ++v) flx[v * r.sv + row + i] = TF(out[v]);
-  if (fp) fp[row + i] = double(pface);
+  if (fp) fp[row + i] = sizeof(T) == 4 ? double(pface) + psf : double(pface);
 }
 
 template <typename T, typename TF>
 __global__ void __launch_bounds__(256)
-    sweep_x1_wb_kernel(T const* __restrict__ w, double const* __restrict__ ref,
+    sweep_x1_wb_kernel(double const* __restrict__ w, double const* __restrict__ ref,
                        TF* __restrict__ flx, double* __restrict__ fp, WbArgs a) {
   Range3 const& r = a.r;
   int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -444,7 +461,7 @@ __global__ void __launch_bounds__(kRefThreads)
     double const* s_rs = sc + 3 * nc1 + 1;
     double lo = floor_mp(s_F[i]);
     double rf = i > 0 ? 0.5 * (s_rs[i - 1] + s_rs[i]) : s_rs[i];
-    wb_face<double, TF>(
+    wb_face<T, TF>(
         w, s_row[c], fi, lo, lo * rf, [&](int m) { return sc[m]; },
         [&](int m) { return sc[nc1 + m]; }, flx, fp, a);
   }
@@ -627,6 +644,13 @@ int snapy_sweep_x1_wb(const snapy_grid* g, const snapy_physics* p, int dtype,
                        as_stream(stream), static_cast<double const*>(w), ref,
                        static_cast<double*>(flux),
                        static_cast<double*>(face_pressure), a);
+  else if (p->riemann == SNAPY_RIEMANN_LMARS)
+    // FP32 fluxes with LMARS: FP32 compute in the anomaly form
+    hipLaunchKernelGGL((sweep_x1_wb_kernel<float, float>),
+                       dim3(blocks_for(tot, tpb)), dim3(tpb), 0,
+                       as_stream(stream), static_cast<double const*>(w), ref,
+                       static_cast<float*>(flux),
+                       static_cast<double*>(face_pressure), a);
   else
     hipLaunchKernelGGL((sweep_x1_wb_kernel<double, float>),
                        dim3(blocks_for(tot, tpb)), dim3(tpb), 0,
@@ -658,6 +682,11 @@ int snapy_sweep_x1_wb_ref(const snapy_grid* g, const snapy_physics* p, int dtype
   if (dtype == SNAPY_F64)
     hipLaunchKernelGGL((sweep_x1_wb_ref_kernel<double, double>), grid, dim3(kRefThreads),
                        lds_of(nc), as_stream(stream), w, ref, static_cast<double*>(flux),
+                       static_cast<double*>(face_pressure), ra, a, nc);
+  else if (p->riemann == SNAPY_RIEMANN_LMARS)
+    // FP32 fluxes with LMARS: FP32 compute in the anomaly form
+    hipLaunchKernelGGL((sweep_x1_wb_ref_kernel<float, float>), grid, dim3(kRefThreads),
+                       lds_of(nc), as_stream(stream), w, ref, static_cast<float*>(flux),
                        static_cast<double*>(face_pressure), ra, a, nc);
   else
     hipLaunchKernelGGL((sweep_x1_wb_ref_kernel<double, float>), grid, dim3(kRefThreads),
