@@ -88,6 +88,42 @@ __device__ __forceinline__ void cov_states(double const* __restrict__ w, CovArgs
   }
 }
 
+// cov_states with the stencil from a staged tile: st(v, m) is row v of the
+// m-th cell of the face's stencil (m = 0..5 for the cells f - 3 .. f + 2
+// along the sweep), already shifted by the gauge and in T.
+template <typename T, int DIR, typename ST>
+__device__ __forceinline__ void cov_states_tile(CovArgs const& a, ST st, double gauge,
+                                                Prim5<T>& L, Prim5<T>& R, double& h,
+                                                double& un) {
+  T* lv[5] = {&L.r, &L.v[0], &L.v[1], &L.v[2], &L.p};
+  T* rv[5] = {&R.r, &R.v[0], &R.v[1], &R.v[2], &R.p};
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    T c0 = st(v, 0), c1 = st(v, 1), c2 = st(v, 2), c3 = st(v, 3), c4 = st(v, 4), c5 = st(v, 5);
+    if (v == IDN || a.shock) {
+      *lv[v] = weno5_plus(c0, c1, c2, c3, c4);
+      *rv[v] = weno5_minus(c1, c2, c3, c4, c5);
+    } else {
+      *lv[v] = cp5_plus(c0, c1, c2, c3, c4);
+      *rv[v] = cp5_minus(c1, c2, c3, c4, c5);
+    }
+  }
+  h = 0.;
+  un = 0.;
+  if (a.cov) {
+    double rho = 0.5 * (double(L.r) + double(R.r));
+    double p = 0.5 * (double(L.p) + double(R.p)) + gauge;
+    if constexpr (sizeof(T) == 8) {
+      double enth = p / (a.gammad - 1.) + p;
+      h = enth / rho;
+    } else {
+      float enth = qdiv(float(p), float(a.gammad - 1.)) + float(p);
+      h = double(qdiv(enth, float(rho)));
+    }
+    un = 0.5 * (double(L.v[DIR - 1]) + double(R.v[DIR - 1]));
+  }
+}
+
 // The flux of an interior face from its states, the covariance from the
 // x1 neighbours' h and u_n (hm, hp, um, up), the wall prescription, stored.
 template <typename T, int DIR>
@@ -206,6 +242,76 @@ __global__ void __launch_bounds__(kPackThreads)
                    sh_un[t + 1]);
 }
 
+// Tiled lines (nx1 + 2 <= kPackMax): as the packed kernel, a workgroup
+// holds `lines` whole x1 lines of nx1 + 2 lanes, the faces f0 .. f0 +
+// lines - 1 of one line along the sweep (a tile never straddles two lines,
+// so the tile shares one pressure gauge per x1 cell).  The tile's cells
+// f0 - 3 .. f0 + lines + 1 along the sweep are staged once in LDS, every
+// row already shifted by the gauge and rounded to T exactly as the lanes of
+// the packed kernel round them (so the fluxes are the same bits), with the
+// loads coalesced along x1; each lane then reads its six-cell stencils from
+// LDS instead of global memory.  Dynamic LDS: 5 (lines + 5) (nx1 + 2) T.
+template <typename T, int DIR>
+__global__ void __launch_bounds__(kPackThreads)
+    sweep23_tile_kernel(double const* __restrict__ w, double const* __restrict__ x1v,
+                        T* __restrict__ flx, CovArgs a, int lines, int tiles_per_line) {
+  extern __shared__ char smem_raw[];
+  T* st = reinterpret_cast<T*>(smem_raw);
+  __shared__ double sh_h[kPackThreads], sh_un[kPackThreads];
+  Range3 const& r = a.r;
+  int const t = threadIdx.x;
+  int const len = r.n1 + 2;
+  int const ncell = lines + 5;
+  // the tile: line o (k for DIR 2, j for DIR 3) and its first face f0
+  int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  int o = int(tile / tiles_per_line);
+  int f0 = int(tile - int64_t(o) * tiles_per_line) * lines;
+  int nl = a.nf - f0 < lines ? a.nf - f0 : lines;
+  int64_t sweep = DIR == 2 ? r.s2 : r.s3;
+  int lo = DIR == 2 ? r.lo2 : r.lo3;
+  int64_t obase = DIR == 2 ? int64_t(r.lo3 + o) * r.s3 : int64_t(r.lo2 + o) * r.s2;
+  // the line's gauge row: the first interior cell along the sweep
+  double const* gp = w + IPR * r.sv + obase + int64_t(lo) * sweep + (r.lo1 - 1);
+  // stage cells lo + f0 - 3 .. lo + f0 + nl + 1 (the rest of the tile area
+  // is left unread)
+  {
+    // rows of len cells, `per` rows at a time: thread t takes position sp
+    // of rows sr, sr + per, ...; each row's loads run along x1
+    int const per = int(blockDim.x) / len;
+    int const sr = t / len, sp = t - sr * len;
+    if (sr < per) {
+      double const g = a.gauge ? gp[sp] : 0.;
+      double const* src = w + obase + int64_t(lo + f0 - 3) * sweep + (r.lo1 - 1 + sp);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        double const gv = v == IPR ? g : 0.;
+        for (int c = sr; c < nl + 5; c += per)
+          st[(v * ncell + c) * len + sp] = T(src[v * r.sv + c * sweep] - gv);
+      }
+    }
+  }
+  __syncthreads();
+  int q = t / len, p = t - q * len;
+  bool in = q < nl;
+  int i = r.lo1 - 1 + p;
+  int f = f0 + q;
+  int j = DIR == 2 ? lo + f : r.lo2 + o;
+  int k = DIR == 2 ? r.lo3 + o : lo + f;
+  Prim5<T> L, R;
+  double h = 0., un = 0., gauge = 0.;
+  if (in) {
+    gauge = a.gauge ? gp[p] : 0.;
+    cov_states_tile<T, DIR>(
+        a, [&](int v, int m) { return st[(v * ncell + q + m) * len + p]; }, gauge, L, R, h, un);
+  }
+  sh_h[t] = h;
+  sh_un[t] = un;
+  __syncthreads();
+  if (!in || p == 0 || p == len - 1) return;
+  cov_flux<T, DIR>(x1v, flx, a, i, j, k, L, R, gauge, sh_h[t - 1], sh_h[t + 1], sh_un[t - 1],
+                   sh_un[t + 1]);
+}
+
 // lines per packed workgroup: the count whose lanes waste the least of
 // their whole wavefronts (ties: fewer lines)
 int packed_lines(int len) {
@@ -257,6 +363,28 @@ int snapy_sweep_cov(int dim, const snapy_grid* g, const snapy_physics* p,
   int len = g->nx1 + 2;
   if (len <= kPackMax) {
     int lines = packed_lines(len);
+    size_t tsize = dtype == SNAPY_F64 ? sizeof(double) : sizeof(float);
+    // the tiled kernel: fewer lines per workgroup when the staged tile
+    // (5 (lines + 5) len values) would pass 48 KiB
+    int tl = lines;
+    while (tl > 1 && size_t(5) * (tl + 5) * len * tsize > 48 * 1024) --tl;
+    size_t tlds = size_t(5) * (tl + 5) * len * tsize;
+    if (tlds <= 48 * 1024 && (dtype == SNAPY_F64 || dtype == SNAPY_F32)) {
+      int per = (a.nf + tl - 1) / tl;
+      int64_t nlines = dim == 2 ? g->nx3 : g->nx2;
+      unsigned threads = unsigned((tl * len + 63) / 64 * 64);
+      unsigned blocks = unsigned(nlines * per);
+      if (dtype == SNAPY_F64) {
+        auto k = dim == 2 ? sweep23_tile_kernel<double, 2> : sweep23_tile_kernel<double, 3>;
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(threads), tlds, s, static_cast<double const*>(w),
+                           x1tab, static_cast<double*>(flux), a, tl, per);
+      } else {
+        auto k = dim == 2 ? sweep23_tile_kernel<float, 2> : sweep23_tile_kernel<float, 3>;
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(threads), tlds, s, static_cast<double const*>(w),
+                           x1tab, static_cast<float*>(flux), a, tl, per);
+      }
+      return launch_status("sweep_cov");
+    }
     unsigned threads = unsigned((lines * len + 63) / 64 * 64);
     unsigned blocks = unsigned((ny + lines - 1) / lines);
     if (dtype == SNAPY_F64) {
