@@ -387,13 +387,16 @@ def density_current_leg(steps=50, warmup=3):
     return out
 
 
-def cubed_leg(steps=50, warmup=3, members=32, seed0=1000):
+def cubed_leg(steps=50, warmup=3, members=32, seed0=1000, precision="float32"):
     """The batched cubed-sphere shallow-water workload: `members` six-panel
-    meshes of 192 x 192 cells per panel (FP64), each seeded with its own
-    Gaussian hill (seed0 + member), advanced together: every stage kernel is
-    one launch over all members and panels, the stages are captured
-    hipGraphs, each member has its own device CFL dt.  Returns the
-    throughput, the per-kernel HIP-event times and the roofline."""
+    meshes of 192 x 192 cells per panel, each seeded with its own Gaussian
+    hill (seed0 + member), advanced together: every stage kernel is one
+    launch over all members and panels, the stages are captured hipGraphs,
+    each member has its own device CFL dt.  precision float32: the FP32
+    form (FP32 pair state, primitives, fluxes and sweeps; FP64 EOS,
+    gathers, seams, update and budgets); float64: FP64 throughout.
+    Returns the throughput, the per-kernel HIP-event times and the
+    roofline."""
     import torch
 
     from snapy_amd import problems
@@ -402,7 +405,9 @@ def cubed_leg(steps=50, warmup=3, members=32, seed0=1000):
 
     mo = MeshOptions.from_yaml(os.path.join(ROOT, "snapy_amd", "decks", "swe_cubed.yaml"))
     mo.block.intg.execution = "hip-graph"
+    mo.block.hydro.precision = precision
     mesh = CubedSphereMesh(mo, members=members).to("cuda")
+    fp32 = precision == "float32"
     vars = [{"hydro_w": w} for m in range(members) for w in problems.swe_hill(mesh, seed0 + m)]
     mesh.initialize(vars)
     m0 = mesh.mass().cpu()
@@ -436,17 +441,22 @@ def cubed_leg(steps=50, warmup=3, members=32, seed0=1000):
     nc = mesh.nc
     blk = members * 6
     # algorithmic bytes per launch: each input read once, each output written
-    # once (FP64, 4 rows)
+    # once (4 rows; FP64, or in the FP32 form 4-byte words with the pair's
+    # two words per state value)
+    sw = 4 if fp32 else 8  # primitive / flux word
+    su = 8  # one state value: FP64, or the FP32 pair
     kbytes = {
-        "cs_cons2prim": 2 * 4 * blk * nc * nc * 8,
-        "cs_sweep_x2": (4 * blk * nc * nc + 4 * blk * mesh.n * (mesh.n + 1)) * 8,
-        "cs_sweep_x3": (4 * blk * nc * nc + 4 * blk * mesh.n * (mesh.n + 1)) * 8,
-        "cs_update": (4 * 4 + 2 * 4) * cells * 8,  # u0, u, w, 2 face totals read; u written
+        "cs_cons2prim": 4 * blk * nc * nc * (su + sw),
+        "cs_sweep_x2": (4 * blk * nc * nc + 4 * blk * mesh.n * (mesh.n + 1)) * sw,
+        "cs_sweep_x3": (4 * blk * nc * nc + 4 * blk * mesh.n * (mesh.n + 1)) * sw,
+        # u0, u read and u written; w and the two face totals read
+        "cs_update": (3 * su + 3 * sw) * 4 * cells,
     }
     dom = max((k for k in kt if k in kbytes), key=kt.get)
     achieved = kbytes[dom] / (kt[dom] * 1e-3) / 1e9
-    b_alg = 4 * (3 * 8 + 2 * 8)
-    return {
+    # B_alg: nv (3 s_u + n_dim s_F) + k 2 nv s_c, nv = 4 stored rows
+    b_alg = 4 * (3 * 4 + 2 * 4) + 2 * 4 * 4 if fp32 else 4 * (3 * 8 + 2 * 8)
+    out = {
         "workload": "swe_cubed_sphere_6x192x192_x32",
         "value": value,
         "unit": "cell-updates/s",
@@ -454,8 +464,10 @@ def cubed_leg(steps=50, warmup=3, members=32, seed0=1000):
         "ms_per_step": el / steps * 1e3,
         "members": members,
         "cells_per_member": 6 * mesh.n * mesh.n,
-        "dtype": "f64",
-        "form": "FP64, every member and panel in each launch, captured stages, per-member device dt",
+        "dtype": "f32" if fp32 else "f64",
+        "form": ("FP32 form: FP32 pair state, primitives, fluxes and sweeps; FP64 EOS, ghost gather, "
+                 "seams, update and budgets" if fp32 else "FP64")
+        + "; every member and panel in each launch, captured stages, per-member device dt",
         "kernel_ms": kt,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS},
@@ -464,6 +476,8 @@ def cubed_leg(steps=50, warmup=3, members=32, seed0=1000):
         "mass_drift_max": float(((m1 - m0).abs() / m0.abs()).max()),
         "state_finite": bool(torch.isfinite(mesh.interior_state()).all().item()),
     }
+    mesh.release()
+    return out
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
@@ -644,6 +658,12 @@ def main():
         print("# batched cubed-sphere shallow water", file=sys.stderr, flush=True)
         cubed = cubed_leg(steps=args.cubed_steps)
         print(f"# cubed {cubed['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        c64 = cubed_leg(steps=args.cubed_steps, precision="float64")
+        cubed["fp64_form"] = {k: c64[k] for k in ("value", "ms_per_step", "dtype", "stage_frac", "roofline",
+                                                  "mass_drift_max")}
+        cubed["speedup_vs_fp64"] = cubed["value"] / c64["value"]
+        print(f"# cubed FP64 {c64['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     metric = "cell-updates/s per RK3 stage, FP32-batched vs FP64-plain; p99 request latency"
     line = {

@@ -571,6 +571,38 @@ int snapy_cs_update(const snapy_cs_mesh *m, const double *u0, double *u,
 int snapy_cs_cfl(const snapy_cs_mesh *m, const double *w, double scale,
                  double *dt_out, void *stream);
 
+/* ---- the FP32 form of the cubed-sphere stage ----------------------------
+ *
+ * The conserved state is the FP32 pair (u, c), whose value is
+ * double(u) + double(c) (the compensated RK accumulation); the primitives,
+ * the face fluxes and the edge states are FP32 and the sweeps compute in
+ * FP32.  The EOS, the ghost gather, the seam fluxes, the divergence, the
+ * sources and the RK combine compute in FP64 from the FP32 operands and
+ * round the result into the pair, so mass is conserved to FP64 rounding.
+ * Each entry replaces the same reference seam as its FP64 counterpart
+ * above (the reference has no FP32 form of the shallow-water or gnomonic
+ * path, src/hydro/hydro.cpp:120-139). */
+int snapy_cs_cons2prim_f32(const snapy_cs_mesh *m, const float *u,
+                           const float *c, float *w, void *stream);
+/* FP64 primitives w -> the pair (u, c). */
+int snapy_cs_prim2cons_f32(const snapy_cs_mesh *m, const double *w, float *u,
+                           float *c, void *stream);
+/* The panel-edge ghost fill of the conserved pair, gathered and
+ * interpolated in FP64. */
+int snapy_cs_ghosts_f32(const snapy_cs_mesh *m, float *u, float *c,
+                        void *stream);
+int snapy_cs_sweep_f32(const snapy_cs_mesh *m, int dim, const float *w,
+                       float *flux, float *edge, void *stream);
+int snapy_cs_seams_f32(const snapy_cs_mesh *m, float *flux2, float *flux3,
+                       const float *edge, void *stream);
+int snapy_cs_update_f32(const snapy_cs_mesh *m, const float *u0,
+                        const float *c0, float *u, float *c, const float *w,
+                        const float *flux2, const float *flux3, double dt,
+                        const double *dt_dev, double w0, double w1, double w2,
+                        void *stream);
+int snapy_cs_cfl_f32(const snapy_cs_mesh *m, const float *w, double scale,
+                     double *dt_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -186,6 +186,13 @@ SIGNATURES = {
     "snapy_cs_seams": (_I, [_MP, _P, _P, _P, _P]),
     "snapy_cs_update": (_I, [_MP, _P, _P, _P, _P, _P, _D, _P, _D, _D, _D, _P]),
     "snapy_cs_cfl": (_I, [_MP, _P, _D, _P, _P]),
+    "snapy_cs_cons2prim_f32": (_I, [_MP, _P, _P, _P, _P]),
+    "snapy_cs_prim2cons_f32": (_I, [_MP, _P, _P, _P, _P]),
+    "snapy_cs_ghosts_f32": (_I, [_MP, _P, _P, _P]),
+    "snapy_cs_sweep_f32": (_I, [_MP, _I, _P, _P, _P, _P]),
+    "snapy_cs_seams_f32": (_I, [_MP, _P, _P, _P, _P]),
+    "snapy_cs_update_f32": (_I, [_MP, _P, _P, _P, _P, _P, _P, _P, _D, _P, _D, _D, _D, _P]),
+    "snapy_cs_cfl_f32": (_I, [_MP, _P, _D, _P, _P]),
     "snapy_tune_set": (_I, [_I, _GP, _I, _I]),
     "snapy_tune_get": (_I, [_I, _GP, _I]),
     "snapy_wl_counts": (_I, [ctypes.POINTER(WlBlock), ctypes.c_int32, ctypes.c_int32, _P, _P]),

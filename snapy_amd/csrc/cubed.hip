@@ -95,39 +95,57 @@ Dims dims_of(snapy_cs_mesh const& m) {
 // ---------------------------------------------------------------------------
 // EOS
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-    cs_cons2prim_kernel(double const* __restrict__ u, double* __restrict__ w,
-                        double const* __restrict__ cos_c, Dims d, int64_t ncell) {
-  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= ncell) return;
-  int64_t b = t / d.plane, c = t % d.plane;
-  double const* ub = u + b * d.block + c;
-  double* wb = w + b * d.block + c;
-  double h = ub[0];
-  double v1 = ub[d.plane] / h, v2 = ub[2 * d.plane] / h, v3 = ub[3 * d.plane] / h;
-  double cth = cos_c[c];
-  double sth2 = 1. - cth * cth;
-  wb[0] = h;
-  wb[d.plane] = v1;
-  wb[2 * d.plane] = v2 / sth2 - v3 * cth / sth2;
-  wb[3 * d.plane] = -v2 * cth / sth2 + v3 / sth2;
+// The FP32 form keeps the conserved state as the FP32 pair (u, c), value
+// double(u) + double(c), and the primitives, face fluxes and edge states in
+// FP32; the EOS, the ghost gather, the seams and the update compute in FP64.
+__device__ __forceinline__ double cs_val(double const* u, float const*, int64_t o) { return u[o]; }
+__device__ __forceinline__ double cs_val(float const* u, float const* c, int64_t o) {
+  return double(u[o]) + double(c[o]);
+}
+__device__ __forceinline__ void cs_put(double* u, float*, int64_t o, double v) { u[o] = v; }
+__device__ __forceinline__ void cs_put(float* u, float* c, int64_t o, double v) {
+  float h = float(v);
+  u[o] = h;
+  c[o] = float(v - double(h));
 }
 
+template <typename TU, typename TW>
 __global__ void __launch_bounds__(256)
-    cs_prim2cons_kernel(double const* __restrict__ w, double* __restrict__ u,
+    cs_cons2prim_kernel(TU const* __restrict__ u, float const* __restrict__ uc,
+                        TW* __restrict__ w, double const* __restrict__ cos_c, Dims d,
+                        int64_t ncell) {
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= ncell) return;
+  int64_t b = t / d.plane, c = t % d.plane;
+  int64_t o = b * d.block + c;
+  TW* wb = w + o;
+  double h = cs_val(u, uc, o);
+  double v1 = cs_val(u, uc, o + d.plane) / h, v2 = cs_val(u, uc, o + 2 * d.plane) / h,
+         v3 = cs_val(u, uc, o + 3 * d.plane) / h;
+  double cth = cos_c[c];
+  double sth2 = 1. - cth * cth;
+  wb[0] = TW(h);
+  wb[d.plane] = TW(v1);
+  wb[2 * d.plane] = TW(v2 / sth2 - v3 * cth / sth2);
+  wb[3 * d.plane] = TW(-v2 * cth / sth2 + v3 / sth2);
+}
+
+template <typename TU>
+__global__ void __launch_bounds__(256)
+    cs_prim2cons_kernel(double const* __restrict__ w, TU* __restrict__ u, float* __restrict__ uc,
                         double const* __restrict__ cos_c, Dims d, int64_t ncell) {
   int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= ncell) return;
   int64_t b = t / d.plane, c = t % d.plane;
-  double const* wb = w + b * d.block + c;
-  double* ub = u + b * d.block + c;
+  int64_t o = b * d.block + c;
+  double const* wb = w + o;
   double h = wb[0];
   double m1 = wb[d.plane] * h, m2 = wb[2 * d.plane] * h, m3 = wb[3 * d.plane] * h;
   double cth = cos_c[c];
-  ub[0] = h;
-  ub[d.plane] = m1;
-  ub[2 * d.plane] = m2 + m3 * cth;
-  ub[3 * d.plane] = m3 + m2 * cth;
+  cs_put(u, uc, o, h);
+  cs_put(u, uc, o + d.plane, m1);
+  cs_put(u, uc, o + 2 * d.plane, m2 + m3 * cth);
+  cs_put(u, uc, o + 3 * d.plane, m3 + m2 * cth);
 }
 
 // ---------------------------------------------------------------------------
@@ -142,8 +160,11 @@ __device__ __forceinline__ void matvec(double const* m, double const* v, double*
 
 // one thread per (member, ghost entry): npts 2 (linear), 4 (cubic) or 1 (a
 // corner cell: a plain copy of the nearest interior cell)
+// TU double: the FP64 state; float: the FP32 pair (u, uc), gathered and
+// interpolated in FP64 (the same arithmetic on the pair's values)
+template <typename TU>
 __global__ void __launch_bounds__(256)
-    cs_ghost_kernel(double* __restrict__ u, int32_t const* __restrict__ idx,
+    cs_ghost_kernel(TU* __restrict__ u, float* __restrict__ uc, int32_t const* __restrict__ idx,
                     double const* __restrict__ wt, double const* __restrict__ mat,
                     int64_t nent, Dims d) {
   int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -152,19 +173,23 @@ __global__ void __launch_bounds__(256)
   int64_t m = t / nent;
   int32_t const* ix = idx + 8 * e;
   int npts = ix[7];
-  double* dst = u + (m * 6 + ix[0]) * d.block + ix[1];
-  double const* src = u + (m * 6 + ix[2]) * d.block;
+  int64_t dst = (m * 6 + ix[0]) * d.block + ix[1];
+  int64_t src = (m * 6 + ix[2]) * d.block;
   if (npts == 1) {
-    for (int v = 0; v < kRows; ++v) dst[v * d.plane] = src[v * d.plane + ix[3]];
+    for (int v = 0; v < kRows; ++v) {
+      u[dst + v * d.plane] = u[src + v * d.plane + ix[3]];
+      if constexpr (sizeof(TU) == 4) uc[dst + v * d.plane] = uc[src + v * d.plane + ix[3]];
+    }
     return;
   }
   double const* M = mat + 45 * e;
   // each source point: phi, and its momentum to the geographic frame
   double b[4][kRows];
   for (int q = 0; q < npts; ++q) {
-    double const* s = src + ix[3 + q];
-    double v[3] = {s[d.plane], s[2 * d.plane], s[3 * d.plane]};
-    b[q][0] = s[0];
+    int64_t s = src + ix[3 + q];
+    double v[3] = {cs_val(u, uc, s + d.plane), cs_val(u, uc, s + 2 * d.plane),
+                   cs_val(u, uc, s + 3 * d.plane)};
+    b[q][0] = cs_val(u, uc, s);
     matvec(M + 9 * q, v, &b[q][1]);
   }
   double g[kRows];
@@ -180,10 +205,10 @@ __global__ void __launch_bounds__(256)
   }
   double o[3];
   matvec(M + 36, &g[1], o);
-  dst[0] = g[0];
-  dst[d.plane] = o[0];
-  dst[2 * d.plane] = o[1];
-  dst[3 * d.plane] = o[2];
+  cs_put(u, uc, dst, g[0]);
+  cs_put(u, uc, dst + d.plane, o[0]);
+  cs_put(u, uc, dst + 2 * d.plane, o[1]);
+  cs_put(u, uc, dst + 3 * d.plane, o[2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -191,22 +216,23 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // shallow_roe_impl (riemann_kernels.h:447-495) with shallow_axes dir yz
 // (:44-47): normal row ivx, tangential ivy, the radial row ivz gets no flux
-__device__ __forceinline__ void roe_yz(double const* wl, double const* wr, int ivx,
-                                       int ivy, int ivz, double* flx) {
-  double sqrtdl = sqrt(wl[0]);
-  double sqrtdr = sqrt(wr[0]);
-  double isdlpdr = 1. / (sqrtdl + sqrtdr);
-  double ubar = (wl[ivx] * sqrtdl + wr[ivx] * sqrtdr) * isdlpdr;
-  double vbar = (wl[ivy] * sqrtdl + wr[ivy] * sqrtdr) * isdlpdr;
-  double cbar = sqrt(0.5 * (wl[0] + wr[0]));
-  double del0 = wr[0] - wl[0];
-  double delx = wr[ivx] - wl[ivx];
-  double dely = wr[ivy] - wl[ivy];
-  double hbar = sqrt(wl[0] * wr[0]);
-  double a1 = 0.5 * (cbar * del0 - hbar * delx) / cbar;
-  double a2 = hbar * dely;
-  double a3 = 0.5 * (cbar * del0 + hbar * delx) / cbar;
-  double wave0[4] = {0, 0, 0, 0}, wave1[4] = {0, 0, 0, 0}, wave2[4] = {0, 0, 0, 0};
+template <typename T>
+__device__ __forceinline__ void roe_yz(T const* wl, T const* wr, int ivx, int ivy, int ivz,
+                                       T* flx) {
+  T sqrtdl = qsqrt(wl[0]);
+  T sqrtdr = qsqrt(wr[0]);
+  T isdlpdr = qdiv(T(1), sqrtdl + sqrtdr);
+  T ubar = (wl[ivx] * sqrtdl + wr[ivx] * sqrtdr) * isdlpdr;
+  T vbar = (wl[ivy] * sqrtdl + wr[ivy] * sqrtdr) * isdlpdr;
+  T cbar = qsqrt(T(0.5) * (wl[0] + wr[0]));
+  T del0 = wr[0] - wl[0];
+  T delx = wr[ivx] - wl[ivx];
+  T dely = wr[ivy] - wl[ivy];
+  T hbar = qsqrt(wl[0] * wr[0]);
+  T a1 = qdiv(T(0.5) * (cbar * del0 - hbar * delx), cbar);
+  T a2 = hbar * dely;
+  T a3 = qdiv(T(0.5) * (cbar * del0 + hbar * delx), cbar);
+  T wave0[4] = {0, 0, 0, 0}, wave1[4] = {0, 0, 0, 0}, wave2[4] = {0, 0, 0, 0};
   wave0[0] = a1;
   wave0[ivx] = a1 * (ubar - cbar);
   wave0[ivy] = a1 * vbar;
@@ -214,62 +240,61 @@ __device__ __forceinline__ void roe_yz(double const* wl, double const* wr, int i
   wave2[0] = a3;
   wave2[ivx] = a3 * (ubar + cbar);
   wave2[ivy] = a3 * vbar;
-  double speed0 = fabs(ubar - cbar);
-  double speed1 = fabs(ubar);
-  double speed2 = fabs(ubar + cbar);
-  flx[0] = 0.5 * (wl[0] * wl[ivx] + wr[0] * wr[ivx]);
-  flx[ivx] = 0.5 * (wl[0] * wl[ivx] * wl[ivx] + 0.5 * wl[0] * wl[0] +
-                    wr[0] * wr[ivx] * wr[ivx] + 0.5 * wr[0] * wr[0]);
-  flx[ivy] = 0.5 * (wl[0] * wl[ivx] * wl[ivy] + wr[0] * wr[ivx] * wr[ivy]);
-  flx[ivz] = 0.;
+  T speed0 = fabs(ubar - cbar);
+  T speed1 = fabs(ubar);
+  T speed2 = fabs(ubar + cbar);
+  flx[0] = T(0.5) * (wl[0] * wl[ivx] + wr[0] * wr[ivx]);
+  flx[ivx] = T(0.5) * (wl[0] * wl[ivx] * wl[ivx] + T(0.5) * wl[0] * wl[0] +
+                       wr[0] * wr[ivx] * wr[ivx] + T(0.5) * wr[0] * wr[0]);
+  flx[ivy] = T(0.5) * (wl[0] * wl[ivx] * wl[ivy] + wr[0] * wr[ivx] * wr[ivy]);
+  flx[ivz] = T(0);
   for (int v = 0; v < 4; ++v)
-    flx[v] = flx[v] - 0.5 * (speed0 * wave0[v] + speed1 * wave1[v] + speed2 * wave2[v]);
+    flx[v] = flx[v] - T(0.5) * (speed0 * wave0[v] + speed1 * wave1[v] + speed2 * wave2[v]);
 }
 
 // the solve at one face of axis AX (2 = x2, 3 = x3) with the face's metric
 // cosine c and sine s: prim2local, the solver, flux2global
-template <int AX>
-__device__ __forceinline__ void face_flux(double* wl, double* wr, double c, double s,
-                                          double* f) {
-  double gi = 1.0 / (s * s);
+template <int AX, typename T>
+__device__ __forceinline__ void face_flux(T* wl, T* wr, T c, T s, T* f) {
+  T gi = qdiv(T(1), s * s);
   if (AX == 2) {
-    double t22 = 1.0 / sqrt(gi);
-    for (double* w : {wl, wr}) {
-      double uu2 = w[2], uu3 = w[3];
-      w[3] = c * uu2 + 1.0 * uu3;
+    T t22 = qdiv(T(1), qsqrt(gi));
+    for (T* w : {wl, wr}) {
+      T uu2 = w[2], uu3 = w[3];
+      w[3] = c * uu2 + T(1) * uu3;
       w[2] = t22 * uu2;
     }
-    roe_yz(wl, wr, 2, 3, 1, f);
-    double sg = sqrt(gi);
-    double t32 = -sg * c / 1.0;
-    double txy = f[2], txz = f[3];
-    f[3] = t32 * txy + 1.0 * txz;
+    roe_yz<T>(wl, wr, 2, 3, 1, f);
+    T sg = qsqrt(gi);
+    T t32 = qdiv(-sg * c, T(1));
+    T txy = f[2], txz = f[3];
+    f[3] = t32 * txy + T(1) * txz;
     f[2] = sg * txy;
   } else {
-    double t33 = 1.0 / sqrt(gi);
-    for (double* w : {wl, wr}) {
-      double uu2 = w[2], uu3 = w[3];
-      w[2] = 1.0 * uu2 + c * uu3;
+    T t33 = qdiv(T(1), qsqrt(gi));
+    for (T* w : {wl, wr}) {
+      T uu2 = w[2], uu3 = w[3];
+      w[2] = T(1) * uu2 + c * uu3;
       w[3] = t33 * uu3;
     }
-    roe_yz(wl, wr, 3, 2, 1, f);
-    double sg = sqrt(gi);
-    double t23 = -c / 1.0 * sg;
-    double txy = f[2], txz = f[3];
-    f[2] = 1.0 * txy + t23 * txz;
+    roe_yz<T>(wl, wr, 3, 2, 1, f);
+    T sg = qsqrt(gi);
+    T t23 = qdiv(-c, T(1)) * sg;
+    T txy = f[2], txz = f[3];
+    f[2] = T(1) * txy + t23 * txz;
     f[3] = sg * txz;
   }
-  double ty = f[2], tz = f[3];
+  T ty = f[2], tz = f[3];
   f[2] = ty + tz * c;
   f[3] = tz + ty * c;
 }
 
 // one thread per interior face (x2: faces ng..ng+n of the interior rows; x3:
 // the transpose); edge faces also store the interior-side state
-template <int AX>
+template <int AX, typename T>
 __global__ void __launch_bounds__(256)
-    cs_sweep_kernel(double const* __restrict__ w, double* __restrict__ flx,
-                    double* __restrict__ edge, Geom gm, Dims d, int shock) {
+    cs_sweep_kernel(T const* __restrict__ w, T* __restrict__ flx,
+                    T* __restrict__ edge, Geom gm, Dims d, int shock) {
   // x3: neighbouring face lines share five of their six stencil rows, so the
   // workgroups go through the XCD map (each XCD then sweeps a contiguous
   // range of lines and the shared rows stay in its L2)
@@ -302,12 +327,12 @@ __global__ void __launch_bounds__(256)
     stride = d.nc;
   }
   int64_t c = int64_t(k) * d.nc + j;
-  double const* wb = w + b * d.block + c;
-  double wl[kRows], wr[kRows];
+  T const* wb = w + b * d.block + c;
+  T wl[kRows], wr[kRows];
   for (int v = 0; v < kRows; ++v) {
-    double const* p = wb + v * d.plane;
-    double c0 = p[-3 * stride], c1 = p[-2 * stride], c2 = p[-stride], c3 = p[0],
-           c4 = p[stride], c5 = p[2 * stride];
+    T const* p = wb + v * d.plane;
+    T c0 = p[-3 * stride], c1 = p[-2 * stride], c2 = p[-stride], c3 = p[0],
+      c4 = p[stride], c5 = p[2 * stride];
     if (v == 0 || shock) {
       wl[v] = weno5_plus(c0, c1, c2, c3, c4);
       wr[v] = weno5_minus(c1, c2, c3, c4, c5);
@@ -319,8 +344,8 @@ __global__ void __launch_bounds__(256)
   if (f == 0 || f == d.n) {
     // sides L/R (x2) or B/T (x3): the state on the panel's own side
     int side = (AX == 2 ? 0 : 2) + (f == d.n ? 1 : 0);
-    double* e = edge + ((b * 4 + side) * int64_t(d.n) + a) * kRows;
-    double const* own = f == d.n ? wl : wr;
+    T* e = edge + ((b * 4 + side) * int64_t(d.n) + a) * kRows;
+    T const* own = f == d.n ? wl : wr;
     for (int v = 0; v < kRows; ++v) e[v] = own[v];
   }
   double cf, sf;
@@ -333,9 +358,9 @@ __global__ void __launch_bounds__(256)
     cf = gm.cos_f3[q];
     sf = gm.sin_f3[q];
   }
-  double out[kRows];
-  face_flux<AX>(wl, wr, cf, sf, out);
-  double* fb = flx + b * d.block + c;
+  T out[kRows];
+  face_flux<AX, T>(wl, wr, T(cf), T(sf), out);
+  T* fb = flx + b * d.block + c;
   for (int v = 0; v < kRows; ++v) fb[v * d.plane] = out[v];
 }
 
@@ -345,9 +370,10 @@ __global__ void __launch_bounds__(256)
 // one thread per (member, seam face): the owner's flux from its own state and
 // the neighbour's (through the two frame matrices), then the neighbour's copy
 // through the global Cartesian frame
+template <typename T>
 __global__ void __launch_bounds__(256)
-    cs_seam_kernel(double* __restrict__ f2, double* __restrict__ f3,
-                   double const* __restrict__ edge, int32_t const* __restrict__ idx,
+    cs_seam_kernel(T* __restrict__ f2, T* __restrict__ f3,
+                   T const* __restrict__ edge, int32_t const* __restrict__ idx,
                    double const* __restrict__ sd, int64_t nseam, Geom gm, Dims d) {
   int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= nseam * d.members) return;
@@ -357,8 +383,8 @@ __global__ void __launch_bounds__(256)
   double sign = double(ix[6]);
   double const* D = sd + 30 * e;
   int64_t ba = m * 6 + pa, bb = m * 6 + pb;
-  double const* own = edge + ((ba * 4 + sa) * int64_t(d.n) + ia) * kRows;
-  double const* oth = edge + ((bb * 4 + sb) * int64_t(d.n) + ib) * kRows;
+  T const* own = edge + ((ba * 4 + sa) * int64_t(d.n) + ia) * kRows;
+  T const* oth = edge + ((bb * 4 + sb) * int64_t(d.n) + ib) * kRows;
   double mine[kRows], other[kRows];
   for (int v = 0; v < kRows; ++v) mine[v] = own[v];
   {
@@ -379,16 +405,16 @@ __global__ void __launch_bounds__(256)
   if (sa < 2) {
     int k = d.ng + ia;
     int64_t q = int64_t(k) * (d.nc + 1) + plane_a;
-    face_flux<2>(wl, wr, gm.cos_f2[q], gm.sin_f2[q], F);
+    face_flux<2, double>(wl, wr, gm.cos_f2[q], gm.sin_f2[q], F);
     ca = int64_t(k) * d.nc + plane_a;
   } else {
     int j = d.ng + ia;
     int64_t q = int64_t(plane_a) * d.nc + j;
-    face_flux<3>(wl, wr, gm.cos_f3[q], gm.sin_f3[q], F);
+    face_flux<3, double>(wl, wr, gm.cos_f3[q], gm.sin_f3[q], F);
     ca = int64_t(plane_a) * d.nc + j;
   }
-  double* fa = (sa < 2 ? f2 : f3) + ba * d.block + ca;
-  for (int v = 0; v < kRows; ++v) fa[v * d.plane] = F[v];
+  T* fa = (sa < 2 ? f2 : f3) + ba * d.block + ca;
+  for (int v = 0; v < kRows; ++v) fa[v * d.plane] = T(F[v]);
 
   // the owner's rows: raise with its face cosine, to Cartesian (z, x, y)
   double const* A = D + 18;  // x, y, delta, C, D, cos at the owner's face
@@ -432,11 +458,11 @@ __global__ void __launch_bounds__(256)
   int plane_b = sb & 1 ? d.ng + d.n : d.ng;
   int64_t cbi = sb < 2 ? int64_t(d.ng + ib) * d.nc + plane_b
                        : int64_t(plane_b) * d.nc + (d.ng + ib);
-  double* fb = (sb < 2 ? f2 : f3) + bb * d.block + cbi;
-  fb[0] = g0;
-  fb[d.plane] = o1;
-  fb[2 * d.plane] = l2;
-  fb[3 * d.plane] = l3;
+  T* fb = (sb < 2 ? f2 : f3) + bb * d.block + cbi;
+  fb[0] = T(g0);
+  fb[d.plane] = T(o1);
+  fb[2 * d.plane] = T(l2);
+  fb[3 * d.plane] = T(l3);
 }
 
 // ---------------------------------------------------------------------------
@@ -449,10 +475,14 @@ struct CsUpdArgs {
   double radius;         // x1v
 };
 
+// TU: the state (double, or the FP32 pair with u0c / uc); TW: the
+// primitives and fluxes; the divergence, sources and combine in FP64
+template <typename TU, typename TW>
 __global__ void __launch_bounds__(256)
-    cs_update_kernel(double const* __restrict__ u0, double* __restrict__ u,
-                     double const* __restrict__ w, double const* __restrict__ f2,
-                     double const* __restrict__ f3, Geom gm, Dims d, CsUpdArgs a) {
+    cs_update_kernel(TU const* __restrict__ u0, float const* __restrict__ u0c,
+                     TU* __restrict__ u, float* __restrict__ uc,
+                     TW const* __restrict__ w, TW const* __restrict__ f2,
+                     TW const* __restrict__ f3, Geom gm, Dims d, CsUpdArgs a) {
   int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t per = int64_t(d.n) * d.n;
   if (t >= per * 6 * d.members) return;
@@ -463,7 +493,7 @@ __global__ void __launch_bounds__(256)
   int64_t c2 = int64_t(k) * (d.nc + 1) + j;  // x2 face j of row k in area2
   double dt = a.dt_dev ? a.dt_dev[b / 6] : a.dt;
   double vol = gm.vol[c];
-  double const* wb = w + b * d.block + c;
+  TW const* wb = w + b * d.block + c;
   double phi = wb[0], v1 = wb[d.plane], v2 = wb[2 * d.plane], v3 = wb[3 * d.plane];
   double cth = gm.cos_c[c];
   double sine2 = gm.sin_c[c] * gm.sin_c[c];
@@ -473,8 +503,8 @@ __global__ void __launch_bounds__(256)
   double a3l = gm.area3[c], a3r = gm.area3[c + d.nc];
   for (int v = 0; v < kRows; ++v) {
     int64_t o = b * d.block + v * d.plane + c;
-    double dv = 0. + (a2r * f2[o + 1] - a2l * f2[o]);
-    dv = dv + (a3r * f3[o + d.nc] - a3l * f3[o]);
+    double dv = 0. + (a2r * double(f2[o + 1]) - a2l * double(f2[o]));
+    dv = dv + (a3r * double(f3[o + d.nc]) - a3l * double(f3[o]));
     dv = dv / vol;
     if (v == 2) dv = dv - gm.xr[c] * (pr + phi * v3 * v3 * sine2);
     if (v == 3) dv = dv - gm.yr[c] * (pr + phi * v2 * v2 * sine2);
@@ -486,7 +516,7 @@ __global__ void __launch_bounds__(256)
       du = du + (0. + mdt * (v3 + v2 * cth));
     else
       du = du + 0.;
-    u[o] = a.w0 * u0[o] + a.w1 * u[o] + a.w2 * du;
+    cs_put(u, uc, o, a.w0 * cs_val(u0, u0c, o) + a.w1 * cs_val(u, uc, o) + a.w2 * du);
   }
 }
 
@@ -503,8 +533,9 @@ __global__ void cs_cfl_init_kernel(unsigned long long* out, int members) {
   }
 }
 
+template <typename TW>
 __global__ void __launch_bounds__(256)
-    cs_cfl_kernel(double const* __restrict__ w, Geom gm, Dims d,
+    cs_cfl_kernel(TW const* __restrict__ w, Geom gm, Dims d,
                   unsigned long long* __restrict__ out) {
   // one block row of 256 threads per (member, chunk); blockIdx.y = member
   int m = blockIdx.y;
@@ -515,10 +546,10 @@ __global__ void __launch_bounds__(256)
     int64_t p = t / (int64_t(d.n) * d.n);
     int r = int(t % (int64_t(d.n) * d.n));
     int64_t c = int64_t(d.ng + r / d.n) * d.nc + (d.ng + r % d.n);
-    double const* wb = w + (int64_t(m) * 6 + p) * d.block + c;
-    double cs = sqrt(wb[0]);
-    best = fmin(best, gm.w2[c] / (fabs(wb[2 * d.plane]) + cs));
-    best = fmin(best, gm.w3[c] / (fabs(wb[3 * d.plane]) + cs));
+    TW const* wb = w + (int64_t(m) * 6 + p) * d.block + c;
+    double cs = sqrt(double(wb[0]));
+    best = fmin(best, gm.w2[c] / (fabs(double(wb[2 * d.plane])) + cs));
+    best = fmin(best, gm.w3[c] / (fabs(double(wb[3 * d.plane])) + cs));
   }
   for (int off = 32; off > 0; off >>= 1) best = fmin(best, __shfl_down(best, off, 64));
   if ((threadIdx.x & 63) == 0) {
@@ -548,6 +579,65 @@ int check_mesh(snapy_cs_mesh const* m) {
 
 using namespace snapy;
 
+namespace {
+
+template <typename T>
+int cs_sweep_launch(const snapy_cs_mesh* m, int dim, const T* w, T* flux, T* edge,
+                           void* stream) {
+  if (int rc = check_mesh(m)) return rc;
+  if (dim != 2 && dim != 3) return fail(-8, "cubed sphere: sweeps are x2 and x3");
+  if (!w || !flux || !edge) return fail(-4, "cubed sphere: null sweep buffer");
+  Dims d = dims_of(*m);
+  Geom gm = geom_of(m->geom, d.nc);
+  int64_t n = int64_t(d.n) * (d.n + 1) * 6 * d.members;
+  if (dim == 2)
+    hipLaunchKernelGGL((cs_sweep_kernel<2, T>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                       as_stream(stream), w, flux, edge, gm, d, m->shock);
+  else
+    hipLaunchKernelGGL((cs_sweep_kernel<3, T>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                       as_stream(stream), w, flux, edge, gm, d, m->shock);
+  return launch_status("cs_sweep");
+}
+
+template <typename T>
+int cs_seams_launch(const snapy_cs_mesh* m, T* flux2, T* flux3, const T* edge,
+                           void* stream) {
+  if (int rc = check_mesh(m)) return rc;
+  if (!m->seam_idx || !m->seam_d) return fail(-4, "cubed sphere: null seam tables");
+  if (!flux2 || !flux3 || !edge) return fail(-4, "cubed sphere: null seam buffer");
+  Dims d = dims_of(*m);
+  Geom gm = geom_of(m->geom, d.nc);
+  int64_t nseam = int64_t(12) * d.n;
+  int64_t n = nseam * d.members;
+  hipLaunchKernelGGL(cs_seam_kernel<T>, dim3(blocks_for(n, 256)), dim3(256), 0,
+                     as_stream(stream), flux2, flux3, edge, m->seam_idx, m->seam_d, nseam,
+                     gm, d);
+  return launch_status("cs_seams");
+}
+
+template <typename TW>
+int cs_cfl_launch(const snapy_cs_mesh* m, const TW* w, double scale, double* dt_out,
+                         void* stream) {
+  if (int rc = check_mesh(m)) return rc;
+  if (!dt_out || !w) return fail(-4, "cubed sphere: null dt output or primitives");
+  Dims d = dims_of(*m);
+  Geom gm = geom_of(m->geom, d.nc);
+  auto* bits = reinterpret_cast<unsigned long long*>(dt_out);
+  unsigned mb = blocks_for(d.members, 64);
+  hipLaunchKernelGGL(cs_cfl_init_kernel, dim3(mb), dim3(64), 0, as_stream(stream), bits,
+                     d.members);
+  int64_t per = int64_t(d.n) * d.n * 6;
+  unsigned gx = blocks_for(per, 256);
+  if (gx > 64) gx = 64;
+  hipLaunchKernelGGL(cs_cfl_kernel<TW>, dim3(gx, d.members), dim3(256), 0, as_stream(stream), w,
+                     gm, d, bits);
+  hipLaunchKernelGGL(cs_cfl_scale_kernel, dim3(mb), dim3(64), 0, as_stream(stream), dt_out,
+                     scale, d.members);
+  return launch_status("cs_cfl");
+}
+
+}  // namespace
+
 extern "C" {
 
 int64_t snapy_cs_geom_size(int32_t n, int32_t nghost) {
@@ -564,8 +654,8 @@ int snapy_cs_cons2prim(const snapy_cs_mesh* m, const double* u, double* w, void*
   if (int rc = check_mesh(m)) return rc;
   Dims d = dims_of(*m);
   int64_t n = d.plane * 6 * d.members;
-  hipLaunchKernelGGL(cs_cons2prim_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-                     as_stream(stream), u, w, m->geom, d, n);
+  hipLaunchKernelGGL((cs_cons2prim_kernel<double, double>), dim3(blocks_for(n, 256)), dim3(256),
+                     0, as_stream(stream), u, nullptr, w, m->geom, d, n);
   return launch_status("cs_cons2prim");
 }
 
@@ -573,8 +663,8 @@ int snapy_cs_prim2cons(const snapy_cs_mesh* m, const double* w, double* u, void*
   if (int rc = check_mesh(m)) return rc;
   Dims d = dims_of(*m);
   int64_t n = d.plane * 6 * d.members;
-  hipLaunchKernelGGL(cs_prim2cons_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-                     as_stream(stream), w, u, m->geom, d, n);
+  hipLaunchKernelGGL(cs_prim2cons_kernel<double>, dim3(blocks_for(n, 256)), dim3(256), 0,
+                     as_stream(stream), w, u, nullptr, m->geom, d, n);
   return launch_status("cs_prim2cons");
 }
 
@@ -584,41 +674,22 @@ int snapy_cs_ghosts(const snapy_cs_mesh* m, int conserved, double* q, void* stre
     return fail(-4, "cubed sphere: null ghost tables");
   Dims d = dims_of(*m);
   int64_t n = m->nghost_entries * d.members;
-  hipLaunchKernelGGL(cs_ghost_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-                     as_stream(stream), q, m->ghost_idx, m->ghost_w,
+  hipLaunchKernelGGL(cs_ghost_kernel<double>, dim3(blocks_for(n, 256)), dim3(256), 0,
+                     as_stream(stream), q, nullptr, m->ghost_idx, m->ghost_w,
                      conserved ? m->ghost_mat_c : m->ghost_mat_p, m->nghost_entries, d);
   return launch_status("cs_ghosts");
 }
 
+
 int snapy_cs_sweep(const snapy_cs_mesh* m, int dim, const double* w, double* flux,
                    double* edge, void* stream) {
-  if (int rc = check_mesh(m)) return rc;
-  if (dim != 2 && dim != 3) return fail(-8, "cubed sphere: sweeps are x2 and x3");
-  if (!edge) return fail(-4, "cubed sphere: null edge buffer");
-  Dims d = dims_of(*m);
-  Geom gm = geom_of(m->geom, d.nc);
-  int64_t n = int64_t(d.n) * (d.n + 1) * 6 * d.members;
-  if (dim == 2)
-    hipLaunchKernelGGL(cs_sweep_kernel<2>, dim3(blocks_for(n, 256)), dim3(256), 0,
-                       as_stream(stream), w, flux, edge, gm, d, m->shock);
-  else
-    hipLaunchKernelGGL(cs_sweep_kernel<3>, dim3(blocks_for(n, 256)), dim3(256), 0,
-                       as_stream(stream), w, flux, edge, gm, d, m->shock);
-  return launch_status("cs_sweep");
+  return cs_sweep_launch<double>(m, dim, w, flux, edge, stream);
 }
+
 
 int snapy_cs_seams(const snapy_cs_mesh* m, double* flux2, double* flux3,
                    const double* edge, void* stream) {
-  if (int rc = check_mesh(m)) return rc;
-  if (!m->seam_idx || !m->seam_d) return fail(-4, "cubed sphere: null seam tables");
-  Dims d = dims_of(*m);
-  Geom gm = geom_of(m->geom, d.nc);
-  int64_t nseam = int64_t(12) * d.n;
-  int64_t n = nseam * d.members;
-  hipLaunchKernelGGL(cs_seam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-                     as_stream(stream), flux2, flux3, edge, m->seam_idx, m->seam_d, nseam,
-                     gm, d);
-  return launch_status("cs_seams");
+  return cs_seams_launch<double>(m, flux2, flux3, edge, stream);
 }
 
 int snapy_cs_update(const snapy_cs_mesh* m, const double* u0, double* u, const double* w,
@@ -629,29 +700,82 @@ int snapy_cs_update(const snapy_cs_mesh* m, const double* u0, double* u, const d
   Geom gm = geom_of(m->geom, d.nc);
   CsUpdArgs a{w0, w1, w2, dt, dt_dev, m->radius};
   int64_t n = int64_t(d.n) * d.n * 6 * d.members;
-  hipLaunchKernelGGL(cs_update_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-                     as_stream(stream), u0, u, w, flux2, flux3, gm, d, a);
+  hipLaunchKernelGGL((cs_update_kernel<double, double>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                     as_stream(stream), u0, nullptr, u, nullptr, w, flux2, flux3, gm, d, a);
   return launch_status("cs_update");
 }
 
+
 int snapy_cs_cfl(const snapy_cs_mesh* m, const double* w, double scale, double* dt_out,
                  void* stream) {
+  return cs_cfl_launch<double>(m, w, scale, dt_out, stream);
+}
+
+// ---- the FP32 form ----------------------------------------------------------
+int snapy_cs_cons2prim_f32(const snapy_cs_mesh* m, const float* u, const float* c, float* w,
+                           void* stream) {
   if (int rc = check_mesh(m)) return rc;
-  if (!dt_out) return fail(-4, "cubed sphere: null dt output");
+  if (!u || !c || !w) return fail(-4, "cubed sphere: null buffer");
+  Dims d = dims_of(*m);
+  int64_t n = d.plane * 6 * d.members;
+  hipLaunchKernelGGL((cs_cons2prim_kernel<float, float>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                     as_stream(stream), u, c, w, m->geom, d, n);
+  return launch_status("cs_cons2prim_f32");
+}
+
+int snapy_cs_prim2cons_f32(const snapy_cs_mesh* m, const double* w, float* u, float* c,
+                           void* stream) {
+  if (int rc = check_mesh(m)) return rc;
+  if (!u || !c || !w) return fail(-4, "cubed sphere: null buffer");
+  Dims d = dims_of(*m);
+  int64_t n = d.plane * 6 * d.members;
+  hipLaunchKernelGGL(cs_prim2cons_kernel<float>, dim3(blocks_for(n, 256)), dim3(256), 0,
+                     as_stream(stream), w, u, c, m->geom, d, n);
+  return launch_status("cs_prim2cons_f32");
+}
+
+int snapy_cs_ghosts_f32(const snapy_cs_mesh* m, float* u, float* c, void* stream) {
+  if (int rc = check_mesh(m)) return rc;
+  if (!m->ghost_idx || !m->ghost_w || !m->ghost_mat_c)
+    return fail(-4, "cubed sphere: null ghost tables");
+  if (!u || !c) return fail(-4, "cubed sphere: null buffer");
+  Dims d = dims_of(*m);
+  int64_t n = m->nghost_entries * d.members;
+  hipLaunchKernelGGL(cs_ghost_kernel<float>, dim3(blocks_for(n, 256)), dim3(256), 0,
+                     as_stream(stream), u, c, m->ghost_idx, m->ghost_w, m->ghost_mat_c,
+                     m->nghost_entries, d);
+  return launch_status("cs_ghosts_f32");
+}
+
+int snapy_cs_sweep_f32(const snapy_cs_mesh* m, int dim, const float* w, float* flux,
+                       float* edge, void* stream) {
+  return cs_sweep_launch<float>(m, dim, w, flux, edge, stream);
+}
+
+int snapy_cs_seams_f32(const snapy_cs_mesh* m, float* flux2, float* flux3, const float* edge,
+                       void* stream) {
+  return cs_seams_launch<float>(m, flux2, flux3, edge, stream);
+}
+
+int snapy_cs_update_f32(const snapy_cs_mesh* m, const float* u0, const float* c0, float* u,
+                        float* c, const float* w, const float* flux2, const float* flux3,
+                        double dt, const double* dt_dev, double w0, double w1, double w2,
+                        void* stream) {
+  if (int rc = check_mesh(m)) return rc;
+  if (!u0 || !c0 || !u || !c || !w || !flux2 || !flux3)
+    return fail(-4, "cubed sphere: null buffer");
   Dims d = dims_of(*m);
   Geom gm = geom_of(m->geom, d.nc);
-  auto* bits = reinterpret_cast<unsigned long long*>(dt_out);
-  unsigned mb = blocks_for(d.members, 64);
-  hipLaunchKernelGGL(cs_cfl_init_kernel, dim3(mb), dim3(64), 0, as_stream(stream), bits,
-                     d.members);
-  int64_t per = int64_t(d.n) * d.n * 6;
-  unsigned gx = blocks_for(per, 256);
-  if (gx > 64) gx = 64;
-  hipLaunchKernelGGL(cs_cfl_kernel, dim3(gx, d.members), dim3(256), 0, as_stream(stream), w,
-                     gm, d, bits);
-  hipLaunchKernelGGL(cs_cfl_scale_kernel, dim3(mb), dim3(64), 0, as_stream(stream), dt_out,
-                     scale, d.members);
-  return launch_status("cs_cfl");
+  CsUpdArgs a{w0, w1, w2, dt, dt_dev, m->radius};
+  int64_t n = int64_t(d.n) * d.n * 6 * d.members;
+  hipLaunchKernelGGL((cs_update_kernel<float, float>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                     as_stream(stream), u0, c0, u, c, w, flux2, flux3, gm, d, a);
+  return launch_status("cs_update_f32");
+}
+
+int snapy_cs_cfl_f32(const snapy_cs_mesh* m, const float* w, double scale, double* dt_out,
+                     void* stream) {
+  return cs_cfl_launch<float>(m, w, scale, dt_out, stream);
 }
 
 }  // extern "C"

@@ -467,6 +467,11 @@ class CubedSphereMesh:
         self.max_redo = 5
         self.density_floor = op.hydro.eos.density_floor
         self._w_stale = False
+        # dynamics/precision float32: the FP32 pair state, FP32 primitives,
+        # fluxes and edge states, FP32 sweeps; FP64 EOS, gathers, seams,
+        # update and budgets
+        self.fp32 = op.hydro.precision == "float32"
+        self._c = self._c0 = None
 
     # -- device set-up ------------------------------------------------------
     def to(self, device):
@@ -501,17 +506,21 @@ class CubedSphereMesh:
         lib = _capi.load()
         shape = (self.nblocks, NROWS, self.nc, self.nc, 1)
         z = dict(dtype=torch.float64, device=device)
-        self._f2 = torch.zeros(shape, **z)
-        self._f3 = torch.zeros(shape, **z)
-        self._edge = torch.zeros(int(lib.snapy_cs_edge_size(ctypes.byref(self._desc))), **z)
+        zf = dict(dtype=torch.float32 if self.fp32 else torch.float64, device=device)
+        self._f2 = torch.zeros(shape, **zf)
+        self._f3 = torch.zeros(shape, **zf)
+        self._edge = torch.zeros(int(lib.snapy_cs_edge_size(ctypes.byref(self._desc))), **zf)
         self._dt_dev = torch.zeros(self.members, **z)
         return self
 
     def hbm_footprint(self) -> int:
-        """Device bytes of the resident state: u, u0, w, two flux totals
-        (FP64), the edge buffer and the tables."""
+        """Device bytes of the resident state: u, u0, w, two flux totals, the
+        edge buffer and the tables (FP64; in the FP32 form the pair u, c and
+        its backup, and FP32 primitives, fluxes and edges)."""
         cells = self.nblocks * NROWS * self.nc * self.nc
         tables = sum(t.numel() * t.element_size() for t in (self._tables or {}).values())
+        if self.fp32:
+            return 4 * cells * 4 + 3 * cells * 4 + self.nblocks * 4 * self.n * NROWS * 4 + tables
         return 5 * cells * 8 + self.nblocks * 4 * self.n * NROWS * 8 + tables
 
     def _stream(self):
@@ -548,16 +557,27 @@ class CubedSphereMesh:
         w = torch.stack([v["hydro_w"].to(self.device, torch.float64) for v in vars]).contiguous()
         if tuple(w.shape[1:]) != self.block_shape():
             raise ValueError(f"initialize: hydro_w has shape {tuple(w.shape[1:])}, expected {self.block_shape()}")
-        self._w = w
-        self._u = torch.empty_like(w)
-        self._u0 = torch.empty_like(w)
-        self._state = _MeshState()
-        self._state._u0 = self._u0
         lib, d, s = _capi.load(), ctypes.byref(self._desc), self._stream()
         P = _capi.ptr
-        _capi.check(lib.snapy_cs_ghosts(d, 0, P(self._w), s), "cs_ghosts")
-        _capi.check(lib.snapy_cs_prim2cons(d, P(self._w), P(self._u), s), "cs_prim2cons")
-        _capi.check(lib.snapy_cs_ghosts(d, 1, P(self._u), s), "cs_ghosts")
+        self._state = _MeshState()
+        _capi.check(lib.snapy_cs_ghosts(d, 0, P(w), s), "cs_ghosts")
+        if self.fp32:
+            f32 = dict(dtype=torch.float32, device=self.device)
+            self._u = torch.empty(w.shape, **f32)
+            self._c = torch.empty(w.shape, **f32)
+            self._u0 = torch.empty(w.shape, **f32)
+            self._c0 = torch.empty(w.shape, **f32)
+            self._w = torch.empty(w.shape, **f32)
+            _capi.check(lib.snapy_cs_prim2cons_f32(d, P(w), P(self._u), P(self._c), s), "cs_prim2cons_f32")
+            _capi.check(lib.snapy_cs_ghosts_f32(d, P(self._u), P(self._c), s), "cs_ghosts_f32")
+            _capi.check(lib.snapy_cs_cons2prim_f32(d, P(self._u), P(self._c), P(self._w), s), "cs_cons2prim_f32")
+        else:
+            self._w = w
+            self._u = torch.empty_like(w)
+            self._u0 = torch.empty_like(w)
+            _capi.check(lib.snapy_cs_prim2cons(d, P(self._w), P(self._u), s), "cs_prim2cons")
+            _capi.check(lib.snapy_cs_ghosts(d, 1, P(self._u), s), "cs_ghosts")
+        self._state._u0 = self._u0
         for b, v in enumerate(vars):
             v["hydro_w"] = self._w[b]
             v["hydro_u"] = self._u[b]
@@ -569,6 +589,17 @@ class CubedSphereMesh:
         P = _capi.ptr
         w0, w1, w2 = ((1.0, 0.0, 1.0), (0.75, 0.25, 0.25), (1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0))[stage]
         u, w, u0, f2, f3, e = self._u, self._w, self._u0, self._f2, self._f3, self._edge
+        if self.fp32:
+            c, c0 = self._c, self._c0
+            return [
+                ("cs_cons2prim", lambda: lib.snapy_cs_cons2prim_f32(d, P(u), P(c), P(w), s)),
+                ("cs_sweep_x2", lambda: lib.snapy_cs_sweep_f32(d, 2, P(w), P(f2), P(e), s)),
+                ("cs_sweep_x3", lambda: lib.snapy_cs_sweep_f32(d, 3, P(w), P(f3), P(e), s)),
+                ("cs_seams", lambda: lib.snapy_cs_seams_f32(d, P(f2), P(f3), P(e), s)),
+                ("cs_update", lambda: lib.snapy_cs_update_f32(d, P(u0), P(c0), P(u), P(c), P(w), P(f2), P(f3),
+                                                             0.0, P(self._dt_dev), w0, w1, w2, s)),
+                ("cs_ghosts", lambda: lib.snapy_cs_ghosts_f32(d, P(u), P(c), s)),
+            ]
         return [
             ("cs_cons2prim", lambda: lib.snapy_cs_cons2prim(d, P(u), P(w), s)),
             ("cs_sweep_x2", lambda: lib.snapy_cs_sweep(d, 2, P(w), P(f2), P(e), s)),
@@ -582,6 +613,8 @@ class CubedSphereMesh:
     def _launch_stage(self, vars, stage: int) -> None:
         if stage == 0:
             self._u0.copy_(self._u)
+            if self._c is not None:
+                self._c0.copy_(self._c)
         for name, fn in self._stage_calls(stage):
             _capi.check(fn(), name)
 
@@ -599,8 +632,12 @@ class CubedSphereMesh:
         """w <- prim(u) on every cell of every panel (the reference's
         check_redo refreshes the primitives, meshblock_redo.cpp:107-112)."""
         lib = _capi.load()
-        _capi.check(lib.snapy_cs_cons2prim(ctypes.byref(self._desc), _capi.ptr(self._u), _capi.ptr(self._w),
-                                           self._stream()), "cs_cons2prim")
+        d, P = ctypes.byref(self._desc), _capi.ptr
+        if self.fp32:
+            _capi.check(lib.snapy_cs_cons2prim_f32(d, P(self._u), P(self._c), P(self._w), self._stream()),
+                        "cs_cons2prim_f32")
+        else:
+            _capi.check(lib.snapy_cs_cons2prim(d, P(self._u), P(self._w), self._stream()), "cs_cons2prim")
         self._w_stale = False
 
     def device_time_step(self, vars=None) -> torch.Tensor:
@@ -611,8 +648,9 @@ class CubedSphereMesh:
             self._refresh_primitives()
         lib = _capi.load()
         scale = math.pow(2.0, -self.current_redo) * self.cfl
-        _capi.check(lib.snapy_cs_cfl(ctypes.byref(self._desc), _capi.ptr(self._w), scale,
-                                     _capi.ptr(self._dt_dev), self._stream()), "cs_cfl")
+        cfl = lib.snapy_cs_cfl_f32 if self.fp32 else lib.snapy_cs_cfl
+        _capi.check(cfl(ctypes.byref(self._desc), _capi.ptr(self._w), scale,
+                        _capi.ptr(self._dt_dev), self._stream()), "cs_cfl")
         return self._dt_dev
 
     def max_time_step(self, vars=None):
@@ -629,6 +667,8 @@ class CubedSphereMesh:
         self._refresh_primitives()
         i = slice(self.ng, self.ng + self.n)
         nan = ~torch.isfinite(self._u[:, :, i, i]).all()
+        if self._c is not None:
+            nan = nan | ~torch.isfinite(self._c[:, :, i, i]).all()
         floor = ~(self._w[:, IDN, i, i].min() > 1.001 * self.density_floor)
         flags = torch.stack([floor, nan]).cpu().tolist()
         if not any(flags):
@@ -638,6 +678,8 @@ class CubedSphereMesh:
         if self.current_redo > self.max_redo:
             return -1
         self._u.copy_(self._u0)
+        if self._c is not None:
+            self._c.copy_(self._c0)
         self._refresh_primitives()
         return 1
 
@@ -646,8 +688,15 @@ class CubedSphereMesh:
         volume (FP64)."""
         i = slice(self.ng, self.ng + self.n)
         vol = torch.from_numpy(np.ascontiguousarray(self.grid.vol[i, i])).to(self.device)
-        phi = self._u[:, IDN, i, i, 0] * vol
+        phi = self.total_u()[:, IDN, i, i, 0] * vol
         return phi.reshape(self.members, 6, -1).sum(dim=(1, 2))
+
+    def total_u(self) -> torch.Tensor:
+        """The conserved state of every block, FP64 (the pair's value
+        u + c in the FP32 form)."""
+        if self._c is not None:
+            return self._u.double() + self._c.double()
+        return self._u
 
     def execution_report(self) -> dict:
         return self._executor.report()
@@ -655,7 +704,7 @@ class CubedSphereMesh:
     def interior_state(self) -> torch.Tensor:
         """The conserved state of every block's interior, (blocks, 4, n, n)."""
         i = slice(self.ng, self.ng + self.n)
-        return self._u[:, :, i, i, 0]
+        return self.total_u()[:, :, i, i, 0]
 
     def release(self, sync: bool = True) -> None:
         """Drop the captured graphs (sync=False: the caller has synchronized

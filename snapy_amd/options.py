@@ -434,8 +434,8 @@ def _check_cubed(op: MeshBlockOptions, layout: str) -> None:
     for v, want in ((c.x2min, -quarter), (c.x2max, quarter), (c.x3min, -quarter), (c.x3max, quarter)):
         if abs(v - want) > 1e-12:
             raise ValueError("cubed sphere: x2 and x3 span [-pi/4, pi/4] (one block per panel)")
-    if h.precision != "float64":
-        raise ValueError("cubed sphere: the FP64 form is built")
+    if h.precision not in ("float64", "float32"):
+        raise ValueError("cubed sphere: dynamics/precision float64 or float32")
     if h.grav1 != 0.0 or h.nu_iso > 0.0 or h.kappa_iso > 0.0 or h.bot_heat_flux != 0.0:
         raise ValueError("cubed sphere: no forcing is implemented")
     if h.eos.limiter:
