@@ -53,7 +53,14 @@ enum snapy_bc {
 
 /* ---- descriptors ------------------------------------------------------ */
 
-/* One block's uniform Cartesian grid (src/coord/coordinate.cpp:340-420). */
+/* One block's uniform Cartesian grid (src/coord/coordinate.cpp:340-420).
+ *
+ * members3 > 1 makes x3 a member axis: the tensor holds members3
+ * independent blocks of a deck whose x3 is a single cell (nx3 = nc3 =
+ * members3, no x3 ghosts), advanced together by every entry point with no
+ * coupling across x3 (no x3 flux, boundary or diffusion), each member with
+ * its own dt: a device dt (dt_dev) then holds members3 values, and
+ * snapy_cfl writes members3 values.  0 or 1: an ordinary block. */
 typedef struct snapy_grid {
   int32_t nvar;   /* stored rows: 5 Euler, 4 shallow water */
   int32_t nghost; /* 3 for weno5 */
@@ -61,6 +68,8 @@ typedef struct snapy_grid {
   int32_t nc1, nc2, nc3;
   double dx1, dx2, dx3; /* cell widths (global extent / global cells) */
   double x1min, x2min, x3min; /* lower edge of the first interior cell */
+  int32_t members3; /* independent members along x3 (see above) */
+  int32_t pad3;
 } snapy_grid;
 
 /* Deck physics the stage kernels read (src/hydro/hydro.hpp options). */

@@ -43,6 +43,8 @@ class Grid(ctypes.Structure):
         ("x1min", ctypes.c_double),
         ("x2min", ctypes.c_double),
         ("x3min", ctypes.c_double),
+        ("members3", ctypes.c_int32),
+        ("pad3", ctypes.c_int32),
     ]
 
 

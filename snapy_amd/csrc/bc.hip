@@ -181,10 +181,9 @@ BcArgs make_args(snapy_grid const& g, snapy_physics const& p, int axis,
 template <typename T>
 int boundaries(snapy_grid const& g, snapy_physics const& p, T* q,
                T const* bref, T* scratch, bool prim, hipStream_t s) {
-  int nc[3] = {g.nc1, g.nc2, g.nc3};
   bool any_outflow = false;
   for (int f = 0; f < 6; ++f)
-    if (p.bc[f] == SNAPY_BC_OUTFLOW && nc[f / 2] > 1) any_outflow = true;
+    if (p.bc[f] == SNAPY_BC_OUTFLOW && axis_active(g, f / 2 + 1)) any_outflow = true;
   if (any_outflow && !bref)
     return fail(-11, "outflow: missing initial boundary reference");
   if (any_outflow && !prim && !scratch)
@@ -209,7 +208,8 @@ int boundaries(snapy_grid const& g, snapy_physics const& p, T* q,
     pending = false;
   };
   for (int ax = 1; ax <= 3; ++ax) {
-    if (nc[ax - 1] == 1) continue;
+    // a single cell or a member axis has no ghosts
+    if (!axis_active(g, ax)) continue;
     BcArgs a = make_args(g, p, ax, prim ? 1 : 0);
     int64_t tot = int64_t(2) * a.ng * a.nrow;
     bool out_lo = a.bc_lo == SNAPY_BC_OUTFLOW, out_hi = a.bc_hi == SNAPY_BC_OUTFLOW;

@@ -22,7 +22,16 @@ struct Range3 {
   int lo1, n1, lo2, n2, lo3, n3;
   int nc1, nc2, nc3;
   int64_t s2, s3, sv;  // strides of x2, x3 and the variable row
+  int mem3;            // 1, or the members along a member x3 axis
 };
+
+// x3 is a member axis (snapy_grid.members3): no x3 ghosts, no coupling
+inline bool member_axis(snapy_grid const& g) { return g.members3 > 1; }
+// an axis with a stencil, fluxes and boundaries
+inline bool axis_active(snapy_grid const& g, int d) {
+  int nc = d == 1 ? g.nc1 : (d == 2 ? g.nc2 : g.nc3);
+  return nc > 1 && !(d == 3 && member_axis(g));
+}
 
 inline Range3 ranges(snapy_grid const& g) {
   Range3 r;
@@ -31,7 +40,8 @@ inline Range3 ranges(snapy_grid const& g) {
   r.nc3 = g.nc3;
   r.lo1 = g.nc1 > 1 ? g.nghost : 0;
   r.lo2 = g.nc2 > 1 ? g.nghost : 0;
-  r.lo3 = g.nc3 > 1 ? g.nghost : 0;
+  r.lo3 = axis_active(g, 3) ? g.nghost : 0;
+  r.mem3 = member_axis(g) ? g.members3 : 1;
   r.n1 = g.nx1;
   r.n2 = g.nx2;
   r.n3 = g.nx3;
@@ -39,6 +49,13 @@ inline Range3 ranges(snapy_grid const& g) {
   r.s3 = int64_t(g.nc1) * g.nc2;
   r.sv = r.s3 * g.nc3;
   return r;
+}
+
+// The stage dt of interior x3 index k: the device dt (one value, or one
+// per member along a member axis), else the host dt.
+__device__ __forceinline__ double dt_at(double const* dt_dev, double dt_host, Range3 const& r,
+                                        int k) {
+  return dt_dev ? dt_dev[r.mem3 > 1 ? k - r.lo3 : 0] : dt_host;
 }
 
 // host-side validation of a grid descriptor; returns 0 or a negative code

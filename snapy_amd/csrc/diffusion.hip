@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(256)
   int k = r.lo3 + int(rest / r.n2);
   int64_t c = k * r.s3 + j * r.s2 + i;
   int64_t ncell = int64_t(r.nc1) * r.nc2 * r.nc3;
-  double dt = a.dt_dev ? *a.dt_dev : -a.mdt;
+  double dt = dt_at(a.dt_dev, -a.mdt, r, k);
   double A[3] = {a.a1, a.a2, a.a3};
   int64_t s[3] = {1, r.s2, r.s3};
   dsrc[IDN * r.sv + c] = 0.;
@@ -170,7 +170,7 @@ int snapy_diffusion(const snapy_grid* g, const snapy_physics* p, int dtype,
     return fail(-8, "diffusion: heat conduction needs a gas constant > 0");
   if (dtype != SNAPY_F64 && dtype != SNAPY_F32) return fail(-10, "unknown dtype");
   Range3 r = ranges(*g);
-  int act[3] = {g->nc1 > 1, g->nc2 > 1, g->nc3 > 1};
+  int act[3] = {g->nc1 > 1, g->nc2 > 1, axis_active(*g, 3)};
   int nx[3] = {g->nx1, g->nx2, g->nx3};
   double dx[3] = {g->dx1, g->dx2, g->dx3};
   for (int ax = 0; ax < 3; ++ax)

@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(256)
   int64_t row = k * r.s3 + j * r.s2;
   int64_t c = row + i;
   T A1 = T(a.a1), A2 = T(a.a2), A3 = T(a.a3), V = T(a.vol);
-  double dt = a.dt_dev ? *a.dt_dev : -a.mdt;
+  double dt = dt_at(a.dt_dev, -a.mdt, r, k);
   T mdt = T(-dt), tdt = T(dt);
   T w0 = T(a.w0), w1 = T(a.w1), w2 = T(a.w2), g1 = T(a.grav1);
   T rho = w[c], vx = w[r.sv + c];

@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(64)
   int n = a.n, nc1 = r.nc1, lo1 = r.lo1;
   double const gm1 = a.gamma - 1.;
   double const grav = a.grav, A = a.area, V = a.vol;
-  double const dt = (a.dt_dev ? *a.dt_dev : a.dt_host) * a.wstage;
+  double const dt = dt_at(a.dt_dev, a.dt_host, r, k) * a.wstage;
   double const hiv = 0.5 / V;
   auto W = [&](int v, int i) { return w[v * r.sv + row + i]; };      // i in [-1, n]
   auto DU = [&](int v, int i) { return du[v * r.sv + row + i]; };    // i in [0, n)
@@ -582,7 +582,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
   double const gm1 = a.gamma - 1.;
   double const gp1_gm1 = (gm1 + 1) / gm1;
   double const grav = a.grav, A = a.area, V = a.vol;
-  double const dt = (a.dt_dev ? *a.dt_dev : a.dt_host) * a.wstage;
+  double const dt = dt_at(a.dt_dev, a.dt_host, r, r.lo3 + int(L.col / r.n2)) * a.wstage;
   double const idt = 1. / dt;
   double const hiv = 0.5 / V;
   double* wsl = ws + int64_t(blockIdx.x) * n * kQ * 64 + threadIdx.x;
@@ -810,7 +810,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
   int const n = a.n, nc1 = r.nc1, lo1 = r.lo1;
   int64_t const row = s_row[slot];
   double const V = a.vol;
-  double const dt = (a.dt_dev ? *a.dt_dev : a.dt_host) * a.wstage;
+  double const dt = dt_at(a.dt_dev, a.dt_host, r, r.lo3 + int(L.col / r.n2)) * a.wstage;
   double* wsw = ws + int64_t(blockIdx.x) * n * kQ * 64;
   double* wsl = wsw + threadIdx.x;
   double* wsb = wsw + base;  // lane 0 of the column

@@ -56,8 +56,9 @@ struct PartArgs {
   double w0, w1, w2;
 };
 
-__device__ __forceinline__ double stage_dt(PartArgs const& a) {
-  return (a.dt_dev ? *a.dt_dev : a.dt_host) * a.wstage;
+// the stage dt of column col (the member's along a member x3 axis)
+__device__ __forceinline__ double stage_dt(PartArgs const& a, int64_t col) {
+  return dt_at(a.dt_dev, a.dt_host, a.r, a.r.lo3 + int(col / a.r.n2)) * a.wstage;
 }
 
 // rows {rho, m1, E} of FluxJacobian (flux_decomposition_impl.h:105-126), dir
@@ -370,7 +371,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
   bool live = col < ncol;
   int n = a.n, nc1 = r.nc1, lo1 = r.lo1;
   double const A = a.area, V = a.vol;
-  double const dt = stage_dt(a);
+  double const dt = stage_dt(a, live ? col : 0);
   auto REC = [&](int l) { return Rec{ws + int64_t(l) * kRec * ncol + col, ncol}; };
   if (producer) {
     // the first interior cell of each of the block's columns

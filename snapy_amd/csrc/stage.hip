@@ -43,8 +43,11 @@ int check_grid(snapy_grid const* g) {
   if (g->nghost < 3) return fail(-2, "nghost must be >= 3 for weno5");
   int nx[3] = {g->nx1, g->nx2, g->nx3};
   int nc[3] = {g->nc1, g->nc2, g->nc3};
+  if (g->members3 > 1 && (g->nx3 != g->members3 || g->nc3 != g->members3))
+    return fail(-4, "a member x3 axis has nx3 = nc3 = members3 (no x3 ghosts)");
   for (int d = 0; d < 3; ++d) {
     if (nx[d] < 1) return fail(-3, "nx must be >= 1");
+    if (d == 2 && g->members3 > 1) continue;
     int want = nx[d] > 1 ? nx[d] + 2 * g->nghost : 1;
     if (nc[d] != want) return fail(-4, "nc does not match nx + 2*nghost");
     if (nx[d] > 1 && nx[d] < g->nghost)
@@ -235,7 +238,7 @@ __global__ void __launch_bounds__(256)
   int k = r.lo3 + int(rest / r.n2);
   int64_t c = k * r.s3 + j * r.s2 + i;
   T A1 = T(a.a1), A2 = T(a.a2), A3 = T(a.a3), V = T(a.vol);
-  T mdt = a.dt_dev ? T(-*a.dt_dev) : T(a.mdt);
+  T mdt = a.dt_dev ? T(-dt_at(a.dt_dev, 0., r, k)) : T(a.mdt);
   T w0 = T(a.w0), w1 = T(a.w1), w2 = T(a.w2);
   for (int v = 0; v < a.nvar; ++v) {
     int64_t o = v * r.sv + c;
@@ -283,7 +286,7 @@ __global__ void __launch_bounds__(256)
   int k = r.lo3 + int(rest / r.n2);
   int64_t c = k * r.s3 + j * r.s2 + i;
   double A1 = a.a1, A2 = a.a2, A3 = a.a3, V = a.vol;
-  double mdt = a.dt_dev ? -*a.dt_dev : a.mdt;
+  double mdt = a.dt_dev ? -dt_at(a.dt_dev, 0., r, k) : a.mdt;
   for (int v = 0; v < a.nvar; ++v) {
     int64_t o = v * r.sv + c;
     double div = 0.;
@@ -317,37 +320,48 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     step_field_kernel(T const* __restrict__ t, double* __restrict__ s,
                       int64_t n, double w1, double w2, double dt,
-                      double const* dt_dev) {
+                      double const* dt_dev, Range3 r) {
   int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  // dt.scaled(w2) then axpy_ (flux_register.cpp:761-777)
-  double w2dt = (dt_dev ? *dt_dev : dt) * w2;
+  // dt.scaled(w2) then axpy_ (flux_register.cpp:761-777); a member axis
+  // has no x3 ghosts, so the x3 index is the member
+  int k = r.mem3 > 1 ? int((c / r.s3) % r.nc3) : 0;
+  double w2dt = (dt_dev ? dt_dev[k] : dt) * w2;
   s[c] = s[c] * w1 + double(t[c]) * w2dt;
 }
 
 // ---------------------------------------------------------------------------
 // CFL: min over interior cells of dx_d / (|v_d| + c), one atomic per block
 // ---------------------------------------------------------------------------
-__global__ void fill_u64_kernel(unsigned long long* p, unsigned long long v) {
-  *p = v;
+__global__ void fill_u64_kernel(unsigned long long* p, unsigned long long v, int n) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < n) p[m] = v;
 }
 
-__global__ void scale_f64_kernel(double* p, double scale) { *p = scale * *p; }
+__global__ void scale_f64_kernel(double* p, double scale, int n) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < n) p[m] = scale * p[m];
+}
 
+// blockIdx.y: the member along a member x3 axis (its own slab and output),
+// else 0 (every interior cell, one output)
 template <typename T>
 __global__ void __launch_bounds__(256)
     cfl_kernel(T const* __restrict__ w, unsigned long long* out, Range3 r,
                double gd, double dx1, double dx2, double dx3, int act1,
                int act2, int act3, double adv1) {
-  int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
+  int const m = blockIdx.y;
+  int const n3 = r.mem3 > 1 ? 1 : r.n3;
+  int64_t tot = int64_t(r.n1) * r.n2 * n3;
   double best = 1.e9;
+  out += m;
   // grid-stride: a bounded grid keeps the final atomics few
   for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < tot;
        idx += int64_t(gridDim.x) * blockDim.x) {
     int i = r.lo1 + int(idx % r.n1);
     int64_t rest = idx / r.n1;
     int j = r.lo2 + int(rest % r.n2);
-    int k = r.lo3 + int(rest / r.n2);
+    int k = r.lo3 + (r.mem3 > 1 ? m : int(rest / r.n2));
     int64_t c = k * r.s3 + j * r.s2 + i;
     T rho = w[c], p = w[4 * r.sv + c];
     // sound_speed: sqrt(gamma * p / rho) (equation_of_state.cpp:235-238)
@@ -702,7 +716,7 @@ int snapy_step_field(const snapy_grid* g, int dtype, const void* total,
     hipLaunchKernelGGL(step_field_kernel<T>, dim3(blocks_for(n, 256)),
                        dim3(256), 0, as_stream(stream),
                        static_cast<T const*>(total), step, n, w1, w2, dt,
-                       dt_dev);
+                       dt_dev, ranges(*g));
     return launch_status("step_field");
   });
 }
@@ -712,9 +726,13 @@ int snapy_cfl(const snapy_grid* g, const snapy_physics* p, int dtype,
   if (int e = check_grid(g)) return e;
   if (int e = check_phys(p)) return e;
   if (!w || !dt_out) return fail(-1, "cfl: NULL buffer");
-  if (p->eos == SNAPY_EOS_SHALLOW_WATER) return swe_cfl(g, dtype, w, scale, dt_out, stream);
+  if (p->eos == SNAPY_EOS_SHALLOW_WATER) {
+    if (member_axis(*g)) return fail(-4, "cfl: a member axis is built for the ideal gas");
+    return swe_cfl(g, dtype, w, scale, dt_out, stream);
+  }
   Range3 r = ranges(*g);
-  int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
+  int const nout = r.mem3;  // one dt per member
+  int64_t tot = int64_t(g->nx1) * g->nx2 * (nout > 1 ? 1 : g->nx3);
   auto out = reinterpret_cast<unsigned long long*>(dt_out);
   double init = 1.e9;
   // the sources' bound joins before the CFL factor (hydro.cpp:396): the
@@ -725,9 +743,8 @@ int snapy_cfl(const snapy_grid* g, const snapy_physics* p, int dtype,
     int ndim = 0;
     double dxm = 1.e300;
     double dxs[3] = {g->dx1, g->dx2, g->dx3};
-    int ncs[3] = {g->nc1, g->nc2, g->nc3};
     for (int ax = 0; ax < 3; ++ax)
-      if (ncs[ax] > 1) {
+      if (axis_active(*g, ax + 1)) {
         ++ndim;
         dxm = dxs[ax] < dxm ? dxs[ax] : dxm;
       }
@@ -739,20 +756,23 @@ int snapy_cfl(const snapy_grid* g, const snapy_physics* p, int dtype,
   unsigned long long bits;
   std::memcpy(&bits, &init, sizeof(bits));
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, s, out, bits);
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(nout, 64)), dim3(64), 0, s, out, bits,
+                     nout);
+  bool act2 = axis_active(*g, 2), act3 = axis_active(*g, 3);
   return dispatch(dtype, [&](auto z) {
     using T = decltype(z);
     unsigned nb = blocks_for(tot, 256);
-    nb = nb < 2048u ? nb : 2048u;
-    hipLaunchKernelGGL(cfl_kernel<T>, dim3(nb), dim3(256), 0,
+    unsigned cap = nout > 1 ? 64u : 2048u;
+    nb = nb < cap ? nb : cap;
+    hipLaunchKernelGGL(cfl_kernel<T>, dim3(nb, unsigned(nout)), dim3(256), 0,
                        s, static_cast<T const*>(w), out, r, p->gammad, g->dx1,
-                       g->dx2, g->dx3, g->nc1 > 1, g->nc2 > 1, g->nc3 > 1,
-                       ((p->implicit_scheme & 1) && (g->nc2 > 1 || g->nc3 > 1))
+                       g->dx2, g->dx3, g->nc1 > 1, act2, act3,
+                       ((p->implicit_scheme & 1) && (act2 || act3))
                            ? p->advection
                            : 0.);
     if (scale != 1.)
-      hipLaunchKernelGGL(scale_f64_kernel, dim3(1), dim3(1), 0, s, dt_out,
-                         scale);
+      hipLaunchKernelGGL(scale_f64_kernel, dim3(blocks_for(nout, 64)), dim3(64), 0, s, dt_out,
+                         scale, nout);
     return launch_status("cfl");
   });
 }

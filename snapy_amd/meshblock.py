@@ -78,7 +78,22 @@ class _Buffers:
 
 
 class MeshBlock:
-    def __init__(self, options: MeshBlockOptions):
+    """MeshBlock(options) is one block; MeshBlock(options, members=M) of a
+    deck whose x3 is a single cell holds M independent members of that
+    block stacked along x3 (tensors (nvar, M, nc2, nc1), no x3 ghosts or
+    coupling), advanced together by every launch, each member with its own
+    dt: the device dt and the time steps are then M values, and forward
+    takes a float (every member) or M values."""
+
+    def __init__(self, options: MeshBlockOptions, members: int = 1):
+        self.members = int(members)
+        if self.members < 1:
+            raise ValueError("members >= 1")
+        if self.members > 1:
+            if options.coord.nx3 != 1:
+                raise ValueError("members: the deck's x3 must be a single cell (members stack along x3)")
+            if options.hydro.eos.type != "ideal-gas":
+                raise ValueError("members: the member axis is built for the ideal-gas stage")
         self.options = options
         self._cycle = 0
         self.last_redo_causes = 0
@@ -101,21 +116,24 @@ class MeshBlock:
         c = options.coord
         # ideal gas: (rho, m1, m2, m3, E); shallow water: (phi, phi v1..v3)
         self.nvar = 4 if options.hydro.eos.type == "shallow-water" else 5
+        m = self.members
         self._grid = _capi.Grid(
             self.nvar,
             c.nghost,
             c.nx1,
             c.nx2,
-            c.nx3,
+            m if m > 1 else c.nx3,
             c.nc1(),
             c.nc2(),
-            c.nc3(),
+            m if m > 1 else c.nc3(),
             c.dx(1),
             c.dx(2),
             c.dx(3),
             c.x1min,
             c.x2min,
             c.x3min,
+            m if m > 1 else 0,
+            0,
         )
         h = options.hydro
         bcs = (ctypes.c_int32 * 6)(*[_capi.BC[n] for n in options.bcnames])
@@ -187,7 +205,11 @@ class MeshBlock:
 
     def shape(self):
         c = self.options.coord
-        return (self.nvar, c.nc3(), c.nc2(), c.nc1())
+        return (self.nvar, self._nc3(), c.nc2(), c.nc1())
+
+    def _nc3(self) -> int:
+        """Cells along x3: the deck's, or the members."""
+        return self.members if self.members > 1 else self.options.coord.nc3()
 
     def part(self, offset=(0, 0, 0), exterior=True, extend_x1=0, extend_x2=0, extend_x3=0):
         """The slices of the block's interior (exterior=False) or of the
@@ -197,6 +219,9 @@ class MeshBlock:
         c = self.options.coord
         out = [slice(None)]
         for d, ext in ((3, extend_x3), (2, extend_x2), (1, extend_x1)):
+            if d == 3 and self.members > 1:
+                out.append(slice(0, self.members))  # every member, no ghosts
+                continue
             nx = (c.nx1, c.nx2, c.nx3)[d - 1]
             nc = c.nc(d)
             if exterior:
@@ -233,7 +258,7 @@ class MeshBlock:
         diffusion and implicit buffers its options call for."""
         c = self.options.coord
         h = self.options.hydro
-        cells = c.nc1() * c.nc2() * c.nc3()
+        cells = c.nc1() * c.nc2() * self._nc3()
         ncell = self.nvar * cells
         fp32 = self.policy.fp32()
         total = ncell * ((4 + 4) * 2 if fp32 else 8 * 2) + ncell * 8  # u, u0 (+c, c0); w
@@ -247,7 +272,7 @@ class MeshBlock:
         if self.options.intg.implicit_scheme in (1, 9):
             lib = _capi.load()
             total += int(self._vic_workspace(lib)) * 8 + ncell * 8
-            total += c.nx2 * c.nx3 * 4
+            total += c.nx2 * c.nx3 * self.members * 4
         return int(total)
 
     def _ensure_buffers(self):
@@ -255,11 +280,12 @@ class MeshBlock:
             return
         c = self.options.coord
         active = (c.nx1 > 1, c.nx2 > 1, c.nx3 > 1)
-        shape = (c.nc3(), c.nc2(), c.nc1())
+        shape = (self._nc3(), c.nc2(), c.nc1())
         self._freg = FluxRegister(self.nvar, shape, active, self.policy.row("flux").storage, self.device)
         if "outflow" in self.options.bcnames:
             self._scratch_w = torch.empty(self.shape(), dtype=torch.float64, device=self.device)
-        self._dt_dev = torch.zeros((), dtype=torch.float64, device=self.device)
+        self._dt_dev = torch.zeros(() if self.members == 1 else (self.members,), dtype=torch.float64,
+                                   device=self.device)
         h = self.options.hydro
         self._ref = None
         self._x1tab = None
@@ -272,7 +298,7 @@ class MeshBlock:
             self.device
         )
         if h.grav1 != 0.0:
-            self._ref = torch.zeros((5, c.nc3(), c.nc2(), c.nc1()), dtype=torch.float64, device=self.device)
+            self._ref = torch.zeros((5, self._nc3(), c.nc2(), c.nc1()), dtype=torch.float64, device=self.device)
         self._diff = None
         if h.nu_iso > 0.0 or h.kappa_iso > 0.0:
             lib = _capi.load()
@@ -296,7 +322,7 @@ class MeshBlock:
                 "t0": t0,
                 "t1": t1,
                 "ws": torch.empty(nws, dtype=torch.float64, device=self.device),
-                "status": torch.zeros(c.nx2 * c.nx3, dtype=torch.int32, device=self.device),
+                "status": torch.zeros(c.nx2 * c.nx3 * self.members, dtype=torch.int32, device=self.device),
                 "du": torch.zeros(self.shape(), dtype=torch.float64, device=self.device),
                 "scheme": int(self.options.intg.implicit_scheme),
                 "name": "vic_partial" if self.options.intg.implicit_scheme == 1 else "vic_full",
@@ -365,6 +391,8 @@ class MeshBlock:
         snapy_amd.restart for what it holds."""
         from . import restart
 
+        if self.members > 1:
+            raise restart.RestartError("restart: a block of members has no restart part (one per member block)")
         torch.cuda.synchronize(self.device)
         return restart.write_restart(self, vars, path, time)
 
@@ -379,6 +407,8 @@ class MeshBlock:
 
         if self.device.type != "cuda":
             raise _capi.SnapyError("initialize_from_restart: call block.to('cuda') first")
+        if self.members > 1:
+            raise restart.RestartError("restart: a block of members has no restart part (one per member block)")
         data = restart.load_restart(path_or_data) if isinstance(path_or_data, str) else path_or_data
         restart.check_restart_metadata(data, restart.grid_faces(self))
         if tuple(data["hydro_u"].shape) != self.shape():
@@ -425,7 +455,9 @@ class MeshBlock:
         self._adopt(vars)
         self._ensure_buffers()
         if isinstance(dt, torch.Tensor):
-            self._dt_dev.copy_(dt)
+            self._dt_dev.copy_(dt.to(self._dt_dev.device, torch.float64).expand_as(self._dt_dev))
+        elif isinstance(dt, (list, tuple)):
+            self._dt_dev.copy_(torch.tensor(dt, dtype=torch.float64))
         else:
             self._dt_dev.fill_(float(dt))
         self._executor.run_stage(vars, stage)
@@ -661,16 +693,17 @@ class MeshBlock:
         )
         return self._dt_dev
 
-    def max_time_step(self, vars) -> float:
+    def max_time_step(self, vars):
         """global_time_step(local_max_time_step(vars))
         (meshblock_advance.cpp:61-85): 2^-redo * cfl * min dx/(|v|+c) over
-        the interior of vars['hydro_w'] (one host read)."""
+        the interior of vars['hydro_w'] (one host read); with members, the
+        list of the members' time steps."""
         self._ensure_buffers()
         if self._w_stale:
             self._refresh_primitives(vars)
         lib = _capi.load()
         w = vars["hydro_w"]
-        out = torch.empty((), dtype=torch.float64, device=self.device)
+        out = torch.empty(self._dt_dev.shape, dtype=torch.float64, device=self.device)
         _capi.check(
             lib.snapy_cfl(
                 ctypes.byref(self._grid), ctypes.byref(self._phys), _capi.dtype_code(w.dtype),
@@ -678,8 +711,10 @@ class MeshBlock:
             ),
             "cfl",
         )
-        dt = float(out.item())
-        return math.pow(2.0, -self.intg.current_redo) * self.intg.options.cfl * dt
+        scale = math.pow(2.0, -self.intg.current_redo) * self.intg.options.cfl
+        if self.members > 1:
+            return [scale * dt for dt in out.cpu().tolist()]
+        return scale * float(out.item())
 
     # retry causes, the reference's bits (meshblock_redo.cpp:44-49)
     REDO_FLOOR, REDO_NAN, REDO_VIC = 1, 8, 32
