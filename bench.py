@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--implicit-scheme", type=int, default=None, choices=(1, 9),
                     help="override the convection deck's implicit scheme")
     ap.add_argument("--cubed-steps", type=int, default=50)
+    ap.add_argument("--ensemble-members", type=int, default=32,
+                    help="members of the density-current ensemble line (one launch per kernel)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default workgroup sizes instead of tuning them for the workload")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -57,7 +59,7 @@ def parse():
 SCHEME = {"value": None}  # --implicit-scheme
 
 
-def make_block(deck, execution="hip-graph", precision="float64", **coord):
+def make_block(deck, execution="hip-graph", precision="float64", members=1, **coord):
     from snapy_amd import MeshBlock, MeshBlockOptions
 
     op = MeshBlockOptions.from_yaml(os.path.join(ROOT, "snapy_amd", "decks", deck))
@@ -67,7 +69,7 @@ def make_block(deck, execution="hip-graph", precision="float64", **coord):
     op.hydro.precision = precision
     if SCHEME["value"] is not None and op.intg.implicit_scheme:
         op.intg.implicit_scheme = SCHEME["value"]
-    block = MeshBlock(op)
+    block = MeshBlock(op, members=members)
     block.to("cuda")
     return block
 
@@ -341,18 +343,26 @@ def fp32_accuracy(name, nsteps=3):
     return out
 
 
-def density_current_leg(steps=50, warmup=3):
+def density_current_leg(steps=50, warmup=3, members=1, seed=2025):
     """The density current deck (512 x 64, dx = dz = 100 m, LMARS, isotropic
     diffusion, the vertical implicit scheme the deck states) in the FP32
-    state form, one block, captured stages: the throughput against its
-    B_alg (FP32 + compensation + FP64 implicit solve, 2 active directions:
-    260 B per cell-update)."""
+    state form, captured stages: one block, or `members` ensemble members
+    (cold bubbles dT ~ U(-20, -5) K from `seed`) on the member x3 axis, each
+    with its own dt, advanced by one launch per kernel.  The throughput is
+    against the B_alg (FP32 + compensation + FP64 implicit solve, 2 active
+    directions: 260 B per cell-update)."""
+    import numpy as np
     import torch
 
     from snapy_amd import problems
 
-    block = make_block("density_current.yaml", precision="float32")
-    vars, _ = block.initialize({"hydro_w": problems.density_current(block).cuda()})
+    block = make_block("density_current.yaml", precision="float32", members=members)
+    if members == 1:
+        w = problems.density_current(block)
+    else:
+        dTs = np.random.default_rng(seed).uniform(-20.0, -5.0, members)
+        w = torch.cat([problems.density_current(block, dT=float(d))[:, i : i + 1] for i, d in enumerate(dTs)], dim=1)
+    vars, _ = block.initialize({"hydro_w": w.cuda()})
     for _ in range(warmup):
         dt = block.device_time_step(vars)
         for s in range(3):
@@ -366,12 +376,12 @@ def density_current_leg(steps=50, warmup=3):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     c = block.options.coord
-    cells = c.nx1 * c.nx2 * c.nx3
+    cells = c.nx1 * c.nx2 * c.nx3 * members
     value = cells * 3 * steps / el
     b_alg = 260
     finite = bool(torch.isfinite(block.total_u(vars)).all())
     out = {
-        "workload": f"density_current_{c.nx2}x{c.nx1}",
+        "workload": f"density_current_{c.nx2}x{c.nx1}" + (f"_x{members}_members" if members > 1 else ""),
         "value": value,
         "unit": "cell-updates/s",
         "steps": steps,
@@ -652,6 +662,12 @@ def main():
         print("# density current", file=sys.stderr, flush=True)
         c2 = density_current_leg()
         print(f"# density current {c2['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        ens = density_current_leg(members=args.ensemble_members)
+        c2["ensemble"] = ens
+        c2["ensemble_speedup_per_member"] = ens["value"] / c2["value"]
+        print(f"# density current ensemble x{args.ensemble_members} {ens['value']:.3e} cell-updates/s",
+              file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     cubed = None
     if not args.no_cubed and args.workload == "convection3d":
