@@ -255,7 +255,12 @@ int snapy_update_gravity(const snapy_grid *g, const snapy_physics *p,
  * with bit 0 and kept by later calls, a step-level latch the caller clears
  * at the step's first stage (the reference ORs every stage's bits into the
  * step status, src/implicit/implicit_hydro.cpp:610-617, and resets it at
- * step start, src/mesh/meshblock_advance.cpp:186-188).
+ * step start, src/mesh/meshblock_advance.cpp:186-188).  moved: NULL, or an
+ * FP64 (nc3, nc2, nc1) array that receives at each interior cell the
+ * clamped dry-gas transfer through the face below it (0 at a column's
+ * bottom cell and in a failed column), the mass_fix row the passive
+ * scalars' implicit transfer reads (src/implicit/vic_redistribute_impl.h:
+ * 80-90, src/mesh/meshblock_advance.cpp:215-233).
  * Replaces: vic_stage assemble/solve/redistribute (src/implicit/
  * implicit_dispatch.hpp:50-57, src/implicit/vic_kernels.h:66-297) and
  * ImplicitHydroImpl::forward_masked (src/implicit/implicit_hydro.cpp:
@@ -266,7 +271,7 @@ int snapy_vic_full(const snapy_grid *g, const snapy_physics *p, const double *w,
                    const double *vtab, const double *x1tab, double t0,
                    double t1, double dt, const double *dt_dev, double wstage,
                    double w0, double w1, double w2, double *ws, int *status,
-                   void *stream);
+                   double *moved, void *stream);
 /* The same solve with one column per thread and textbook Gaussian
  * elimination with scaled partial pivoting on each block: the same
  * arguments and outputs as snapy_vic_full, kept as the independent
@@ -278,7 +283,7 @@ int snapy_vic_full_serial(const snapy_grid *g, const snapy_physics *p,
                           double t0, double t1, double dt,
                           const double *dt_dev, double wstage, double w0,
                           double w1, double w2, double *ws, int *status,
-                          void *stream);
+                          double *moved, void *stream);
 
 /* Isotropic Cartesian diffusion, kinematic form (forcing/diffusion with
  * nu_iso, kappa_iso; no x1 profiles; stress-free walls).  w is the FP64
@@ -352,7 +357,7 @@ int snapy_vic_partial(const snapy_grid *g, const snapy_physics *p,
                       double *u, const double *vtab, const double *x1tab,
                       double t0, double t1, double dt, const double *dt_dev,
                       double wstage, double w0, double w1, double w2,
-                      double *ws, int *status, void *stream);
+                      double *ws, int *status, double *moved, void *stream);
 /* snapy_vic_partial finishing into the FP32 pair (u, c) from (u0, c0). */
 int snapy_vic_partial_f32(const snapy_grid *g, const snapy_physics *p,
                           const double *w, const double *du, const float *u0,
@@ -360,7 +365,8 @@ int snapy_vic_partial_f32(const snapy_grid *g, const snapy_physics *p,
                           const double *vtab, const double *x1tab, double t0,
                           double t1, double dt, const double *dt_dev,
                           double wstage, double w0, double w1, double w2,
-                          double *ws, int *status, void *stream);
+                          double *ws, int *status, double *moved,
+                          void *stream);
 
 /* snapy_vic_full finishing into the pair (u, c) from (u0, c0). */
 int snapy_vic_full_f32(const snapy_grid *g, const snapy_physics *p,
@@ -369,7 +375,63 @@ int snapy_vic_full_f32(const snapy_grid *g, const snapy_physics *p,
                        const double *vtab, const double *x1tab, double t0,
                        double t1, double dt, const double *dt_dev,
                        double wstage, double w0, double w1, double w2,
-                       double *ws, int *status, void *stream);
+                       double *ws, int *status, double *moved,
+                       void *stream);
+
+/* ---- passive scalars (tracers) ----------------------------------------
+ * ns tracer rows s = rho r (FP64 in both state forms), (ns, nc3, nc2, nc1)
+ * like the hydro rows; the stage's dry density is the primitive density
+ * w[IDN].  Dry ideal gas, no member axis.  Per stage, after the hydro
+ * sweeps: snapy_scalar_flux per active direction, snapy_scalar_theta when
+ * the covariance is on, then (after the hydro update, whose implicit solve
+ * writes the dry-gas transfer) snapy_scalar_update and, after the hydro
+ * boundaries, snapy_scalar_prim. */
+
+/* The face flux of direction dir (1, 2, 3) at the faces of the interior
+ * cells (ScalarImpl::forward, src/scalar/scalar.cpp:104-174): the scaled
+ * WENO5 faces of rho and s clamped at 0 (recon_point.h:69-147,
+ * reconstruct.cpp:262-266), r = s_face / rho_face or the donor cell's r,
+ * the upwind flux on row IDN of hflux (the hydro's stage flux total of
+ * dir, dtype fdtype; riemann.cpp:164-171), on x2/x3 with
+ * p->flux_covariance the covariance row sigma1^2 rho D1[u_n] D1[q]
+ * (hydro_forward.cpp:233-297; x1tab row 0 = x1v), and 0 on reflecting
+ * faces.  sflux: FP64, the face below each cell index. */
+int snapy_scalar_flux(int dir, const snapy_grid *g, const snapy_physics *p,
+                      int ns, const double *w, const double *s, int fdtype,
+                      const void *hflux, const double *x1tab, double *sflux,
+                      void *stream);
+/* The scalars' positivity limiter factors (flux_positivity_theta, _share,
+ * _relax, src/hydro/flux_positivity.cpp:39-149, as scalar.cpp:193-205
+ * applies them): theta of every row on the interior cells relaxed by the
+ * dry air's theta of the mass fluxes, 1 on the ghosts, then the ghosts
+ * filled as a scalar field. */
+int snapy_scalar_theta(const snapy_grid *g, const snapy_physics *p, int ns,
+                       const double *w, const double *s, int fdtype,
+                       const void *const hflux[3], const double *const sflux[3],
+                       double dt, const double *dt_dev, double *theta,
+                       void *stream);
+/* ds = -dt div(F) on the interior, each face flux scaled by its donor's
+ * theta (NULL: none); with moved (the dry-gas transfer of the implicit
+ * solve, snapy_vic_*) the transfer P = r_donor M capped up each column by
+ * what the donors hold (meshblock_advance.cpp:215-233, donor_cap_cuda,
+ * implicit_dispatch.cu:69-87; r the stage's scalar primitives, ds an FP64
+ * workspace like s); then s = w0 s0 + w1 s + w2 ds and the ghosts of s. */
+int snapy_scalar_update(const snapy_grid *g, const snapy_physics *p, int ns,
+                        const double *const sflux[3], const double *theta,
+                        const double *r, const double *moved, const double *s0,
+                        double *s, double *ds, double dt, const double *dt_dev,
+                        double w0, double w1, double w2, void *stream);
+/* r = s / rho of the state u (dtype udtype; comp the FP32 pair's
+ * compensation or NULL) on the interior, then the ghosts of r
+ * (set_scalar_primitive, src/mesh/meshblock.cpp:67-79). */
+int snapy_scalar_prim(const snapy_grid *g, const snapy_physics *p, int ns,
+                      int udtype, const void *u, const float *comp,
+                      const double *s, double *r, void *stream);
+/* The boundary functions on a kScalar field of ns FP64 rows: reflecting
+ * mirrors without a sign, periodic, extrapolation (bc_func.cpp:10-65); a
+ * radiating face is refused. */
+int snapy_scalar_boundaries(const snapy_grid *g, const snapy_physics *p,
+                            int ns, double *q, void *stream);
 
 /* FP64 budget: out[v] = sum over interior cells of u[v] * volume, for
  * v < nvar, deterministic (fixed reduction tree).  out is device FP64 of

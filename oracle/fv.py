@@ -429,9 +429,10 @@ def _sl(ndim, axis, s):
     return tuple(idx)
 
 
-def bc_gather(var, dim, kind, outer, ng=3):
+def bc_gather(var, dim, kind, outer, ng=3, flip=True):
     """reflecting / periodic / extrapolation on a (nvar, nc3, nc2, nc1) field;
-    dim is the tensor axis (3 = x1, 2 = x2, 1 = x3) (bc_func.cpp:10-65)."""
+    dim is the tensor axis (3 = x1, 2 = x2, 1 = x3) (bc_func.cpp:10-65).
+    flip=False: a kScalar field (no normal-velocity row to negate)."""
     nc = var.shape[dim]
     if nc == 1:
         return
@@ -439,12 +440,14 @@ def bc_gather(var, dim, kind, outer, ng=3):
     if kind == "reflecting":
         if not outer:
             var[_sl(nd, dim, slice(0, ng))] = np.flip(var[_sl(nd, dim, slice(ng, 2 * ng))], axis=dim)
-            var[4 - dim][_sl(nd - 1, dim - 1, slice(0, ng))] *= -1
+            if flip:
+                var[4 - dim][_sl(nd - 1, dim - 1, slice(0, ng))] *= -1
         else:
             var[_sl(nd, dim, slice(nc - ng, nc))] = np.flip(
                 var[_sl(nd, dim, slice(nc - 2 * ng, nc - ng))], axis=dim
             )
-            var[4 - dim][_sl(nd - 1, dim - 1, slice(nc - ng, nc))] *= -1
+            if flip:
+                var[4 - dim][_sl(nd - 1, dim - 1, slice(nc - ng, nc))] *= -1
     elif kind == "periodic":
         if not outer:
             var[_sl(nd, dim, slice(0, ng))] = var[_sl(nd, dim, slice(nc - 2 * ng, nc - ng))]
@@ -629,6 +632,8 @@ class Deck:
     nu_iso: float = 0.0
     kappa_iso: float = 0.0
     weight: float = 0.02897025302492418
+    # passive scalars (tracers), transported as oracle.scalar restates
+    nscalar: int = 0
 
 
 @dataclass
@@ -644,9 +649,15 @@ class Block:
     bref: np.ndarray = None
     flux: dict = field(default_factory=dict)
     face_pressure1: np.ndarray = None
+    # passive scalars: s = rho r (conserved), its stage backup, r = s / rho
+    s: np.ndarray = None
+    s0: np.ndarray = None
+    r: np.ndarray = None
 
-    def initialize(self, w):
-        """initialize_local (src/mesh/meshblock.cpp:306-343) + prim->cons."""
+    def initialize(self, w, r=None):
+        """initialize_local (src/mesh/meshblock.cpp:306-343) + prim->cons;
+        with scalars, s = u[IDN] r on every cell (finalize_initialization,
+        meshblock.cpp:380-383)."""
         d = self.deck
         w = w.copy()
         if "outflow" in d.bcs:
@@ -655,7 +666,27 @@ class Block:
         self.w = w
         self._w_stale = False
         self.u = prim2cons(w, d.gammad)
+        if d.nscalar:
+            self.r = np.array(r, dtype=np.float64, copy=True)
+            self.s = self.u[IDN][None] * self.r
         return self.u
+
+    def _scalar_tendency(self, dt, cov):
+        """ds of the stage before the implicit transfer (scalar.cpp:104-218)."""
+        from . import scalar
+
+        d = self.deck
+        g = d.grid
+        rho = self.u[IDN][None]
+        fl, mass, used = scalar.fluxes(self.s, rho, self.flux, g, d.bcs, w=self.w, shock23=d.shock_x23, cov=cov)
+        if used:
+            fl, _ = scalar.limit(self.s, rho, fl, mass, g, dt, d.bcs)
+        self.scalar_flux = fl
+        div = divergence(g, fl)
+        ds = np.zeros_like(div)
+        sub = (slice(None),) + g.interior()
+        ds[sub] = -dt * div[sub]
+        return ds
 
     def tendency(self, dt):
         """du of one stage (hydro_forward.cpp steps 1, 2, 4, 5, 6)."""
@@ -694,6 +725,8 @@ class Block:
             fdiff = diffusion.cartesian_fluxes(self.w, g, d.nu_iso, d.kappa_iso, d.gammad, d.weight, d.bcs)
             diffusion.post_fluxes(flux, fdiff, g, d.bcs)
         self.flux = flux
+        if d.nscalar:
+            self._ds = self._scalar_tendency(dt, d.flux_covariance)
         div = divergence(g, flux)
         du = np.zeros_like(div)
         sub = (slice(None),) + g.interior()
@@ -733,22 +766,40 @@ class Block:
         x1f = g.faces(1)[ng : ng + n + 1]
         x1v = g.centers(1)[ng : ng + n]
         solve = implicit.vic_partial if self.deck.implicit_scheme == 1 else implicit.vic_full
-        out = solve(
+        out, moved = solve(
             dcol, wcol, self.deck.gammad, self.deck.grav1, dt_corr, g.dx(1), x1f, x1v,
-            g.dx(3) * g.dx(2), g.dx(3) * (g.dx(2) * g.dx(1)),
+            g.dx(3) * g.dx(2), g.dx(3) * (g.dx(2) * g.dx(1)), return_moved=True,
         )
         du = du.copy()
         du[:, k, j, ng : ng + n] = np.moveaxis(out.reshape(wc.shape[1], wc.shape[2], n, 5), -1, 0)
+        if self.deck.nscalar:
+            # the dry-gas transfer through the face below each interior cell
+            from . import scalar
+
+            mv = np.zeros(g.shape)
+            mv[k, j, ng : ng + n] = moved[:, :n].reshape(wc.shape[1], wc.shape[2], n)
+            self._ds = scalar.implicit_transfer(self._ds, self.s, self.r, mv, g)
         return du
 
     def forward(self, dt, stage):
         d = self.deck
         if stage == 0:
             self.u0 = self.u.copy()
+            if d.nscalar:
+                self.s0 = self.s.copy()
         self._stage = stage
         du = self.tendency(dt)
         self.u = rk_combine(self.u0, self.u, du, stage)
         apply_boundaries(self.u, d.bcs, d.gammad, self.bref, dfloor=d.dfloor, pfloor=d.pfloor)
+        if d.nscalar:
+            from . import scalar
+
+            # the scalar combine, ghosts, then r = s / rho of the new state
+            # (meshblock_advance.cpp:338-341, meshblock_exchange.cpp:195-200)
+            s = rk_combine(self.s0, self.s, self._ds, stage)
+            scalar.boundaries(s, d.bcs)
+            self.s = s
+            self.r = s / self.u[IDN][None]
         self._w_stale = True
 
     def check_redo(self):

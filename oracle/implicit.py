@@ -231,11 +231,14 @@ def block_solve(a, b, c, rhs):
     return d
 
 
-def vic_full(du, w, gamma, grav1, dt, dx1, x1f, x1v, area1, vol):
+def vic_full(du, w, gamma, grav1, dt, dx1, x1f, x1v, area1, vol, return_moved=False):
     """ImplicitHydroImpl::forward_masked, scheme 9, dir 0, no solids, for
     columns of one block.  du: (ncol, n, 5) explicit increments (interior);
     w: (ncol, n + 2, 5); x1f: the n + 1 interior faces, x1v the n centres.
-    Returns the corrected du (ncol, n, 5)."""
+    Returns the corrected du (ncol, n, 5); with return_moved also the
+    clamped dry-gas transfer through the face below each cell (ncol, n + 1),
+    the mass_fix row the tracer transfer reads (vic_redistribute_impl.h:
+    80-90, 139-141)."""
     ncol, n, _ = du.shape
     du0 = du.copy()
     du = du.copy()
@@ -325,6 +328,8 @@ def vic_full(du, w, gamma, grav1, dt, dx1, x1f, x1v, area1, vol):
     work = gvar * centroid_slope(moved_rho, x1v)
     work = work - apply_rows(rows, delta[..., 0] - mass0)
     out[..., IPR] = out[..., IPR] + work
+    if return_moved:
+        return out, moved
     return out
 
 
@@ -504,7 +509,7 @@ def tangential_transfer(du, w, moved, V):
             du[:, i + 1, m] = np.where(nz, du[:, i + 1, m] + p / V, du[:, i + 1, m])
 
 
-def vic_partial(du, w, gamma, grav1, dt, dx1, x1f, x1v, area1, vol):
+def vic_partial(du, w, gamma, grav1, dt, dx1, x1f, x1v, area1, vol, return_moved=False):
     """ImplicitHydroImpl::forward_masked, scheme 1 (the partial 3x3 matrix
     with the m_t fold), dir 0, no solids, for the columns of one block.
     Arguments and result as vic_full."""
@@ -587,4 +592,6 @@ def vic_partial(du, w, gamma, grav1, dt, dx1, x1f, x1v, area1, vol):
     work = gvar * centroid_slope(moved_rho, x1v)
     work = work - apply_rows(rows, delta[..., 0] - mass0)
     out[..., IPR] = out[..., IPR] + work
+    if return_moved:
+        return out, moved
     return out

@@ -162,20 +162,25 @@ SIGNATURES = {
     ),
     "snapy_vic_full_f32": (
         _I,
-        [_GP, _PP, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P],
+        [_GP, _PP, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P, _P],
     ),
     "snapy_vic_partial_workspace": (ctypes.c_int64, [_GP]),
     "snapy_vic_partial": (
         _I,
-        [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P],
+        [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P, _P],
     ),
     "snapy_vic_partial_f32": (
         _I,
-        [_GP, _PP, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P],
+        [_GP, _PP, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P, _P],
     ),
     "snapy_diffusion": (_I, [_GP, _PP, _I, _P, _P * 3, _P, _P, _D, _P, _P]),
-    "snapy_vic_full": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P]),
-    "snapy_vic_full_serial": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P]),
+    "snapy_vic_full": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P, _P]),
+    "snapy_vic_full_serial": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P, _P]),
+    "snapy_scalar_flux": (_I, [_I, _GP, _PP, _I, _P, _P, _I, _P, _P, _P, _P]),
+    "snapy_scalar_theta": (_I, [_GP, _PP, _I, _P, _P, _I, _P * 3, _P * 3, _D, _P, _P, _P]),
+    "snapy_scalar_update": (_I, [_GP, _PP, _I, _P * 3, _P, _P, _P, _P, _P, _P, _D, _P, _D, _D, _D, _P]),
+    "snapy_scalar_prim": (_I, [_GP, _PP, _I, _I, _P, _P, _P, _P, _P]),
+    "snapy_scalar_boundaries": (_I, [_GP, _PP, _I, _P, _P]),
     "snapy_budget_workspace": (ctypes.c_int64, [_GP]),
     "snapy_budget": (_I, [_GP, _I, _P, _P, _P, _P]),
     "snapy_budget_noalloc": (_I, [_GP, _I, _P, _P, _P]),

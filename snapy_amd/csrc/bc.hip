@@ -25,6 +25,8 @@ struct BcArgs {
   int64_t other_s_fast, other_s_slow;
   int bc_lo, bc_hi;
   int prim;    // 1: the field is primitive (kPrimitive), 0 conserved
+  int vn;      // the row whose sign a reflecting face flips; -1: a kScalar
+               // field (tracers, theta), which flips none
 };
 
 __device__ __forceinline__ int64_t row_offset(BcArgs const& a, int row) {
@@ -86,7 +88,7 @@ __global__ void __launch_bounds__(256)
     return;
   }
   int64_t off = row_offset(a, row);
-  int vn = a.axis;  // IVX for x1, IVY for x2, IVZ for x3
+  int vn = a.vn;
   for (int v = 0; v < a.nvar; ++v) {
     T x = q[v * a.r.sv + off + src * a.st];
     if (flip && v == vn) x = x * T(-1);
@@ -149,6 +151,7 @@ BcArgs make_args(snapy_grid const& g, snapy_physics const& p, int axis,
   a.r = ranges(g);
   a.nvar = g.nvar;
   a.axis = axis;
+  a.vn = axis;  // IVX for x1, IVY for x2, IVZ for x3
   a.ng = g.nghost;
   a.prim = prim;
   if (axis == 1) {
@@ -252,6 +255,25 @@ int boundaries(snapy_grid const& g, snapy_physics const& p, T* q,
 using namespace snapy;
 
 extern "C" {
+
+int snapy_scalar_boundaries(const snapy_grid* g, const snapy_physics* p, int ns, double* q,
+                            void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (!p || !q) return fail(-1, "scalar_boundaries: NULL argument");
+  if (ns < 1) return fail(-13, "scalar_boundaries: ns must be >= 1");
+  hipStream_t s = as_stream(stream);
+  for (int ax = 1; ax <= 3; ++ax) {
+    if (!axis_active(*g, ax)) continue;
+    BcArgs a = make_args(*g, *p, ax, 0);
+    if (a.bc_lo == SNAPY_BC_OUTFLOW || a.bc_hi == SNAPY_BC_OUTFLOW)
+      return fail(-11, "scalar_boundaries: no radiating boundary for a scalar field");
+    a.nvar = ns;
+    a.vn = -1;
+    int64_t tot = int64_t(2) * a.ng * a.nrow;
+    hipLaunchKernelGGL(bc_copy_kernel<double>, dim3(blocks_for(tot, 256)), dim3(256), 0, s, q, a);
+  }
+  return launch_status("scalar_boundaries");
+}
 
 int snapy_apply_boundaries(const snapy_grid* g, const snapy_physics* p,
                            int dtype, void* u, const void* bref_w,

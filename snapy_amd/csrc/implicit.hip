@@ -35,6 +35,9 @@ struct VicArgs {
   double dt_host;  // used when dt_dev is NULL
   double wstage;   // the stage weight inside the solve: dt_c = wstage * dt
   double w0, w1, w2;
+  // NULL, or (nc3, nc2, nc1) FP64: the clamped dry-gas transfer through the
+  // face below each interior cell (0 in a failed column)
+  double* moved;
 };
 
 using M5 = double[5][5];
@@ -302,11 +305,13 @@ __global__ void __launch_bounds__(64)
     // a failed column keeps its explicit increments (the reference undoes
     // the column with its status bits and flags a retry)
     vic_mark(status, col, false);
-    for (int i = 0; i < n; ++i)
+    for (int i = 0; i < n; ++i) {
       for (int v = 0; v < 5; ++v) {
         int64_t o = v * r.sv + row + i;
         u[o] = a.w0 * u0[o] + a.w1 * u[o] + a.w2 * du[o];
       }
+      if (a.moved) a.moved[row + i] = 0.;
+    }
     return;
   }
   vic_mark(status, col, true);
@@ -380,6 +385,7 @@ __global__ void __launch_bounds__(64)
   auto moved_rho = [&](int i) { return (DU(IDN, i) + WS(i, 2)) - DU(IDN, i); };
   auto raw_minus_mass0 = [&](int i) { return WS(i, 25) - DU(IDN, i); };
   for (int i = 0; i < n; ++i) {
+    if (a.moved) a.moved[row + i] = WS(i, 3);
     double out[5];
     out[IDN] = DU(IDN, i) + WS(i, 2);
     out[IVX] = WS(i, 26);
@@ -958,6 +964,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
       if (i < n) {
         if (p == IDN) {
           out = M0(i) + WSB(i, 1);
+          if (a.moved && L.live) a.moved[row + i] = failed ? 0. : WSB(i, 2);
         } else if (p == IPR) {
           out = ywb(i);
           double req_lo = WSB(i, 0), req_hi = i + 1 < n ? WSB(i + 1, 0) : 0.;
@@ -1057,7 +1064,7 @@ static int vic_launch(const snapy_grid* g, const snapy_physics* p,
                       const double* x1tab, double t0, double t1, double dt,
                       const double* dt_dev, double wstage, double w0,
                       double w1, double w2, double* ws, int* status,
-                      void* stream, bool serial) {
+                      double* moved, void* stream, bool serial) {
   if (int e = check_grid(g)) return e;
   if (!p || !w || !du || !u0 || !u || !vtab || !x1tab || !ws || !status)
     return fail(-1, "vic_full: NULL argument");
@@ -1079,6 +1086,7 @@ static int vic_launch(const snapy_grid* g, const snapy_physics* p,
   a.w0 = w0;
   a.w1 = w1;
   a.w2 = w2;
+  a.moved = moved;
   int64_t ncol = int64_t(g->nx2) * g->nx3;
   if (serial)
     hipLaunchKernelGGL(vic_full_kernel, dim3(blocks_for(ncol, 64)), dim3(64),
@@ -1107,10 +1115,10 @@ int snapy_vic_full(const snapy_grid* g, const snapy_physics* p, const double* w,
                    const double* vtab, const double* x1tab, double t0,
                    double t1, double dt, const double* dt_dev, double wstage,
                    double w0, double w1, double w2, double* ws, int* status,
-                   void* stream) {
+                   double* moved, void* stream) {
   return vic_launch(g, p, w, du, u0, nullptr, u, nullptr, vtab, x1tab, t0,
-                    t1, dt, dt_dev, wstage, w0, w1, w2, ws, status, stream,
-                    false);
+                    t1, dt, dt_dev, wstage, w0, w1, w2, ws, status, moved,
+                    stream, false);
 }
 
 int snapy_vic_full_f32(const snapy_grid* g, const snapy_physics* p,
@@ -1119,10 +1127,11 @@ int snapy_vic_full_f32(const snapy_grid* g, const snapy_physics* p,
                        const double* vtab, const double* x1tab, double t0,
                        double t1, double dt, const double* dt_dev,
                        double wstage, double w0, double w1, double w2,
-                       double* ws, int* status, void* stream) {
+                       double* ws, int* status, double* moved, void* stream) {
   if (!c0 || !c) return fail(-1, "vic_full_f32: NULL compensation");
   return vic_launch(g, p, w, du, u0, c0, u, c, vtab, x1tab, t0, t1, dt,
-                    dt_dev, wstage, w0, w1, w2, ws, status, stream, false);
+                    dt_dev, wstage, w0, w1, w2, ws, status, moved, stream,
+                    false);
 }
 
 int snapy_vic_full_serial(const snapy_grid* g, const snapy_physics* p,
@@ -1131,10 +1140,10 @@ int snapy_vic_full_serial(const snapy_grid* g, const snapy_physics* p,
                           double t0, double t1, double dt,
                           const double* dt_dev, double wstage, double w0,
                           double w1, double w2, double* ws, int* status,
-                          void* stream) {
+                          double* moved, void* stream) {
   return vic_launch(g, p, w, du, u0, nullptr, u, nullptr, vtab, x1tab, t0,
-                    t1, dt, dt_dev, wstage, w0, w1, w2, ws, status, stream,
-                    true);
+                    t1, dt, dt_dev, wstage, w0, w1, w2, ws, status, moved,
+                    stream, true);
 }
 
 }  // extern "C"

@@ -54,6 +54,9 @@ struct PartArgs {
   double dt_host;
   double wstage;
   double w0, w1, w2;
+  // NULL, or (nc3, nc2, nc1) FP64: the clamped dry-gas transfer through the
+  // face below each interior cell (0 in a failed column)
+  double* moved;
 };
 
 // the stage dt of column col (the member's along a member x3 axis)
@@ -855,6 +858,7 @@ __global__ void __launch_bounds__(kFinThreads)
         u[o] = a.w0 * static_cast<double const*>(u0v)[o] + a.w1 * u[o] + a.w2 * x;
       }
     };
+    if (a.moved) a.moved[row + i] = s_ok[c] ? s_C[c * L + i] : 0.;
     if (!s_ok[c]) {
       for (int v = 0; v < 5; ++v) combine(v, DU(v, i));
       continue;
@@ -927,7 +931,7 @@ static int vicp_launch(const snapy_grid* g, const snapy_physics* p, const double
                        float* c, const double* vtab, const double* x1tab, double t0,
                        double t1, double dt, const double* dt_dev, double wstage,
                        double w0, double w1, double w2, double* ws, int* status,
-                       void* stream) {
+                       double* moved, void* stream) {
   if (int e = check_grid(g)) return e;
   if (!p || !w || !du || !u0 || !u || !vtab || !x1tab || !ws || !status)
     return fail(-1, "vic_partial: NULL argument");
@@ -949,6 +953,7 @@ static int vicp_launch(const snapy_grid* g, const snapy_physics* p, const double
   a.w0 = w0;
   a.w1 = w1;
   a.w2 = w2;
+  a.moved = moved;
   int64_t ncol = int64_t(g->nx2) * g->nx3;
   if (ncol > (int64_t(1) << 30)) return fail(-9, "vic_partial: too many columns");
   // the records (kF per layer, lane-contiguous) then yc (3 per layer)
@@ -983,9 +988,9 @@ int snapy_vic_partial(const snapy_grid* g, const snapy_physics* p, const double*
                       const double* du, const double* u0, double* u, const double* vtab,
                       const double* x1tab, double t0, double t1, double dt,
                       const double* dt_dev, double wstage, double w0, double w1,
-                      double w2, double* ws, int* status, void* stream) {
+                      double w2, double* ws, int* status, double* moved, void* stream) {
   return vicp_launch(g, p, w, du, u0, nullptr, u, nullptr, vtab, x1tab, t0, t1, dt, dt_dev,
-                     wstage, w0, w1, w2, ws, status, stream);
+                     wstage, w0, w1, w2, ws, status, moved, stream);
 }
 
 int snapy_vic_partial_f32(const snapy_grid* g, const snapy_physics* p, const double* w,
@@ -993,10 +998,10 @@ int snapy_vic_partial_f32(const snapy_grid* g, const snapy_physics* p, const dou
                           float* c, const double* vtab, const double* x1tab, double t0,
                           double t1, double dt, const double* dt_dev, double wstage,
                           double w0, double w1, double w2, double* ws, int* status,
-                          void* stream) {
+                          double* moved, void* stream) {
   if (!c0 || !c) return fail(-1, "vic_partial_f32: NULL compensation");
   return vicp_launch(g, p, w, du, u0, c0, u, c, vtab, x1tab, t0, t1, dt, dt_dev, wstage, w0,
-                     w1, w2, ws, status, stream);
+                     w1, w2, ws, status, moved, stream);
 }
 
 }  // extern "C"

@@ -51,8 +51,10 @@ __device__ __forceinline__ float qsqrt<float>(float a) {
 
 // weno5 with the uniform table and scale = false (recon_point.h:114-147;
 // rows reconstruct.cpp:74-86, d = .3,.6,.1 at :84, eps = 1e-6 at :65-67).
+// The weight-averaged candidate split into its numerator sum(a_k p_k) and
+// denominator sum(a_k), which the scaled form needs apart.
 template <typename T>
-__device__ __forceinline__ T weno5_minus(T v0, T v1, T v2, T v3, T v4) {
+__device__ __forceinline__ void weno5_minus_parts(T v0, T v1, T v2, T v3, T v4, T& num, T& den) {
   // candidate rows
   T p0 = T(-1. / 6.) * v0;
   p0 = p0 + T(5. / 6.) * v1;
@@ -83,11 +85,19 @@ __device__ __forceinline__ T weno5_minus(T v0, T v1, T v2, T v3, T v4) {
   T a0 = qdiv(T(.3), sqr(beta0 + T(1e-6)));
   T a1 = qdiv(T(.6), sqr(beta1 + T(1e-6)));
   T a2 = qdiv(T(.1), sqr(beta2 + T(1e-6)));
-  return qdiv(a0 * p0 + a1 * p1 + a2 * p2, a0 + a1 + a2);
+  num = a0 * p0 + a1 * p1 + a2 * p2;
+  den = a0 + a1 + a2;
 }
 
 template <typename T>
-__device__ __forceinline__ T weno5_plus(T v0, T v1, T v2, T v3, T v4) {
+__device__ __forceinline__ T weno5_minus(T v0, T v1, T v2, T v3, T v4) {
+  T num, den;
+  weno5_minus_parts(v0, v1, v2, v3, v4, num, den);
+  return qdiv(num, den);
+}
+
+template <typename T>
+__device__ __forceinline__ void weno5_plus_parts(T v0, T v1, T v2, T v3, T v4, T& num, T& den) {
   // the reversed rows, accumulated in window order as the reference does
   T p0 = T(1. / 3.) * v2;
   p0 = p0 + T(5. / 6.) * v3;
@@ -116,7 +126,15 @@ __device__ __forceinline__ T weno5_plus(T v0, T v1, T v2, T v3, T v4) {
   T a0 = qdiv(T(.3), sqr(beta0 + T(1e-6)));
   T a1 = qdiv(T(.6), sqr(beta1 + T(1e-6)));
   T a2 = qdiv(T(.1), sqr(beta2 + T(1e-6)));
-  return qdiv(a0 * p0 + a1 * p1 + a2 * p2, a0 + a1 + a2);
+  num = a0 * p0 + a1 * p1 + a2 * p2;
+  den = a0 + a1 + a2;
+}
+
+template <typename T>
+__device__ __forceinline__ T weno5_plus(T v0, T v1, T v2, T v3, T v4) {
+  T num, den;
+  weno5_plus_parts(v0, v1, v2, v3, v4, num, den);
+  return qdiv(num, den);
 }
 
 // cp5 (poly_point<T,5>, recon_point.h:54-61; table reconstruct.cpp:65-67)

@@ -94,7 +94,13 @@ class MeshBlock:
                 raise ValueError("members: the deck's x3 must be a single cell (members stack along x3)")
             if options.hydro.eos.type != "ideal-gas":
                 raise ValueError("members: the member axis is built for the ideal-gas stage")
+            if options.scalar.nvar > 0:
+                raise ValueError("members: a block of members carries no passive scalars")
         self.options = options
+        # passive tracers (ScalarImpl, src/scalar/scalar.cpp): ns FP64 rows
+        # scalar_s = rho r with their RK backup, and the stage's fluxes
+        self.nscalar = int(options.scalar.nvar)
+        self._scal = None
         self._cycle = 0
         self.last_redo_causes = 0
         self.intg = Integrator(options.intg)
@@ -273,6 +279,11 @@ class MeshBlock:
             lib = _capi.load()
             total += int(self._vic_workspace(lib)) * 8 + ncell * 8
             total += c.nx2 * c.nx3 * self.members * 4
+            if self.nscalar:
+                total += cells * 8  # the dry-gas transfer
+        if self.nscalar:
+            # s, r, s0, theta, ds and the face fluxes of the active directions
+            total += self.nscalar * cells * 8 * (5 + ndim)
         return int(total)
 
     def _ensure_buffers(self):
@@ -326,6 +337,16 @@ class MeshBlock:
                 "du": torch.zeros(self.shape(), dtype=torch.float64, device=self.device),
                 "scheme": int(self.options.intg.implicit_scheme),
                 "name": "vic_partial" if self.options.intg.implicit_scheme == 1 else "vic_full",
+                "moved": torch.zeros(shape, dtype=torch.float64, device=self.device) if self.nscalar else None,
+            }
+        if self.nscalar:
+            sshape = (self.nscalar,) + shape
+            z = lambda: torch.zeros(sshape, dtype=torch.float64, device=self.device)  # noqa: E731
+            self._scal = {
+                "s0": z(),
+                "theta": z(),
+                "ds": z(),
+                "flux": [z() if a else None for a in active],
             }
 
     def flux_register(self) -> FluxRegister:
@@ -346,6 +367,12 @@ class MeshBlock:
         # primitives are FP64 in both precision forms
         w = w.to(device=self.device, dtype=torch.float64).contiguous()
         vars["hydro_w"] = w
+        if self.nscalar:
+            r = vars.get("scalar_r")
+            want = (self.nscalar,) + self.shape()[1:]
+            if r is None or tuple(r.shape) != want:
+                raise ValueError(f"initialize: scalar_r of shape {want} is required (nvar {self.nscalar} scalars)")
+            vars["scalar_r"] = r.to(device=self.device, dtype=torch.float64).contiguous()
         if "outflow" in self.options.bcnames:
             vars["boundary_reference_w"] = w.clone()
         lib = _capi.load()
@@ -358,6 +385,10 @@ class MeshBlock:
             ),
             "apply_boundaries(prim)",
         )
+        if self.nscalar:
+            # seeded with the conserved (dry) density of the filled block on
+            # every cell (finalize_initialization, meshblock.cpp:380-383)
+            vars["scalar_s"] = w[IDN].unsqueeze(0) * vars["scalar_r"]
         if self.policy.fp32():
             # the FP32 state form: the pair (u, c) of the FP64 conserved state
             u = torch.empty(w.shape, dtype=torch.float32, device=self.device)
@@ -436,6 +467,11 @@ class MeshBlock:
             u = data["hydro_u"].to(device=dev, dtype=torch.float64).contiguous()
             vars["hydro_u"] = u
             self._state.adopt(u)
+        if self.nscalar:
+            for k in ("scalar_s", "scalar_r"):
+                if k not in data:
+                    raise restart.RestartError(f"restart: a block with scalars needs its {k}")
+                vars[k] = data[k].to(device=dev, dtype=torch.float64).contiguous()
         self._cycle = int(data["last_cycle"][0])
         self._w_stale = False
         return vars, float(data["last_time"][0])
@@ -490,6 +526,8 @@ class MeshBlock:
         captures)."""
         if stage == 0:
             self._state.backup()
+            if self.nscalar:
+                self._scal["s0"].copy_(vars["scalar_s"])
         for name, fn in self._stage_calls(vars, stage):
             _capi.check(fn(), name)
 
@@ -552,6 +590,8 @@ class MeshBlock:
                 )
             else:
                 calls.append((f"sweep_x{d}", lambda d=d, t=t, fp=fp: lib.snapy_sweep(d, g, p, dtc, P(w), P(t), P(fp), s)))
+        if self.nscalar:
+            calls += self._scalar_flux_calls(vars, dtc, dtp, s)
         # the diffusion posts: the energy row joins the totals, the momentum
         # rows become the sources' du (before the step fields read the totals)
         dsrc = None
@@ -599,7 +639,7 @@ class MeshBlock:
                         lambda: vic_fn(
                             g, p, P(w), P(vic["du"]), P(u0), P(c0), P(u), P(c), P(vic["tab"]), P(self._x1tab),
                             vic["t0"], vic["t1"], 0.0, dtp, st.wght2(), st.wght0(), st.wght1(), st.wght2(),
-                            P(vic["ws"]), P(vic["status"]), s,
+                            P(vic["ws"]), P(vic["status"]), P(vic["moved"]), s,
                         ),
                     )
                 )
@@ -622,6 +662,8 @@ class MeshBlock:
                         ),
                     )
                 )
+            if self.nscalar:
+                calls += self._scalar_update_calls(vars, st, dtp, s)
             return calls
         if self._vic is not None:
             vic = self._vic
@@ -641,7 +683,8 @@ class MeshBlock:
                     vic["name"],
                     lambda: vic_fn(
                         g, p, P(w), P(vic["du"]), P(u0), P(u), P(vic["tab"]), P(self._x1tab), vic["t0"], vic["t1"],
-                        0.0, dtp, st.wght2(), st.wght0(), st.wght1(), st.wght2(), P(vic["ws"]), P(vic["status"]), s,
+                        0.0, dtp, st.wght2(), st.wght0(), st.wght1(), st.wght2(), P(vic["ws"]), P(vic["status"]),
+                        P(vic["moved"]), s,
                     ),
                 )
             )
@@ -673,7 +716,74 @@ class MeshBlock:
                 ),
             )
         )
+        if self.nscalar:
+            calls += self._scalar_update_calls(vars, st, dtp, s)
         return calls
+
+    def _scalar_cov(self) -> bool:
+        """The scalars' fluxes carry the covariance row, and so the
+        positivity limiter runs (scalar.cpp:146-160, 193-205)."""
+        c = self.options.coord
+        return bool(self.options.hydro.flux_covariance) and c.nc1() >= 3 and (c.nx2 > 1 or c.nx3 > 1)
+
+    def _scalar_flux_calls(self, vars, dtc, dtp, s):
+        """The scalars' face fluxes of the stage (ScalarImpl::forward,
+        scalar.cpp:104-174) on the hydro's mass fluxes, then with the
+        covariance the positivity limiter's theta (scalar.cpp:193-205)."""
+        lib = _capi.load()
+        g, p, P = ctypes.byref(self._grid), ctypes.byref(self._phys), _capi.ptr
+        sc, freg, ns = self._scal, self._freg, self.nscalar
+        w, sv = vars["hydro_w"], vars["scalar_s"]
+        calls = []
+        for d in (1, 2, 3):
+            if sc["flux"][d - 1] is None:
+                continue
+            calls.append(
+                (
+                    f"scalar_flux_x{d}",
+                    lambda d=d: lib.snapy_scalar_flux(
+                        d, g, p, ns, P(w), P(sv), dtc, P(freg.total(d)), P(self._x1tab), P(sc["flux"][d - 1]), s
+                    ),
+                )
+            )
+        if self._scalar_cov():
+            hf = (ctypes.c_void_p * 3)(*[P(freg.total(d)) for d in (1, 2, 3)])
+            sf = (ctypes.c_void_p * 3)(*[P(f) for f in sc["flux"]])
+            calls.append(
+                (
+                    "scalar_theta",
+                    lambda: lib.snapy_scalar_theta(g, p, ns, P(w), P(sv), dtc, hf, sf, 0.0, dtp, P(sc["theta"]), s),
+                )
+            )
+        return calls
+
+    def _scalar_update_calls(self, vars, st, dtp, s):
+        """ds = -dt div(F), the implicit tracer transfer, the RK combine and
+        the ghosts of s (meshblock_advance.cpp:212-233, 338-340), then
+        r = s / rho of the new state (set_scalar_primitive)."""
+        lib = _capi.load()
+        g, p, P = ctypes.byref(self._grid), ctypes.byref(self._phys), _capi.ptr
+        sc, ns = self._scal, self.nscalar
+        sv, rv, u = vars["scalar_s"], vars["scalar_r"], vars["hydro_u"]
+        sf = (ctypes.c_void_p * 3)(*[P(f) for f in sc["flux"]])
+        theta = P(sc["theta"]) if self._scalar_cov() else None
+        moved = P(self._vic["moved"]) if self._vic is not None else None
+        comp = self._state.comp()
+        return [
+            (
+                "scalar_update",
+                lambda: lib.snapy_scalar_update(
+                    g, p, ns, sf, theta, P(rv), moved, P(sc["s0"]), P(sv), P(sc["ds"]), 0.0, dtp,
+                    st.wght0(), st.wght1(), st.wght2(), s,
+                ),
+            ),
+            (
+                "scalar_prim",
+                lambda: lib.snapy_scalar_prim(
+                    g, p, ns, _capi.dtype_code(u.dtype), P(u), P(comp), P(sv), P(rv), s
+                ),
+            ),
+        ]
 
     def device_time_step(self, vars) -> torch.Tensor:
         """max_time_step without the host read: the block's device dt
@@ -766,10 +876,28 @@ class MeshBlock:
         if self.intg.current_redo > self.intg.max_redo and not (causes & self.REDO_VIC):
             return -1
         self._state.restore()
+        if self.nscalar:
+            # the scalars' backup too (StateStore::restore, state_store.hpp:
+            # 62-65), and their primitives of the restored state
+            vars["scalar_s"].copy_(self._scal["s0"])
+            self._scalar_prim(vars)
         self._refresh_primitives(vars)
         if self.intg.current_redo > self.intg.max_redo:
             return -1
         return 1
+
+    def _scalar_prim(self, vars) -> None:
+        """scalar_r <- scalar_s / rho of hydro_u (interior, then its ghosts)."""
+        lib = _capi.load()
+        u = vars["hydro_u"]
+        _capi.check(
+            lib.snapy_scalar_prim(
+                ctypes.byref(self._grid), ctypes.byref(self._phys), self.nscalar, _capi.dtype_code(u.dtype),
+                _capi.ptr(u), _capi.ptr(self._state.comp()), _capi.ptr(vars["scalar_s"]),
+                _capi.ptr(vars["scalar_r"]), _capi.stream_of(self.device),
+            ),
+            "scalar_prim",
+        )
 
     def total_u(self, vars):
         """The conserved state; in the FP32 state form the FP64 value of the
@@ -854,6 +982,8 @@ class StageExecutor:
             vars["hydro_u"].data_ptr(),
             vars["hydro_w"].data_ptr(),
             b._state.u0().data_ptr(),
+            vars["scalar_s"].data_ptr() if "scalar_s" in vars else 0,
+            vars["scalar_r"].data_ptr() if "scalar_r" in vars else 0,
         )
 
     @property

@@ -4,10 +4,11 @@ Restates the YAML surface the reference reads for this path
 (MeshBlockOptionsImpl::from_yaml, src/mesh/meshblock_options.cpp:18-77, and
 the per-module readers it calls).  Keys the stage does not implement raise a
 ValueError naming the key, as the reference's strict key checks do
-(src/input/check_keys.cpp).  Sections that would add state rows or physics
-the stage does not carry are refused by name: `scalar` with nvar > 0,
-`species` beyond the single dry gas, `reactions`, and a `distribute` that
-splits the domain over blocks.  Sections read by the driver rather than the
+(src/input/check_keys.cpp).  The `scalar` section (passive tracers,
+ScalarOptionsImpl::from_yaml, src/scalar/scalar_options.cpp:11-57) is read;
+sections that would add state rows or physics the stage does not carry are
+refused by name: `species` beyond the single dry gas, `reactions`, and a
+`distribute` that splits the domain over blocks.  Sections read by the driver rather than the
 stage (`outputs`, `problem`) and the dry gas's inert `reference-state` are
 accepted.
 """
@@ -134,10 +135,25 @@ class IntegratorOptions:
 
 
 @dataclass
+class ScalarOptions:
+    """ScalarOptionsImpl (src/scalar/scalar.hpp:24-58): nvar passive tracers
+    named `names`, reconstructed with the horizontal reconstruction's type
+    scale-invariant (meshblock_options.cpp:37-40, scalar.cpp:31-37) and
+    transported by the upwind solver on the hydro mass flux."""
+
+    nvar: int = 0
+    names: list = field(default_factory=list)
+    upper_bound: float = -1.0
+    recon_type: str = "weno5"
+    riemann: str = "upwind"
+
+
+@dataclass
 class MeshBlockOptions:
     coord: CoordinateOptions = field(default_factory=CoordinateOptions)
     hydro: HydroOptions = field(default_factory=HydroOptions)
     intg: IntegratorOptions = field(default_factory=IntegratorOptions)
+    scalar: ScalarOptions = field(default_factory=ScalarOptions)
     bcnames: list = field(default_factory=lambda: ["reflecting"] * 6)
     verbose: bool = False
     filename: str = ""
@@ -180,6 +196,7 @@ class MeshBlockOptions:
         op.hydro = _read_dynamics(cfg.get("dynamics") or {}, cfg.get("forcing") or {})
         op.intg = _read_integration(cfg.get("integration") or {})
         op.bcnames = _read_bcs(cfg.get("boundary-condition") or {})
+        op.scalar = _read_scalar(cfg.get("scalar"), op.hydro.recon23)
         _check_supported(op)
         return op
 
@@ -187,18 +204,60 @@ class MeshBlockOptions:
 _DIST_KEYS = {"layout", "nb1", "nb2", "nb3", "blocks_per_process", "verbose"}
 
 
+def _read_scalar(node, recon23) -> ScalarOptions:
+    """ScalarOptionsImpl::from_yaml (scalar_options.cpp:11-57): nvar, names
+    (their count sets nvar), upper-bound, reconstruct {type, shock} (default:
+    the horizontal reconstruction), riemann-solver (upwind)."""
+    sc = ScalarOptions()
+    sc.recon_type = recon23.type
+    if not node:
+        return sc
+    _check_keys(node, "scalar", {"verbose", "nvar", "upper-bound", "names", "reconstruct", "riemann-solver"})
+    sc.nvar = int(node.get("nvar", 0))
+    sc.upper_bound = float(node.get("upper-bound", sc.upper_bound))
+    if node.get("names"):
+        sc.names = [str(x) for x in node["names"]]
+        sc.nvar = len(sc.names)
+    rc = node.get("reconstruct")
+    if rc is not None:
+        _check_keys(rc, "scalar/reconstruct", {"shock", "type"})
+        sc.recon_type = rc.get("type", sc.recon_type)
+    rs = node.get("riemann-solver")
+    if rs is not None:
+        _check_keys(rs, "scalar/riemann-solver", {"type", "dir"})
+        sc.riemann = rs.get("type", "upwind")
+    if not sc.names:
+        sc.names = [f"scalar{n}" for n in range(sc.nvar)]
+    return sc
+
+
+def _check_scalar(op) -> None:
+    """What the tracer stage carries (ScalarImpl::reset, scalar.cpp:21-48,
+    plus this build's scope)."""
+    sc = op.scalar
+    if sc.nvar <= 0:
+        return
+    if sc.riemann != "upwind":
+        raise ValueError("[Scalar] Passive scalars currently require the upwind solver")
+    if sc.upper_bound == 0.0:
+        raise ValueError("[Scalar] upper-bound = 0 pins the tracer to zero and freezes transport; "
+                         "use a negative value to disable the bound")
+    if sc.upper_bound > 0.0:
+        raise ValueError("[Scalar] upper-bound needs the eos limiter, which is not implemented")
+    if sc.recon_type != "weno5":
+        raise ValueError(f"scalar/reconstruct/type '{sc.recon_type}': weno5 only")
+    if op.hydro.eos.type != "ideal-gas" or op.coord.type != "cartesian":
+        raise ValueError("scalar: tracers ride the Cartesian ideal-gas stage")
+    if "outflow" in op.bcnames:
+        raise ValueError("scalar: a radiating (outflow) boundary for the tracers is not implemented")
+
+
 def _check_sections(cfg) -> None:
     """The deck sections that would add state rows or physics this stage
-    does not carry are refused by name: transported scalars (tracers),
-    species beyond the single dry gas, reactions.  `outputs` (I/O) and
-    `problem` (driver parameters) are read by the caller, not the stage;
-    `reference-state` is the thermodynamic reference of moist species, inert
-    for the dry ideal gas."""
-    sc = cfg.get("scalar")
-    if sc:
-        _check_keys(sc, "scalar", {"nvar", "names"})
-        if int(sc.get("nvar", 0)) > 0:
-            raise ValueError(f"scalar: nvar {int(sc['nvar'])} (transported tracers) is not implemented")
+    does not carry are refused by name: species beyond the single dry gas,
+    reactions.  `outputs` (I/O) and `problem` (driver parameters) are read
+    by the caller, not the stage; `reference-state` is the thermodynamic
+    reference of moist species, inert for the dry ideal gas."""
     sp = cfg.get("species")
     if sp is not None:
         if not isinstance(sp, list):
@@ -355,6 +414,7 @@ def _read_bcs(bc) -> list:
 
 def _check_supported(op: MeshBlockOptions, layout: str = "") -> None:
     """Refuse deck features outside the implemented stage, loudly."""
+    _check_scalar(op)
     if op.coord.type == "gnomonic-equiangle":
         _check_cubed(op, layout)
         return
@@ -479,6 +539,7 @@ class MeshOptions:
         op.hydro = _read_dynamics(cfg.get("dynamics") or {}, cfg.get("forcing") or {})
         op.intg = _read_integration(cfg.get("integration") or {})
         op.bcnames = _read_bcs(cfg.get("boundary-condition") or {})
+        op.scalar = _read_scalar(cfg.get("scalar"), op.hydro.recon23)
         _check_supported(op, mo.layout)
         mo.block = op
         return mo

@@ -7,7 +7,8 @@ them with kintera's tensor archive; here the archive is safetensors, which
 loads without executing anything from the file.
 
 What a part holds (the reference's keys where it has them):
-- every tensor of the block's variables (`hydro_w`, `boundary_reference_w`);
+- every tensor of the block's variables (`hydro_w`, `boundary_reference_w`,
+  and with passive scalars `scalar_s`, `scalar_r`);
 - `hydro_u`: the conserved totals, FP64;
 - `hydro_u_stored`: the stored state's own bits, and for the FP32 state form
   `hydro_u_comp`, its compensation word (this build's pair), so a resume of
@@ -54,7 +55,7 @@ def restart_variables(block, vars, time: float) -> dict:
     if getattr(block, "_w_stale", False):
         block._refresh_primitives(vars)
     out = {}
-    for k in ("hydro_w", "boundary_reference_w"):
+    for k in ("hydro_w", "boundary_reference_w", "scalar_s", "scalar_r"):
         if vars.get(k) is not None:
             out[k] = vars[k].detach().to("cpu").contiguous()
     u = vars["hydro_u"]

@@ -32,7 +32,8 @@ _BC = {"reflecting": 1, "periodic": 2, "extrapolation": 3, "outflow": 4}
 
 def eligible(block: MeshBlock) -> bool:
     """A block the work list can carry: x1 only, ideal gas (5 rows), FP64,
-    no gravity, diffusion or implicit solve, nghost 3, on a HIP device."""
+    no gravity, diffusion, implicit solve or tracers, nghost 3, on a HIP
+    device."""
     c = block.options.coord
     h = block.options.hydro
     return (
@@ -46,6 +47,7 @@ def eligible(block: MeshBlock) -> bool:
         and h.nu_iso == 0.0
         and h.kappa_iso == 0.0
         and block.options.intg.implicit_scheme == 0
+        and block.nscalar == 0
         and block.device.type == "cuda"
         and all(b in _BC for b in block.options.bcnames[:2])
     )

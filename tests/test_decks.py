@@ -19,7 +19,7 @@ REF_EXAMPLES = "/root/reference/examples"
 # deck -> None (parses) or the key names one of which the refusal must name
 REFERENCE_DECKS = {
     "shock.yaml": None,
-    "straka.yaml": ("distribute/nb2", "scalar"),
+    "straka.yaml": ("distribute/nb2",),
     "shallow_xy.yaml": ("distribute/nb2", "scalar"),
     "shallow_splash.yaml": ("scalar",),
     "planar_convection.yaml": ("reconstruct/vertical/scale", "forcing/diffusion"),
@@ -74,7 +74,12 @@ def _deck(tmp_path, extra):
 @pytest.mark.parametrize(
     "extra,key",
     [
-        ("scalar: {nvar: 1, names: [tracer]}\n", "scalar"),
+        ("scalar: {nvar: 1, upper-bound: 1.}\n", "upper-bound"),
+        ("scalar: {nvar: 1, riemann-solver: {type: hllc}}\n", "upwind"),
+        ("scalar: {nvar: 1, reconstruct: {type: cp5}}\n", "scalar/reconstruct/type"),
+        ("scalar: {nvar: 1, colour: red}\n", "colour"),
+        ("scalar: {nvar: 1}\nboundary-condition: {external: {x1-inner: outflow, x1-outer: outflow}}\n",
+         "outflow"),
         ("species:\n  - {name: dry}\n  - {name: H2O}\n", "species"),
         ("reactions:\n  - {equation: H2O => H2O(l)}\n", "reactions"),
         ("distribute: {layout: slab, nb2: 2}\n", "distribute/nb2"),
@@ -84,6 +89,14 @@ def _deck(tmp_path, extra):
 def test_state_changing_sections_are_refused(tmp_path, extra, key):
     with pytest.raises(ValueError, match=key):
         MeshBlockOptions.from_yaml(_deck(tmp_path, extra))
+
+
+def test_scalar_section_is_read(tmp_path):
+    op = MeshBlockOptions.from_yaml(_deck(tmp_path, "scalar: {nvar: 2, names: [tracer, dye]}\n"))
+    assert op.scalar.nvar == 2 and op.scalar.names == ["tracer", "dye"]
+    assert op.scalar.recon_type == "weno5" and op.scalar.riemann == "upwind"
+    op = MeshBlockOptions.from_yaml(_deck(tmp_path, "scalar: {nvar: 1}\n"))
+    assert op.scalar.names == ["scalar0"]
 
 
 def test_inert_sections_are_accepted(tmp_path):

@@ -85,7 +85,7 @@ def test_vic_against_fixture(scheme):
     print(f"golden: vic scheme {scheme}", flush=True)
     rc = entry(ctypes.byref(gd), ctypes.byref(pd), wd.data_ptr(), dud.data_ptr(), u0.data_ptr(), u.data_ptr(),
                tabd.data_ptr(), xtd.data_ptr(), t0, t1, mg.VIC_DT, None, 1.0, 0.0, 0.0, 1.0, ws.data_ptr(),
-               st.data_ptr(), None)
+               st.data_ptr(), None, None)
     assert rc == 0, lib.snapy_last_error()
     assert int(host(st).max()) == 0
     got = host(u)

@@ -50,9 +50,10 @@ def test_vic_kernel_matches_oracle(shape, entry):
     u = torch.zeros_like(u0)
     tabd, xtd, wd, dud = tab.cuda(), xt.cuda(), dev(w), dev(du)
     print(f"vic kernel {entry} {shape}", flush=True)
+    moved = torch.full(g.shape, -7.0, dtype=torch.float64, device="cuda")
     rc = getattr(lib, entry)(ctypes.byref(gd), ctypes.byref(pd), wd.data_ptr(), dud.data_ptr(), u0.data_ptr(),
                             u.data_ptr(), tabd.data_ptr(), xtd.data_ptr(), t0, t1, dt, None, 1.0, 0.0, 0.0, 1.0,
-                            ws.data_ptr(), st.data_ptr(), None)
+                            ws.data_ptr(), st.data_ptr(), moved.data_ptr(), None)
     assert rc == 0, lib.snapy_last_error()
     assert int(host(st).max()) == 0
     got = host(u)
@@ -64,12 +65,20 @@ def test_vic_kernel_matches_oracle(shape, entry):
     wcol = np.moveaxis(wc, 0, -1).reshape(ncol, n + 2, 5)
     dcol = np.moveaxis(du[:, k, j, 3 : 3 + n], 0, -1).reshape(ncol, n, 5)
     solve = implicit.vic_partial if scheme == 1 else implicit.vic_full
-    ref = solve(dcol, wcol, GAMMA, -9.81, dt, g.dx(1), g.faces(1)[3 : 3 + n + 1], g.centers(1)[3 : 3 + n], a1, vol)
+    ref, ref_moved = solve(dcol, wcol, GAMMA, -9.81, dt, g.dx(1), g.faces(1)[3 : 3 + n + 1], g.centers(1)[3 : 3 + n],
+                           a1, vol, return_moved=True)
     gc = np.moveaxis(got[:, k, j, 3 : 3 + n], 0, -1).reshape(ncol, n, 5)
     for v in range(5):
         err = rel_l2(gc[..., v], ref[..., v])
         print("row", v, err, flush=True)
         assert err < 1e-11
+    # the dry-gas transfer through the face below each cell, the passive
+    # scalars' input (0 through the column's bottom face)
+    mv = host(moved)[k, j, 3 : 3 + n].reshape(ncol, n)
+    assert np.array_equal(mv[:, 0], np.zeros(ncol))
+    err = rel_l2(mv, ref_moved[:, :n])
+    print("moved", err, flush=True)
+    assert err < 1e-11
 
 
 CASES = [

@@ -57,7 +57,7 @@ def _vic_call(lib, scheme, g, w, du, st, dt=0.7):
     tabd, xtd, wd, dud = tab.cuda(), xt.cuda(), dev(w), dev(du)
     rc = entry(ctypes.byref(gd), ctypes.byref(pd), wd.data_ptr(), dud.data_ptr(), u0.data_ptr(),
                u.data_ptr(), tabd.data_ptr(), xtd.data_ptr(), t0, t1, dt, None, 1.0, 0.0, 0.0, 1.0, ws.data_ptr(),
-               st.data_ptr(), None)
+               st.data_ptr(), None, None)
     assert rc == 0, lib.snapy_last_error()
     out = host(u)
     del wd, dud, tabd, xtd, ws
