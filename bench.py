@@ -276,7 +276,7 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "convection3d_hbm_traffic.md")
 STAGE_KERNELS = {
     "vic_partial": (["vicp_forward_kernel", "vicp_finish_kernel<false>"],
                     ["vicp_forward_kernel", "vicp_finish_kernel<true>"]),
-    "sweep_x1_wb": (["sweep_x1_wb_ref_kernel<double, double>"], ["sweep_x1_wb_ref_kernel<double, float>"]),
+    "sweep_x1_wb": (["sweep_x1_wb_ref_kernel<double, double>"], ["sweep_x1_wb_ref_kernel<float, float>"]),
     "stage_increment": (["update_gravity_kernel<double, true, false>"], ["update_gravity_kernel<float, true, true>"]),
 }
 
