@@ -19,7 +19,12 @@ produces them:
   16 x 6 stratified block with random increments;
 * swe_seams.npz: the cubed-sphere shallow-water panel-edge ghost fill
   (interpolation orders 2 and 4, conserved) and one stage's x2/x3 face
-  fluxes with the seams reconciled, on 8 x 8 panels.
+  fluxes with the seams reconciled, on 8 x 8 panels;
+* tracer_block.npz: two passive tracers (a sharp slab and a height ramp)
+  on a 16 x 12 stratified block with gravity, diffusion, the covariance
+  with its positivity limiter and the scheme-9 implicit correction with
+  the donor-capped tracer transfer: the initial state, every step's dt and
+  the final hydro and tracer states after 2 RK3 steps.
 """
 
 from __future__ import annotations
@@ -205,7 +210,43 @@ def swe_seams():
     return out
 
 
-CASES = {"sod_c1": sod_c1, "sweeps": sweeps, "vic_columns": vic_columns, "swe_seams": swe_seams}
+TRACER_GRID = dict(nx1=16, nx2=12, nx3=1, x1min=0.0, x1max=6400.0, x2min=-6400.0, x2max=6400.0, x3min=0.0,
+                   x3max=100.0)
+TRACER_NU = 25.0
+
+
+def tracer_grid():
+    a = TRACER_GRID
+    return fv.Grid(a["nx1"], a["nx2"], a["nx3"], a["x1min"], a["x1max"], a["x2min"], a["x2max"], a["x3min"],
+                   a["x3max"])
+
+
+def tracer_deck():
+    return fv.Deck(grid=tracer_grid(), gammad=GAMMA_DRY, riemann="lmars", shock_x1=False, shock_x23=False,
+                   bcs=("reflecting",) * 6, grav1=VIC_GRAV, cfl=0.9, implicit_scheme=9, flux_covariance=True,
+                   nu_iso=TRACER_NU, kappa_iso=TRACER_NU, nscalar=2)
+
+
+def tracer_block():
+    d = tracer_deck()
+    g = d.grid
+    w = stratified(g, seed=6, noise=0.5)
+    x2 = g.centers(2)[None, :, None] + np.zeros(g.shape)
+    z = g.centers(1)[None, None, :] + np.zeros(g.shape)
+    r = np.stack([np.where(np.abs(x2) < 2000.0, 1.0, 0.0), 0.2 + z / (g.x1max - g.x1min)])
+    b = fv.Block(d)
+    b.initialize(w, r=r)
+    dts = []
+    for _ in range(2):
+        dt = b.max_time_step()
+        dts.append(dt)
+        for s in range(3):
+            b.forward(dt, s)
+    return {"w": w, "r": r, "dts": np.array(dts), "u": b.u, "s": b.s}
+
+
+CASES = {"sod_c1": sod_c1, "sweeps": sweeps, "vic_columns": vic_columns, "swe_seams": swe_seams,
+         "tracer_block": tracer_block}
 
 
 def path(name):
