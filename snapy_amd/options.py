@@ -9,8 +9,8 @@ ScalarOptionsImpl::from_yaml, src/scalar/scalar_options.cpp:11-57) is read;
 sections that would add state rows or physics the stage does not carry are
 refused by name: `species` beyond the single dry gas, `reactions`, and a
 `distribute` that splits the domain over blocks.  Sections read by the driver rather than the
-stage (`outputs`, `problem`) and the dry gas's inert `reference-state` are
-accepted.
+stage (`problem`) and the dry gas's inert `reference-state` are accepted; the
+`outputs` list is read into `MeshBlockOptions.outputs` (snapy_amd.output).
 """
 
 from __future__ import annotations
@@ -154,6 +154,8 @@ class MeshBlockOptions:
     hydro: HydroOptions = field(default_factory=HydroOptions)
     intg: IntegratorOptions = field(default_factory=IntegratorOptions)
     scalar: ScalarOptions = field(default_factory=ScalarOptions)
+    # the deck's outputs (snapy_amd.output.OutputOptions), in deck order
+    outputs: list = field(default_factory=list)
     bcnames: list = field(default_factory=lambda: ["reflecting"] * 6)
     verbose: bool = False
     filename: str = ""
@@ -197,6 +199,9 @@ class MeshBlockOptions:
         op.intg = _read_integration(cfg.get("integration") or {})
         op.bcnames = _read_bcs(cfg.get("boundary-condition") or {})
         op.scalar = _read_scalar(cfg.get("scalar"), op.hydro.recon23)
+        from .output import read_outputs
+
+        op.outputs = read_outputs(cfg.get("outputs"))
         _check_supported(op)
         return op
 
