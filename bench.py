@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--implicit-scheme", type=int, default=None, choices=(1, 9),
                     help="override the convection deck's implicit scheme")
     ap.add_argument("--cubed-steps", type=int, default=50)
+    ap.add_argument("--accuracy-steps", type=int, default=200,
+                    help="steps of the FP32-vs-FP64 accuracy run (the deck's nlim)")
     ap.add_argument("--ensemble-members", type=int, default=32,
                     help="members of the density-current ensemble line (one launch per kernel)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -313,22 +315,8 @@ def timed_run(block, vars, steps, warmup):
     return time.perf_counter() - t0
 
 
-def fp32_accuracy(name, nsteps=3):
-    """Relative L2 of the FP32 form against the FP64 GPU form (itself within
-    1e-12 of the FP64 oracle) after `nsteps` steps from the same state with
-    the same host dt, per row group; and the FP32 form's relative mass
-    drift."""
-    import numpy as np
-
-    b64, v64, _, _, _, _ = workload(name, "float64", execution="eager")
-    b32, v32, _, _, _, _ = workload(name, "float32", execution="eager")
-    m0 = float(b32.budget(v32)[0].item())
-    for _ in range(nsteps):
-        dt = b64.max_time_step(v64)
-        for s in range(3):
-            b64.forward(v64, dt, s)
-            b32.forward(v32, dt, s)
-    m1 = float(b32.budget(v32)[0].item())
+def _rel_l2_rows(b64, v64, b32, v32):
+    """Relative L2 of the FP32 form against the FP64 form per row group."""
     sl = tuple(b64.part(exterior=False))
     u64 = b64.total_u(v64)[sl].double()
     u32 = b32.total_u(v32)[sl].double()
@@ -337,15 +325,106 @@ def fp32_accuracy(name, nsteps=3):
         d = (u32[rows] - u64[rows]).square().sum().sqrt()
         n = u64[rows].square().sum().sqrt()
         out[key] = float((d / n).item())
+    return out
+
+
+def _advance_pair(b64, v64, b32, v32, nsteps=None, tlim=None):
+    """Advance both forms from the same state with the FP64 form's host dt
+    (the last step cut to reach tlim); returns (steps, time)."""
+    t, n = 0.0, 0
+    while (nsteps is None or n < nsteps) and (tlim is None or t < tlim * (1.0 - 1e-12)):
+        dt = b64.max_time_step(v64)
+        if tlim is not None:
+            dt = min(dt, tlim - t)
+        for s in range(3):
+            b64.forward(v64, dt, s)
+            b32.forward(v32, dt, s)
+        t += dt
+        n += 1
+    return n, t
+
+
+def fp32_accuracy(name, nsteps=200):
+    """Relative L2 of the FP32 form against the FP64 GPU form (itself within
+    1e-12 of the FP64 oracle) after `nsteps` steps (the deck's nlim) from the
+    same state with the same host dt, per row group; and the FP32 form's
+    relative mass drift."""
+    b64, v64, _, _, _, _ = workload(name, "float64", execution="eager")
+    b32, v32, _, _, _, _ = workload(name, "float32", execution="eager")
+    m0 = float(b32.budget(v32)[0].item())
+    n, t = _advance_pair(b64, v64, b32, v32, nsteps=nsteps)
+    m1 = float(b32.budget(v32)[0].item())
+    out = _rel_l2_rows(b64, v64, b32, v32)
     out["mass_drift"] = abs(m1 - m0) / abs(m0)
-    out["steps"] = nsteps
-    del b64, v64, b32, v32
+    out["steps"] = n
+    out["time_s"] = t
+    b64.release()
+    b32.release()
+    return out
+
+
+def straka_diagnostics(block, vars, theta0=300.0, p0=1.0e5):
+    """The density current's published diagnostics: the minimum potential
+    temperature perturbation and the front position, the largest x2 where
+    theta' reaches -1 K in the lowest layer (linear between cells)."""
+    import numpy as np
+
+    sl = tuple(block.part(exterior=False))
+    w = vars["hydro_w"][sl].double().cpu().numpy()  # (5, x3, x2, x1)
+    op = block.options.hydro.eos
+    rd = 8.31446261815324 / op.weight
+    cp = op.gammad / (op.gammad - 1.0) * rd
+    t = w[4] / (rd * w[0])
+    th = t * (p0 / w[4]) ** (rd / cp) - theta0
+    x2 = block.module("coord").buffer("x2v").numpy()
+    c = block.options.coord
+    x2 = x2[c.nghost : c.nghost + c.nx2]
+    row = th[:, :, 0].min(axis=0)  # the lowest layer along x2
+    front = None
+    for j in range(len(x2) - 1, 0, -1):
+        if x2[j - 1] >= 0.0 and row[j - 1] <= -1.0 < row[j]:
+            a = (-1.0 - row[j - 1]) / (row[j] - row[j - 1])
+            front = float(x2[j - 1] + a * (x2[j] - x2[j - 1]))
+            break
+    return {"theta_prime_min_K": float(th.min()), "front_km": None if front is None else front / 1e3}
+
+
+def density_current_accuracy(tlim=900.0):
+    """The density current deck (512 x 64, scheme 9, viscosity) run to its
+    tlim in both forms from the same state with the FP64 form's host dt:
+    the FP32 form's relative L2 against the FP64 GPU form, its mass drift,
+    and each form's minimum theta' and front position beside the published
+    ones (Straka et al. 1993, 25 m reference solution: -9.77 K, 15.54 km;
+    that solution also diffuses potential temperature, which this deck does
+    not, so its minimum is deeper)."""
+    from snapy_amd import problems
+
+    blocks = {}
+    for prec in ("float64", "float32"):
+        b = make_block("density_current.yaml", execution="eager", precision=prec)
+        v, _ = b.initialize({"hydro_w": problems.density_current(b).cuda()})
+        blocks[prec] = (b, v)
+    (b64, v64), (b32, v32) = blocks["float64"], blocks["float32"]
+    m0 = float(b32.budget(v32)[0].item())
+    n, t = _advance_pair(b64, v64, b32, v32, tlim=tlim)
+    m1 = float(b32.budget(v32)[0].item())
+    out = _rel_l2_rows(b64, v64, b32, v32)
+    out["mass_drift"] = abs(m1 - m0) / abs(m0)
+    out["steps"] = n
+    out["time_s"] = t
+    out["fp64_form"] = straka_diagnostics(b64, v64)
+    out["fp32_form"] = straka_diagnostics(b32, v32)
+    out["published"] = {"theta_prime_min_K": -9.77, "front_km": 15.54,
+                        "source": "Straka et al. 1993, Int. J. Numer. Meth. Fluids 17, 25 m reference, "
+                                  "K = 75 m^2/s on momentum and potential temperature"}
+    b64.release()
+    b32.release()
     return out
 
 
 def density_current_leg(steps=50, warmup=3, members=1, seed=2025):
     """The density current deck (512 x 64, dx = dz = 100 m, LMARS, isotropic
-    diffusion, the vertical implicit scheme the deck states) in the FP32
+    viscosity, the vertical implicit scheme the deck states) in the FP32
     state form, captured stages: one block, or `members` ensemble members
     (cold bubbles dT ~ U(-20, -5) K from `seed`) on the member x3 axis, each
     with its own dt, advanced by one launch per kernel.  The throughput is
@@ -388,7 +467,7 @@ def density_current_leg(steps=50, warmup=3, members=1, seed=2025):
         "ms_per_step": el / steps * 1e3,
         "dtype": "f32",
         "implicit_scheme": block.options.intg.implicit_scheme,
-        "form": "FP32 state form, diffusion, FP64 vertical implicit solve, captured stages",
+        "form": "FP32 state form, viscosity, FP64 vertical implicit solve, captured stages",
         "stage_b_alg_per_cell": b_alg,
         "stage_frac": value * b_alg / (HBM_PEAK_GBS * 1e9),
         "state_finite": finite,
@@ -644,8 +723,13 @@ def main():
     if precision == "float32":
         del block, vars
         torch.cuda.empty_cache()
-        accuracy = fp32_accuracy(args.workload)
+        accuracy = fp32_accuracy(args.workload, nsteps=args.accuracy_steps)
         print(f"# FP32 vs FP64 {accuracy}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        if not args.no_other_scheme:
+            accuracy["density_current"] = density_current_accuracy()
+            print(f"# density current to 900 s {accuracy['density_current']}", file=sys.stderr, flush=True)
+            torch.cuda.empty_cache()
     # the metric as the baseline names it: the FP32 form is `value`, the
     # FP64-plain form sits beside it
     serving_summary = None

@@ -20,15 +20,26 @@ def sod(block, x0=0.5, left=(1.0, 0.0, 1.0), right=(0.125, 0.0, 0.1)):
     return torch.from_numpy(w)
 
 
-def isentropic(block, theta=300.0, psfc=1.0e5, grav=9.81):
+def _grav(block, grav):
+    """The gravity a state is balanced for: the given magnitude, else the
+    block's own x1 gravity (the reference's example drivers read it from the
+    deck, examples/straka.cpp:62), else 9.81."""
+    if grav is not None:
+        return float(grav)
+    g1 = float(getattr(block.options.hydro, "grav1", 0.0) or 0.0)
+    return -g1 if g1 != 0.0 else 9.81
+
+
+def isentropic(block, theta=300.0, psfc=1.0e5, grav=None):
     """Dry isentropic column: T = theta - g z / cp, p = psfc (T/theta)^(cp/Rd)."""
+    grav = _grav(block, grav)
     z = block.module("coord").buffer("x1v").numpy()[None, None, :]
     t = theta - grav * z / CP + 0.0 * np.zeros(block.shape()[1:])
     p = psfc * (t / theta) ** (CP / RD)
     return z, t, p
 
 
-def convection3d(block, seed=2024, amp=0.1, layers=10, theta=300.0, psfc=1.0e5, grav=9.81):
+def convection3d(block, seed=2024, amp=0.1, layers=10, theta=300.0, psfc=1.0e5, grav=None):
     """3D dry convection: isentropic base, theta noise U(-amp, amp) K in the
     lowest `layers` interior layers from a seeded generator, at rest."""
     z, t, p = isentropic(block, theta, psfc, grav)
@@ -46,7 +57,7 @@ def convection3d(block, seed=2024, amp=0.1, layers=10, theta=300.0, psfc=1.0e5, 
     return torch.from_numpy(w)
 
 
-def density_current(block, dT=-15.0, theta=300.0, psfc=1.0e5, grav=9.81, xc=0.0, xr=4000.0, zc=3000.0, zr=2000.0):
+def density_current(block, dT=-15.0, theta=300.0, psfc=1.0e5, grav=None, xc=0.0, xr=4000.0, zc=3000.0, zr=2000.0):
     """The dry density current (examples/straka.cpp): a cold bubble
     dT (1 + cos(pi L)) / 2 for L <= 1 applied to T at fixed p."""
     z, t, p = isentropic(block, theta, psfc, grav)

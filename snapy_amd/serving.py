@@ -91,7 +91,7 @@ def _options(kind: str, p: dict) -> MeshBlockOptions:
     elif kind == "density_current":
         nx, nz = int(p["nx"]), int(p["nz"])
         op.coord = CoordinateOptions("cartesian", 0.0, 6400.0, -25600.0, 25600.0, -50.0, 50.0, nz, nx, 1, 3)
-        op.hydro = HydroOptions(riemann="lmars", grav1=-GRAV, nu_iso=75.0, kappa_iso=75.0,
+        op.hydro = HydroOptions(riemann="lmars", grav1=-GRAV, nu_iso=75.0,
                                 precision=p.get("precision", "float64"))
         op.hydro.eos.gammad = GAMMA_DRY
         op.hydro.eos.weight = WEIGHT_DRY
