@@ -218,23 +218,35 @@ class MeshBlock:
         return self.members if self.members > 1 else self.options.coord.nc3()
 
     def part(self, offset=(0, 0, 0), exterior=True, extend_x1=0, extend_x2=0, extend_x3=0):
-        """The slices of the block's interior (exterior=False) or of the
-        whole block, as MeshBlockImpl::part for offset (0,0,0)."""
-        if tuple(offset) != (0, 0, 0):
-            raise NotImplementedError("part: only offset (0,0,0)")
+        """The index slices (all variables, x3, x2, x1) of a part of the
+        block, as MeshBlockImpl::part (src/mesh/meshblock.cpp:242-284)
+        through its binding (python/csrc/pymesh.cpp:193-215).  offset is
+        (o3, o2, o1); along an axis, 0 is the interior widened by extend_xd
+        cells on each side, and -1 / +1 the ghost layers beyond the interior
+        on that side (exterior=True) or the interior layers next to them
+        (exterior=False).  An axis without ghosts (one cell) has empty -1 /
+        +1 parts; so has a member x3 axis, whose offset-0 part is every
+        member."""
+        off3 = tuple(int(x) for x in offset)
+        if len(off3) != 3 or any(x not in (-1, 0, 1) for x in off3):
+            raise ValueError(f"part: offset {tuple(offset)} must be three of -1, 0, 1")
         c = self.options.coord
+        depth = 99  # PartOptions::depth's default; the binding does not set it
         out = [slice(None)]
-        for d, ext in ((3, extend_x3), (2, extend_x2), (1, extend_x1)):
+        for d, off, ext in ((3, off3[0], extend_x3), (2, off3[1], extend_x2), (1, off3[2], extend_x1)):
             if d == 3 and self.members > 1:
-                out.append(slice(0, self.members))  # every member, no ghosts
+                out.append(slice(0, self.members) if off == 0 else slice(0, 0))
                 continue
-            nx = (c.nx1, c.nx2, c.nx3)[d - 1]
             nc = c.nc(d)
-            if exterior:
-                out.append(slice(0, nc))
+            nx = nc - 2 * c.nghost if nc > 1 else 1
+            ng = 0 if nx == 1 else c.nghost
+            if off == -1:
+                start, n = (0 if exterior else ng), min(ng, depth)
+            elif off == 0:
+                start, n = ng - int(ext), nx + 2 * int(ext)
             else:
-                lo = c.nghost if nx > 1 else 0
-                out.append(slice(lo, lo + nx + ext))
+                start, n = nx + (ng if exterior else 0), min(ng, depth)
+            out.append(slice(start, start + n))
         return tuple(out)
 
     def _vic_workspace(self, lib) -> int:
