@@ -79,3 +79,27 @@ def test_preempted_requests_resume_bit_for_bit():
         print("request", r.rid, r.kind, flush=True)
         assert got[r.rid].finite
         assert np.array_equal(got[r.rid].state, ref[r.rid].state), r
+
+
+def test_density_currents_batched_equal_alone():
+    """Density-current requests of one grid and state form run as the
+    members of one block, joining and leaving at different rounds (three
+    slots for five requests, different step counts), and each ends with the
+    same bits as on its own block."""
+    reqs = []
+    for i, (prec, dT, steps) in enumerate(
+        [("float64", -15.0, 9), ("float64", -8.0, 5), ("float32", -12.0, 7), ("float64", -19.0, 6),
+         ("float32", -6.0, 8)]
+    ):
+        reqs.append(serving.Request(i, "density_current", {"nx": 64, "nz": 16, "dT": dT, "precision": prec},
+                                    100 + i, steps, 0.0))
+    print("member batches of", [(r.params["precision"], r.steps) for r in reqs], flush=True)
+    srv = serving.Server(realtime=False, max_slots=3, batch_members=True)
+    a = {r.rid: r for r in srv.run(reqs)}
+    built = [e for e in srv.log if e[0] == "batch"]
+    print("batches", built, flush=True)
+    assert len(built) >= 2
+    b = {r.rid: r for r in serving.Server(realtime=False, max_slots=3, batch_members=False).run(reqs)}
+    for r in reqs:
+        assert a[r.rid].finite
+        assert np.array_equal(a[r.rid].state, b[r.rid].state), r

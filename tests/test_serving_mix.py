@@ -34,3 +34,15 @@ def test_stream_is_reproducible():
     assert [(r.kind, r.params, r.seed, r.steps, r.arrival) for r in a] == [
         (r.kind, r.params, r.seed, r.steps, r.arrival) for r in b
     ]
+
+
+def test_member_key_groups_density_currents_of_one_grid_and_form():
+    from snapy_amd import serving
+
+    a = serving.Request(0, "density_current", {"nx": 256, "nz": 32, "dT": -5.0, "precision": "float32"}, 1, 10)
+    b = serving.Request(1, "density_current", {"nx": 256, "nz": 32, "dT": -9.0, "precision": "float32"}, 2, 20)
+    c = serving.Request(2, "density_current", {"nx": 256, "nz": 32, "dT": -9.0, "precision": "float64"}, 3, 20)
+    d = serving.Request(3, "sod", {"n": 256}, 4, 20)
+    assert serving.member_key(a) == serving.member_key(b)
+    assert serving.member_key(a) != serving.member_key(c)
+    assert serving.member_key(d) is None
