@@ -352,10 +352,13 @@ def fp32_accuracy(name, nsteps=200):
     b64, v64, _, _, _, _ = workload(name, "float64", execution="eager")
     b32, v32, _, _, _, _ = workload(name, "float32", execution="eager")
     m0 = float(b32.budget(v32)[0].item())
+    f0 = float(b64.budget(v64)[0].item())
     n, t = _advance_pair(b64, v64, b32, v32, nsteps=nsteps)
     m1 = float(b32.budget(v32)[0].item())
+    f1 = float(b64.budget(v64)[0].item())
     out = _rel_l2_rows(b64, v64, b32, v32)
     out["mass_drift"] = abs(m1 - m0) / abs(m0)
+    out["mass_drift_fp64_form"] = abs(f1 - f0) / abs(f0)
     out["steps"] = n
     out["time_s"] = t
     b64.release()
@@ -406,10 +409,14 @@ def density_current_accuracy(tlim=900.0):
         blocks[prec] = (b, v)
     (b64, v64), (b32, v32) = blocks["float64"], blocks["float32"]
     m0 = float(b32.budget(v32)[0].item())
+    f0 = float(b64.budget(v64)[0].item())
     n, t = _advance_pair(b64, v64, b32, v32, tlim=tlim)
     m1 = float(b32.budget(v32)[0].item())
+    f1 = float(b64.budget(v64)[0].item())
     out = _rel_l2_rows(b64, v64, b32, v32)
     out["mass_drift"] = abs(m1 - m0) / abs(m0)
+    # the FP64 form's own drift over the same steps, for scale
+    out["mass_drift_fp64_form"] = abs(f1 - f0) / abs(f0)
     out["steps"] = n
     out["time_s"] = t
     out["fp64_form"] = straka_diagnostics(b64, v64)
