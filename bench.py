@@ -630,7 +630,7 @@ def serving_leg(nreq, seed=2025):
     """A bounded serving stream: Poisson arrivals (20/s, seeded) of the
     Sod / density-current / shallow-water mix, continuous batching with
     admission by HBM footprint (200 GB budget; the shock tubes as one work
-    list, the density currents of one grid and form as block members);
+    list, the density currents of one deck, form and height as one group);
     p50/p99 request latency."""
     from snapy_amd import serving
 
@@ -642,9 +642,9 @@ def serving_leg(nreq, seed=2025):
     out = serving.summarize(res, wall)
     out["arrival_rate_per_s"] = 20.0
     out["seed"] = seed
-    # density-current requests of one grid and state form stepped as the
-    # members of one block (built when the group's membership changed)
-    out["member_batches_built"] = sum(1 for e in srv.log if e[0] == "batch")
+    # density-current requests of one deck, state form and height stepped as
+    # one group (built when a group's requests changed)
+    out["groups_built"] = sum(1 for e in srv.log if e[0] == "group")
     return out
 
 

@@ -455,6 +455,35 @@ int snapy_budget(const snapy_grid *g, int dtype, const void *u, double *ws,
 int snapy_budget_noalloc(const snapy_grid *g, int dtype, const void *u,
                          double *out, void *stream);
 
+/* ---- grouped launches: many blocks of different grids, one launch -------
+ *
+ * Replaces the reference's worker threads, one per block of a process, each
+ * launching its block's kernels on its own stream (src/mesh/mesh.cpp:
+ * 74-110): here the blocks that share a deck share each launch.  Between
+ * snapy_record_begin and snapy_record_end the calling thread's entry points
+ * make no launch: they record it (its kernel, arguments and grid) in a
+ * plan.  Plans of blocks that made the same launches in the same order (one
+ * deck on grids that differ only where the launches allow: the x2/x3 extents
+ * and spacings, with the same x1 cells and launch shapes) become a group,
+ * whose launch makes each recorded launch once over all the blocks'
+ * workgroups; every workgroup runs its own block's arguments, so the group
+ * step is bit-for-bit the blocks' own steps.  A plan and a group keep the
+ * buffer addresses and scalars they were recorded with (a stage's dt is read
+ * from each block's device dt).  Errors: -20 recording state, -21 plans that
+ * do not match, -22 no device memory. */
+int snapy_record_begin(void);
+int snapy_record_end(void **plan);
+/* the launches a plan holds */
+int snapy_plan_launches(const void *plan);
+void snapy_plan_free(void *plan);
+/* one group of nplans plans (its argument records in device memory) */
+int snapy_group_build(void *const *plans, int nplans, void **group);
+/* the group's launches, in the recorded order, on the stream */
+int snapy_group_launch(const void *group, void *stream);
+int snapy_group_launches(const void *group);
+/* frees the group (after the work launched from it has finished) */
+void snapy_group_free(void *group);
+
 /* ---- launch-shape tuning --------------------------------------------------
  *
  * The workgroup size of a kernel family for blocks of one shape (nx1, nx2,

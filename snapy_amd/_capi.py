@@ -184,6 +184,14 @@ SIGNATURES = {
     "snapy_budget_workspace": (ctypes.c_int64, [_GP]),
     "snapy_budget": (_I, [_GP, _I, _P, _P, _P, _P]),
     "snapy_budget_noalloc": (_I, [_GP, _I, _P, _P, _P]),
+    "snapy_record_begin": (_I, []),
+    "snapy_record_end": (_I, [ctypes.POINTER(ctypes.c_void_p)]),
+    "snapy_plan_launches": (_I, [_P]),
+    "snapy_plan_free": (None, [_P]),
+    "snapy_group_build": (_I, [ctypes.POINTER(ctypes.c_void_p), _I, ctypes.POINTER(ctypes.c_void_p)]),
+    "snapy_group_launch": (_I, [_P, _P]),
+    "snapy_group_launches": (_I, [_P]),
+    "snapy_group_free": (None, [_P]),
     "snapy_cs_geom_size": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32]),
     "snapy_cs_edge_size": (ctypes.c_int64, [_MP]),
     "snapy_cs_cons2prim": (_I, [_MP, _P, _P, _P]),

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
+#include "group.hpp"
 #include "point.hpp"
 
 namespace snapy {
@@ -37,112 +38,147 @@ __device__ __forceinline__ int64_t row_offset(BcArgs const& a, int row) {
 // U->W of the whole block into the outflow workspace (the fresh primitive
 // view apply_boundaries converts before its first radiating face)
 template <typename T>
+struct Cons2PrimBcBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             T const* __restrict__ u, T* __restrict__ w, int64_t n,
+                                             int64_t sv, double gd) {
+    int64_t c = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    T uu[5], ww[5];
+    for (int v = 0; v < 5; ++v) uu[v] = u[v * sv + c];
+    cons2prim5(uu, ww, gd);
+    for (int v = 0; v < 5; ++v) w[v * sv + c] = ww[v];
+  }
+};
+
+template <typename T>
 __global__ void __launch_bounds__(256)
     cons2prim_kernel_bc(T const* __restrict__ u, T* __restrict__ w, int64_t n,
                         int64_t sv, double gd) {
-  int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  T uu[5], ww[5];
-  for (int v = 0; v < 5; ++v) uu[v] = u[v * sv + c];
-  cons2prim5(uu, ww, gd);
-  for (int v = 0; v < 5; ++v) w[v * sv + c] = ww[v];
+  Cons2PrimBcBody<T>::run(block_index(), grid_size(), u, w, n, sv, gd);
 }
 
 // reflecting / periodic / extrapolation gathers; outflow is skipped here.
 template <typename T>
+struct BcCopyBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim, T* __restrict__ q,
+                                             BcArgs a) {
+    int64_t idx = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    int64_t tot = int64_t(2) * a.ng * a.nrow;
+    if (idx >= tot) return;
+    int row, g, side;
+    if (a.st == 1) {
+      // x1: the ghost layers of one column are adjacent in memory, so the
+      // ghost index runs fastest (a wavefront touches ~11 columns' lines, not
+      // 64)
+      int gs = int(idx % (2 * a.ng));
+      row = int(idx / (2 * a.ng));
+      side = gs / a.ng;
+      g = gs - side * a.ng;
+    } else {
+      row = int(idx % a.nrow);
+      int64_t rest = idx / a.nrow;
+      g = int(rest % a.ng);
+      side = int(rest / a.ng);
+    }
+    int bc = side ? a.bc_hi : a.bc_lo;
+    int dst, src;
+    bool flip = false;
+    if (bc == SNAPY_BC_REFLECTING) {
+      // narrow(0, ng) = narrow(ng, ng).flip; normal velocity *= -1
+      dst = side ? a.nc - a.ng + g : g;
+      src = side ? a.nc - a.ng - 1 - g : 2 * a.ng - 1 - g;
+      flip = true;
+    } else if (bc == SNAPY_BC_PERIODIC) {
+      dst = side ? a.nc - a.ng + g : g;
+      src = side ? a.ng + g : a.nc - 2 * a.ng + g;
+    } else if (bc == SNAPY_BC_EXTRAPOLATION) {
+      dst = side ? a.nc - a.ng + g : g;
+      src = side ? a.nc - a.ng - 1 : a.ng;
+    } else {
+      return;
+    }
+    int64_t off = row_offset(a, row);
+    int vn = a.vn;
+    for (int v = 0; v < a.nvar; ++v) {
+      T x = q[v * a.r.sv + off + src * a.st];
+      if (flip && v == vn) x = x * T(-1);
+      q[v * a.r.sv + off + dst * a.st] = x;
+    }
+  }
+};
+
+template <typename T>
 __global__ void __launch_bounds__(256)
     bc_copy_kernel(T* __restrict__ q, BcArgs a) {
-  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t tot = int64_t(2) * a.ng * a.nrow;
-  if (idx >= tot) return;
-  int row, g, side;
-  if (a.st == 1) {
-    // x1: the ghost layers of one column are adjacent in memory, so the
-    // ghost index runs fastest (a wavefront touches ~11 columns' lines, not
-    // 64)
-    int gs = int(idx % (2 * a.ng));
-    row = int(idx / (2 * a.ng));
-    side = gs / a.ng;
-    g = gs - side * a.ng;
-  } else {
-    row = int(idx % a.nrow);
-    int64_t rest = idx / a.nrow;
-    g = int(rest % a.ng);
-    side = int(rest / a.ng);
-  }
-  int bc = side ? a.bc_hi : a.bc_lo;
-  int dst, src;
-  bool flip = false;
-  if (bc == SNAPY_BC_REFLECTING) {
-    // narrow(0, ng) = narrow(ng, ng).flip; normal velocity *= -1
-    dst = side ? a.nc - a.ng + g : g;
-    src = side ? a.nc - a.ng - 1 - g : 2 * a.ng - 1 - g;
-    flip = true;
-  } else if (bc == SNAPY_BC_PERIODIC) {
-    dst = side ? a.nc - a.ng + g : g;
-    src = side ? a.ng + g : a.nc - 2 * a.ng + g;
-  } else if (bc == SNAPY_BC_EXTRAPOLATION) {
-    dst = side ? a.nc - a.ng + g : g;
-    src = side ? a.nc - a.ng - 1 : a.ng;
-  } else {
-    return;
-  }
-  int64_t off = row_offset(a, row);
-  int vn = a.vn;
-  for (int v = 0; v < a.nvar; ++v) {
-    T x = q[v * a.r.sv + off + src * a.st];
-    if (flip && v == vn) x = x * T(-1);
-    q[v * a.r.sv + off + dst * a.st] = x;
-  }
+  BcCopyBody<T>::run(block_index(), grid_size(), q, a);
 }
 
 // outflow (radiating about the saved background; bc_func.cpp:110-220) on a
 // primitive field, ideal gas, no tracers, Cartesian (boundary_velocity_ is
 // the identity there).  Writes the ghost cells of w for each outflow side.
 template <typename T>
+struct BcOutflowBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim, T* __restrict__ w,
+                                             T const* __restrict__ b, BcArgs a, double gd,
+                                             double dfloor, double pfloor) {
+    int64_t idx = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    int64_t tot = int64_t(2) * a.ng * a.nrow;
+    if (idx >= tot) return;
+    int row = int(idx % a.nrow);
+    int64_t rest = idx / a.nrow;
+    int g = int(rest % a.ng);
+    int side = int(rest / a.ng);
+    if ((side ? a.bc_hi : a.bc_lo) != SNAPY_BC_OUTFLOW) return;
+    int64_t off = row_offset(a, row);
+    int src = side ? a.nc - a.ng - 1 : a.ng;
+    int dst = (side ? a.nc - a.ng : 0) + g;
+    int vn = a.axis;
+    T wi[5], bi[5], bg[5], out[5];
+    for (int v = 0; v < 5; ++v) {
+      wi[v] = w[v * a.r.sv + off + src * a.st];
+      bi[v] = b[v * a.r.sv + off + src * a.st];
+      bg[v] = b[v * a.r.sv + off + dst * a.st];
+    }
+    radiating_ghost(wi, bi, bg, side, vn, gd, dfloor, pfloor, out);
+    for (int v = 0; v < 5; ++v) w[v * a.r.sv + off + dst * a.st] = out[v];
+  }
+};
+
+template <typename T>
 __global__ void __launch_bounds__(256)
     bc_outflow_kernel(T* __restrict__ w, T const* __restrict__ b, BcArgs a,
                       double gd, double dfloor, double pfloor) {
-  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t tot = int64_t(2) * a.ng * a.nrow;
-  if (idx >= tot) return;
-  int row = int(idx % a.nrow);
-  int64_t rest = idx / a.nrow;
-  int g = int(rest % a.ng);
-  int side = int(rest / a.ng);
-  if ((side ? a.bc_hi : a.bc_lo) != SNAPY_BC_OUTFLOW) return;
-  int64_t off = row_offset(a, row);
-  int src = side ? a.nc - a.ng - 1 : a.ng;
-  int dst = (side ? a.nc - a.ng : 0) + g;
-  int vn = a.axis;
-  T wi[5], bi[5], bg[5], out[5];
-  for (int v = 0; v < 5; ++v) {
-    wi[v] = w[v * a.r.sv + off + src * a.st];
-    bi[v] = b[v * a.r.sv + off + src * a.st];
-    bg[v] = b[v * a.r.sv + off + dst * a.st];
-  }
-  radiating_ghost(wi, bi, bg, side, vn, gd, dfloor, pfloor, out);
-  for (int v = 0; v < 5; ++v) w[v * a.r.sv + off + dst * a.st] = out[v];
+  BcOutflowBody<T>::run(block_index(), grid_size(), w, b, a, gd, dfloor, pfloor);
 }
 
 // W->U of the ghost layers of the outflow sides of one axis
 template <typename T>
+struct BcFlushBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             T const* __restrict__ w, T* __restrict__ u, BcArgs a,
+                                             T gm1) {
+    int64_t idx = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    int64_t tot = int64_t(2) * a.ng * a.nrow;
+    if (idx >= tot) return;
+    int row = int(idx % a.nrow);
+    int64_t rest = idx / a.nrow;
+    int g = int(rest % a.ng);
+    int side = int(rest / a.ng);
+    if ((side ? a.bc_hi : a.bc_lo) != SNAPY_BC_OUTFLOW) return;
+    int64_t o = row_offset(a, row) + int64_t((side ? a.nc - a.ng : 0) + g) * a.st;
+    T ww[5], uu[5];
+    for (int v = 0; v < 5; ++v) ww[v] = w[v * a.r.sv + o];
+    prim2cons5(ww, uu, gm1);
+    for (int v = 0; v < 5; ++v) u[v * a.r.sv + o] = uu[v];
+  }
+};
+
+template <typename T>
 __global__ void __launch_bounds__(256)
     bc_flush_kernel(T const* __restrict__ w, T* __restrict__ u, BcArgs a,
                     T gm1) {
-  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t tot = int64_t(2) * a.ng * a.nrow;
-  if (idx >= tot) return;
-  int row = int(idx % a.nrow);
-  int64_t rest = idx / a.nrow;
-  int g = int(rest % a.ng);
-  int side = int(rest / a.ng);
-  if ((side ? a.bc_hi : a.bc_lo) != SNAPY_BC_OUTFLOW) return;
-  int64_t o = row_offset(a, row) + int64_t((side ? a.nc - a.ng : 0) + g) * a.st;
-  T ww[5], uu[5];
-  for (int v = 0; v < 5; ++v) ww[v] = w[v * a.r.sv + o];
-  prim2cons5(ww, uu, gm1);
-  for (int v = 0; v < 5; ++v) u[v * a.r.sv + o] = uu[v];
+  BcFlushBody<T>::run(block_index(), grid_size(), w, u, a, gm1);
 }
 
 BcArgs make_args(snapy_grid const& g, snapy_physics const& p, int axis,
@@ -204,7 +240,7 @@ int boundaries(snapy_grid const& g, snapy_physics const& p, T* q,
       if (!pending_axes[ax - 1]) continue;
       BcArgs a = make_args(g, p, ax, 0);
       int64_t tot = int64_t(2) * a.ng * a.nrow;
-      hipLaunchKernelGGL(bc_flush_kernel<T>, dim3(blocks_for(tot, 256)),
+      launch<BcFlushBody<T>, 256, 1>(bc_flush_kernel<T>, dim3(blocks_for(tot, 256)),
                          dim3(256), 0, s, scratch, q, a, gm1);
       pending_axes[ax - 1] = 0;
     }
@@ -223,25 +259,25 @@ int boundaries(snapy_grid const& g, snapy_physics const& p, T* q,
       T* wv = q;
       if (!prim) {
         if (!pending) {
-          hipLaunchKernelGGL(cons2prim_kernel_bc<T>, dim3(blocks_for(r.sv, 256)),
+          launch<Cons2PrimBcBody<T>, 256, 1>(cons2prim_kernel_bc<T>, dim3(blocks_for(r.sv, 256)),
                              dim3(256), 0, s, q, scratch, r.sv, r.sv, p.gammad);
         }
         wv = scratch;
         pending = true;
         pending_axes[ax - 1] = 1;
       }
-      hipLaunchKernelGGL(bc_outflow_kernel<T>, dim3(blocks_for(tot, 256)),
+      launch<BcOutflowBody<T>, 256, 1>(bc_outflow_kernel<T>, dim3(blocks_for(tot, 256)),
                          dim3(256), 0, s, wv, bref, a, p.gammad,
                          p.density_floor, p.pressure_floor);
       if (!(out_lo && out_hi)) {
         // the other face of this axis is a gather on q: flush first
         flush();
-        hipLaunchKernelGGL(bc_copy_kernel<T>, dim3(blocks_for(tot, 256)),
+        launch<BcCopyBody<T>, 256, 1>(bc_copy_kernel<T>, dim3(blocks_for(tot, 256)),
                            dim3(256), 0, s, q, a);
       }
     } else {
       flush();
-      hipLaunchKernelGGL(bc_copy_kernel<T>, dim3(blocks_for(tot, 256)),
+      launch<BcCopyBody<T>, 256, 1>(bc_copy_kernel<T>, dim3(blocks_for(tot, 256)),
                          dim3(256), 0, s, q, a);
     }
   }
@@ -270,7 +306,7 @@ int snapy_scalar_boundaries(const snapy_grid* g, const snapy_physics* p, int ns,
     a.nvar = ns;
     a.vn = -1;
     int64_t tot = int64_t(2) * a.ng * a.nrow;
-    hipLaunchKernelGGL(bc_copy_kernel<double>, dim3(blocks_for(tot, 256)), dim3(256), 0, s, q, a);
+    launch<BcCopyBody<double>, 256, 1>(bc_copy_kernel<double>, dim3(blocks_for(tot, 256)), dim3(256), 0, s, q, a);
   }
   return launch_status("scalar_boundaries");
 }

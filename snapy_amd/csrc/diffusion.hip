@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
+#include "group.hpp"
 #include "point.hpp"
 
 namespace snapy {
@@ -31,75 +32,85 @@ struct DiffArgs {
 };
 
 template <typename TT>
+struct DiffusionFaceBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w,
+                                             double* __restrict__ work, TT* __restrict__ total,
+                                             DiffArgs a) {
+    Range3 const& r = a.r;
+    int n[3] = {r.n1, r.n2, r.n3};
+    int lo[3] = {r.lo1, r.lo2, r.lo3};
+    n[a.ax] = a.nf;
+    int64_t idx = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    int64_t tot = int64_t(n[0]) * n[1] * n[2];
+    if (idx >= tot) return;
+    int c0 = int(idx % n[0]);
+    int64_t rest = idx / n[0];
+    int c1 = int(rest % n[1]);
+    int c2 = int(rest / n[1]);
+    int ci[3] = {lo[0] + c0, lo[1] + c1, lo[2] + c2};
+    int f = ci[a.ax] - lo[a.ax];  // face index along the direction
+    int64_t c = ci[2] * r.s3 + ci[1] * r.s2 + ci[0];  // the cell above the face
+    int64_t sd = a.s[a.ax];
+    int64_t cm = c - sd;  // the cell below
+    auto W = [&](int v, int64_t q) { return w[v * r.sv + q]; };
+    // centered_derivative (diffusion.cpp:73-95)
+    auto cd = [&](int v, int64_t q, int ax) {
+      return (W(v, q + a.s[ax]) - W(v, q - a.s[ax])) / (2. * a.dx[ax]);
+    };
+    auto divv = [&](int64_t q) {
+      double d = 0.;
+      for (int ax = 0; ax < 3; ++ax)
+        if (a.act[ax]) d = d + cd(IVX + ax, q, ax);
+      return d;
+    };
+    // face_coefficient (diffusion.cpp:164-185): the two-cell average; a
+    // reflecting x1 wall face extrapolates from the two nearest active cells
+    bool xwall = a.ax == 0 && a.nf >= 3 &&
+                 ((f == 0 && a.xwall_lo) || (f == a.nf - 1 && a.xwall_hi));
+    auto coef = [&](auto val) {
+      if (!xwall) return 0.5 * (val(c) + val(cm));
+      double t = 0.5;
+      double near = f == 0 ? val(c) : val(cm);
+      double next = f == 0 ? val(c + 1) : val(cm - 1);
+      double wall = (1. + t) * near - t * next;
+      return wall > 0. ? wall : near;
+    };
+    double const dxd = a.dx[a.ax];
+    double out[4] = {0., 0., 0., 0.};  // momentum x1, x2, x3, energy
+    double rho_face = coef([&](int64_t q) { return W(IDN, q); });
+    if (a.nu > 0.) {
+      double div_face = 0.5 * (divv(c) + divv(cm));
+      for (int iv = 0; iv < 3; ++iv) {
+        double stress = (W(IVX + iv, c) - W(IVX + iv, cm)) / dxd;
+        if (iv == a.ax) {
+          stress = 2. * stress - (2. / 3.) * div_face;
+        } else if (a.act[iv]) {
+          stress = stress + 0.5 * (cd(IVX + a.ax, c, iv) + cd(IVX + a.ax, cm, iv));
+        }
+        double mflux = -a.nu * rho_face * stress;
+        out[iv] = mflux;
+        out[3] = out[3] + 0.5 * (W(IVX + iv, c) + W(IVX + iv, cm)) * mflux;
+      }
+    }
+    if (a.kappa > 0.) {
+      auto temp = [&](int64_t q) { return W(IPR, q) / (W(IDN, q) * a.rd); };
+      double dtdn = (temp(c) - temp(cm)) / dxd;
+      double rcv = coef([&](int64_t q) { return W(IDN, q) * a.cv; });
+      out[3] = out[3] - a.kappa * rcv * dtdn;
+    }
+    int64_t ncell = int64_t(r.nc1) * r.nc2 * r.nc3;
+    for (int q = 0; q < 4; ++q) work[(a.ax * 4 + q) * ncell + c] = out[q];
+    bool wall = (f == 0 && a.pres_lo) || (f == a.nf - 1 && a.pres_hi);
+    if (!wall) total[IPR * r.sv + c] = TT(double(total[IPR * r.sv + c]) + out[3]);
+  }
+};
+
+template <typename TT>
 __global__ void __launch_bounds__(256)
     diffusion_face_kernel(double const* __restrict__ w, double* __restrict__ work,
                           TT* __restrict__ total, DiffArgs a) {
-  Range3 const& r = a.r;
-  int n[3] = {r.n1, r.n2, r.n3};
-  int lo[3] = {r.lo1, r.lo2, r.lo3};
-  n[a.ax] = a.nf;
-  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t tot = int64_t(n[0]) * n[1] * n[2];
-  if (idx >= tot) return;
-  int c0 = int(idx % n[0]);
-  int64_t rest = idx / n[0];
-  int c1 = int(rest % n[1]);
-  int c2 = int(rest / n[1]);
-  int ci[3] = {lo[0] + c0, lo[1] + c1, lo[2] + c2};
-  int f = ci[a.ax] - lo[a.ax];  // face index along the direction
-  int64_t c = ci[2] * r.s3 + ci[1] * r.s2 + ci[0];  // the cell above the face
-  int64_t sd = a.s[a.ax];
-  int64_t cm = c - sd;  // the cell below
-  auto W = [&](int v, int64_t q) { return w[v * r.sv + q]; };
-  // centered_derivative (diffusion.cpp:73-95)
-  auto cd = [&](int v, int64_t q, int ax) {
-    return (W(v, q + a.s[ax]) - W(v, q - a.s[ax])) / (2. * a.dx[ax]);
-  };
-  auto divv = [&](int64_t q) {
-    double d = 0.;
-    for (int ax = 0; ax < 3; ++ax)
-      if (a.act[ax]) d = d + cd(IVX + ax, q, ax);
-    return d;
-  };
-  // face_coefficient (diffusion.cpp:164-185): the two-cell average; a
-  // reflecting x1 wall face extrapolates from the two nearest active cells
-  bool xwall = a.ax == 0 && a.nf >= 3 &&
-               ((f == 0 && a.xwall_lo) || (f == a.nf - 1 && a.xwall_hi));
-  auto coef = [&](auto val) {
-    if (!xwall) return 0.5 * (val(c) + val(cm));
-    double t = 0.5;
-    double near = f == 0 ? val(c) : val(cm);
-    double next = f == 0 ? val(c + 1) : val(cm - 1);
-    double wall = (1. + t) * near - t * next;
-    return wall > 0. ? wall : near;
-  };
-  double const dxd = a.dx[a.ax];
-  double out[4] = {0., 0., 0., 0.};  // momentum x1, x2, x3, energy
-  double rho_face = coef([&](int64_t q) { return W(IDN, q); });
-  if (a.nu > 0.) {
-    double div_face = 0.5 * (divv(c) + divv(cm));
-    for (int iv = 0; iv < 3; ++iv) {
-      double stress = (W(IVX + iv, c) - W(IVX + iv, cm)) / dxd;
-      if (iv == a.ax) {
-        stress = 2. * stress - (2. / 3.) * div_face;
-      } else if (a.act[iv]) {
-        stress = stress + 0.5 * (cd(IVX + a.ax, c, iv) + cd(IVX + a.ax, cm, iv));
-      }
-      double mflux = -a.nu * rho_face * stress;
-      out[iv] = mflux;
-      out[3] = out[3] + 0.5 * (W(IVX + iv, c) + W(IVX + iv, cm)) * mflux;
-    }
-  }
-  if (a.kappa > 0.) {
-    auto temp = [&](int64_t q) { return W(IPR, q) / (W(IDN, q) * a.rd); };
-    double dtdn = (temp(c) - temp(cm)) / dxd;
-    double rcv = coef([&](int64_t q) { return W(IDN, q) * a.cv; });
-    out[3] = out[3] - a.kappa * rcv * dtdn;
-  }
-  int64_t ncell = int64_t(r.nc1) * r.nc2 * r.nc3;
-  for (int q = 0; q < 4; ++q) work[(a.ax * 4 + q) * ncell + c] = out[q];
-  bool wall = (f == 0 && a.pres_lo) || (f == a.nf - 1 && a.pres_hi);
-  if (!wall) total[IPR * r.sv + c] = TT(double(total[IPR * r.sv + c]) + out[3]);
+  DiffusionFaceBody<TT>::run(block_index(), grid_size(), w, work, total, a);
 }
 
 struct DiffDivArgs {
@@ -109,38 +120,46 @@ struct DiffDivArgs {
   double const* dt_dev;
 };
 
+struct DiffusionDuBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ work,
+                                             double* __restrict__ dsrc, DiffDivArgs a) {
+    Range3 const& r = a.r;
+    int64_t idx = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
+    if (idx >= tot) return;
+    int i = r.lo1 + int(idx % r.n1);
+    int64_t rest = idx / r.n1;
+    int j = r.lo2 + int(rest % r.n2);
+    int k = r.lo3 + int(rest / r.n2);
+    int64_t c = k * r.s3 + j * r.s2 + i;
+    int64_t ncell = int64_t(r.nc1) * r.nc2 * r.nc3;
+    double dt = dt_at(a.dt_dev, -a.mdt, r, k);
+    double A[3] = {a.a1, a.a2, a.a3};
+    int64_t s[3] = {1, r.s2, r.s3};
+    dsrc[IDN * r.sv + c] = 0.;
+    dsrc[IPR * r.sv + c] = 0.;
+    for (int q = 0; q < 3; ++q) {
+      // CoordinateImpl::divergence (coordinate.cpp:618-662)
+      double div = 0.;
+      bool first = true;
+      for (int ax = 0; ax < 3; ++ax) {
+        if (!a.act[ax]) continue;
+        double const* F = work + (ax * 4 + q) * ncell;
+        double d = A[ax] * F[c + s[ax]] - A[ax] * F[c];
+        div = first ? d : div + d;
+        first = false;
+      }
+      div = div / a.vol;
+      dsrc[(IVX + q) * r.sv + c] = -(dt * div);
+    }
+  }
+};
+
 __global__ void __launch_bounds__(256)
     diffusion_du_kernel(double const* __restrict__ work, double* __restrict__ dsrc,
                         DiffDivArgs a) {
-  Range3 const& r = a.r;
-  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
-  if (idx >= tot) return;
-  int i = r.lo1 + int(idx % r.n1);
-  int64_t rest = idx / r.n1;
-  int j = r.lo2 + int(rest % r.n2);
-  int k = r.lo3 + int(rest / r.n2);
-  int64_t c = k * r.s3 + j * r.s2 + i;
-  int64_t ncell = int64_t(r.nc1) * r.nc2 * r.nc3;
-  double dt = dt_at(a.dt_dev, -a.mdt, r, k);
-  double A[3] = {a.a1, a.a2, a.a3};
-  int64_t s[3] = {1, r.s2, r.s3};
-  dsrc[IDN * r.sv + c] = 0.;
-  dsrc[IPR * r.sv + c] = 0.;
-  for (int q = 0; q < 3; ++q) {
-    // CoordinateImpl::divergence (coordinate.cpp:618-662)
-    double div = 0.;
-    bool first = true;
-    for (int ax = 0; ax < 3; ++ax) {
-      if (!a.act[ax]) continue;
-      double const* F = work + (ax * 4 + q) * ncell;
-      double d = A[ax] * F[c + s[ax]] - A[ax] * F[c];
-      div = first ? d : div + d;
-      first = false;
-    }
-    div = div / a.vol;
-    dsrc[(IVX + q) * r.sv + c] = -(dt * div);
-  }
+  DiffusionDuBody::run(block_index(), grid_size(), work, dsrc, a);
 }
 
 }  // namespace
@@ -200,11 +219,11 @@ int snapy_diffusion(const snapy_grid* g, const snapy_physics* p, int dtype,
     int64_t nfaces = int64_t(a.nf) * (ax == 0 ? 1 : nx[0]) *
                      (ax == 1 ? 1 : nx[1]) * (ax == 2 ? 1 : nx[2]);
     if (dtype == SNAPY_F64)
-      hipLaunchKernelGGL(diffusion_face_kernel<double>,
+      launch<DiffusionFaceBody<double>, 256, 1>(diffusion_face_kernel<double>,
                          dim3(blocks_for(nfaces, 256)), dim3(256), 0, s, w,
                          work, static_cast<double*>(totals[ax]), a);
     else
-      hipLaunchKernelGGL(diffusion_face_kernel<float>,
+      launch<DiffusionFaceBody<float>, 256, 1>(diffusion_face_kernel<float>,
                          dim3(blocks_for(nfaces, 256)), dim3(256), 0, s, w,
                          work, static_cast<float*>(totals[ax]), a);
     if (int e = launch_status("diffusion_faces")) return e;
@@ -219,7 +238,7 @@ int snapy_diffusion(const snapy_grid* g, const snapy_physics* p, int dtype,
   d.mdt = -dt;
   d.dt_dev = dt_dev;
   int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
-  hipLaunchKernelGGL(diffusion_du_kernel, dim3(blocks_for(tot, 256)),
+  launch<DiffusionDuBody, 256, 1>(diffusion_du_kernel, dim3(blocks_for(tot, 256)),
                      dim3(256), 0, s, work, du_src, d);
   return launch_status("diffusion_du");
 }

@@ -12,6 +12,7 @@
 #include <cfloat>
 
 #include "common.hpp"
+#include "group.hpp"
 #include "point.hpp"
 
 namespace snapy {
@@ -346,125 +347,135 @@ __global__ void __launch_bounds__(256)
 // LDS.  The arithmetic is that of the two kernels, so the fluxes are the
 // same bits.
 template <typename T, typename TF>
+struct SweepX1WbRefBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w, double* __restrict__ ref,
+                                             TF* __restrict__ flx, double* __restrict__ fp,
+                                             RefArgs ra, WbArgs a, int nc) {
+    extern __shared__ double smem[];
+    __shared__ int64_t s_row[kRefCols];
+    Range3 const& r = a.r;
+    int64_t ncol = int64_t(r.n2) * r.n3;
+    int64_t col0 = int64_t(bid.x) * nc;
+    int const tid = threadIdx.x;
+    int nc1 = r.nc1, iu = ra.iu;
+    int const stride = 4 * nc1 + 1;  // doubles per column
+    int ncl = ncol - col0 < nc ? int(ncol - col0) : nc;
+    double const g = ra.grav, dx = ra.dx1, mp = DBL_MIN;
+    auto floor_mp = [&](double v) { return v > mp ? v : mp; };
+    if (tid < ncl) {
+      int64_t col = col0 + tid;
+      int j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
+      s_row[tid] = k * r.s3 + j * r.s2;
+    }
+    __syncthreads();
+    int const cells = ncl * nc1;
+    for (int e = tid; e < cells; e += kRefThreads) {
+      int c = e / nc1, i = e - c * nc1;
+      int64_t base = s_row[c];
+      double rho = w[base + i], pr = w[4 * r.sv + base + i];
+      double* sc = smem + c * stride;
+      sc[i] = g * rho * dx;
+      sc[nc1 + i] = rho / pr;
+    }
+    __syncthreads();
+    int il = nc1 - 1 - iu;
+    bool wc = ra.wall_clamp;
+    bool thin = (iu - il + 1) < 5, fits = iu >= 4;
+    int jlo = wc ? il : 0, jhi = wc ? iu : nc1 - 1;
+    bool ext_lo = wc && il + 1 <= iu, ext_hi = wc && iu - 1 >= il;
+    if (tid < ncl) {
+      double* sc = smem + tid * stride;
+      double const* s_dp = sc;
+      double* s_F = sc + 2 * nc1;
+      int64_t base = s_row[tid];
+      double rho_u = w[base + iu], pr_u = w[4 * r.sv + base + iu];
+      double rdt_top = pr_u / rho_u;
+      double top = pr_u * exp(-g * 0.5 * dx / rdt_top);
+      s_F[iu + 1] = top;
+      double face = top;
+      constexpr int kB = 8;
+      int i = iu;
+      for (; i - kB + 1 >= 0; i -= kB) {
+        double t[kB];
+#pragma unroll
+        for (int m = 0; m < kB; ++m) t[m] = s_dp[i - m];
+#pragma unroll
+        for (int m = 0; m < kB; ++m) {
+          face = face + t[m];
+          s_F[i - m] = face;
+        }
+      }
+      for (; i >= 0; --i) {
+        face = face + s_dp[i];
+        s_F[i] = face;
+      }
+      face = top;
+      for (int k = iu + 1; k < nc1; ++k) {
+        face = face - s_dp[k];
+        s_F[k + 1] = face;
+      }
+    }
+    for (int e = tid; e < cells; e += kRefThreads) {
+      int c = e / nc1, i = e - c * nc1;
+      double* sc = smem + c * stride;
+      sc[3 * nc1 + 1 + i] = rop_smooth(sc + nc1, i, jlo, jhi, ext_lo, ext_hi);
+    }
+    __syncthreads();
+    // the cell pass: pref into the pressure-step slot, dref into the rho / p
+    // slot (neither is read again)
+    for (int e = tid; e < cells; e += kRefThreads) {
+      int c = e / nc1, i = e - c * nc1;
+      double* sc = smem + c * stride;
+      double const* s_F = sc + 2 * nc1;
+      double const* s_rs = sc + 3 * nc1 + 1;
+      bool wall_in = wc && i >= il && i < il + 2;
+      bool wall_out = wc && i > iu - 2 && i <= iu;
+      bool clamp_in = wall_in && fits && !thin;
+      bool clamp_out = wall_out && fits && !thin;
+      double lo = floor_mp(s_F[i]), hi = floor_mp(s_F[i + 1]);
+      double cell = 0.5 * (lo + hi);
+      double lower = lo < hi ? lo : hi, upper = lo > hi ? lo : hi;
+      auto six = [&](int start, double const* wt, bool rev) {
+        double value = 0.;
+        for (int m = 0; m < 6; ++m)
+          value += wt[rev ? 5 - m : m] * floor_mp(s_F[start + m]);
+        if (value >= lower && value <= upper) cell = value;
+      };
+      if (i >= 2 && i < nc1 - 2 && !wall_in && !wall_out) six(i - 2, kW6, false);
+      if (clamp_in) six(il, kW6E[i - il], false);
+      if (clamp_out) {
+        int s = iu + 1 - 5;
+        six(s, kW6E[4 - (i - s)], true);
+      }
+      sc[i] = cell;
+      sc[nc1 + i] = cell * s_rs[i];
+      ref[R_PSF_LO * r.sv + s_row[c] + i] = lo;
+    }
+    __syncthreads();
+    int const nf = a.nf;
+    int const faces = ncl * nf;
+    for (int e = tid; e < faces; e += kRefThreads) {
+      int c = e / nf, fi = e - c * nf;
+      int i = r.lo1 + fi;
+      double const* sc = smem + c * stride;
+      double const* s_F = sc + 2 * nc1;
+      double const* s_rs = sc + 3 * nc1 + 1;
+      double lo = floor_mp(s_F[i]);
+      double rf = i > 0 ? 0.5 * (s_rs[i - 1] + s_rs[i]) : s_rs[i];
+      wb_face<T, TF>(
+          w, s_row[c], fi, lo, lo * rf, [&](int m) { return sc[m]; },
+          [&](int m) { return sc[nc1 + m]; }, flx, fp, a);
+    }
+  }
+};
+
+template <typename T, typename TF>
 __global__ void __launch_bounds__(kRefThreads)
     sweep_x1_wb_ref_kernel(double const* __restrict__ w, double* __restrict__ ref,
                            TF* __restrict__ flx, double* __restrict__ fp, RefArgs ra,
                            WbArgs a, int nc) {
-  extern __shared__ double smem[];
-  __shared__ int64_t s_row[kRefCols];
-  Range3 const& r = a.r;
-  int64_t ncol = int64_t(r.n2) * r.n3;
-  int64_t col0 = int64_t(blockIdx.x) * nc;
-  int const tid = threadIdx.x;
-  int nc1 = r.nc1, iu = ra.iu;
-  int const stride = 4 * nc1 + 1;  // doubles per column
-  int ncl = ncol - col0 < nc ? int(ncol - col0) : nc;
-  double const g = ra.grav, dx = ra.dx1, mp = DBL_MIN;
-  auto floor_mp = [&](double v) { return v > mp ? v : mp; };
-  if (tid < ncl) {
-    int64_t col = col0 + tid;
-    int j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
-    s_row[tid] = k * r.s3 + j * r.s2;
-  }
-  __syncthreads();
-  int const cells = ncl * nc1;
-  for (int e = tid; e < cells; e += kRefThreads) {
-    int c = e / nc1, i = e - c * nc1;
-    int64_t base = s_row[c];
-    double rho = w[base + i], pr = w[4 * r.sv + base + i];
-    double* sc = smem + c * stride;
-    sc[i] = g * rho * dx;
-    sc[nc1 + i] = rho / pr;
-  }
-  __syncthreads();
-  int il = nc1 - 1 - iu;
-  bool wc = ra.wall_clamp;
-  bool thin = (iu - il + 1) < 5, fits = iu >= 4;
-  int jlo = wc ? il : 0, jhi = wc ? iu : nc1 - 1;
-  bool ext_lo = wc && il + 1 <= iu, ext_hi = wc && iu - 1 >= il;
-  if (tid < ncl) {
-    double* sc = smem + tid * stride;
-    double const* s_dp = sc;
-    double* s_F = sc + 2 * nc1;
-    int64_t base = s_row[tid];
-    double rho_u = w[base + iu], pr_u = w[4 * r.sv + base + iu];
-    double rdt_top = pr_u / rho_u;
-    double top = pr_u * exp(-g * 0.5 * dx / rdt_top);
-    s_F[iu + 1] = top;
-    double face = top;
-    constexpr int kB = 8;
-    int i = iu;
-    for (; i - kB + 1 >= 0; i -= kB) {
-      double t[kB];
-#pragma unroll
-      for (int m = 0; m < kB; ++m) t[m] = s_dp[i - m];
-#pragma unroll
-      for (int m = 0; m < kB; ++m) {
-        face = face + t[m];
-        s_F[i - m] = face;
-      }
-    }
-    for (; i >= 0; --i) {
-      face = face + s_dp[i];
-      s_F[i] = face;
-    }
-    face = top;
-    for (int k = iu + 1; k < nc1; ++k) {
-      face = face - s_dp[k];
-      s_F[k + 1] = face;
-    }
-  }
-  for (int e = tid; e < cells; e += kRefThreads) {
-    int c = e / nc1, i = e - c * nc1;
-    double* sc = smem + c * stride;
-    sc[3 * nc1 + 1 + i] = rop_smooth(sc + nc1, i, jlo, jhi, ext_lo, ext_hi);
-  }
-  __syncthreads();
-  // the cell pass: pref into the pressure-step slot, dref into the rho / p
-  // slot (neither is read again)
-  for (int e = tid; e < cells; e += kRefThreads) {
-    int c = e / nc1, i = e - c * nc1;
-    double* sc = smem + c * stride;
-    double const* s_F = sc + 2 * nc1;
-    double const* s_rs = sc + 3 * nc1 + 1;
-    bool wall_in = wc && i >= il && i < il + 2;
-    bool wall_out = wc && i > iu - 2 && i <= iu;
-    bool clamp_in = wall_in && fits && !thin;
-    bool clamp_out = wall_out && fits && !thin;
-    double lo = floor_mp(s_F[i]), hi = floor_mp(s_F[i + 1]);
-    double cell = 0.5 * (lo + hi);
-    double lower = lo < hi ? lo : hi, upper = lo > hi ? lo : hi;
-    auto six = [&](int start, double const* wt, bool rev) {
-      double value = 0.;
-      for (int m = 0; m < 6; ++m)
-        value += wt[rev ? 5 - m : m] * floor_mp(s_F[start + m]);
-      if (value >= lower && value <= upper) cell = value;
-    };
-    if (i >= 2 && i < nc1 - 2 && !wall_in && !wall_out) six(i - 2, kW6, false);
-    if (clamp_in) six(il, kW6E[i - il], false);
-    if (clamp_out) {
-      int s = iu + 1 - 5;
-      six(s, kW6E[4 - (i - s)], true);
-    }
-    sc[i] = cell;
-    sc[nc1 + i] = cell * s_rs[i];
-    ref[R_PSF_LO * r.sv + s_row[c] + i] = lo;
-  }
-  __syncthreads();
-  int const nf = a.nf;
-  int const faces = ncl * nf;
-  for (int e = tid; e < faces; e += kRefThreads) {
-    int c = e / nf, fi = e - c * nf;
-    int i = r.lo1 + fi;
-    double const* sc = smem + c * stride;
-    double const* s_F = sc + 2 * nc1;
-    double const* s_rs = sc + 3 * nc1 + 1;
-    double lo = floor_mp(s_F[i]);
-    double rf = i > 0 ? 0.5 * (s_rs[i - 1] + s_rs[i]) : s_rs[i];
-    wb_face<T, TF>(
-        w, s_row[c], fi, lo, lo * rf, [&](int m) { return sc[m]; },
-        [&](int m) { return sc[nc1 + m]; }, flx, fp, a);
-  }
+  SweepX1WbRefBody<T, TF>::run(block_index(), grid_size(), w, ref, flx, fp, ra, a, nc);
 }
 
 // ---------------------------------------------------------------------------
@@ -491,6 +502,96 @@ struct GUpdArgs {
 // be NULL) is the reference face pressure that an FP32 x1 momentum flux
 // carries as an anomaly.
 template <typename TF, bool WRITE_DU, bool PAIR>
+struct UpdateGravityBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             TF const* __restrict__ f1, TF const* __restrict__ f2,
+                                             TF const* __restrict__ f3,
+                                             void const* __restrict__ u0v,
+                                             float const* __restrict__ c0, void* __restrict__ uv,
+                                             float* __restrict__ cc, double const* __restrict__ w,
+                                             double const* __restrict__ tab,
+                                             double const* __restrict__ dsrc,
+                                             double const* __restrict__ psf, GUpdArgs a) {
+    using T = double;
+    Range3 const& r = a.r;
+    int64_t idx = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
+    if (idx >= tot) return;
+    int ii = int(idx % r.n1);
+    int i = r.lo1 + ii;
+    int64_t rest = idx / r.n1;
+    int j = r.lo2 + int(rest % r.n2);
+    int k = r.lo3 + int(rest / r.n2);
+    int64_t row = k * r.s3 + j * r.s2;
+    int64_t c = row + i;
+    T A1 = T(a.a1), A2 = T(a.a2), A3 = T(a.a3), V = T(a.vol);
+    double dt = dt_at(a.dt_dev, -a.mdt, r, k);
+    T mdt = T(-dt), tdt = T(dt);
+    T w0 = T(a.w0), w1 = T(a.w1), w2 = T(a.w2), g1 = T(a.grav1);
+    T rho = w[c], vx = w[r.sv + c];
+    // x1 face flux of row v at face q (the reference face pressure restored
+    // on an FP32 momentum anomaly)
+    auto F1 = [&](int v, int64_t q) {
+      T x = T(f1[v * r.sv + q]);
+      if (psf && v == IVX) x = x + psf[q];
+      return x;
+    };
+    T du[5];
+    for (int v = 0; v < 5; ++v) {
+      int64_t o = v * r.sv + c;
+      T div = A1 * F1(v, c + 1) - A1 * F1(v, c);
+      if (f2) div = div + (A2 * T(f2[o + r.s2]) - A2 * T(f2[o]));
+      if (f3) div = div + (A3 * T(f3[o + r.s3]) - A3 * T(f3[o]));
+      div = div / V;
+      du[v] = mdt * div;
+    }
+    // ConstGravityImpl::forward_split (const_gravity.cpp:35-43), nh = 1
+    du[IVX] = du[IVX] + tdt * rho * g1 * T(1.);
+    du[IPR] = du[IPR] + tdt * rho * vx * g1 * T(1.);
+    if (dsrc)
+      for (int v = 0; v < 5; ++v) du[v] = du[v] + dsrc[v * r.sv + c];
+    // LayerHeat bot-heat, an integer depth of bottom cells
+    // (src/forcing/layer_heat.cpp:78-85): du[IPR] += dt * flux / (dz * depth)
+    if (ii < a.heat_depth) du[IPR] = du[IPR] + T(dt * a.heat_flux / a.heat_dzd);
+    // gravity_energy_correction (hydro_forward.cpp:1049-1124), Cartesian
+    TF const* m1 = f1 + row;  // the IDN row of F1 along this column
+    auto drho = [&](int m) {  // -dt * x1_face_difference / V at cell lo1 + m
+      int q = r.lo1 + m;
+      return mdt * ((A1 * T(m1[q + 1]) - A1 * T(m1[q])) / V);
+    };
+    T gw = (T(tab[T_GW_HI * r.nc1 + i]) * (A1 * T(m1[i + 1])) -
+            T(tab[T_GW_LO * r.nc1 + i]) * (A1 * T(m1[i]))) /
+           V;
+    T fgw = tdt * gw;
+    int jj = (ii < 1 ? 1 : (ii > r.n1 - 2 ? r.n1 - 2 : ii)) - 1;
+    T s = T(tab[T_WT0 * r.nc1 + i]) * drho(jj);
+    s = s + T(tab[T_WT1 * r.nc1 + i]) * drho(jj + 1);
+    s = s + T(tab[T_WT2 * r.nc1 + i]) * drho(jj + 2);
+    fgw = fgw + T(tab[T_GVAR * r.nc1 + i]) * s;
+    T ogw = tdt * rho * vx * g1 * T(1.);
+    du[IPR] = du[IPR] + (fgw - ogw);
+    for (int v = 0; v < 5; ++v) {
+      int64_t o = v * r.sv + c;
+      if (WRITE_DU) {
+        static_cast<double*>(uv)[o] = du[v];
+      } else if (PAIR) {
+        float* uf = static_cast<float*>(uv);
+        float const* u0f = static_cast<float const*>(u0v);
+        double x = w0 * (double(u0f[o]) + double(c0[o])) +
+                   w1 * (double(uf[o]) + double(cc[o])) + w2 * du[v];
+        float h = float(x);
+        uf[o] = h;
+        cc[o] = float(x - double(h));
+      } else {
+        double* ud = static_cast<double*>(uv);
+        double const* u0d = static_cast<double const*>(u0v);
+        ud[o] = w0 * u0d[o] + w1 * ud[o] + w2 * du[v];
+      }
+    }
+  }
+};
+
+template <typename TF, bool WRITE_DU, bool PAIR>
 __global__ void __launch_bounds__(256)
     update_gravity_kernel(TF const* __restrict__ f1, TF const* __restrict__ f2,
                           TF const* __restrict__ f3, void const* __restrict__ u0v,
@@ -499,82 +600,8 @@ __global__ void __launch_bounds__(256)
                           double const* __restrict__ tab,
                           double const* __restrict__ dsrc,
                           double const* __restrict__ psf, GUpdArgs a) {
-  using T = double;
-  Range3 const& r = a.r;
-  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
-  if (idx >= tot) return;
-  int ii = int(idx % r.n1);
-  int i = r.lo1 + ii;
-  int64_t rest = idx / r.n1;
-  int j = r.lo2 + int(rest % r.n2);
-  int k = r.lo3 + int(rest / r.n2);
-  int64_t row = k * r.s3 + j * r.s2;
-  int64_t c = row + i;
-  T A1 = T(a.a1), A2 = T(a.a2), A3 = T(a.a3), V = T(a.vol);
-  double dt = dt_at(a.dt_dev, -a.mdt, r, k);
-  T mdt = T(-dt), tdt = T(dt);
-  T w0 = T(a.w0), w1 = T(a.w1), w2 = T(a.w2), g1 = T(a.grav1);
-  T rho = w[c], vx = w[r.sv + c];
-  // x1 face flux of row v at face q (the reference face pressure restored
-  // on an FP32 momentum anomaly)
-  auto F1 = [&](int v, int64_t q) {
-    T x = T(f1[v * r.sv + q]);
-    if (psf && v == IVX) x = x + psf[q];
-    return x;
-  };
-  T du[5];
-  for (int v = 0; v < 5; ++v) {
-    int64_t o = v * r.sv + c;
-    T div = A1 * F1(v, c + 1) - A1 * F1(v, c);
-    if (f2) div = div + (A2 * T(f2[o + r.s2]) - A2 * T(f2[o]));
-    if (f3) div = div + (A3 * T(f3[o + r.s3]) - A3 * T(f3[o]));
-    div = div / V;
-    du[v] = mdt * div;
-  }
-  // ConstGravityImpl::forward_split (const_gravity.cpp:35-43), nh = 1
-  du[IVX] = du[IVX] + tdt * rho * g1 * T(1.);
-  du[IPR] = du[IPR] + tdt * rho * vx * g1 * T(1.);
-  if (dsrc)
-    for (int v = 0; v < 5; ++v) du[v] = du[v] + dsrc[v * r.sv + c];
-  // LayerHeat bot-heat, an integer depth of bottom cells
-  // (src/forcing/layer_heat.cpp:78-85): du[IPR] += dt * flux / (dz * depth)
-  if (ii < a.heat_depth) du[IPR] = du[IPR] + T(dt * a.heat_flux / a.heat_dzd);
-  // gravity_energy_correction (hydro_forward.cpp:1049-1124), Cartesian
-  TF const* m1 = f1 + row;  // the IDN row of F1 along this column
-  auto drho = [&](int m) {  // -dt * x1_face_difference / V at cell lo1 + m
-    int q = r.lo1 + m;
-    return mdt * ((A1 * T(m1[q + 1]) - A1 * T(m1[q])) / V);
-  };
-  T gw = (T(tab[T_GW_HI * r.nc1 + i]) * (A1 * T(m1[i + 1])) -
-          T(tab[T_GW_LO * r.nc1 + i]) * (A1 * T(m1[i]))) /
-         V;
-  T fgw = tdt * gw;
-  int jj = (ii < 1 ? 1 : (ii > r.n1 - 2 ? r.n1 - 2 : ii)) - 1;
-  T s = T(tab[T_WT0 * r.nc1 + i]) * drho(jj);
-  s = s + T(tab[T_WT1 * r.nc1 + i]) * drho(jj + 1);
-  s = s + T(tab[T_WT2 * r.nc1 + i]) * drho(jj + 2);
-  fgw = fgw + T(tab[T_GVAR * r.nc1 + i]) * s;
-  T ogw = tdt * rho * vx * g1 * T(1.);
-  du[IPR] = du[IPR] + (fgw - ogw);
-  for (int v = 0; v < 5; ++v) {
-    int64_t o = v * r.sv + c;
-    if (WRITE_DU) {
-      static_cast<double*>(uv)[o] = du[v];
-    } else if (PAIR) {
-      float* uf = static_cast<float*>(uv);
-      float const* u0f = static_cast<float const*>(u0v);
-      double x = w0 * (double(u0f[o]) + double(c0[o])) +
-                 w1 * (double(uf[o]) + double(cc[o])) + w2 * du[v];
-      float h = float(x);
-      uf[o] = h;
-      cc[o] = float(x - double(h));
-    } else {
-      double* ud = static_cast<double*>(uv);
-      double const* u0d = static_cast<double const*>(u0v);
-      ud[o] = w0 * u0d[o] + w1 * ud[o] + w2 * du[v];
-    }
-  }
+  UpdateGravityBody<TF, WRITE_DU, PAIR>::run(block_index(), grid_size(), f1, f2, f3, u0v, c0, uv,
+                                             cc, w, tab, dsrc, psf, a);
 }
 
 }  // namespace
@@ -680,16 +707,16 @@ int snapy_sweep_x1_wb_ref(const snapy_grid* g, const snapy_physics* p, int dtype
   if (lds_of(nc) > 64 * 1024) return fail(-9, "sweep_x1_wb_ref: an x1 column above 2047 cells");
   dim3 grid(unsigned((ncol + nc - 1) / nc));
   if (dtype == SNAPY_F64)
-    hipLaunchKernelGGL((sweep_x1_wb_ref_kernel<double, double>), grid, dim3(kRefThreads),
+    launch<SweepX1WbRefBody<double, double>, kRefThreads, 1>((sweep_x1_wb_ref_kernel<double, double>), grid, dim3(kRefThreads),
                        lds_of(nc), as_stream(stream), w, ref, static_cast<double*>(flux),
                        static_cast<double*>(face_pressure), ra, a, nc);
   else if (p->riemann == SNAPY_RIEMANN_LMARS)
     // FP32 fluxes with LMARS: FP32 compute in the anomaly form
-    hipLaunchKernelGGL((sweep_x1_wb_ref_kernel<float, float>), grid, dim3(kRefThreads),
+    launch<SweepX1WbRefBody<float, float>, kRefThreads, 1>((sweep_x1_wb_ref_kernel<float, float>), grid, dim3(kRefThreads),
                        lds_of(nc), as_stream(stream), w, ref, static_cast<float*>(flux),
                        static_cast<double*>(face_pressure), ra, a, nc);
   else
-    hipLaunchKernelGGL((sweep_x1_wb_ref_kernel<double, float>), grid, dim3(kRefThreads),
+    launch<SweepX1WbRefBody<double, float>, kRefThreads, 1>((sweep_x1_wb_ref_kernel<double, float>), grid, dim3(kRefThreads),
                        lds_of(nc), as_stream(stream), w, ref, static_cast<float*>(flux),
                        static_cast<double*>(face_pressure), ra, a, nc);
   return launch_status("sweep_x1_wb_ref");
@@ -721,16 +748,19 @@ int snapy_update_gravity(const snapy_grid* g, const snapy_physics* p,
   a.heat_flux = p->bot_heat_flux;
   a.heat_dzd = g->dx1 * double(p->bot_heat_depth);
   int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
-  auto kern = u0 ? update_gravity_kernel<double, false, false>
-                 : update_gravity_kernel<double, true, false>;
   unsigned tpb = tuned(SNAPY_TUNE_UPDATE, g, SNAPY_F64, 256);
-  hipLaunchKernelGGL(kern,
-                     dim3(blocks_for(tot, tpb)), dim3(tpb), 0,
-                     as_stream(stream), static_cast<double const*>(flux[0]),
-                     static_cast<double const*>(flux[1]),
-                     static_cast<double const*>(flux[2]), u0, nullptr, u,
-                     nullptr, static_cast<double const*>(w), x1tab,
-                     static_cast<double const*>(du_src), nullptr, a);
+  auto go = [&](auto body, auto kern) {
+    launch<decltype(body), 256, 1>(kern, dim3(blocks_for(tot, tpb)), dim3(tpb), 0,
+                                   as_stream(stream), static_cast<double const*>(flux[0]),
+                                   static_cast<double const*>(flux[1]),
+                                   static_cast<double const*>(flux[2]), u0, nullptr, u,
+                                   nullptr, static_cast<double const*>(w), x1tab,
+                                   static_cast<double const*>(du_src), nullptr, a);
+  };
+  if (u0)
+    go(UpdateGravityBody<double, false, false>{}, update_gravity_kernel<double, false, false>);
+  else
+    go(UpdateGravityBody<double, true, false>{}, update_gravity_kernel<double, true, false>);
   return launch_status("update_gravity");
 }
 
@@ -764,15 +794,19 @@ int snapy_update_gravity_f32(const snapy_grid* g, const snapy_physics* p,
   a.heat_dzd = g->dx1 * double(p->bot_heat_depth);
   int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
   double const* psf = ref + R_PSF_LO * ranges(*g).sv;
-  auto kern = du_out ? update_gravity_kernel<float, true, true>
-                     : update_gravity_kernel<float, false, true>;
   void* uo = du_out ? static_cast<void*>(du_out) : static_cast<void*>(u);
   unsigned tpb = tuned(SNAPY_TUNE_UPDATE, g, SNAPY_F32, 256);
-  hipLaunchKernelGGL(kern, dim3(blocks_for(tot, tpb)), dim3(tpb), 0,
-                     as_stream(stream), static_cast<float const*>(flux[0]),
-                     static_cast<float const*>(flux[1]),
-                     static_cast<float const*>(flux[2]), u0, c0, uo, c, w,
-                     x1tab, du_src, psf, a);
+  auto go = [&](auto body, auto kern) {
+    launch<decltype(body), 256, 1>(kern, dim3(blocks_for(tot, tpb)), dim3(tpb), 0,
+                                   as_stream(stream), static_cast<float const*>(flux[0]),
+                                   static_cast<float const*>(flux[1]),
+                                   static_cast<float const*>(flux[2]), u0, c0, uo, c, w,
+                                   x1tab, du_src, psf, a);
+  };
+  if (du_out)
+    go(UpdateGravityBody<float, true, true>{}, update_gravity_kernel<float, true, true>);
+  else
+    go(UpdateGravityBody<float, false, true>{}, update_gravity_kernel<float, false, true>);
   return launch_status("update_gravity_f32");
 }
 

@@ -15,6 +15,7 @@
 #include <cmath>
 
 #include "common.hpp"
+#include "group.hpp"
 #include "point.hpp"
 
 namespace snapy {
@@ -517,14 +518,14 @@ struct WaveLanes {
   int64_t col;  // the column computed on
 };
 
-__device__ __forceinline__ WaveLanes wave_lanes(int64_t ncol) {
+__device__ __forceinline__ WaveLanes wave_lanes(int64_t ncol, unsigned bx) {
   WaveLanes L;
   int lane = threadIdx.x;
   L.shadow = lane >= 5 * kSlots;
   L.slot = L.shadow ? kSlots - 1 : lane / 5;
   L.p = L.shadow ? lane - 5 * kSlots : lane - 5 * L.slot;
   L.base = 5 * L.slot;
-  int64_t c = int64_t(blockIdx.x) * kSlots + L.slot;
+  int64_t c = int64_t(bx) * kSlots + L.slot;
   L.live = !L.shadow && c < ncol;
   L.col = c < ncol ? c : ncol - 1;
   return L;
@@ -570,217 +571,228 @@ __device__ __forceinline__ void stage_layers(double* dst, double const* __restri
 // block-tridiagonal system.  The column state is staged through LDS in
 // blocks of kLB layers; each lane writes its row of [A' | d | z0 | z1] to a
 // lane-contiguous workspace ws[blk][layer][q][64].
+struct VicForwardBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w,
+                                             double const* __restrict__ du,
+                                             double const* __restrict__ vtab,
+                                             double* __restrict__ ws, int* __restrict__ status,
+                                             VicArgs a) {
+    Range3 const& r = a.r;
+    int64_t const ncol = int64_t(r.n2) * r.n3;
+    WaveLanes const L = wave_lanes(ncol, bid.x);
+    int const p = L.p, base = L.base, slot = L.slot;
+    __shared__ int64_t s_row[kSlots];
+    __shared__ double s_w[5 * kSlots * kWin], s_du[5 * kSlots * kWin];
+    if (threadIdx.x < kSlots) {
+      int64_t c = int64_t(bid.x) * kSlots + threadIdx.x;
+      s_row[threadIdx.x] = column_row(r, c < ncol ? c : ncol - 1);
+    }
+    int const n = a.n, nc1 = r.nc1, lo1 = r.lo1;
+    double const gm1 = a.gamma - 1.;
+    double const gp1_gm1 = (gm1 + 1) / gm1;
+    double const grav = a.grav, A = a.area, V = a.vol;
+    double const dt = dt_at(a.dt_dev, a.dt_host, r, r.lo3 + int(L.col / r.n2)) * a.wstage;
+    double const idt = 1. / dt;
+    double const hiv = 0.5 / V;
+    double* wsl = ws + int64_t(bid.x) * n * kQ * 64 + threadIdx.x;
+    auto WS = [&](int l, int q) -> double& { return wsl[(l * kQ + q) * 64]; };
+    auto tab = [&](double const* t, int rowi, int i) { return t[rowi * nc1 + lo1 + i]; };
+    int l0 = 0;  // first layer of the staged block
+    auto SW = [&](int v, int i) { return s_w[(v * kSlots + slot) * kWin + (i - l0 + 2)]; };
+    auto SD = [&](int v, int i) { return s_du[(v * kSlots + slot) * kWin + (i - l0 + 2)]; };
+    auto stage = [&](int start) {
+      __syncthreads();
+      l0 = start;
+      stage_layers<5, kWin>(s_w, w, r.sv, 0, s_row, start - 2, n);
+      stage_layers<5, kWin>(s_du, du, r.sv, 0, s_row, start - 2, n);
+      __syncthreads();
+    };
+    auto rows_at = [&](int i, auto q) {
+      double s = tab(vtab, V_KMID, i) * q(i);
+      if (i > 0) s = s + tab(vtab, V_KLO, i) * q(i - 1);
+      if (i < n - 1) s = s + tab(vtab, V_KHI, i) * q(i + 1);
+      if (i == 0) s = s + a.t0 * q(2);
+      if (i == n - 1) s = s + a.t1 * q(n - 3);
+      return s;
+    };
+    auto mass0 = [&](int i) { return SD(IDN, i); };
+
+    stage(0);
+    // carried state: cell i, the face below it, the Jacobians of cells i-1, i
+    double wt[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) wt[v] = SW(v, -1);
+    RoeCell cl = roe_cell(wt, gm1);
+    double fl[5], fc[5];
+    jacobian_row(fl, p, gm1, gp1_gm1, wt);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) wt[v] = SW(v, 0);
+    RoeCell cc = roe_cell(wt, gm1);
+    jacobian_row(fc, p, gm1, gp1_gm1, wt);
+    double am[5], am0[5];
+    roe_rows(am, am0, p, gm1, gp1_gm1, cl, cc);
+    double Ap[5] = {0, 0, 0, 0, 0}, dpv = 0., z0v = 0., z1v = 0.;
+    bool ok = true;
+    double const tol = 8. * 5. * 2.220446049250313e-16;  // ludcmp's singularity test
+
+    for (int i = 0; i < n; ++i) {
+      if (i > 0 && i % kLB == 0) stage(i);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) wt[v] = SW(v, i + 1);
+      RoeCell cr = roe_cell(wt, gm1);
+      double fr[5], apm[5], ap0[5];
+      jacobian_row(fr, p, gm1, gp1_gm1, wt);
+      roe_rows(apm, ap0, p, gm1, gp1_gm1, cc, cr);
+      // block rows (the (A - A) * FluxJacobian(cell) term of the diagonal is
+      // zero on the uniform Cartesian x1 grid)
+      double ar[5], br[5], crw[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        br[q] = -(am[q] + fl[q]) * A * hiv;
+        crw[q] = -(apm[q] - fr[q]) * A * hiv;
+        ar[q] = (am[q] * A + apm[q] * A) * hiv;
+      }
+      ar[IDN] -= p == IVX ? grav : 0.;
+      ar[IVX] -= p == IPR ? grav : 0.;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) ar[q] += q == p ? idt : 0.;
+      double rhs = SD(p, i) * idt;
+      if (p == IPR) {
+        // gravity-work rows (vic_gravity_work_impl.h:24-36)
+        double lo = tab(vtab, V_WLO, i), up = tab(vtab, V_WHI, i);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          ar[q] -= grav * (up * ap0[q] - lo * am0[q]);
+          br[q] -= grav * lo * am0[q];
+          crw[q] += grav * up * ap0[q];
+        }
+        ar[IVX] += grav * (1. - lo - up);
+        br[IVX] -= grav * lo;
+        crw[IVX] -= grav * up;
+      }
+      // the wall mirror merges the boundary face: Bnd = (1, -1, 1, 1, 1)
+      if (i == 0)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) ar[q] += br[q] * (q == IVX ? -1. : 1.);
+      if (i == n - 1)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) ar[q] += crw[q] * (q == IVX ? -1. : 1.);
+      if (p == IPR) {
+        // couple_radial_exact (implicit_hydro.cpp:408-446)
+        ar[IDN] -= tab(vtab, V_KMID, i) / dt;
+        br[IDN] -= tab(vtab, V_KLO, i) / dt;
+        crw[IDN] -= tab(vtab, V_KHI, i) / dt;
+        rhs = (SD(IPR, i) - rows_at(i, mass0)) / dt;
+      }
+      // right-hand sides: c | du/dt | unit E at the bottom | unit E at the top
+      double x[8];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) x[q] = crw[q];
+      x[5] = rhs;
+      x[6] = (i == 0 && p == IPR) ? 1. : 0.;
+      x[7] = (i == n - 1 && p == IPR) ? 1. : 0.;
+      if (i > 0) {
+        // a_i -= b_i A'_{i-1}, rhs -= b_i d_{i-1}, e0 -= b_i z0_{i-1}; the
+        // top-wall column z1 is zero below the top layer
+        double s[5] = {0, 0, 0, 0, 0}, s5 = 0., s6 = 0.;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          double bm = br[m];
+#pragma unroll
+          for (int q = 0; q < 5; ++q) s[q] += bm * from_lane(Ap[q], base + m);
+          s5 += bm * from_lane(dpv, base + m);
+          s6 += bm * from_lane(z0v, base + m);
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) ar[q] -= s[q];
+        x[5] -= s5;
+        x[6] -= s6;
+      }
+      // Gauss-Jordan, scaled partial pivoting (the last maximum wins, as in
+      // ludcmp); columns left of the pivot are not needed again
+      double big = 0.;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) big = fmax(big, fabs(ar[q]));
+      double scale = big > 0. ? 1. / big : 0.;
+      unsigned used = 0;
+      int piv[5];
+#pragma unroll
+      for (int jc = 0; jc < 5; ++jc) {
+        double bv = (used >> p) & 1 ? -1. : scale * fabs(ar[jc]);
+        double best = -1.;
+        int bl = 0;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          double c = from_lane(bv, base + m);
+          if (c >= best) {
+            best = c;
+            bl = m;
+          }
+        }
+        if (!(best > tol) || !isfinite(best)) ok = false;
+        used |= 1u << bl;
+        piv[jc] = bl;
+        int src = base + bl;
+        double pj = from_lane(ar[jc], src);
+        double fct = p == bl ? 0. : ar[jc] * (1. / pj);
+#pragma unroll
+        for (int q = jc + 1; q < 5; ++q) ar[q] -= fct * from_lane(ar[q], src);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) x[t] -= fct * from_lane(x[t], src);
+      }
+      // this lane pivoted the unknown whose column it still holds; lane p
+      // takes the solution row of unknown p from its pivot lane
+      int myj = 0;
+#pragma unroll
+      for (int jc = 0; jc < 5; ++jc) myj = piv[jc] == p ? jc : myj;
+      double dj = ar[0];
+#pragma unroll
+      for (int jc = 1; jc < 5; ++jc) dj = myj == jc ? ar[jc] : dj;
+      double idj = 1. / dj;
+      int srcp = piv[0];
+#pragma unroll
+      for (int jc = 1; jc < 5; ++jc) srcp = p == jc ? piv[jc] : srcp;
+      double X[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) X[t] = from_lane(x[t] * idj, base + srcp);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) Ap[q] = X[q];
+      dpv = X[5];
+      z0v = X[6];
+      z1v = X[7];
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (!isfinite(X[t])) ok = false;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) WS(i, q) = X[q];
+      WS(i, 5) = dpv;
+      WS(i, 6) = z0v;
+      // slide the window up one layer
+      cl = cc;
+      cc = cr;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        fl[q] = fc[q];
+        fc[q] = fr[q];
+        am[q] = apm[q];
+        am0[q] = ap0[q];
+      }
+    }
+    // a column fails as a whole: its five lanes agree
+    int okv = ok ? 1 : 0;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) okv &= __shfl(okv, base + m);
+    if (p == 0 && L.live) vic_mark(status, L.col, okv != 0);
+    WS(n - 1, 7) = z1v;  // the top-wall column, zero below the top layer
+  }
+};
+
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     vic_forward_kernel(double const* __restrict__ w, double const* __restrict__ du,
                        double const* __restrict__ vtab, double* __restrict__ ws,
                        int* __restrict__ status, VicArgs a) {
-  Range3 const& r = a.r;
-  int64_t const ncol = int64_t(r.n2) * r.n3;
-  WaveLanes const L = wave_lanes(ncol);
-  int const p = L.p, base = L.base, slot = L.slot;
-  __shared__ int64_t s_row[kSlots];
-  __shared__ double s_w[5 * kSlots * kWin], s_du[5 * kSlots * kWin];
-  if (threadIdx.x < kSlots) {
-    int64_t c = int64_t(blockIdx.x) * kSlots + threadIdx.x;
-    s_row[threadIdx.x] = column_row(r, c < ncol ? c : ncol - 1);
-  }
-  int const n = a.n, nc1 = r.nc1, lo1 = r.lo1;
-  double const gm1 = a.gamma - 1.;
-  double const gp1_gm1 = (gm1 + 1) / gm1;
-  double const grav = a.grav, A = a.area, V = a.vol;
-  double const dt = dt_at(a.dt_dev, a.dt_host, r, r.lo3 + int(L.col / r.n2)) * a.wstage;
-  double const idt = 1. / dt;
-  double const hiv = 0.5 / V;
-  double* wsl = ws + int64_t(blockIdx.x) * n * kQ * 64 + threadIdx.x;
-  auto WS = [&](int l, int q) -> double& { return wsl[(l * kQ + q) * 64]; };
-  auto tab = [&](double const* t, int rowi, int i) { return t[rowi * nc1 + lo1 + i]; };
-  int l0 = 0;  // first layer of the staged block
-  auto SW = [&](int v, int i) { return s_w[(v * kSlots + slot) * kWin + (i - l0 + 2)]; };
-  auto SD = [&](int v, int i) { return s_du[(v * kSlots + slot) * kWin + (i - l0 + 2)]; };
-  auto stage = [&](int start) {
-    __syncthreads();
-    l0 = start;
-    stage_layers<5, kWin>(s_w, w, r.sv, 0, s_row, start - 2, n);
-    stage_layers<5, kWin>(s_du, du, r.sv, 0, s_row, start - 2, n);
-    __syncthreads();
-  };
-  auto rows_at = [&](int i, auto q) {
-    double s = tab(vtab, V_KMID, i) * q(i);
-    if (i > 0) s = s + tab(vtab, V_KLO, i) * q(i - 1);
-    if (i < n - 1) s = s + tab(vtab, V_KHI, i) * q(i + 1);
-    if (i == 0) s = s + a.t0 * q(2);
-    if (i == n - 1) s = s + a.t1 * q(n - 3);
-    return s;
-  };
-  auto mass0 = [&](int i) { return SD(IDN, i); };
-
-  stage(0);
-  // carried state: cell i, the face below it, the Jacobians of cells i-1, i
-  double wt[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) wt[v] = SW(v, -1);
-  RoeCell cl = roe_cell(wt, gm1);
-  double fl[5], fc[5];
-  jacobian_row(fl, p, gm1, gp1_gm1, wt);
-#pragma unroll
-  for (int v = 0; v < 5; ++v) wt[v] = SW(v, 0);
-  RoeCell cc = roe_cell(wt, gm1);
-  jacobian_row(fc, p, gm1, gp1_gm1, wt);
-  double am[5], am0[5];
-  roe_rows(am, am0, p, gm1, gp1_gm1, cl, cc);
-  double Ap[5] = {0, 0, 0, 0, 0}, dpv = 0., z0v = 0., z1v = 0.;
-  bool ok = true;
-  double const tol = 8. * 5. * 2.220446049250313e-16;  // ludcmp's singularity test
-
-  for (int i = 0; i < n; ++i) {
-    if (i > 0 && i % kLB == 0) stage(i);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) wt[v] = SW(v, i + 1);
-    RoeCell cr = roe_cell(wt, gm1);
-    double fr[5], apm[5], ap0[5];
-    jacobian_row(fr, p, gm1, gp1_gm1, wt);
-    roe_rows(apm, ap0, p, gm1, gp1_gm1, cc, cr);
-    // block rows (the (A - A) * FluxJacobian(cell) term of the diagonal is
-    // zero on the uniform Cartesian x1 grid)
-    double ar[5], br[5], crw[5];
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      br[q] = -(am[q] + fl[q]) * A * hiv;
-      crw[q] = -(apm[q] - fr[q]) * A * hiv;
-      ar[q] = (am[q] * A + apm[q] * A) * hiv;
-    }
-    ar[IDN] -= p == IVX ? grav : 0.;
-    ar[IVX] -= p == IPR ? grav : 0.;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) ar[q] += q == p ? idt : 0.;
-    double rhs = SD(p, i) * idt;
-    if (p == IPR) {
-      // gravity-work rows (vic_gravity_work_impl.h:24-36)
-      double lo = tab(vtab, V_WLO, i), up = tab(vtab, V_WHI, i);
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        ar[q] -= grav * (up * ap0[q] - lo * am0[q]);
-        br[q] -= grav * lo * am0[q];
-        crw[q] += grav * up * ap0[q];
-      }
-      ar[IVX] += grav * (1. - lo - up);
-      br[IVX] -= grav * lo;
-      crw[IVX] -= grav * up;
-    }
-    // the wall mirror merges the boundary face: Bnd = (1, -1, 1, 1, 1)
-    if (i == 0)
-#pragma unroll
-      for (int q = 0; q < 5; ++q) ar[q] += br[q] * (q == IVX ? -1. : 1.);
-    if (i == n - 1)
-#pragma unroll
-      for (int q = 0; q < 5; ++q) ar[q] += crw[q] * (q == IVX ? -1. : 1.);
-    if (p == IPR) {
-      // couple_radial_exact (implicit_hydro.cpp:408-446)
-      ar[IDN] -= tab(vtab, V_KMID, i) / dt;
-      br[IDN] -= tab(vtab, V_KLO, i) / dt;
-      crw[IDN] -= tab(vtab, V_KHI, i) / dt;
-      rhs = (SD(IPR, i) - rows_at(i, mass0)) / dt;
-    }
-    // right-hand sides: c | du/dt | unit E at the bottom | unit E at the top
-    double x[8];
-#pragma unroll
-    for (int q = 0; q < 5; ++q) x[q] = crw[q];
-    x[5] = rhs;
-    x[6] = (i == 0 && p == IPR) ? 1. : 0.;
-    x[7] = (i == n - 1 && p == IPR) ? 1. : 0.;
-    if (i > 0) {
-      // a_i -= b_i A'_{i-1}, rhs -= b_i d_{i-1}, e0 -= b_i z0_{i-1}; the
-      // top-wall column z1 is zero below the top layer
-      double s[5] = {0, 0, 0, 0, 0}, s5 = 0., s6 = 0.;
-#pragma unroll
-      for (int m = 0; m < 5; ++m) {
-        double bm = br[m];
-#pragma unroll
-        for (int q = 0; q < 5; ++q) s[q] += bm * from_lane(Ap[q], base + m);
-        s5 += bm * from_lane(dpv, base + m);
-        s6 += bm * from_lane(z0v, base + m);
-      }
-#pragma unroll
-      for (int q = 0; q < 5; ++q) ar[q] -= s[q];
-      x[5] -= s5;
-      x[6] -= s6;
-    }
-    // Gauss-Jordan, scaled partial pivoting (the last maximum wins, as in
-    // ludcmp); columns left of the pivot are not needed again
-    double big = 0.;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) big = fmax(big, fabs(ar[q]));
-    double scale = big > 0. ? 1. / big : 0.;
-    unsigned used = 0;
-    int piv[5];
-#pragma unroll
-    for (int jc = 0; jc < 5; ++jc) {
-      double bv = (used >> p) & 1 ? -1. : scale * fabs(ar[jc]);
-      double best = -1.;
-      int bl = 0;
-#pragma unroll
-      for (int m = 0; m < 5; ++m) {
-        double c = from_lane(bv, base + m);
-        if (c >= best) {
-          best = c;
-          bl = m;
-        }
-      }
-      if (!(best > tol) || !isfinite(best)) ok = false;
-      used |= 1u << bl;
-      piv[jc] = bl;
-      int src = base + bl;
-      double pj = from_lane(ar[jc], src);
-      double fct = p == bl ? 0. : ar[jc] * (1. / pj);
-#pragma unroll
-      for (int q = jc + 1; q < 5; ++q) ar[q] -= fct * from_lane(ar[q], src);
-#pragma unroll
-      for (int t = 0; t < 8; ++t) x[t] -= fct * from_lane(x[t], src);
-    }
-    // this lane pivoted the unknown whose column it still holds; lane p
-    // takes the solution row of unknown p from its pivot lane
-    int myj = 0;
-#pragma unroll
-    for (int jc = 0; jc < 5; ++jc) myj = piv[jc] == p ? jc : myj;
-    double dj = ar[0];
-#pragma unroll
-    for (int jc = 1; jc < 5; ++jc) dj = myj == jc ? ar[jc] : dj;
-    double idj = 1. / dj;
-    int srcp = piv[0];
-#pragma unroll
-    for (int jc = 1; jc < 5; ++jc) srcp = p == jc ? piv[jc] : srcp;
-    double X[8];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) X[t] = from_lane(x[t] * idj, base + srcp);
-#pragma unroll
-    for (int q = 0; q < 5; ++q) Ap[q] = X[q];
-    dpv = X[5];
-    z0v = X[6];
-    z1v = X[7];
-#pragma unroll
-    for (int t = 0; t < 8; ++t)
-      if (!isfinite(X[t])) ok = false;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) WS(i, q) = X[q];
-    WS(i, 5) = dpv;
-    WS(i, 6) = z0v;
-    // slide the window up one layer
-    cl = cc;
-    cc = cr;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      fl[q] = fc[q];
-      fc[q] = fr[q];
-      am[q] = apm[q];
-      am0[q] = ap0[q];
-    }
-  }
-  // a column fails as a whole: its five lanes agree
-  int okv = ok ? 1 : 0;
-#pragma unroll
-  for (int m = 0; m < 5; ++m) okv &= __shfl(okv, base + m);
-  if (p == 0 && L.live) vic_mark(status, L.col, okv != 0);
-  WS(n - 1, 7) = z1v;  // the top-wall column, zero below the top layer
+  VicForwardBody::run(block_index(), grid_size(), w, du, vtab, ws, status, a);
 }
 
 // The back half of the VIC, with the lanes and workspace of
@@ -792,6 +804,261 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 // PAIR: the state is the FP32 pair (u, c) of the FP32 state form, combined
 // in FP64; otherwise FP64 u0, u.
 template <bool PAIR>
+struct VicBackBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w,
+                                             double const* __restrict__ du,
+                                             void const* __restrict__ u0v,
+                                             float const* __restrict__ c0, void* __restrict__ uv,
+                                             float* __restrict__ cc,
+                                             double const* __restrict__ vtab,
+                                             double const* __restrict__ xtab,
+                                             double* __restrict__ ws,
+                                             int const* __restrict__ status, VicArgs a) {
+    Range3 const& r = a.r;
+    int64_t const ncol = int64_t(r.n2) * r.n3;
+    WaveLanes const L = wave_lanes(ncol, bid.x);
+    int const p = L.p, base = L.base, slot = L.slot;
+    __shared__ int64_t s_row[kSlots];
+    __shared__ int s_live[kSlots];
+    __shared__ double s_buf[5 * kSlots * kOB];  // staged mass/density, then outputs
+    __shared__ double s_mw[kSlots * kOWin];     // staged du row IDN
+    if (threadIdx.x < kSlots) {
+      int64_t c = int64_t(bid.x) * kSlots + threadIdx.x;
+      s_row[threadIdx.x] = column_row(r, c < ncol ? c : ncol - 1);
+      s_live[threadIdx.x] = c < ncol;
+    }
+    __syncthreads();
+    int const n = a.n, nc1 = r.nc1, lo1 = r.lo1;
+    int64_t const row = s_row[slot];
+    double const V = a.vol;
+    double const dt = dt_at(a.dt_dev, a.dt_host, r, r.lo3 + int(L.col / r.n2)) * a.wstage;
+    double* wsw = ws + int64_t(bid.x) * n * kQ * 64;
+    double* wsl = wsw + threadIdx.x;
+    double* wsb = wsw + base;  // lane 0 of the column
+    auto WS = [&](int l, int q) -> double& { return wsl[(l * kQ + q) * 64]; };
+    auto WSB = [&](int l, int q) -> double& { return wsb[(l * kQ + q) * 64]; };
+    auto tab = [&](double const* t, int rowi, int i) { return t[rowi * nc1 + lo1 + i]; };
+    bool const failed = (status[L.col] & kVicStageFailed) != 0;
+    constexpr int B = 4;  // layers per batch of workspace loads
+
+    // ---- back substitution (vic_backward_substitute) ------------------------
+    int const i0 = 2, i1 = n - 3;
+    double dpv = WS(n - 1, 5), z0v = WS(n - 1, 6), z1v = WS(n - 1, 7);
+    double d_i0 = 0., z0_i0 = 0., z1_i0 = 0., d_i1 = 0., z0_i1 = 0., z1_i1 = 0.;
+    for (int ib = n - 2; ib >= 0; ib -= B) {
+      double Ab[B][5], db[B], zb[B];
+#pragma unroll
+      for (int t = 0; t < B; ++t) {
+        int i = ib - t > 0 ? ib - t : 0;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) Ab[t][q] = WS(i, q);
+        db[t] = WS(i, 5);
+        zb[t] = WS(i, 6);
+      }
+#pragma unroll
+      for (int t = 0; t < B; ++t) {
+        int i = ib - t;
+        if (i >= 0) {
+          double sd = 0., s0 = 0., s1 = 0.;
+#pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            double aq = Ab[t][q];
+            sd += aq * from_lane(dpv, base + q);
+            s0 += aq * from_lane(z0v, base + q);
+            s1 += aq * from_lane(z1v, base + q);
+          }
+          dpv = db[t] - sd;
+          z0v = zb[t] - s0;
+          z1v = 0. - s1;  // the forward z1 is zero below the top layer
+          WS(i, 5) = dpv;
+          WS(i, 6) = z0v;
+          WS(i, 7) = z1v;
+          if (i == i0) {
+            d_i0 = from_lane(dpv, base);
+            z0_i0 = from_lane(z0v, base);
+            z1_i0 = from_lane(z1v, base);
+          }
+          if (i == i1) {
+            d_i1 = from_lane(dpv, base);
+            z0_i1 = from_lane(z0v, base);
+            z1_i1 = from_lane(z1v, base);
+          }
+        }
+      }
+    }
+    // ---- Woodbury 2x2 (vic_woodbury_column); every lane solves it -----------
+    double k0 = -a.t0 / dt, k1 = -a.t1 / dt;
+    double m00 = 1. + k0 * z0_i0, m01 = k0 * z1_i0;
+    double m10 = k1 * z0_i1, m11 = 1. + k1 * z1_i1;
+    double r0 = k0 * d_i0, r1 = k1 * d_i1;
+    double det = m00 * m11 - m01 * m10;
+    double al0 = (m11 * r0 - m01 * r1) / det, al1 = (m00 * r1 - m10 * r0) / det;
+    // the corrected increment of this lane's row: y = d - z0 al0 - z1 al1
+    auto ywb = [&](int l) {
+      double y = WS(l, 5);
+      y -= WS(l, 6) * al0;
+      y -= WS(l, 7) * al1;
+      return y;
+    };
+    // ---- mass redistribution (vic_constituent_column, dry) ------------------
+    // lane 0 of each column scans; lane 0's dead A' row keeps per layer
+    // [0] M at the face below, [1] the mass change, [2] the moved q below,
+    // and its [5] becomes the corrected mass increment
+    constexpr int SB = kOB + 1;
+    double* s_m0 = s_buf;
+    double* s_rho = s_buf + kSlots * SB;
+    double R = 0., S = 0.;
+    for (int b0 = 0; b0 < n; b0 += kOB) {
+      __syncthreads();
+      stage_layers<1, SB>(s_m0, du, r.sv, IDN, s_row, b0, n);
+      stage_layers<1, SB>(s_rho, w, r.sv, IDN, s_row, b0, n);
+      __syncthreads();
+      if (p == 0) {
+#pragma unroll
+        for (int t = 0; t < kOB; ++t)
+          if (b0 + t < n) {
+            R += (ywb(b0 + t) - s_m0[slot * SB + t]) * V;
+            S += s_rho[slot * SB + t] * V;
+          }
+      }
+    }
+    double M = 0., avail = 0., carry = 0.;
+    for (int b0 = 0; b0 < n; b0 += kOB) {
+      __syncthreads();
+      stage_layers<1, SB>(s_m0, du, r.sv, IDN, s_row, b0, n);
+      stage_layers<1, SB>(s_rho, w, r.sv, IDN, s_row, b0, n);
+      __syncthreads();
+      if (p == 0) {
+        if (b0 == 0) {
+          avail = (s_rho[slot * SB] * 1. + s_m0[slot * SB]) * V;
+          if (avail < 0) avail = 0;
+          WS(0, 2) = 0.;
+        }
+#pragma unroll 8
+        for (int t = 0; t < kOB; ++t) {
+          int i = b0 + t;
+          if (i < n) {
+            double y0 = ywb(i);
+            WS(i, 5) = y0;  // read back as the mass row by the other lanes
+            WS(i, 0) = M;
+            double phi = (y0 - s_m0[slot * SB + t]) * V;
+            M -= phi - R * (s_rho[slot * SB + t] * V / S);
+            if (i + 1 < n) {
+              double avail_up = (s_rho[slot * SB + t + 1] * 1. + s_m0[slot * SB + t + 1]) * V;
+              if (avail_up < 0) avail_up = 0;
+              double mf = M;
+              double q = mf > 0 ? mf * 1. : mf * 1.;
+              if (q > avail) q = avail;
+              if (q < -avail_up) q = -avail_up;
+              WS(i, 1) = carry - q / V;
+              carry = 0. + q / V;
+              WS(i + 1, 2) = q;
+              avail = avail_up + q;
+            } else {
+              WS(i, 1) = carry;
+            }
+          }
+        }
+      }
+    }
+    // lane 0's rows become visible to the column's other lanes
+    __threadfence();
+    // ---- outputs, the remainder (book_remainder_), the RK combine -----------
+    double* s_out = s_buf;
+    for (int b0 = 0; b0 < n; b0 += kOB) {
+      __syncthreads();
+      stage_layers<1, kOWin>(s_mw, du, r.sv, IDN, s_row, b0 - 2, n);
+      __syncthreads();
+      auto M0 = [&](int i) { return s_mw[slot * kOWin + (i - b0 + 2)]; };
+      auto moved_rho = [&](int i) { return (M0(i) + WSB(i, 1)) - M0(i); };
+      auto raw_minus_mass0 = [&](int i) { return WSB(i, 5) - M0(i); };
+#pragma unroll 8
+      for (int t = 0; t < kOB; ++t) {
+        int i = b0 + t;
+        double out = 0.;
+        if (i < n) {
+          if (p == IDN) {
+            out = M0(i) + WSB(i, 1);
+            if (a.moved && L.live) a.moved[row + i] = failed ? 0. : WSB(i, 2);
+          } else if (p == IPR) {
+            out = ywb(i);
+            double req_lo = WSB(i, 0), req_hi = i + 1 < n ? WSB(i + 1, 0) : 0.;
+            double mv_lo = WSB(i, 2) - req_lo;
+            double mv_hi = (i + 1 < n ? WSB(i + 1, 2) : 0.) - req_hi;
+            double projected = (req_lo - req_hi) / V;
+            double pw = tab(vtab, V_PHI, i) * (WSB(i, 5) - M0(i) - projected);
+            double cw = -(tab(vtab, V_DPHI, i) * mv_hi - tab(vtab, V_DPLO, i) * mv_lo) / V;
+            out = out + (pw + cw);
+            // corrected-PE work of the moved mass minus the operator's part
+            int jj = (i < 1 ? 1 : (i > n - 2 ? n - 2 : i)) - 1;
+            double s = tab(xtab, T_WT0, i) * moved_rho(jj);
+            s = s + tab(xtab, T_WT1, i) * moved_rho(jj + 1);
+            s = s + tab(xtab, T_WT2, i) * moved_rho(jj + 2);
+            double work = tab(xtab, T_GVAR, i) * s;
+            double rs = tab(vtab, V_KMID, i) * raw_minus_mass0(i);
+            if (i > 0) rs = rs + tab(vtab, V_KLO, i) * raw_minus_mass0(i - 1);
+            if (i < n - 1) rs = rs + tab(vtab, V_KHI, i) * raw_minus_mass0(i + 1);
+            if (i == 0) rs = rs + a.t0 * raw_minus_mass0(2);
+            if (i == n - 1) rs = rs + a.t1 * raw_minus_mass0(n - 3);
+            work = work - rs;
+            out = out + work;
+          } else {
+            out = ywb(i);
+          }
+          // a failed column keeps its explicit increments (the reference
+          // undoes the column with its status bits and flags a retry)
+          if (failed) out = du[p * r.sv + row + i];
+        }
+        if (!L.shadow) s_out[(p * kSlots + slot) * kOB + t] = out;
+      }
+      __syncthreads();
+      // the RK combine u = w0 u0 + w1 u + w2 du' over the block, coalesced,
+      // in chunks whose loads are all issued before their stores
+      constexpr int TOT = 5 * kSlots * kOB;
+      constexpr int CH = 5;  // elements per lane per chunk
+      for (int f0 = 0; f0 < TOT; f0 += 64 * CH) {
+        double x0[CH], x1[CH];
+        int64_t oo[CH];
+#pragma unroll
+        for (int t = 0; t < CH; ++t) {
+          int f = f0 + threadIdx.x + 64 * t;
+          oo[t] = -1;
+          x0[t] = 0.;
+          x1[t] = 0.;
+          if (f < TOT) {
+            int v = f / (kSlots * kOB), h = f - v * (kSlots * kOB);
+            int sl = h / kOB, i = b0 + (h - sl * kOB);
+            if (i < n && s_live[sl]) {
+              oo[t] = v * r.sv + s_row[sl] + i;
+              if constexpr (PAIR) {
+                x0[t] = double(static_cast<float const*>(u0v)[oo[t]]) + double(c0[oo[t]]);
+                x1[t] = double(static_cast<float*>(uv)[oo[t]]) + double(cc[oo[t]]);
+              } else {
+                x0[t] = static_cast<double const*>(u0v)[oo[t]];
+                x1[t] = static_cast<double*>(uv)[oo[t]];
+              }
+            }
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < CH; ++t)
+          if (oo[t] >= 0) {
+            double x = a.w0 * x0[t] + a.w1 * x1[t] + a.w2 * s_out[f0 + threadIdx.x + 64 * t];
+            if constexpr (PAIR) {
+              float h = float(x);
+              static_cast<float*>(uv)[oo[t]] = h;
+              cc[oo[t]] = float(x - double(h));
+            } else {
+              static_cast<double*>(uv)[oo[t]] = x;
+            }
+          }
+      }
+    }
+  }
+};
+
+template <bool PAIR>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
     vic_back_kernel(double const* __restrict__ w, double const* __restrict__ du,
                     void const* __restrict__ u0v, float const* __restrict__ c0,
@@ -799,246 +1066,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
                     double const* __restrict__ vtab,
                     double const* __restrict__ xtab, double* __restrict__ ws,
                     int const* __restrict__ status, VicArgs a) {
-  Range3 const& r = a.r;
-  int64_t const ncol = int64_t(r.n2) * r.n3;
-  WaveLanes const L = wave_lanes(ncol);
-  int const p = L.p, base = L.base, slot = L.slot;
-  __shared__ int64_t s_row[kSlots];
-  __shared__ int s_live[kSlots];
-  __shared__ double s_buf[5 * kSlots * kOB];  // staged mass/density, then outputs
-  __shared__ double s_mw[kSlots * kOWin];     // staged du row IDN
-  if (threadIdx.x < kSlots) {
-    int64_t c = int64_t(blockIdx.x) * kSlots + threadIdx.x;
-    s_row[threadIdx.x] = column_row(r, c < ncol ? c : ncol - 1);
-    s_live[threadIdx.x] = c < ncol;
-  }
-  __syncthreads();
-  int const n = a.n, nc1 = r.nc1, lo1 = r.lo1;
-  int64_t const row = s_row[slot];
-  double const V = a.vol;
-  double const dt = dt_at(a.dt_dev, a.dt_host, r, r.lo3 + int(L.col / r.n2)) * a.wstage;
-  double* wsw = ws + int64_t(blockIdx.x) * n * kQ * 64;
-  double* wsl = wsw + threadIdx.x;
-  double* wsb = wsw + base;  // lane 0 of the column
-  auto WS = [&](int l, int q) -> double& { return wsl[(l * kQ + q) * 64]; };
-  auto WSB = [&](int l, int q) -> double& { return wsb[(l * kQ + q) * 64]; };
-  auto tab = [&](double const* t, int rowi, int i) { return t[rowi * nc1 + lo1 + i]; };
-  bool const failed = (status[L.col] & kVicStageFailed) != 0;
-  constexpr int B = 4;  // layers per batch of workspace loads
-
-  // ---- back substitution (vic_backward_substitute) ------------------------
-  int const i0 = 2, i1 = n - 3;
-  double dpv = WS(n - 1, 5), z0v = WS(n - 1, 6), z1v = WS(n - 1, 7);
-  double d_i0 = 0., z0_i0 = 0., z1_i0 = 0., d_i1 = 0., z0_i1 = 0., z1_i1 = 0.;
-  for (int ib = n - 2; ib >= 0; ib -= B) {
-    double Ab[B][5], db[B], zb[B];
-#pragma unroll
-    for (int t = 0; t < B; ++t) {
-      int i = ib - t > 0 ? ib - t : 0;
-#pragma unroll
-      for (int q = 0; q < 5; ++q) Ab[t][q] = WS(i, q);
-      db[t] = WS(i, 5);
-      zb[t] = WS(i, 6);
-    }
-#pragma unroll
-    for (int t = 0; t < B; ++t) {
-      int i = ib - t;
-      if (i >= 0) {
-        double sd = 0., s0 = 0., s1 = 0.;
-#pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          double aq = Ab[t][q];
-          sd += aq * from_lane(dpv, base + q);
-          s0 += aq * from_lane(z0v, base + q);
-          s1 += aq * from_lane(z1v, base + q);
-        }
-        dpv = db[t] - sd;
-        z0v = zb[t] - s0;
-        z1v = 0. - s1;  // the forward z1 is zero below the top layer
-        WS(i, 5) = dpv;
-        WS(i, 6) = z0v;
-        WS(i, 7) = z1v;
-        if (i == i0) {
-          d_i0 = from_lane(dpv, base);
-          z0_i0 = from_lane(z0v, base);
-          z1_i0 = from_lane(z1v, base);
-        }
-        if (i == i1) {
-          d_i1 = from_lane(dpv, base);
-          z0_i1 = from_lane(z0v, base);
-          z1_i1 = from_lane(z1v, base);
-        }
-      }
-    }
-  }
-  // ---- Woodbury 2x2 (vic_woodbury_column); every lane solves it -----------
-  double k0 = -a.t0 / dt, k1 = -a.t1 / dt;
-  double m00 = 1. + k0 * z0_i0, m01 = k0 * z1_i0;
-  double m10 = k1 * z0_i1, m11 = 1. + k1 * z1_i1;
-  double r0 = k0 * d_i0, r1 = k1 * d_i1;
-  double det = m00 * m11 - m01 * m10;
-  double al0 = (m11 * r0 - m01 * r1) / det, al1 = (m00 * r1 - m10 * r0) / det;
-  // the corrected increment of this lane's row: y = d - z0 al0 - z1 al1
-  auto ywb = [&](int l) {
-    double y = WS(l, 5);
-    y -= WS(l, 6) * al0;
-    y -= WS(l, 7) * al1;
-    return y;
-  };
-  // ---- mass redistribution (vic_constituent_column, dry) ------------------
-  // lane 0 of each column scans; lane 0's dead A' row keeps per layer
-  // [0] M at the face below, [1] the mass change, [2] the moved q below,
-  // and its [5] becomes the corrected mass increment
-  constexpr int SB = kOB + 1;
-  double* s_m0 = s_buf;
-  double* s_rho = s_buf + kSlots * SB;
-  double R = 0., S = 0.;
-  for (int b0 = 0; b0 < n; b0 += kOB) {
-    __syncthreads();
-    stage_layers<1, SB>(s_m0, du, r.sv, IDN, s_row, b0, n);
-    stage_layers<1, SB>(s_rho, w, r.sv, IDN, s_row, b0, n);
-    __syncthreads();
-    if (p == 0) {
-#pragma unroll
-      for (int t = 0; t < kOB; ++t)
-        if (b0 + t < n) {
-          R += (ywb(b0 + t) - s_m0[slot * SB + t]) * V;
-          S += s_rho[slot * SB + t] * V;
-        }
-    }
-  }
-  double M = 0., avail = 0., carry = 0.;
-  for (int b0 = 0; b0 < n; b0 += kOB) {
-    __syncthreads();
-    stage_layers<1, SB>(s_m0, du, r.sv, IDN, s_row, b0, n);
-    stage_layers<1, SB>(s_rho, w, r.sv, IDN, s_row, b0, n);
-    __syncthreads();
-    if (p == 0) {
-      if (b0 == 0) {
-        avail = (s_rho[slot * SB] * 1. + s_m0[slot * SB]) * V;
-        if (avail < 0) avail = 0;
-        WS(0, 2) = 0.;
-      }
-#pragma unroll 8
-      for (int t = 0; t < kOB; ++t) {
-        int i = b0 + t;
-        if (i < n) {
-          double y0 = ywb(i);
-          WS(i, 5) = y0;  // read back as the mass row by the other lanes
-          WS(i, 0) = M;
-          double phi = (y0 - s_m0[slot * SB + t]) * V;
-          M -= phi - R * (s_rho[slot * SB + t] * V / S);
-          if (i + 1 < n) {
-            double avail_up = (s_rho[slot * SB + t + 1] * 1. + s_m0[slot * SB + t + 1]) * V;
-            if (avail_up < 0) avail_up = 0;
-            double mf = M;
-            double q = mf > 0 ? mf * 1. : mf * 1.;
-            if (q > avail) q = avail;
-            if (q < -avail_up) q = -avail_up;
-            WS(i, 1) = carry - q / V;
-            carry = 0. + q / V;
-            WS(i + 1, 2) = q;
-            avail = avail_up + q;
-          } else {
-            WS(i, 1) = carry;
-          }
-        }
-      }
-    }
-  }
-  // lane 0's rows become visible to the column's other lanes
-  __threadfence();
-  // ---- outputs, the remainder (book_remainder_), the RK combine -----------
-  double* s_out = s_buf;
-  for (int b0 = 0; b0 < n; b0 += kOB) {
-    __syncthreads();
-    stage_layers<1, kOWin>(s_mw, du, r.sv, IDN, s_row, b0 - 2, n);
-    __syncthreads();
-    auto M0 = [&](int i) { return s_mw[slot * kOWin + (i - b0 + 2)]; };
-    auto moved_rho = [&](int i) { return (M0(i) + WSB(i, 1)) - M0(i); };
-    auto raw_minus_mass0 = [&](int i) { return WSB(i, 5) - M0(i); };
-#pragma unroll 8
-    for (int t = 0; t < kOB; ++t) {
-      int i = b0 + t;
-      double out = 0.;
-      if (i < n) {
-        if (p == IDN) {
-          out = M0(i) + WSB(i, 1);
-          if (a.moved && L.live) a.moved[row + i] = failed ? 0. : WSB(i, 2);
-        } else if (p == IPR) {
-          out = ywb(i);
-          double req_lo = WSB(i, 0), req_hi = i + 1 < n ? WSB(i + 1, 0) : 0.;
-          double mv_lo = WSB(i, 2) - req_lo;
-          double mv_hi = (i + 1 < n ? WSB(i + 1, 2) : 0.) - req_hi;
-          double projected = (req_lo - req_hi) / V;
-          double pw = tab(vtab, V_PHI, i) * (WSB(i, 5) - M0(i) - projected);
-          double cw = -(tab(vtab, V_DPHI, i) * mv_hi - tab(vtab, V_DPLO, i) * mv_lo) / V;
-          out = out + (pw + cw);
-          // corrected-PE work of the moved mass minus the operator's part
-          int jj = (i < 1 ? 1 : (i > n - 2 ? n - 2 : i)) - 1;
-          double s = tab(xtab, T_WT0, i) * moved_rho(jj);
-          s = s + tab(xtab, T_WT1, i) * moved_rho(jj + 1);
-          s = s + tab(xtab, T_WT2, i) * moved_rho(jj + 2);
-          double work = tab(xtab, T_GVAR, i) * s;
-          double rs = tab(vtab, V_KMID, i) * raw_minus_mass0(i);
-          if (i > 0) rs = rs + tab(vtab, V_KLO, i) * raw_minus_mass0(i - 1);
-          if (i < n - 1) rs = rs + tab(vtab, V_KHI, i) * raw_minus_mass0(i + 1);
-          if (i == 0) rs = rs + a.t0 * raw_minus_mass0(2);
-          if (i == n - 1) rs = rs + a.t1 * raw_minus_mass0(n - 3);
-          work = work - rs;
-          out = out + work;
-        } else {
-          out = ywb(i);
-        }
-        // a failed column keeps its explicit increments (the reference
-        // undoes the column with its status bits and flags a retry)
-        if (failed) out = du[p * r.sv + row + i];
-      }
-      if (!L.shadow) s_out[(p * kSlots + slot) * kOB + t] = out;
-    }
-    __syncthreads();
-    // the RK combine u = w0 u0 + w1 u + w2 du' over the block, coalesced,
-    // in chunks whose loads are all issued before their stores
-    constexpr int TOT = 5 * kSlots * kOB;
-    constexpr int CH = 5;  // elements per lane per chunk
-    for (int f0 = 0; f0 < TOT; f0 += 64 * CH) {
-      double x0[CH], x1[CH];
-      int64_t oo[CH];
-#pragma unroll
-      for (int t = 0; t < CH; ++t) {
-        int f = f0 + threadIdx.x + 64 * t;
-        oo[t] = -1;
-        x0[t] = 0.;
-        x1[t] = 0.;
-        if (f < TOT) {
-          int v = f / (kSlots * kOB), h = f - v * (kSlots * kOB);
-          int sl = h / kOB, i = b0 + (h - sl * kOB);
-          if (i < n && s_live[sl]) {
-            oo[t] = v * r.sv + s_row[sl] + i;
-            if constexpr (PAIR) {
-              x0[t] = double(static_cast<float const*>(u0v)[oo[t]]) + double(c0[oo[t]]);
-              x1[t] = double(static_cast<float*>(uv)[oo[t]]) + double(cc[oo[t]]);
-            } else {
-              x0[t] = static_cast<double const*>(u0v)[oo[t]];
-              x1[t] = static_cast<double*>(uv)[oo[t]];
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < CH; ++t)
-        if (oo[t] >= 0) {
-          double x = a.w0 * x0[t] + a.w1 * x1[t] + a.w2 * s_out[f0 + threadIdx.x + 64 * t];
-          if constexpr (PAIR) {
-            float h = float(x);
-            static_cast<float*>(uv)[oo[t]] = h;
-            cc[oo[t]] = float(x - double(h));
-          } else {
-            static_cast<double*>(uv)[oo[t]] = x;
-          }
-        }
-    }
-  }
+  VicBackBody<PAIR>::run(block_index(), grid_size(), w, du, u0v, c0, uv, cc, vtab, xtab, ws, status,
+                         a);
 }
 
 }  // namespace
@@ -1095,17 +1124,17 @@ static int vic_launch(const snapy_grid* g, const snapy_physics* p,
                        vtab, x1tab, ws, status, a);
   else {
     unsigned nb = blocks_for(ncol, kSlots);
-    hipLaunchKernelGGL(vic_forward_kernel, dim3(nb), dim3(64), 0,
-                       as_stream(stream), w, du, vtab, ws, status, a);
+    launch<VicForwardBody, 64, 2>(vic_forward_kernel, dim3(nb), dim3(64), 0, as_stream(stream),
+                                  w, du, vtab, ws, status, a);
     if (int e = launch_status("vic_forward")) return e;
     if (pair)
-      hipLaunchKernelGGL(vic_back_kernel<true>, dim3(nb), dim3(64), 0,
-                         as_stream(stream), w, du, u0, c0, u, c, vtab, x1tab,
-                         ws, status, a);
+      launch<VicBackBody<true>, 64, 3>(vic_back_kernel<true>, dim3(nb), dim3(64), 0,
+                                       as_stream(stream), w, du, u0, c0, u, c, vtab, x1tab, ws,
+                                       status, a);
     else
-      hipLaunchKernelGGL(vic_back_kernel<false>, dim3(nb), dim3(64), 0,
-                         as_stream(stream), w, du, u0, c0, u, c, vtab, x1tab,
-                         ws, status, a);
+      launch<VicBackBody<false>, 64, 3>(vic_back_kernel<false>, dim3(nb), dim3(64), 0,
+                                        as_stream(stream), w, du, u0, c0, u, c, vtab, x1tab, ws,
+                                        status, a);
   }
   return launch_status("vic_full");
 }

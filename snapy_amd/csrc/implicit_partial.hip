@@ -29,6 +29,7 @@
 #include <cmath>
 
 #include "common.hpp"
+#include "group.hpp"
 #include "point.hpp"
 
 namespace snapy {
@@ -355,364 +356,375 @@ constexpr int kLds = 64 * kYS > kFwdLds ? 64 * kYS : kFwdLds;
 // The role is a scalar value, so each wavefront takes one branch, and both
 // branches pass the same barriers (1 + 2 n in the sweep, 2 per gather
 // flush).
+struct VicpForwardBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w,
+                                             double const* __restrict__ du,
+                                             double const* __restrict__ vtab,
+                                             double* __restrict__ ws, double* __restrict__ yc,
+                                             int* __restrict__ status, PartArgs a) {
+    __shared__ double s_buf[kLds];
+    __shared__ int64_t s_row[64];
+    __shared__ int s_act[64];
+    double* s_w = s_buf;  // [v][ll][cc]
+    double* s_du = s_buf + 5 * kLB * 64;
+    double* s_h = s_buf + 2 * 5 * kLB * 64;  // [e][cc]
+    Range3 const& r = a.r;
+    int ncol = r.n2 * r.n3;  // < 2^30 (checked by the launcher)
+    int const lane = threadIdx.x & 63;
+    bool const producer = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0;
+    int col0 = bid.x * 64;
+    int col = col0 + lane;
+    bool live = col < ncol;
+    int n = a.n, nc1 = r.nc1, lo1 = r.lo1;
+    double const A = a.area, V = a.vol;
+    double const dt = stage_dt(a, live ? col : 0);
+    auto REC = [&](int l) { return Rec{ws + int64_t(l) * kRec * ncol + col, ncol}; };
+    if (producer) {
+      // the first interior cell of each of the block's columns
+      // (column 0's for a lane past the grid, so its staging loads stay in
+      // bounds)
+      int cl = live ? col : 0;
+      int jj = r.lo2 + cl % r.n2, kk = r.lo3 + cl / r.n2;
+      s_row[lane] = kk * r.s3 + jj * r.s2 + r.lo1;
+    }
+    __syncthreads();
+
+    // the consumer's Woodbury inputs for the back substitution: row 0 of
+    // layer 2's solution for y and z0, and for z1 with its top-layer vector
+    double acc[2] = {0., 0.}, acc_z1 = 0., z1top[3] = {0., 0., 0.};
+    if (producer) {
+      // ---- producer: staging and assembly ------------------------------------
+      double const gm1 = a.gamma - 1.;
+      double const gme = 0.5 * (a.gamma + a.gamma) - 1.;
+      GasK const K{gme, 1. / gme, (gme + 1.) / gme, gme / (gme + 1.)};
+      double const gp_g = (gm1 + 1.) / gm1;
+      double const hiv = 0.5 / V;
+      auto tab = [&](int rowi, int l) { return vtab[rowi * nc1 + lo1 + l]; };
+      double d2 = 0., dn3 = 0.;
+      if (live) {
+        int64_t row = s_row[lane];
+        d2 = du[IDN * r.sv + row + 2];
+        dn3 = du[IDN * r.sv + row + n - 3];
+      }
+      // stage layers c0 + 1 .. c0 + kLB of w and du (the layer above each of
+      // the chunk's layers; the layer itself and the one below are carried in
+      // registers) for the 64 columns, lanes along x1 within a column
+      // (coalesced runs); only this wavefront reads the staging area.  Every
+      // load is unconditional and in bounds (the layer index clamped to the
+      // ghost layers of w and the interior of du; a column past the grid reads
+      // column 0), so a chunk's loads are in flight together; the out-of-range
+      // values are then replaced.
+      auto stage = [&](int c0) {
+        double xw[kLB][5], xd[kLB][5];
+#pragma unroll
+        for (int t = 0; t < kLB; ++t) {
+          int e = lane + 64 * t;
+          int cc = e / kLB, layer = c0 + 1 + e % kLB;
+          int64_t grow = s_row[cc];
+          int lw = layer > n ? n : layer;
+          int ld = layer > n - 1 ? n - 1 : layer;
+          for (int v = 0; v < 5; ++v) {
+            xw[t][v] = w[v * r.sv + grow + lw];
+            xd[t][v] = du[v * r.sv + grow + ld];
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int t = 0; t < kLB; ++t) {
+          int e = lane + 64 * t;
+          int cc = e / kLB, ll = e % kLB, layer = c0 + 1 + ll;
+          bool in = col0 + cc < ncol;
+          for (int v = 0; v < 5; ++v) {
+            s_w[(v * kLB + ll) * 64 + cc] = in && layer <= n ? xw[t][v] : 1.;
+            s_du[(v * kLB + ll) * 64 + cc] = in && layer < n ? xd[t][v] : 0.;
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      };
+      // carried from the layer below: the state and increments of cells i - 1
+      // and i, and the lower face's |A| rows (the Jacobian rows, cheap, are
+      // formed again each layer)
+      double wl[5], wc[5], dl[5], dmv[5], fa_lo[3][5], prl, prc;
+      {
+        int64_t row = s_row[lane];
+        for (int v = 0; v < 5; ++v) {
+          double x0 = w[v * r.sv + row - 1], x1 = w[v * r.sv + row], d0 = du[v * r.sv + row];
+          wl[v] = live ? x0 : 1.;
+          wc[v] = live ? x1 : 1.;
+          dl[v] = 0.;
+          dmv[v] = live ? d0 : 0.;
+        }
+        roe_rows3(fa_lo, K, wl, wc);
+        prl = wl[IPR] / wl[IDN];
+        prc = wc[IPR] / wc[IDN];
+      }
+      // assemble layer i from the carried cells and the staged layer above,
+      // and hand it to the consumer
+      auto assemble = [&](int i) {
+        int ll = i % kLB;
+        if (ll == 0) stage(i);
+        double wr[5], dr[5];
+        for (int v = 0; v < 5; ++v) {
+          wr[v] = s_w[(v * kLB + ll) * 64 + lane];
+          dr[v] = s_du[(v * kLB + ll) * 64 + lane];
+        }
+        double fa_hi[3][5], jp[3][5], jc[3][5], jn[3][5];
+        roe_rows3(fa_hi, K, wc, wr);
+        double prr = wr[IPR] / wr[IDN];
+        jac_rows(jp, gm1, gp_g, wl, prl);
+        jac_rows(jc, gm1, gp_g, wc, prc);
+        jac_rows(jn, gm1, gp_g, wr, prr);
+        CellTab ct;
+        ct.wlo = tab(V_WLO, i);
+        ct.whi = tab(V_WHI, i);
+        ct.klo = tab(V_KLO, i);
+        ct.kmid = tab(V_KMID, i);
+        ct.khi = tab(V_KHI, i);
+        ct.d2 = d2;
+        ct.dn3 = dn3;
+        double m[3][3], bb[3][3], c3[3][3], rhs[3];
+        assemble_cell(fa_lo, fa_hi, jp, jc, jn, wl, wc, wr, dl, dmv, dr, ct, i, n, A, hiv, a.grav,
+                      dt, a.t0, a.t1, m, bb, c3, rhs);
+        for (int v = 0; v < 5; ++v) {
+          wl[v] = wc[v];
+          wc[v] = wr[v];
+          dl[v] = dmv[v];
+          dmv[v] = dr[v];
+        }
+        prl = prc;
+        prc = prr;
+        for (int p = 0; p < 3; ++p)
+          for (int q = 0; q < 5; ++q) fa_lo[p][q] = fa_hi[p][q];
+        for (int p = 0; p < 3; ++p) {
+          for (int q = 0; q < 3; ++q) {
+            s_h[(3 * p + q) * 64 + lane] = m[p][q];
+            s_h[(9 + 3 * p + q) * 64 + lane] = bb[p][q];
+            s_h[(18 + 3 * p + q) * 64 + lane] = c3[p][q];
+          }
+          s_h[(27 + p) * 64 + lane] = rhs[p];
+        }
+      };
+      assemble(0);
+      __syncthreads();  // layer 0 handed off
+      for (int i = 0; i < n; ++i) {
+        __syncthreads();  // the consumer has taken layer i
+        if (i + 1 < n) assemble(i + 1);
+        __syncthreads();  // layer i + 1 handed off
+      }
+    } else {
+      // ---- consumer: the forward sweep ---------------------------------------
+      double ap[3][3], yp[3], z0p[3];
+      bool ok = live;
+      // row 0 of the back substitution's layer 2 for y, z0, z1, accumulated
+      // over the sweep: x_2 = sum_{j >= 2} r_j zeta_j with r_2 = e0 and
+      // r_{j+1} = -r_j A'_j (the recursion x_j = zeta_j - A'_j x_{j+1}
+      // unrolled), so the Woodbury rows are known when the back substitution
+      // starts
+      double rv[3] = {1., 0., 0.};
+      double acc_y = 0., acc_z0 = 0.;
+      __syncthreads();  // layer 0 handed off
+      for (int i = 0; i < n; ++i) {
+        double m[3][3], bb[3][3], c3[3][3], rhs[3];
+        for (int p = 0; p < 3; ++p) {
+          for (int q = 0; q < 3; ++q) {
+            m[p][q] = s_h[(3 * p + q) * 64 + lane];
+            bb[p][q] = s_h[(9 + 3 * p + q) * 64 + lane];
+            c3[p][q] = s_h[(18 + 3 * p + q) * 64 + lane];
+          }
+          rhs[p] = s_h[(27 + p) * 64 + lane];
+        }
+        __syncthreads();  // layer i taken
+        // the factorisation and the singularity test of the pivot block are
+        // independent work (the test only decides the column's status), so
+        // both run every layer; a failed column's values are never read
+        {
+          double e0[3] = {0., 0., i == 0 ? 1. : 0.};
+          if (i > 0)
+            for (int p = 0; p < 3; ++p)
+              for (int s = 0; s < 3; ++s)
+                m[p][s] -= bb[p][0] * ap[0][s] + bb[p][1] * ap[1][s] + bb[p][2] * ap[2][s];
+          bool piv = ludcmp3_ok(m);
+          double inv[3][3];
+          inverse3(m, inv);
+          double y[3], z0[3];
+          if (i > 0) {
+            double t[3], s0[3], by[3], b0[3];
+            mv3(bb, yp, by);
+            mv3(bb, z0p, b0);
+            for (int p = 0; p < 3; ++p) {
+              t[p] = rhs[p] - by[p];
+              s0[p] = e0[p] - b0[p];
+            }
+            mv3(inv, t, y);
+            mv3(inv, s0, z0);
+          } else {
+            mv3(inv, rhs, y);
+            mv3(inv, e0, z0);
+          }
+          for (int p = 0; p < 3; ++p)
+            for (int s = 0; s < 3; ++s)
+              ap[p][s] = inv[p][0] * c3[0][s] + inv[p][1] * c3[1][s] + inv[p][2] * c3[2][s];
+          bool fin = true;
+          for (int p = 0; p < 3; ++p) {
+            if (!isfinite(y[p])) fin = false;
+            for (int s = 0; s < 3; ++s)
+              if (!isfinite(ap[p][s])) fin = false;
+            yp[p] = y[p];
+            z0p[p] = z0[p];
+          }
+          if (i >= 2) {
+            acc_y += rv[0] * y[0] + rv[1] * y[1] + rv[2] * y[2];
+            acc_z0 += rv[0] * z0[0] + rv[1] * z0[1] + rv[2] * z0[2];
+            if (i == n - 1) {
+              // the second Woodbury column: right-hand side e2 in the top
+              // layer only, so its sweep vector is inv e2 there
+              for (int p = 0; p < 3; ++p) z1top[p] = inv[p][2];
+              acc_z1 = rv[0] * z1top[0] + rv[1] * z1top[1] + rv[2] * z1top[2];
+            }
+            double rn[3];
+            for (int s = 0; s < 3; ++s)
+              rn[s] = -(rv[0] * ap[0][s] + rv[1] * ap[1][s] + rv[2] * ap[2][s]);
+            for (int s = 0; s < 3; ++s) rv[s] = rn[s];
+          }
+          ok = ok && piv && fin;
+          if (ok) {
+            Rec rec = REC(i);
+            for (int p = 0; p < 3; ++p) {
+              for (int s = 0; s < 3; ++s) rec[R_A + 3 * p + s] = ap[p][s];
+              rec[R_Y + p] = y[p];
+              rec[R_Z0 + p] = z0[p];
+            }
+          }
+        }
+        __syncthreads();  // layer i + 1 handed off
+      }
+      acc[0] = acc_y;
+      acc[1] = acc_z0;
+      // a failed column keeps its explicit increments (the finish kernel
+      // reads the flag; the reference undoes the column with its status bits
+      // and flags a retry)
+      if (live) vic_mark(status, col, ok);
+      s_act[lane] = ok;
+    }
+
+    // ---- back substitution (vic_backward_substitute) + Woodbury -------------
+    // (vic_woodbury_column).  The top three layers are solved for the three
+    // columns y, z0, z1; row 0 of layer n - 3 and the swept row 0 of layer 2
+    // give the 2x2 and its coefficients al0, al1; every lower layer then
+    // solves the one combined column, zeta = zeta_y - al0 zeta_z0 (the z1
+    // vector is zero below the top layer), since the substitution is linear.
+    // The staging and hand-off areas become the gather (the last hand-off was
+    // read before the sweep's last barrier); both wavefronts run the loop, the
+    // consumer solves, both write the gathered layers.
+    bool act = !producer && live && s_act[lane];
+    double* s_y = s_buf;  // [cc][p * kYL + layer % kYL], row kYS
+    double ytop[3][3];    // the combined solution of layers n - 1, n - 2, n - 3
+    double xc[3] = {0., 0., 0.};
+    double al0 = 0., al1 = 0.;
+    for (int t = 0; t < 3; ++t)
+      for (int p = 0; p < 3; ++p) ytop[t][p] = 0.;
+    double nx[kRec];
+    for (int e = 0; e < kRec; ++e) nx[e] = 0.;
+    if (act) {
+      double xy[3][3], x0[3][3], x1[3][3];
+      for (int t = 0; t < 3; ++t) {
+        Rec rec = REC(n - 1 - t);
+        double cy[3], cz[3];
+        for (int p = 0; p < 3; ++p) {
+          cy[p] = rec[R_Y + p];
+          cz[p] = rec[R_Z0 + p];
+        }
+        if (t == 0) {
+          for (int p = 0; p < 3; ++p) {
+            xy[0][p] = cy[p];
+            x0[0][p] = cz[p];
+            x1[0][p] = z1top[p];
+          }
+        } else {
+          double A3[3][3];
+          for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) A3[p][q] = rec[R_A + 3 * p + q];
+          double sy[3], s0[3], s1[3];
+          mv3(A3, xy[t - 1], sy);
+          mv3(A3, x0[t - 1], s0);
+          mv3(A3, x1[t - 1], s1);
+          for (int p = 0; p < 3; ++p) {
+            xy[t][p] = cy[p] - sy[p];
+            x0[t][p] = cz[p] - s0[p];
+            x1[t][p] = 0. - s1[p];
+          }
+        }
+      }
+      // the record of layer n - 4, loaded while the 2x2 is solved
+      {
+        Rec rec = REC(n - 4);
+        for (int e = 0; e < kRec; ++e) nx[e] = rec[e];
+      }
+      double k0 = -a.t0 / dt, k1 = -a.t1 / dt;
+      double m00 = 1. + k0 * acc[1], m01 = k0 * acc_z1;
+      double m10 = k1 * x0[2][0], m11 = 1. + k1 * x1[2][0];
+      double r0 = k0 * acc[0], r1 = k1 * xy[2][0];
+      double det = m00 * m11 - m01 * m10;
+      al0 = (m11 * r0 - m01 * r1) / det;
+      al1 = (m00 * r1 - m10 * r0) / det;
+      for (int t = 0; t < 3; ++t)
+        for (int p = 0; p < 3; ++p) {
+          double y = xy[t][p];
+          y -= x0[t][p] * al0;
+          y -= x1[t][p] * al1;
+          ytop[t][p] = y;
+        }
+      for (int p = 0; p < 3; ++p) xc[p] = ytop[2][p];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      if (!producer) {
+        double yv[3];
+        if (i >= n - 3) {
+          int t = n - 1 - i;
+          for (int p = 0; p < 3; ++p) yv[p] = t == 0 ? ytop[0][p] : (t == 1 ? ytop[1][p] : ytop[2][p]);
+        } else {
+          double cur[kRec];
+          for (int e = 0; e < kRec; ++e) cur[e] = nx[e];
+          // the record of the layer below, loaded while this layer is solved
+          if (act && i > 0) {
+            Rec rec = REC(i - 1);
+            for (int e = 0; e < kRec; ++e) nx[e] = rec[e];
+          }
+          double A3[3][3];
+          for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) A3[p][q] = cur[R_A + 3 * p + q];
+          double sx[3];
+          mv3(A3, xc, sx);
+          for (int p = 0; p < 3; ++p) {
+            double z = cur[R_Y + p] - cur[R_Z0 + p] * al0;
+            xc[p] = z - sx[p];
+            yv[p] = xc[p];
+          }
+        }
+        double* o = s_y + lane * kYS + (i % kYL);
+        for (int p = 0; p < 3; ++p) o[p * kYL] = yv[p];
+      }
+      if (i % kYL == 0) {
+        // layers i .. i + cnt - 1 of the block's columns, lanes along x1
+        int cnt = n - i < kYL ? n - i : kYL;
+        __syncthreads();
+        for (int t = threadIdx.x; t < 64 * 3 * kYL; t += 128) {
+          int ll = t % kYL, pe = t / kYL, cc = pe / 3, e = pe % 3;
+          if (ll < cnt && col0 + cc < ncol && s_act[cc])
+            yc[(int64_t(col0 + cc) * 3 + e) * n + i + ll] = s_y[cc * kYS + e * kYL + ll];
+        }
+        __syncthreads();
+      }
+    }
+  }
+};
+
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
     vicp_forward_kernel(double const* __restrict__ w, double const* __restrict__ du,
                         double const* __restrict__ vtab, double* __restrict__ ws,
                         double* __restrict__ yc, int* __restrict__ status, PartArgs a) {
-  __shared__ double s_buf[kLds];
-  __shared__ int64_t s_row[64];
-  __shared__ int s_act[64];
-  double* s_w = s_buf;  // [v][ll][cc]
-  double* s_du = s_buf + 5 * kLB * 64;
-  double* s_h = s_buf + 2 * 5 * kLB * 64;  // [e][cc]
-  Range3 const& r = a.r;
-  int ncol = r.n2 * r.n3;  // < 2^30 (checked by the launcher)
-  int const lane = threadIdx.x & 63;
-  bool const producer = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0;
-  int col0 = blockIdx.x * 64;
-  int col = col0 + lane;
-  bool live = col < ncol;
-  int n = a.n, nc1 = r.nc1, lo1 = r.lo1;
-  double const A = a.area, V = a.vol;
-  double const dt = stage_dt(a, live ? col : 0);
-  auto REC = [&](int l) { return Rec{ws + int64_t(l) * kRec * ncol + col, ncol}; };
-  if (producer) {
-    // the first interior cell of each of the block's columns
-    // (column 0's for a lane past the grid, so its staging loads stay in
-    // bounds)
-    int cl = live ? col : 0;
-    int jj = r.lo2 + cl % r.n2, kk = r.lo3 + cl / r.n2;
-    s_row[lane] = kk * r.s3 + jj * r.s2 + r.lo1;
-  }
-  __syncthreads();
-
-  // the consumer's Woodbury inputs for the back substitution: row 0 of
-  // layer 2's solution for y and z0, and for z1 with its top-layer vector
-  double acc[2] = {0., 0.}, acc_z1 = 0., z1top[3] = {0., 0., 0.};
-  if (producer) {
-    // ---- producer: staging and assembly ------------------------------------
-    double const gm1 = a.gamma - 1.;
-    double const gme = 0.5 * (a.gamma + a.gamma) - 1.;
-    GasK const K{gme, 1. / gme, (gme + 1.) / gme, gme / (gme + 1.)};
-    double const gp_g = (gm1 + 1.) / gm1;
-    double const hiv = 0.5 / V;
-    auto tab = [&](int rowi, int l) { return vtab[rowi * nc1 + lo1 + l]; };
-    double d2 = 0., dn3 = 0.;
-    if (live) {
-      int64_t row = s_row[lane];
-      d2 = du[IDN * r.sv + row + 2];
-      dn3 = du[IDN * r.sv + row + n - 3];
-    }
-    // stage layers c0 + 1 .. c0 + kLB of w and du (the layer above each of
-    // the chunk's layers; the layer itself and the one below are carried in
-    // registers) for the 64 columns, lanes along x1 within a column
-    // (coalesced runs); only this wavefront reads the staging area.  Every
-    // load is unconditional and in bounds (the layer index clamped to the
-    // ghost layers of w and the interior of du; a column past the grid reads
-    // column 0), so a chunk's loads are in flight together; the out-of-range
-    // values are then replaced.
-    auto stage = [&](int c0) {
-      double xw[kLB][5], xd[kLB][5];
-#pragma unroll
-      for (int t = 0; t < kLB; ++t) {
-        int e = lane + 64 * t;
-        int cc = e / kLB, layer = c0 + 1 + e % kLB;
-        int64_t grow = s_row[cc];
-        int lw = layer > n ? n : layer;
-        int ld = layer > n - 1 ? n - 1 : layer;
-        for (int v = 0; v < 5; ++v) {
-          xw[t][v] = w[v * r.sv + grow + lw];
-          xd[t][v] = du[v * r.sv + grow + ld];
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-#pragma unroll
-      for (int t = 0; t < kLB; ++t) {
-        int e = lane + 64 * t;
-        int cc = e / kLB, ll = e % kLB, layer = c0 + 1 + ll;
-        bool in = col0 + cc < ncol;
-        for (int v = 0; v < 5; ++v) {
-          s_w[(v * kLB + ll) * 64 + cc] = in && layer <= n ? xw[t][v] : 1.;
-          s_du[(v * kLB + ll) * 64 + cc] = in && layer < n ? xd[t][v] : 0.;
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-    };
-    // carried from the layer below: the state and increments of cells i - 1
-    // and i, and the lower face's |A| rows (the Jacobian rows, cheap, are
-    // formed again each layer)
-    double wl[5], wc[5], dl[5], dmv[5], fa_lo[3][5], prl, prc;
-    {
-      int64_t row = s_row[lane];
-      for (int v = 0; v < 5; ++v) {
-        double x0 = w[v * r.sv + row - 1], x1 = w[v * r.sv + row], d0 = du[v * r.sv + row];
-        wl[v] = live ? x0 : 1.;
-        wc[v] = live ? x1 : 1.;
-        dl[v] = 0.;
-        dmv[v] = live ? d0 : 0.;
-      }
-      roe_rows3(fa_lo, K, wl, wc);
-      prl = wl[IPR] / wl[IDN];
-      prc = wc[IPR] / wc[IDN];
-    }
-    // assemble layer i from the carried cells and the staged layer above,
-    // and hand it to the consumer
-    auto assemble = [&](int i) {
-      int ll = i % kLB;
-      if (ll == 0) stage(i);
-      double wr[5], dr[5];
-      for (int v = 0; v < 5; ++v) {
-        wr[v] = s_w[(v * kLB + ll) * 64 + lane];
-        dr[v] = s_du[(v * kLB + ll) * 64 + lane];
-      }
-      double fa_hi[3][5], jp[3][5], jc[3][5], jn[3][5];
-      roe_rows3(fa_hi, K, wc, wr);
-      double prr = wr[IPR] / wr[IDN];
-      jac_rows(jp, gm1, gp_g, wl, prl);
-      jac_rows(jc, gm1, gp_g, wc, prc);
-      jac_rows(jn, gm1, gp_g, wr, prr);
-      CellTab ct;
-      ct.wlo = tab(V_WLO, i);
-      ct.whi = tab(V_WHI, i);
-      ct.klo = tab(V_KLO, i);
-      ct.kmid = tab(V_KMID, i);
-      ct.khi = tab(V_KHI, i);
-      ct.d2 = d2;
-      ct.dn3 = dn3;
-      double m[3][3], bb[3][3], c3[3][3], rhs[3];
-      assemble_cell(fa_lo, fa_hi, jp, jc, jn, wl, wc, wr, dl, dmv, dr, ct, i, n, A, hiv, a.grav,
-                    dt, a.t0, a.t1, m, bb, c3, rhs);
-      for (int v = 0; v < 5; ++v) {
-        wl[v] = wc[v];
-        wc[v] = wr[v];
-        dl[v] = dmv[v];
-        dmv[v] = dr[v];
-      }
-      prl = prc;
-      prc = prr;
-      for (int p = 0; p < 3; ++p)
-        for (int q = 0; q < 5; ++q) fa_lo[p][q] = fa_hi[p][q];
-      for (int p = 0; p < 3; ++p) {
-        for (int q = 0; q < 3; ++q) {
-          s_h[(3 * p + q) * 64 + lane] = m[p][q];
-          s_h[(9 + 3 * p + q) * 64 + lane] = bb[p][q];
-          s_h[(18 + 3 * p + q) * 64 + lane] = c3[p][q];
-        }
-        s_h[(27 + p) * 64 + lane] = rhs[p];
-      }
-    };
-    assemble(0);
-    __syncthreads();  // layer 0 handed off
-    for (int i = 0; i < n; ++i) {
-      __syncthreads();  // the consumer has taken layer i
-      if (i + 1 < n) assemble(i + 1);
-      __syncthreads();  // layer i + 1 handed off
-    }
-  } else {
-    // ---- consumer: the forward sweep ---------------------------------------
-    double ap[3][3], yp[3], z0p[3];
-    bool ok = live;
-    // row 0 of the back substitution's layer 2 for y, z0, z1, accumulated
-    // over the sweep: x_2 = sum_{j >= 2} r_j zeta_j with r_2 = e0 and
-    // r_{j+1} = -r_j A'_j (the recursion x_j = zeta_j - A'_j x_{j+1}
-    // unrolled), so the Woodbury rows are known when the back substitution
-    // starts
-    double rv[3] = {1., 0., 0.};
-    double acc_y = 0., acc_z0 = 0.;
-    __syncthreads();  // layer 0 handed off
-    for (int i = 0; i < n; ++i) {
-      double m[3][3], bb[3][3], c3[3][3], rhs[3];
-      for (int p = 0; p < 3; ++p) {
-        for (int q = 0; q < 3; ++q) {
-          m[p][q] = s_h[(3 * p + q) * 64 + lane];
-          bb[p][q] = s_h[(9 + 3 * p + q) * 64 + lane];
-          c3[p][q] = s_h[(18 + 3 * p + q) * 64 + lane];
-        }
-        rhs[p] = s_h[(27 + p) * 64 + lane];
-      }
-      __syncthreads();  // layer i taken
-      // the factorisation and the singularity test of the pivot block are
-      // independent work (the test only decides the column's status), so
-      // both run every layer; a failed column's values are never read
-      {
-        double e0[3] = {0., 0., i == 0 ? 1. : 0.};
-        if (i > 0)
-          for (int p = 0; p < 3; ++p)
-            for (int s = 0; s < 3; ++s)
-              m[p][s] -= bb[p][0] * ap[0][s] + bb[p][1] * ap[1][s] + bb[p][2] * ap[2][s];
-        bool piv = ludcmp3_ok(m);
-        double inv[3][3];
-        inverse3(m, inv);
-        double y[3], z0[3];
-        if (i > 0) {
-          double t[3], s0[3], by[3], b0[3];
-          mv3(bb, yp, by);
-          mv3(bb, z0p, b0);
-          for (int p = 0; p < 3; ++p) {
-            t[p] = rhs[p] - by[p];
-            s0[p] = e0[p] - b0[p];
-          }
-          mv3(inv, t, y);
-          mv3(inv, s0, z0);
-        } else {
-          mv3(inv, rhs, y);
-          mv3(inv, e0, z0);
-        }
-        for (int p = 0; p < 3; ++p)
-          for (int s = 0; s < 3; ++s)
-            ap[p][s] = inv[p][0] * c3[0][s] + inv[p][1] * c3[1][s] + inv[p][2] * c3[2][s];
-        bool fin = true;
-        for (int p = 0; p < 3; ++p) {
-          if (!isfinite(y[p])) fin = false;
-          for (int s = 0; s < 3; ++s)
-            if (!isfinite(ap[p][s])) fin = false;
-          yp[p] = y[p];
-          z0p[p] = z0[p];
-        }
-        if (i >= 2) {
-          acc_y += rv[0] * y[0] + rv[1] * y[1] + rv[2] * y[2];
-          acc_z0 += rv[0] * z0[0] + rv[1] * z0[1] + rv[2] * z0[2];
-          if (i == n - 1) {
-            // the second Woodbury column: right-hand side e2 in the top
-            // layer only, so its sweep vector is inv e2 there
-            for (int p = 0; p < 3; ++p) z1top[p] = inv[p][2];
-            acc_z1 = rv[0] * z1top[0] + rv[1] * z1top[1] + rv[2] * z1top[2];
-          }
-          double rn[3];
-          for (int s = 0; s < 3; ++s)
-            rn[s] = -(rv[0] * ap[0][s] + rv[1] * ap[1][s] + rv[2] * ap[2][s]);
-          for (int s = 0; s < 3; ++s) rv[s] = rn[s];
-        }
-        ok = ok && piv && fin;
-        if (ok) {
-          Rec rec = REC(i);
-          for (int p = 0; p < 3; ++p) {
-            for (int s = 0; s < 3; ++s) rec[R_A + 3 * p + s] = ap[p][s];
-            rec[R_Y + p] = y[p];
-            rec[R_Z0 + p] = z0[p];
-          }
-        }
-      }
-      __syncthreads();  // layer i + 1 handed off
-    }
-    acc[0] = acc_y;
-    acc[1] = acc_z0;
-    // a failed column keeps its explicit increments (the finish kernel
-    // reads the flag; the reference undoes the column with its status bits
-    // and flags a retry)
-    if (live) vic_mark(status, col, ok);
-    s_act[lane] = ok;
-  }
-
-  // ---- back substitution (vic_backward_substitute) + Woodbury -------------
-  // (vic_woodbury_column).  The top three layers are solved for the three
-  // columns y, z0, z1; row 0 of layer n - 3 and the swept row 0 of layer 2
-  // give the 2x2 and its coefficients al0, al1; every lower layer then
-  // solves the one combined column, zeta = zeta_y - al0 zeta_z0 (the z1
-  // vector is zero below the top layer), since the substitution is linear.
-  // The staging and hand-off areas become the gather (the last hand-off was
-  // read before the sweep's last barrier); both wavefronts run the loop, the
-  // consumer solves, both write the gathered layers.
-  bool act = !producer && live && s_act[lane];
-  double* s_y = s_buf;  // [cc][p * kYL + layer % kYL], row kYS
-  double ytop[3][3];    // the combined solution of layers n - 1, n - 2, n - 3
-  double xc[3] = {0., 0., 0.};
-  double al0 = 0., al1 = 0.;
-  for (int t = 0; t < 3; ++t)
-    for (int p = 0; p < 3; ++p) ytop[t][p] = 0.;
-  double nx[kRec];
-  for (int e = 0; e < kRec; ++e) nx[e] = 0.;
-  if (act) {
-    double xy[3][3], x0[3][3], x1[3][3];
-    for (int t = 0; t < 3; ++t) {
-      Rec rec = REC(n - 1 - t);
-      double cy[3], cz[3];
-      for (int p = 0; p < 3; ++p) {
-        cy[p] = rec[R_Y + p];
-        cz[p] = rec[R_Z0 + p];
-      }
-      if (t == 0) {
-        for (int p = 0; p < 3; ++p) {
-          xy[0][p] = cy[p];
-          x0[0][p] = cz[p];
-          x1[0][p] = z1top[p];
-        }
-      } else {
-        double A3[3][3];
-        for (int p = 0; p < 3; ++p)
-          for (int q = 0; q < 3; ++q) A3[p][q] = rec[R_A + 3 * p + q];
-        double sy[3], s0[3], s1[3];
-        mv3(A3, xy[t - 1], sy);
-        mv3(A3, x0[t - 1], s0);
-        mv3(A3, x1[t - 1], s1);
-        for (int p = 0; p < 3; ++p) {
-          xy[t][p] = cy[p] - sy[p];
-          x0[t][p] = cz[p] - s0[p];
-          x1[t][p] = 0. - s1[p];
-        }
-      }
-    }
-    // the record of layer n - 4, loaded while the 2x2 is solved
-    {
-      Rec rec = REC(n - 4);
-      for (int e = 0; e < kRec; ++e) nx[e] = rec[e];
-    }
-    double k0 = -a.t0 / dt, k1 = -a.t1 / dt;
-    double m00 = 1. + k0 * acc[1], m01 = k0 * acc_z1;
-    double m10 = k1 * x0[2][0], m11 = 1. + k1 * x1[2][0];
-    double r0 = k0 * acc[0], r1 = k1 * xy[2][0];
-    double det = m00 * m11 - m01 * m10;
-    al0 = (m11 * r0 - m01 * r1) / det;
-    al1 = (m00 * r1 - m10 * r0) / det;
-    for (int t = 0; t < 3; ++t)
-      for (int p = 0; p < 3; ++p) {
-        double y = xy[t][p];
-        y -= x0[t][p] * al0;
-        y -= x1[t][p] * al1;
-        ytop[t][p] = y;
-      }
-    for (int p = 0; p < 3; ++p) xc[p] = ytop[2][p];
-  }
-  for (int i = n - 1; i >= 0; --i) {
-    if (!producer) {
-      double yv[3];
-      if (i >= n - 3) {
-        int t = n - 1 - i;
-        for (int p = 0; p < 3; ++p) yv[p] = t == 0 ? ytop[0][p] : (t == 1 ? ytop[1][p] : ytop[2][p]);
-      } else {
-        double cur[kRec];
-        for (int e = 0; e < kRec; ++e) cur[e] = nx[e];
-        // the record of the layer below, loaded while this layer is solved
-        if (act && i > 0) {
-          Rec rec = REC(i - 1);
-          for (int e = 0; e < kRec; ++e) nx[e] = rec[e];
-        }
-        double A3[3][3];
-        for (int p = 0; p < 3; ++p)
-          for (int q = 0; q < 3; ++q) A3[p][q] = cur[R_A + 3 * p + q];
-        double sx[3];
-        mv3(A3, xc, sx);
-        for (int p = 0; p < 3; ++p) {
-          double z = cur[R_Y + p] - cur[R_Z0 + p] * al0;
-          xc[p] = z - sx[p];
-          yv[p] = xc[p];
-        }
-      }
-      double* o = s_y + lane * kYS + (i % kYL);
-      for (int p = 0; p < 3; ++p) o[p * kYL] = yv[p];
-    }
-    if (i % kYL == 0) {
-      // layers i .. i + cnt - 1 of the block's columns, lanes along x1
-      int cnt = n - i < kYL ? n - i : kYL;
-      __syncthreads();
-      for (int t = threadIdx.x; t < 64 * 3 * kYL; t += 128) {
-        int ll = t % kYL, pe = t / kYL, cc = pe / 3, e = pe % 3;
-        if (ll < cnt && col0 + cc < ncol && s_act[cc])
-          yc[(int64_t(col0 + cc) * 3 + e) * n + i + ll] = s_y[cc * kYS + e * kYL + ll];
-      }
-      __syncthreads();
-    }
-  }
+  VicpForwardBody::run(block_index(), grid_size(), w, du, vtab, ws, yc, status, a);
 }
 
 // the rest of the column program for kFinCols columns per workgroup
@@ -727,6 +739,204 @@ constexpr int kFinCols = 8;
 constexpr int kFinThreads = 256;
 
 template <bool PAIR>
+struct VicpFinishBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w,
+                                             double const* __restrict__ du,
+                                             void const* __restrict__ u0v,
+                                             float const* __restrict__ c0, void* __restrict__ uv,
+                                             float* __restrict__ cc,
+                                             double const* __restrict__ vtab,
+                                             double const* __restrict__ xtab,
+                                             double const* __restrict__ yc,
+                                             int const* __restrict__ status, PartArgs a, int nc) {
+    extern __shared__ double lds[];
+    __shared__ int64_t s_row[kFinCols];
+    __shared__ int s_in[kFinCols], s_ok[kFinCols];
+    Range3 const& r = a.r;
+    int64_t ncol = int64_t(r.n2) * r.n3;
+    int64_t col0 = int64_t(bid.x) * nc;
+    int const tid = threadIdx.x;
+    int n = a.n, nc1 = r.nc1, lo1 = r.lo1, L = n + 1;
+    double const V = a.vol;
+    // per column c: phi_i, then M_i (the requested transfer through the face
+    // below cell i); (w_rho V)_i; (w_rho + du_rho)_i V, then q_i (the moved
+    // transfer through the face below cell i)
+    double* s_A = lds;
+    double* s_B = lds + nc * L;
+    double* s_C = lds + 2 * nc * L;
+    if (tid < nc) {
+      int64_t col = col0 + tid;
+      bool in = col < ncol;
+      int64_t cl = in ? col : 0;
+      int j = r.lo2 + int(cl % r.n2), k = r.lo3 + int(cl / r.n2);
+      s_row[tid] = k * r.s3 + j * r.s2 + r.lo1;
+      s_in[tid] = in;
+      s_ok[tid] = in && (status[col] & kVicStageFailed) == 0;
+    }
+    __syncthreads();
+    int const cells = nc * n;
+    for (int e = tid; e < cells; e += kFinThreads) {
+      int c = e / n, l = e - c * n;
+      if (!s_ok[c]) continue;
+      int64_t row = s_row[c];
+      double y0 = yc[((col0 + c) * 3) * n + l];
+      double d0 = du[IDN * r.sv + row + l], wr = w[IDN * r.sv + row + l];
+      s_A[c * L + l] = (y0 - d0) * V;
+      s_B[c * L + l] = wr * V;
+      s_C[c * L + l] = (wr * 1. + d0) * V;
+    }
+    __syncthreads();
+    // ---- the serial chains of vic_constituent_column, one thread per column,
+    // in the column's order: the column sums R and S, then, face by face, the
+    // requested transfer M (the prefix of phi_i - R (w_rho V)_i / S) and the
+    // moved transfer q clamped by what the cells on either side hold
+    if (tid < nc && s_ok[tid]) {
+      double* A_ = s_A + tid * L;
+      double* B_ = s_B + tid * L;
+      double* C_ = s_C + tid * L;
+      constexpr int kB = 8;
+      double R = 0., S = 0.;
+      int i0 = 0;
+      for (; i0 + kB <= n; i0 += kB) {
+        double ta[kB], tb[kB];
+#pragma unroll
+        for (int t = 0; t < kB; ++t) {
+          ta[t] = A_[i0 + t];
+          tb[t] = B_[i0 + t];
+        }
+#pragma unroll
+        for (int t = 0; t < kB; ++t) {
+          R += ta[t];
+          S += tb[t];
+        }
+      }
+      for (; i0 < n; ++i0) {
+        R += A_[i0];
+        S += B_[i0];
+      }
+      double M = 0.;
+      double avail = C_[0];
+      if (avail < 0) avail = 0;
+      C_[0] = 0.;
+      // one face: M_i stored, M_{i+1} formed, then the face above cell i
+      // (i + 1 < n) clamped
+      auto face = [&](int i, double phi, double wv, double av_up) {
+        A_[i] = M;
+        double wvp = phi - R * (wv / S);
+        M -= wvp;
+        if (i + 1 < n) {
+          double avail_up = av_up;
+          if (avail_up < 0) avail_up = 0;
+          double mf = M;
+          double q = mf > 0 ? mf * 1. : mf * 1.;
+          if (q > avail) q = avail;
+          if (q < -avail_up) q = -avail_up;
+          C_[i + 1] = q;
+          avail = avail_up + q;
+        }
+      };
+      i0 = 0;
+      for (; i0 + kB < n; i0 += kB) {
+        double ta[kB], tb[kB], tc[kB];
+#pragma unroll
+        for (int t = 0; t < kB; ++t) {
+          ta[t] = A_[i0 + t];
+          tb[t] = B_[i0 + t];
+          tc[t] = C_[i0 + t + 1];
+        }
+#pragma unroll
+        for (int t = 0; t < kB; ++t) face(i0 + t, ta[t], tb[t], tc[t]);
+      }
+      for (; i0 < n; ++i0) face(i0, A_[i0], B_[i0], i0 + 1 < n ? C_[i0 + 1] : 0.);
+      A_[n] = 0.;
+      C_[n] = 0.;
+    }
+    __syncthreads();
+    // ---- outputs, the tangential transfers, remainder, RK combine ------------
+    for (int e = tid; e < cells; e += kFinThreads) {
+      int c = e / n, i = e - c * n;
+      if (!s_in[c]) continue;
+      int64_t row = s_row[c];
+      auto W = [&](int v, int l) { return w[v * r.sv + row + l]; };
+      auto DU = [&](int v, int l) { return du[v * r.sv + row + l]; };
+      auto tab = [&](double const* t, int rowi, int l) { return t[rowi * nc1 + lo1 + l]; };
+      auto combine = [&](int v, double x) {
+        int64_t o = v * r.sv + row + i;
+        if constexpr (PAIR) {
+          double y = a.w0 * (double(static_cast<float const*>(u0v)[o]) + double(c0[o])) +
+                     a.w1 * (double(static_cast<float*>(uv)[o]) + double(cc[o])) + a.w2 * x;
+          float h = float(y);
+          static_cast<float*>(uv)[o] = h;
+          cc[o] = float(y - double(h));
+        } else {
+          double* u = static_cast<double*>(uv);
+          u[o] = a.w0 * static_cast<double const*>(u0v)[o] + a.w1 * u[o] + a.w2 * x;
+        }
+      };
+      if (a.moved) a.moved[row + i] = s_ok[c] ? s_C[c * L + i] : 0.;
+      if (!s_ok[c]) {
+        for (int v = 0; v < 5; ++v) combine(v, DU(v, i));
+        continue;
+      }
+      double const* ycol = yc + ((col0 + c) * 3) * n;
+      double const* s_M = s_A + c * L;
+      double const* s_q = s_C + c * L;
+      // the mass change of cell l: the face below adds, the face above removes
+      auto dm = [&](int l) { return (0. + s_q[l] / V) - s_q[l + 1] / V; };
+      auto moved_rho = [&](int l) {
+        double d0 = DU(IDN, l);
+        return (d0 + dm(l)) - d0;
+      };
+      auto raw_minus_mass0 = [&](int l) { return ycol[l] - DU(IDN, l); };
+      auto rows_at = [&](int l, auto q) {
+        double s = tab(vtab, V_KMID, l) * q(l);
+        if (l > 0) s = s + tab(vtab, V_KLO, l) * q(l - 1);
+        if (l < n - 1) s = s + tab(vtab, V_KHI, l) * q(l + 1);
+        if (l == 0) s = s + a.t0 * q(2);
+        if (l == n - 1) s = s + a.t1 * q(n - 3);
+        return s;
+      };
+      double out[5];
+      double y0 = ycol[i], d0 = DU(IDN, i);
+      out[IDN] = d0 + dm(i);
+      out[IVX] = ycol[n + i];
+      out[IPR] = ycol[2 * n + i];
+      // vic_tangential_momentum_column: the face below adds (its upper cell),
+      // then the face above removes (its lower cell), in the column's order
+      double q_lo = s_q[i], q_hi = s_q[i + 1];
+      for (int m = IVY; m <= IVZ; ++m) {
+        double x = DU(m, i);
+        if (i > 0) {
+          double pv = q_lo * W(m, q_lo > 0 ? i - 1 : i);
+          if (pv != 0.) x = x + pv / V;
+        }
+        if (i + 1 < n) {
+          double pv = q_hi * W(m, q_hi > 0 ? i : i + 1);
+          if (pv != 0.) x = x - pv / V;
+        }
+        out[m] = x;
+      }
+      double req_lo = s_M[i], req_hi = s_M[i + 1];
+      double mv_lo = s_q[i] - req_lo;
+      double mv_hi = s_q[i + 1] - req_hi;
+      double projected = (req_lo - req_hi) / V;
+      double pw = tab(vtab, V_PHI, i) * (y0 - d0 - projected);
+      double cw = -(tab(vtab, V_DPHI, i) * mv_hi - tab(vtab, V_DPLO, i) * mv_lo) / V;
+      out[IPR] = out[IPR] + (pw + cw);
+      int jj = (i < 1 ? 1 : (i > n - 2 ? n - 2 : i)) - 1;
+      double s = tab(xtab, T_WT0, i) * moved_rho(jj);
+      s = s + tab(xtab, T_WT1, i) * moved_rho(jj + 1);
+      s = s + tab(xtab, T_WT2, i) * moved_rho(jj + 2);
+      double work = tab(xtab, T_GVAR, i) * s;
+      work = work - rows_at(i, raw_minus_mass0);
+      out[IPR] = out[IPR] + work;
+      for (int v = 0; v < 5; ++v) combine(v, out[v]);
+    }
+  }
+};
+
+template <bool PAIR>
 __global__ void __launch_bounds__(kFinThreads)
     vicp_finish_kernel(double const* __restrict__ w, double const* __restrict__ du,
                        void const* __restrict__ u0v, float const* __restrict__ c0,
@@ -734,189 +944,8 @@ __global__ void __launch_bounds__(kFinThreads)
                        double const* __restrict__ vtab, double const* __restrict__ xtab,
                        double const* __restrict__ yc, int const* __restrict__ status, PartArgs a,
                        int nc) {
-  extern __shared__ double lds[];
-  __shared__ int64_t s_row[kFinCols];
-  __shared__ int s_in[kFinCols], s_ok[kFinCols];
-  Range3 const& r = a.r;
-  int64_t ncol = int64_t(r.n2) * r.n3;
-  int64_t col0 = int64_t(blockIdx.x) * nc;
-  int const tid = threadIdx.x;
-  int n = a.n, nc1 = r.nc1, lo1 = r.lo1, L = n + 1;
-  double const V = a.vol;
-  // per column c: phi_i, then M_i (the requested transfer through the face
-  // below cell i); (w_rho V)_i; (w_rho + du_rho)_i V, then q_i (the moved
-  // transfer through the face below cell i)
-  double* s_A = lds;
-  double* s_B = lds + nc * L;
-  double* s_C = lds + 2 * nc * L;
-  if (tid < nc) {
-    int64_t col = col0 + tid;
-    bool in = col < ncol;
-    int64_t cl = in ? col : 0;
-    int j = r.lo2 + int(cl % r.n2), k = r.lo3 + int(cl / r.n2);
-    s_row[tid] = k * r.s3 + j * r.s2 + r.lo1;
-    s_in[tid] = in;
-    s_ok[tid] = in && (status[col] & kVicStageFailed) == 0;
-  }
-  __syncthreads();
-  int const cells = nc * n;
-  for (int e = tid; e < cells; e += kFinThreads) {
-    int c = e / n, l = e - c * n;
-    if (!s_ok[c]) continue;
-    int64_t row = s_row[c];
-    double y0 = yc[((col0 + c) * 3) * n + l];
-    double d0 = du[IDN * r.sv + row + l], wr = w[IDN * r.sv + row + l];
-    s_A[c * L + l] = (y0 - d0) * V;
-    s_B[c * L + l] = wr * V;
-    s_C[c * L + l] = (wr * 1. + d0) * V;
-  }
-  __syncthreads();
-  // ---- the serial chains of vic_constituent_column, one thread per column,
-  // in the column's order: the column sums R and S, then, face by face, the
-  // requested transfer M (the prefix of phi_i - R (w_rho V)_i / S) and the
-  // moved transfer q clamped by what the cells on either side hold
-  if (tid < nc && s_ok[tid]) {
-    double* A_ = s_A + tid * L;
-    double* B_ = s_B + tid * L;
-    double* C_ = s_C + tid * L;
-    constexpr int kB = 8;
-    double R = 0., S = 0.;
-    int i0 = 0;
-    for (; i0 + kB <= n; i0 += kB) {
-      double ta[kB], tb[kB];
-#pragma unroll
-      for (int t = 0; t < kB; ++t) {
-        ta[t] = A_[i0 + t];
-        tb[t] = B_[i0 + t];
-      }
-#pragma unroll
-      for (int t = 0; t < kB; ++t) {
-        R += ta[t];
-        S += tb[t];
-      }
-    }
-    for (; i0 < n; ++i0) {
-      R += A_[i0];
-      S += B_[i0];
-    }
-    double M = 0.;
-    double avail = C_[0];
-    if (avail < 0) avail = 0;
-    C_[0] = 0.;
-    // one face: M_i stored, M_{i+1} formed, then the face above cell i
-    // (i + 1 < n) clamped
-    auto face = [&](int i, double phi, double wv, double av_up) {
-      A_[i] = M;
-      double wvp = phi - R * (wv / S);
-      M -= wvp;
-      if (i + 1 < n) {
-        double avail_up = av_up;
-        if (avail_up < 0) avail_up = 0;
-        double mf = M;
-        double q = mf > 0 ? mf * 1. : mf * 1.;
-        if (q > avail) q = avail;
-        if (q < -avail_up) q = -avail_up;
-        C_[i + 1] = q;
-        avail = avail_up + q;
-      }
-    };
-    i0 = 0;
-    for (; i0 + kB < n; i0 += kB) {
-      double ta[kB], tb[kB], tc[kB];
-#pragma unroll
-      for (int t = 0; t < kB; ++t) {
-        ta[t] = A_[i0 + t];
-        tb[t] = B_[i0 + t];
-        tc[t] = C_[i0 + t + 1];
-      }
-#pragma unroll
-      for (int t = 0; t < kB; ++t) face(i0 + t, ta[t], tb[t], tc[t]);
-    }
-    for (; i0 < n; ++i0) face(i0, A_[i0], B_[i0], i0 + 1 < n ? C_[i0 + 1] : 0.);
-    A_[n] = 0.;
-    C_[n] = 0.;
-  }
-  __syncthreads();
-  // ---- outputs, the tangential transfers, remainder, RK combine ------------
-  for (int e = tid; e < cells; e += kFinThreads) {
-    int c = e / n, i = e - c * n;
-    if (!s_in[c]) continue;
-    int64_t row = s_row[c];
-    auto W = [&](int v, int l) { return w[v * r.sv + row + l]; };
-    auto DU = [&](int v, int l) { return du[v * r.sv + row + l]; };
-    auto tab = [&](double const* t, int rowi, int l) { return t[rowi * nc1 + lo1 + l]; };
-    auto combine = [&](int v, double x) {
-      int64_t o = v * r.sv + row + i;
-      if constexpr (PAIR) {
-        double y = a.w0 * (double(static_cast<float const*>(u0v)[o]) + double(c0[o])) +
-                   a.w1 * (double(static_cast<float*>(uv)[o]) + double(cc[o])) + a.w2 * x;
-        float h = float(y);
-        static_cast<float*>(uv)[o] = h;
-        cc[o] = float(y - double(h));
-      } else {
-        double* u = static_cast<double*>(uv);
-        u[o] = a.w0 * static_cast<double const*>(u0v)[o] + a.w1 * u[o] + a.w2 * x;
-      }
-    };
-    if (a.moved) a.moved[row + i] = s_ok[c] ? s_C[c * L + i] : 0.;
-    if (!s_ok[c]) {
-      for (int v = 0; v < 5; ++v) combine(v, DU(v, i));
-      continue;
-    }
-    double const* ycol = yc + ((col0 + c) * 3) * n;
-    double const* s_M = s_A + c * L;
-    double const* s_q = s_C + c * L;
-    // the mass change of cell l: the face below adds, the face above removes
-    auto dm = [&](int l) { return (0. + s_q[l] / V) - s_q[l + 1] / V; };
-    auto moved_rho = [&](int l) {
-      double d0 = DU(IDN, l);
-      return (d0 + dm(l)) - d0;
-    };
-    auto raw_minus_mass0 = [&](int l) { return ycol[l] - DU(IDN, l); };
-    auto rows_at = [&](int l, auto q) {
-      double s = tab(vtab, V_KMID, l) * q(l);
-      if (l > 0) s = s + tab(vtab, V_KLO, l) * q(l - 1);
-      if (l < n - 1) s = s + tab(vtab, V_KHI, l) * q(l + 1);
-      if (l == 0) s = s + a.t0 * q(2);
-      if (l == n - 1) s = s + a.t1 * q(n - 3);
-      return s;
-    };
-    double out[5];
-    double y0 = ycol[i], d0 = DU(IDN, i);
-    out[IDN] = d0 + dm(i);
-    out[IVX] = ycol[n + i];
-    out[IPR] = ycol[2 * n + i];
-    // vic_tangential_momentum_column: the face below adds (its upper cell),
-    // then the face above removes (its lower cell), in the column's order
-    double q_lo = s_q[i], q_hi = s_q[i + 1];
-    for (int m = IVY; m <= IVZ; ++m) {
-      double x = DU(m, i);
-      if (i > 0) {
-        double pv = q_lo * W(m, q_lo > 0 ? i - 1 : i);
-        if (pv != 0.) x = x + pv / V;
-      }
-      if (i + 1 < n) {
-        double pv = q_hi * W(m, q_hi > 0 ? i : i + 1);
-        if (pv != 0.) x = x - pv / V;
-      }
-      out[m] = x;
-    }
-    double req_lo = s_M[i], req_hi = s_M[i + 1];
-    double mv_lo = s_q[i] - req_lo;
-    double mv_hi = s_q[i + 1] - req_hi;
-    double projected = (req_lo - req_hi) / V;
-    double pw = tab(vtab, V_PHI, i) * (y0 - d0 - projected);
-    double cw = -(tab(vtab, V_DPHI, i) * mv_hi - tab(vtab, V_DPLO, i) * mv_lo) / V;
-    out[IPR] = out[IPR] + (pw + cw);
-    int jj = (i < 1 ? 1 : (i > n - 2 ? n - 2 : i)) - 1;
-    double s = tab(xtab, T_WT0, i) * moved_rho(jj);
-    s = s + tab(xtab, T_WT1, i) * moved_rho(jj + 1);
-    s = s + tab(xtab, T_WT2, i) * moved_rho(jj + 2);
-    double work = tab(xtab, T_GVAR, i) * s;
-    work = work - rows_at(i, raw_minus_mass0);
-    out[IPR] = out[IPR] + work;
-    for (int v = 0; v < 5; ++v) combine(v, out[v]);
-  }
+  VicpFinishBody<PAIR>::run(block_index(), grid_size(), w, du, u0v, c0, uv, cc, vtab, xtab, yc,
+                            status, a, nc);
 }
 
 }  // namespace
@@ -958,8 +987,8 @@ static int vicp_launch(const snapy_grid* g, const snapy_physics* p, const double
   if (ncol > (int64_t(1) << 30)) return fail(-9, "vic_partial: too many columns");
   // the records (kF per layer, lane-contiguous) then yc (3 per layer)
   double* yc = ws + int64_t(kF) * g->nx1 * ncol;
-  hipLaunchKernelGGL(vicp_forward_kernel, dim3(blocks_for(ncol, 64)), dim3(128), 0,
-                     as_stream(stream), w, du, vtab, ws, yc, status, a);
+  launch<VicpForwardBody, 128, 2>(vicp_forward_kernel, dim3(blocks_for(ncol, 64)), dim3(128), 0,
+                                  as_stream(stream), w, du, vtab, ws, yc, status, a);
   if (int e = launch_status("vic_partial_forward")) return e;
   // columns per finish workgroup: the tuned value (default kFinCols), fewer
   // for a tall column
@@ -970,11 +999,13 @@ static int vicp_launch(const snapy_grid* g, const snapy_physics* p, const double
   if (lds > 64 * 1024) return fail(-9, "vic_partial: the column does not fit the LDS");
   dim3 fgrid(unsigned((ncol + nc - 1) / nc));
   if (pair)
-    hipLaunchKernelGGL(vicp_finish_kernel<true>, fgrid, dim3(kFinThreads), lds,
-                       as_stream(stream), w, du, u0, c0, u, c, vtab, x1tab, yc, status, a, nc);
+    launch<VicpFinishBody<true>, kFinThreads, 1>(vicp_finish_kernel<true>, fgrid, dim3(kFinThreads),
+                                                 lds, as_stream(stream), w, du, u0, c0, u, c, vtab,
+                                                 x1tab, yc, status, a, nc);
   else
-    hipLaunchKernelGGL(vicp_finish_kernel<false>, fgrid, dim3(kFinThreads), lds,
-                       as_stream(stream), w, du, u0, c0, u, c, vtab, x1tab, yc, status, a, nc);
+    launch<VicpFinishBody<false>, kFinThreads, 1>(vicp_finish_kernel<false>, fgrid,
+                                                  dim3(kFinThreads), lds, as_stream(stream), w, du,
+                                                  u0, c0, u, c, vtab, x1tab, yc, status, a, nc);
   return launch_status("vic_partial");
 }
 

@@ -14,6 +14,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "group.hpp"
 #include "point.hpp"
 
 namespace snapy {
@@ -61,17 +62,26 @@ int check_grid(snapy_grid const* g) {
 // EOS
 // ---------------------------------------------------------------------------
 template <typename T>
+struct Cons2PrimBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             T const* __restrict__ u, T* __restrict__ w, int64_t n,
+                                             int64_t sv, double gd) {
+    int64_t c = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    T uu[5], ww[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) uu[v] = u[v * sv + c];
+    cons2prim5(uu, ww, gd);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) w[v * sv + c] = ww[v];
+  }
+};
+
+template <typename T>
 __global__ void __launch_bounds__(256)
     cons2prim_kernel(T const* __restrict__ u, T* __restrict__ w, int64_t n,
                      int64_t sv, double gd) {
-  int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  T uu[5], ww[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) uu[v] = u[v * sv + c];
-  cons2prim5(uu, ww, gd);
-#pragma unroll
-  for (int v = 0; v < 5; ++v) w[v * sv + c] = ww[v];
+  Cons2PrimBody<T>::run(block_index(), grid_size(), u, w, n, sv, gd);
 }
 
 template <typename T>
@@ -115,18 +125,27 @@ __device__ __forceinline__ void pair_store(float* hi, float* lo, double x) {
   *lo = float(x - double(h));
 }
 
+struct Cons2PrimPairBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             float const* __restrict__ u,
+                                             float const* __restrict__ c, double* __restrict__ w,
+                                             int64_t n, int64_t sv, double gd) {
+    int64_t q = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double uu[5], ww[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) uu[v] = pair_value(u[v * sv + q], c[v * sv + q]);
+    cons2prim5(uu, ww, gd);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) w[v * sv + q] = ww[v];
+  }
+};
+
 __global__ void __launch_bounds__(256)
     cons2prim_pair_kernel(float const* __restrict__ u, float const* __restrict__ c,
                           double* __restrict__ w, int64_t n, int64_t sv,
                           double gd) {
-  int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  double uu[5], ww[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) uu[v] = pair_value(u[v * sv + q], c[v * sv + q]);
-  cons2prim5(uu, ww, gd);
-#pragma unroll
-  for (int v = 0; v < 5; ++v) w[v * sv + q] = ww[v];
+  Cons2PrimPairBody::run(block_index(), grid_size(), u, c, w, n, sv, gd);
 }
 
 __global__ void __launch_bounds__(256)
@@ -333,67 +352,91 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // CFL: min over interior cells of dx_d / (|v_d| + c), one atomic per block
 // ---------------------------------------------------------------------------
+struct FillU64Body {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             unsigned long long* p, unsigned long long v, int n) {
+    int m = bid.x * blockDim.x + threadIdx.x;
+    if (m < n) p[m] = v;
+  }
+};
+
 __global__ void fill_u64_kernel(unsigned long long* p, unsigned long long v, int n) {
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < n) p[m] = v;
+  FillU64Body::run(block_index(), grid_size(), p, v, n);
 }
 
+struct ScaleF64Body {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim, double* p,
+                                             double scale, int n) {
+    int m = bid.x * blockDim.x + threadIdx.x;
+    if (m < n) p[m] = scale * p[m];
+  }
+};
+
 __global__ void scale_f64_kernel(double* p, double scale, int n) {
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < n) p[m] = scale * p[m];
+  ScaleF64Body::run(block_index(), grid_size(), p, scale, n);
 }
 
 // blockIdx.y: the member along a member x3 axis (its own slab and output),
 // else 0 (every interior cell, one output)
 template <typename T>
+struct CflBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             T const* __restrict__ w, unsigned long long* out,
+                                             Range3 r, double gd, double dx1, double dx2,
+                                             double dx3, int act1, int act2, int act3, double adv1) {
+    int const m = bid.y;
+    int const n3 = r.mem3 > 1 ? 1 : r.n3;
+    int64_t tot = int64_t(r.n1) * r.n2 * n3;
+    double best = 1.e9;
+    out += m;
+    // grid-stride: a bounded grid keeps the final atomics few
+    for (int64_t idx = int64_t(bid.x) * blockDim.x + threadIdx.x; idx < tot;
+         idx += int64_t(gdim.x) * blockDim.x) {
+      int i = r.lo1 + int(idx % r.n1);
+      int64_t rest = idx / r.n1;
+      int j = r.lo2 + int(rest % r.n2);
+      int k = r.lo3 + (r.mem3 > 1 ? m : int(rest / r.n2));
+      int64_t c = k * r.s3 + j * r.s2 + i;
+      T rho = w[c], p = w[4 * r.sv + c];
+      // sound_speed: sqrt(gamma * p / rho) (equation_of_state.cpp:235-238)
+      T cs = sqrt(T(gd) * p / rho);
+      if (act1) {
+        // adv1 > 0: the implicit x1 direction's advection bound
+        T t = adv1 > 0. ? T(dx1) / (fabs(w[r.sv + c]) * T(adv1))
+                        : T(dx1) / (fabs(w[r.sv + c]) + cs);
+        best = fmin(best, double(t));
+      }
+      if (act2) {
+        T t = T(dx2) / (fabs(w[2 * r.sv + c]) + cs);
+        best = fmin(best, double(t));
+      }
+      if (act3) {
+        T t = T(dx3) / (fabs(w[3 * r.sv + c]) + cs);
+        best = fmin(best, double(t));
+      }
+    }
+    // wavefront reduction (64 lanes), then across the 4 wavefronts of the block
+    for (int off = 32; off > 0; off >>= 1)
+      best = fmin(best, __shfl_xor(best, off, 64));
+    __shared__ double part[4];
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) part[wid] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = part[0];
+      for (int q = 1; q < int(blockDim.x >> 6); ++q) m = fmin(m, part[q]);
+      // positive doubles order like their bit patterns
+      atomicMin(out, static_cast<unsigned long long>(__double_as_longlong(m)));
+    }
+  }
+};
+
+template <typename T>
 __global__ void __launch_bounds__(256)
     cfl_kernel(T const* __restrict__ w, unsigned long long* out, Range3 r,
                double gd, double dx1, double dx2, double dx3, int act1,
                int act2, int act3, double adv1) {
-  int const m = blockIdx.y;
-  int const n3 = r.mem3 > 1 ? 1 : r.n3;
-  int64_t tot = int64_t(r.n1) * r.n2 * n3;
-  double best = 1.e9;
-  out += m;
-  // grid-stride: a bounded grid keeps the final atomics few
-  for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < tot;
-       idx += int64_t(gridDim.x) * blockDim.x) {
-    int i = r.lo1 + int(idx % r.n1);
-    int64_t rest = idx / r.n1;
-    int j = r.lo2 + int(rest % r.n2);
-    int k = r.lo3 + (r.mem3 > 1 ? m : int(rest / r.n2));
-    int64_t c = k * r.s3 + j * r.s2 + i;
-    T rho = w[c], p = w[4 * r.sv + c];
-    // sound_speed: sqrt(gamma * p / rho) (equation_of_state.cpp:235-238)
-    T cs = sqrt(T(gd) * p / rho);
-    if (act1) {
-      // adv1 > 0: the implicit x1 direction's advection bound
-      T t = adv1 > 0. ? T(dx1) / (fabs(w[r.sv + c]) * T(adv1))
-                      : T(dx1) / (fabs(w[r.sv + c]) + cs);
-      best = fmin(best, double(t));
-    }
-    if (act2) {
-      T t = T(dx2) / (fabs(w[2 * r.sv + c]) + cs);
-      best = fmin(best, double(t));
-    }
-    if (act3) {
-      T t = T(dx3) / (fabs(w[3 * r.sv + c]) + cs);
-      best = fmin(best, double(t));
-    }
-  }
-  // wavefront reduction (64 lanes), then across the 4 wavefronts of the block
-  for (int off = 32; off > 0; off >>= 1)
-    best = fmin(best, __shfl_xor(best, off, 64));
-  __shared__ double part[4];
-  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) part[wid] = best;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = part[0];
-    for (int q = 1; q < int(blockDim.x >> 6); ++q) m = fmin(m, part[q]);
-    // positive doubles order like their bit patterns
-    atomicMin(out, static_cast<unsigned long long>(__double_as_longlong(m)));
-  }
+  CflBody<T>::run(block_index(), grid_size(), w, out, r, gd, dx1, dx2, dx3, act1, act2, act3, adv1);
 }
 
 // ---------------------------------------------------------------------------
@@ -540,7 +583,7 @@ int snapy_cons2prim(const snapy_grid* g, const snapy_physics* p, int dtype,
   return dispatch(dtype, [&](auto z) {
     using T = decltype(z);
     unsigned tpb = tuned(SNAPY_TUNE_EOS, g, dtype, 256);
-    hipLaunchKernelGGL(cons2prim_kernel<T>, dim3(blocks_for(r.sv, tpb)),
+    launch<Cons2PrimBody<T>, 256, 1>(cons2prim_kernel<T>, dim3(blocks_for(r.sv, tpb)),
                        dim3(tpb), 0, as_stream(stream),
                        static_cast<T const*>(u), static_cast<T*>(w), r.sv,
                        r.sv, p->gammad);
@@ -576,7 +619,7 @@ int snapy_cons2prim_f32(const snapy_grid* g, const snapy_physics* p,
   if (g->nvar != 5) return fail(-6, "cons2prim_f32: ideal gas needs nvar 5");
   Range3 r = ranges(*g);
   unsigned tpb = tuned(SNAPY_TUNE_EOS, g, SNAPY_F32, 256);
-  hipLaunchKernelGGL(cons2prim_pair_kernel, dim3(blocks_for(r.sv, tpb)),
+  launch<Cons2PrimPairBody, 256, 1>(cons2prim_pair_kernel, dim3(blocks_for(r.sv, tpb)),
                      dim3(tpb), 0, as_stream(stream), u, c, w, r.sv, r.sv,
                      p->gammad);
   return launch_status("cons2prim_f32");
@@ -756,7 +799,7 @@ int snapy_cfl(const snapy_grid* g, const snapy_physics* p, int dtype,
   unsigned long long bits;
   std::memcpy(&bits, &init, sizeof(bits));
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(nout, 64)), dim3(64), 0, s, out, bits,
+  launch<FillU64Body, 1024, 1>(fill_u64_kernel, dim3(blocks_for(nout, 64)), dim3(64), 0, s, out, bits,
                      nout);
   bool act2 = axis_active(*g, 2), act3 = axis_active(*g, 3);
   return dispatch(dtype, [&](auto z) {
@@ -764,14 +807,14 @@ int snapy_cfl(const snapy_grid* g, const snapy_physics* p, int dtype,
     unsigned nb = blocks_for(tot, 256);
     unsigned cap = nout > 1 ? 64u : 2048u;
     nb = nb < cap ? nb : cap;
-    hipLaunchKernelGGL(cfl_kernel<T>, dim3(nb, unsigned(nout)), dim3(256), 0,
+    launch<CflBody<T>, 256, 1>(cfl_kernel<T>, dim3(nb, unsigned(nout)), dim3(256), 0,
                        s, static_cast<T const*>(w), out, r, p->gammad, g->dx1,
                        g->dx2, g->dx3, g->nc1 > 1, act2, act3,
                        ((p->implicit_scheme & 1) && (act2 || act3))
                            ? p->advection
                            : 0.);
     if (scale != 1.)
-      hipLaunchKernelGGL(scale_f64_kernel, dim3(blocks_for(nout, 64)), dim3(64), 0, s, dt_out,
+      launch<ScaleF64Body, 1024, 1>(scale_f64_kernel, dim3(blocks_for(nout, 64)), dim3(64), 0, s, dt_out,
                          scale, nout);
     return launch_status("cfl");
   });

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
+#include "group.hpp"
 #include "point.hpp"
 
 namespace snapy {
@@ -167,39 +168,49 @@ __device__ __forceinline__ void cov_flux(double const* __restrict__ x1v, T* __re
 // Segmented lines: a workgroup of 64 lanes walks 62 consecutive x1 cells
 // plus one halo lane on each side (any nx1).
 template <typename T, int DIR>
+struct Sweep23Body {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w,
+                                             double const* __restrict__ x1v, T* __restrict__ flx,
+                                             CovArgs a) {
+    Range3 const& r = a.r;
+    __shared__ double sh_h[kLanes], sh_un[kLanes];
+    int t = threadIdx.x;
+    int i = r.lo1 - 1 + int(bid.y) * kInner + t;
+    // bid.x: (other axis, face) -- DIR 2: (k, j face); DIR 3: (k face, j).
+    // DIR 2 goes through the XCD map, so the neighbouring faces that share
+    // stencil lines share an L2; in DIR 3 the faces that share a line are n2
+    // workgroups apart (the same XCD when 8 divides n2), and the natural order
+    // keeps each XCD on an eighth of the lines, whose reuse distance then fits
+    // its L2
+    int64_t y = DIR == 2 ? xcd_tile(bid.x, gdim.x) : bid.x;
+    int j, k;
+    if (DIR == 2) {
+      j = r.lo2 + int(y % a.nf);
+      k = r.lo3 + int(y / a.nf);
+    } else {
+      j = r.lo2 + int(y % r.n2);
+      k = r.lo3 + int(y / r.n2);
+    }
+    bool live = i >= r.lo1 - 1 && i <= r.lo1 + r.n1;
+    Prim5<T> L, R;
+    double h, un, gauge;
+    cov_states<T, DIR>(w, a, i, j, k, live, L, R, h, un, gauge);
+    sh_h[t] = h;
+    sh_un[t] = un;
+    __syncthreads();
+    if (t == 0 || t == kLanes - 1) return;
+    if (!(i >= r.lo1 && i < r.lo1 + r.n1)) return;
+    cov_flux<T, DIR>(x1v, flx, a, i, j, k, L, R, gauge, sh_h[t - 1], sh_h[t + 1], sh_un[t - 1],
+                     sh_un[t + 1]);
+  }
+};
+
+template <typename T, int DIR>
 __global__ void __launch_bounds__(kLanes)
     sweep23_kernel(double const* __restrict__ w, double const* __restrict__ x1v,
                    T* __restrict__ flx, CovArgs a) {
-  Range3 const& r = a.r;
-  __shared__ double sh_h[kLanes], sh_un[kLanes];
-  int t = threadIdx.x;
-  int i = r.lo1 - 1 + int(blockIdx.y) * kInner + t;
-  // blockIdx.x: (other axis, face) -- DIR 2: (k, j face); DIR 3: (k face, j).
-  // DIR 2 goes through the XCD map, so the neighbouring faces that share
-  // stencil lines share an L2; in DIR 3 the faces that share a line are n2
-  // workgroups apart (the same XCD when 8 divides n2), and the natural order
-  // keeps each XCD on an eighth of the lines, whose reuse distance then fits
-  // its L2
-  int64_t y = DIR == 2 ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
-  int j, k;
-  if (DIR == 2) {
-    j = r.lo2 + int(y % a.nf);
-    k = r.lo3 + int(y / a.nf);
-  } else {
-    j = r.lo2 + int(y % r.n2);
-    k = r.lo3 + int(y / r.n2);
-  }
-  bool live = i >= r.lo1 - 1 && i <= r.lo1 + r.n1;
-  Prim5<T> L, R;
-  double h, un, gauge;
-  cov_states<T, DIR>(w, a, i, j, k, live, L, R, h, un, gauge);
-  sh_h[t] = h;
-  sh_un[t] = un;
-  __syncthreads();
-  if (t == 0 || t == kLanes - 1) return;
-  if (!(i >= r.lo1 && i < r.lo1 + r.n1)) return;
-  cov_flux<T, DIR>(x1v, flx, a, i, j, k, L, R, gauge, sh_h[t - 1], sh_h[t + 1], sh_un[t - 1],
-                   sh_un[t + 1]);
+  Sweep23Body<T, DIR>::run(block_index(), grid_size(), w, x1v, flx, a);
 }
 
 // Packed lines (nx1 + 2 <= kPackMax): a workgroup holds `lines` whole x1
@@ -211,35 +222,45 @@ constexpr int kPackThreads = 1024;
 constexpr int kPackMax = 512;
 
 template <typename T, int DIR>
+struct Sweep23PackedBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w,
+                                             double const* __restrict__ x1v, T* __restrict__ flx,
+                                             CovArgs a, int lines) {
+    Range3 const& r = a.r;
+    __shared__ double sh_h[kPackThreads], sh_un[kPackThreads];
+    int t = threadIdx.x;
+    int len = r.n1 + 2;
+    int q = t / len, p = t - q * len;
+    int64_t nline = int64_t(a.nf) * (DIR == 2 ? r.n3 : r.n2);
+    int64_t y = int64_t(xcd_tile(bid.x, gdim.x)) * lines + q;
+    bool in = q < lines && y < nline;
+    int i = r.lo1 - 1 + p;
+    int j, k;
+    if (DIR == 2) {
+      j = r.lo2 + int(y % a.nf);
+      k = r.lo3 + int(y / a.nf);
+    } else {
+      k = r.lo3 + int(y % a.nf);
+      j = r.lo2 + int(y / a.nf);
+    }
+    Prim5<T> L, R;
+    double h, un, gauge;
+    cov_states<T, DIR>(w, a, i, j, k, in, L, R, h, un, gauge);
+    sh_h[t] = h;
+    sh_un[t] = un;
+    __syncthreads();
+    if (!in || p == 0 || p == len - 1) return;
+    cov_flux<T, DIR>(x1v, flx, a, i, j, k, L, R, gauge, sh_h[t - 1], sh_h[t + 1], sh_un[t - 1],
+                     sh_un[t + 1]);
+  }
+};
+
+template <typename T, int DIR>
 __global__ void __launch_bounds__(kPackThreads)
     sweep23_packed_kernel(double const* __restrict__ w, double const* __restrict__ x1v,
                           T* __restrict__ flx, CovArgs a, int lines) {
-  Range3 const& r = a.r;
-  __shared__ double sh_h[kPackThreads], sh_un[kPackThreads];
-  int t = threadIdx.x;
-  int len = r.n1 + 2;
-  int q = t / len, p = t - q * len;
-  int64_t nline = int64_t(a.nf) * (DIR == 2 ? r.n3 : r.n2);
-  int64_t y = int64_t(xcd_tile(blockIdx.x, gridDim.x)) * lines + q;
-  bool in = q < lines && y < nline;
-  int i = r.lo1 - 1 + p;
-  int j, k;
-  if (DIR == 2) {
-    j = r.lo2 + int(y % a.nf);
-    k = r.lo3 + int(y / a.nf);
-  } else {
-    k = r.lo3 + int(y % a.nf);
-    j = r.lo2 + int(y / a.nf);
-  }
-  Prim5<T> L, R;
-  double h, un, gauge;
-  cov_states<T, DIR>(w, a, i, j, k, in, L, R, h, un, gauge);
-  sh_h[t] = h;
-  sh_un[t] = un;
-  __syncthreads();
-  if (!in || p == 0 || p == len - 1) return;
-  cov_flux<T, DIR>(x1v, flx, a, i, j, k, L, R, gauge, sh_h[t - 1], sh_h[t + 1], sh_un[t - 1],
-                   sh_un[t + 1]);
+  Sweep23PackedBody<T, DIR>::run(block_index(), grid_size(), w, x1v, flx, a, lines);
 }
 
 // Tiled lines (nx1 + 2 <= kPackMax): as the packed kernel, a workgroup
@@ -252,64 +273,74 @@ __global__ void __launch_bounds__(kPackThreads)
 // loads coalesced along x1; each lane then reads its six-cell stencils from
 // LDS instead of global memory.  Dynamic LDS: 5 (lines + 5) (nx1 + 2) T.
 template <typename T, int DIR>
+struct Sweep23TileBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w,
+                                             double const* __restrict__ x1v, T* __restrict__ flx,
+                                             CovArgs a, int lines, int tiles_per_line) {
+    extern __shared__ char smem_raw[];
+    T* st = reinterpret_cast<T*>(smem_raw);
+    __shared__ double sh_h[kPackThreads], sh_un[kPackThreads];
+    Range3 const& r = a.r;
+    int const t = threadIdx.x;
+    int const len = r.n1 + 2;
+    int const ncell = lines + 5;
+    // the tile: line o (k for DIR 2, j for DIR 3) and its first face f0
+    int64_t tile = xcd_tile(bid.x, gdim.x);
+    int o = int(tile / tiles_per_line);
+    int f0 = int(tile - int64_t(o) * tiles_per_line) * lines;
+    int nl = a.nf - f0 < lines ? a.nf - f0 : lines;
+    int64_t sweep = DIR == 2 ? r.s2 : r.s3;
+    int lo = DIR == 2 ? r.lo2 : r.lo3;
+    int64_t obase = DIR == 2 ? int64_t(r.lo3 + o) * r.s3 : int64_t(r.lo2 + o) * r.s2;
+    // the line's gauge row: the first interior cell along the sweep
+    double const* gp = w + IPR * r.sv + obase + int64_t(lo) * sweep + (r.lo1 - 1);
+    // stage cells lo + f0 - 3 .. lo + f0 + nl + 1 (the rest of the tile area
+    // is left unread)
+    {
+      // rows of len cells, `per` rows at a time: thread t takes position sp
+      // of rows sr, sr + per, ...; each row's loads run along x1
+      int const per = int(blockDim.x) / len;
+      int const sr = t / len, sp = t - sr * len;
+      if (sr < per) {
+        double const g = a.gauge ? gp[sp] : 0.;
+        double const* src = w + obase + int64_t(lo + f0 - 3) * sweep + (r.lo1 - 1 + sp);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          double const gv = v == IPR ? g : 0.;
+          for (int c = sr; c < nl + 5; c += per)
+            st[(v * ncell + c) * len + sp] = T(src[v * r.sv + c * sweep] - gv);
+        }
+      }
+    }
+    __syncthreads();
+    int q = t / len, p = t - q * len;
+    bool in = q < nl;
+    int i = r.lo1 - 1 + p;
+    int f = f0 + q;
+    int j = DIR == 2 ? lo + f : r.lo2 + o;
+    int k = DIR == 2 ? r.lo3 + o : lo + f;
+    Prim5<T> L, R;
+    double h = 0., un = 0., gauge = 0.;
+    if (in) {
+      gauge = a.gauge ? gp[p] : 0.;
+      cov_states_tile<T, DIR>(
+          a, [&](int v, int m) { return st[(v * ncell + q + m) * len + p]; }, gauge, L, R, h, un);
+    }
+    sh_h[t] = h;
+    sh_un[t] = un;
+    __syncthreads();
+    if (!in || p == 0 || p == len - 1) return;
+    cov_flux<T, DIR>(x1v, flx, a, i, j, k, L, R, gauge, sh_h[t - 1], sh_h[t + 1], sh_un[t - 1],
+                     sh_un[t + 1]);
+  }
+};
+
+template <typename T, int DIR>
 __global__ void __launch_bounds__(kPackThreads)
     sweep23_tile_kernel(double const* __restrict__ w, double const* __restrict__ x1v,
                         T* __restrict__ flx, CovArgs a, int lines, int tiles_per_line) {
-  extern __shared__ char smem_raw[];
-  T* st = reinterpret_cast<T*>(smem_raw);
-  __shared__ double sh_h[kPackThreads], sh_un[kPackThreads];
-  Range3 const& r = a.r;
-  int const t = threadIdx.x;
-  int const len = r.n1 + 2;
-  int const ncell = lines + 5;
-  // the tile: line o (k for DIR 2, j for DIR 3) and its first face f0
-  int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  int o = int(tile / tiles_per_line);
-  int f0 = int(tile - int64_t(o) * tiles_per_line) * lines;
-  int nl = a.nf - f0 < lines ? a.nf - f0 : lines;
-  int64_t sweep = DIR == 2 ? r.s2 : r.s3;
-  int lo = DIR == 2 ? r.lo2 : r.lo3;
-  int64_t obase = DIR == 2 ? int64_t(r.lo3 + o) * r.s3 : int64_t(r.lo2 + o) * r.s2;
-  // the line's gauge row: the first interior cell along the sweep
-  double const* gp = w + IPR * r.sv + obase + int64_t(lo) * sweep + (r.lo1 - 1);
-  // stage cells lo + f0 - 3 .. lo + f0 + nl + 1 (the rest of the tile area
-  // is left unread)
-  {
-    // rows of len cells, `per` rows at a time: thread t takes position sp
-    // of rows sr, sr + per, ...; each row's loads run along x1
-    int const per = int(blockDim.x) / len;
-    int const sr = t / len, sp = t - sr * len;
-    if (sr < per) {
-      double const g = a.gauge ? gp[sp] : 0.;
-      double const* src = w + obase + int64_t(lo + f0 - 3) * sweep + (r.lo1 - 1 + sp);
-#pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        double const gv = v == IPR ? g : 0.;
-        for (int c = sr; c < nl + 5; c += per)
-          st[(v * ncell + c) * len + sp] = T(src[v * r.sv + c * sweep] - gv);
-      }
-    }
-  }
-  __syncthreads();
-  int q = t / len, p = t - q * len;
-  bool in = q < nl;
-  int i = r.lo1 - 1 + p;
-  int f = f0 + q;
-  int j = DIR == 2 ? lo + f : r.lo2 + o;
-  int k = DIR == 2 ? r.lo3 + o : lo + f;
-  Prim5<T> L, R;
-  double h = 0., un = 0., gauge = 0.;
-  if (in) {
-    gauge = a.gauge ? gp[p] : 0.;
-    cov_states_tile<T, DIR>(
-        a, [&](int v, int m) { return st[(v * ncell + q + m) * len + p]; }, gauge, L, R, h, un);
-  }
-  sh_h[t] = h;
-  sh_un[t] = un;
-  __syncthreads();
-  if (!in || p == 0 || p == len - 1) return;
-  cov_flux<T, DIR>(x1v, flx, a, i, j, k, L, R, gauge, sh_h[t - 1], sh_h[t + 1], sh_un[t - 1],
-                   sh_un[t + 1]);
+  Sweep23TileBody<T, DIR>::run(block_index(), grid_size(), w, x1v, flx, a, lines, tiles_per_line);
 }
 
 // lines per packed workgroup: the count whose lanes waste the least of
@@ -377,41 +408,75 @@ int snapy_sweep_cov(int dim, const snapy_grid* g, const snapy_physics* p,
       unsigned threads = unsigned((tl * len + 63) / 64 * 64);
       unsigned blocks = unsigned(nlines * per);
       if (dtype == SNAPY_F64) {
-        auto k = dim == 2 ? sweep23_tile_kernel<double, 2> : sweep23_tile_kernel<double, 3>;
-        hipLaunchKernelGGL(k, dim3(blocks), dim3(threads), tlds, s, static_cast<double const*>(w),
-                           x1tab, static_cast<double*>(flux), a, tl, per);
+        auto go = [&](auto body, auto k) {
+          launch<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), tlds, s,
+                                                  static_cast<double const*>(w), x1tab,
+                                                  static_cast<double*>(flux), a, tl, per);
+        };
+        if (dim == 2)
+          go(Sweep23TileBody<double, 2>{}, sweep23_tile_kernel<double, 2>);
+        else
+          go(Sweep23TileBody<double, 3>{}, sweep23_tile_kernel<double, 3>);
       } else {
-        auto k = dim == 2 ? sweep23_tile_kernel<float, 2> : sweep23_tile_kernel<float, 3>;
-        hipLaunchKernelGGL(k, dim3(blocks), dim3(threads), tlds, s, static_cast<double const*>(w),
-                           x1tab, static_cast<float*>(flux), a, tl, per);
+        auto go = [&](auto body, auto k) {
+          launch<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), tlds, s,
+                                                  static_cast<double const*>(w), x1tab,
+                                                  static_cast<float*>(flux), a, tl, per);
+        };
+        if (dim == 2)
+          go(Sweep23TileBody<float, 2>{}, sweep23_tile_kernel<float, 2>);
+        else
+          go(Sweep23TileBody<float, 3>{}, sweep23_tile_kernel<float, 3>);
       }
       return launch_status("sweep_cov");
     }
     unsigned threads = unsigned((lines * len + 63) / 64 * 64);
     unsigned blocks = unsigned((ny + lines - 1) / lines);
     if (dtype == SNAPY_F64) {
-      auto k = dim == 2 ? sweep23_packed_kernel<double, 2> : sweep23_packed_kernel<double, 3>;
-      hipLaunchKernelGGL(k, dim3(blocks), dim3(threads), 0, s, static_cast<double const*>(w), x1tab,
-                         static_cast<double*>(flux), a, lines);
+      auto go = [&](auto body, auto k) {
+        launch<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), 0, s,
+                                                static_cast<double const*>(w), x1tab,
+                                                static_cast<double*>(flux), a, lines);
+      };
+      if (dim == 2)
+        go(Sweep23PackedBody<double, 2>{}, sweep23_packed_kernel<double, 2>);
+      else
+        go(Sweep23PackedBody<double, 3>{}, sweep23_packed_kernel<double, 3>);
     } else if (dtype == SNAPY_F32) {
-      auto k = dim == 2 ? sweep23_packed_kernel<float, 2> : sweep23_packed_kernel<float, 3>;
-      hipLaunchKernelGGL(k, dim3(blocks), dim3(threads), 0, s, static_cast<double const*>(w), x1tab,
-                         static_cast<float*>(flux), a, lines);
+      auto go = [&](auto body, auto k) {
+        launch<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), 0, s,
+                                                static_cast<double const*>(w), x1tab,
+                                                static_cast<float*>(flux), a, lines);
+      };
+      if (dim == 2)
+        go(Sweep23PackedBody<float, 2>{}, sweep23_packed_kernel<float, 2>);
+      else
+        go(Sweep23PackedBody<float, 3>{}, sweep23_packed_kernel<float, 3>);
     } else {
       return fail(-10, "unknown dtype");
     }
     return launch_status("sweep_cov");
   }
   if (dtype == SNAPY_F64) {
-    auto k = dim == 2 ? sweep23_kernel<double, 2> : sweep23_kernel<double, 3>;
-    hipLaunchKernelGGL(k, grid, dim3(kLanes), 0, s,
-                       static_cast<double const*>(w), x1tab,
-                       static_cast<double*>(flux), a);
+    auto go = [&](auto body, auto k) {
+      launch<decltype(body), kLanes, 1>(k, grid, dim3(kLanes), 0, s,
+                                        static_cast<double const*>(w), x1tab,
+                                        static_cast<double*>(flux), a);
+    };
+    if (dim == 2)
+      go(Sweep23Body<double, 2>{}, sweep23_kernel<double, 2>);
+    else
+      go(Sweep23Body<double, 3>{}, sweep23_kernel<double, 3>);
   } else if (dtype == SNAPY_F32) {
-    auto k = dim == 2 ? sweep23_kernel<float, 2> : sweep23_kernel<float, 3>;
-    hipLaunchKernelGGL(k, grid, dim3(kLanes), 0, s,
-                       static_cast<double const*>(w), x1tab,
-                       static_cast<float*>(flux), a);
+    auto go = [&](auto body, auto k) {
+      launch<decltype(body), kLanes, 1>(k, grid, dim3(kLanes), 0, s,
+                                        static_cast<double const*>(w), x1tab,
+                                        static_cast<float*>(flux), a);
+    };
+    if (dim == 2)
+      go(Sweep23Body<float, 2>{}, sweep23_kernel<float, 2>);
+    else
+      go(Sweep23Body<float, 3>{}, sweep23_kernel<float, 3>);
   } else {
     return fail(-10, "unknown dtype");
   }

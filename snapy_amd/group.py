@@ -1,0 +1,161 @@
+"""Many blocks of one deck on different grids, advanced with one launch per
+kernel.
+
+The reference steps the blocks of a process on worker threads, one stream
+each (src/mesh/mesh.cpp:74-110).  A `BlockGroup` instead records every
+block's launches once, per phase of a step (the primitives and the CFL dt,
+then each RK stage), through the library's recording mode
+(`snapy_record_begin` / `snapy_record_end`), and groups them per launch
+(`snapy_group_build`): a step then makes each kernel launch once over the
+workgroups of all the blocks (`snapy_group_launch`), every workgroup with
+its own block's arguments.  A block's cells see the arithmetic of its own
+launches, so a grouped step is bit-for-bit the blocks' own steps.
+
+The blocks must make the same launches: the same deck (physics, schemes,
+boundaries, state form) on grids that may differ in their x2/x3 extents and
+spacings but share the x1 cells and the launch shapes.  Each block keeps its
+own buffers and its own device dt.  The recorded launches carry the
+blocks' buffer addresses and the stages' weights, so a group is rebuilt
+when a block's buffers change (a new block, a restart).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _capi
+
+# stage launches that are torch operations, made per block at each step
+# rather than recorded
+_TORCH_CALLS = ("vic_status_clear",)
+
+
+def groupable(block) -> bool:
+    """A block the group can take: a single block (no member axis) of an
+    ideal gas without scalars, on the GPU."""
+    return block.members == 1 and block.nscalar == 0 and block.nvar == 5
+
+
+def group_key(block):
+    """Blocks with equal keys make the same launches: the deck's options but
+    the x2/x3 extents and spacings, and the x1 cells."""
+    op = block.options
+    c = op.coord
+    coord = (c.type, c.x1min, c.x1max, c.nx1, c.nghost, c.nx2 > 1, c.nx3 > 1)
+    return (repr(op.hydro), repr(op.intg), tuple(op.bcnames), coord)
+
+
+class BlockGroup:
+    """Blocks (with their vars) advanced together, one launch per kernel
+    for all of them per step."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        if not self.members:
+            raise ValueError("BlockGroup: no blocks")
+        for blk, _ in self.members:
+            if not groupable(blk):
+                raise ValueError("BlockGroup: a block of members, or with scalars, is not grouped")
+            if blk.device.type != "cuda" or blk._freg is None:
+                raise ValueError("BlockGroup: initialize every block on the GPU first")
+        keys = {group_key(blk) for blk, _ in self.members}
+        if len(keys) != 1:
+            raise ValueError("BlockGroup: the blocks do not share a deck and x1 grid")
+        self.device = self.members[0][0].device
+        self._lib = _capi.load()
+        self._groups = {}
+        try:
+            for phase in ("dt", 0, 1, 2):
+                self._groups[phase] = self._build(phase)
+        except BaseException:
+            self.release(sync=False)
+            raise
+
+    # ------------------------------------------------------------- building
+    def _record(self, blk, vars, phase) -> None:
+        if phase == "dt":
+            # MeshBlock.device_time_step of a block whose primitives are stale
+            # (recording launches nothing: the block's own flag stays as it
+            # was)
+            stale = blk._w_stale
+            blk._refresh_primitives(vars)
+            blk._w_stale = stale
+            g, p = ctypes.byref(blk._grid), ctypes.byref(blk._phys)
+            w = vars["hydro_w"]
+            scale = 2.0 ** (-blk.intg.current_redo) * blk.intg.options.cfl
+            _capi.check(
+                self._lib.snapy_cfl(g, p, _capi.dtype_code(w.dtype), _capi.ptr(w), scale, _capi.ptr(blk._dt_dev),
+                                    _capi.stream_of(self.device)),
+                "cfl",
+            )
+            return
+        for name, fn in blk._stage_calls(vars, phase):
+            if name in _TORCH_CALLS:
+                continue
+            _capi.check(fn(), name)
+
+    def _build(self, phase):
+        lib = self._lib
+        plans = []
+        try:
+            for blk, vars in self.members:
+                _capi.check(lib.snapy_record_begin(), "record_begin")
+                plan = ctypes.c_void_p()
+                try:
+                    self._record(blk, vars, phase)
+                finally:
+                    _capi.check(lib.snapy_record_end(ctypes.byref(plan)), "record_end")
+                    plans.append(plan)
+            arr = (ctypes.c_void_p * len(plans))(*[p.value for p in plans])
+            grp = ctypes.c_void_p()
+            _capi.check(lib.snapy_group_build(arr, len(plans), ctypes.byref(grp)), f"group_build({phase})")
+            return grp
+        finally:
+            for p in plans:
+                lib.snapy_plan_free(p)
+
+    # -------------------------------------------------------------- running
+    def launches(self, phase) -> int:
+        """The grouped launches of a phase ("dt", 0, 1 or 2)."""
+        return int(self._lib.snapy_group_launches(self._groups[phase]))
+
+    def step(self) -> None:
+        """One RK3 step of every block on the current stream: each block's
+        CFL dt (grouped when every block's primitives are stale, as they are
+        after a step; else each block's own), then the three stages."""
+        s = _capi.stream_of(self.device)
+        lib = self._lib
+        if all(blk._w_stale for blk, _ in self.members):
+            _capi.check(lib.snapy_group_launch(self._groups["dt"], s), "group_launch(dt)")
+        else:
+            for blk, vars in self.members:
+                blk.device_time_step(vars)
+        for k in range(3):
+            if k == 0:
+                # the RK backup and the implicit solve's status latch, per
+                # block (MeshBlock._launch_stage and its first call)
+                for blk, _ in self.members:
+                    blk._state.backup()
+                    if blk._vic is not None:
+                        blk._vic["status"].zero_()
+            _capi.check(lib.snapy_group_launch(self._groups[k], s), f"group_launch({k})")
+        for blk, _ in self.members:
+            blk._w_stale = True
+
+    def release(self, sync: bool = True) -> None:
+        """Free the group's device records (after its launches finished)."""
+        if sync and self.device.type == "cuda":
+            torch.cuda.synchronize(self.device)
+        for g in self._groups.values():
+            if g:
+                self._lib.snapy_group_free(g)
+        self._groups = {}
+
+    def __del__(self):
+        try:
+            if self._groups:
+                self.release()
+        except Exception:
+            pass

@@ -12,10 +12,11 @@ device while each request's own stages stay ordered.  The shock tubes of a
 round are batched: one work list (`worklist.BlockBatch`) advances all of
 them with one launch per phase on a stream of their own, rebuilt when a
 tube joins or leaves; a tube's result is bit-identical batched or alone.
-The density-current requests of one grid and state form are batched too:
-they run as the members of one block (`MemberBatch`, the member x3 axis),
-one launch per kernel for all of them, each with its own dt, rebuilt when
-one joins or leaves; a result is bit-identical batched or alone.  A request leaves when
+The density-current requests of one deck and state form and the same
+vertical cells are grouped too, whatever their widths (`group.BlockGroup`):
+every kernel launch of their step is one launch over all of them, each
+block with its own buffers and dt, the group rebuilt when one joins or
+leaves; a result is bit-identical grouped or alone.  A request leaves when
 its step count is reached, freeing its slot and memory for the next
 arrival; its latency is completion time minus arrival time.  With
 `preempt`, a request that does not fit may instead park the active block
@@ -44,6 +45,7 @@ import torch
 from . import problems
 from .cubed import CubedSphereMesh
 from .meshblock import MeshBlock
+from .group import BlockGroup, group_key, groupable
 from .worklist import BlockBatch, eligible
 from .options import CoordinateOptions, HydroOptions, IntegratorOptions, MeshBlockOptions, MeshOptions
 
@@ -207,68 +209,6 @@ def sample_requests(n: int, seed: int = 7, rate: float = 20.0, kmin: int = 50, k
 
 
 # ----------------------------------------------------------------------------
-# density-current requests as the members of one block
-# ----------------------------------------------------------------------------
-def member_key(req: Request):
-    """The batch a request can join: density-current requests of one grid
-    and state form (the same options but for the initial bubble); None for
-    any other request."""
-    if req.kind != "density_current":
-        return None
-    p = req.params
-    return (int(p["nx"]), int(p["nz"]), p.get("precision", "float64"), p.get("execution", "hip-graph"))
-
-
-class MemberBatch:
-    """Requests of one member_key advanced as the members of one block
-    (MeshBlock(options, members=M)): every stage kernel is one launch for
-    all of them, each member with its own device dt.  Each request keeps its
-    own block as the holder of its state: the batch takes the members'
-    conserved state (and the FP32 form's compensation) bit for bit when it
-    is built and gives it back (scatter) before a member leaves, is parked
-    or the batch is rebuilt.  The primitives are recomputed from the state
-    at the next time step, as a solo block does after a step, so a member's
-    steps are the same bits batched or alone."""
-
-    def __init__(self, slots):
-        self.slots = list(slots)
-        self.ids = tuple(sl.req.rid for sl in self.slots)
-        self.bytes = 0  # the footprint the server books for it
-        req = self.slots[0].req
-        op = _options(req.kind, req.params)
-        self.block = MeshBlock(op, members=len(self.slots))
-        self.block.to("cuda")
-        # the members' primitives give the block its buffers; the state
-        # itself is then taken with its own bits
-        w = torch.cat([sl.vars["hydro_w"] for sl in self.slots], dim=1)
-        self.vars, _ = self.block.initialize({"hydro_w": w})
-        u, comp = self.vars["hydro_u"], self.block._state.comp()
-        for m, sl in enumerate(self.slots):
-            u[:, m : m + 1].copy_(sl.vars["hydro_u"])
-            if comp is not None:
-                comp[:, m : m + 1].copy_(sl.block._state.comp())
-        self.block._w_stale = True
-
-    def step(self) -> None:
-        dt = self.block.device_time_step(self.vars)
-        for s in range(3):
-            self.block.forward(self.vars, dt, s)
-
-    def scatter(self) -> None:
-        """Give each member's state back to its own block."""
-        u, comp = self.vars["hydro_u"], self.block._state.comp()
-        for m, sl in enumerate(self.slots):
-            sl.vars["hydro_u"].copy_(u[:, m : m + 1])
-            if comp is not None:
-                sl.block._state.comp().copy_(comp[:, m : m + 1])
-            sl.block._w_stale = True
-
-    def release(self) -> None:
-        self.block.release(sync=False)
-        self.vars.clear()
-
-
-# ----------------------------------------------------------------------------
 # the server
 # ----------------------------------------------------------------------------
 @dataclass
@@ -282,7 +222,7 @@ class _Slot:
     done_steps: int = 0
     si: int = 0  # the index of the stream the request was admitted on
     batched: bool = False  # stepped by the round's work list
-    key: object = None  # the member batch it joins (member_key), or None
+    key: object = None  # the block group it joins (group_key), or None
 
 
 @dataclass
@@ -291,9 +231,9 @@ class Server:
     max_slots: int = 16
     realtime: bool = True  # honour arrival times against the wall clock
     batch_tubes: bool = True  # advance the shock tubes as one work list
-    # advance the density-current requests of one grid and state form as
-    # the members of one block
-    batch_members: bool = True
+    # advance the density-current requests of one deck, state form and x1
+    # grid as one group (one launch per kernel), whatever their widths
+    batch_groups: bool = True
     # a request that does not fit may preempt the active block request with
     # the most steps left (more than the newcomer's): that request is written
     # to a host restart part, its memory freed, and it resumes later from
@@ -307,8 +247,8 @@ class Server:
         streams = [torch.cuda.Stream() for _ in range(self.max_slots)]
         tube_stream = torch.cuda.Stream()
         batch, batch_ids = None, ()
-        # member batches and their streams, by member_key
-        self._members = {}
+        # block groups and their streams, by group_key
+        self._groups = {}
         group_streams = {}
         free_streams = list(range(self.max_slots))
         active: list[_Slot] = []
@@ -333,8 +273,8 @@ class Server:
                     victim = self._victim(active, req, used + need - self.budget_bytes) if self.preempt else None
                     if victim is None:
                         break
-                    if victim.key in self._members:
-                        used -= self._drop_members(victim.key, group_streams[victim.key])
+                    if victim.key in self._groups:
+                        self._drop_group(victim.key, group_streams[victim.key])
                     used -= self._park(victim, parked, now())
                     active.remove(victim)
                     free_streams.append(victim.si)
@@ -358,10 +298,10 @@ class Server:
                     tube_stream.wait_stream(streams[si])
                     sl.stream = tube_stream
                     sl.batched = True
-                elif self.batch_members and member_key(req) is not None:
-                    # its steps run on its member group's stream, alone or
-                    # in the group's batch
-                    sl.key = member_key(req)
+                elif self.batch_groups and req.kind == "density_current" and groupable(block):
+                    # its steps run on its group's stream, alone or in the
+                    # group
+                    sl.key = group_key(block)
                     gs = group_streams.setdefault(sl.key, torch.cuda.Stream())
                     gs.wait_stream(streams[si])
                     sl.stream = gs
@@ -384,33 +324,26 @@ class Server:
                     batch.step()
                 for sl in tubes:
                     sl.done_steps += 1
-            # the member groups: a group of two or more as one batch, rebuilt
-            # when its membership changed
+            # the block groups: two or more requests of one key as one group,
+            # rebuilt when the key's requests changed
             groups = {}
             for sl in active:
                 if sl.key is not None:
                     groups.setdefault(sl.key, []).append(sl)
-            for key in list(self._members):
-                if key not in groups or self._members[key].ids != tuple(x.req.rid for x in groups[key]):
-                    used -= self._drop_members(key, group_streams[key])
+            for key in list(self._groups):
+                ids = tuple(x.req.rid for x in groups.get(key, []))
+                if self._groups[key][0] != ids:
+                    self._drop_group(key, group_streams[key])
             stepped = set()
             for key, sls in groups.items():
                 if len(sls) < 2:
                     continue
-                mb = self._members.get(key)
-                if mb is None:
-                    # the batch's own buffers count against the budget; without
-                    # room its members step alone
-                    need = MeshBlock(_options(sls[0].req.kind, sls[0].req.params), members=len(sls)).hbm_footprint()
-                    if used + need > self.budget_bytes:
-                        continue
-                    with torch.cuda.stream(group_streams[key]):
-                        mb = self._members[key] = MemberBatch(sls)
-                    mb.bytes = need
-                    used += need
-                    self.log.append(("batch", mb.ids, now(), used))
                 with torch.cuda.stream(group_streams[key]):
-                    mb.step()
+                    if key not in self._groups:
+                        ids = tuple(x.req.rid for x in sls)
+                        self._groups[key] = (ids, BlockGroup([(x.block, x.vars) for x in sls]))
+                        self.log.append(("group", ids, now(), used))
+                    self._groups[key][1].step()
                 for sl in sls:
                     sl.done_steps += 1
                     stepped.add(sl.req.rid)
@@ -422,15 +355,15 @@ class Server:
                     for s in range(3):
                         sl.block.forward(sl.vars, dt, s)
                 sl.done_steps += 1
-            # departures (a departing member's batch gives its states back
-            # first and is rebuilt without it next round)
+            # departures (a departing request's group is freed first and
+            # rebuilt without it next round)
             keep = []
             for sl in active:
                 if sl.done_steps < sl.req.steps:
                     keep.append(sl)
                     continue
-                if sl.key in self._members:
-                    used -= self._drop_members(sl.key, group_streams[sl.key])
+                if sl.key in self._groups:
+                    self._drop_group(sl.key, group_streams[sl.key])
                 sl.stream.synchronize()
                 b = sl.block
                 u = _final_state(b, sl.vars)
@@ -447,15 +380,12 @@ class Server:
         torch.cuda.synchronize()
         return results
 
-    def _drop_members(self, key, stream) -> int:
-        """Scatter the member batch of `key` back into its members' blocks
-        and free it; returns the bytes freed."""
-        mb = self._members.pop(key)
-        with torch.cuda.stream(stream):
-            mb.scatter()
+    def _drop_group(self, key, stream) -> None:
+        """Free the block group of `key` once its launches have finished (its
+        blocks keep their states)."""
+        _, grp = self._groups.pop(key)
         stream.synchronize()
-        mb.release()
-        return mb.bytes
+        grp.release(sync=False)
 
     @staticmethod
     def _victim(active, req, short):

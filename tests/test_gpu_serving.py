@@ -81,25 +81,25 @@ def test_preempted_requests_resume_bit_for_bit():
         assert np.array_equal(got[r.rid].state, ref[r.rid].state), r
 
 
-def test_density_currents_batched_equal_alone():
-    """Density-current requests of one grid and state form run as the
-    members of one block, joining and leaving at different rounds (three
-    slots for five requests, different step counts), and each ends with the
-    same bits as on its own block."""
+def test_density_currents_grouped_equal_alone():
+    """Density-current requests of one deck, state form and height stepped
+    as one group (one launch per kernel over requests of different widths),
+    joining and leaving at different rounds (three slots for six requests,
+    different step counts), and each ends with the same bits as alone."""
     reqs = []
-    for i, (prec, dT, steps) in enumerate(
-        [("float64", -15.0, 9), ("float64", -8.0, 5), ("float32", -12.0, 7), ("float64", -19.0, 6),
-         ("float32", -6.0, 8)]
+    for i, (prec, nx, dT, steps) in enumerate(
+        [("float64", 64, -15.0, 9), ("float64", 96, -8.0, 5), ("float32", 48, -12.0, 7),
+         ("float64", 32, -19.0, 6), ("float32", 64, -6.0, 8), ("float32", 96, -10.0, 4)]
     ):
-        reqs.append(serving.Request(i, "density_current", {"nx": 64, "nz": 16, "dT": dT, "precision": prec},
+        reqs.append(serving.Request(i, "density_current", {"nx": nx, "nz": 16, "dT": dT, "precision": prec},
                                     100 + i, steps, 0.0))
-    print("member batches of", [(r.params["precision"], r.steps) for r in reqs], flush=True)
-    srv = serving.Server(realtime=False, max_slots=3, batch_members=True)
+    print("groups of", [(r.params["precision"], r.params["nx"], r.steps) for r in reqs], flush=True)
+    srv = serving.Server(realtime=False, max_slots=4, batch_groups=True)
     a = {r.rid: r for r in srv.run(reqs)}
-    built = [e for e in srv.log if e[0] == "batch"]
-    print("batches", built, flush=True)
+    built = [e for e in srv.log if e[0] == "group"]
+    print("groups", built, flush=True)
     assert len(built) >= 2
-    b = {r.rid: r for r in serving.Server(realtime=False, max_slots=3, batch_members=False).run(reqs)}
+    b = {r.rid: r for r in serving.Server(realtime=False, max_slots=4, batch_groups=False).run(reqs)}
     for r in reqs:
         assert a[r.rid].finite
         assert np.array_equal(a[r.rid].state, b[r.rid].state), r

@@ -36,13 +36,17 @@ def test_stream_is_reproducible():
     ]
 
 
-def test_member_key_groups_density_currents_of_one_grid_and_form():
+def test_group_key_groups_one_deck_and_height_of_any_width():
     from snapy_amd import serving
+    from snapy_amd.group import group_key, groupable
+    from snapy_amd.meshblock import MeshBlock
 
-    a = serving.Request(0, "density_current", {"nx": 256, "nz": 32, "dT": -5.0, "precision": "float32"}, 1, 10)
-    b = serving.Request(1, "density_current", {"nx": 256, "nz": 32, "dT": -9.0, "precision": "float32"}, 2, 20)
-    c = serving.Request(2, "density_current", {"nx": 256, "nz": 32, "dT": -9.0, "precision": "float64"}, 3, 20)
-    d = serving.Request(3, "sod", {"n": 256}, 4, 20)
-    assert serving.member_key(a) == serving.member_key(b)
-    assert serving.member_key(a) != serving.member_key(c)
-    assert serving.member_key(d) is None
+    def key(nx, nz, prec):
+        op = serving._options("density_current", {"nx": nx, "nz": nz, "dT": -5.0, "precision": prec})
+        blk = MeshBlock(op)
+        assert groupable(blk)
+        return group_key(blk)
+
+    assert key(256, 32, "float32") == key(1024, 32, "float32")
+    assert key(256, 32, "float32") != key(256, 64, "float32")
+    assert key(256, 32, "float32") != key(256, 32, "float64")
