@@ -76,8 +76,9 @@ int snapy_group_build(void* const* plans, int nplans, void** group) {
       return fail(-21, "group_build: the plans make different numbers of launches");
     for (size_t i = 0; i < nl; ++i) {
       RecordedLaunch const &x = P(0)->launches[i], &y = P(b)->launches[i];
-      if (x.grouped != y.grouped || x.rec.size() != y.rec.size() || x.block.x != y.block.x ||
-          x.block.y != y.block.y || x.block.z != y.block.z)
+      if (x.grouped != y.grouped || x.rec.size() != y.rec.size() || x.flex != y.flex ||
+          (x.block.x != y.block.x && !x.flex) || x.block.y != y.block.y ||
+          x.block.z != y.block.z)
         return fail(-21, "group_build: launch " + std::to_string(i) +
                              " differs between the plans (a different kernel or workgroup)");
     }
@@ -100,6 +101,7 @@ int snapy_group_build(void* const* plans, int nplans, void** group) {
       RecordedLaunch const& L = P(b)->launches[i];
       img.insert(img.end(), L.rec.begin(), L.rec.end());
       if (L.lds > gl.lds) gl.lds = L.lds;
+      if (L.block.x > gl.block.x) gl.block.x = L.block.x;  // (flex launches only)
     }
     align(16);
     gl.prefix_off = img.size();

@@ -42,6 +42,10 @@ struct RecordedLaunch {
   std::vector<unsigned char> rec;  // a GroupRec<A...>
   dim3 grid, block;
   size_t lds = 0;
+  // the body masks threads past the ones its arguments ask for, so the
+  // blocks of a group may ask for different workgroup sizes along x (the
+  // group takes the largest)
+  bool flex = false;
 };
 
 struct LaunchPlan {
@@ -92,11 +96,9 @@ void grouped_launch(void const* recs, int32_t const* prefix, int nb, unsigned to
                      static_cast<GroupRec<A...> const*>(recs), prefix, nb);
 }
 
-// the launch of a groupable kernel (see the top of this file); WPE is the
-// kernel's amdgpu_waves_per_eu minimum (1 for none)
 template <typename Body, int BOUNDS, int WPE, typename... A, typename... B>
-void launch(void (*kernel)(A...), dim3 grid, dim3 block, size_t lds, hipStream_t s,
-            B&&... args) {
+void launch_as(bool flex, void (*kernel)(A...), dim3 grid, dim3 block, size_t lds, hipStream_t s,
+               B&&... args) {
   LaunchPlan* plan = recording_plan();
   if (!plan) {
     hipLaunchKernelGGL(kernel, grid, block, lds, s, static_cast<A>(args)...);
@@ -110,7 +112,24 @@ void launch(void (*kernel)(A...), dim3 grid, dim3 block, size_t lds, hipStream_t
   L.grid = grid;
   L.block = block;
   L.lds = lds;
+  L.flex = flex;
   plan->launches.push_back(std::move(L));
+}
+
+// the launch of a groupable kernel (see the top of this file); WPE is the
+// kernel's amdgpu_waves_per_eu minimum (1 for none)
+template <typename Body, int BOUNDS, int WPE, typename... A, typename... B>
+void launch(void (*kernel)(A...), dim3 grid, dim3 block, size_t lds, hipStream_t s,
+            B&&... args) {
+  launch_as<Body, BOUNDS, WPE>(false, kernel, grid, block, lds, s, static_cast<B&&>(args)...);
+}
+
+// the launch of a kernel whose body masks the threads past the ones its
+// arguments ask for (RecordedLaunch::flex)
+template <typename Body, int BOUNDS, int WPE, typename... A, typename... B>
+void launch_flex(void (*kernel)(A...), dim3 grid, dim3 block, size_t lds, hipStream_t s,
+                 B&&... args) {
+  launch_as<Body, BOUNDS, WPE>(true, kernel, grid, block, lds, s, static_cast<B&&>(args)...);
 }
 
 }  // namespace snapy

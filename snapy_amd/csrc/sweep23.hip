@@ -409,7 +409,7 @@ int snapy_sweep_cov(int dim, const snapy_grid* g, const snapy_physics* p,
       unsigned blocks = unsigned(nlines * per);
       if (dtype == SNAPY_F64) {
         auto go = [&](auto body, auto k) {
-          launch<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), tlds, s,
+          launch_flex<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), tlds, s,
                                                   static_cast<double const*>(w), x1tab,
                                                   static_cast<double*>(flux), a, tl, per);
         };
@@ -419,7 +419,7 @@ int snapy_sweep_cov(int dim, const snapy_grid* g, const snapy_physics* p,
           go(Sweep23TileBody<double, 3>{}, sweep23_tile_kernel<double, 3>);
       } else {
         auto go = [&](auto body, auto k) {
-          launch<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), tlds, s,
+          launch_flex<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), tlds, s,
                                                   static_cast<double const*>(w), x1tab,
                                                   static_cast<float*>(flux), a, tl, per);
         };
@@ -434,7 +434,7 @@ int snapy_sweep_cov(int dim, const snapy_grid* g, const snapy_physics* p,
     unsigned blocks = unsigned((ny + lines - 1) / lines);
     if (dtype == SNAPY_F64) {
       auto go = [&](auto body, auto k) {
-        launch<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), 0, s,
+        launch_flex<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), 0, s,
                                                 static_cast<double const*>(w), x1tab,
                                                 static_cast<double*>(flux), a, lines);
       };
@@ -444,7 +444,7 @@ int snapy_sweep_cov(int dim, const snapy_grid* g, const snapy_physics* p,
         go(Sweep23PackedBody<double, 3>{}, sweep23_packed_kernel<double, 3>);
     } else if (dtype == SNAPY_F32) {
       auto go = [&](auto body, auto k) {
-        launch<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), 0, s,
+        launch_flex<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), 0, s,
                                                 static_cast<double const*>(w), x1tab,
                                                 static_cast<float*>(flux), a, lines);
       };

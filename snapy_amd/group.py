@@ -12,9 +12,11 @@ its own block's arguments.  A block's cells see the arithmetic of its own
 launches, so a grouped step is bit-for-bit the blocks' own steps.
 
 The blocks must make the same launches: the same deck (physics, schemes,
-boundaries, state form) on grids that may differ in their x2/x3 extents and
-spacings but share the x1 cells and the launch shapes.  Each block keeps its
-own buffers and its own device dt.  The recorded launches carry the
+boundaries, state form) on grids of any extents and spacings, as long as
+each launch runs the same kernel with the same workgroup shape (the x2/x3
+sweeps, whose workgroups cover whole x1 lines, may differ in their
+workgroup size: the group takes the largest and their bodies idle the
+extra threads).  Each block keeps its own buffers and its own device dt.  The recorded launches carry the
 blocks' buffer addresses and the stages' weights, so a group is rebuilt
 when a block's buffers change (a new block, a restart).
 """
@@ -39,11 +41,12 @@ def groupable(block) -> bool:
 
 
 def group_key(block):
-    """Blocks with equal keys make the same launches: the deck's options but
-    the x2/x3 extents and spacings, and the x1 cells."""
+    """Blocks with equal keys run the same deck: the options but the grid's
+    extents, spacings and cell counts (the active axes stay).  The library
+    checks the launches themselves when the group is built."""
     op = block.options
     c = op.coord
-    coord = (c.type, c.x1min, c.x1max, c.nx1, c.nghost, c.nx2 > 1, c.nx3 > 1)
+    coord = (c.type, c.nghost, c.nx1 > 1, c.nx2 > 1, c.nx3 > 1)
     return (repr(op.hydro), repr(op.intg), tuple(op.bcnames), coord)
 
 
@@ -62,7 +65,7 @@ class BlockGroup:
                 raise ValueError("BlockGroup: initialize every block on the GPU first")
         keys = {group_key(blk) for blk, _ in self.members}
         if len(keys) != 1:
-            raise ValueError("BlockGroup: the blocks do not share a deck and x1 grid")
+            raise ValueError("BlockGroup: the blocks do not share a deck")
         self.device = self.members[0][0].device
         self._lib = _capi.load()
         self._groups = {}

@@ -12,8 +12,8 @@ device while each request's own stages stay ordered.  The shock tubes of a
 round are batched: one work list (`worklist.BlockBatch`) advances all of
 them with one launch per phase on a stream of their own, rebuilt when a
 tube joins or leaves; a tube's result is bit-identical batched or alone.
-The density-current requests of one deck and state form and the same
-vertical cells are grouped too, whatever their widths (`group.BlockGroup`):
+The density-current requests of one state form are grouped too, whatever
+their grids (`group.BlockGroup`):
 every kernel launch of their step is one launch over all of them, each
 block with its own buffers and dt, the group rebuilt when one joins or
 leaves; a result is bit-identical grouped or alone.  A request leaves when
@@ -45,6 +45,7 @@ import torch
 from . import problems
 from .cubed import CubedSphereMesh
 from .meshblock import MeshBlock
+from ._capi import SnapyError
 from .group import BlockGroup, group_key, groupable
 from .worklist import BlockBatch, eligible
 from .options import CoordinateOptions, HydroOptions, IntegratorOptions, MeshBlockOptions, MeshOptions
@@ -231,8 +232,8 @@ class Server:
     max_slots: int = 16
     realtime: bool = True  # honour arrival times against the wall clock
     batch_tubes: bool = True  # advance the shock tubes as one work list
-    # advance the density-current requests of one deck, state form and x1
-    # grid as one group (one launch per kernel), whatever their widths
+    # advance the density-current requests of one state form as one group
+    # (one launch per kernel), whatever their grids
     batch_groups: bool = True
     # a request that does not fit may preempt the active block request with
     # the most steps left (more than the newcomer's): that request is written
@@ -341,9 +342,20 @@ class Server:
                 with torch.cuda.stream(group_streams[key]):
                     if key not in self._groups:
                         ids = tuple(x.req.rid for x in sls)
-                        self._groups[key] = (ids, BlockGroup([(x.block, x.vars) for x in sls]))
-                        self.log.append(("group", ids, now(), used))
-                    self._groups[key][1].step()
+                        try:
+                            grp = BlockGroup([(x.block, x.vars) for x in sls])
+                            self.log.append(("group", ids, now(), used))
+                        except SnapyError:
+                            # launches the library cannot group (another
+                            # kernel choice on one of the grids): these
+                            # requests step alone until the group changes
+                            grp = None
+                            self.log.append(("ungrouped", ids, now(), used))
+                        self._groups[key] = (ids, grp)
+                    grp = self._groups[key][1]
+                    if grp is None:
+                        continue
+                    grp.step()
                 for sl in sls:
                     sl.done_steps += 1
                     stepped.add(sl.req.rid)
@@ -384,8 +396,9 @@ class Server:
         """Free the block group of `key` once its launches have finished (its
         blocks keep their states)."""
         _, grp = self._groups.pop(key)
-        stream.synchronize()
-        grp.release(sync=False)
+        if grp is not None:
+            stream.synchronize()
+            grp.release(sync=False)
 
     @staticmethod
     def _victim(active, req, short):

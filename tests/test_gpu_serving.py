@@ -82,18 +82,18 @@ def test_preempted_requests_resume_bit_for_bit():
 
 
 def test_density_currents_grouped_equal_alone():
-    """Density-current requests of one deck, state form and height stepped
-    as one group (one launch per kernel over requests of different widths),
-    joining and leaving at different rounds (three slots for six requests,
+    """Density-current requests of one state form stepped as one group (one
+    launch per kernel over requests of different widths and heights),
+    joining and leaving at different rounds (four slots for six requests,
     different step counts), and each ends with the same bits as alone."""
     reqs = []
-    for i, (prec, nx, dT, steps) in enumerate(
-        [("float64", 64, -15.0, 9), ("float64", 96, -8.0, 5), ("float32", 48, -12.0, 7),
-         ("float64", 32, -19.0, 6), ("float32", 64, -6.0, 8), ("float32", 96, -10.0, 4)]
+    for i, (prec, nx, nz, dT, steps) in enumerate(
+        [("float64", 64, 16, -15.0, 9), ("float64", 96, 24, -8.0, 5), ("float32", 48, 16, -12.0, 7),
+         ("float64", 32, 32, -19.0, 6), ("float32", 64, 24, -6.0, 8), ("float32", 96, 16, -10.0, 4)]
     ):
-        reqs.append(serving.Request(i, "density_current", {"nx": nx, "nz": 16, "dT": dT, "precision": prec},
+        reqs.append(serving.Request(i, "density_current", {"nx": nx, "nz": nz, "dT": dT, "precision": prec},
                                     100 + i, steps, 0.0))
-    print("groups of", [(r.params["precision"], r.params["nx"], r.steps) for r in reqs], flush=True)
+    print("groups of", [(r.params["precision"], r.params["nx"], r.params["nz"], r.steps) for r in reqs], flush=True)
     srv = serving.Server(realtime=False, max_slots=4, batch_groups=True)
     a = {r.rid: r for r in srv.run(reqs)}
     built = [e for e in srv.log if e[0] == "group"]

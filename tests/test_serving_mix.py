@@ -36,7 +36,7 @@ def test_stream_is_reproducible():
     ]
 
 
-def test_group_key_groups_one_deck_and_height_of_any_width():
+def test_group_key_groups_one_deck_on_any_grid():
     from snapy_amd import serving
     from snapy_amd.group import group_key, groupable
     from snapy_amd.meshblock import MeshBlock
@@ -47,6 +47,5 @@ def test_group_key_groups_one_deck_and_height_of_any_width():
         assert groupable(blk)
         return group_key(blk)
 
-    assert key(256, 32, "float32") == key(1024, 32, "float32")
-    assert key(256, 32, "float32") != key(256, 64, "float32")
+    assert key(256, 32, "float32") == key(1024, 32, "float32") == key(512, 128, "float32")
     assert key(256, 32, "float32") != key(256, 32, "float64")
