@@ -85,17 +85,26 @@ __global__ void __launch_bounds__(256)
 }
 
 template <typename T>
+struct Prim2ConsBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             T const* __restrict__ w, T* __restrict__ u, int64_t n,
+                                             int64_t sv, T gm1) {
+    int64_t c = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    T uu[5], ww[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) ww[v] = w[v * sv + c];
+    prim2cons5(ww, uu, gm1);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) u[v * sv + c] = uu[v];
+  }
+};
+
+template <typename T>
 __global__ void __launch_bounds__(256)
     prim2cons_kernel(T const* __restrict__ w, T* __restrict__ u, int64_t n,
                      int64_t sv, T gm1) {
-  int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  T uu[5], ww[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) ww[v] = w[v * sv + c];
-  prim2cons5(ww, uu, gm1);
-#pragma unroll
-  for (int v = 0; v < 5; ++v) u[v * sv + c] = uu[v];
+  Prim2ConsBody<T>::run(block_index(), grid_size(), w, u, n, sv, gm1);
 }
 
 // ---------------------------------------------------------------------------
@@ -148,18 +157,27 @@ __global__ void __launch_bounds__(256)
   Cons2PrimPairBody::run(block_index(), grid_size(), u, c, w, n, sv, gd);
 }
 
+struct Prim2ConsPairBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w, float* __restrict__ u,
+                                             float* __restrict__ c, int64_t n, int64_t sv,
+                                             double gm1) {
+    int64_t q = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double uu[5], ww[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) ww[v] = w[v * sv + q];
+    prim2cons5(ww, uu, gm1);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) pair_store(u + v * sv + q, c + v * sv + q, uu[v]);
+  }
+};
+
 __global__ void __launch_bounds__(256)
     prim2cons_pair_kernel(double const* __restrict__ w, float* __restrict__ u,
                           float* __restrict__ c, int64_t n, int64_t sv,
                           double gm1) {
-  int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  double uu[5], ww[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) ww[v] = w[v * sv + q];
-  prim2cons5(ww, uu, gm1);
-#pragma unroll
-  for (int v = 0; v < 5; ++v) pair_store(u + v * sv + q, c + v * sv + q, uu[v]);
+  Prim2ConsPairBody::run(block_index(), grid_size(), w, u, c, n, sv, gm1);
 }
 
 // one thread per interior face.  DIR 1: the lane walks the faces along x1;
@@ -171,63 +189,72 @@ __global__ void __launch_bounds__(256)
 // cell, so an FP32 reconstruction keeps the small pressure differences; the
 // stored momentum flux is then gauge-free (g cancels in its divergence).
 template <typename T, int DIR>
+struct SweepBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w, T* __restrict__ flx,
+                                             double* __restrict__ fp, SweepArgs a) {
+    Range3 const& r = a.r;
+    int64_t idx = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    int i, j, k;
+    int64_t stride;
+    if (DIR == 1) {
+      int64_t tot = int64_t(a.nf) * r.n2 * r.n3;
+      if (idx >= tot) return;
+      i = r.lo1 + int(idx % a.nf);
+      int64_t rest = idx / a.nf;
+      j = r.lo2 + int(rest % r.n2);
+      k = r.lo3 + int(rest / r.n2);
+      stride = 1;
+    } else if (DIR == 2) {
+      int64_t tot = int64_t(r.n1) * a.nf * r.n3;
+      if (idx >= tot) return;
+      i = r.lo1 + int(idx % r.n1);
+      int64_t rest = idx / r.n1;
+      j = r.lo2 + int(rest % a.nf);
+      k = r.lo3 + int(rest / a.nf);
+      stride = r.s2;
+    } else {
+      int64_t tot = int64_t(r.n1) * r.n2 * a.nf;
+      if (idx >= tot) return;
+      i = r.lo1 + int(idx % r.n1);
+      int64_t rest = idx / r.n1;
+      j = r.lo2 + int(rest % r.n2);
+      k = r.lo3 + int(rest / r.n2);
+      stride = r.s3;
+    }
+    int64_t base = k * r.s3 + j * r.s2 + i;  // cell f: face f is its lower face
+
+    double gauge = 0.;
+    if (a.gauge) {
+      int64_t first = DIR == 1 ? k * r.s3 + j * r.s2 + r.lo1
+                    : DIR == 2 ? k * r.s3 + r.lo2 * r.s2 + i
+                               : r.lo3 * r.s3 + j * r.s2 + i;
+      gauge = w[IPR * r.sv + first];
+    }
+    T out[5];
+    T pface;
+    face_flux_line(w + base, r.sv, stride, gauge, a.gammad, a.riemann, a.shock, DIR - 1, out,
+                   &pface);
+    // boundary.prescribed at a wall plane (FluxRegister::apply_prescribed,
+    // flux_register.cpp:707-748, planes and free rows from supply_prescribed,
+    // seam_register.cpp:184-220): the momentum rows stay free, the mass and
+    // energy rows are zero
+    int f = DIR == 1 ? i - r.lo1 : (DIR == 2 ? j - r.lo2 : k - r.lo3);
+    if ((f == 0 && a.wall_lo) || (f == a.nf - 1 && a.wall_hi)) {
+      out[IDN] = T(0);
+      out[IPR] = T(0);
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) flx[v * r.sv + base] = out[v];
+    if (DIR == 1 && fp != nullptr) fp[base] = double(pface);
+  }
+};
+
+template <typename T, int DIR>
 __global__ void __launch_bounds__(256)
     sweep_kernel(double const* __restrict__ w, T* __restrict__ flx,
                  double* __restrict__ fp, SweepArgs a) {
-  Range3 const& r = a.r;
-  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  int i, j, k;
-  int64_t stride;
-  if (DIR == 1) {
-    int64_t tot = int64_t(a.nf) * r.n2 * r.n3;
-    if (idx >= tot) return;
-    i = r.lo1 + int(idx % a.nf);
-    int64_t rest = idx / a.nf;
-    j = r.lo2 + int(rest % r.n2);
-    k = r.lo3 + int(rest / r.n2);
-    stride = 1;
-  } else if (DIR == 2) {
-    int64_t tot = int64_t(r.n1) * a.nf * r.n3;
-    if (idx >= tot) return;
-    i = r.lo1 + int(idx % r.n1);
-    int64_t rest = idx / r.n1;
-    j = r.lo2 + int(rest % a.nf);
-    k = r.lo3 + int(rest / a.nf);
-    stride = r.s2;
-  } else {
-    int64_t tot = int64_t(r.n1) * r.n2 * a.nf;
-    if (idx >= tot) return;
-    i = r.lo1 + int(idx % r.n1);
-    int64_t rest = idx / r.n1;
-    j = r.lo2 + int(rest % r.n2);
-    k = r.lo3 + int(rest / r.n2);
-    stride = r.s3;
-  }
-  int64_t base = k * r.s3 + j * r.s2 + i;  // cell f: face f is its lower face
-
-  double gauge = 0.;
-  if (a.gauge) {
-    int64_t first = DIR == 1 ? k * r.s3 + j * r.s2 + r.lo1
-                  : DIR == 2 ? k * r.s3 + r.lo2 * r.s2 + i
-                             : r.lo3 * r.s3 + j * r.s2 + i;
-    gauge = w[IPR * r.sv + first];
-  }
-  T out[5];
-  T pface;
-  face_flux_line(w + base, r.sv, stride, gauge, a.gammad, a.riemann, a.shock, DIR - 1, out,
-                 &pface);
-  // boundary.prescribed at a wall plane (FluxRegister::apply_prescribed,
-  // flux_register.cpp:707-748, planes and free rows from supply_prescribed,
-  // seam_register.cpp:184-220): the momentum rows stay free, the mass and
-  // energy rows are zero
-  int f = DIR == 1 ? i - r.lo1 : (DIR == 2 ? j - r.lo2 : k - r.lo3);
-  if ((f == 0 && a.wall_lo) || (f == a.nf - 1 && a.wall_hi)) {
-    out[IDN] = T(0);
-    out[IPR] = T(0);
-  }
-#pragma unroll
-  for (int v = 0; v < 5; ++v) flx[v * r.sv + base] = out[v];
-  if (DIR == 1 && fp != nullptr) fp[base] = double(pface);
+  SweepBody<T, DIR>::run(block_index(), grid_size(), w, flx, fp, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -243,110 +270,144 @@ struct UpdateArgs {
 };
 
 template <typename T>
+struct UpdateBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             T const* __restrict__ f1, T const* __restrict__ f2,
+                                             T const* __restrict__ f3, T const* __restrict__ u0,
+                                             T* __restrict__ u, T const* __restrict__ dsrc,
+                                             UpdateArgs a) {
+    Range3 const& r = a.r;
+    int64_t idx = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
+    if (idx >= tot) return;
+    int i = r.lo1 + int(idx % r.n1);
+    int64_t rest = idx / r.n1;
+    int j = r.lo2 + int(rest % r.n2);
+    int k = r.lo3 + int(rest / r.n2);
+    int64_t c = k * r.s3 + j * r.s2 + i;
+    T A1 = T(a.a1), A2 = T(a.a2), A3 = T(a.a3), V = T(a.vol);
+    T mdt = a.dt_dev ? T(-dt_at(a.dt_dev, 0., r, k)) : T(a.mdt);
+    T w0 = T(a.w0), w1 = T(a.w1), w2 = T(a.w2);
+    for (int v = 0; v < a.nvar; ++v) {
+      int64_t o = v * r.sv + c;
+      // dflx = 0; dflx += A F(+) - A F(-) per active direction; / vol
+      // (CoordinateImpl::divergence, coordinate.cpp:618-662)
+      T div = T(0);
+      bool first = true;
+      if (f1) {
+        T d = A1 * f1[o + 1] - A1 * f1[o];
+        div = first ? d : div + d;
+        first = false;
+      }
+      if (f2) {
+        T d = A2 * f2[o + r.s2] - A2 * f2[o];
+        div = first ? d : div + d;
+        first = false;
+      }
+      if (f3) {
+        T d = A3 * f3[o + r.s3] - A3 * f3[o];
+        div = first ? d : div + d;
+        first = false;
+      }
+      div = div / V;
+      T du = mdt * div;
+      if (dsrc) du = du + dsrc[o];
+      u[o] = w0 * u0[o] + w1 * u[o] + w2 * du;
+    }
+  }
+};
+
+template <typename T>
 __global__ void __launch_bounds__(256)
     update_kernel(T const* __restrict__ f1, T const* __restrict__ f2,
                   T const* __restrict__ f3, T const* __restrict__ u0,
                   T* __restrict__ u, T const* __restrict__ dsrc, UpdateArgs a) {
-  Range3 const& r = a.r;
-  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
-  if (idx >= tot) return;
-  int i = r.lo1 + int(idx % r.n1);
-  int64_t rest = idx / r.n1;
-  int j = r.lo2 + int(rest % r.n2);
-  int k = r.lo3 + int(rest / r.n2);
-  int64_t c = k * r.s3 + j * r.s2 + i;
-  T A1 = T(a.a1), A2 = T(a.a2), A3 = T(a.a3), V = T(a.vol);
-  T mdt = a.dt_dev ? T(-dt_at(a.dt_dev, 0., r, k)) : T(a.mdt);
-  T w0 = T(a.w0), w1 = T(a.w1), w2 = T(a.w2);
-  for (int v = 0; v < a.nvar; ++v) {
-    int64_t o = v * r.sv + c;
-    // dflx = 0; dflx += A F(+) - A F(-) per active direction; / vol
-    // (CoordinateImpl::divergence, coordinate.cpp:618-662)
-    T div = T(0);
-    bool first = true;
-    if (f1) {
-      T d = A1 * f1[o + 1] - A1 * f1[o];
-      div = first ? d : div + d;
-      first = false;
-    }
-    if (f2) {
-      T d = A2 * f2[o + r.s2] - A2 * f2[o];
-      div = first ? d : div + d;
-      first = false;
-    }
-    if (f3) {
-      T d = A3 * f3[o + r.s3] - A3 * f3[o];
-      div = first ? d : div + d;
-      first = false;
-    }
-    div = div / V;
-    T du = mdt * div;
-    if (dsrc) du = du + dsrc[o];
-    u[o] = w0 * u0[o] + w1 * u[o] + w2 * du;
-  }
+  UpdateBody<T>::run(block_index(), grid_size(), f1, f2, f3, u0, u, dsrc, a);
 }
 
 // The FP32 state form of update_kernel: FP32 face fluxes, the divergence and
 // the combine in FP64 on the (u, c) pairs, du_src FP64.
+struct UpdatePairBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             float const* __restrict__ f1,
+                                             float const* __restrict__ f2,
+                                             float const* __restrict__ f3,
+                                             float const* __restrict__ u0,
+                                             float const* __restrict__ c0, float* __restrict__ u,
+                                             float* __restrict__ cc,
+                                             double const* __restrict__ dsrc, UpdateArgs a) {
+    Range3 const& r = a.r;
+    int64_t idx = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
+    if (idx >= tot) return;
+    int i = r.lo1 + int(idx % r.n1);
+    int64_t rest = idx / r.n1;
+    int j = r.lo2 + int(rest % r.n2);
+    int k = r.lo3 + int(rest / r.n2);
+    int64_t c = k * r.s3 + j * r.s2 + i;
+    double A1 = a.a1, A2 = a.a2, A3 = a.a3, V = a.vol;
+    double mdt = a.dt_dev ? -dt_at(a.dt_dev, 0., r, k) : a.mdt;
+    for (int v = 0; v < a.nvar; ++v) {
+      int64_t o = v * r.sv + c;
+      double div = 0.;
+      bool first = true;
+      if (f1) {
+        double d = A1 * double(f1[o + 1]) - A1 * double(f1[o]);
+        div = first ? d : div + d;
+        first = false;
+      }
+      if (f2) {
+        double d = A2 * double(f2[o + r.s2]) - A2 * double(f2[o]);
+        div = first ? d : div + d;
+        first = false;
+      }
+      if (f3) {
+        double d = A3 * double(f3[o + r.s3]) - A3 * double(f3[o]);
+        div = first ? d : div + d;
+        first = false;
+      }
+      div = div / V;
+      double du = mdt * div;
+      if (dsrc) du = du + dsrc[o];
+      double x = a.w0 * pair_value(u0[o], c0[o]) + a.w1 * pair_value(u[o], cc[o]) +
+                 a.w2 * du;
+      pair_store(u + o, cc + o, x);
+    }
+  }
+};
+
 __global__ void __launch_bounds__(256)
     update_pair_kernel(float const* __restrict__ f1, float const* __restrict__ f2,
                        float const* __restrict__ f3, float const* __restrict__ u0,
                        float const* __restrict__ c0, float* __restrict__ u,
                        float* __restrict__ cc, double const* __restrict__ dsrc,
                        UpdateArgs a) {
-  Range3 const& r = a.r;
-  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
-  if (idx >= tot) return;
-  int i = r.lo1 + int(idx % r.n1);
-  int64_t rest = idx / r.n1;
-  int j = r.lo2 + int(rest % r.n2);
-  int k = r.lo3 + int(rest / r.n2);
-  int64_t c = k * r.s3 + j * r.s2 + i;
-  double A1 = a.a1, A2 = a.a2, A3 = a.a3, V = a.vol;
-  double mdt = a.dt_dev ? -dt_at(a.dt_dev, 0., r, k) : a.mdt;
-  for (int v = 0; v < a.nvar; ++v) {
-    int64_t o = v * r.sv + c;
-    double div = 0.;
-    bool first = true;
-    if (f1) {
-      double d = A1 * double(f1[o + 1]) - A1 * double(f1[o]);
-      div = first ? d : div + d;
-      first = false;
-    }
-    if (f2) {
-      double d = A2 * double(f2[o + r.s2]) - A2 * double(f2[o]);
-      div = first ? d : div + d;
-      first = false;
-    }
-    if (f3) {
-      double d = A3 * double(f3[o + r.s3]) - A3 * double(f3[o]);
-      div = first ? d : div + d;
-      first = false;
-    }
-    div = div / V;
-    double du = mdt * div;
-    if (dsrc) du = du + dsrc[o];
-    double x = a.w0 * pair_value(u0[o], c0[o]) + a.w1 * pair_value(u[o], cc[o]) +
-               a.w2 * du;
-    pair_store(u + o, cc + o, x);
-  }
+  UpdatePairBody::run(block_index(), grid_size(), f1, f2, f3, u0, c0, u, cc, dsrc, a);
 }
 
 // S <- w1*S + w2*dt*T over every face of the total, in FP64
+template <typename T>
+struct StepFieldBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             T const* __restrict__ t, double* __restrict__ s,
+                                             int64_t n, double w1, double w2, double dt,
+                                             double const* dt_dev, Range3 r) {
+    int64_t c = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    // dt.scaled(w2) then axpy_ (flux_register.cpp:761-777); a member axis
+    // has no x3 ghosts, so the x3 index is the member
+    int k = r.mem3 > 1 ? int((c / r.s3) % r.nc3) : 0;
+    double w2dt = (dt_dev ? dt_dev[k] : dt) * w2;
+    s[c] = s[c] * w1 + double(t[c]) * w2dt;
+  }
+};
+
 template <typename T>
 __global__ void __launch_bounds__(256)
     step_field_kernel(T const* __restrict__ t, double* __restrict__ s,
                       int64_t n, double w1, double w2, double dt,
                       double const* dt_dev, Range3 r) {
-  int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  // dt.scaled(w2) then axpy_ (flux_register.cpp:761-777); a member axis
-  // has no x3 ghosts, so the x3 index is the member
-  int k = r.mem3 > 1 ? int((c / r.s3) % r.nc3) : 0;
-  double w2dt = (dt_dev ? dt_dev[k] : dt) * w2;
-  s[c] = s[c] * w1 + double(t[c]) * w2dt;
+  StepFieldBody<T>::run(block_index(), grid_size(), t, s, n, w1, w2, dt, dt_dev, r);
 }
 
 // ---------------------------------------------------------------------------
@@ -602,7 +663,7 @@ int snapy_prim2cons(const snapy_grid* g, const snapy_physics* p, int dtype,
   return dispatch(dtype, [&](auto z) {
     using T = decltype(z);
     unsigned tpb = tuned(SNAPY_TUNE_EOS, g, dtype, 256);
-    hipLaunchKernelGGL(prim2cons_kernel<T>, dim3(blocks_for(r.sv, tpb)),
+    launch<Prim2ConsBody<T>, 256, 1>(prim2cons_kernel<T>, dim3(blocks_for(r.sv, tpb)),
                        dim3(tpb), 0, as_stream(stream),
                        static_cast<T const*>(w), static_cast<T*>(u), r.sv,
                        r.sv, T(p->gammad - 1.));
@@ -633,7 +694,7 @@ int snapy_prim2cons_f32(const snapy_grid* g, const snapy_physics* p,
   if (g->nvar != 5) return fail(-6, "prim2cons_f32: ideal gas needs nvar 5");
   Range3 r = ranges(*g);
   unsigned tpb = tuned(SNAPY_TUNE_EOS, g, SNAPY_F32, 256);
-  hipLaunchKernelGGL(prim2cons_pair_kernel, dim3(blocks_for(r.sv, tpb)),
+  launch<Prim2ConsPairBody, 256, 1>(prim2cons_pair_kernel, dim3(blocks_for(r.sv, tpb)),
                      dim3(tpb), 0, as_stream(stream), w, u, c, r.sv, r.sv,
                      p->gammad - 1.);
   return launch_status("prim2cons_f32");
@@ -659,7 +720,7 @@ int snapy_update_f32(const snapy_grid* g, const void* const flux[3],
   a.w2 = w2;
   int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
   unsigned tpb = tuned(SNAPY_TUNE_UPDATE, g, SNAPY_F32, 256);
-  hipLaunchKernelGGL(update_pair_kernel, dim3(blocks_for(tot, tpb)), dim3(tpb),
+  launch<UpdatePairBody, 256, 1>(update_pair_kernel, dim3(blocks_for(tot, tpb)), dim3(tpb),
                      0, as_stream(stream), static_cast<float const*>(flux[0]),
                      static_cast<float const*>(flux[1]),
                      static_cast<float const*>(flux[2]), u0, c0, u, c, du_src,
@@ -707,11 +768,11 @@ int snapy_sweep(int dim, const snapy_grid* g, const snapy_physics* p,
     dim3 grid(blocks_for(tot, tpb)), blk(tpb);
     hipStream_t s = as_stream(stream);
     if (dim == 1)
-      hipLaunchKernelGGL((sweep_kernel<T, 1>), grid, blk, 0, s, wp, fl, fp, a);
+      launch<SweepBody<T, 1>, 256, 1>((sweep_kernel<T, 1>), grid, blk, 0, s, wp, fl, fp, a);
     else if (dim == 2)
-      hipLaunchKernelGGL((sweep_kernel<T, 2>), grid, blk, 0, s, wp, fl, fp, a);
+      launch<SweepBody<T, 2>, 256, 1>((sweep_kernel<T, 2>), grid, blk, 0, s, wp, fl, fp, a);
     else
-      hipLaunchKernelGGL((sweep_kernel<T, 3>), grid, blk, 0, s, wp, fl, fp, a);
+      launch<SweepBody<T, 3>, 256, 1>((sweep_kernel<T, 3>), grid, blk, 0, s, wp, fl, fp, a);
     return launch_status("sweep");
   });
 }
@@ -738,7 +799,7 @@ int snapy_update(const snapy_grid* g, int dtype, const void* const flux[3],
   return dispatch(dtype, [&](auto z) {
     using T = decltype(z);
     unsigned tpb = tuned(SNAPY_TUNE_UPDATE, g, dtype, 256);
-    hipLaunchKernelGGL(update_kernel<T>, dim3(blocks_for(tot, tpb)), dim3(tpb),
+    launch<UpdateBody<T>, 256, 1>(update_kernel<T>, dim3(blocks_for(tot, tpb)), dim3(tpb),
                        0, as_stream(stream), static_cast<T const*>(flux[0]),
                        static_cast<T const*>(flux[1]),
                        static_cast<T const*>(flux[2]),
@@ -756,7 +817,7 @@ int snapy_step_field(const snapy_grid* g, int dtype, const void* total,
   int64_t n = int64_t(g->nvar) * g->nc1 * g->nc2 * g->nc3;
   return dispatch(dtype, [&](auto z) {
     using T = decltype(z);
-    hipLaunchKernelGGL(step_field_kernel<T>, dim3(blocks_for(n, 256)),
+    launch<StepFieldBody<T>, 256, 1>(step_field_kernel<T>, dim3(blocks_for(n, 256)),
                        dim3(256), 0, as_stream(stream),
                        static_cast<T const*>(total), step, n, w1, w2, dt,
                        dt_dev, ranges(*g));

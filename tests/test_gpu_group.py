@@ -87,3 +87,78 @@ def test_groups_refuse_blocks_that_launch_differently():
         lib.snapy_plan_free(p)
     assert rc == -21
     assert b"differ" in lib.snapy_last_error()
+
+
+def _explicit_block(nx1, nx2, nx3, precision, seed):
+    """A shock-capturing block without gravity (the plain x1 sweep and the
+    divergence + combine kernels): HLLC, outflow x1, periodic x2/x3."""
+    import numpy as np
+
+    from snapy_amd.options import CoordinateOptions, HydroOptions, IntegratorOptions, MeshBlockOptions
+
+    op = MeshBlockOptions()
+    op.coord = CoordinateOptions("cartesian", 0.0, 1.0, 0.0, 0.5, 0.0, 0.5, nx1, nx2, nx3, 3)
+    op.hydro = HydroOptions(riemann="hllc", precision=precision)
+    op.hydro.eos.gammad = 1.4
+    op.intg = IntegratorOptions(cfl=0.8, execution="eager")
+    op.bcnames = ["outflow", "outflow", "periodic", "periodic", "periodic", "periodic"]
+    blk = MeshBlock(op)
+    blk.to("cuda")
+    rng = np.random.default_rng(seed)
+    w = np.zeros(blk.shape())
+    w[0] = 1.0 + 0.2 * rng.random(w[0].shape)
+    w[1] = 0.1 * rng.standard_normal(w[0].shape)
+    w[4] = 1.0 + 0.2 * rng.random(w[0].shape)
+    vars, _ = blk.initialize({"hydro_w": torch.from_numpy(w).cuda()})
+    return blk, vars
+
+
+def _convection_block(nx1, nx2, nx3, precision, seed):
+    """A small 3D convection block (the convection deck's physics: gravity,
+    the covariance sweeps, bot-heat, implicit scheme 1)."""
+    import os
+
+    from snapy_amd.options import MeshBlockOptions
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    op = MeshBlockOptions.from_yaml(os.path.join(root, "snapy_amd", "decks", "convection3d.yaml"))
+    op.coord.nx1, op.coord.nx2, op.coord.nx3 = nx1, nx2, nx3
+    op.coord.x1max = 50.0 * nx1
+    op.coord.x2max = 100.0 * nx2
+    op.coord.x3max = 100.0 * nx3
+    op.hydro.precision = precision
+    op.intg.execution = "eager"
+    blk = MeshBlock(op)
+    blk.to("cuda")
+    vars, _ = blk.initialize({"hydro_w": problems.convection3d(blk, seed=seed, amp=2.0, layers=4).cuda()})
+    return blk, vars
+
+
+@pytest.mark.parametrize("kind,precision", [("explicit", "float64"), ("explicit", "float32"),
+                                            ("convection", "float64"), ("convection", "float32")])
+def test_grouped_3d_blocks_equal_each_blocks_own(kind, precision):
+    shapes = [(16, 8, 8), (24, 6, 10), (16, 12, 4)]
+    make = _explicit_block if kind == "explicit" else _convection_block
+    print(f"group of {kind} blocks {shapes}, {precision}", flush=True)
+    nsteps = 3
+    solo = []
+    for n, shp in enumerate(shapes):
+        blk, vars = make(*shp, precision, 40 + n)
+        for _ in range(nsteps):
+            dt = blk.device_time_step(vars)
+            for s in range(3):
+                blk.forward(vars, dt, s)
+        torch.cuda.synchronize()
+        solo.append(_state(blk, vars))
+        blk.release()
+    members = [make(*shp, precision, 40 + n) for n, shp in enumerate(shapes)]
+    grp = BlockGroup(members)
+    for _ in range(nsteps):
+        grp.step()
+    torch.cuda.synchronize()
+    for (blk, vars), ref in zip(members, solo):
+        got = _state(blk, vars)
+        assert torch.isfinite(got[0]).all()
+        for a, b in zip(got, ref):
+            assert torch.equal(a, b), (kind, precision, blk.options.coord.nx1)
+    grp.release()
