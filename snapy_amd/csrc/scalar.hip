@@ -34,6 +34,7 @@
 #include <cmath>
 
 #include "common.hpp"
+#include "group.hpp"
 #include "point.hpp"
 
 namespace snapy {
@@ -129,58 +130,70 @@ struct FluxArgs {
 
 // one thread per face: faces lo..lo+n along DIR, the interior across it
 template <int DIR, typename TF>
+struct ScalarFluxBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w,
+                                             double const* __restrict__ s,
+                                             TF const* __restrict__ hflux,
+                                             double const* __restrict__ x1v,
+                                             double* __restrict__ sflux, FluxArgs a) {
+    Range3 const& r = a.r;
+    int n1 = DIR == 1 ? a.nf : r.n1, n2 = DIR == 2 ? a.nf : r.n2, n3 = DIR == 3 ? a.nf : r.n3;
+    int64_t tot = int64_t(n1) * n2 * n3;
+    int64_t idx = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (idx >= tot) return;
+    int i = r.lo1 + int(idx % n1);
+    int64_t rest = idx / n1;
+    int j = r.lo2 + int(rest % n2);
+    int k = r.lo3 + int(rest / n2);
+    int f = DIR == 1 ? i - r.lo1 : (DIR == 2 ? j - r.lo2 : k - r.lo3);
+    int64_t c = k * r.s3 + j * r.s2 + i;
+    bool wall = (f == 0 && a.wall_lo) || (f == a.nf - 1 && a.wall_hi);
+    double m = double(hflux[IDN * r.sv + c]);
+    double const* rho = w + IDN * r.sv;
+    // the covariance's hydro parts, shared by every row: rho at the face and
+    // the normal velocity of the x1 neighbours' faces
+    double crho = 0., d1un = 0., dx1c = 1.;
+    bool cov = DIR > 1 && a.cov;
+    if (cov) {
+      int vn = DIR == 2 ? IVY : IVZ;
+      double rl, rr, ulm, urm, ulp, urp;
+      hydro_face(rho + c, a.st, true, rl, rr);
+      crho = 0.5 * (rl + rr);
+      hydro_face(w + vn * r.sv + c - 1, a.st, a.shock != 0, ulm, urm);
+      hydro_face(w + vn * r.sv + c + 1, a.st, a.shock != 0, ulp, urp);
+      dx1c = x1v[i + 1] - x1v[i - 1];
+      d1un = (0.5 * (ulp + urp) - 0.5 * (ulm + urm)) / dx1c;
+    }
+    for (int v = 0; v < a.ns; ++v) {
+      double const* sv = s + v * r.sv;
+      double rl, rr;
+      face_ratio(rho + c, sv + c, a.st, rl, rr);
+      // the upwind solver (riemann.cpp:164-171)
+      double ui = m > 0. ? 1. : 0.;
+      double F = m * (ui * rl + (1. - ui) * rr);
+      if (cov) {
+        // HydroImpl::scalar_flux_covariance (hydro_forward.cpp:233-297):
+        // sigma1^2 rho D1[u_n] D1[q], q the mean of the face ratios
+        double lm, rm, lp, rp;
+        face_ratio(rho + c - 1, sv + c - 1, a.st, lm, rm);
+        face_ratio(rho + c + 1, sv + c + 1, a.st, lp, rp);
+        double qm = 0.5 * (lm * 1. + rm * 1.), qp = 0.5 * (lp * 1. + rp * 1.);
+        double d1q = (qp - qm) / dx1c;
+        F = F + a.s2 * crho * d1un * d1q;
+      }
+      // boundary.prescribed: every tracer row is 0 on a wall face
+      sflux[v * r.sv + c] = wall ? 0. : F;
+    }
+  }
+};
+
+template <int DIR, typename TF>
 __global__ void __launch_bounds__(256)
     scalar_flux_kernel(double const* __restrict__ w, double const* __restrict__ s,
                        TF const* __restrict__ hflux, double const* __restrict__ x1v,
                        double* __restrict__ sflux, FluxArgs a) {
-  Range3 const& r = a.r;
-  int n1 = DIR == 1 ? a.nf : r.n1, n2 = DIR == 2 ? a.nf : r.n2, n3 = DIR == 3 ? a.nf : r.n3;
-  int64_t tot = int64_t(n1) * n2 * n3;
-  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= tot) return;
-  int i = r.lo1 + int(idx % n1);
-  int64_t rest = idx / n1;
-  int j = r.lo2 + int(rest % n2);
-  int k = r.lo3 + int(rest / n2);
-  int f = DIR == 1 ? i - r.lo1 : (DIR == 2 ? j - r.lo2 : k - r.lo3);
-  int64_t c = k * r.s3 + j * r.s2 + i;
-  bool wall = (f == 0 && a.wall_lo) || (f == a.nf - 1 && a.wall_hi);
-  double m = double(hflux[IDN * r.sv + c]);
-  double const* rho = w + IDN * r.sv;
-  // the covariance's hydro parts, shared by every row: rho at the face and
-  // the normal velocity of the x1 neighbours' faces
-  double crho = 0., d1un = 0., dx1c = 1.;
-  bool cov = DIR > 1 && a.cov;
-  if (cov) {
-    int vn = DIR == 2 ? IVY : IVZ;
-    double rl, rr, ulm, urm, ulp, urp;
-    hydro_face(rho + c, a.st, true, rl, rr);
-    crho = 0.5 * (rl + rr);
-    hydro_face(w + vn * r.sv + c - 1, a.st, a.shock != 0, ulm, urm);
-    hydro_face(w + vn * r.sv + c + 1, a.st, a.shock != 0, ulp, urp);
-    dx1c = x1v[i + 1] - x1v[i - 1];
-    d1un = (0.5 * (ulp + urp) - 0.5 * (ulm + urm)) / dx1c;
-  }
-  for (int v = 0; v < a.ns; ++v) {
-    double const* sv = s + v * r.sv;
-    double rl, rr;
-    face_ratio(rho + c, sv + c, a.st, rl, rr);
-    // the upwind solver (riemann.cpp:164-171)
-    double ui = m > 0. ? 1. : 0.;
-    double F = m * (ui * rl + (1. - ui) * rr);
-    if (cov) {
-      // HydroImpl::scalar_flux_covariance (hydro_forward.cpp:233-297):
-      // sigma1^2 rho D1[u_n] D1[q], q the mean of the face ratios
-      double lm, rm, lp, rp;
-      face_ratio(rho + c - 1, sv + c - 1, a.st, lm, rm);
-      face_ratio(rho + c + 1, sv + c + 1, a.st, lp, rp);
-      double qm = 0.5 * (lm * 1. + rm * 1.), qp = 0.5 * (lp * 1. + rp * 1.);
-      double d1q = (qp - qm) / dx1c;
-      F = F + a.s2 * crho * d1un * d1q;
-    }
-    // boundary.prescribed: every tracer row is 0 on a wall face
-    sflux[v * r.sv + c] = wall ? 0. : F;
-  }
+  ScalarFluxBody<DIR, TF>::run(block_index(), grid_size(), w, s, hflux, x1v, sflux, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -223,55 +236,70 @@ __device__ __forceinline__ double theta_of(double x, double out, double vol, dou
 }
 
 template <typename TF>
+struct ScalarThetaBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w,
+                                             double const* __restrict__ s,
+                                             TF const* __restrict__ h1, TF const* __restrict__ h2,
+                                             TF const* __restrict__ h3,
+                                             double const* __restrict__ f1,
+                                             double const* __restrict__ f2,
+                                             double const* __restrict__ f3,
+                                             double* __restrict__ theta, ThetaArgs a) {
+    Range3 const& r = a.r;
+    int64_t tot = int64_t(r.nc1) * r.nc2 * r.nc3;
+    int64_t c = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (c >= tot) return;
+    int i = int(c % r.nc1);
+    int64_t rest = c / r.nc1;
+    int j = int(rest % r.nc2);
+    int k = int(rest / r.nc2);
+    bool inner = i >= r.lo1 && i < r.lo1 + r.n1 && j >= r.lo2 && j < r.lo2 + r.n2 &&
+                 k >= r.lo3 && k < r.lo3 + r.n3;
+    if (!inner) {
+      for (int v = 0; v < a.ns; ++v) theta[v * r.sv + c] = 1.;
+      return;
+    }
+    double dt = dt_at(a.dt_dev, a.dt, r, k);
+    TF const* hf[3] = {h1, h2, h3};
+    double const* sf[3] = {f1, f2, f3};
+    // the dry air's theta from the mass fluxes
+    double rho = w[IDN * r.sv + c];
+    double outr = drain_of(a, c, [&](int d, int64_t o) { return double(hf[d][IDN * r.sv + o]); });
+    double td = theta_of(rho, outr, a.vol, a.margin, dt);
+    for (int v = 0; v < a.ns; ++v) {
+      int64_t vo = v * r.sv;
+      double x = s[vo + c];
+      double outs = drain_of(a, c, [&](int d, int64_t o) { return sf[d][vo + o]; });
+      double th = theta_of(x, outs, a.vol, a.margin, dt);
+      // flux_positivity_share (flux_positivity.cpp:80-88): q = s / rho, the
+      // neighbours along x1, then x2 and x3 when resolved
+      auto q = [&](int64_t o) { return s[vo + o] / w[IDN * r.sv + o]; };
+      double qc = q(c);
+      // a single-cell axis rolls onto itself: the cell's own q
+      double low = r.nc1 > 1 ? nmin(q(c - 1), q(c + 1)) : nmin(qc, qc);
+      if (r.nc2 > 1) low = nmin(low, nmin(q(c - r.s2), q(c + r.s2)));
+      if (r.nc3 > 1) low = nmin(low, nmin(q(c - r.s3), q(c + r.s3)));
+      double phi = 0.;
+      if (qc > 0.) {
+        double t = low / qc;
+        phi = t != t ? t : (t < 0. ? 0. : (t > 1. ? 1. : t));
+      }
+      // flux_positivity_relax (flux_positivity.cpp:143-149)
+      if (td < 1. && td > 0.) th = th + phi * (nmin(th / td, 1.) - th);
+      theta[vo + c] = th;
+    }
+  }
+};
+
+template <typename TF>
 __global__ void __launch_bounds__(256)
     scalar_theta_kernel(double const* __restrict__ w, double const* __restrict__ s,
                         TF const* __restrict__ h1, TF const* __restrict__ h2,
                         TF const* __restrict__ h3, double const* __restrict__ f1,
                         double const* __restrict__ f2, double const* __restrict__ f3,
                         double* __restrict__ theta, ThetaArgs a) {
-  Range3 const& r = a.r;
-  int64_t tot = int64_t(r.nc1) * r.nc2 * r.nc3;
-  int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= tot) return;
-  int i = int(c % r.nc1);
-  int64_t rest = c / r.nc1;
-  int j = int(rest % r.nc2);
-  int k = int(rest / r.nc2);
-  bool inner = i >= r.lo1 && i < r.lo1 + r.n1 && j >= r.lo2 && j < r.lo2 + r.n2 &&
-               k >= r.lo3 && k < r.lo3 + r.n3;
-  if (!inner) {
-    for (int v = 0; v < a.ns; ++v) theta[v * r.sv + c] = 1.;
-    return;
-  }
-  double dt = dt_at(a.dt_dev, a.dt, r, k);
-  TF const* hf[3] = {h1, h2, h3};
-  double const* sf[3] = {f1, f2, f3};
-  // the dry air's theta from the mass fluxes
-  double rho = w[IDN * r.sv + c];
-  double outr = drain_of(a, c, [&](int d, int64_t o) { return double(hf[d][IDN * r.sv + o]); });
-  double td = theta_of(rho, outr, a.vol, a.margin, dt);
-  for (int v = 0; v < a.ns; ++v) {
-    int64_t vo = v * r.sv;
-    double x = s[vo + c];
-    double outs = drain_of(a, c, [&](int d, int64_t o) { return sf[d][vo + o]; });
-    double th = theta_of(x, outs, a.vol, a.margin, dt);
-    // flux_positivity_share (flux_positivity.cpp:80-88): q = s / rho, the
-    // neighbours along x1, then x2 and x3 when resolved
-    auto q = [&](int64_t o) { return s[vo + o] / w[IDN * r.sv + o]; };
-    double qc = q(c);
-    // a single-cell axis rolls onto itself: the cell's own q
-    double low = r.nc1 > 1 ? nmin(q(c - 1), q(c + 1)) : nmin(qc, qc);
-    if (r.nc2 > 1) low = nmin(low, nmin(q(c - r.s2), q(c + r.s2)));
-    if (r.nc3 > 1) low = nmin(low, nmin(q(c - r.s3), q(c + r.s3)));
-    double phi = 0.;
-    if (qc > 0.) {
-      double t = low / qc;
-      phi = t != t ? t : (t < 0. ? 0. : (t > 1. ? 1. : t));
-    }
-    // flux_positivity_relax (flux_positivity.cpp:143-149)
-    if (td < 1. && td > 0.) th = th + phi * (nmin(th / td, 1.) - th);
-    theta[vo + c] = th;
-  }
+  ScalarThetaBody<TF>::run(block_index(), grid_size(), w, s, h1, h2, h3, f1, f2, f3, theta, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -288,108 +316,142 @@ struct UpdArgs {
 };
 
 template <bool COMBINE>
+struct ScalarUpdateBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ f1,
+                                             double const* __restrict__ f2,
+                                             double const* __restrict__ f3,
+                                             double const* __restrict__ theta,
+                                             double const* __restrict__ s0, double* __restrict__ s,
+                                             double* __restrict__ ds_out, UpdArgs a) {
+    Range3 const& r = a.r;
+    int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
+    int64_t idx = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (idx >= tot) return;
+    int i = r.lo1 + int(idx % r.n1);
+    int64_t rest = idx / r.n1;
+    int j = r.lo2 + int(rest % r.n2);
+    int k = r.lo3 + int(rest / r.n2);
+    int64_t c = k * r.s3 + j * r.s2 + i;
+    double dt = dt_at(a.dt_dev, a.dt, r, k);
+    double const* fl[3] = {f1, f2, f3};
+    int64_t st[3] = {1, r.s2, r.s3};
+    for (int v = 0; v < a.ns; ++v) {
+      int64_t vo = v * r.sv;
+      // CoordinateImpl::divergence in direction order, each face flux scaled
+      // by its donor's theta (flux_positivity_factor: the lower cell where
+      // F > 0)
+      double div = 0.;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (!a.act[d]) continue;
+        double lo = fl[d][vo + c], hi = fl[d][vo + c + st[d]];
+        if (theta) {
+          lo = lo * (lo > 0. ? theta[vo + c - st[d]] : theta[vo + c]);
+          hi = hi * (hi > 0. ? theta[vo + c] : theta[vo + c + st[d]]);
+        }
+        div = div + (a.area[d] * hi - a.area[d] * lo);
+      }
+      double ds = -dt * (div / a.vol);
+      if constexpr (COMBINE)
+        s[vo + c] = a.w0 * s0[vo + c] + a.w1 * s[vo + c] + a.w2 * ds;
+      else
+        ds_out[vo + c] = ds;
+    }
+  }
+};
+
+template <bool COMBINE>
 __global__ void __launch_bounds__(256)
     scalar_update_kernel(double const* __restrict__ f1, double const* __restrict__ f2,
                          double const* __restrict__ f3, double const* __restrict__ theta,
                          double const* __restrict__ s0, double* __restrict__ s,
                          double* __restrict__ ds_out, UpdArgs a) {
-  Range3 const& r = a.r;
-  int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
-  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= tot) return;
-  int i = r.lo1 + int(idx % r.n1);
-  int64_t rest = idx / r.n1;
-  int j = r.lo2 + int(rest % r.n2);
-  int k = r.lo3 + int(rest / r.n2);
-  int64_t c = k * r.s3 + j * r.s2 + i;
-  double dt = dt_at(a.dt_dev, a.dt, r, k);
-  double const* fl[3] = {f1, f2, f3};
-  int64_t st[3] = {1, r.s2, r.s3};
-  for (int v = 0; v < a.ns; ++v) {
-    int64_t vo = v * r.sv;
-    // CoordinateImpl::divergence in direction order, each face flux scaled
-    // by its donor's theta (flux_positivity_factor: the lower cell where
-    // F > 0)
-    double div = 0.;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (!a.act[d]) continue;
-      double lo = fl[d][vo + c], hi = fl[d][vo + c + st[d]];
-      if (theta) {
-        lo = lo * (lo > 0. ? theta[vo + c - st[d]] : theta[vo + c]);
-        hi = hi * (hi > 0. ? theta[vo + c] : theta[vo + c + st[d]]);
-      }
-      div = div + (a.area[d] * hi - a.area[d] * lo);
-    }
-    double ds = -dt * (div / a.vol);
-    if constexpr (COMBINE)
-      s[vo + c] = a.w0 * s0[vo + c] + a.w1 * s[vo + c] + a.w2 * ds;
-    else
-      ds_out[vo + c] = ds;
-  }
+  ScalarUpdateBody<COMBINE>::run(block_index(), grid_size(), f1, f2, f3, theta, s0, s, ds_out, a);
 }
 
 // one thread per (row, column): P = r_donor M through the face below each
 // cell, the donor cap sequentially upward (donor_cap_cuda), then
 // ds + (P - P_above) / V and the combine, cell by cell as the cap passes
+struct ScalarTransferBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ moved,
+                                             double const* __restrict__ rr,
+                                             double const* __restrict__ s0, double* __restrict__ s,
+                                             double const* __restrict__ ds, UpdArgs a) {
+    Range3 const& r = a.r;
+    int64_t ncol = int64_t(r.n2) * r.n3;
+    int64_t t = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (t >= ncol * a.ns) return;
+    int v = int(t / ncol);
+    int64_t col = t - v * ncol;
+    int j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
+    int64_t row = k * r.s3 + j * r.s2;
+    int64_t vo = v * r.sv;
+    double const V = a.vol;
+    int const il = r.lo1, n = r.n1;
+    double avail = 0., Pprev = 0., sprev = 0., dprev = 0., s0prev = 0.;
+    for (int l = 0; l < n; ++l) {
+      int64_t c = row + il + l;
+      double M = moved[c];
+      double P = (M > 0. ? rr[vo + c - 1] : rr[vo + c]) * M;
+      double sc = s[vo + c], dc = ds[vo + c], s0c = s0[vo + c];
+      double x = sc + dc;
+      double held = (x < 0. ? 0. : x) * V;
+      if (l == 0) {
+        avail = held;
+      } else {
+        double cap = P < avail ? P : avail;
+        P = cap > -held ? cap : -held;
+        avail = held + P;
+        // the cell below is final: its transfer in (Pprev) and out (P)
+        double d = dprev + (Pprev - P) / V;
+        s[vo + c - 1] = a.w0 * s0prev + a.w1 * sprev + a.w2 * d;
+      }
+      Pprev = P;
+      sprev = sc;
+      dprev = dc;
+      s0prev = s0c;
+    }
+    // the top cell: nothing leaves through the column's top face
+    double d = dprev + (Pprev - 0.) / V;
+    s[vo + row + il + n - 1] = a.w0 * s0prev + a.w1 * sprev + a.w2 * d;
+  }
+};
+
 __global__ void __launch_bounds__(128)
     scalar_transfer_kernel(double const* __restrict__ moved, double const* __restrict__ rr,
                            double const* __restrict__ s0, double* __restrict__ s,
                            double const* __restrict__ ds, UpdArgs a) {
-  Range3 const& r = a.r;
-  int64_t ncol = int64_t(r.n2) * r.n3;
-  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= ncol * a.ns) return;
-  int v = int(t / ncol);
-  int64_t col = t - v * ncol;
-  int j = r.lo2 + int(col % r.n2), k = r.lo3 + int(col / r.n2);
-  int64_t row = k * r.s3 + j * r.s2;
-  int64_t vo = v * r.sv;
-  double const V = a.vol;
-  int const il = r.lo1, n = r.n1;
-  double avail = 0., Pprev = 0., sprev = 0., dprev = 0., s0prev = 0.;
-  for (int l = 0; l < n; ++l) {
-    int64_t c = row + il + l;
-    double M = moved[c];
-    double P = (M > 0. ? rr[vo + c - 1] : rr[vo + c]) * M;
-    double sc = s[vo + c], dc = ds[vo + c], s0c = s0[vo + c];
-    double x = sc + dc;
-    double held = (x < 0. ? 0. : x) * V;
-    if (l == 0) {
-      avail = held;
-    } else {
-      double cap = P < avail ? P : avail;
-      P = cap > -held ? cap : -held;
-      avail = held + P;
-      // the cell below is final: its transfer in (Pprev) and out (P)
-      double d = dprev + (Pprev - P) / V;
-      s[vo + c - 1] = a.w0 * s0prev + a.w1 * sprev + a.w2 * d;
-    }
-    Pprev = P;
-    sprev = sc;
-    dprev = dc;
-    s0prev = s0c;
-  }
-  // the top cell: nothing leaves through the column's top face
-  double d = dprev + (Pprev - 0.) / V;
-  s[vo + row + il + n - 1] = a.w0 * s0prev + a.w1 * sprev + a.w2 * d;
+  ScalarTransferBody::run(block_index(), grid_size(), moved, rr, s0, s, ds, a);
 }
+
+template <typename TU>
+struct ScalarPrimBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             TU const* __restrict__ u,
+                                             float const* __restrict__ comp,
+                                             double const* __restrict__ s, double* __restrict__ rr,
+                                             int ns, Range3 r) {
+    int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
+    int64_t idx = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (idx >= tot) return;
+    int i = r.lo1 + int(idx % r.n1);
+    int64_t rest = idx / r.n1;
+    int j = r.lo2 + int(rest % r.n2);
+    int k = r.lo3 + int(rest / r.n2);
+    int64_t c = k * r.s3 + j * r.s2 + i;
+    double rho = double(u[IDN * r.sv + c]);
+    if (comp) rho = rho + double(comp[IDN * r.sv + c]);
+    for (int v = 0; v < ns; ++v) rr[v * r.sv + c] = s[v * r.sv + c] / rho;
+  }
+};
 
 template <typename TU>
 __global__ void __launch_bounds__(256)
     scalar_prim_kernel(TU const* __restrict__ u, float const* __restrict__ comp,
                        double const* __restrict__ s, double* __restrict__ rr, int ns, Range3 r) {
-  int64_t tot = int64_t(r.n1) * r.n2 * r.n3;
-  int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= tot) return;
-  int i = r.lo1 + int(idx % r.n1);
-  int64_t rest = idx / r.n1;
-  int j = r.lo2 + int(rest % r.n2);
-  int k = r.lo3 + int(rest / r.n2);
-  int64_t c = k * r.s3 + j * r.s2 + i;
-  double rho = double(u[IDN * r.sv + c]);
-  if (comp) rho = rho + double(comp[IDN * r.sv + c]);
-  for (int v = 0; v < ns; ++v) rr[v * r.sv + c] = s[v * r.sv + c] / rho;
+  ScalarPrimBody<TU>::run(block_index(), grid_size(), u, comp, s, rr, ns, r);
 }
 
 int scalar_checks(const snapy_grid* g, const snapy_physics* p, int ns, char const* what) {
@@ -436,11 +498,11 @@ int snapy_scalar_flux(int dir, const snapy_grid* g, const snapy_physics* p, int 
     using TF = decltype(tf);
     TF const* h = static_cast<TF const*>(hflux);
     if (dir == 1)
-      hipLaunchKernelGGL((scalar_flux_kernel<1, TF>), grid, dim3(256), 0, st, w, s, h, x1tab, sflux, a);
+      launch<ScalarFluxBody<1, TF>, 256, 1>((scalar_flux_kernel<1, TF>), grid, dim3(256), 0, st, w, s, h, x1tab, sflux, a);
     else if (dir == 2)
-      hipLaunchKernelGGL((scalar_flux_kernel<2, TF>), grid, dim3(256), 0, st, w, s, h, x1tab, sflux, a);
+      launch<ScalarFluxBody<2, TF>, 256, 1>((scalar_flux_kernel<2, TF>), grid, dim3(256), 0, st, w, s, h, x1tab, sflux, a);
     else
-      hipLaunchKernelGGL((scalar_flux_kernel<3, TF>), grid, dim3(256), 0, st, w, s, h, x1tab, sflux, a);
+      launch<ScalarFluxBody<3, TF>, 256, 1>((scalar_flux_kernel<3, TF>), grid, dim3(256), 0, st, w, s, h, x1tab, sflux, a);
   };
   if (fdtype == SNAPY_F64)
     go(double(0));
@@ -474,11 +536,11 @@ int snapy_scalar_theta(const snapy_grid* g, const snapy_physics* p, int ns, cons
   dim3 grid(blocks_for(tot, 256));
   hipStream_t st = as_stream(stream);
   if (fdtype == SNAPY_F64)
-    hipLaunchKernelGGL(scalar_theta_kernel<double>, grid, dim3(256), 0, st, w, s,
+    launch<ScalarThetaBody<double>, 256, 1>(scalar_theta_kernel<double>, grid, dim3(256), 0, st, w, s,
                        static_cast<double const*>(hflux[0]), static_cast<double const*>(hflux[1]),
                        static_cast<double const*>(hflux[2]), sflux[0], sflux[1], sflux[2], theta, a);
   else
-    hipLaunchKernelGGL(scalar_theta_kernel<float>, grid, dim3(256), 0, st, w, s,
+    launch<ScalarThetaBody<float>, 256, 1>(scalar_theta_kernel<float>, grid, dim3(256), 0, st, w, s,
                        static_cast<float const*>(hflux[0]), static_cast<float const*>(hflux[1]),
                        static_cast<float const*>(hflux[2]), sflux[0], sflux[1], sflux[2], theta, a);
   if (int e = launch_status("scalar_theta")) return e;
@@ -512,14 +574,14 @@ int snapy_scalar_update(const snapy_grid* g, const snapy_physics* p, int ns,
   int64_t tot = int64_t(g->nx1) * g->nx2 * g->nx3;
   hipStream_t st = as_stream(stream);
   if (!moved) {
-    hipLaunchKernelGGL(scalar_update_kernel<true>, dim3(blocks_for(tot, 256)), dim3(256), 0, st,
+    launch<ScalarUpdateBody<true>, 256, 1>(scalar_update_kernel<true>, dim3(blocks_for(tot, 256)), dim3(256), 0, st,
                        sflux[0], sflux[1], sflux[2], theta, s0, s, nullptr, a);
   } else {
-    hipLaunchKernelGGL(scalar_update_kernel<false>, dim3(blocks_for(tot, 256)), dim3(256), 0, st,
+    launch<ScalarUpdateBody<false>, 256, 1>(scalar_update_kernel<false>, dim3(blocks_for(tot, 256)), dim3(256), 0, st,
                        sflux[0], sflux[1], sflux[2], theta, s0, s, ds, a);
     if (int e = launch_status("scalar_update")) return e;
     int64_t cols = int64_t(g->nx2) * g->nx3 * ns;
-    hipLaunchKernelGGL(scalar_transfer_kernel, dim3(blocks_for(cols, 128)), dim3(128), 0, st, moved,
+    launch<ScalarTransferBody, 128, 1>(scalar_transfer_kernel, dim3(blocks_for(cols, 128)), dim3(128), 0, st, moved,
                        r, s0, s, ds, a);
   }
   if (int e = launch_status("scalar_update")) return e;
@@ -536,10 +598,10 @@ int snapy_scalar_prim(const snapy_grid* g, const snapy_physics* p, int ns, int u
   hipStream_t st = as_stream(stream);
   if (udtype == SNAPY_F64) {
     if (comp) return fail(-10, "scalar_prim: a compensation goes with the FP32 state");
-    hipLaunchKernelGGL(scalar_prim_kernel<double>, grid, dim3(256), 0, st,
+    launch<ScalarPrimBody<double>, 256, 1>(scalar_prim_kernel<double>, grid, dim3(256), 0, st,
                        static_cast<double const*>(u), nullptr, s, r, ns, ranges(*g));
   } else if (udtype == SNAPY_F32) {
-    hipLaunchKernelGGL(scalar_prim_kernel<float>, grid, dim3(256), 0, st,
+    launch<ScalarPrimBody<float>, 256, 1>(scalar_prim_kernel<float>, grid, dim3(256), 0, st,
                        static_cast<float const*>(u), comp, s, r, ns, ranges(*g));
   } else {
     return fail(-10, "unknown dtype");

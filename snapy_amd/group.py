@@ -36,8 +36,8 @@ _TORCH_CALLS = ("vic_status_clear",)
 
 def groupable(block) -> bool:
     """A block the group can take: a single block (no member axis) of an
-    ideal gas without scalars, on the GPU."""
-    return block.members == 1 and block.nscalar == 0 and block.nvar == 5
+    ideal gas (tracers included)."""
+    return block.members == 1 and block.nvar == 5
 
 
 def group_key(block):
@@ -60,7 +60,7 @@ class BlockGroup:
             raise ValueError("BlockGroup: no blocks")
         for blk, _ in self.members:
             if not groupable(blk):
-                raise ValueError("BlockGroup: a block of members, or with scalars, is not grouped")
+                raise ValueError("BlockGroup: a block of members, or not of an ideal gas, is not grouped")
             if blk.device.type != "cuda" or blk._freg is None:
                 raise ValueError("BlockGroup: initialize every block on the GPU first")
         keys = {group_key(blk) for blk, _ in self.members}
@@ -137,10 +137,13 @@ class BlockGroup:
                 blk.device_time_step(vars)
         for k in range(3):
             if k == 0:
-                # the RK backup and the implicit solve's status latch, per
-                # block (MeshBlock._launch_stage and its first call)
-                for blk, _ in self.members:
+                # the RK backups (state and tracers) and the implicit
+                # solve's status latch, per block (MeshBlock._launch_stage
+                # and its first call)
+                for blk, vars in self.members:
                     blk._state.backup()
+                    if blk.nscalar:
+                        blk._scal["s0"].copy_(vars["scalar_s"])
                     if blk._vic is not None:
                         blk._vic["status"].zero_()
             _capi.check(lib.snapy_group_launch(self._groups[k], s), f"group_launch({k})")

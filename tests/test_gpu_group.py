@@ -162,3 +162,40 @@ def test_grouped_3d_blocks_equal_each_blocks_own(kind, precision):
         for a, b in zip(got, ref):
             assert torch.equal(a, b), (kind, precision, blk.options.coord.nx1)
     grp.release()
+
+
+@pytest.mark.parametrize("precision,scheme", [("float64", 9), ("float32", 1)])
+def test_grouped_tracer_blocks_equal_each_blocks_own(precision, scheme):
+    """Blocks with two tracers (the covariance, the positivity limiter and
+    the donor-capped implicit transfer) of different sizes, grouped."""
+    from oracle import fv
+    from tests.test_gpu_scalar import _pair
+
+    grids = [fv.Grid(16, 12, 1, 0.0, 6400.0, -6400.0, 6400.0, 0.0, 100.0),
+             fv.Grid(24, 20, 1, 0.0, 6400.0, -6400.0, 6400.0, 0.0, 100.0),
+             fv.Grid(16, 16, 1, 0.0, 4800.0, -4800.0, 4800.0, 0.0, 100.0)]
+    print(f"group of tracer blocks, {precision}, scheme {scheme}", flush=True)
+    nsteps = 3
+
+    def tracer_state(blk, vars):
+        return _state(blk, vars) + [vars["scalar_s"].clone(), vars["scalar_r"].clone()]
+
+    solo = []
+    for g in grids:
+        _, blk, vars = _pair(g, ("reflecting",) * 6, scheme, True, precision=precision)
+        for _ in range(nsteps):
+            dt = blk.device_time_step(vars)
+            for s in range(3):
+                blk.forward(vars, dt, s)
+        torch.cuda.synchronize()
+        solo.append(tracer_state(blk, vars))
+        blk.release()
+    members = [_pair(g, ("reflecting",) * 6, scheme, True, precision=precision)[1:] for g in grids]
+    grp = BlockGroup(members)
+    for _ in range(nsteps):
+        grp.step()
+    torch.cuda.synchronize()
+    for (blk, vars), ref in zip(members, solo):
+        for a, b in zip(tracer_state(blk, vars), ref):
+            assert torch.equal(a, b), (precision, scheme, blk.options.coord.nx2)
+    grp.release()
