@@ -483,6 +483,84 @@ def density_current_leg(steps=50, warmup=3, members=1, seed=2025):
     return out
 
 
+def grouped_leg(steps=20, warmup=2, precision="float32", seed=2025):
+    """The serving mix's density currents as one group: nine blocks, every
+    (nx, nz) of nx in {256, 512, 1024} and nz in {32, 64, 128} (dT from the
+    seed), stepped with one launch per kernel over all nine
+    (snapy_amd.group.BlockGroup), against the same blocks stepped one after
+    another on their own captured stages.  Bit-identical either way
+    (tests/test_gpu_group.py)."""
+    import numpy as np
+    import torch
+
+    from snapy_amd import problems, serving
+    from snapy_amd.group import BlockGroup
+    from snapy_amd.meshblock import MeshBlock
+
+    rng = np.random.default_rng(seed)
+    shapes = [(nx, nz) for nz in (32, 64, 128) for nx in (256, 512, 1024)]
+
+    def blocks(execution):
+        out = []
+        for nx, nz in shapes:
+            op = serving._options("density_current", {"nx": nx, "nz": nz, "dT": -15.0, "precision": precision,
+                                                     "execution": execution})
+            blk = MeshBlock(op)
+            blk.to("cuda")
+            w = problems.density_current(blk, dT=float(rng.uniform(-20.0, -5.0)), grav=serving.GRAV)
+            vars, _ = blk.initialize({"hydro_w": w.cuda()})
+            out.append((blk, vars))
+        return out
+
+    cells = sum(nx * nz for nx, nz in shapes)
+    solo = blocks("hip-graph")
+
+    def solo_steps(n):
+        for _ in range(n):
+            for blk, vars in solo:
+                dt = blk.device_time_step(vars)
+                for k in range(3):
+                    blk.forward(vars, dt, k)
+
+    solo_steps(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solo_steps(steps)
+    torch.cuda.synchronize()
+    el_solo = time.perf_counter() - t0
+    for blk, _ in solo:
+        blk.release()
+    members = blocks("eager")
+    grp = BlockGroup(members)
+    for _ in range(warmup):
+        grp.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        grp.step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    finite = all(bool(torch.isfinite(v["hydro_u"]).all()) for _, v in members)
+    launches = {k: grp.launches(k) for k in ("dt", 0, 1, 2)}
+    grp.release()
+    for blk, _ in members:
+        blk.release()
+    return {
+        "workload": "density_current_group_9_grids",
+        "grids": [f"{nx}x{nz}" for nx, nz in shapes],
+        "value": cells * 3 * steps / el,
+        "unit": "cell-updates/s",
+        "steps": steps,
+        "ms_per_step": el / steps * 1e3,
+        "one_after_another": {"value": cells * 3 * steps / el_solo, "ms_per_step": el_solo / steps * 1e3,
+                              "form": "each block on its own captured stages, in turn"},
+        "speedup": el_solo / el,
+        "dtype": "f32" if precision == "float32" else "f64",
+        "launches_per_phase": launches,
+        "state_finite": finite,
+    }
+
+
 def cubed_leg(steps=50, warmup=3, members=32, seed0=1000, precision="float32"):
     """The batched cubed-sphere shallow-water workload: `members` six-panel
     meshes of 192 x 192 cells per panel, each seeded with its own Gaussian
@@ -765,6 +843,13 @@ def main():
         print(f"# density current ensemble x{args.ensemble_members} {ens['value']:.3e} cell-updates/s",
               file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
+    grouped = None
+    if not args.no_other_scheme and args.workload == "convection3d":
+        print("# grouped density currents", file=sys.stderr, flush=True)
+        grouped = grouped_leg()
+        print(f"# grouped {grouped['value']:.3e} cell-updates/s ({grouped['speedup']:.2f}x one after another)",
+              file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
     cubed = None
     if not args.no_cubed and args.workload == "convection3d":
         print("# batched cubed-sphere shallow water", file=sys.stderr, flush=True)
@@ -805,6 +890,8 @@ def main():
         line["other_implicit_scheme"] = other
     if c2 is not None:
         line["density_current"] = c2
+    if grouped is not None:
+        line["grouped"] = grouped
     if plain is not None:
         line["fp64_plain"] = plain
         line["speedup_vs_fp64_plain"] = value / plain["value"]
