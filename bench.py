@@ -487,9 +487,9 @@ def grouped_leg(steps=20, warmup=2, precision="float32", seed=2025):
     """The serving mix's density currents as one group: nine blocks, every
     (nx, nz) of nx in {256, 512, 1024} and nz in {32, 64, 128} (dT from the
     seed), stepped with one launch per kernel over all nine
-    (snapy_amd.group.BlockGroup), against the same blocks stepped one after
-    another on their own captured stages.  Bit-identical either way
-    (tests/test_gpu_group.py)."""
+    (snapy_amd.group.BlockGroup, its step captured), against the same blocks
+    stepped one after another on their own captured stages.  Bit-identical
+    either way (tests/test_gpu_group.py)."""
     import numpy as np
     import torch
 
@@ -531,7 +531,7 @@ def grouped_leg(steps=20, warmup=2, precision="float32", seed=2025):
     for blk, _ in solo:
         blk.release()
     members = blocks("eager")
-    grp = BlockGroup(members)
+    grp = BlockGroup(members, capture=True)
     for _ in range(warmup):
         grp.step()
     torch.cuda.synchronize()

@@ -52,10 +52,14 @@ def group_key(block):
 
 class BlockGroup:
     """Blocks (with their vars) advanced together, one launch per kernel
-    for all of them per step."""
+    for all of them per step.  capture: from the second step on, a step is
+    one captured hipGraph replayed (the group's launches and the per-block
+    backups), once every block's primitives are stale, as after a step."""
 
-    def __init__(self, members):
+    def __init__(self, members, capture: bool = False):
         self.members = list(members)
+        self.capture = capture
+        self._graph = None
         if not self.members:
             raise ValueError("BlockGroup: no blocks")
         for blk, _ in self.members:
@@ -128,6 +132,21 @@ class BlockGroup:
         """One RK3 step of every block on the current stream: each block's
         CFL dt (grouped when every block's primitives are stale, as they are
         after a step; else each block's own), then the three stages."""
+        stale = all(blk._w_stale for blk, _ in self.members)
+        if self._graph is not None and stale:
+            self._graph.replay()
+            return
+        if self.capture and stale:
+            g = torch.cuda.CUDAGraph()
+            torch.cuda.synchronize(self.device)
+            with torch.cuda.graph(g):
+                self._step()
+            self._graph = g
+            g.replay()
+            return
+        self._step()
+
+    def _step(self) -> None:
         s = _capi.stream_of(self.device)
         lib = self._lib
         if all(blk._w_stale for blk, _ in self.members):
@@ -154,6 +173,7 @@ class BlockGroup:
         """Free the group's device records (after its launches finished)."""
         if sync and self.device.type == "cuda":
             torch.cuda.synchronize(self.device)
+        self._graph = None
         for g in self._groups.values():
             if g:
                 self._lib.snapy_group_free(g)

@@ -34,9 +34,11 @@ def _state(blk, vars):
     return out
 
 
-@pytest.mark.parametrize("precision,scheme", [("float64", 9), ("float32", 9), ("float64", 1), ("float32", 1)])
-def test_grouped_steps_equal_each_blocks_own(precision, scheme):
-    print(f"group of {SIZES}, {precision}, scheme {scheme}", flush=True)
+@pytest.mark.parametrize("precision,scheme,capture", [("float64", 9, False), ("float32", 9, False),
+                                                    ("float64", 1, False), ("float32", 1, False),
+                                                    ("float32", 9, True), ("float64", 1, True)])
+def test_grouped_steps_equal_each_blocks_own(precision, scheme, capture):
+    print(f"group of {SIZES}, {precision}, scheme {scheme}, captured {capture}", flush=True)
     nsteps = 4
     solo = []
     for nx, nz, dT in SIZES:
@@ -49,7 +51,7 @@ def test_grouped_steps_equal_each_blocks_own(precision, scheme):
         solo.append(_state(blk, vars))
         blk.release()
     members = [_block(nx, nz, dT, precision, scheme) for nx, nz, dT in SIZES]
-    grp = BlockGroup(members)
+    grp = BlockGroup(members, capture=capture)
     assert grp.launches(0) > 5
     for _ in range(nsteps):
         grp.step()
