@@ -35,6 +35,7 @@
 #include <cstring>
 
 #include "common.hpp"
+#include "group.hpp"
 #include "point.hpp"
 
 namespace snapy {
@@ -110,42 +111,63 @@ __device__ __forceinline__ void cs_put(float* u, float* c, int64_t o, double v) 
 }
 
 template <typename TU, typename TW>
+struct CsCons2PrimBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             TU const* __restrict__ u, float const* __restrict__ uc,
+                                             TW* __restrict__ w, double const* __restrict__ cos_c,
+                                             Dims d, int64_t ncell) {
+    int64_t t = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (t >= ncell) return;
+    int64_t b = t / d.plane, c = t % d.plane;
+    int64_t o = b * d.block + c;
+    TW* wb = w + o;
+    double h = cs_val(u, uc, o);
+    double v1 = cs_val(u, uc, o + d.plane) / h, v2 = cs_val(u, uc, o + 2 * d.plane) / h,
+           v3 = cs_val(u, uc, o + 3 * d.plane) / h;
+    double cth = cos_c[c];
+    double sth2 = 1. - cth * cth;
+    wb[0] = TW(h);
+    wb[d.plane] = TW(v1);
+    wb[2 * d.plane] = TW(v2 / sth2 - v3 * cth / sth2);
+    wb[3 * d.plane] = TW(-v2 * cth / sth2 + v3 / sth2);
+  }
+};
+
+template <typename TU, typename TW>
 __global__ void __launch_bounds__(256)
     cs_cons2prim_kernel(TU const* __restrict__ u, float const* __restrict__ uc,
                         TW* __restrict__ w, double const* __restrict__ cos_c, Dims d,
                         int64_t ncell) {
-  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= ncell) return;
-  int64_t b = t / d.plane, c = t % d.plane;
-  int64_t o = b * d.block + c;
-  TW* wb = w + o;
-  double h = cs_val(u, uc, o);
-  double v1 = cs_val(u, uc, o + d.plane) / h, v2 = cs_val(u, uc, o + 2 * d.plane) / h,
-         v3 = cs_val(u, uc, o + 3 * d.plane) / h;
-  double cth = cos_c[c];
-  double sth2 = 1. - cth * cth;
-  wb[0] = TW(h);
-  wb[d.plane] = TW(v1);
-  wb[2 * d.plane] = TW(v2 / sth2 - v3 * cth / sth2);
-  wb[3 * d.plane] = TW(-v2 * cth / sth2 + v3 / sth2);
+  CsCons2PrimBody<TU, TW>::run(block_index(), grid_size(), u, uc, w, cos_c, d, ncell);
 }
+
+template <typename TU>
+struct CsPrim2ConsBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w, TU* __restrict__ u,
+                                             float* __restrict__ uc,
+                                             double const* __restrict__ cos_c, Dims d,
+                                             int64_t ncell) {
+    int64_t t = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (t >= ncell) return;
+    int64_t b = t / d.plane, c = t % d.plane;
+    int64_t o = b * d.block + c;
+    double const* wb = w + o;
+    double h = wb[0];
+    double m1 = wb[d.plane] * h, m2 = wb[2 * d.plane] * h, m3 = wb[3 * d.plane] * h;
+    double cth = cos_c[c];
+    cs_put(u, uc, o, h);
+    cs_put(u, uc, o + d.plane, m1);
+    cs_put(u, uc, o + 2 * d.plane, m2 + m3 * cth);
+    cs_put(u, uc, o + 3 * d.plane, m3 + m2 * cth);
+  }
+};
 
 template <typename TU>
 __global__ void __launch_bounds__(256)
     cs_prim2cons_kernel(double const* __restrict__ w, TU* __restrict__ u, float* __restrict__ uc,
                         double const* __restrict__ cos_c, Dims d, int64_t ncell) {
-  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= ncell) return;
-  int64_t b = t / d.plane, c = t % d.plane;
-  int64_t o = b * d.block + c;
-  double const* wb = w + o;
-  double h = wb[0];
-  double m1 = wb[d.plane] * h, m2 = wb[2 * d.plane] * h, m3 = wb[3 * d.plane] * h;
-  double cth = cos_c[c];
-  cs_put(u, uc, o, h);
-  cs_put(u, uc, o + d.plane, m1);
-  cs_put(u, uc, o + 2 * d.plane, m2 + m3 * cth);
-  cs_put(u, uc, o + 3 * d.plane, m3 + m2 * cth);
+  CsPrim2ConsBody<TU>::run(block_index(), grid_size(), w, u, uc, cos_c, d, ncell);
 }
 
 // ---------------------------------------------------------------------------
@@ -163,52 +185,63 @@ __device__ __forceinline__ void matvec(double const* m, double const* v, double*
 // TU double: the FP64 state; float: the FP32 pair (u, uc), gathered and
 // interpolated in FP64 (the same arithmetic on the pair's values)
 template <typename TU>
+struct CsGhostBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim, TU* __restrict__ u,
+                                             float* __restrict__ uc,
+                                             int32_t const* __restrict__ idx,
+                                             double const* __restrict__ wt,
+                                             double const* __restrict__ mat, int64_t nent, Dims d) {
+    int64_t t = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (t >= nent * d.members) return;
+    int64_t e = t % nent;
+    int64_t m = t / nent;
+    int32_t const* ix = idx + 8 * e;
+    int npts = ix[7];
+    int64_t dst = (m * 6 + ix[0]) * d.block + ix[1];
+    int64_t src = (m * 6 + ix[2]) * d.block;
+    if (npts == 1) {
+      for (int v = 0; v < kRows; ++v) {
+        u[dst + v * d.plane] = u[src + v * d.plane + ix[3]];
+        if constexpr (sizeof(TU) == 4) uc[dst + v * d.plane] = uc[src + v * d.plane + ix[3]];
+      }
+      return;
+    }
+    double const* M = mat + 45 * e;
+    // each source point: phi, and its momentum to the geographic frame
+    double b[4][kRows];
+    for (int q = 0; q < npts; ++q) {
+      int64_t s = src + ix[3 + q];
+      double v[3] = {cs_val(u, uc, s + d.plane), cs_val(u, uc, s + 2 * d.plane),
+                     cs_val(u, uc, s + 3 * d.plane)};
+      b[q][0] = cs_val(u, uc, s);
+      matvec(M + 9 * q, v, &b[q][1]);
+    }
+    double g[kRows];
+    double const* wq = wt + 3 * e;
+    if (npts == 2) {
+      double x = wq[0];
+      for (int v = 0; v < kRows; ++v) g[v] = b[0][v] + x * (b[1][v] - b[0][v]);
+    } else {
+      double c0 = wq[0], c2 = wq[1], c3 = wq[2];
+      for (int v = 0; v < kRows; ++v)
+        g[v] = b[1][v] + c0 * (b[0][v] - b[1][v]) + c2 * (b[2][v] - b[1][v]) +
+               c3 * (b[3][v] - b[1][v]);
+    }
+    double o[3];
+    matvec(M + 36, &g[1], o);
+    cs_put(u, uc, dst, g[0]);
+    cs_put(u, uc, dst + d.plane, o[0]);
+    cs_put(u, uc, dst + 2 * d.plane, o[1]);
+    cs_put(u, uc, dst + 3 * d.plane, o[2]);
+  }
+};
+
+template <typename TU>
 __global__ void __launch_bounds__(256)
     cs_ghost_kernel(TU* __restrict__ u, float* __restrict__ uc, int32_t const* __restrict__ idx,
                     double const* __restrict__ wt, double const* __restrict__ mat,
                     int64_t nent, Dims d) {
-  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= nent * d.members) return;
-  int64_t e = t % nent;
-  int64_t m = t / nent;
-  int32_t const* ix = idx + 8 * e;
-  int npts = ix[7];
-  int64_t dst = (m * 6 + ix[0]) * d.block + ix[1];
-  int64_t src = (m * 6 + ix[2]) * d.block;
-  if (npts == 1) {
-    for (int v = 0; v < kRows; ++v) {
-      u[dst + v * d.plane] = u[src + v * d.plane + ix[3]];
-      if constexpr (sizeof(TU) == 4) uc[dst + v * d.plane] = uc[src + v * d.plane + ix[3]];
-    }
-    return;
-  }
-  double const* M = mat + 45 * e;
-  // each source point: phi, and its momentum to the geographic frame
-  double b[4][kRows];
-  for (int q = 0; q < npts; ++q) {
-    int64_t s = src + ix[3 + q];
-    double v[3] = {cs_val(u, uc, s + d.plane), cs_val(u, uc, s + 2 * d.plane),
-                   cs_val(u, uc, s + 3 * d.plane)};
-    b[q][0] = cs_val(u, uc, s);
-    matvec(M + 9 * q, v, &b[q][1]);
-  }
-  double g[kRows];
-  double const* wq = wt + 3 * e;
-  if (npts == 2) {
-    double x = wq[0];
-    for (int v = 0; v < kRows; ++v) g[v] = b[0][v] + x * (b[1][v] - b[0][v]);
-  } else {
-    double c0 = wq[0], c2 = wq[1], c3 = wq[2];
-    for (int v = 0; v < kRows; ++v)
-      g[v] = b[1][v] + c0 * (b[0][v] - b[1][v]) + c2 * (b[2][v] - b[1][v]) +
-             c3 * (b[3][v] - b[1][v]);
-  }
-  double o[3];
-  matvec(M + 36, &g[1], o);
-  cs_put(u, uc, dst, g[0]);
-  cs_put(u, uc, dst + d.plane, o[0]);
-  cs_put(u, uc, dst + 2 * d.plane, o[1]);
-  cs_put(u, uc, dst + 3 * d.plane, o[2]);
+  CsGhostBody<TU>::run(block_index(), grid_size(), u, uc, idx, wt, mat, nent, d);
 }
 
 // ---------------------------------------------------------------------------
@@ -292,76 +325,85 @@ __device__ __forceinline__ void face_flux(T* wl, T* wr, T c, T s, T* f) {
 // one thread per interior face (x2: faces ng..ng+n of the interior rows; x3:
 // the transpose); edge faces also store the interior-side state
 template <int AX, typename T>
+struct CsSweepBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             T const* __restrict__ w, T* __restrict__ flx,
+                                             T* __restrict__ edge, Geom gm, Dims d, int shock) {
+    // x3: neighbouring face lines share five of their six stencil rows, so the
+    // workgroups go through the XCD map (each XCD then sweeps a contiguous
+    // range of lines and the shared rows stay in its L2)
+    unsigned wg = AX == 3 ? xcd_tile(bid.x, gdim.x) : bid.x;
+    int64_t t = int64_t(wg) * blockDim.x + threadIdx.x;
+    int nf = d.n + 1;
+    int64_t per = int64_t(d.n) * nf;
+    if (t >= per * 6 * d.members) return;
+    int64_t b = t / per;
+    int r = int(t % per);
+    // consecutive threads step along x2 (the contiguous axis) in both sweeps:
+    // along the faces of an x2 sweep, across the lines of an x3 sweep
+    int f, a;  // face along the sweep, interior line across
+    if (AX == 2) {
+      f = r % nf;
+      a = r / nf;
+    } else {
+      a = r % d.n;
+      f = r / d.n;
+    }
+    int k, j;
+    int64_t stride;
+    if (AX == 2) {
+      k = d.ng + a;
+      j = d.ng + f;
+      stride = 1;
+    } else {
+      k = d.ng + f;
+      j = d.ng + a;
+      stride = d.nc;
+    }
+    int64_t c = int64_t(k) * d.nc + j;
+    T const* wb = w + b * d.block + c;
+    T wl[kRows], wr[kRows];
+    for (int v = 0; v < kRows; ++v) {
+      T const* p = wb + v * d.plane;
+      T c0 = p[-3 * stride], c1 = p[-2 * stride], c2 = p[-stride], c3 = p[0],
+        c4 = p[stride], c5 = p[2 * stride];
+      if (v == 0 || shock) {
+        wl[v] = weno5_plus(c0, c1, c2, c3, c4);
+        wr[v] = weno5_minus(c1, c2, c3, c4, c5);
+      } else {
+        wl[v] = cp5_plus(c0, c1, c2, c3, c4);
+        wr[v] = cp5_minus(c1, c2, c3, c4, c5);
+      }
+    }
+    if (f == 0 || f == d.n) {
+      // sides L/R (x2) or B/T (x3): the state on the panel's own side
+      int side = (AX == 2 ? 0 : 2) + (f == d.n ? 1 : 0);
+      T* e = edge + ((b * 4 + side) * int64_t(d.n) + a) * kRows;
+      T const* own = f == d.n ? wl : wr;
+      for (int v = 0; v < kRows; ++v) e[v] = own[v];
+    }
+    double cf, sf;
+    if (AX == 2) {
+      int64_t q = int64_t(k) * (d.nc + 1) + j;
+      cf = gm.cos_f2[q];
+      sf = gm.sin_f2[q];
+    } else {
+      int64_t q = int64_t(k) * d.nc + j;
+      cf = gm.cos_f3[q];
+      sf = gm.sin_f3[q];
+    }
+    T out[kRows];
+    face_flux<AX, T>(wl, wr, T(cf), T(sf), out);
+    T* fb = flx + b * d.block + c;
+    for (int v = 0; v < kRows; ++v) fb[v * d.plane] = out[v];
+  }
+};
+
+template <int AX, typename T>
 __global__ void __launch_bounds__(256)
     cs_sweep_kernel(T const* __restrict__ w, T* __restrict__ flx,
                     T* __restrict__ edge, Geom gm, Dims d, int shock) {
-  // x3: neighbouring face lines share five of their six stencil rows, so the
-  // workgroups go through the XCD map (each XCD then sweeps a contiguous
-  // range of lines and the shared rows stay in its L2)
-  unsigned wg = AX == 3 ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
-  int64_t t = int64_t(wg) * blockDim.x + threadIdx.x;
-  int nf = d.n + 1;
-  int64_t per = int64_t(d.n) * nf;
-  if (t >= per * 6 * d.members) return;
-  int64_t b = t / per;
-  int r = int(t % per);
-  // consecutive threads step along x2 (the contiguous axis) in both sweeps:
-  // along the faces of an x2 sweep, across the lines of an x3 sweep
-  int f, a;  // face along the sweep, interior line across
-  if (AX == 2) {
-    f = r % nf;
-    a = r / nf;
-  } else {
-    a = r % d.n;
-    f = r / d.n;
-  }
-  int k, j;
-  int64_t stride;
-  if (AX == 2) {
-    k = d.ng + a;
-    j = d.ng + f;
-    stride = 1;
-  } else {
-    k = d.ng + f;
-    j = d.ng + a;
-    stride = d.nc;
-  }
-  int64_t c = int64_t(k) * d.nc + j;
-  T const* wb = w + b * d.block + c;
-  T wl[kRows], wr[kRows];
-  for (int v = 0; v < kRows; ++v) {
-    T const* p = wb + v * d.plane;
-    T c0 = p[-3 * stride], c1 = p[-2 * stride], c2 = p[-stride], c3 = p[0],
-      c4 = p[stride], c5 = p[2 * stride];
-    if (v == 0 || shock) {
-      wl[v] = weno5_plus(c0, c1, c2, c3, c4);
-      wr[v] = weno5_minus(c1, c2, c3, c4, c5);
-    } else {
-      wl[v] = cp5_plus(c0, c1, c2, c3, c4);
-      wr[v] = cp5_minus(c1, c2, c3, c4, c5);
-    }
-  }
-  if (f == 0 || f == d.n) {
-    // sides L/R (x2) or B/T (x3): the state on the panel's own side
-    int side = (AX == 2 ? 0 : 2) + (f == d.n ? 1 : 0);
-    T* e = edge + ((b * 4 + side) * int64_t(d.n) + a) * kRows;
-    T const* own = f == d.n ? wl : wr;
-    for (int v = 0; v < kRows; ++v) e[v] = own[v];
-  }
-  double cf, sf;
-  if (AX == 2) {
-    int64_t q = int64_t(k) * (d.nc + 1) + j;
-    cf = gm.cos_f2[q];
-    sf = gm.sin_f2[q];
-  } else {
-    int64_t q = int64_t(k) * d.nc + j;
-    cf = gm.cos_f3[q];
-    sf = gm.sin_f3[q];
-  }
-  T out[kRows];
-  face_flux<AX, T>(wl, wr, T(cf), T(sf), out);
-  T* fb = flx + b * d.block + c;
-  for (int v = 0; v < kRows; ++v) fb[v * d.plane] = out[v];
+  CsSweepBody<AX, T>::run(block_index(), grid_size(), w, flx, edge, gm, d, shock);
 }
 
 // ---------------------------------------------------------------------------
@@ -371,98 +413,109 @@ __global__ void __launch_bounds__(256)
 // the neighbour's (through the two frame matrices), then the neighbour's copy
 // through the global Cartesian frame
 template <typename T>
+struct CsSeamBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim, T* __restrict__ f2,
+                                             T* __restrict__ f3, T const* __restrict__ edge,
+                                             int32_t const* __restrict__ idx,
+                                             double const* __restrict__ sd, int64_t nseam, Geom gm,
+                                             Dims d) {
+    int64_t t = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    if (t >= nseam * d.members) return;
+    int64_t e = t % nseam, m = t / nseam;
+    int32_t const* ix = idx + 8 * e;
+    int pa = ix[0], sa = ix[1], ia = ix[2], pb = ix[3], sb = ix[4], ib = ix[5];
+    double sign = double(ix[6]);
+    double const* D = sd + 30 * e;
+    int64_t ba = m * 6 + pa, bb = m * 6 + pb;
+    T const* own = edge + ((ba * 4 + sa) * int64_t(d.n) + ia) * kRows;
+    T const* oth = edge + ((bb * 4 + sb) * int64_t(d.n) + ib) * kRows;
+    double mine[kRows], other[kRows];
+    for (int v = 0; v < kRows; ++v) mine[v] = own[v];
+    {
+      double v0[3] = {oth[1], oth[2], oth[3]}, g[3], o[3];
+      matvec(D, v0, g);      // the neighbour's components to the geographic frame
+      matvec(D + 9, g, o);   // and into the owner's
+      other[0] = oth[0];
+      other[1] = o[0];
+      other[2] = o[1];
+      other[3] = o[2];
+    }
+    bool up = sa == 1 || sa == 3;
+    double* wl = up ? mine : other;
+    double* wr = up ? other : mine;
+    int plane_a = sa & 1 ? d.ng + d.n : d.ng;
+    double F[kRows];
+    int64_t ca;
+    if (sa < 2) {
+      int k = d.ng + ia;
+      int64_t q = int64_t(k) * (d.nc + 1) + plane_a;
+      face_flux<2, double>(wl, wr, gm.cos_f2[q], gm.sin_f2[q], F);
+      ca = int64_t(k) * d.nc + plane_a;
+    } else {
+      int j = d.ng + ia;
+      int64_t q = int64_t(plane_a) * d.nc + j;
+      face_flux<3, double>(wl, wr, gm.cos_f3[q], gm.sin_f3[q], F);
+      ca = int64_t(plane_a) * d.nc + j;
+    }
+    T* fa = (sa < 2 ? f2 : f3) + ba * d.block + ca;
+    for (int v = 0; v < kRows; ++v) fa[v * d.plane] = T(F[v]);
+
+    // the owner's rows: raise with its face cosine, to Cartesian (z, x, y)
+    double const* A = D + 18;  // x, y, delta, C, D, cos at the owner's face
+    double const* Bm = D + 24;
+    double cth = A[5];
+    double sth2 = 1. - cth * cth;
+    double r1 = F[1];
+    double r2 = F[2] / sth2 - F[3] * cth / sth2;
+    double r3 = -F[2] * cth / sth2 + F[3] / sth2;
+    double cart[3];
+    {
+      double x = A[0], y = A[1], delta = A[2], C = A[3], Dd = A[4];
+      double vz = r1, vx = r2, vy = r3;
+      cart[kL2CIdx[pa][0]] = double(kL2CSgn[pa][0]) * (vz - vx * x / Dd - vy * y / C) / delta;
+      cart[kL2CIdx[pa][1]] = double(kL2CSgn[pa][1]) * (vz * x + vx * Dd - (vy * x * y) / C) / delta;
+      cart[kL2CIdx[pa][2]] = double(kL2CSgn[pa][2]) * (vz * y + vy * C - (vx * x * y) / Dd) / delta;
+    }
+    double g0 = F[0];
+    if (sign < 0.) {
+      g0 = -g0;
+      cart[0] = -cart[0];
+      cart[1] = -cart[1];
+      cart[2] = -cart[2];
+    }
+    // into the other panel's contravariant, then lower with its face cosine
+    double o1, o2, o3;
+    {
+      double x = Bm[0], y = Bm[1], delta = Bm[2], C = Bm[3], Dd = Bm[4];
+      double loc[3];
+      loc[kC2LIdx[pb][0]] = double(kC2LSgn[pb][0]) * cart[0];
+      loc[kC2LIdx[pb][1]] = double(kC2LSgn[pb][1]) * cart[1];
+      loc[kC2LIdx[pb][2]] = double(kC2LSgn[pb][2]) * cart[2];
+      double vz = loc[0], vx = loc[1], vy = loc[2];
+      o1 = (vz + x * vx + y * vy) / delta;
+      o2 = Dd / delta * (vx - x * vz);
+      o3 = C / delta * (vy - y * vz);
+    }
+    double cb = Bm[5];
+    double l2 = o2 + o3 * cb;
+    double l3 = o3 + o2 * cb;
+    int plane_b = sb & 1 ? d.ng + d.n : d.ng;
+    int64_t cbi = sb < 2 ? int64_t(d.ng + ib) * d.nc + plane_b
+                         : int64_t(plane_b) * d.nc + (d.ng + ib);
+    T* fb = (sb < 2 ? f2 : f3) + bb * d.block + cbi;
+    fb[0] = T(g0);
+    fb[d.plane] = T(o1);
+    fb[2 * d.plane] = T(l2);
+    fb[3 * d.plane] = T(l3);
+  }
+};
+
+template <typename T>
 __global__ void __launch_bounds__(256)
     cs_seam_kernel(T* __restrict__ f2, T* __restrict__ f3,
                    T const* __restrict__ edge, int32_t const* __restrict__ idx,
                    double const* __restrict__ sd, int64_t nseam, Geom gm, Dims d) {
-  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= nseam * d.members) return;
-  int64_t e = t % nseam, m = t / nseam;
-  int32_t const* ix = idx + 8 * e;
-  int pa = ix[0], sa = ix[1], ia = ix[2], pb = ix[3], sb = ix[4], ib = ix[5];
-  double sign = double(ix[6]);
-  double const* D = sd + 30 * e;
-  int64_t ba = m * 6 + pa, bb = m * 6 + pb;
-  T const* own = edge + ((ba * 4 + sa) * int64_t(d.n) + ia) * kRows;
-  T const* oth = edge + ((bb * 4 + sb) * int64_t(d.n) + ib) * kRows;
-  double mine[kRows], other[kRows];
-  for (int v = 0; v < kRows; ++v) mine[v] = own[v];
-  {
-    double v0[3] = {oth[1], oth[2], oth[3]}, g[3], o[3];
-    matvec(D, v0, g);      // the neighbour's components to the geographic frame
-    matvec(D + 9, g, o);   // and into the owner's
-    other[0] = oth[0];
-    other[1] = o[0];
-    other[2] = o[1];
-    other[3] = o[2];
-  }
-  bool up = sa == 1 || sa == 3;
-  double* wl = up ? mine : other;
-  double* wr = up ? other : mine;
-  int plane_a = sa & 1 ? d.ng + d.n : d.ng;
-  double F[kRows];
-  int64_t ca;
-  if (sa < 2) {
-    int k = d.ng + ia;
-    int64_t q = int64_t(k) * (d.nc + 1) + plane_a;
-    face_flux<2, double>(wl, wr, gm.cos_f2[q], gm.sin_f2[q], F);
-    ca = int64_t(k) * d.nc + plane_a;
-  } else {
-    int j = d.ng + ia;
-    int64_t q = int64_t(plane_a) * d.nc + j;
-    face_flux<3, double>(wl, wr, gm.cos_f3[q], gm.sin_f3[q], F);
-    ca = int64_t(plane_a) * d.nc + j;
-  }
-  T* fa = (sa < 2 ? f2 : f3) + ba * d.block + ca;
-  for (int v = 0; v < kRows; ++v) fa[v * d.plane] = T(F[v]);
-
-  // the owner's rows: raise with its face cosine, to Cartesian (z, x, y)
-  double const* A = D + 18;  // x, y, delta, C, D, cos at the owner's face
-  double const* Bm = D + 24;
-  double cth = A[5];
-  double sth2 = 1. - cth * cth;
-  double r1 = F[1];
-  double r2 = F[2] / sth2 - F[3] * cth / sth2;
-  double r3 = -F[2] * cth / sth2 + F[3] / sth2;
-  double cart[3];
-  {
-    double x = A[0], y = A[1], delta = A[2], C = A[3], Dd = A[4];
-    double vz = r1, vx = r2, vy = r3;
-    cart[kL2CIdx[pa][0]] = double(kL2CSgn[pa][0]) * (vz - vx * x / Dd - vy * y / C) / delta;
-    cart[kL2CIdx[pa][1]] = double(kL2CSgn[pa][1]) * (vz * x + vx * Dd - (vy * x * y) / C) / delta;
-    cart[kL2CIdx[pa][2]] = double(kL2CSgn[pa][2]) * (vz * y + vy * C - (vx * x * y) / Dd) / delta;
-  }
-  double g0 = F[0];
-  if (sign < 0.) {
-    g0 = -g0;
-    cart[0] = -cart[0];
-    cart[1] = -cart[1];
-    cart[2] = -cart[2];
-  }
-  // into the other panel's contravariant, then lower with its face cosine
-  double o1, o2, o3;
-  {
-    double x = Bm[0], y = Bm[1], delta = Bm[2], C = Bm[3], Dd = Bm[4];
-    double loc[3];
-    loc[kC2LIdx[pb][0]] = double(kC2LSgn[pb][0]) * cart[0];
-    loc[kC2LIdx[pb][1]] = double(kC2LSgn[pb][1]) * cart[1];
-    loc[kC2LIdx[pb][2]] = double(kC2LSgn[pb][2]) * cart[2];
-    double vz = loc[0], vx = loc[1], vy = loc[2];
-    o1 = (vz + x * vx + y * vy) / delta;
-    o2 = Dd / delta * (vx - x * vz);
-    o3 = C / delta * (vy - y * vz);
-  }
-  double cb = Bm[5];
-  double l2 = o2 + o3 * cb;
-  double l3 = o3 + o2 * cb;
-  int plane_b = sb & 1 ? d.ng + d.n : d.ng;
-  int64_t cbi = sb < 2 ? int64_t(d.ng + ib) * d.nc + plane_b
-                       : int64_t(plane_b) * d.nc + (d.ng + ib);
-  T* fb = (sb < 2 ? f2 : f3) + bb * d.block + cbi;
-  fb[0] = T(g0);
-  fb[d.plane] = T(o1);
-  fb[2 * d.plane] = T(l2);
-  fb[3 * d.plane] = T(l3);
+  CsSeamBody<T>::run(block_index(), grid_size(), f2, f3, edge, idx, sd, nseam, gm, d);
 }
 
 // ---------------------------------------------------------------------------
@@ -478,90 +531,125 @@ struct CsUpdArgs {
 // TU: the state (double, or the FP32 pair with u0c / uc); TW: the
 // primitives and fluxes; the divergence, sources and combine in FP64
 template <typename TU, typename TW>
+struct CsUpdateBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             TU const* __restrict__ u0,
+                                             float const* __restrict__ u0c, TU* __restrict__ u,
+                                             float* __restrict__ uc, TW const* __restrict__ w,
+                                             TW const* __restrict__ f2, TW const* __restrict__ f3,
+                                             Geom gm, Dims d, CsUpdArgs a) {
+    int64_t t = int64_t(bid.x) * blockDim.x + threadIdx.x;
+    int64_t per = int64_t(d.n) * d.n;
+    if (t >= per * 6 * d.members) return;
+    int64_t b = t / per;
+    int r = int(t % per);
+    int k = d.ng + r / d.n, j = d.ng + r % d.n;
+    int64_t c = int64_t(k) * d.nc + j;
+    int64_t c2 = int64_t(k) * (d.nc + 1) + j;  // x2 face j of row k in area2
+    double dt = a.dt_dev ? a.dt_dev[b / 6] : a.dt;
+    double vol = gm.vol[c];
+    TW const* wb = w + b * d.block + c;
+    double phi = wb[0], v1 = wb[d.plane], v2 = wb[2 * d.plane], v3 = wb[3 * d.plane];
+    double cth = gm.cos_c[c];
+    double sine2 = gm.sin_c[c] * gm.sin_c[c];
+    double pr = 0.5 * (phi * phi);
+    double mdt = (-dt) * ((phi * v1) / a.radius);
+    double a2l = gm.area2[c2], a2r = gm.area2[c2 + 1];
+    double a3l = gm.area3[c], a3r = gm.area3[c + d.nc];
+    for (int v = 0; v < kRows; ++v) {
+      int64_t o = b * d.block + v * d.plane + c;
+      double dv = 0. + (a2r * double(f2[o + 1]) - a2l * double(f2[o]));
+      dv = dv + (a3r * double(f3[o + d.nc]) - a3l * double(f3[o]));
+      dv = dv / vol;
+      if (v == 2) dv = dv - gm.xr[c] * (pr + phi * v3 * v3 * sine2);
+      if (v == 3) dv = dv - gm.yr[c] * (pr + phi * v2 * v2 * sine2);
+      double du = -dt * dv;
+      // the x1 mass source is added to every row (zero on phi and phi v1)
+      if (v == 2)
+        du = du + (0. + mdt * (v2 + v3 * cth));
+      else if (v == 3)
+        du = du + (0. + mdt * (v3 + v2 * cth));
+      else
+        du = du + 0.;
+      cs_put(u, uc, o, a.w0 * cs_val(u0, u0c, o) + a.w1 * cs_val(u, uc, o) + a.w2 * du);
+    }
+  }
+};
+
+template <typename TU, typename TW>
 __global__ void __launch_bounds__(256)
     cs_update_kernel(TU const* __restrict__ u0, float const* __restrict__ u0c,
                      TU* __restrict__ u, float* __restrict__ uc,
                      TW const* __restrict__ w, TW const* __restrict__ f2,
                      TW const* __restrict__ f3, Geom gm, Dims d, CsUpdArgs a) {
-  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t per = int64_t(d.n) * d.n;
-  if (t >= per * 6 * d.members) return;
-  int64_t b = t / per;
-  int r = int(t % per);
-  int k = d.ng + r / d.n, j = d.ng + r % d.n;
-  int64_t c = int64_t(k) * d.nc + j;
-  int64_t c2 = int64_t(k) * (d.nc + 1) + j;  // x2 face j of row k in area2
-  double dt = a.dt_dev ? a.dt_dev[b / 6] : a.dt;
-  double vol = gm.vol[c];
-  TW const* wb = w + b * d.block + c;
-  double phi = wb[0], v1 = wb[d.plane], v2 = wb[2 * d.plane], v3 = wb[3 * d.plane];
-  double cth = gm.cos_c[c];
-  double sine2 = gm.sin_c[c] * gm.sin_c[c];
-  double pr = 0.5 * (phi * phi);
-  double mdt = (-dt) * ((phi * v1) / a.radius);
-  double a2l = gm.area2[c2], a2r = gm.area2[c2 + 1];
-  double a3l = gm.area3[c], a3r = gm.area3[c + d.nc];
-  for (int v = 0; v < kRows; ++v) {
-    int64_t o = b * d.block + v * d.plane + c;
-    double dv = 0. + (a2r * double(f2[o + 1]) - a2l * double(f2[o]));
-    dv = dv + (a3r * double(f3[o + d.nc]) - a3l * double(f3[o]));
-    dv = dv / vol;
-    if (v == 2) dv = dv - gm.xr[c] * (pr + phi * v3 * v3 * sine2);
-    if (v == 3) dv = dv - gm.yr[c] * (pr + phi * v2 * v2 * sine2);
-    double du = -dt * dv;
-    // the x1 mass source is added to every row (zero on phi and phi v1)
-    if (v == 2)
-      du = du + (0. + mdt * (v2 + v3 * cth));
-    else if (v == 3)
-      du = du + (0. + mdt * (v3 + v2 * cth));
-    else
-      du = du + 0.;
-    cs_put(u, uc, o, a.w0 * cs_val(u0, u0c, o) + a.w1 * cs_val(u, uc, o) + a.w2 * du);
-  }
+  CsUpdateBody<TU, TW>::run(block_index(), grid_size(), u0, u0c, u, uc, w, f2, f3, gm, d, a);
 }
 
 // ---------------------------------------------------------------------------
 // CFL, per member
 // ---------------------------------------------------------------------------
-__global__ void cs_cfl_init_kernel(unsigned long long* out, int members) {
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < members) {
-    double big = 1.e9;
-    unsigned long long bits;
-    memcpy(&bits, &big, sizeof(bits));
-    out[m] = bits;
+struct CsCflInitBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             unsigned long long* out, int members) {
+    int m = bid.x * blockDim.x + threadIdx.x;
+    if (m < members) {
+      double big = 1.e9;
+      unsigned long long bits;
+      memcpy(&bits, &big, sizeof(bits));
+      out[m] = bits;
+    }
   }
+};
+
+__global__ void cs_cfl_init_kernel(unsigned long long* out, int members) {
+  CsCflInitBody::run(block_index(), grid_size(), out, members);
 }
+
+template <typename TW>
+struct CsCflBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             TW const* __restrict__ w, Geom gm, Dims d,
+                                             unsigned long long* __restrict__ out) {
+    // one block row of 256 threads per (member, chunk); bid.y = member
+    int m = bid.y;
+    int64_t per = int64_t(d.n) * d.n * 6;
+    double best = 1.e9;
+    for (int64_t t = int64_t(bid.x) * blockDim.x + threadIdx.x; t < per;
+         t += int64_t(gdim.x) * blockDim.x) {
+      int64_t p = t / (int64_t(d.n) * d.n);
+      int r = int(t % (int64_t(d.n) * d.n));
+      int64_t c = int64_t(d.ng + r / d.n) * d.nc + (d.ng + r % d.n);
+      TW const* wb = w + (int64_t(m) * 6 + p) * d.block + c;
+      double cs = sqrt(double(wb[0]));
+      best = fmin(best, gm.w2[c] / (fabs(double(wb[2 * d.plane])) + cs));
+      best = fmin(best, gm.w3[c] / (fabs(double(wb[3 * d.plane])) + cs));
+    }
+    for (int off = 32; off > 0; off >>= 1) best = fmin(best, __shfl_down(best, off, 64));
+    if ((threadIdx.x & 63) == 0) {
+      unsigned long long bits;
+      memcpy(&bits, &best, sizeof(bits));
+      atomicMin(out + m, bits);
+    }
+  }
+};
 
 template <typename TW>
 __global__ void __launch_bounds__(256)
     cs_cfl_kernel(TW const* __restrict__ w, Geom gm, Dims d,
                   unsigned long long* __restrict__ out) {
-  // one block row of 256 threads per (member, chunk); blockIdx.y = member
-  int m = blockIdx.y;
-  int64_t per = int64_t(d.n) * d.n * 6;
-  double best = 1.e9;
-  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < per;
-       t += int64_t(gridDim.x) * blockDim.x) {
-    int64_t p = t / (int64_t(d.n) * d.n);
-    int r = int(t % (int64_t(d.n) * d.n));
-    int64_t c = int64_t(d.ng + r / d.n) * d.nc + (d.ng + r % d.n);
-    TW const* wb = w + (int64_t(m) * 6 + p) * d.block + c;
-    double cs = sqrt(double(wb[0]));
-    best = fmin(best, gm.w2[c] / (fabs(double(wb[2 * d.plane])) + cs));
-    best = fmin(best, gm.w3[c] / (fabs(double(wb[3 * d.plane])) + cs));
-  }
-  for (int off = 32; off > 0; off >>= 1) best = fmin(best, __shfl_down(best, off, 64));
-  if ((threadIdx.x & 63) == 0) {
-    unsigned long long bits;
-    memcpy(&bits, &best, sizeof(bits));
-    atomicMin(out + m, bits);
-  }
+  CsCflBody<TW>::run(block_index(), grid_size(), w, gm, d, out);
 }
 
+struct CsCflScaleBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim, double* out,
+                                             double s, int members) {
+    int m = bid.x * blockDim.x + threadIdx.x;
+    if (m < members) out[m] = out[m] * s;
+  }
+};
+
 __global__ void cs_cfl_scale_kernel(double* out, double s, int members) {
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < members) out[m] = out[m] * s;
+  CsCflScaleBody::run(block_index(), grid_size(), out, s, members);
 }
 
 int check_mesh(snapy_cs_mesh const* m) {
@@ -591,10 +679,10 @@ int cs_sweep_launch(const snapy_cs_mesh* m, int dim, const T* w, T* flux, T* edg
   Geom gm = geom_of(m->geom, d.nc);
   int64_t n = int64_t(d.n) * (d.n + 1) * 6 * d.members;
   if (dim == 2)
-    hipLaunchKernelGGL((cs_sweep_kernel<2, T>), dim3(blocks_for(n, 256)), dim3(256), 0,
+    launch<CsSweepBody<2, T>, 256, 1>((cs_sweep_kernel<2, T>), dim3(blocks_for(n, 256)), dim3(256), 0,
                        as_stream(stream), w, flux, edge, gm, d, m->shock);
   else
-    hipLaunchKernelGGL((cs_sweep_kernel<3, T>), dim3(blocks_for(n, 256)), dim3(256), 0,
+    launch<CsSweepBody<3, T>, 256, 1>((cs_sweep_kernel<3, T>), dim3(blocks_for(n, 256)), dim3(256), 0,
                        as_stream(stream), w, flux, edge, gm, d, m->shock);
   return launch_status("cs_sweep");
 }
@@ -609,7 +697,7 @@ int cs_seams_launch(const snapy_cs_mesh* m, T* flux2, T* flux3, const T* edge,
   Geom gm = geom_of(m->geom, d.nc);
   int64_t nseam = int64_t(12) * d.n;
   int64_t n = nseam * d.members;
-  hipLaunchKernelGGL(cs_seam_kernel<T>, dim3(blocks_for(n, 256)), dim3(256), 0,
+  launch<CsSeamBody<T>, 256, 1>(cs_seam_kernel<T>, dim3(blocks_for(n, 256)), dim3(256), 0,
                      as_stream(stream), flux2, flux3, edge, m->seam_idx, m->seam_d, nseam,
                      gm, d);
   return launch_status("cs_seams");
@@ -624,14 +712,14 @@ int cs_cfl_launch(const snapy_cs_mesh* m, const TW* w, double scale, double* dt_
   Geom gm = geom_of(m->geom, d.nc);
   auto* bits = reinterpret_cast<unsigned long long*>(dt_out);
   unsigned mb = blocks_for(d.members, 64);
-  hipLaunchKernelGGL(cs_cfl_init_kernel, dim3(mb), dim3(64), 0, as_stream(stream), bits,
+  launch<CsCflInitBody, 1024, 1>(cs_cfl_init_kernel, dim3(mb), dim3(64), 0, as_stream(stream), bits,
                      d.members);
   int64_t per = int64_t(d.n) * d.n * 6;
   unsigned gx = blocks_for(per, 256);
   if (gx > 64) gx = 64;
-  hipLaunchKernelGGL(cs_cfl_kernel<TW>, dim3(gx, d.members), dim3(256), 0, as_stream(stream), w,
+  launch<CsCflBody<TW>, 256, 1>(cs_cfl_kernel<TW>, dim3(gx, d.members), dim3(256), 0, as_stream(stream), w,
                      gm, d, bits);
-  hipLaunchKernelGGL(cs_cfl_scale_kernel, dim3(mb), dim3(64), 0, as_stream(stream), dt_out,
+  launch<CsCflScaleBody, 1024, 1>(cs_cfl_scale_kernel, dim3(mb), dim3(64), 0, as_stream(stream), dt_out,
                      scale, d.members);
   return launch_status("cs_cfl");
 }
@@ -654,7 +742,7 @@ int snapy_cs_cons2prim(const snapy_cs_mesh* m, const double* u, double* w, void*
   if (int rc = check_mesh(m)) return rc;
   Dims d = dims_of(*m);
   int64_t n = d.plane * 6 * d.members;
-  hipLaunchKernelGGL((cs_cons2prim_kernel<double, double>), dim3(blocks_for(n, 256)), dim3(256),
+  launch<CsCons2PrimBody<double, double>, 256, 1>((cs_cons2prim_kernel<double, double>), dim3(blocks_for(n, 256)), dim3(256),
                      0, as_stream(stream), u, nullptr, w, m->geom, d, n);
   return launch_status("cs_cons2prim");
 }
@@ -663,7 +751,7 @@ int snapy_cs_prim2cons(const snapy_cs_mesh* m, const double* w, double* u, void*
   if (int rc = check_mesh(m)) return rc;
   Dims d = dims_of(*m);
   int64_t n = d.plane * 6 * d.members;
-  hipLaunchKernelGGL(cs_prim2cons_kernel<double>, dim3(blocks_for(n, 256)), dim3(256), 0,
+  launch<CsPrim2ConsBody<double>, 256, 1>(cs_prim2cons_kernel<double>, dim3(blocks_for(n, 256)), dim3(256), 0,
                      as_stream(stream), w, u, nullptr, m->geom, d, n);
   return launch_status("cs_prim2cons");
 }
@@ -674,7 +762,7 @@ int snapy_cs_ghosts(const snapy_cs_mesh* m, int conserved, double* q, void* stre
     return fail(-4, "cubed sphere: null ghost tables");
   Dims d = dims_of(*m);
   int64_t n = m->nghost_entries * d.members;
-  hipLaunchKernelGGL(cs_ghost_kernel<double>, dim3(blocks_for(n, 256)), dim3(256), 0,
+  launch<CsGhostBody<double>, 256, 1>(cs_ghost_kernel<double>, dim3(blocks_for(n, 256)), dim3(256), 0,
                      as_stream(stream), q, nullptr, m->ghost_idx, m->ghost_w,
                      conserved ? m->ghost_mat_c : m->ghost_mat_p, m->nghost_entries, d);
   return launch_status("cs_ghosts");
@@ -700,7 +788,7 @@ int snapy_cs_update(const snapy_cs_mesh* m, const double* u0, double* u, const d
   Geom gm = geom_of(m->geom, d.nc);
   CsUpdArgs a{w0, w1, w2, dt, dt_dev, m->radius};
   int64_t n = int64_t(d.n) * d.n * 6 * d.members;
-  hipLaunchKernelGGL((cs_update_kernel<double, double>), dim3(blocks_for(n, 256)), dim3(256), 0,
+  launch<CsUpdateBody<double, double>, 256, 1>((cs_update_kernel<double, double>), dim3(blocks_for(n, 256)), dim3(256), 0,
                      as_stream(stream), u0, nullptr, u, nullptr, w, flux2, flux3, gm, d, a);
   return launch_status("cs_update");
 }
@@ -718,7 +806,7 @@ int snapy_cs_cons2prim_f32(const snapy_cs_mesh* m, const float* u, const float* 
   if (!u || !c || !w) return fail(-4, "cubed sphere: null buffer");
   Dims d = dims_of(*m);
   int64_t n = d.plane * 6 * d.members;
-  hipLaunchKernelGGL((cs_cons2prim_kernel<float, float>), dim3(blocks_for(n, 256)), dim3(256), 0,
+  launch<CsCons2PrimBody<float, float>, 256, 1>((cs_cons2prim_kernel<float, float>), dim3(blocks_for(n, 256)), dim3(256), 0,
                      as_stream(stream), u, c, w, m->geom, d, n);
   return launch_status("cs_cons2prim_f32");
 }
@@ -729,7 +817,7 @@ int snapy_cs_prim2cons_f32(const snapy_cs_mesh* m, const double* w, float* u, fl
   if (!u || !c || !w) return fail(-4, "cubed sphere: null buffer");
   Dims d = dims_of(*m);
   int64_t n = d.plane * 6 * d.members;
-  hipLaunchKernelGGL(cs_prim2cons_kernel<float>, dim3(blocks_for(n, 256)), dim3(256), 0,
+  launch<CsPrim2ConsBody<float>, 256, 1>(cs_prim2cons_kernel<float>, dim3(blocks_for(n, 256)), dim3(256), 0,
                      as_stream(stream), w, u, c, m->geom, d, n);
   return launch_status("cs_prim2cons_f32");
 }
@@ -741,7 +829,7 @@ int snapy_cs_ghosts_f32(const snapy_cs_mesh* m, float* u, float* c, void* stream
   if (!u || !c) return fail(-4, "cubed sphere: null buffer");
   Dims d = dims_of(*m);
   int64_t n = m->nghost_entries * d.members;
-  hipLaunchKernelGGL(cs_ghost_kernel<float>, dim3(blocks_for(n, 256)), dim3(256), 0,
+  launch<CsGhostBody<float>, 256, 1>(cs_ghost_kernel<float>, dim3(blocks_for(n, 256)), dim3(256), 0,
                      as_stream(stream), u, c, m->ghost_idx, m->ghost_w, m->ghost_mat_c,
                      m->nghost_entries, d);
   return launch_status("cs_ghosts_f32");
@@ -768,7 +856,7 @@ int snapy_cs_update_f32(const snapy_cs_mesh* m, const float* u0, const float* c0
   Geom gm = geom_of(m->geom, d.nc);
   CsUpdArgs a{w0, w1, w2, dt, dt_dev, m->radius};
   int64_t n = int64_t(d.n) * d.n * 6 * d.members;
-  hipLaunchKernelGGL((cs_update_kernel<float, float>), dim3(blocks_for(n, 256)), dim3(256), 0,
+  launch<CsUpdateBody<float, float>, 256, 1>((cs_update_kernel<float, float>), dim3(blocks_for(n, 256)), dim3(256), 0,
                      as_stream(stream), u0, c0, u, c, w, flux2, flux3, gm, d, a);
   return launch_status("cs_update_f32");
 }

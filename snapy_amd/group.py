@@ -11,6 +11,10 @@ workgroups of all the blocks (`snapy_group_launch`), every workgroup with
 its own block's arguments.  A block's cells see the arithmetic of its own
 launches, so a grouped step is bit-for-bit the blocks' own steps.
 
+Cubed-sphere meshes (`CubedSphereMesh`, six panels and their members per
+launch already) group the same way, meshes of different panel sizes in one
+launch per kernel.
+
 The blocks must make the same launches: the same deck (physics, schemes,
 boundaries, state form) on grids of any extents and spacings, as long as
 each launch runs the same kernel with the same workgroup shape (the x2/x3
@@ -34,9 +38,17 @@ from . import _capi
 _TORCH_CALLS = ("vic_status_clear",)
 
 
+def _is_cubed(block) -> bool:
+    from .cubed import CubedSphereMesh
+
+    return isinstance(block, CubedSphereMesh)
+
+
 def groupable(block) -> bool:
     """A block the group can take: a single block (no member axis) of an
-    ideal gas (tracers included)."""
+    ideal gas (tracers included), or a cubed-sphere mesh."""
+    if _is_cubed(block):
+        return True
     return block.members == 1 and block.nvar == 5
 
 
@@ -44,6 +56,9 @@ def group_key(block):
     """Blocks with equal keys run the same deck: the options but the grid's
     extents, spacings and cell counts (the active axes stay).  The library
     checks the launches themselves when the group is built."""
+    if _is_cubed(block):
+        b = block.options.block
+        return ("cubed-sphere", repr(b.hydro), repr(b.intg), tuple(b.bcnames), b.coord.nghost)
     op = block.options
     c = op.coord
     coord = (c.type, c.nghost, c.nx1 > 1, c.nx2 > 1, c.nx3 > 1)
@@ -65,7 +80,8 @@ class BlockGroup:
         for blk, _ in self.members:
             if not groupable(blk):
                 raise ValueError("BlockGroup: a block of members, or not of an ideal gas, is not grouped")
-            if blk.device.type != "cuda" or blk._freg is None:
+            ready = blk._u is not None if _is_cubed(blk) else blk._freg is not None
+            if blk.device.type != "cuda" or not ready:
                 raise ValueError("BlockGroup: initialize every block on the GPU first")
         keys = {group_key(blk) for blk, _ in self.members}
         if len(keys) != 1:
@@ -82,6 +98,9 @@ class BlockGroup:
 
     # ------------------------------------------------------------- building
     def _record(self, blk, vars, phase) -> None:
+        if _is_cubed(blk):
+            self._record_cubed(blk, phase)
+            return
         if phase == "dt":
             # MeshBlock.device_time_step of a block whose primitives are stale
             # (recording launches nothing: the block's own flag stays as it
@@ -101,6 +120,20 @@ class BlockGroup:
         for name, fn in blk._stage_calls(vars, phase):
             if name in _TORCH_CALLS:
                 continue
+            _capi.check(fn(), name)
+
+    def _record_cubed(self, mesh, phase) -> None:
+        if phase == "dt":
+            # CubedSphereMesh.device_time_step with stale primitives
+            stale = mesh._w_stale
+            mesh._refresh_primitives()
+            mesh._w_stale = stale
+            cfl = self._lib.snapy_cs_cfl_f32 if mesh.fp32 else self._lib.snapy_cs_cfl
+            scale = 2.0 ** (-mesh.current_redo) * mesh.cfl
+            _capi.check(cfl(ctypes.byref(mesh._desc), _capi.ptr(mesh._w), scale, _capi.ptr(mesh._dt_dev),
+                            _capi.stream_of(self.device)), "cs_cfl")
+            return
+        for name, fn in mesh._stage_calls(phase):
             _capi.check(fn(), name)
 
     def _build(self, phase):
@@ -158,8 +191,13 @@ class BlockGroup:
             if k == 0:
                 # the RK backups (state and tracers) and the implicit
                 # solve's status latch, per block (MeshBlock._launch_stage
-                # and its first call)
+                # and its first call; CubedSphereMesh._launch_stage)
                 for blk, vars in self.members:
+                    if _is_cubed(blk):
+                        blk._u0.copy_(blk._u)
+                        if blk._c is not None:
+                            blk._c0.copy_(blk._c)
+                        continue
                     blk._state.backup()
                     if blk.nscalar:
                         blk._scal["s0"].copy_(vars["scalar_s"])

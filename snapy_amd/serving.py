@@ -13,7 +13,8 @@ round are batched: one work list (`worklist.BlockBatch`) advances all of
 them with one launch per phase on a stream of their own, rebuilt when a
 tube joins or leaves; a tube's result is bit-identical batched or alone.
 The density-current requests of one state form are grouped too, whatever
-their grids (`group.BlockGroup`):
+their grids, and so are the shallow-water meshes of any panel size
+(`group.BlockGroup`):
 every kernel launch of their step is one launch over all of them, each
 block with its own buffers and dt, the group rebuilt when one joins or
 leaves; a result is bit-identical grouped or alone.  A request leaves when
@@ -232,8 +233,9 @@ class Server:
     max_slots: int = 16
     realtime: bool = True  # honour arrival times against the wall clock
     batch_tubes: bool = True  # advance the shock tubes as one work list
-    # advance the density-current requests of one state form as one group
-    # (one launch per kernel), whatever their grids
+    # advance the density-current requests of one state form, and the
+    # shallow-water meshes, as groups (one launch per kernel), whatever their
+    # grids
     batch_groups: bool = True
     # a request that does not fit may preempt the active block request with
     # the most steps left (more than the newcomer's): that request is written
@@ -299,7 +301,7 @@ class Server:
                     tube_stream.wait_stream(streams[si])
                     sl.stream = tube_stream
                     sl.batched = True
-                elif self.batch_groups and req.kind == "density_current" and groupable(block):
+                elif self.batch_groups and req.kind in ("density_current", "swe") and groupable(block):
                     # its steps run on its group's stream, alone or in the
                     # group
                     sl.key = group_key(block)

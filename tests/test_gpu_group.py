@@ -201,3 +201,49 @@ def test_grouped_tracer_blocks_equal_each_blocks_own(precision, scheme):
         for a, b in zip(tracer_state(blk, vars), ref):
             assert torch.equal(a, b), (precision, scheme, blk.options.coord.nx2)
     grp.release()
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_grouped_cubed_spheres_equal_each_meshes_own(precision):
+    """Cubed-sphere shallow-water meshes of different panel sizes (and
+    member counts) grouped: each ends with its own bits."""
+    import os
+
+    from snapy_amd.cubed import CubedSphereMesh
+    from snapy_amd.options import MeshOptions
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    specs = [(16, 1, 11), (24, 2, 12), (20, 1, 13)]
+
+    def make(n, members, seed):
+        mo = MeshOptions.from_yaml(os.path.join(root, "snapy_amd", "decks", "swe_cubed.yaml"))
+        mo.block.coord.nx2 = mo.block.coord.nx3 = n
+        mo.block.intg.execution = "eager"
+        mo.block.hydro.precision = precision
+        m = CubedSphereMesh(mo, members=members).to("cuda")
+        vars = [{"hydro_w": w} for k in range(members) for w in problems.swe_hill(m, seed + k)]
+        m.initialize(vars)
+        return m, vars
+
+    print(f"group of cubed spheres {specs}, {precision}", flush=True)
+    nsteps = 3
+    solo = []
+    for n, members, seed in specs:
+        m, vars = make(n, members, seed)
+        for _ in range(nsteps):
+            dt = m.device_time_step(vars)
+            for s in range(3):
+                m.forward(vars, dt, s)
+        torch.cuda.synchronize()
+        solo.append(m.total_u().clone())
+        m.release()
+    meshes = [make(n, members, seed) for n, members, seed in specs]
+    grp = BlockGroup(meshes)
+    for _ in range(nsteps):
+        grp.step()
+    torch.cuda.synchronize()
+    for (m, _), ref in zip(meshes, solo):
+        got = m.total_u()
+        assert torch.isfinite(got).all()
+        assert torch.equal(got, ref), (m.n, precision)
+    grp.release()
