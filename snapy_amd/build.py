@@ -93,5 +93,25 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     return LIB
 
 
+HOST_SRC = ROOT / "tests" / "cxx" / "sod_host.cpp"
+HOST_BIN = ROOT / "tests" / "cxx" / "sod_host"
+
+
+def build_host_example(verbose: bool = False) -> Path | None:
+    """Compile the C++ host that drives the library through its C-ABI alone
+    (no Python, no torch) next to its source; it links the built LIB, found
+    at run time through its rpath.  None when the source is absent."""
+    if not HOST_SRC.exists():
+        return None
+    rel = os.path.relpath(LIB.parent, HOST_BIN.parent)
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I", str(ROOT / "include"), str(HOST_SRC),
+           "-L", str(LIB.parent), f"-l:{LIB.name}", f"-Wl,-rpath,$ORIGIN/{rel}",
+           "-Wl,-rpath,/opt/rocm/lib", "-o", str(HOST_BIN) + ".tmp"]
+    _run(cmd, verbose)
+    os.replace(str(HOST_BIN) + ".tmp", HOST_BIN)
+    return HOST_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_host_example(verbose=True))
