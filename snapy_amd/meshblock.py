@@ -428,16 +428,17 @@ class MeshBlock:
         self._w_stale = False
         return vars, 0.0
 
-    def write_restart(self, vars, path: str, time: float) -> str:
+    def write_restart(self, vars, path: str, time: float, extra: dict | None = None) -> str:
         """Write this block's restart part to `path` (RestartOutput::
         write_output_file, src/output/restart.cpp:42-147); see
-        snapy_amd.restart for what it holds."""
+        snapy_amd.restart for what it holds.  `extra` adds tensors (an output
+        schedule, a budget carry)."""
         from . import restart
 
         if self.members > 1:
             raise restart.RestartError("restart: a block of members has no restart part (one per member block)")
         torch.cuda.synchronize(self.device)
-        return restart.write_restart(self, vars, path, time)
+        return restart.write_restart(self, vars, path, time, extra=extra)
 
     def initialize_from_restart(self, path_or_data):
         """initialize from a restart part (MeshBlockImpl::_init_from_restart,

@@ -32,7 +32,8 @@ not in this image.  The dimensions, variables, attributes and names are the
 reference's, so a NetCDF reader sees the same dataset; classic NetCDF has no
 64-bit integer, so the series' `cycle` is a double.  Not written: slices,
 sums over an axis, ghost zones, statistics (prim_stat), user output
-variables, the budget's counters and minima, the budget carry of a restart.
+variables, the budget's counters and minima (a restart's budget carry holds
+them empty).
 """
 
 from __future__ import annotations
@@ -337,6 +338,35 @@ class BudgetConsumer:
         self._pending = self._pending + self.boundary()
         self._interval += float(dt)
 
+    def carry(self, vars, fires_now: bool):
+        """BudgetConsumer::carry (budget.cpp:214-234): the open interval's
+        tensors for a restart, or None before the first observe; a record
+        firing at this time closes the interval first."""
+        from .restart import carry_tensors
+
+        if not self._started:
+            return None
+        rows = row_names(self.block.nvar)
+        if fires_now:
+            total = self.block.budget(vars)
+            return carry_tensors(rows, total, torch.zeros_like(self._pending), 0.0)
+        return carry_tensors(rows, self._total_start, self._pending, self._interval)
+
+    def resume(self, data: dict) -> None:
+        """BudgetConsumer::resume (budget.cpp:236-245): an interval carried by
+        a restart with the same rows continues; otherwise a new one starts at
+        the first observe."""
+        from .restart import read_carry
+
+        c = read_carry(data, row_names(self.block.nvar))
+        if c is None:
+            return
+        dev = self.block.device
+        self._total_start = c[0].to(dev)
+        self._pending = c[1].to(dev)
+        self._interval = c[2]
+        self._started = True
+
     def flush(self, vars, time: float) -> BudgetRecord:
         """Close the interval (BudgetConsumer::flush)."""
         if not self._started:
@@ -401,7 +431,8 @@ class BlockOutputs:
     """The deck's outputs of one block (MeshBlockImpl::make_outputs): call
     `observe(vars, dt)` after every accepted step and `make_outputs(vars,
     time)` after it; `finalize(vars, time)` writes every output once more.
-    Restart outputs write restart parts (snapy_amd.restart)."""
+    Restart outputs write restart files (snapy_amd.restart); `resume(data,
+    time)` takes the schedule and the budget intervals back from one."""
 
     def __init__(self, block, outputs=None, basename=None, output_dir="."):
         self.block = block
@@ -421,15 +452,47 @@ class BlockOutputs:
         for b in self.budget.values():
             b.observe(vars, dt)
 
-    def _write(self, k, vars, time):
+    def _write_restart(self, k, vars, time, final):
+        """RestartOutput::write_output_file (restart.cpp:42-147): the part
+        with the advanced schedule of this output and of the outputs after it
+        that fire in the same pass, and each budget output's open interval;
+        then the parts combined into the restart file."""
+        from . import restart
+
+        file_number, next_time = [], []
+        firing = False
+        extra = {}
+        for j, oj in enumerate(self.outputs):
+            if j == k:
+                firing = not final
+            advance = firing and time >= self.next_time[j]
+            file_number.append(self.file_number[j] + (1 if advance else 0))
+            next_time.append(self.next_time[j] + (oj.dt if advance else 0.0))
+            if j in self.budget:
+                c = self.budget[j].carry(vars, advance)
+                if c is not None:
+                    extra.update(c)
+        extra.update(restart.schedule_tensors(self.outputs, file_number, next_time))
+        number = None if final else self.file_number[k]
+        self.block.write_restart(vars, os.path.join(self.output_dir, restart.part_name(self.basename, 0, number)),
+                                 time, extra=extra)
+        return restart.combine_parts(self.output_dir, self.basename, number)
+
+    def resume(self, data: dict, time: float) -> None:
+        """The schedule and the budget intervals of a restart
+        (meshblock_restart.cpp:61-125)."""
+        from . import restart
+
+        self.file_number, self.next_time = restart.restore_schedule(data, self.outputs, time)
+        for b in self.budget.values():
+            b.resume(data)
+
+    def _write(self, k, vars, time, final=False):
         o = self.outputs[k]
         if o.file_type in ("netcdf", "pnetcdf"):
             p = write_netcdf(self.block, vars, o, self.file_number[k], time, self.basename, self.output_dir)
         elif o.file_type == "restart":
-            from .restart import part_name
-
-            p = self.block.write_restart(vars, os.path.join(self.output_dir, part_name(self.basename, 0,
-                                                                                        self.file_number[k])), time)
+            p = self._write_restart(k, vars, time, final)
         else:
             rec = self.budget[k].flush(vars, time)
             p = os.path.join(self.output_dir, f"{self.basename}.budget.nc")
@@ -441,7 +504,7 @@ class BlockOutputs:
     def make_outputs(self, vars, time: float, final: bool = False) -> None:
         for k, o in enumerate(self.outputs):
             if final:
-                self._write(k, vars, time)
+                self._write(k, vars, time, final=True)
             elif time >= self.next_time[k]:
                 self._write(k, vars, time)
                 self.next_time[k] += o.dt

@@ -148,3 +148,48 @@ def test_netcdf_snapshot_reads_back(tmp_path):
     assert line.startswith(f"cycle={blk.cycle()} redo=0 time=")
     mass = float(blk.budget(vars)[0])
     assert f"mass0={mass:.13e}" in line and " ie=" in line and " pe=" in line
+
+
+def test_restart_output_resumes_schedule_and_budget(tmp_path):
+    """A run with budget and restart outputs, and the same run resumed from
+    one of its combined restart files (the reference's TorchScript archive)
+    in another directory: the resumed block's state is bit for bit the
+    uninterrupted one's, its restart files carry on the same numbers, and its
+    budget records after the resume (the open interval carried) are the
+    uninterrupted run's, bit for bit."""
+    from scipy.io import netcdf_file
+
+    from snapy_amd import restart
+    from snapy_amd.output import BlockOutputs, OutputOptions
+
+    print("output: restart output resumed with its schedule and budget carry", flush=True)
+    opts = [OutputOptions(fid=0, file_type="budget", dt=0.05), OutputOptions(fid=1, file_type="restart", dt=0.05)]
+    da, db = tmp_path / "a", tmp_path / "b"
+    blk, vars = _open_block(outputs=opts)
+    outs = BlockOutputs(blk, opts, basename="open", output_dir=str(da))
+    t = _run(blk, vars, outs, 30)
+    names_a = sorted(p.name for p in da.glob("*.restart"))
+    assert len(names_a) >= 4 and not list(da.glob("*.part"))
+    path = str(da / names_a[2])
+    data = restart.load_restart(path)
+    t0, c0 = float(data["last_time"][0]), int(data["last_cycle"][0])
+    b2, _ = _open_block(outputs=opts)
+    vb, tb = b2.initialize_from_restart(path)
+    assert tb == t0 and b2.cycle() == c0
+    outs_b = BlockOutputs(b2, opts, basename="open", output_dir=str(db))
+    outs_b.resume(data, tb)
+    assert outs_b.file_number[1] == 3
+    tb = _run(b2, vb, outs_b, 30 - c0, tb)
+    assert tb == t
+    assert torch.equal(vb["hydro_u"], vars["hydro_u"])
+    assert sorted(p.name for p in db.glob("*.restart")) == names_a[3:]
+
+    def records(d):
+        with netcdf_file(os.path.join(str(d), "open.budget.nc"), "r", mmap=False) as f:
+            return {k: f.variables[k][:].copy() for k in ("time", "total", "change", "boundary", "interval")}
+
+    ra, rb = records(da), records(db)
+    later = ra["time"] > t0
+    assert later.sum() == len(rb["time"]) >= 2
+    for k in ra:
+        np.testing.assert_array_equal(ra[k][later], rb[k], err_msg=k)

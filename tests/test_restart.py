@@ -1,5 +1,9 @@
-"""Restart parts on the CPU: the part name, the archive round trip, and the
-metadata checks of check_restart_metadata (restart.cpp:323-358)."""
+"""Restart files on the CPU: the part name, the TorchScript tensor archive
+(kintera::save_tensors' format), the bundle of several parts
+(restart.cpp:150-321), the metadata checks of check_restart_metadata
+(restart.cpp:323-358), the output schedule's keys and its restore by key
+(output_type.cpp:14-42, meshblock_restart.cpp:61-123), and the budget carry
+(restart.cpp:360-407)."""
 
 import pytest
 import torch
@@ -99,3 +103,122 @@ def test_refuses_missing_and_foreign_files(tmp_path):
     save_file({"x": torch.zeros(2)}, q)
     with pytest.raises(restart.RestartError, match="not a restart part"):
         restart.load_restart(q)
+
+
+def test_archive_is_the_reference_tensor_archive(tmp_path):
+    """The part is a TorchScript module whose buffers are the tensors, as the
+    reference's tests/test_fp32_restart.py:22-29 writes one: torch.jit.load
+    reads it back by name, dotted keys included, and a part written the
+    reference's way (a traced module with buffers) loads here."""
+    blk = _Block(True)
+    vars = {"hydro_u": blk.u, "hydro_w": torch.rand((5, 1, 12, 14), dtype=torch.float64)}
+    path = str(tmp_path / restart.part_name("run", 0, 2))
+    extra = restart.carry_tensors(["mass", "energy"], torch.tensor([1.0, 2.0], dtype=torch.float64),
+                                  torch.tensor([0.5, -0.5], dtype=torch.float64), 3.0)
+    restart.write_restart(blk, vars, path, 1.0, extra=extra)
+    with open(path, "rb") as f:
+        assert f.read(2) == b"PK"
+    mod = torch.jit.load(path, map_location="cpu")
+    bufs = dict(mod.named_buffers())
+    assert torch.equal(bufs["hydro_u_stored"], blk.u)
+    assert torch.equal(bufs["budget_carry.pending"], extra["budget_carry.pending"])
+
+    class Archive(torch.nn.Module):
+        def __init__(self, data):
+            super().__init__()
+            for name, tensor in data.items():
+                self.register_buffer(name, tensor.clone())
+
+        def forward(self, x):
+            return x
+
+    theirs = str(tmp_path / "theirs.part")
+    data = restart.restart_variables(blk, vars, 4.0)
+    torch.jit.trace(Archive(data), torch.zeros(1), check_trace=False).save(theirs)
+    back = restart.load_restart(theirs)
+    assert set(back) == set(data)
+    for k in data:
+        assert torch.equal(back[k], data[k]), k
+
+
+def test_bundle_of_parts(tmp_path):
+    """Three block parts combine into one bundle in rank order
+    (make_restart_bundle): the magic line, the count, name<TAB>bytes lines and
+    an empty line; each block's part reads back from it, a missing rank is
+    refused, and a lone part is renamed, not bundled."""
+    d = str(tmp_path)
+    for rank in (2, 0, 1):
+        restart.save_tensors({"hydro_u": torch.full((3,), float(rank), dtype=torch.float64)},
+                             f"{d}/{restart.part_name('run', rank, 7)}")
+    sizes = [__import__("os").path.getsize(f"{d}/{restart.part_name('run', r, 7)}") for r in range(3)]
+    out = restart.combine_parts(d, "run", 7)
+    assert out.endswith("run.00007.restart")
+    with open(out, "rb") as f:
+        head = f.read().split(b"\n", 5)
+    assert head[0] == b"SNAPY_RESTART_BUNDLE_V1" and head[1] == b"3"
+    assert head[2:5] == [f"run.block{r}.00007.part\t{sizes[r]}".encode() for r in range(3)]
+    assert head[5].startswith(b"\nPK")
+    for rank in range(3):
+        assert float(restart.load_restart(out, rank)["hydro_u"][0]) == rank
+    with pytest.raises(restart.RestartError, match="no part for block 5"):
+        restart.load_restart(out, 5)
+    assert not list(tmp_path.glob("*.part"))
+    restart.save_tensors({"hydro_u": torch.zeros(2)}, f"{d}/{restart.part_name('run', 0)}")
+    lone = restart.combine_parts(d, "run", None)
+    assert lone.endswith("run.final.restart") and float(restart.load_restart(lone)["hydro_u"].sum()) == 0.0
+
+
+def test_schedule_keys_are_fnv1a():
+    """FNV-1a 64 (the published test vectors pin the hash), over the
+    reference's key strings."""
+    assert restart._fnv(b"") == 0xCBF29CE484222325
+    assert restart._fnv(b"a") == 0xAF63DC4C8601EC8C
+    assert restart._fnv(b"foobar") == 0x85944171F73967E8
+    o = output_options("netcdf", 300.0, ["prim", "uov"])
+    want = restart._signed(restart._fnv(b"netcdf|300.000000|prim|uov"))
+    assert restart.schedule_key(o) == want
+    import struct
+
+    raw = b"netcdf\0" + struct.pack("<d", 300.0) + b"\0prim\0uov"
+    assert restart.schedule_key_v2(o) == restart._signed(restart._fnv(raw))
+    assert restart.schedule_key_v2(o) != restart.schedule_key_v2(output_options("netcdf", 300.0, ["prim"]))
+
+
+def output_options(t, dt, variables=()):
+    from snapy_amd.output import OutputOptions
+
+    return OutputOptions(file_type=t, dt=dt, variables=list(variables))
+
+
+def test_schedule_restored_by_key():
+    """Saved slots match by exact key whatever the order; the file counter
+    follows the position; a next time more than dt past the resume is pulled
+    back; a new output joins the grid of its dt; legacy keys are used when
+    the file holds only those."""
+    a = output_options("netcdf", 10.0, ["prim"])
+    b = output_options("restart", 100.0)
+    c = output_options("budget", 5.0)
+    data = restart.schedule_tensors([a, b], [3, 1], [40.0, 1.0e9])
+    fn, nt = restart.restore_schedule(data, [b, a], 35.0)
+    assert fn == [3, 1]
+    assert nt == [135.0, 40.0]  # b's 1e9 pulled back to 35 + 100; a keeps its slot's 40
+    fn, nt = restart.restore_schedule(data, [a, b, c], 37.0)
+    assert fn == [3, 1, 0] and nt[:2] == [40.0, 137.0] and nt[2] == 35.0
+    legacy = dict(data)
+    legacy["output_key"] = data["output_key"][:2]
+    assert restart.restore_schedule(legacy, [b, a], 35.0)[1] == [135.0, 40.0]
+    del legacy["output_key"]
+    assert restart.restore_schedule(legacy, [a, b], 35.0)[1] == [40.0, 135.0]
+
+
+def test_budget_carry_round_trip():
+    rows = ["mass", "momentum1", "energy"]
+    c = restart.carry_tensors(rows, torch.tensor([1.0, 2.0, 3.0], dtype=torch.float64),
+                              torch.tensor([0.1, 0.2, 0.3], dtype=torch.float64), 12.5)
+    assert restart.unpack_names(c["budget_carry.rows"]) == rows
+    tp, pend, interval = restart.read_carry(c, rows)
+    assert tp.tolist() == [1.0, 2.0, 3.0] and pend.tolist() == [0.1, 0.2, 0.3] and interval == 12.5
+    assert restart.read_carry(c, rows[:2]) is None
+    bad = dict(c)
+    del bad["budget_carry.minimum"]
+    assert restart.read_carry(bad, rows) is None
