@@ -8,9 +8,10 @@
 // every lane reconstructs its own face states, the enthalpy h and the normal
 // velocity u_n go through LDS, and the inner lanes solve the Riemann problem
 // and add the covariance.  When a whole x1 line (nx1 + 2 cells with the two
-// halo cells) fits, a workgroup packs several whole lines; otherwise a
-// workgroup of 64 lanes walks 62 consecutive x1 cells plus one halo lane on
-// each side.
+// halo cells) fits, a workgroup packs several whole lines (FP64: one face
+// per lane; FP32: each lane walks a run of faces along the sweep and
+// reconstructs every cell once); otherwise a workgroup of 64 lanes walks 62
+// consecutive x1 cells plus one halo lane on each side.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -76,42 +77,6 @@ __device__ __forceinline__ void cov_states(double const* __restrict__ w, CovArgs
     // wbar = 0.5 (wl + wr); h = (W->I(wbar) + p) / rho; u_n.  FP64 in the
     // FP64 form; in the FP32 form the FP32 quotients of qdiv (the term is a
     // small correction to the energy flux)
-    double rho = 0.5 * (double(L.r) + double(R.r));
-    double p = 0.5 * (double(L.p) + double(R.p)) + gauge;
-    if constexpr (sizeof(T) == 8) {
-      double enth = p / (a.gammad - 1.) + p;
-      h = enth / rho;
-    } else {
-      float enth = qdiv(float(p), float(a.gammad - 1.)) + float(p);
-      h = double(qdiv(enth, float(rho)));
-    }
-    un = 0.5 * (double(L.v[DIR - 1]) + double(R.v[DIR - 1]));
-  }
-}
-
-// cov_states with the stencil from a staged tile: st(v, m) is row v of the
-// m-th cell of the face's stencil (m = 0..5 for the cells f - 3 .. f + 2
-// along the sweep), already shifted by the gauge and in T.
-template <typename T, int DIR, typename ST>
-__device__ __forceinline__ void cov_states_tile(CovArgs const& a, ST st, double gauge,
-                                                Prim5<T>& L, Prim5<T>& R, double& h,
-                                                double& un) {
-  T* lv[5] = {&L.r, &L.v[0], &L.v[1], &L.v[2], &L.p};
-  T* rv[5] = {&R.r, &R.v[0], &R.v[1], &R.v[2], &R.p};
-#pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    T c0 = st(v, 0), c1 = st(v, 1), c2 = st(v, 2), c3 = st(v, 3), c4 = st(v, 4), c5 = st(v, 5);
-    if (v == IDN || a.shock) {
-      *lv[v] = weno5_plus(c0, c1, c2, c3, c4);
-      *rv[v] = weno5_minus(c1, c2, c3, c4, c5);
-    } else {
-      *lv[v] = cp5_plus(c0, c1, c2, c3, c4);
-      *rv[v] = cp5_minus(c1, c2, c3, c4, c5);
-    }
-  }
-  h = 0.;
-  un = 0.;
-  if (a.cov) {
     double rho = 0.5 * (double(L.r) + double(R.r));
     double p = 0.5 * (double(L.p) + double(R.p)) + gauge;
     if constexpr (sizeof(T) == 8) {
@@ -263,84 +228,208 @@ __global__ void __launch_bounds__(kPackThreads)
   Sweep23PackedBody<T, DIR>::run(block_index(), grid_size(), w, x1v, flx, a, lines);
 }
 
-// Tiled lines (nx1 + 2 <= kPackMax): as the packed kernel, a workgroup
-// holds `lines` whole x1 lines of nx1 + 2 lanes, the faces f0 .. f0 +
-// lines - 1 of one line along the sweep (a tile never straddles two lines,
-// so the tile shares one pressure gauge per x1 cell).  The tile's cells
-// f0 - 3 .. f0 + lines + 1 along the sweep are staged once in LDS, every
-// row already shifted by the gauge and rounded to T exactly as the lanes of
-// the packed kernel round them (so the fluxes are the same bits), with the
-// loads coalesced along x1; each lane then reads its six-cell stencils from
-// LDS instead of global memory.  Dynamic LDS: 5 (lines + 5) (nx1 + 2) T.
-template <typename T, int DIR>
-struct Sweep23TileBody {
+// Both edges of the centre cell of the window v0..v4 with weno5 (the left
+// edge em = weno5_minus(v0..v4), the right edge ep = weno5_plus(v0..v4)),
+// each term in the order of weno5_minus_parts / weno5_plus_parts, so the
+// values are those two functions' bits.  The two share their smoothness
+// indicators (the right edge's beta0 and beta2 are the left edge's beta2
+// and beta0, beta1 the same, and its s7 is the left edge's with the sign
+// flipped, which squares to the same value) and the reciprocals of the
+// squared indicators, which qdiv<float> forms as its quotients' divisors.
+__device__ __forceinline__ void weno5_edges_f32(float v0, float v1, float v2, float v3, float v4,
+                                                float& em, float& ep) {
+  float s4 = v0 + -2.f * v1;
+  s4 = s4 + v2;
+  float s5 = v0 + -4.f * v1;
+  s5 = s5 + 3.f * v2;
+  float s6 = v1 + -2.f * v2;
+  s6 = s6 + v3;
+  float s7 = -1.f * v1;
+  s7 = s7 + v3;
+  float s8 = v2 + -2.f * v3;
+  s8 = s8 + v4;
+  float s9 = 3.f * v2;
+  s9 = s9 + -4.f * v3;
+  s9 = s9 + v4;
+  float const c13 = float(13. / 12.);
+  float beta0 = c13 * sqr(s4) + .25f * sqr(s5);
+  float beta1 = c13 * sqr(s6) + .25f * sqr(s7);
+  float beta2 = c13 * sqr(s8) + .25f * sqr(s9);
+  float r0 = __builtin_amdgcn_rcpf(sqr(beta0 + 1e-6f));
+  float r1 = __builtin_amdgcn_rcpf(sqr(beta1 + 1e-6f));
+  float r2 = __builtin_amdgcn_rcpf(sqr(beta2 + 1e-6f));
+  float const d0 = .3f, d1 = .6f, d2 = .1f;
+  float const m6 = float(-1. / 6.), p56 = float(5. / 6.), p13 = float(1. / 3.),
+              p116 = float(11. / 6.), m76 = float(-7. / 6.);
+  {
+    float p0 = m6 * v0;
+    p0 = p0 + p56 * v1;
+    p0 = p0 + p13 * v2;
+    float p1 = p13 * v1;
+    p1 = p1 + p56 * v2;
+    p1 = p1 + m6 * v3;
+    float p2 = p116 * v2;
+    p2 = p2 + m76 * v3;
+    p2 = p2 + p13 * v4;
+    float a0 = d0 * r0, a1 = d1 * r1, a2 = d2 * r2;
+    float num = a0 * p0 + a1 * p1 + a2 * p2;
+    float den = a0 + a1 + a2;
+    em = qdiv(num, den);
+  }
+  {
+    float p0 = p13 * v2;
+    p0 = p0 + p56 * v3;
+    p0 = p0 + m6 * v4;
+    float p1 = m6 * v1;
+    p1 = p1 + p56 * v2;
+    p1 = p1 + p13 * v3;
+    float p2 = p13 * v0;
+    p2 = p2 + m76 * v1;
+    p2 = p2 + p116 * v2;
+    float a0 = d0 * r2, a1 = d1 * r1, a2 = d2 * r0;
+    float num = a0 * p0 + a1 * p1 + a2 * p2;
+    float den = a0 + a1 + a2;
+    ep = qdiv(num, den);
+  }
+}
+
+// Runs of faces (FP32, nx1 + 2 <= kPackMax): a workgroup holds `rows`
+// rows of nx1 + 2 lanes (one whole x1 line each, the two halo cells
+// included), and each row walks kRunFaces consecutive faces of one line
+// along the sweep, the rows taking the consecutive runs of the workgroup's
+// rows x kRunFaces faces; the workgroups go through the XCD map, so the
+// neighbouring runs of a line, which share stencil cells, share an L2.  A
+// lane keeps a five-cell window of its x1 position in registers (each
+// cell's FP64 primitives read once, along x1 across the lanes, one cell
+// ahead of their use, shifted by the gauge and rounded to FP32) and
+// reconstructs every cell of its run once, both edges together: the right
+// edge of cell f - 1 and the left edge of cell f are the states of face f.
+// Per face that is one weno5 smoothness evaluation instead of two and one
+// load of the primitives instead of six.  The covariance's h and u_n go
+// through LDS one face at a time, double-buffered (one barrier per face).
+// Every value is rounded as a one-face-per-lane reconstruction of the face's
+// six-cell stencil rounds it (weno5_plus/minus, cp5_plus/minus), so the
+// fluxes are the same bits.  Dynamic LDS: 2 rows (nx1 + 2) (8 + 4) bytes.
+constexpr int kRunFaces = 8;
+
+template <int DIR>
+struct Sweep23RunBody {
   static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
                                              double const* __restrict__ w,
-                                             double const* __restrict__ x1v, T* __restrict__ flx,
-                                             CovArgs a, int lines, int tiles_per_line) {
+                                             double const* __restrict__ x1v,
+                                             float* __restrict__ flx, CovArgs a, int rows,
+                                             int tiles_per_line) {
     extern __shared__ char smem_raw[];
-    T* st = reinterpret_cast<T*>(smem_raw);
-    __shared__ double sh_h[kPackThreads], sh_un[kPackThreads];
     Range3 const& r = a.r;
     int const t = threadIdx.x;
     int const len = r.n1 + 2;
-    int const ncell = lines + 5;
-    // the tile: line o (k for DIR 2, j for DIR 3) and its first face f0
+    int const nt = rows * len;
+    double* sh_un = reinterpret_cast<double*>(smem_raw);  // [2][nt]
+    float* sh_h = reinterpret_cast<float*>(sh_un + 2 * nt);  // [2][nt]
     int64_t tile = xcd_tile(bid.x, gdim.x);
     int o = int(tile / tiles_per_line);
-    int f0 = int(tile - int64_t(o) * tiles_per_line) * lines;
-    int nl = a.nf - f0 < lines ? a.nf - f0 : lines;
+    int ftile = int(tile - int64_t(o) * tiles_per_line) * rows * kRunFaces;
+    // a thread past the rows (a group launch may run more) walks clamped
+    // cells and stores nothing
+    bool const act = t < nt;
+    int const tc = act ? t : 0;
+    int q = tc / len, p = tc - q * len;
+    int fs = ftile + q * kRunFaces;
+    int const nf = a.nf;
     int64_t sweep = DIR == 2 ? r.s2 : r.s3;
     int lo = DIR == 2 ? r.lo2 : r.lo3;
     int64_t obase = DIR == 2 ? int64_t(r.lo3 + o) * r.s3 : int64_t(r.lo2 + o) * r.s2;
-    // the line's gauge row: the first interior cell along the sweep
-    double const* gp = w + IPR * r.sv + obase + int64_t(lo) * sweep + (r.lo1 - 1);
-    // stage cells lo + f0 - 3 .. lo + f0 + nl + 1 (the rest of the tile area
-    // is left unread)
-    {
-      // rows of len cells, `per` rows at a time: thread t takes position sp
-      // of rows sr, sr + per, ...; each row's loads run along x1
-      int const per = int(blockDim.x) / len;
-      int const sr = t / len, sp = t - sr * len;
-      if (sr < per) {
-        double const g = a.gauge ? gp[sp] : 0.;
-        double const* src = w + obase + int64_t(lo + f0 - 3) * sweep + (r.lo1 - 1 + sp);
+    int i = r.lo1 - 1 + p;
+    double const* col = w + obase + i;
+    // the line's gauge at this x1 position: its first interior cell along
+    // the sweep
+    double const gauge = a.gauge ? col[IPR * r.sv + lo * sweep] : 0.;
+    // the last ghost cell along the sweep; a run past the last face reads
+    // clamped cells
+    int const cmax = lo + nf + 1;
+    auto ld = [&](int v, int cell) -> float {
+      cell = cell > cmax ? cmax : cell;
+      double gv = v == IPR ? gauge : 0.;
+      return float(col[v * r.sv + cell * sweep] - gv);
+    };
+    float win[5][5];  // [row][window position]
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
-          double const gv = v == IPR ? g : 0.;
-          for (int c = sr; c < nl + 5; c += per)
-            st[(v * ncell + c) * len + sp] = T(src[v * r.sv + c * sweep] - gv);
+    for (int v = 0; v < 5; ++v)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) win[v][m] = ld(v, lo + fs - 3 + m);
+    // the raw FP64 row values of the next cell, loaded one cell ahead
+    double nxt[5];
+    auto fetch = [&](int cell) {
+      cell = cell > cmax ? cmax : cell;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) nxt[v] = col[v * r.sv + cell * sweep];
+    };
+    fetch(lo + fs + 1);
+    float ep_prev[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    bool const inner = act && p != 0 && p != len - 1;
+    int j = DIR == 2 ? 0 : r.lo2 + o;
+    int k = DIR == 2 ? r.lo3 + o : 0;
+    // cell fs - 1 + m has the window lo + fs - 3 + m .. lo + fs + 1 + m
+#pragma unroll 1
+    for (int m = 0; m <= kRunFaces; ++m) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) win[v][4] = float(nxt[v] - (v == IPR ? gauge : 0.));
+      if (m < kRunFaces) fetch(lo + fs + 2 + m);
+      float em[5], ep[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        if (v == IDN || a.shock) {
+          weno5_edges_f32(win[v][0], win[v][1], win[v][2], win[v][3], win[v][4], em[v], ep[v]);
+        } else {
+          em[v] = cp5_minus(win[v][0], win[v][1], win[v][2], win[v][3], win[v][4]);
+          ep[v] = cp5_plus(win[v][0], win[v][1], win[v][2], win[v][3], win[v][4]);
         }
       }
+      if (m >= 1) {
+        // face f = fs + m - 1: the left state is the right edge of cell
+        // f - 1, the right state the left edge of cell f
+        int f = fs + m - 1;
+        Prim5<float> L{ep_prev[IDN], {ep_prev[IVX], ep_prev[IVY], ep_prev[IVZ]}, ep_prev[IPR]};
+        Prim5<float> R{em[IDN], {em[IVX], em[IVY], em[IVZ]}, em[IPR]};
+        double h = 0., un = 0.;
+        if (a.cov) {
+          double rho = 0.5 * (double(L.r) + double(R.r));
+          double pr = 0.5 * (double(L.p) + double(R.p)) + gauge;
+          float enth = qdiv(float(pr), float(a.gammad - 1.)) + float(pr);
+          h = double(qdiv(enth, float(rho)));
+          un = 0.5 * (double(L.v[DIR - 1]) + double(R.v[DIR - 1]));
+        }
+        int const slot = (m - 1) & 1;
+        if (act) {
+          sh_h[slot * nt + t] = float(h);
+          sh_un[slot * nt + t] = un;
+        }
+        __syncthreads();
+        if (inner && f < nf) {
+          if (DIR == 2)
+            j = lo + f;
+          else
+            k = lo + f;
+          cov_flux<float, DIR>(x1v, flx, a, i, j, k, L, R, gauge, double(sh_h[slot * nt + t - 1]),
+                               double(sh_h[slot * nt + t + 1]), sh_un[slot * nt + t - 1],
+                               sh_un[slot * nt + t + 1]);
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        ep_prev[v] = ep[v];
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm) win[v][mm] = win[v][mm + 1];
+      }
     }
-    __syncthreads();
-    int q = t / len, p = t - q * len;
-    bool in = q < nl;
-    int i = r.lo1 - 1 + p;
-    int f = f0 + q;
-    int j = DIR == 2 ? lo + f : r.lo2 + o;
-    int k = DIR == 2 ? r.lo3 + o : lo + f;
-    Prim5<T> L, R;
-    double h = 0., un = 0., gauge = 0.;
-    if (in) {
-      gauge = a.gauge ? gp[p] : 0.;
-      cov_states_tile<T, DIR>(
-          a, [&](int v, int m) { return st[(v * ncell + q + m) * len + p]; }, gauge, L, R, h, un);
-    }
-    sh_h[t] = h;
-    sh_un[t] = un;
-    __syncthreads();
-    if (!in || p == 0 || p == len - 1) return;
-    cov_flux<T, DIR>(x1v, flx, a, i, j, k, L, R, gauge, sh_h[t - 1], sh_h[t + 1], sh_un[t - 1],
-                     sh_un[t + 1]);
   }
 };
 
-template <typename T, int DIR>
+template <int DIR>
 __global__ void __launch_bounds__(kPackThreads)
-    sweep23_tile_kernel(double const* __restrict__ w, double const* __restrict__ x1v,
-                        T* __restrict__ flx, CovArgs a, int lines, int tiles_per_line) {
-  Sweep23TileBody<T, DIR>::run(block_index(), grid_size(), w, x1v, flx, a, lines, tiles_per_line);
+    sweep23_run_kernel(double const* __restrict__ w, double const* __restrict__ x1v,
+                       float* __restrict__ flx, CovArgs a, int rows, int tiles_per_line) {
+  Sweep23RunBody<DIR>::run(block_index(), grid_size(), w, x1v, flx, a, rows, tiles_per_line);
 }
 
 // lines per packed workgroup: the count whose lanes waste the least of
@@ -387,74 +476,44 @@ int snapy_sweep_cov(int dim, const snapy_grid* g, const snapy_physics* p,
   a.cov = p->flux_covariance;
   a.s2 = g->dx1 * g->dx1 / 12.;
   a.gauge = dtype == SNAPY_F32 && p->riemann == SNAPY_RIEMANN_LMARS;
+  if (dtype != SNAPY_F64 && dtype != SNAPY_F32) return fail(-10, "unknown dtype");
   unsigned nseg = blocks_for(g->nx1, kInner);
   int64_t ny = dim == 2 ? int64_t(a.nf) * g->nx3 : int64_t(g->nx2) * a.nf;
   dim3 grid(unsigned(ny), nseg);
   hipStream_t s = as_stream(stream);
   int len = g->nx1 + 2;
+  if (len <= kPackMax && dtype == SNAPY_F32) {
+    int rows = packed_lines(len);
+    int per = (a.nf + rows * kRunFaces - 1) / (rows * kRunFaces);
+    int64_t nlines = dim == 2 ? g->nx3 : g->nx2;
+    unsigned threads = unsigned((rows * len + 63) / 64 * 64);
+    unsigned blocks = unsigned(nlines * per);
+    size_t lds = size_t(2) * rows * len * (sizeof(double) + sizeof(float));
+    auto go = [&](auto body, auto k) {
+      launch_flex<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), lds, s,
+                                              static_cast<double const*>(w), x1tab,
+                                              static_cast<float*>(flux), a, rows, per);
+    };
+    if (dim == 2)
+      go(Sweep23RunBody<2>{}, sweep23_run_kernel<2>);
+    else
+      go(Sweep23RunBody<3>{}, sweep23_run_kernel<3>);
+    return launch_status("sweep_cov");
+  }
   if (len <= kPackMax) {
+    // FP64: the packed kernel, one face per lane
     int lines = packed_lines(len);
-    size_t tsize = dtype == SNAPY_F64 ? sizeof(double) : sizeof(float);
-    // the tiled kernel: fewer lines per workgroup when the staged tile
-    // (5 (lines + 5) len values) would pass 48 KiB
-    // (FP32 only: in FP64 the tile halves the lines per workgroup, and the
-    // packed kernel is faster: 0.33 against 0.49 ms at 256x256x128)
-    int tl = lines;
-    while (tl > 1 && size_t(5) * (tl + 5) * len * tsize > 48 * 1024) --tl;
-    size_t tlds = size_t(5) * (tl + 5) * len * tsize;
-    if (tlds <= 48 * 1024 && dtype == SNAPY_F32) {
-      int per = (a.nf + tl - 1) / tl;
-      int64_t nlines = dim == 2 ? g->nx3 : g->nx2;
-      unsigned threads = unsigned((tl * len + 63) / 64 * 64);
-      unsigned blocks = unsigned(nlines * per);
-      if (dtype == SNAPY_F64) {
-        auto go = [&](auto body, auto k) {
-          launch_flex<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), tlds, s,
-                                                  static_cast<double const*>(w), x1tab,
-                                                  static_cast<double*>(flux), a, tl, per);
-        };
-        if (dim == 2)
-          go(Sweep23TileBody<double, 2>{}, sweep23_tile_kernel<double, 2>);
-        else
-          go(Sweep23TileBody<double, 3>{}, sweep23_tile_kernel<double, 3>);
-      } else {
-        auto go = [&](auto body, auto k) {
-          launch_flex<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), tlds, s,
-                                                  static_cast<double const*>(w), x1tab,
-                                                  static_cast<float*>(flux), a, tl, per);
-        };
-        if (dim == 2)
-          go(Sweep23TileBody<float, 2>{}, sweep23_tile_kernel<float, 2>);
-        else
-          go(Sweep23TileBody<float, 3>{}, sweep23_tile_kernel<float, 3>);
-      }
-      return launch_status("sweep_cov");
-    }
     unsigned threads = unsigned((lines * len + 63) / 64 * 64);
     unsigned blocks = unsigned((ny + lines - 1) / lines);
-    if (dtype == SNAPY_F64) {
-      auto go = [&](auto body, auto k) {
-        launch_flex<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), 0, s,
-                                                static_cast<double const*>(w), x1tab,
-                                                static_cast<double*>(flux), a, lines);
-      };
-      if (dim == 2)
-        go(Sweep23PackedBody<double, 2>{}, sweep23_packed_kernel<double, 2>);
-      else
-        go(Sweep23PackedBody<double, 3>{}, sweep23_packed_kernel<double, 3>);
-    } else if (dtype == SNAPY_F32) {
-      auto go = [&](auto body, auto k) {
-        launch_flex<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), 0, s,
-                                                static_cast<double const*>(w), x1tab,
-                                                static_cast<float*>(flux), a, lines);
-      };
-      if (dim == 2)
-        go(Sweep23PackedBody<float, 2>{}, sweep23_packed_kernel<float, 2>);
-      else
-        go(Sweep23PackedBody<float, 3>{}, sweep23_packed_kernel<float, 3>);
-    } else {
-      return fail(-10, "unknown dtype");
-    }
+    auto go = [&](auto body, auto k) {
+      launch_flex<decltype(body), kPackThreads, 1>(k, dim3(blocks), dim3(threads), 0, s,
+                                              static_cast<double const*>(w), x1tab,
+                                              static_cast<double*>(flux), a, lines);
+    };
+    if (dim == 2)
+      go(Sweep23PackedBody<double, 2>{}, sweep23_packed_kernel<double, 2>);
+    else
+      go(Sweep23PackedBody<double, 3>{}, sweep23_packed_kernel<double, 3>);
     return launch_status("sweep_cov");
   }
   if (dtype == SNAPY_F64) {

@@ -63,6 +63,63 @@ def test_sweep_matches_oracle(shape, riemann, shock):
             assert rel_l2(host(fp)[sl[1:]], pref[sl[1:]]) < TOL
 
 
+COV_CASES = [
+    # (nx1, nx2, nx3, walls): several faces per lane along the sweep, runs
+    # past the last face, one to eight x1 lines per workgroup, two tiles
+    # along a line
+    (24, 20, 1, False),
+    (16, 12, 10, False),
+    (13, 7, 9, True),
+    (8, 50, 3, True),
+    (126, 70, 3, False),
+]
+
+
+@pytest.mark.parametrize("case,riemann,shock,dtype",
+                         list(itertools.product(COV_CASES, ["hllc", "lmars"], [False, True], [0, 1])))
+def test_cov_sweep_matches_oracle(case, riemann, shock, dtype):
+    """The x2/x3 sweep with the x1 flux covariance (snapy_sweep_cov) against
+    the oracle's face fluxes (reconstruction, Riemann solver, covariance,
+    wall prescription): FP64 within 1e-12, FP32 (one face per lane in FP64,
+    runs of faces per lane in FP32) within 1e-5 relative L2 per row.  The
+    FP32 LMARS fluxes carry the normal momentum row about the line's gauge,
+    the pressure of its first interior cell along the sweep, which the test
+    adds back."""
+    nx1, nx2, nx3, walls = case
+    lib = _capi.load()
+    g = fv.Grid(nx1, nx2, nx3, x1min=0.0, x1max=2.0, x2min=-1.0, x2max=1.0, x3min=0.0, x3max=3.0)
+    w = random_state(g, seed=nx1 + 3 * nx2 + 7 * nx3)
+    bcs = ("reflecting",) * 6 if walls else ("outflow",) * 6
+    gd = grid_desc(g)
+    pd = phys_desc(riemann=riemann, shock_x1=shock, shock_x23=shock, bcs=bcs)
+    pd.flux_covariance = 1
+    wd = dev(w)
+    tol = TOL if dtype == 0 else 1e-5
+    for axis, tdim in ((2, 2), (3, 1)):
+        if g.nc(axis) == 1:
+            continue
+        print(f"cov sweep x{axis} {case} {riemann} shock={shock} dtype={dtype}", flush=True)
+        flx = torch.zeros(wd.shape, dtype=torch.float64 if dtype == 0 else torch.float32, device="cuda")
+        rc = lib.snapy_sweep_cov(axis, ctypes.byref(gd), ctypes.byref(pd), dtype, wd.data_ptr(),
+                                 dev(g.centers(1)).data_ptr(), flx.data_ptr(), None)
+        assert rc == 0, lib.snapy_last_error()
+        wlr = fv.reconstruct(w, tdim, shock)
+        solver = fv.hllc if riemann == "hllc" else fv.lmars
+        ref, _ = solver(wlr[fv.ILT], wlr[fv.IRT], 1.4, tdim)
+        ref[fv.IPR][..., 1 : g.nc(1) - 1] += fv.flux_covariance(wlr[fv.ILT], wlr[fv.IRT], tdim, g, 1.4)
+        fv.prescribe_walls(ref, tdim, bcs, axis, g)
+        got = host(flx).astype(np.float64)
+        if dtype == 1 and riemann == "lmars":
+            # the gauge of each line: its first interior cell along the sweep
+            lo = g.nghost
+            gauge = w[fv.IPR][:, lo : lo + 1, :] if axis == 2 else w[fv.IPR][lo : lo + 1, :, :]
+            got[fv.IVX + axis - 1] += gauge
+        sl = _faces_slice(g, axis)
+        for v in range(5):
+            err = rel_l2(got[v][sl[1:]], ref[v][sl[1:]])
+            assert err < tol, (axis, v, err)
+
+
 @pytest.mark.parametrize("shape", CASES)
 def test_update_matches_oracle(shape):
     lib = _capi.load()
