@@ -453,19 +453,142 @@ struct SweepX1WbRefBody {
       ref[R_PSF_LO * r.sv + s_row[c] + i] = lo;
     }
     __syncthreads();
-    int const nf = a.nf;
-    int const faces = ncl * nf;
-    for (int e = tid; e < faces; e += kRefThreads) {
-      int c = e / nf, fi = e - c * nf;
-      int i = r.lo1 + fi;
+    if constexpr (sizeof(T) == 4) {
+      wb_faces_f32(w, flx, fp, a, smem, s_row, ncl, stride, floor_mp);
+    } else {
+      int const nf = a.nf;
+      int const faces = ncl * nf;
+      for (int e = tid; e < faces; e += kRefThreads) {
+        int c = e / nf, fi = e - c * nf;
+        int i = r.lo1 + fi;
+        double const* sc = smem + c * stride;
+        double const* s_F = sc + 2 * nc1;
+        double const* s_rs = sc + 3 * nc1 + 1;
+        double lo = floor_mp(s_F[i]);
+        double rf = i > 0 ? 0.5 * (s_rs[i - 1] + s_rs[i]) : s_rs[i];
+        wb_face<T, TF>(
+            w, s_row[c], fi, lo, lo * rf, [&](int m) { return sc[m]; },
+            [&](int m) { return sc[nc1 + m]; }, flx, fp, a);
+      }
+    }
+  }
+
+  // The faces of the FP32 form (wb_face<float, float> on every face of the
+  // workgroup's columns), cell by cell.  The workgroup's cells are taken in
+  // column order, 64 consecutive cells per wavefront pass, and each lane
+  // forms its cell's five face-state inputs once (the velocities; the
+  // pressure and density perturbations about the column's reference,
+  // mirrored at the walls); the five-cell window comes from the neighbouring
+  // lanes by wavefront shuffles, each lane reconstructs both edges of its
+  // cell (weno5 edges sharing their smoothness indicators, or cp5), and the
+  // face below the cell takes the right edge of the cell below from the
+  // lane below.  Lanes 0-1 and 62-63 are the window's halo and lane 2 only
+  // feeds lane 3, so a pass owns the faces of 59 cells and passes overlap by
+  // five; the passes need no LDS and no barrier.  Each value is rounded as
+  // wb_face<float, float> rounds it, so the fluxes and face pressures are
+  // its bits.
+  template <typename FLOOR>
+  static __device__ __forceinline__ void wb_faces_f32(double const* __restrict__ w,
+                                                      float* __restrict__ flx,
+                                                      double* __restrict__ fp, WbArgs const& a,
+                                                      double const* smem, int64_t const* s_row,
+                                                      int ncl, int stride, FLOOR floor_mp) {
+    Range3 const& r = a.r;
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    int const nc1 = r.nc1;
+    int const il = r.lo1, iu = r.lo1 + r.n1 - 1;
+    constexpr int kOwn = 59;
+    int const ncell = ncl * nc1;
+    int const npass = (ncell + kOwn - 1) / kOwn;
+    float const g = float(a.gammad);
+    for (int ps = wave; ps < npass; ps += kRefThreads / 64) {
+      int const L = ps * kOwn - 3 + lane;
+      bool const inb = L >= 0 && L < ncell;
+      int const c = inb ? L / nc1 : 0;
+      int const m = inb ? L - c * nc1 : 0;
       double const* sc = smem + c * stride;
+      int64_t const row = s_row[c];
+      float q[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+      if (inb) {
+        int src = m;
+        if (m < il && a.mirror_lo) src = 2 * il - 1 - m;
+        if (m > iu && a.mirror_hi) src = 2 * iu + 1 - m;
+        q[IDN] = float(double(w[IDN * r.sv + row + src]) - sc[nc1 + src]);
+        q[IVX] = float(w[IVX * r.sv + row + m]);
+        q[IVY] = float(w[IVY * r.sv + row + m]);
+        q[IVZ] = float(w[IVZ * r.sv + row + m]);
+        q[IPR] = float(double(w[IPR * r.sv + row + src]) - sc[src]);
+      }
+      // the edge cells lo1 - 1 .. lo1 + n1 (their windows stay in the column)
+      bool const edge = inb && lane >= 2 && lane <= 61 && m >= r.lo1 - 1 && m <= r.lo1 + r.n1;
+      float em[5], ep[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        float v0 = __shfl_up(q[v], 2), v1 = __shfl_up(q[v], 1), v2 = q[v];
+        float v3 = __shfl_down(q[v], 1), v4 = __shfl_down(q[v], 2);
+        em[v] = 0.f;
+        ep[v] = 0.f;
+        if (edge) {
+          if (v == IDN || a.shock) {
+            weno5_edges_f32(v0, v1, v2, v3, v4, em[v], ep[v]);
+          } else {
+            em[v] = cp5_minus(v0, v1, v2, v3, v4);
+            ep[v] = cp5_plus(v0, v1, v2, v3, v4);
+          }
+        }
+      }
+      float lq[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) lq[v] = __shfl_up(ep[v], 1);
+      if (!(edge && lane >= 3 && m >= r.lo1)) continue;
+      int const i = m, fi = m - r.lo1;
       double const* s_F = sc + 2 * nc1;
       double const* s_rs = sc + 3 * nc1 + 1;
-      double lo = floor_mp(s_F[i]);
-      double rf = i > 0 ? 0.5 * (s_rs[i - 1] + s_rs[i]) : s_rs[i];
-      wb_face<T, TF>(
-          w, s_row[c], fi, lo, lo * rf, [&](int m) { return sc[m]; },
-          [&](int m) { return sc[nc1 + m]; }, flx, fp, a);
+      double const psf = floor_mp(s_F[i]);
+      double const rf = i > 0 ? 0.5 * (s_rs[i - 1] + s_rs[i]) : s_rs[i];
+      double const dsf = psf * rf;
+      Prim5<float> Lf, Rf;
+      Lf.v[0] = lq[IVX];
+      Lf.v[1] = lq[IVY];
+      Lf.v[2] = lq[IVZ];
+      Rf.v[0] = em[IVX];
+      Rf.v[1] = em[IVY];
+      Rf.v[2] = em[IVZ];
+      {
+        float pl = lq[IPR], pr = em[IPR];
+        double tl = double(pl) + psf, tr = double(pr) + psf;
+        Lf.p = tl > 0. ? pl : 0.f;
+        Rf.p = tr > 0. ? pr : 0.f;
+      }
+      {
+        float dlp = lq[IDN], drp = em[IDN];
+        double const* dref = sc + nc1;
+        double d0 = dref[i - 3], d1 = dref[i - 2], d2 = dref[i - 1], d3 = dref[i],
+               d4 = dref[i + 1], d5 = dref[i + 2];
+        double fl = cp5_plus(d0, d1, d2, d3, d4);
+        double fr = cp5_minus(d1, d2, d3, d4, d5);
+        double lo = d2 < d3 ? d2 : d3, hi = d2 > d3 ? d2 : d3;
+        double sl = (fl >= lo && fl <= hi) ? fl : dsf;
+        double sr = (fr >= lo && fr <= hi) ? fr : dsf;
+        if ((fi == 0 && a.refl_lo) || (fi == a.nf - 1 && a.refl_hi)) {
+          double mean = 0.5 * (sl + sr);
+          sl = mean;
+          sr = mean;
+        }
+        double tl = double(dlp) + sl, tr = double(drp) + sr;
+        Lf.r = float(tl > 0. ? tl : double(w[row + i - 1]));
+        Rf.r = float(tr > 0. ? tr : double(w[row + i]));
+      }
+      float out[5];
+      float pface;
+      lmars(Lf, Rf, g, g, 0, out, &pface, float(psf));
+      if ((fi == 0 && a.wall_lo) || (fi == a.nf - 1 && a.wall_hi)) {
+        out[IDN] = 0.f;
+        out[IPR] = 0.f;
+      }
+      for (int v = 0; v < 5; ++v) flx[v * r.sv + row + i] = out[v];
+      if (fp) fp[row + i] = double(pface) + psf;
     }
   }
 };
@@ -704,7 +827,7 @@ int snapy_sweep_x1_wb_ref(const snapy_grid* g, const snapy_physics* p, int dtype
   if (nc > kRefCols) nc = kRefCols;
   auto lds_of = [&](int c) { return size_t(c) * (4 * g->nc1 + 1) * sizeof(double); };
   while (nc > 1 && lds_of(nc) > 64 * 1024) nc /= 2;
-  if (lds_of(nc) > 64 * 1024) return fail(-9, "sweep_x1_wb_ref: an x1 column above 2047 cells");
+  if (lds_of(nc) > 64 * 1024) return fail(-9, "sweep_x1_wb_ref: an x1 column too tall for LDS");
   dim3 grid(unsigned((ncol + nc - 1) / nc));
   if (dtype == SNAPY_F64)
     launch<SweepX1WbRefBody<double, double>, kRefThreads, 1>((sweep_x1_wb_ref_kernel<double, double>), grid, dim3(kRefThreads),

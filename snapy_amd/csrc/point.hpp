@@ -137,6 +137,71 @@ __device__ __forceinline__ T weno5_plus(T v0, T v1, T v2, T v3, T v4) {
   return qdiv(num, den);
 }
 
+// Both edges of the centre cell of the window v0..v4 with weno5 (the left
+// edge em = weno5_minus(v0..v4), the right edge ep = weno5_plus(v0..v4)),
+// each term in the order of weno5_minus_parts / weno5_plus_parts, so the
+// values are those two functions' bits.  The two share their smoothness
+// indicators (the right edge's beta0 and beta2 are the left edge's beta2
+// and beta0, beta1 the same, and its s7 is the left edge's with the sign
+// flipped, which squares to the same value) and the reciprocals of the
+// squared indicators, which qdiv<float> forms as its quotients' divisors.
+__device__ __forceinline__ void weno5_edges_f32(float v0, float v1, float v2, float v3, float v4,
+                                                float& em, float& ep) {
+  float s4 = v0 + -2.f * v1;
+  s4 = s4 + v2;
+  float s5 = v0 + -4.f * v1;
+  s5 = s5 + 3.f * v2;
+  float s6 = v1 + -2.f * v2;
+  s6 = s6 + v3;
+  float s7 = -1.f * v1;
+  s7 = s7 + v3;
+  float s8 = v2 + -2.f * v3;
+  s8 = s8 + v4;
+  float s9 = 3.f * v2;
+  s9 = s9 + -4.f * v3;
+  s9 = s9 + v4;
+  float const c13 = float(13. / 12.);
+  float beta0 = c13 * sqr(s4) + .25f * sqr(s5);
+  float beta1 = c13 * sqr(s6) + .25f * sqr(s7);
+  float beta2 = c13 * sqr(s8) + .25f * sqr(s9);
+  float r0 = __builtin_amdgcn_rcpf(sqr(beta0 + 1e-6f));
+  float r1 = __builtin_amdgcn_rcpf(sqr(beta1 + 1e-6f));
+  float r2 = __builtin_amdgcn_rcpf(sqr(beta2 + 1e-6f));
+  float const d0 = .3f, d1 = .6f, d2 = .1f;
+  float const m6 = float(-1. / 6.), p56 = float(5. / 6.), p13 = float(1. / 3.),
+              p116 = float(11. / 6.), m76 = float(-7. / 6.);
+  {
+    float p0 = m6 * v0;
+    p0 = p0 + p56 * v1;
+    p0 = p0 + p13 * v2;
+    float p1 = p13 * v1;
+    p1 = p1 + p56 * v2;
+    p1 = p1 + m6 * v3;
+    float p2 = p116 * v2;
+    p2 = p2 + m76 * v3;
+    p2 = p2 + p13 * v4;
+    float a0 = d0 * r0, a1 = d1 * r1, a2 = d2 * r2;
+    float num = a0 * p0 + a1 * p1 + a2 * p2;
+    float den = a0 + a1 + a2;
+    em = qdiv(num, den);
+  }
+  {
+    float p0 = p13 * v2;
+    p0 = p0 + p56 * v3;
+    p0 = p0 + m6 * v4;
+    float p1 = m6 * v1;
+    p1 = p1 + p56 * v2;
+    p1 = p1 + p13 * v3;
+    float p2 = p13 * v0;
+    p2 = p2 + m76 * v1;
+    p2 = p2 + p116 * v2;
+    float a0 = d0 * r2, a1 = d1 * r1, a2 = d2 * r0;
+    float num = a0 * p0 + a1 * p1 + a2 * p2;
+    float den = a0 + a1 + a2;
+    ep = qdiv(num, den);
+  }
+}
+
 // cp5 (poly_point<T,5>, recon_point.h:54-61; table reconstruct.cpp:65-67)
 template <typename T>
 __device__ __forceinline__ T cp5_minus(T v0, T v1, T v2, T v3, T v4) {
