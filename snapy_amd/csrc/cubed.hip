@@ -180,7 +180,20 @@ __device__ __forceinline__ void matvec(double const* m, double const* v, double*
   o[2] = m[6] * v[0] + m[7] * v[1] + m[8] * v[2];
 }
 
-// one thread per (member, ghost entry): npts 2 (linear), 4 (cubic) or 1 (a
+// the members of a ghost-fill tile: all of them, up to 64 (a wavefront is
+// 64 / mt entries x mt members).  Of 4, 8, 16 and 32 members per wavefront
+// at 32 members, 32 (two entries per wavefront) was the fastest: 75 us
+// against 104 us for 8 and 131 us for one entry per 64 lanes of a member
+// (FP32 pair, 6 x 192^2 panels)
+__host__ __device__ inline int ghost_tile_members(int members) { return members < 64 ? members : 64; }
+
+// the threads of a ghost fill over nent entries and `members` members
+inline int64_t ghost_threads(int64_t nent, int members) {
+  int mt = ghost_tile_members(members), et = 64 / mt;
+  return ((nent + et - 1) / et) * ((members + mt - 1) / mt) * 64;
+}
+
+// one thread per (ghost entry, member): npts 2 (linear), 4 (cubic) or 1 (a
 // corner cell: a plain copy of the nearest interior cell)
 // TU double: the FP64 state; float: the FP32 pair (u, uc), gathered and
 // interpolated in FP64 (the same arithmetic on the pair's values)
@@ -191,10 +204,19 @@ struct CsGhostBody {
                                              int32_t const* __restrict__ idx,
                                              double const* __restrict__ wt,
                                              double const* __restrict__ mat, int64_t nent, Dims d) {
+    // a wavefront takes a tile of et consecutive entries x mt members, the
+    // entries fastest: the lanes of one entry share its tables (indices,
+    // weights, the 45 frame-matrix values), read once per wavefront for its
+    // mt members instead of once per member
     int64_t t = int64_t(bid.x) * blockDim.x + threadIdx.x;
-    if (t >= nent * d.members) return;
-    int64_t e = t % nent;
-    int64_t m = t / nent;
+    int const mt = ghost_tile_members(d.members), et = 64 / mt;
+    int64_t const net = (nent + et - 1) / et;
+    int64_t const tile = t >> 6;
+    int const l = int(t & 63);
+    int const moff = l / et, eoff = l - moff * et;
+    int64_t e = (tile % net) * et + eoff;
+    int64_t m = (tile / net) * mt + moff;
+    if (moff >= mt || e >= nent || m >= d.members) return;
     int32_t const* ix = idx + 8 * e;
     int npts = ix[7];
     int64_t dst = (m * 6 + ix[0]) * d.block + ix[1];
@@ -409,7 +431,7 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // seams
 // ---------------------------------------------------------------------------
-// one thread per (member, seam face): the owner's flux from its own state and
+// one thread per (seam face, member): the owner's flux from its own state and
 // the neighbour's (through the two frame matrices), then the neighbour's copy
 // through the global Cartesian frame
 template <typename T>
@@ -421,7 +443,9 @@ struct CsSeamBody {
                                              Dims d) {
     int64_t t = int64_t(bid.x) * blockDim.x + threadIdx.x;
     if (t >= nseam * d.members) return;
-    int64_t e = t % nseam, m = t / nseam;
+    // the members of one seam face on consecutive lanes (its 30 frame values
+    // read once per wavefront, as the ghost fill does)
+    int64_t m = t % d.members, e = t / d.members;
     int32_t const* ix = idx + 8 * e;
     int pa = ix[0], sa = ix[1], ia = ix[2], pb = ix[3], sb = ix[4], ib = ix[5];
     double sign = double(ix[6]);
@@ -761,7 +785,7 @@ int snapy_cs_ghosts(const snapy_cs_mesh* m, int conserved, double* q, void* stre
   if (!m->ghost_idx || !m->ghost_w || !m->ghost_mat_c || !m->ghost_mat_p)
     return fail(-4, "cubed sphere: null ghost tables");
   Dims d = dims_of(*m);
-  int64_t n = m->nghost_entries * d.members;
+  int64_t n = ghost_threads(m->nghost_entries, d.members);
   launch<CsGhostBody<double>, 256, 1>(cs_ghost_kernel<double>, dim3(blocks_for(n, 256)), dim3(256), 0,
                      as_stream(stream), q, nullptr, m->ghost_idx, m->ghost_w,
                      conserved ? m->ghost_mat_c : m->ghost_mat_p, m->nghost_entries, d);
@@ -828,7 +852,7 @@ int snapy_cs_ghosts_f32(const snapy_cs_mesh* m, float* u, float* c, void* stream
     return fail(-4, "cubed sphere: null ghost tables");
   if (!u || !c) return fail(-4, "cubed sphere: null buffer");
   Dims d = dims_of(*m);
-  int64_t n = m->nghost_entries * d.members;
+  int64_t n = ghost_threads(m->nghost_entries, d.members);
   launch<CsGhostBody<float>, 256, 1>(cs_ghost_kernel<float>, dim3(blocks_for(n, 256)), dim3(256), 0,
                      as_stream(stream), u, c, m->ghost_idx, m->ghost_w, m->ghost_mat_c,
                      m->nghost_entries, d);
