@@ -368,6 +368,23 @@ int snapy_vic_partial_f32(const snapy_grid *g, const snapy_physics *p,
                           double *ws, int *status, double *moved,
                           void *stream);
 
+/* snapy_vic_partial_f32 that also writes the primitives of the combined
+ * pair into the interior of w (w is read first, as in
+ * snapy_vic_partial_f32; nvar 5): the values snapy_cons2prim_f32 would form
+ * from (u, c) there, so a caller's next stage needs only the ghost fill on
+ * the primitives.
+ * Replaces: the same solve, followed by the next stage's
+ * peos->forward(u, w) at the start of HydroImpl::forward
+ * (src/hydro/hydro_forward.cpp:340) on the interior. */
+int snapy_vic_partial_prim_f32(const snapy_grid *g, const snapy_physics *p,
+                               double *w, const double *du, const float *u0,
+                               const float *c0, float *u, float *c,
+                               const double *vtab, const double *x1tab,
+                               double t0, double t1, double dt,
+                               const double *dt_dev, double wstage, double w0,
+                               double w1, double w2, double *ws, int *status,
+                               double *moved, void *stream);
+
 /* snapy_vic_full finishing into the pair (u, c) from (u0, c0). */
 int snapy_vic_full_f32(const snapy_grid *g, const snapy_physics *p,
                        const double *w, const double *du, const float *u0,

@@ -173,6 +173,10 @@ SIGNATURES = {
         _I,
         [_GP, _PP, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P, _P],
     ),
+    "snapy_vic_partial_prim_f32": (
+        _I,
+        [_GP, _PP, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P, _P],
+    ),
     "snapy_diffusion": (_I, [_GP, _PP, _I, _P, _P * 3, _P, _P, _D, _P, _P]),
     "snapy_vic_full": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P, _P]),
     "snapy_vic_full_serial": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _D, _D, _D, _D, _P, _P, _P, _P]),

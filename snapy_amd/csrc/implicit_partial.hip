@@ -58,6 +58,9 @@ struct PartArgs {
   // NULL, or (nc3, nc2, nc1) FP64: the clamped dry-gas transfer through the
   // face below each interior cell (0 in a failed column)
   double* moved;
+  // NULL, or the primitives (the w the solve reads): the FP32 pair finish
+  // then writes the primitives of the combined state into the interior
+  double* wout;
 };
 
 // the stage dt of column col (the member's along a member x3 axis)
@@ -741,7 +744,7 @@ constexpr int kFinThreads = 256;
 template <bool PAIR>
 struct VicpFinishBody {
   static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
-                                             double const* __restrict__ w,
+                                             double const* w,
                                              double const* __restrict__ du,
                                              void const* __restrict__ u0v,
                                              float const* __restrict__ c0, void* __restrict__ uv,
@@ -933,12 +936,32 @@ struct VicpFinishBody {
       out[IPR] = out[IPR] + work;
       for (int v = 0; v < 5; ++v) combine(v, out[v]);
     }
+    if constexpr (PAIR) {
+      if (a.wout) {
+        // the primitives of the combined pair, as cons2prim_f32 forms them
+        // (pair_value, cons2prim5), once every thread of the workgroup has
+        // read its columns' w and stored its pairs
+        __syncthreads();
+        float const* uf = static_cast<float const*>(uv);
+        for (int e = tid; e < cells; e += kFinThreads) {
+          int c = e / n, i = e - c * n;
+          if (!s_in[c]) continue;
+          int64_t o = s_row[c] + i;
+          double uu[5], ww[5];
+#pragma unroll
+          for (int v = 0; v < 5; ++v) uu[v] = double(uf[v * r.sv + o]) + double(cc[v * r.sv + o]);
+          cons2prim5(uu, ww, a.gamma);
+#pragma unroll
+          for (int v = 0; v < 5; ++v) a.wout[v * r.sv + o] = ww[v];
+        }
+      }
+    }
   }
 };
 
 template <bool PAIR>
 __global__ void __launch_bounds__(kFinThreads)
-    vicp_finish_kernel(double const* __restrict__ w, double const* __restrict__ du,
+    vicp_finish_kernel(double const* w, double const* __restrict__ du,
                        void const* __restrict__ u0v, float const* __restrict__ c0,
                        void* __restrict__ uv, float* __restrict__ cc,
                        double const* __restrict__ vtab, double const* __restrict__ xtab,
@@ -960,7 +983,7 @@ static int vicp_launch(const snapy_grid* g, const snapy_physics* p, const double
                        float* c, const double* vtab, const double* x1tab, double t0,
                        double t1, double dt, const double* dt_dev, double wstage,
                        double w0, double w1, double w2, double* ws, int* status,
-                       double* moved, void* stream) {
+                       double* moved, void* stream, double* wout = nullptr) {
   if (int e = check_grid(g)) return e;
   if (!p || !w || !du || !u0 || !u || !vtab || !x1tab || !ws || !status)
     return fail(-1, "vic_partial: NULL argument");
@@ -983,6 +1006,7 @@ static int vicp_launch(const snapy_grid* g, const snapy_physics* p, const double
   a.w1 = w1;
   a.w2 = w2;
   a.moved = moved;
+  a.wout = wout;
   int64_t ncol = int64_t(g->nx2) * g->nx3;
   if (ncol > (int64_t(1) << 30)) return fail(-9, "vic_partial: too many columns");
   // the records (kF per layer, lane-contiguous) then yc (3 per layer)
@@ -1033,6 +1057,18 @@ int snapy_vic_partial_f32(const snapy_grid* g, const snapy_physics* p, const dou
   if (!c0 || !c) return fail(-1, "vic_partial_f32: NULL compensation");
   return vicp_launch(g, p, w, du, u0, c0, u, c, vtab, x1tab, t0, t1, dt, dt_dev, wstage, w0,
                      w1, w2, ws, status, moved, stream);
+}
+
+int snapy_vic_partial_prim_f32(const snapy_grid* g, const snapy_physics* p, double* w,
+                               const double* du, const float* u0, const float* c0, float* u,
+                               float* c, const double* vtab, const double* x1tab, double t0,
+                               double t1, double dt, const double* dt_dev, double wstage,
+                               double w0, double w1, double w2, double* ws, int* status,
+                               double* moved, void* stream) {
+  if (!c0 || !c) return fail(-1, "vic_partial_prim_f32: NULL compensation");
+  if (g && g->nvar != 5) return fail(-6, "vic_partial_prim_f32: ideal gas needs nvar 5");
+  return vicp_launch(g, p, w, du, u0, c0, u, c, vtab, x1tab, t0, t1, dt, dt_dev, wstage, w0,
+                     w1, w2, ws, status, moved, stream, w);
 }
 
 }  // extern "C"

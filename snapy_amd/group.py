@@ -206,6 +206,8 @@ class BlockGroup:
             _capi.check(lib.snapy_group_launch(self._groups[k], s), f"group_launch({k})")
         for blk, _ in self.members:
             blk._w_stale = True
+            if not _is_cubed(blk):
+                blk._w_interior_fresh = blk._prims_from_finish()
 
     def release(self, sync: bool = True) -> None:
         """Free the group's device records (after its launches finished)."""

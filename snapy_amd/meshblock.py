@@ -118,6 +118,9 @@ class MeshBlock:
         # stage's input (True, after forward until check_redo or a time step
         # refreshes them)
         self._w_stale = False
+        # the interior of hydro_w holds prim(hydro_u) bit for bit (written by
+        # the last stage's implicit finish); only the ghosts may be stale
+        self._w_interior_fresh = False
         self._executor = StageExecutor(self, options.intg.execution)
         c = options.coord
         # ideal gas: (rho, m1, m2, m3, E); shallow water: (phi, phi v1..v3)
@@ -415,6 +418,7 @@ class MeshBlock:
             vars["hydro_u"] = u
             self._state.adopt(u, comp)
             self._w_stale = False
+            self._w_interior_fresh = False
             return vars, 0.0
         u = torch.empty_like(w)
         _capi.check(
@@ -426,6 +430,7 @@ class MeshBlock:
         vars["hydro_u"] = u
         self._state.adopt(u)
         self._w_stale = False
+        self._w_interior_fresh = False
         return vars, 0.0
 
     def write_restart(self, vars, path: str, time: float, extra: dict | None = None) -> str:
@@ -487,6 +492,7 @@ class MeshBlock:
                 vars[k] = data[k].to(device=dev, dtype=torch.float64).contiguous()
         self._cycle = int(data["last_cycle"][0])
         self._w_stale = False
+        self._w_interior_fresh = False
         return vars, float(data["last_time"][0])
 
     def _adopt(self, vars):
@@ -511,6 +517,18 @@ class MeshBlock:
             self._dt_dev.fill_(float(dt))
         self._executor.run_stage(vars, stage)
         self._w_stale = True
+        self._w_interior_fresh = self._prims_from_finish()
+
+    def _prims_from_finish(self) -> bool:
+        """The FP32 pair form with the partial implicit solve (and no
+        tracers, whose update follows the solve and reads the stage's
+        primitives): the solve's finish writes the primitives of the
+        combined pair into the interior of hydro_w, the values cons2prim
+        would form at the next stage's start, so stages 1 and 2 start with
+        the ghost fill alone and a refresh between steps fills the ghosts."""
+        return bool(
+            self.policy.fp32() and self._vic is not None and self._vic["scheme"] == 1 and not self.nscalar
+        )
 
     def _refresh_primitives(self, vars) -> None:
         """hydro_w <- prim(hydro_u) on every cell, on the current stream
@@ -525,7 +543,8 @@ class MeshBlock:
         P = _capi.ptr
         u, w = vars["hydro_u"], vars["hydro_w"]
         if self.policy.fp32():
-            _capi.check(lib.snapy_cons2prim_f32(g, p, P(u), P(self._state.comp()), P(w), s), "cons2prim")
+            if not self._w_interior_fresh:
+                _capi.check(lib.snapy_cons2prim_f32(g, p, P(u), P(self._state.comp()), P(w), s), "cons2prim")
             _capi.check(
                 lib.snapy_apply_boundaries_prim(g, p, 0, P(w), P(vars.get("boundary_reference_w")), s),
                 "apply_boundaries(prim)",
@@ -572,10 +591,14 @@ class MeshBlock:
             st_words = self._vic["status"]
             calls.append(("vic_status_clear", lambda: (st_words.zero_(), 0)[1]))
         # (1) primitives
+        fused = self._prims_from_finish()
         if pair:
             # the FP32 pair form fills the ghosts on the FP64 primitives (a
-            # mirror, copy or radiating fill commutes with prim(u + c))
-            calls.append(("cons2prim", lambda: lib.snapy_cons2prim_f32(g, p, P(u), P(c), P(w), s)))
+            # mirror, copy or radiating fill commutes with prim(u + c)); after
+            # a stage whose implicit finish wrote the interior primitives, the
+            # ghost fill alone
+            if not (fused and stage > 0):
+                calls.append(("cons2prim", lambda: lib.snapy_cons2prim_f32(g, p, P(u), P(c), P(w), s)))
             calls.append(("apply_boundaries", lambda: lib.snapy_apply_boundaries_prim(g, p, 0, P(w), bref, s)))
         else:
             calls.append(("cons2prim", lambda: lib.snapy_cons2prim(g, p, dtc, P(u), P(w), s)))
@@ -645,7 +668,10 @@ class MeshBlock:
                         ),
                     )
                 )
-                vic_fn = lib.snapy_vic_partial_f32 if vic["scheme"] == 1 else lib.snapy_vic_full_f32
+                if fused:
+                    vic_fn = lib.snapy_vic_partial_prim_f32
+                else:
+                    vic_fn = lib.snapy_vic_partial_f32 if vic["scheme"] == 1 else lib.snapy_vic_full_f32
                 calls.append(
                     (
                         vic["name"],
@@ -889,6 +915,7 @@ class MeshBlock:
         if self.intg.current_redo > self.intg.max_redo and not (causes & self.REDO_VIC):
             return -1
         self._state.restore()
+        self._w_interior_fresh = False
         if self.nscalar:
             # the scalars' backup too (StateStore::restore, state_store.hpp:
             # 62-65), and their primitives of the restored state
