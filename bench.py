@@ -258,9 +258,11 @@ def kernel_bytes(block, name, sb=8, pair=False):
         # psf_lo written; five flux rows and the face pressure written
         "sweep_x1_wb": (5 * 8 + 8) * ncell + (5 * sb + 8) * nint,
         "stage_increment": (5 * nd + 2 + 5) * nint * sb,
-        # du, w (FP64) read; u0, u read and u written: FP64, or FP32 pairs
+        # du, w (FP64) read; u0, u read and u written: FP64, or FP32 pairs;
+        # with the FP32 pair and scheme 1 the interior primitives written too
         "vic_full": (10 * 8 + 15 * (2 * 4 if pair else 8)) * nint,
-        "vic_partial": (10 * 8 + 15 * (2 * 4 if pair else 8)) * nint,
+        "vic_partial": (10 * 8 + 15 * (2 * 4 if pair else 8) + (5 * 8 if block._prims_from_finish() else 0))
+        * nint,
         "update": (3 + nd) * 5 * nint * sb,
         # outflow: the primitive view of the block (u read, w written), then
         # the ghost layers; the gathers read and write the ghost layers only
