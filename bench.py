@@ -258,6 +258,10 @@ def kernel_bytes(block, name, sb=8, pair=False):
         # psf_lo written; five flux rows and the face pressure written
         "sweep_x1_wb": (5 * 8 + 8) * ncell + (5 * sb + 8) * nint,
         "stage_increment": (5 * nd + 2 + 5) * nint * sb,
+        # the x1 sweep with the stage increment: w read, psf_lo written; the x1
+        # flux rows and face pressure written, the x2/x3 fluxes read, du
+        # written (FP64)
+        "sweep_x1_increment": (5 * 8 + 8) * ncell + (5 * sb + 8 + 5 * (nd - 1) * sb + 5 * 8) * nint,
         # du, w (FP64) read; u0, u read and u written: FP64, or FP32 pairs;
         # with the FP32 pair and scheme 1 the interior primitives written too
         "vic_full": (10 * 8 + 15 * (2 * 4 if pair else 8)) * nint,
@@ -282,6 +286,7 @@ STAGE_KERNELS = {
                     ["vicp_forward_kernel", "vicp_finish_kernel<true>"]),
     "sweep_x1_wb": (["sweep_x1_wb_ref_kernel<double, double>"], ["sweep_x1_wb_ref_kernel<float, float>"]),
     "stage_increment": (["update_gravity_kernel<double, true, false>"], ["update_gravity_kernel<float, true, true>"]),
+    "sweep_x1_increment": ([], ["sweep_x1_wb_inc_kernel"]),
 }
 
 

@@ -223,6 +223,25 @@ int snapy_sweep_x1_wb_ref(const snapy_grid *g, const snapy_physics *p, int dtype
                           const double *w, double *ref, void *flux,
                           void *face_pressure, void *stream);
 
+/* snapy_sweep_x1_wb_ref in the FP32 form (LMARS, dtype FP32, nvar 5) with
+ * the stage increment fused: each interior cell also forms its du, FP64, as
+ * snapy_update_gravity_f32 with du_out forms it from the stored fluxes (the
+ * x1 rows of this launch, the x2/x3 totals flux2/flux3, NULL when
+ * inactive, the x1 table; no diffusion source).  flux2/flux3 must hold
+ * this stage's x2/x3 sweeps.  flux1 and face_pressure (may be NULL) are
+ * written as snapy_sweep_x1_wb_ref writes them, du_out the same bits as
+ * snapy_update_gravity_f32's.
+ * Replaces: the x1 sweep (as snapy_sweep_x1_wb_ref) followed by the
+ * divergence, const gravity, bot-heat and gravity work of the stage
+ * (src/hydro/hydro_forward.cpp:479-546, 1049-1124;
+ * src/forcing/const_gravity.cpp:35-43; src/forcing/layer_heat.cpp:78-85). */
+int snapy_sweep_x1_wb_inc_f32(const snapy_grid *g, const snapy_physics *p,
+                              const double *w, double *ref, float *flux1,
+                              double *face_pressure, const float *flux2,
+                              const float *flux3, const double *x1tab,
+                              double *du_out, double dt, const double *dt_dev,
+                              void *stream);
+
 /* snapy_update with x1 gravity: du also receives the const-gravity source
  * (src/forcing/const_gravity.cpp:30-55) and the face-based gravity work plus
  * the corrected potential-energy work in place of the cell-centred work

@@ -150,6 +150,7 @@ SIGNATURES = {
     "snapy_hydro_ref_x1": (_I, [_GP, _PP, _P, _P, _P]),
     "snapy_sweep_x1_wb": (_I, [_GP, _PP, _I, _P, _P, _P, _P, _P]),
     "snapy_sweep_x1_wb_ref": (_I, [_GP, _PP, _I, _P, _P, _P, _P, _P]),
+    "snapy_sweep_x1_wb_inc_f32": (_I, [_GP, _PP, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _P]),
     "snapy_update_gravity": (_I, [_GP, _PP, _I, _P * 3, _P, _P, _P, _P, _P, _D, _P, _D, _D, _D, _P]),
     "snapy_vic_workspace": (ctypes.c_int64, [_GP]),
     "snapy_diffusion_workspace": (ctypes.c_int64, [_GP]),

@@ -36,7 +36,7 @@ FAMILIES = {
     "eos": (TUNE_EOS, ("cons2prim",), WORKGROUPS, 256),
     "sweep": (TUNE_SWEEP, ("sweep_x1", "sweep_x2", "sweep_x3"), WORKGROUPS, 256),
     "update": (TUNE_UPDATE, ("update", "update_gravity", "stage_increment"), WORKGROUPS, 256),
-    "ref_cols": (TUNE_REF_COLS, ("sweep_x1_wb",), (1, 2, 4), 4),
+    "ref_cols": (TUNE_REF_COLS, ("sweep_x1_wb", "sweep_x1_increment"), (1, 2, 4), 4),
 }
 # set by hand only (set_choice): timed inside the whole vertical solve, whose
 # forward kernel dominates, the finish's column count is not resolved

@@ -337,6 +337,86 @@ __global__ void __launch_bounds__(256)
       flx, fp, a);
 }
 
+// ---------------------------------------------------------------------------
+// fused divergence + const gravity + gravity work + RK combine
+// ---------------------------------------------------------------------------
+// x1 table: 8 rows of nc1 doubles
+enum { T_X1V = 0, T_GW_HI = 1, T_GW_LO = 2, T_GVAR = 3, T_WT0 = 4, T_WT1 = 5,
+       T_WT2 = 6 };
+
+struct GUpdArgs {
+  Range3 r;
+  double a1, a2, a3, vol;
+  double mdt;
+  double const* dt_dev;
+  double w0, w1, w2;
+  double grav1;
+  double heat_flux, heat_dzd;  // bot-heat flux and dz * depth
+  int heat_depth;               // 0 = no bot-heat
+};
+
+// The stage increment of cell (i, row) from its face fluxes (the x1 rows
+// through f1v(v, s), s = 0 the face below and 1 the face above, the
+// reference face pressure restored; the x1 mass flux of face q through
+// m1(q)) and the x2/x3 flux arrays (NULL when inactive): the divergence, the
+// constant gravity, the diffusion source dsrc (may be NULL), bot-heat and
+// the corrected-PE gravity work.  ii is the cell's interior x1 index, c its
+// offset in a row, dt the stage dt.
+template <typename F1V, typename M1, typename F2V, typename F3V>
+__device__ __forceinline__ void gravity_increment(GUpdArgs const& a, int ii, int i, double dt,
+                                                  double rho, double vx, F1V f1v, M1 m1, F2V f2v,
+                                                  F3V f3v, bool has2, bool has3,
+                                                  double const* __restrict__ tab, int,
+                                                  double const* __restrict__ dsrc, int64_t c,
+                                                  double (&du)[5]) {
+  using T = double;
+  Range3 const& r = a.r;
+  T A1 = T(a.a1), A2 = T(a.a2), A3 = T(a.a3), V = T(a.vol);
+  T mdt = T(-dt), tdt = T(dt);
+  T g1 = T(a.grav1);
+  for (int v = 0; v < 5; ++v) {
+    T div = A1 * f1v(v, 1) - A1 * f1v(v, 0);
+    if (has2) div = div + (A2 * T(f2v(v, 1)) - A2 * T(f2v(v, 0)));
+    if (has3) div = div + (A3 * T(f3v(v, 1)) - A3 * T(f3v(v, 0)));
+    div = div / V;
+    du[v] = mdt * div;
+  }
+  // ConstGravityImpl::forward_split (const_gravity.cpp:35-43), nh = 1
+  du[IVX] = du[IVX] + tdt * rho * g1 * T(1.);
+  du[IPR] = du[IPR] + tdt * rho * vx * g1 * T(1.);
+  if (dsrc)
+    for (int v = 0; v < 5; ++v) du[v] = du[v] + dsrc[v * r.sv + c];
+  // LayerHeat bot-heat, an integer depth of bottom cells
+  // (src/forcing/layer_heat.cpp:78-85): du[IPR] += dt * flux / (dz * depth)
+  if (ii < a.heat_depth) du[IPR] = du[IPR] + T(dt * a.heat_flux / a.heat_dzd);
+  // gravity_energy_correction (hydro_forward.cpp:1049-1124), Cartesian
+  auto drho = [&](int m) {  // -dt * x1_face_difference / V at cell lo1 + m
+    int q = r.lo1 + m;
+    return mdt * ((A1 * m1(q + 1) - A1 * m1(q)) / V);
+  };
+  T gw = (T(tab[T_GW_HI * r.nc1 + i]) * (A1 * m1(i + 1)) - T(tab[T_GW_LO * r.nc1 + i]) * (A1 * m1(i))) /
+         V;
+  T fgw = tdt * gw;
+  int jj = (ii < 1 ? 1 : (ii > r.n1 - 2 ? r.n1 - 2 : ii)) - 1;
+  T s = T(tab[T_WT0 * r.nc1 + i]) * drho(jj);
+  s = s + T(tab[T_WT1 * r.nc1 + i]) * drho(jj + 1);
+  s = s + T(tab[T_WT2 * r.nc1 + i]) * drho(jj + 2);
+  fgw = fgw + T(tab[T_GVAR * r.nc1 + i]) * s;
+  T ogw = tdt * rho * vx * g1 * T(1.);
+  du[IPR] = du[IPR] + (fgw - ogw);
+}
+
+// The stage increment fused into the FP32 x1 reference sweep: its
+// arguments (w0..w2 unused), the x2/x3 flux totals (NULL when inactive), the
+// x1 table and du, (nvar, nc3, nc2, nc1) FP64.
+struct X1IncArgs {
+  GUpdArgs g;
+  float const* f2;
+  float const* f3;
+  double const* tab;
+  double* du;
+};
+
 // The reference and the well-balanced sweep fused, for the interior
 // columns: kRefCols columns per workgroup run hydro_ref_x1_kernel's phases
 // (the pressure steps and rho / p, the scans one thread per column, the
@@ -354,6 +434,41 @@ struct SweepX1WbRefBody {
                                              RefArgs ra, WbArgs a, int nc) {
     extern __shared__ double smem[];
     __shared__ int64_t s_row[kRefCols];
+    int ncl = ref_phases(bid, w, ref, ra, a, nc, smem, s_row);
+    Range3 const& r = a.r;
+    int const tid = threadIdx.x;
+    int const nc1 = r.nc1;
+    int const stride = 4 * nc1 + 1;
+    double const mp = DBL_MIN;
+    auto floor_mp = [&](double v) { return v > mp ? v : mp; };
+    if constexpr (sizeof(T) == 4) {
+      wb_faces_f32(w, flx, fp, a, smem, s_row, ncl, stride, floor_mp, nullptr, 0);
+    } else {
+      int const nf = a.nf;
+      int const faces = ncl * nf;
+      for (int e = tid; e < faces; e += kRefThreads) {
+        int c = e / nf, fi = e - c * nf;
+        int i = r.lo1 + fi;
+        double const* sc = smem + c * stride;
+        double const* s_F = sc + 2 * nc1;
+        double const* s_rs = sc + 3 * nc1 + 1;
+        double lo = floor_mp(s_F[i]);
+        double rf = i > 0 ? 0.5 * (s_rs[i - 1] + s_rs[i]) : s_rs[i];
+        wb_face<T, TF>(
+            w, s_row[c], fi, lo, lo * rf, [&](int m) { return sc[m]; },
+            [&](int m) { return sc[nc1 + m]; }, flx, fp, a);
+      }
+    }
+  }
+
+  // hydro_ref_x1_kernel's phases for the workgroup's columns, into LDS (4
+  // nc1 + 1 doubles per column: the reference pressure and density of every
+  // cell, the raw faces, the smoothed rho / p), the face pressure psf_lo to
+  // ref; returns the workgroup's column count
+  static __device__ __forceinline__ int ref_phases(dim3 const& bid, double const* __restrict__ w,
+                                                   double* __restrict__ ref, RefArgs const& ra,
+                                                   WbArgs const& a, int nc, double* smem,
+                                                   int64_t* s_row) {
     Range3 const& r = a.r;
     int64_t ncol = int64_t(r.n2) * r.n3;
     int64_t col0 = int64_t(bid.x) * nc;
@@ -453,24 +568,7 @@ struct SweepX1WbRefBody {
       ref[R_PSF_LO * r.sv + s_row[c] + i] = lo;
     }
     __syncthreads();
-    if constexpr (sizeof(T) == 4) {
-      wb_faces_f32(w, flx, fp, a, smem, s_row, ncl, stride, floor_mp);
-    } else {
-      int const nf = a.nf;
-      int const faces = ncl * nf;
-      for (int e = tid; e < faces; e += kRefThreads) {
-        int c = e / nf, fi = e - c * nf;
-        int i = r.lo1 + fi;
-        double const* sc = smem + c * stride;
-        double const* s_F = sc + 2 * nc1;
-        double const* s_rs = sc + 3 * nc1 + 1;
-        double lo = floor_mp(s_F[i]);
-        double rf = i > 0 ? 0.5 * (s_rs[i - 1] + s_rs[i]) : s_rs[i];
-        wb_face<T, TF>(
-            w, s_row[c], fi, lo, lo * rf, [&](int m) { return sc[m]; },
-            [&](int m) { return sc[nc1 + m]; }, flx, fp, a);
-      }
-    }
+    return ncl;
   }
 
   // The faces of the FP32 form (wb_face<float, float> on every face of the
@@ -483,32 +581,60 @@ struct SweepX1WbRefBody {
   // cell (weno5 edges sharing their smoothness indicators, or cp5), and the
   // face below the cell takes the right edge of the cell below from the
   // lane below.  Lanes 0-1 and 62-63 are the window's halo and lane 2 only
-  // feeds lane 3, so a pass owns the faces of 59 cells and passes overlap by
-  // five; the passes need no LDS and no barrier.  Each value is rounded as
+  // feeds lane 3, so faces come out on lanes 3-61; a pass owns (stores) the
+  // faces of lanes 3-61, or with inc of lanes 5-58, and passes overlap.  The
+  // passes need no LDS and no barrier.  Each value is rounded as
   // wb_face<float, float> rounds it, so the fluxes and face pressures are
   // its bits.
+  //
+  // With inc (non-NULL), each owned interior cell also forms its stage
+  // increment (gravity_increment, as update_gravity_kernel<float, true,
+  // true> forms it from the stored fluxes): the x1 rows of the face above
+  // and the mass fluxes of the faces two below to three above come from the
+  // neighbouring lanes, the x2/x3 fluxes from inc's arrays; col0 is the
+  // workgroup's first column.
   template <typename FLOOR>
   static __device__ __forceinline__ void wb_faces_f32(double const* __restrict__ w,
                                                       float* __restrict__ flx,
                                                       double* __restrict__ fp, WbArgs const& a,
                                                       double const* smem, int64_t const* s_row,
-                                                      int ncl, int stride, FLOOR floor_mp) {
+                                                      int ncl, int stride, FLOOR floor_mp,
+                                                      X1IncArgs const* inc, int64_t col0) {
     Range3 const& r = a.r;
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
     int const nc1 = r.nc1;
     int const il = r.lo1, iu = r.lo1 + r.n1 - 1;
-    constexpr int kOwn = 59;
+    int const off = inc ? 5 : 3, kOwn = inc ? 54 : 59;
     int const ncell = ncl * nc1;
     int const npass = (ncell + kOwn - 1) / kOwn;
     float const g = float(a.gammad);
+    bool const own = lane >= off && lane < off + kOwn;
     for (int ps = wave; ps < npass; ps += kRefThreads / 64) {
-      int const L = ps * kOwn - 3 + lane;
+      int const L = ps * kOwn - off + lane;
       bool const inb = L >= 0 && L < ncell;
       int const c = inb ? L / nc1 : 0;
       int const m = inb ? L - c * nc1 : 0;
       double const* sc = smem + c * stride;
       int64_t const row = s_row[c];
+      // the increment's inputs of an owned interior cell, loaded before the
+      // faces so their latency hides behind the reconstruction
+      bool const inc_cell = inc && own && inb && m >= il && m <= iu;
+      float x2lo[5], x2hi[5], x3lo[5], x3hi[5];
+      double rho_c = 0., vx_c = 0.;
+      if (inc_cell) {
+        int64_t cc = row + m;
+        rho_c = w[cc];
+        vx_c = w[r.sv + cc];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          int64_t o = v * r.sv + cc;
+          x2lo[v] = inc->f2 ? inc->f2[o] : 0.f;
+          x2hi[v] = inc->f2 ? inc->f2[o + r.s2] : 0.f;
+          x3lo[v] = inc->f3 ? inc->f3[o] : 0.f;
+          x3hi[v] = inc->f3 ? inc->f3[o + r.s3] : 0.f;
+        }
+      }
       float q[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
       if (inb) {
         int src = m;
@@ -541,57 +667,116 @@ struct SweepX1WbRefBody {
       float lq[5];
 #pragma unroll
       for (int v = 0; v < 5; ++v) lq[v] = __shfl_up(ep[v], 1);
-      if (!(edge && lane >= 3 && m >= r.lo1)) continue;
+      bool const face = edge && lane >= 3 && m >= r.lo1;
       int const i = m, fi = m - r.lo1;
       double const* s_F = sc + 2 * nc1;
-      double const* s_rs = sc + 3 * nc1 + 1;
-      double const psf = floor_mp(s_F[i]);
-      double const rf = i > 0 ? 0.5 * (s_rs[i - 1] + s_rs[i]) : s_rs[i];
-      double const dsf = psf * rf;
-      Prim5<float> Lf, Rf;
-      Lf.v[0] = lq[IVX];
-      Lf.v[1] = lq[IVY];
-      Lf.v[2] = lq[IVZ];
-      Rf.v[0] = em[IVX];
-      Rf.v[1] = em[IVY];
-      Rf.v[2] = em[IVZ];
-      {
-        float pl = lq[IPR], pr = em[IPR];
-        double tl = double(pl) + psf, tr = double(pr) + psf;
-        Lf.p = tl > 0. ? pl : 0.f;
-        Rf.p = tr > 0. ? pr : 0.f;
-      }
-      {
-        float dlp = lq[IDN], drp = em[IDN];
-        double const* dref = sc + nc1;
-        double d0 = dref[i - 3], d1 = dref[i - 2], d2 = dref[i - 1], d3 = dref[i],
-               d4 = dref[i + 1], d5 = dref[i + 2];
-        double fl = cp5_plus(d0, d1, d2, d3, d4);
-        double fr = cp5_minus(d1, d2, d3, d4, d5);
-        double lo = d2 < d3 ? d2 : d3, hi = d2 > d3 ? d2 : d3;
-        double sl = (fl >= lo && fl <= hi) ? fl : dsf;
-        double sr = (fr >= lo && fr <= hi) ? fr : dsf;
-        if ((fi == 0 && a.refl_lo) || (fi == a.nf - 1 && a.refl_hi)) {
-          double mean = 0.5 * (sl + sr);
-          sl = mean;
-          sr = mean;
+      float out[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+      if (face) {
+        double const* s_rs = sc + 3 * nc1 + 1;
+        double const psf = floor_mp(s_F[i]);
+        double const rf = i > 0 ? 0.5 * (s_rs[i - 1] + s_rs[i]) : s_rs[i];
+        double const dsf = psf * rf;
+        Prim5<float> Lf, Rf;
+        Lf.v[0] = lq[IVX];
+        Lf.v[1] = lq[IVY];
+        Lf.v[2] = lq[IVZ];
+        Rf.v[0] = em[IVX];
+        Rf.v[1] = em[IVY];
+        Rf.v[2] = em[IVZ];
+        {
+          float pl = lq[IPR], pr = em[IPR];
+          double tl = double(pl) + psf, tr = double(pr) + psf;
+          Lf.p = tl > 0. ? pl : 0.f;
+          Rf.p = tr > 0. ? pr : 0.f;
         }
-        double tl = double(dlp) + sl, tr = double(drp) + sr;
-        Lf.r = float(tl > 0. ? tl : double(w[row + i - 1]));
-        Rf.r = float(tr > 0. ? tr : double(w[row + i]));
+        {
+          float dlp = lq[IDN], drp = em[IDN];
+          double const* dref = sc + nc1;
+          double d0 = dref[i - 3], d1 = dref[i - 2], d2 = dref[i - 1], d3 = dref[i],
+                 d4 = dref[i + 1], d5 = dref[i + 2];
+          double fl = cp5_plus(d0, d1, d2, d3, d4);
+          double fr = cp5_minus(d1, d2, d3, d4, d5);
+          double lo = d2 < d3 ? d2 : d3, hi = d2 > d3 ? d2 : d3;
+          double sl = (fl >= lo && fl <= hi) ? fl : dsf;
+          double sr = (fr >= lo && fr <= hi) ? fr : dsf;
+          if ((fi == 0 && a.refl_lo) || (fi == a.nf - 1 && a.refl_hi)) {
+            double mean = 0.5 * (sl + sr);
+            sl = mean;
+            sr = mean;
+          }
+          double tl = double(dlp) + sl, tr = double(drp) + sr;
+          Lf.r = float(tl > 0. ? tl : double(w[row + i - 1]));
+          Rf.r = float(tr > 0. ? tr : double(w[row + i]));
+        }
+        float pface;
+        lmars(Lf, Rf, g, g, 0, out, &pface, float(psf));
+        if ((fi == 0 && a.wall_lo) || (fi == a.nf - 1 && a.wall_hi)) {
+          out[IDN] = 0.f;
+          out[IPR] = 0.f;
+        }
+        if (own) {
+          for (int v = 0; v < 5; ++v) flx[v * r.sv + row + i] = out[v];
+          if (fp) fp[row + i] = double(pface) + psf;
+        }
       }
-      float out[5];
-      float pface;
-      lmars(Lf, Rf, g, g, 0, out, &pface, float(psf));
-      if ((fi == 0 && a.wall_lo) || (fi == a.nf - 1 && a.wall_hi)) {
-        out[IDN] = 0.f;
-        out[IPR] = 0.f;
-      }
-      for (int v = 0; v < 5; ++v) flx[v * r.sv + row + i] = out[v];
-      if (fp) fp[row + i] = double(pface) + psf;
+      if (!inc) continue;
+      // the stage increment of the owned interior cells
+      float up[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) up[v] = __shfl_down(out[v], 1);
+      float const m1w[6] = {__shfl_up(out[IDN], 2), __shfl_up(out[IDN], 1), out[IDN],
+                            __shfl_down(out[IDN], 1), __shfl_down(out[IDN], 2),
+                            __shfl_down(out[IDN], 3)};
+      if (!inc_cell) continue;
+      int const ii = m - il;
+      int64_t const cc = row + i;
+      int const k = r.lo3 + int((col0 + c) / r.n2);
+      double const dt = dt_at(inc->g.dt_dev, -inc->g.mdt, r, k);
+      double const psf0 = floor_mp(s_F[i]), psf1 = floor_mp(s_F[i + 1]);
+      auto f1v = [&](int v, int sgn) {
+        double x = double(sgn ? up[v] : out[v]);
+        if (v == IVX) x = x + (sgn ? psf1 : psf0);
+        return x;
+      };
+      auto m1 = [&](int qf) {
+        int d = qf - i;
+        float x = d == -2 ? m1w[0] : (d == -1 ? m1w[1] : (d == 0 ? m1w[2] : (d == 1 ? m1w[3] : (d == 2 ? m1w[4] : m1w[5]))));
+        return double(x);
+      };
+      double du[5];
+      gravity_increment(inc->g, ii, i, dt, rho_c, vx_c, f1v, m1,
+                        [&](int v, int sgn) { return sgn ? x2hi[v] : x2lo[v]; },
+                        [&](int v, int sgn) { return sgn ? x3hi[v] : x3lo[v]; }, inc->f2 != nullptr,
+                        inc->f3 != nullptr, inc->tab, i, static_cast<double const*>(nullptr), cc, du);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) inc->du[v * r.sv + cc] = du[v];
     }
   }
 };
+
+// The FP32 reference sweep with the stage increment fused (the VIC path's
+// du): the reference phases, then wb_faces_f32 with X1IncArgs.
+struct SweepX1WbIncBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w, double* __restrict__ ref,
+                                             float* __restrict__ flx, double* __restrict__ fp,
+                                             RefArgs ra, WbArgs a, int nc, X1IncArgs ia) {
+    extern __shared__ double smem[];
+    __shared__ int64_t s_row[kRefCols];
+    int ncl = SweepX1WbRefBody<float, float>::ref_phases(bid, w, ref, ra, a, nc, smem, s_row);
+    double const mp = DBL_MIN;
+    auto floor_mp = [&](double v) { return v > mp ? v : mp; };
+    SweepX1WbRefBody<float, float>::wb_faces_f32(w, flx, fp, a, smem, s_row, ncl, 4 * a.r.nc1 + 1,
+                                                 floor_mp, &ia, int64_t(bid.x) * nc);
+  }
+};
+
+__global__ void __launch_bounds__(kRefThreads)
+    sweep_x1_wb_inc_kernel(double const* __restrict__ w, double* __restrict__ ref,
+                           float* __restrict__ flx, double* __restrict__ fp, RefArgs ra, WbArgs a,
+                           int nc, X1IncArgs ia) {
+  SweepX1WbIncBody::run(block_index(), grid_size(), w, ref, flx, fp, ra, a, nc, ia);
+}
 
 template <typename T, typename TF>
 __global__ void __launch_bounds__(kRefThreads)
@@ -600,24 +785,6 @@ __global__ void __launch_bounds__(kRefThreads)
                            WbArgs a, int nc) {
   SweepX1WbRefBody<T, TF>::run(block_index(), grid_size(), w, ref, flx, fp, ra, a, nc);
 }
-
-// ---------------------------------------------------------------------------
-// fused divergence + const gravity + gravity work + RK combine
-// ---------------------------------------------------------------------------
-// x1 table: 8 rows of nc1 doubles
-enum { T_X1V = 0, T_GW_HI = 1, T_GW_LO = 2, T_GVAR = 3, T_WT0 = 4, T_WT1 = 5,
-       T_WT2 = 6 };
-
-struct GUpdArgs {
-  Range3 r;
-  double a1, a2, a3, vol;
-  double mdt;
-  double const* dt_dev;
-  double w0, w1, w2;
-  double grav1;
-  double heat_flux, heat_dzd;  // bot-heat flux and dz * depth
-  int heat_depth;               // 0 = no bot-heat
-};
 
 // WRITE_DU: store the stage increment du (for the implicit solve to finish)
 // instead of combining it into u.  TF is the flux precision; PAIR: the state
@@ -647,52 +814,24 @@ struct UpdateGravityBody {
     int k = r.lo3 + int(rest / r.n2);
     int64_t row = k * r.s3 + j * r.s2;
     int64_t c = row + i;
-    T A1 = T(a.a1), A2 = T(a.a2), A3 = T(a.a3), V = T(a.vol);
     double dt = dt_at(a.dt_dev, -a.mdt, r, k);
-    T mdt = T(-dt), tdt = T(dt);
-    T w0 = T(a.w0), w1 = T(a.w1), w2 = T(a.w2), g1 = T(a.grav1);
+    T w0 = T(a.w0), w1 = T(a.w1), w2 = T(a.w2);
     T rho = w[c], vx = w[r.sv + c];
     // x1 face flux of row v at face q (the reference face pressure restored
     // on an FP32 momentum anomaly)
-    auto F1 = [&](int v, int64_t q) {
+    auto F1 = [&](int v, int s) {
+      int64_t q = c + s;
       T x = T(f1[v * r.sv + q]);
       if (psf && v == IVX) x = x + psf[q];
       return x;
     };
-    T du[5];
-    for (int v = 0; v < 5; ++v) {
-      int64_t o = v * r.sv + c;
-      T div = A1 * F1(v, c + 1) - A1 * F1(v, c);
-      if (f2) div = div + (A2 * T(f2[o + r.s2]) - A2 * T(f2[o]));
-      if (f3) div = div + (A3 * T(f3[o + r.s3]) - A3 * T(f3[o]));
-      div = div / V;
-      du[v] = mdt * div;
-    }
-    // ConstGravityImpl::forward_split (const_gravity.cpp:35-43), nh = 1
-    du[IVX] = du[IVX] + tdt * rho * g1 * T(1.);
-    du[IPR] = du[IPR] + tdt * rho * vx * g1 * T(1.);
-    if (dsrc)
-      for (int v = 0; v < 5; ++v) du[v] = du[v] + dsrc[v * r.sv + c];
-    // LayerHeat bot-heat, an integer depth of bottom cells
-    // (src/forcing/layer_heat.cpp:78-85): du[IPR] += dt * flux / (dz * depth)
-    if (ii < a.heat_depth) du[IPR] = du[IPR] + T(dt * a.heat_flux / a.heat_dzd);
-    // gravity_energy_correction (hydro_forward.cpp:1049-1124), Cartesian
     TF const* m1 = f1 + row;  // the IDN row of F1 along this column
-    auto drho = [&](int m) {  // -dt * x1_face_difference / V at cell lo1 + m
-      int q = r.lo1 + m;
-      return mdt * ((A1 * T(m1[q + 1]) - A1 * T(m1[q])) / V);
-    };
-    T gw = (T(tab[T_GW_HI * r.nc1 + i]) * (A1 * T(m1[i + 1])) -
-            T(tab[T_GW_LO * r.nc1 + i]) * (A1 * T(m1[i]))) /
-           V;
-    T fgw = tdt * gw;
-    int jj = (ii < 1 ? 1 : (ii > r.n1 - 2 ? r.n1 - 2 : ii)) - 1;
-    T s = T(tab[T_WT0 * r.nc1 + i]) * drho(jj);
-    s = s + T(tab[T_WT1 * r.nc1 + i]) * drho(jj + 1);
-    s = s + T(tab[T_WT2 * r.nc1 + i]) * drho(jj + 2);
-    fgw = fgw + T(tab[T_GVAR * r.nc1 + i]) * s;
-    T ogw = tdt * rho * vx * g1 * T(1.);
-    du[IPR] = du[IPR] + (fgw - ogw);
+    T du[5];
+    gravity_increment(
+        a, ii, i, dt, rho, vx, F1, [&](int q) { return T(m1[q]); },
+        [&](int v, int sgn) { return f2[v * r.sv + c + (sgn ? r.s2 : 0)]; },
+        [&](int v, int sgn) { return f3[v * r.sv + c + (sgn ? r.s3 : 0)]; }, f2 != nullptr,
+        f3 != nullptr, tab, i, dsrc, c, du);
     for (int v = 0; v < 5; ++v) {
       int64_t o = v * r.sv + c;
       if (WRITE_DU) {
@@ -843,6 +982,51 @@ int snapy_sweep_x1_wb_ref(const snapy_grid* g, const snapy_physics* p, int dtype
                        lds_of(nc), as_stream(stream), w, ref, static_cast<float*>(flux),
                        static_cast<double*>(face_pressure), ra, a, nc);
   return launch_status("sweep_x1_wb_ref");
+}
+
+int snapy_sweep_x1_wb_inc_f32(const snapy_grid* g, const snapy_physics* p, const double* w,
+                              double* ref, float* flux1, double* face_pressure,
+                              const float* flux2, const float* flux3, const double* x1tab,
+                              double* du_out, double dt, const double* dt_dev, void* stream) {
+  if (int e = check_grid(g)) return e;
+  if (!p || !w || !ref || !flux1 || !x1tab || !du_out)
+    return fail(-1, "sweep_x1_wb_inc_f32: NULL argument");
+  if (g->nvar != 5) return fail(-6, "sweep_x1_wb_inc_f32: ideal gas needs nvar 5");
+  if (g->nc1 < 6 || g->nx1 < 3) return fail(-9, "sweep_x1_wb_inc_f32: x1 is not resolved");
+  if (p->grav1 == 0.) return fail(-12, "sweep_x1_wb_inc_f32: no x1 gravity");
+  if (p->riemann != SNAPY_RIEMANN_LMARS)
+    return fail(-13, "sweep_x1_wb_inc_f32: the FP32 faces take LMARS");
+  WbArgs a = wb_args(g, p);
+  RefArgs ra = ref_args(g, p);
+  int64_t ncol = int64_t(g->nx2) * g->nx3;
+  int nc = int(tuned(SNAPY_TUNE_REF_COLS, g, SNAPY_F32, kRefCols));
+  if (nc > kRefCols) nc = kRefCols;
+  auto lds_of = [&](int c) { return size_t(c) * (4 * g->nc1 + 1) * sizeof(double); };
+  while (nc > 1 && lds_of(nc) > 64 * 1024) nc /= 2;
+  if (lds_of(nc) > 64 * 1024) return fail(-9, "sweep_x1_wb_inc_f32: an x1 column too tall for LDS");
+  X1IncArgs ia;
+  GUpdArgs& ga = ia.g;
+  ga.r = ranges(*g);
+  ga.a1 = g->dx3 * g->dx2;
+  ga.a2 = g->dx3 * g->dx1;
+  ga.a3 = g->dx2 * g->dx1;
+  ga.vol = g->dx3 * (g->dx2 * g->dx1);
+  ga.mdt = -dt;
+  ga.dt_dev = dt_dev;
+  ga.w0 = ga.w1 = ga.w2 = 0.;
+  ga.grav1 = p->grav1;
+  ga.heat_depth = p->bot_heat_flux != 0. ? p->bot_heat_depth : 0;
+  ga.heat_flux = p->bot_heat_flux;
+  ga.heat_dzd = g->dx1 * double(p->bot_heat_depth);
+  ia.f2 = flux2;
+  ia.f3 = flux3;
+  ia.tab = x1tab;
+  ia.du = du_out;
+  dim3 grid(unsigned((ncol + nc - 1) / nc));
+  launch<SweepX1WbIncBody, kRefThreads, 1>(sweep_x1_wb_inc_kernel, grid, dim3(kRefThreads), lds_of(nc),
+                                           as_stream(stream), w, ref, flux1, face_pressure, ra, a, nc,
+                                           ia);
+  return launch_status("sweep_x1_wb_inc_f32");
 }
 
 int snapy_update_gravity(const snapy_grid* g, const snapy_physics* p,

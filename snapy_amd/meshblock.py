@@ -519,6 +519,20 @@ class MeshBlock:
         self._w_stale = True
         self._w_interior_fresh = self._prims_from_finish()
 
+    def _increment_in_x1_sweep(self) -> bool:
+        """The FP32 pair form with x1 gravity (LMARS faces), a vertical
+        implicit solve and no diffusion: the stage increment the solve reads
+        is formed by the x1 sweep's launch (snapy_sweep_x1_wb_inc_f32), the
+        bits of the separate increment launch, after the x2/x3 sweeps."""
+        return bool(
+            self.policy.fp32()
+            and self._ref is not None
+            and self._vic is not None
+            and self._diff is None
+            and self.nvar == 5
+            and self.options.hydro.riemann == "lmars"
+        )
+
     def _prims_from_finish(self) -> bool:
         """The FP32 pair form with the partial implicit solve (and no
         tracers, whose update follows the solve and reads the stage's
@@ -604,13 +618,28 @@ class MeshBlock:
             calls.append(("cons2prim", lambda: lib.snapy_cons2prim(g, p, dtc, P(u), P(w), s)))
         # (2)-(4) fused reconstruction + Riemann per active direction; with
         # x1 gravity the x1 sweep is the well-balanced one about the
-        # hydrostatic reference of every column
-        for d in (1, 2, 3):
+        # hydrostatic reference of every column.  With the stage increment
+        # fused into the x1 sweep, x2 and x3 go first (their fluxes are the
+        # increment's input)
+        inc_x1 = self._increment_in_x1_sweep()
+        for d in ((2, 3, 1) if inc_x1 else (1, 2, 3)):
             t = freg.total(d)
             if t is None:
                 continue
             fp = freg.face_pressure1 if d == 1 else None
-            if d == 1 and grav:
+            if d == 1 and inc_x1:
+                vic = self._vic
+                fl23 = [P(freg.total(2)), P(freg.total(3))]
+                calls.append(
+                    (
+                        "sweep_x1_increment",
+                        lambda t=t, fp=fp: lib.snapy_sweep_x1_wb_inc_f32(
+                            g, p, P(w), P(self._ref), P(t), P(fp), fl23[0], fl23[1], P(self._x1tab),
+                            P(vic["du"]), 0.0, dtp, s,
+                        ),
+                    )
+                )
+            elif d == 1 and grav:
                 # the reference (kept on chip but for its face pressure) and
                 # the well-balanced sweep in one launch
                 calls.append(
@@ -659,15 +688,16 @@ class MeshBlock:
             ref = P(self._ref) if grav else None
             if self._vic is not None:
                 vic = self._vic
-                calls.append(
-                    (
-                        "stage_increment",
-                        lambda: lib.snapy_update_gravity_f32(
-                            g, p, fl, ref, None, None, None, None, P(vic["du"]), P(w), P(self._x1tab), dsrc,
-                            0.0, dtp, 1.0, 0.0, 1.0, s,
-                        ),
+                if not inc_x1:
+                    calls.append(
+                        (
+                            "stage_increment",
+                            lambda: lib.snapy_update_gravity_f32(
+                                g, p, fl, ref, None, None, None, None, P(vic["du"]), P(w), P(self._x1tab), dsrc,
+                                0.0, dtp, 1.0, 0.0, 1.0, s,
+                            ),
+                        )
                     )
-                )
                 if fused:
                     vic_fn = lib.snapy_vic_partial_prim_f32
                 else:

@@ -44,7 +44,10 @@ def test_tuned_shapes_give_the_same_bits(precision, tmp_path):
     u_ref = _steps(ref_blk, ref_vars)
     blk, vars = _block(precision)
     res = autotune.tune_block(blk, vars, reps=3, cache=tmp_path / "tune.json")
-    assert set(res) == {"eos", "sweep", "update", "ref_cols"}
+    # the FP32 form forms the stage increment inside the x1 sweep's launch,
+    # so it has no launch of the update family
+    fams = {"eos", "sweep", "ref_cols"} | ({"update"} if precision == "float64" else set())
+    assert set(res) == fams
     for fam, r in res.items():
         assert r["choice"] in autotune.FAMILIES[fam][2]
         assert autotune.get_choice(blk, fam) == r["choice"]
