@@ -117,7 +117,16 @@ class BlockGroup:
                 "cfl",
             )
             return
-        for name, fn in blk._stage_calls(vars, phase):
+        # every block's plan with the primitives' conversion at the step's
+        # start, whatever its primitives are now (a block just admitted and
+        # one mid-run make the same launches, so their plans group)
+        fresh = blk._w_interior_fresh
+        blk._w_interior_fresh = False
+        try:
+            calls = blk._stage_calls(vars, phase)
+        finally:
+            blk._w_interior_fresh = fresh
+        for name, fn in calls:
             if name in _TORCH_CALLS:
                 continue
             _capi.check(fn(), name)

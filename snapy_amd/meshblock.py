@@ -609,9 +609,10 @@ class MeshBlock:
         if pair:
             # the FP32 pair form fills the ghosts on the FP64 primitives (a
             # mirror, copy or radiating fill commutes with prim(u + c)); after
-            # a stage whose implicit finish wrote the interior primitives, the
-            # ghost fill alone
-            if not (fused and stage > 0):
+            # a stage whose implicit finish wrote the interior primitives (in
+            # a step, and at a step's start when nothing changed the state
+            # since), the ghost fill alone
+            if not (fused and (stage > 0 or self._w_interior_fresh)):
                 calls.append(("cons2prim", lambda: lib.snapy_cons2prim_f32(g, p, P(u), P(c), P(w), s)))
             calls.append(("apply_boundaries", lambda: lib.snapy_apply_boundaries_prim(g, p, 0, P(w), bref, s)))
         else:
@@ -1048,6 +1049,9 @@ class StageExecutor:
         b = self.block
         return (
             stage,
+            # a step's first stage without the primitives' conversion when
+            # the last stage left them fresh (MeshBlock._prims_from_finish)
+            stage == 0 and b._w_interior_fresh,
             b._state.generation,
             vars["hydro_u"].data_ptr(),
             vars["hydro_w"].data_ptr(),
