@@ -39,6 +39,9 @@ struct VicArgs {
   // NULL, or (nc3, nc2, nc1) FP64: the clamped dry-gas transfer through the
   // face below each interior cell (0 in a failed column)
   double* moved;
+  // the workspace records hold the solved y, z0, z1 of every layer (the
+  // cyclic-reduction solve) rather than the forward sweep's rows
+  int solved;
 };
 
 using M5 = double[5][5];
@@ -502,6 +505,124 @@ __device__ __forceinline__ void jacobian_row(double* f, int p, double gm1,
   f[4] = r1 ? gm1 : r4 ? (gm1 + 1) * v1 : 0.;
 }
 
+// Gauss-Jordan on the five lanes of a column group (lane base + p holds row
+// p of the 5x5 matrix in ar and of the NX right-hand sides in x), scaled
+// partial pivoting with ludcmp's pivot choice (the last maximum wins) and
+// its singularity test (ok becomes false on a pivot not above tol or not
+// finite); X is row p of the solution.  Columns left of a pivot are not
+// needed again, and ar and x are consumed.
+template <int NX>
+__device__ __forceinline__ void gj5(double (&ar)[5], double (&x)[NX], int p, int base, double tol,
+                                    bool& ok, double (&X)[NX]) {
+  double big = 0.;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) big = fmax(big, fabs(ar[q]));
+  double scale = big > 0. ? 1. / big : 0.;
+  unsigned used = 0;
+  int piv[5];
+#pragma unroll
+  for (int jc = 0; jc < 5; ++jc) {
+    double bv = (used >> p) & 1 ? -1. : scale * fabs(ar[jc]);
+    double best = -1.;
+    int bl = 0;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      double c = from_lane(bv, base + m);
+      if (c >= best) {
+        best = c;
+        bl = m;
+      }
+    }
+    if (!(best > tol) || !isfinite(best)) ok = false;
+    used |= 1u << bl;
+    piv[jc] = bl;
+    int src = base + bl;
+    double pj = from_lane(ar[jc], src);
+    double fct = p == bl ? 0. : ar[jc] * (1. / pj);
+#pragma unroll
+    for (int q = jc + 1; q < 5; ++q) ar[q] -= fct * from_lane(ar[q], src);
+#pragma unroll
+    for (int t = 0; t < NX; ++t) x[t] -= fct * from_lane(x[t], src);
+  }
+  // this lane pivoted the unknown whose column it still holds; lane p takes
+  // the solution row of unknown p from its pivot lane
+  int myj = 0;
+#pragma unroll
+  for (int jc = 0; jc < 5; ++jc) myj = piv[jc] == p ? jc : myj;
+  double dj = ar[0];
+#pragma unroll
+  for (int jc = 1; jc < 5; ++jc) dj = myj == jc ? ar[jc] : dj;
+  double idj = 1. / dj;
+  int srcp = piv[0];
+#pragma unroll
+  for (int jc = 1; jc < 5; ++jc) srcp = p == jc ? piv[jc] : srcp;
+#pragma unroll
+  for (int t = 0; t < NX; ++t) X[t] = from_lane(x[t] * idj, base + srcp);
+}
+
+// Row p of layer i's block equation of the full-matrix solve (scheme 9):
+// the diagonal ar, the lower block br, the upper block crw and the
+// right-hand side, from the Roe rows of the faces below (am, am0) and above
+// (apm, ap0) and the flux-Jacobian rows of the cells below (fl) and above
+// (fr); sd(v, l) is row v of the explicit increment at layer l, tab(t, row,
+// l) the column table.  The (A - A) * FluxJacobian(cell) term of the
+// diagonal is zero on the uniform Cartesian x1 grid.
+template <typename TAB, typename SDF>
+__device__ __forceinline__ void vic9_rows(VicArgs const& a, double const* vtab, int i, int p, double A,
+                                          double hiv,
+                                          double idt, double dt, double const (&am)[5],
+                                          double const (&am0)[5], double const (&apm)[5],
+                                          double const (&ap0)[5], double const (&fl)[5],
+                                          double const (&fr)[5], TAB tab, SDF sd,
+                                          double (&ar)[5], double (&br)[5], double (&crw)[5],
+                                          double& rhs) {
+  int const n = a.n;
+  double const grav = a.grav;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    br[q] = -(am[q] + fl[q]) * A * hiv;
+    crw[q] = -(apm[q] - fr[q]) * A * hiv;
+    ar[q] = (am[q] * A + apm[q] * A) * hiv;
+  }
+  ar[IDN] -= p == IVX ? grav : 0.;
+  ar[IVX] -= p == IPR ? grav : 0.;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) ar[q] += q == p ? idt : 0.;
+  rhs = sd(p, i) * idt;
+  if (p == IPR) {
+    // gravity-work rows (vic_gravity_work_impl.h:24-36)
+    double lo = tab(vtab, V_WLO, i), up = tab(vtab, V_WHI, i);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      ar[q] -= grav * (up * ap0[q] - lo * am0[q]);
+      br[q] -= grav * lo * am0[q];
+      crw[q] += grav * up * ap0[q];
+    }
+    ar[IVX] += grav * (1. - lo - up);
+    br[IVX] -= grav * lo;
+    crw[IVX] -= grav * up;
+  }
+  // the wall mirror merges the boundary face: Bnd = (1, -1, 1, 1, 1)
+  if (i == 0)
+#pragma unroll
+    for (int q = 0; q < 5; ++q) ar[q] += br[q] * (q == IVX ? -1. : 1.);
+  if (i == n - 1)
+#pragma unroll
+    for (int q = 0; q < 5; ++q) ar[q] += crw[q] * (q == IVX ? -1. : 1.);
+  if (p == IPR) {
+    // couple_radial_exact (implicit_hydro.cpp:408-446)
+    ar[IDN] -= tab(vtab, V_KMID, i) / dt;
+    br[IDN] -= tab(vtab, V_KLO, i) / dt;
+    crw[IDN] -= tab(vtab, V_KHI, i) / dt;
+    double s = tab(vtab, V_KMID, i) * sd(IDN, i);
+    if (i > 0) s = s + tab(vtab, V_KLO, i) * sd(IDN, i - 1);
+    if (i < n - 1) s = s + tab(vtab, V_KHI, i) * sd(IDN, i + 1);
+    if (i == 0) s = s + a.t0 * sd(IDN, 2);
+    if (i == n - 1) s = s + a.t1 * sd(IDN, n - 3);
+    rhs = (sd(IPR, i) - s) / dt;
+  }
+}
+
 constexpr int kQ = 8;          // workspace doubles per lane per layer: A' row, d, z0, z1
 constexpr int kLB = 8;         // layers per staged block of the forward sweep
 constexpr int kWin = kLB + 4;  // staged layers [l0 - 2, l0 + kLB + 2)
@@ -591,7 +712,7 @@ struct VicForwardBody {
     int const n = a.n, nc1 = r.nc1, lo1 = r.lo1;
     double const gm1 = a.gamma - 1.;
     double const gp1_gm1 = (gm1 + 1) / gm1;
-    double const grav = a.grav, A = a.area, V = a.vol;
+    double const A = a.area, V = a.vol;
     double const dt = dt_at(a.dt_dev, a.dt_host, r, r.lo3 + int(L.col / r.n2)) * a.wstage;
     double const idt = 1. / dt;
     double const hiv = 0.5 / V;
@@ -608,15 +729,6 @@ struct VicForwardBody {
       stage_layers<5, kWin>(s_du, du, r.sv, 0, s_row, start - 2, n);
       __syncthreads();
     };
-    auto rows_at = [&](int i, auto q) {
-      double s = tab(vtab, V_KMID, i) * q(i);
-      if (i > 0) s = s + tab(vtab, V_KLO, i) * q(i - 1);
-      if (i < n - 1) s = s + tab(vtab, V_KHI, i) * q(i + 1);
-      if (i == 0) s = s + a.t0 * q(2);
-      if (i == n - 1) s = s + a.t1 * q(n - 3);
-      return s;
-    };
-    auto mass0 = [&](int i) { return SD(IDN, i); };
 
     stage(0);
     // carried state: cell i, the face below it, the Jacobians of cells i-1, i
@@ -644,47 +756,9 @@ struct VicForwardBody {
       double fr[5], apm[5], ap0[5];
       jacobian_row(fr, p, gm1, gp1_gm1, wt);
       roe_rows(apm, ap0, p, gm1, gp1_gm1, cc, cr);
-      // block rows (the (A - A) * FluxJacobian(cell) term of the diagonal is
-      // zero on the uniform Cartesian x1 grid)
-      double ar[5], br[5], crw[5];
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        br[q] = -(am[q] + fl[q]) * A * hiv;
-        crw[q] = -(apm[q] - fr[q]) * A * hiv;
-        ar[q] = (am[q] * A + apm[q] * A) * hiv;
-      }
-      ar[IDN] -= p == IVX ? grav : 0.;
-      ar[IVX] -= p == IPR ? grav : 0.;
-#pragma unroll
-      for (int q = 0; q < 5; ++q) ar[q] += q == p ? idt : 0.;
-      double rhs = SD(p, i) * idt;
-      if (p == IPR) {
-        // gravity-work rows (vic_gravity_work_impl.h:24-36)
-        double lo = tab(vtab, V_WLO, i), up = tab(vtab, V_WHI, i);
-#pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          ar[q] -= grav * (up * ap0[q] - lo * am0[q]);
-          br[q] -= grav * lo * am0[q];
-          crw[q] += grav * up * ap0[q];
-        }
-        ar[IVX] += grav * (1. - lo - up);
-        br[IVX] -= grav * lo;
-        crw[IVX] -= grav * up;
-      }
-      // the wall mirror merges the boundary face: Bnd = (1, -1, 1, 1, 1)
-      if (i == 0)
-#pragma unroll
-        for (int q = 0; q < 5; ++q) ar[q] += br[q] * (q == IVX ? -1. : 1.);
-      if (i == n - 1)
-#pragma unroll
-        for (int q = 0; q < 5; ++q) ar[q] += crw[q] * (q == IVX ? -1. : 1.);
-      if (p == IPR) {
-        // couple_radial_exact (implicit_hydro.cpp:408-446)
-        ar[IDN] -= tab(vtab, V_KMID, i) / dt;
-        br[IDN] -= tab(vtab, V_KLO, i) / dt;
-        crw[IDN] -= tab(vtab, V_KHI, i) / dt;
-        rhs = (SD(IPR, i) - rows_at(i, mass0)) / dt;
-      }
+      double ar[5], br[5], crw[5], rhs;
+      vic9_rows(a, vtab, i, p, A, hiv, idt, dt, am, am0, apm, ap0, fl, fr, tab,
+                [&](int v, int l) { return SD(v, l); }, ar, br, crw, rhs);
       // right-hand sides: c | du/dt | unit E at the bottom | unit E at the top
       double x[8];
 #pragma unroll
@@ -709,53 +783,8 @@ struct VicForwardBody {
         x[5] -= s5;
         x[6] -= s6;
       }
-      // Gauss-Jordan, scaled partial pivoting (the last maximum wins, as in
-      // ludcmp); columns left of the pivot are not needed again
-      double big = 0.;
-#pragma unroll
-      for (int q = 0; q < 5; ++q) big = fmax(big, fabs(ar[q]));
-      double scale = big > 0. ? 1. / big : 0.;
-      unsigned used = 0;
-      int piv[5];
-#pragma unroll
-      for (int jc = 0; jc < 5; ++jc) {
-        double bv = (used >> p) & 1 ? -1. : scale * fabs(ar[jc]);
-        double best = -1.;
-        int bl = 0;
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-          double c = from_lane(bv, base + m);
-          if (c >= best) {
-            best = c;
-            bl = m;
-          }
-        }
-        if (!(best > tol) || !isfinite(best)) ok = false;
-        used |= 1u << bl;
-        piv[jc] = bl;
-        int src = base + bl;
-        double pj = from_lane(ar[jc], src);
-        double fct = p == bl ? 0. : ar[jc] * (1. / pj);
-#pragma unroll
-        for (int q = jc + 1; q < 5; ++q) ar[q] -= fct * from_lane(ar[q], src);
-#pragma unroll
-        for (int t = 0; t < 8; ++t) x[t] -= fct * from_lane(x[t], src);
-      }
-      // this lane pivoted the unknown whose column it still holds; lane p
-      // takes the solution row of unknown p from its pivot lane
-      int myj = 0;
-#pragma unroll
-      for (int jc = 0; jc < 5; ++jc) myj = piv[jc] == p ? jc : myj;
-      double dj = ar[0];
-#pragma unroll
-      for (int jc = 1; jc < 5; ++jc) dj = myj == jc ? ar[jc] : dj;
-      double idj = 1. / dj;
-      int srcp = piv[0];
-#pragma unroll
-      for (int jc = 1; jc < 5; ++jc) srcp = p == jc ? piv[jc] : srcp;
       double X[8];
-#pragma unroll
-      for (int t = 0; t < 8; ++t) X[t] = from_lane(x[t] * idj, base + srcp);
+      gj5(ar, x, p, base, tol, ok, X);
 #pragma unroll
       for (int q = 0; q < 5; ++q) Ap[q] = X[q];
       dpv = X[5];
@@ -793,6 +822,197 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
                        double const* __restrict__ vtab, double* __restrict__ ws,
                        int* __restrict__ status, VicArgs a) {
   VicForwardBody::run(block_index(), grid_size(), w, du, vtab, ws, status, a);
+}
+
+// The full-matrix solve (scheme 9) for grids of few columns, where the
+// forward sweep's chain of layers cannot fill the chip: one workgroup per
+// column, a five-lane group per layer (kSlots groups per wavefront; lanes
+// 60..63 shadow lanes 0..3 of the last group and never write), the block
+// equations of all layers assembled at once (vic9_rows on the same Roe and
+// Jacobian rows as the forward sweep) and solved by cyclic reduction with
+// the three right-hand sides of the Woodbury form (du / dt, the unit E rows
+// at the bottom and the top).  At stride s, an equation l with (l + 1) a
+// multiple of s is eliminated when (l + 1) / s is odd: its group inverts
+// its diagonal block by gj5 (ludcmp's pivot choice and singularity test)
+// into [P | Q | q] = D^-1 [B | C | R], kept in registers and in LDS; the
+// equations kept at that stride fold in their neighbours' rows (D -= B Q -
+// C P, B <- -B P, C <- -C Q, R -= B q + C q).  Every layer is eliminated at
+// one stride; the solution then comes back down the strides, x = q - P
+// x_below - Q x_above.  The records of the workspace become the solved y,
+// z0, z1 of every layer (A' rows zero) in the forward sweep's layout, and
+// vic_back_kernel reads them with VicArgs.solved.  The solution rounds
+// differently from the sweep's (the same system, another elimination
+// order).  Dynamic LDS: 65 n doubles.
+struct VicCrBody {
+  static __device__ __forceinline__ void run(dim3 const& bid, dim3 const& gdim,
+                                             double const* __restrict__ w,
+                                             double const* __restrict__ du,
+                                             double const* __restrict__ vtab,
+                                             double* __restrict__ ws, int* __restrict__ status,
+                                             VicArgs a) {
+    extern __shared__ double lds[];  // [layer][row][13]: P, Q, q; then x (3 per row)
+    __shared__ int s_fail;
+    Range3 const& r = a.r;
+    int64_t const col = bid.x;
+    int const n = a.n, nc1 = r.nc1, lo1 = r.lo1;
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    bool const shadow = lane >= 5 * kSlots;
+    int const slot = shadow ? kSlots - 1 : lane / 5;
+    int const p = shadow ? lane - 5 * kSlots : lane - 5 * slot;
+    int const base = 5 * slot;
+    int const li = wave * kSlots + slot;
+    bool const lay = li < n;
+    bool const own = lay && !shadow;
+    int const i = lay ? li : n - 1;
+    if (threadIdx.x == 0) s_fail = 0;
+    int64_t const row = column_row(r, col);
+    double const gm1 = a.gamma - 1.;
+    double const gp1_gm1 = (gm1 + 1) / gm1;
+    double const A = a.area, V = a.vol;
+    double const dt = dt_at(a.dt_dev, a.dt_host, r, r.lo3 + int(col / r.n2)) * a.wstage;
+    double const idt = 1. / dt;
+    double const hiv = 0.5 / V;
+    auto tab = [&](double const* t, int rowi, int l) { return t[rowi * nc1 + lo1 + l]; };
+    auto sd = [&](int v, int l) { return du[v * r.sv + row + l]; };
+    // ---- the layer's block equation (row p) --------------------------------
+    double wm[5], wc[5], wp[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      wm[v] = w[v * r.sv + row + i - 1];
+      wc[v] = w[v * r.sv + row + i];
+      wp[v] = w[v * r.sv + row + i + 1];
+    }
+    RoeCell const cl = roe_cell(wm, gm1), cc = roe_cell(wc, gm1), cr = roe_cell(wp, gm1);
+    double fl[5], fr[5], am[5], am0[5], apm[5], ap0[5];
+    jacobian_row(fl, p, gm1, gp1_gm1, wm);
+    jacobian_row(fr, p, gm1, gp1_gm1, wp);
+    roe_rows(am, am0, p, gm1, gp1_gm1, cl, cc);
+    roe_rows(apm, ap0, p, gm1, gp1_gm1, cc, cr);
+    double D[5], Bm[5], Cm[5], R[3];
+    vic9_rows(a, vtab, i, p, A, hiv, idt, dt, am, am0, apm, ap0, fl, fr, tab, sd, D, Bm, Cm, R[0]);
+    R[1] = (i == 0 && p == IPR) ? 1. : 0.;
+    R[2] = (i == n - 1 && p == IPR) ? 1. : 0.;
+    // ---- reduction ------------------------------------------------------------
+    bool ok = true;
+    double const tol = 8. * 5. * 2.220446049250313e-16;  // ludcmp's singularity test
+    double X[13];
+#pragma unroll
+    for (int t = 0; t < 13; ++t) X[t] = 0.;
+    int se = 1;  // the stride this layer is eliminated at
+    __syncthreads();
+    int smax = 1;
+    while (smax < n) smax *= 2;
+    for (int s = 1; s <= smax; s *= 2) {
+      bool const act = lay && (li + 1) % s == 0;
+      bool const elim = act && ((li + 1) / s) % 2 == 1;
+      bool const keep = act && !elim;
+      if (elim) {
+        double x[13];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          x[q] = Bm[q];
+          x[5 + q] = Cm[q];
+        }
+#pragma unroll
+        for (int t = 0; t < 3; ++t) x[10 + t] = R[t];
+        gj5<13>(D, x, p, base, tol, ok, X);
+#pragma unroll
+        for (int t = 0; t < 13; ++t)
+          if (!isfinite(X[t])) ok = false;
+        se = s;
+        if (own)
+#pragma unroll
+          for (int t = 0; t < 13; ++t) lds[(li * 5 + p) * 13 + t] = X[t];
+      }
+      __syncthreads();
+      if (keep) {
+        // the neighbour below (li - s >= 0 here) and above (li + s, when a layer)
+        double const* Lw = lds + (li - s) * 65;
+        double nB[5] = {0., 0., 0., 0., 0.}, nC[5] = {0., 0., 0., 0., 0.};
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          double bm = Bm[m];
+#pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            nB[q] -= bm * Lw[m * 13 + q];
+            D[q] -= bm * Lw[m * 13 + 5 + q];
+          }
+#pragma unroll
+          for (int t = 0; t < 3; ++t) R[t] -= bm * Lw[m * 13 + 10 + t];
+        }
+        if (li + s < n) {
+          double const* Uw = lds + (li + s) * 65;
+#pragma unroll
+          for (int m = 0; m < 5; ++m) {
+            double cm = Cm[m];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+              D[q] -= cm * Uw[m * 13 + q];
+              nC[q] -= cm * Uw[m * 13 + 5 + q];
+            }
+#pragma unroll
+            for (int t = 0; t < 3; ++t) R[t] -= cm * Uw[m * 13 + 10 + t];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          Bm[q] = nB[q];
+          Cm[q] = nC[q];
+        }
+      }
+    }
+    if (!ok) s_fail = 1;
+    __syncthreads();
+    // ---- the solution, down the strides ----------------------------------------
+    // x of layer l, row m, right-hand side t at lds[(l * 5 + m) * 13 + t]
+    for (int s = smax; s >= 1; s /= 2) {
+      double xr[3] = {0., 0., 0.};
+      bool const mine = lay && se == s;
+      if (mine) {
+#pragma unroll
+        for (int t = 0; t < 3; ++t) xr[t] = X[10 + t];
+        if (li - s >= 0) {
+          double const* Lw = lds + (li - s) * 65;
+#pragma unroll
+          for (int m = 0; m < 5; ++m)
+#pragma unroll
+            for (int t = 0; t < 3; ++t) xr[t] -= X[m] * Lw[m * 13 + t];
+        }
+        if (li + s < n) {
+          double const* Uw = lds + (li + s) * 65;
+#pragma unroll
+          for (int m = 0; m < 5; ++m)
+#pragma unroll
+            for (int t = 0; t < 3; ++t) xr[t] -= X[5 + m] * Uw[m * 13 + t];
+        }
+      }
+      if (mine && own)
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+          if (!isfinite(xr[t])) s_fail = 1;
+          lds[(li * 5 + p) * 13 + t] = xr[t];
+        }
+      __syncthreads();
+    }
+    // ---- the records: A' rows zero, then y, z0, z1 ----------------------------
+    if (own) {
+      int64_t const bx = col / kSlots;
+      int const ln = 5 * int(col - bx * kSlots) + p;
+      double* rec = ws + (bx * n + li) * kQ * 64 + ln;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) rec[q * 64] = 0.;
+#pragma unroll
+      for (int t = 0; t < 3; ++t) rec[(5 + t) * 64] = lds[(li * 5 + p) * 13 + t];
+    }
+    if (threadIdx.x == 0) vic_mark(status, col, s_fail == 0);
+  }
+};
+
+__global__ void __launch_bounds__(768)
+    vic_cr_kernel(double const* __restrict__ w, double const* __restrict__ du,
+                  double const* __restrict__ vtab, double* __restrict__ ws,
+                  int* __restrict__ status, VicArgs a) {
+  VicCrBody::run(block_index(), grid_size(), w, du, vtab, ws, status, a);
 }
 
 // The back half of the VIC, with the lanes and workspace of
@@ -846,7 +1066,16 @@ struct VicBackBody {
     int const i0 = 2, i1 = n - 3;
     double dpv = WS(n - 1, 5), z0v = WS(n - 1, 6), z1v = WS(n - 1, 7);
     double d_i0 = 0., z0_i0 = 0., z1_i0 = 0., d_i1 = 0., z0_i1 = 0., z1_i1 = 0.;
-    for (int ib = n - 2; ib >= 0; ib -= B) {
+    if (a.solved) {
+      // the records are the solution: the Woodbury rows read directly
+      d_i0 = WSB(i0, 5);
+      z0_i0 = WSB(i0, 6);
+      z1_i0 = WSB(i0, 7);
+      d_i1 = WSB(i1, 5);
+      z0_i1 = WSB(i1, 6);
+      z1_i1 = WSB(i1, 7);
+    }
+    for (int ib = a.solved ? -1 : n - 2; ib >= 0; ib -= B) {
       double Ab[B][5], db[B], zb[B];
 #pragma unroll
       for (int t = 0; t < B; ++t) {
@@ -1087,6 +1316,12 @@ int64_t snapy_vic_workspace(const snapy_grid* g) {
   return group > serial ? group : serial;
 }
 
+// the cyclic-reduction solve's reach: columns (beyond them the forward
+// sweep's wavefronts fill the chip) and layers (a workgroup of at most 12
+// wavefronts, the kernel's launch bound, and 65 doubles of LDS per layer)
+constexpr int64_t kCrMaxCols = 1536;
+constexpr int kCrMaxLayers = 144;
+
 static int vic_launch(const snapy_grid* g, const snapy_physics* p,
                       const double* w, const double* du, const void* u0,
                       const float* c0, void* u, float* c, const double* vtab,
@@ -1116,7 +1351,31 @@ static int vic_launch(const snapy_grid* g, const snapy_physics* p,
   a.w1 = w1;
   a.w2 = w2;
   a.moved = moved;
+  a.solved = 0;
   int64_t ncol = int64_t(g->nx2) * g->nx3;
+  // few columns: the cyclic-reduction solve (one workgroup per column, a
+  // layer per lane group) instead of the forward sweep.  A recorded launch
+  // takes the same solve as the block's own, so a block steps to the same
+  // bits alone and grouped.
+  bool const cr = !serial && ncol <= kCrMaxCols && g->nx1 <= kCrMaxLayers;
+  if (cr) {
+    unsigned nw = unsigned((g->nx1 + kSlots - 1) / kSlots);
+    size_t lds = size_t(65) * g->nx1 * sizeof(double);
+    launch_flex<VicCrBody, 768, 1>(vic_cr_kernel, dim3(unsigned(ncol)), dim3(64 * nw), lds,
+                                    as_stream(stream), w, du, vtab, ws, status, a);
+    if (int e = launch_status("vic_cr")) return e;
+    a.solved = 1;
+    unsigned nb = blocks_for(ncol, kSlots);
+    if (pair)
+      launch<VicBackBody<true>, 64, 3>(vic_back_kernel<true>, dim3(nb), dim3(64), 0,
+                                       as_stream(stream), w, du, u0, c0, u, c, vtab, x1tab, ws,
+                                       status, a);
+    else
+      launch<VicBackBody<false>, 64, 3>(vic_back_kernel<false>, dim3(nb), dim3(64), 0,
+                                        as_stream(stream), w, du, u0, c0, u, c, vtab, x1tab, ws,
+                                        status, a);
+    return launch_status("vic_full");
+  }
   if (serial)
     hipLaunchKernelGGL(vic_full_kernel, dim3(blocks_for(ncol, 64)), dim3(64),
                        0, as_stream(stream), w, du,

@@ -18,14 +18,18 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("entry", ["snapy_vic_full", "snapy_vic_full_serial", "snapy_vic_partial"])
-@pytest.mark.parametrize("shape", [(16, 6, 1), (12, 4, 3), (40, 37, 3), (400, 6, 1), (5, 9, 1)])
+@pytest.mark.parametrize("shape", [(16, 6, 1), (12, 4, 3), (40, 37, 3), (400, 6, 1), (5, 9, 1), (16, 70, 60)])
 def test_vic_kernel_matches_oracle(shape, entry):
     """One call on random columns against the oracle; the 400-layer columns
     take the finish kernel's fallback to fewer columns per workgroup (its
     LDS holds 3 (n + 1) doubles per column); the 5-layer columns (the
-    fewest scheme 1 takes) put both Woodbury rows on layer 2."""
+    fewest scheme 1 takes) put both Woodbury rows on layer 2.  Scheme 9
+    solves grids of at most 1536 columns by cyclic reduction and larger ones
+    with the forward sweep's lane kernels (the 70 x 60 grid)."""
     if (shape[0] > 100 or shape[0] < 8) and entry != "snapy_vic_partial":
         pytest.skip("the tall and the 5-layer columns exercise the scheme-1 kernels' edge cases")
+    if shape[1] * shape[2] > 1536 and entry != "snapy_vic_full":
+        pytest.skip("the 4200-column grid exercises scheme 9's lane kernels")
     lib = _capi.load()
     g = fv.Grid(*shape, 0.0, 50.0 * shape[0], -400.0, 400.0, 0.0, 300.0)
     w = stratified(g, seed=5, noise=0.5)
@@ -78,7 +82,12 @@ def test_vic_kernel_matches_oracle(shape, entry):
     assert np.array_equal(mv[:, 0], np.zeros(ncol))
     err = rel_l2(mv, ref_moved[:, :n])
     print("moved", err, flush=True)
-    assert err < 1e-11
+    # the transfer is a running sum over the column of the implicit minus
+    # the explicit mass increment, nearly equal numbers, so it carries the
+    # rows' rounding (1e-13 here) magnified by their ratio to the
+    # difference; the cyclic reduction rounds in another order than the
+    # oracle's sweep
+    assert err < (1e-11 if scheme == 1 else 3e-11)
 
 
 CASES = [
